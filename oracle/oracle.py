@@ -1,0 +1,312 @@
+"""Python access to the CPU checkers.  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import this module, and only to check (or time beside) the HIP product path.
+
+* ``liboracle.so`` (built from ``tm2d_oracle.cpp`` by ``oracle/Makefile``):
+  this project's CPU restatement of the reference's TM2D forward, exact
+  adjoint (FP64 gradient accumulation) and tangent-linear forward.
+* ``_ref/libtide_ref.so``: the reference's own CPU propagators compiled from
+  the reference sources in place (``make -C oracle ref``), called through
+  their own C ABI (reference ``backend_utils.py:256-308``).
+
+Arrays are numpy, C-contiguous; fields are ``[shots, ny, nx]``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_LIB = HERE / "_build" / "liboracle.so"
+REF_LIB = HERE / "_ref" / "libtide_ref.so"
+
+_oracle: ctypes.CDLL | None = None
+_ref: ctypes.CDLL | None = None
+
+
+def build(ref_root: str | None = None) -> None:
+    """Compile the checkers (the reference library only if its sources exist)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "oracle"], check=True)
+    root = Path(ref_root or "/root/reference")
+    if (root / "src" / "tide" / "csrc").exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REF_ROOT={root}"], check=True)
+
+
+def oracle_lib() -> ctypes.CDLL:
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_LIB.exists():
+            build()
+        _oracle = ctypes.CDLL(str(ORACLE_LIB))
+    return _oracle
+
+
+def ref_available() -> bool:
+    return REF_LIB.exists()
+
+
+def ref_lib() -> ctypes.CDLL:
+    global _ref
+    if _ref is None:
+        _ref = ctypes.CDLL(str(REF_LIB))
+    return _ref
+
+
+def _p(a: np.ndarray | None):
+    if a is None or a.size == 0:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _parr(arrs, ctype=ctypes.c_void_p):
+    return (ctype * len(arrs))(*[_p(a) for a in arrs])
+
+
+@dataclass
+class TMCase:
+    """One TM2D propagation in padded layout (mirrors ``TM2DSetup``)."""
+
+    ca: np.ndarray  # [1|N, ny, nx]
+    cb: np.ndarray
+    cq: np.ndarray
+    f: np.ndarray  # [nt, N, n_src]
+    profiles: list  # ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh
+    src: np.ndarray  # [N, n_src] int64
+    rec: np.ndarray  # [N, n_rec] int64
+    rdy: float
+    rdx: float
+    dt: float
+    nt: int
+    pml: tuple  # pml_y0, pml_x0, pml_y1, pml_x1
+    fields: list = field(default_factory=list)  # 7 initial fields
+    stencil: int = 2
+
+    @property
+    def n_shots(self) -> int:
+        return int(self.fields[0].shape[0])
+
+    @property
+    def ny(self) -> int:
+        return int(self.fields[0].shape[1])
+
+    @property
+    def nx(self) -> int:
+        return int(self.fields[0].shape[2])
+
+    @property
+    def dtype(self):
+        return self.ca.dtype
+
+    @property
+    def batched(self) -> bool:
+        return self.ca.shape[0] > 1 or (self.ca.shape[0] == 1 and self.n_shots == 1 and False)
+
+    def astype(self, dtype) -> "TMCase":
+        c = lambda a: np.ascontiguousarray(a, dtype=dtype)  # noqa: E731
+        return TMCase(c(self.ca), c(self.cb), c(self.cq), c(self.f),
+                      [c(p) for p in self.profiles], self.src, self.rec,
+                      self.rdy, self.rdx, self.dt, self.nt, self.pml,
+                      [c(x) for x in self.fields], self.stencil)
+
+
+def case_from_setup(setup) -> TMCase:
+    """Convert a ``tide.maxwell.tm2d.TM2DSetup`` (any device) to numpy."""
+    n = lambda t: np.ascontiguousarray(t.detach().cpu().numpy())  # noqa: E731
+    return TMCase(
+        ca=n(setup.ca), cb=n(setup.cb), cq=n(setup.cq),
+        f=n(setup.f).reshape(setup.nt, setup.n_shots, setup.n_src) if setup.n_src else np.zeros((setup.nt, setup.n_shots, 0), n(setup.ca).dtype),
+        profiles=[n(p) for p in setup.profiles],
+        src=n(setup.sources_i).reshape(setup.n_shots, setup.n_src).astype(np.int64),
+        rec=n(setup.receivers_i).reshape(setup.n_shots, setup.n_rec).astype(np.int64),
+        rdy=setup.rdy, rdx=setup.rdx, dt=setup.dt, nt=setup.nt, pml=tuple(setup.pml),
+        fields=[n(t) for t in setup.fields],
+    )
+
+
+def _batched_flags(case: TMCase):
+    b = case.ca.shape[0] != 1 or (case.n_shots == 1 and False)
+    return (ctypes.c_bool * 3)(b, b, b)
+
+
+def tm2d_forward(case: TMCase, *, store: str | None = None, interval: int = 1,
+                 start_t: int = 0, nt: int | None = None):
+    """Oracle forward.  ``store`` None / 'full' / 'bf16'.
+
+    Returns ``(fields, r, ey_store, curl_store)``; fields are new arrays.
+    """
+    lib = oracle_lib()
+    dt_ = case.dtype
+    f64 = dt_ == np.float64
+    nt = case.nt if nt is None else nt
+    fields = [np.ascontiguousarray(x.copy()) for x in case.fields]
+    n_rec = case.rec.shape[1]
+    r = np.zeros((start_t + nt, case.n_shots, n_rec), dt_)
+    es = cs = None
+    fmt = 0
+    if store is not None:
+        n_store = (start_t + nt + interval - 1) // interval
+        sdt = np.uint16 if store == "bf16" else dt_
+        fmt = 1 if store == "bf16" else 0
+        es = np.zeros((n_store, case.n_shots, case.ny, case.nx), sdt)
+        cs = np.zeros_like(es)
+    step_bytes = case.n_shots * case.ny * case.nx * (2 if fmt == 1 else dt_.itemsize)
+    ct = ctypes.c_double if f64 else ctypes.c_float
+    fn = lib.oracle_tm2d_forward_f64 if f64 else lib.oracle_tm2d_forward_f32
+    fn.restype = ctypes.c_int
+    f = case.f if case.f.shape[0] >= start_t + nt else np.zeros((start_t + nt, case.n_shots, case.src.shape[1]), dt_)
+    rc = fn(ctypes.c_int(case.stencil), _p(case.ca), _p(case.cb), _p(case.cq), _p(f),
+            _parr(fields), _p(r), _parr(case.profiles), _p(case.src), _p(case.rec),
+            ct(case.rdy), ct(case.rdx), ctypes.c_int64(nt), ctypes.c_int64(case.n_shots),
+            ctypes.c_int64(case.ny), ctypes.c_int64(case.nx),
+            ctypes.c_int64(case.src.shape[1]), ctypes.c_int64(n_rec),
+            ctypes.c_int64(interval), ctypes.c_int64(start_t),
+            (ctypes.c_int64 * 4)(*case.pml), _batched_flags(case),
+            _p(es), _p(cs), ctypes.c_int(fmt), ctypes.c_int64(step_bytes))
+    if rc != 0:
+        raise RuntimeError(f"oracle forward failed ({rc})")
+    return fields, r, es, cs
+
+
+def tm2d_backward(case: TMCase, grad_r: np.ndarray, ey_store, curl_store, *,
+                  store: str = "full", interval: int = 1, want_ca: bool = True,
+                  want_cb: bool = True):
+    """Oracle exact adjoint.  Returns ``(grad_ca, grad_cb, grad_f, lam)`` with
+    FP64 coefficient gradients (``[ny, nx]``, or ``[N, ny, nx]`` batched)."""
+    lib = oracle_lib()
+    dt_ = case.dtype
+    f64 = dt_ == np.float64
+    shape = (case.n_shots, case.ny, case.nx)
+    lam = [np.zeros(shape, dt_) for _ in range(7)]
+    n_src = case.src.shape[1]
+    grad_f = np.zeros((case.nt, case.n_shots, n_src), dt_)
+    batched = case.ca.shape[0] != 1
+    gshape = shape if batched else (case.ny, case.nx)
+    gca = np.zeros(gshape, np.float64)
+    gcb = np.zeros(gshape, np.float64)
+    fmt = 1 if store == "bf16" else 0
+    step_bytes = case.n_shots * case.ny * case.nx * (2 if fmt == 1 else dt_.itemsize)
+    ct = ctypes.c_double if f64 else ctypes.c_float
+    fn = lib.oracle_tm2d_backward_f64 if f64 else lib.oracle_tm2d_backward_f32
+    fn.restype = ctypes.c_int
+    grad_r = np.ascontiguousarray(grad_r, dtype=dt_)
+    rc = fn(ctypes.c_int(case.stencil), _p(case.ca), _p(case.cb), _p(case.cq), _p(grad_r),
+            _parr(lam), _p(ey_store), _p(curl_store), ctypes.c_int(fmt),
+            ctypes.c_int64(step_bytes), _p(grad_f) if n_src else None, _p(gca), _p(gcb),
+            _parr(case.profiles), _p(case.src), _p(case.rec), ct(case.rdy), ct(case.rdx),
+            ctypes.c_int64(case.nt), ctypes.c_int64(case.n_shots), ctypes.c_int64(case.ny),
+            ctypes.c_int64(case.nx), ctypes.c_int64(n_src), ctypes.c_int64(case.rec.shape[1]),
+            ctypes.c_int64(interval), ctypes.c_int64(case.nt), (ctypes.c_int64 * 4)(*case.pml),
+            _batched_flags(case), ctypes.c_bool(want_ca), ctypes.c_bool(want_cb))
+    if rc != 0:
+        raise RuntimeError(f"oracle backward failed ({rc})")
+    return (gca if want_ca else None), (gcb if want_cb else None), grad_f, lam
+
+
+def tm2d_tangent(case: TMCase, dca: np.ndarray, dcb: np.ndarray, df: np.ndarray):
+    """FP64 tangent-linear forward: returns ``(r, dr)``."""
+    lib = oracle_lib()
+    c = case.astype(np.float64)
+    fields = [x.copy() for x in c.fields]
+    dfields = [np.zeros_like(x) for x in c.fields]
+    n_rec = c.rec.shape[1]
+    r = np.zeros((c.nt, c.n_shots, n_rec))
+    dr = np.zeros_like(r)
+    fn = lib.oracle_tm2d_tangent_f64
+    fn.restype = ctypes.c_int
+    rc = fn(ctypes.c_int(c.stencil), _p(c.ca), _p(c.cb), _p(c.cq), _p(c.f),
+            _p(np.ascontiguousarray(dca, np.float64)), _p(np.ascontiguousarray(dcb, np.float64)),
+            _p(np.ascontiguousarray(df, np.float64)), _parr(fields), _parr(dfields), _p(r), _p(dr),
+            _parr(c.profiles), _p(c.src), _p(c.rec), ctypes.c_double(c.rdy), ctypes.c_double(c.rdx),
+            ctypes.c_int64(c.nt), ctypes.c_int64(c.n_shots), ctypes.c_int64(c.ny),
+            ctypes.c_int64(c.nx), ctypes.c_int64(c.src.shape[1]), ctypes.c_int64(n_rec),
+            (ctypes.c_int64 * 4)(*c.pml))
+    if rc != 0:
+        raise RuntimeError(f"oracle tangent failed ({rc})")
+    return r, dr
+
+
+def bf16_encode(x: np.ndarray) -> np.ndarray:
+    """Round-to-nearest-even fp32 -> bf16 bits (same rule as the kernels)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    u = u + 0x7FFF + ((u >> 16) & 1)
+    return ((u >> 16) & 0xFFFF).astype(np.uint16)
+
+
+def bf16_decode(h: np.ndarray) -> np.ndarray:
+    return (h.astype(np.uint32) << 16).view(np.float32)
+
+
+# ------------------------------------------------------------ reference lib
+def ref_tm2d_forward(case: TMCase, *, n_threads: int = 0, nt: int | None = None):
+    """The reference's own CPU forward (``maxwell_tm_<s>_<dtype>_forward_cpu``).
+
+    Returns ``(fields, r)``; ``n_threads`` 0 lets OpenMP decide."""
+    lib = ref_lib()
+    dt_ = case.dtype
+    name = "double" if dt_ == np.float64 else "float"
+    fn = getattr(lib, f"maxwell_tm_{case.stencil}_{name}_forward_cpu")
+    ct = ctypes.c_double if dt_ == np.float64 else ctypes.c_float
+    I, P, B = ctypes.c_int64, ctypes.c_void_p, ctypes.c_bool
+    fn.restype = None
+    fn.argtypes = ([P] * 17 + [I] + [P] * 14 + [ct] * 3 + [I] * 7 + [B] * 4 + [I] * 7 + [P])
+    nt = case.nt if nt is None else nt
+    fields = [np.ascontiguousarray(x.copy()) for x in case.fields]
+    n_rec = case.rec.shape[1]
+    r = np.zeros((nt, case.n_shots, n_rec), dt_)
+    b = case.ca.shape[0] != 1
+    fn(_p(case.ca), _p(case.cb), _p(case.cq), _p(case.f), *[_p(x) for x in fields],
+       None, None, None, None, None, _p(r), 0, *[_p(p) for p in case.profiles],
+       _p(case.src), _p(case.rec), case.rdy, case.rdx, case.dt, nt, case.n_shots,
+       case.ny, case.nx, case.src.shape[1], n_rec, 1, False, b, b, b, 0,
+       *case.pml, n_threads, 0, None)
+    return fields, r
+
+
+def ref_tm2d_forward_with_storage_and_backward(case: TMCase, grad_r: np.ndarray, *, n_threads: int = 0):
+    """The reference CPU forward_with_storage followed by its CPU backward.
+
+    Used only to time the reference CPU path (its CPU backward re-runs the
+    forward kernels on the adjoint fields and is not the exact adjoint)."""
+    lib = ref_lib()
+    dt_ = case.dtype
+    name = "double" if dt_ == np.float64 else "float"
+    ct = ctypes.c_double if dt_ == np.float64 else ctypes.c_float
+    I, P, B = ctypes.c_int64, ctypes.c_void_p, ctypes.c_bool
+    fws = getattr(lib, f"maxwell_tm_{case.stencil}_{name}_forward_with_storage_cpu")
+    fws.restype = None
+    fws.argtypes = [P] * 12 + [P] * 6 + [P] * 14 + [ct] * 3 + [I] * 7 + [I] * 3 + [B] * 2 + [B] * 3 + [I] * 7 + [P] * 2
+    bwd = getattr(lib, f"maxwell_tm_{case.stencil}_{name}_backward_cpu")
+    bwd.restype = None
+    bwd.argtypes = [P] * 4 + [P] * 3 + [P] * 4 + [P] + [P] * 6 + [P] + [P] * 4 + [P] * 14 + [ct] * 3 + [I] * 7 + [I] * 3 + [B] * 5 + [I] * 7 + [P] * 2
+    N, ny, nx, nt = case.n_shots, case.ny, case.nx, case.nt
+    n_src, n_rec = case.src.shape[1], case.rec.shape[1]
+    fields = [np.ascontiguousarray(x.copy()) for x in case.fields]
+    r = np.zeros((nt, N, n_rec), dt_)
+    es = np.zeros((nt, N, ny, nx), dt_)
+    cs = np.zeros_like(es)
+    shot_bytes = ny * nx * dt_.itemsize
+    fws(_p(case.ca), _p(case.cb), _p(case.cq), _p(case.f), *[_p(x) for x in fields], _p(r),
+        _p(es), _p(es), None, _p(cs), _p(cs), None, *[_p(p) for p in case.profiles],
+        _p(case.src), _p(case.rec), case.rdy, case.rdx, case.dt, nt, N, ny, nx, n_src, n_rec, 1,
+        0, 0, shot_bytes, True, True, False, False, False, 0, *case.pml, n_threads, 0, None, None)
+    lam = [np.zeros((N, ny, nx), dt_) for _ in range(7)]
+    scratch = np.zeros((4, N, ny, nx), dt_)
+    grad_f = np.zeros((nt, N, n_src), dt_)
+    gca = np.zeros((ny, nx), dt_)
+    gcb = np.zeros((ny, nx), dt_)
+    gcas = np.zeros((N, ny, nx), dt_)
+    gcbs = np.zeros((N, ny, nx), dt_)
+    grad_r = np.ascontiguousarray(grad_r, dt_)
+    bwd(_p(case.ca), _p(case.cb), _p(case.cq), _p(grad_r), *[_p(x) for x in lam], _p(scratch),
+        _p(es), _p(es), None, _p(cs), _p(cs), None, _p(grad_f), _p(gca), _p(gcb), _p(gcas), _p(gcbs),
+        *[_p(p) for p in case.profiles], _p(case.src), _p(case.rec), case.rdy, case.rdx, case.dt,
+        nt, N, ny, nx, n_src, n_rec, 1, 0, 0, shot_bytes, True, True, False, False, False, nt,
+        *case.pml, n_threads, 0, None, None)
+    return r, gca, gcb

@@ -1,0 +1,95 @@
+// Shared host/device helpers for the MI355X (gfx950) TIDE propagators.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace tide_hip {
+
+// ------------------------------------------------------------------ errors
+// Every C-ABI entry point returns 0 on success or a nonzero code; the text
+// of the last failure on the calling thread is kept for tide_last_error().
+enum Status : int {
+  kOk = 0,
+  kInvalidArgument = 1,
+  kUnsupported = 2,
+  kHipError = 3,
+  kOutOfMemory = 4,
+};
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+#define TIDE_HIP_TRY(expr)                                                   \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      ::tide_hip::set_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+      return _e == hipErrorOutOfMemory ? ::tide_hip::kOutOfMemory            \
+                                       : ::tide_hip::kHipError;              \
+    }                                                                        \
+  } while (0)
+
+// --------------------------------------------------------------- BF16 codec
+// Round-to-nearest-even on the upper 16 bits, identical on host and device.
+__host__ __device__ __forceinline__ uint16_t f32_to_bf16(float v) {
+  uint32_t u = __builtin_bit_cast(uint32_t, v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
+  return __builtin_bit_cast(float, ((uint32_t)h) << 16);
+}
+
+// -------------------------------------------------------------- point table
+// Sources and receivers are applied inside the step kernels by the thread
+// that owns the cell.  The table groups the (shot, point) entries by padded
+// row, shot and 64-column tile so that a wavefront walking row y reads only
+// the handful of entries that fall in its columns:
+//   row_slot[y]                      -1 or the index of row y among point rows
+//   seg[(slot*n_shots+shot)*n_xt+xt] first entry of that (row, shot, tile);
+//                                     seg[last+1] is the end
+//   ent[e] = {xloc | kind<<8, q}      kind 0 = source (q indexes f[t]),
+//                                     kind 1 = receiver (q indexes r[t]);
+//                                     sources precede receivers in a segment,
+//                                     each kind in ascending point order.
+struct PointTable {
+  std::vector<int32_t> row_slot;
+  std::vector<int32_t> seg;
+  std::vector<int2> ent;
+  int32_t n_slots = 0;
+  int32_t n_xt = 0;
+};
+
+// Builds the table from host copies of sources_i / receivers_i ([n_shots, n]
+// flat indices into one shot's [ny, nx] plane; negative entries are
+// skipped).  Returns false (with set_error) on an index outside the plane.
+bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
+                       int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
+                       int tile_w, PointTable &out);
+
+// Device copy of a PointTable; owns three device allocations.
+struct DevicePointTable {
+  int32_t *row_slot = nullptr;
+  int32_t *seg = nullptr;
+  int2 *ent = nullptr;
+  int32_t n_xt = 0;
+  int upload(const PointTable &t, hipStream_t stream);
+  void release(hipStream_t stream);
+};
+
+// ----------------------------------------------------------- tile mapping
+// Maps a flat workgroup id onto (tile, group) so that the 8 XCDs of the
+// MI355X each receive a contiguous band of tiles.  Workgroups are dispatched
+// to XCDs round-robin, so id b runs on XCD b % 8; the band of XCD k starts at
+// k*per + min(k, rem).  The map is a bijection on [0, n).
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
+  const int64_t per = n >> 3, rem = n & 7;
+  const int64_t xcd = b & 7, idx = b >> 3;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+}  // namespace tide_hip

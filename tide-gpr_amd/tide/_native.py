@@ -1,0 +1,230 @@
+"""ctypes binding to the MI355X HIP library (``_lib/libtide_hip.so``).
+
+The entry points keep the reference ABI: symbol names
+``maxwell_tm_{stencil}_{dtype}_{pass}_cuda`` and argument order of the
+reference's ctypes specs (reference ``src/tide/backend_utils.py:227-308``,
+symbol naming ``:771-781``).  They return an ``int`` status instead of
+``void``; a nonzero status is raised here as :class:`TideNativeError` with
+the library's last error message.  There is no fallback: a missing library
+is an error.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Any
+
+import torch
+
+_LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libtide_hip.so"
+_lib: ctypes.CDLL | None = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int64
+_B = ctypes.c_bool
+
+
+class TideNativeError(RuntimeError):
+    """A HIP entry point returned a failure status."""
+
+
+def library_path() -> Path:
+    return Path(os.environ.get("TIDE_HIP_LIBRARY", _LIB_PATH))
+
+
+def _tm_forward_types(f: Any) -> list[Any]:
+    return [
+        *([_P] * 3),  # ca, cb, cq
+        _P,  # f
+        *([_P] * 3),  # ey, hx, hz
+        *([_P] * 4),  # m_ey_x, m_ey_z, m_hx_z, m_hz_x
+        *([_P] * 5),  # debye_a, debye_b, debye_cp, polarization, ey_prev
+        _P,  # r
+        _I,  # n_poles
+        *([_P] * 12),  # ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh
+        _P,
+        _P,  # sources_i, receivers_i
+        f,
+        f,
+        f,  # rdy, rdx, dt
+        *([_I] * 7),  # nt, n_shots, ny, nx, n_src, n_rec, step_ratio
+        _B,  # has_dispersion
+        *([_B] * 3),  # ca_batched, cb_batched, cq_batched
+        _I,  # start_t
+        *([_I] * 4),  # pml_y0, pml_x0, pml_y1, pml_x1
+        _I,
+        _I,  # n_threads, device
+        _P,  # compute_stream
+    ]
+
+
+def _tm_forward_storage_types(f: Any) -> list[Any]:
+    return [
+        *([_P] * 3),  # ca, cb, cq
+        _P,  # f
+        *([_P] * 3),  # ey, hx, hz
+        *([_P] * 4),  # m_ey_x, m_ey_z, m_hx_z, m_hz_x
+        _P,  # r
+        *([_P] * 6),  # ey_store_1, ey_store_3, ey_filenames, curl_store_1/3, curl_filenames
+        *([_P] * 12),
+        _P,
+        _P,
+        f,
+        f,
+        f,
+        *([_I] * 7),
+        *([_I] * 3),  # storage_mode, storage_format, shot_bytes_uncomp
+        _B,
+        _B,  # ca_requires_grad, cb_requires_grad
+        *([_B] * 3),
+        _I,
+        *([_I] * 4),
+        _I,
+        _I,
+        _P,
+        _P,  # compute_stream, storage_stream
+    ]
+
+
+def _tm_backward_types(f: Any) -> list[Any]:
+    return [
+        *([_P] * 3),  # ca, cb, cq
+        _P,  # grad_r
+        *([_P] * 3),  # lambda_ey, lambda_hx, lambda_hz
+        *([_P] * 4),  # m_lambda_*
+        _P,  # adjoint_memory_scratch
+        *([_P] * 6),  # storage pointers / filenames
+        _P,  # grad_f
+        _P,
+        _P,  # grad_ca, grad_cb
+        _P,
+        _P,  # grad_ca_shot, grad_cb_shot
+        *([_P] * 12),
+        _P,
+        _P,
+        f,
+        f,
+        f,
+        *([_I] * 7),
+        *([_I] * 3),
+        _B,
+        _B,
+        *([_B] * 3),
+        _I,
+        *([_I] * 4),
+        _I,
+        _I,
+        _P,
+        _P,
+    ]
+
+
+def _tm_backward_f64acc_types(f: Any) -> list[Any]:
+    return [
+        *([_P] * 3),  # ca, cb, cq
+        _P,  # grad_r
+        *([_P] * 3),  # lambda fields
+        *([_P] * 4),  # lambda memories
+        _P,
+        _P,  # ey_store, curl_store
+        _P,  # grad_f
+        _P,
+        _P,  # grad_ca (fp64), grad_cb (fp64)
+        *([_P] * 12),
+        _P,
+        _P,
+        f,
+        f,  # rdy, rdx
+        *([_I] * 7),  # nt, n_shots, ny, nx, n_src, n_rec, step_ratio
+        _I,
+        _I,  # storage_format, shot_bytes_uncomp
+        *([_B] * 3),
+        _I,  # start_t
+        *([_I] * 4),
+        _I,  # device
+        _P,  # compute_stream
+    ]
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library, raising if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not path.exists():
+        raise TideNativeError(
+            f"HIP library {path} is missing; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'`."
+        )
+    lib = ctypes.CDLL(str(path))
+    lib.tide_last_error.restype = ctypes.c_char_p
+    lib.tide_last_error.argtypes = []
+    for name, ctype in (("float", ctypes.c_float), ("double", ctypes.c_double)):
+        for pass_name, types in (
+            ("forward", _tm_forward_types(ctype)),
+            ("forward_with_storage", _tm_forward_storage_types(ctype)),
+            ("backward", _tm_backward_types(ctype)),
+        ):
+            fn = getattr(lib, f"maxwell_tm_2_{name}_{pass_name}_cuda")
+            fn.restype = ctypes.c_int
+            fn.argtypes = types
+        fn = getattr(lib, f"tide_tm2d_{name}_backward_f64acc")
+        fn.restype = ctypes.c_int
+        fn.argtypes = _tm_backward_f64acc_types(ctype)
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = load().tide_last_error().decode(errors="replace")
+        raise TideNativeError(f"HIP propagator failed (status {status}): {msg}")
+
+
+def dtype_name(dtype: torch.dtype) -> str:
+    if dtype == torch.float32:
+        return "float"
+    if dtype == torch.float64:
+        return "double"
+    raise TypeError(f"unsupported field dtype {dtype}; use float32 or float64")
+
+
+def get_function(propagator: str, pass_name: str, stencil: int, dtype: torch.dtype):
+    """Resolve ``{propagator}_{stencil}_{dtype}_{pass}_cuda``."""
+    lib = load()
+    name = f"{propagator}_{stencil}_{dtype_name(dtype)}_{pass_name}_cuda"
+    try:
+        return getattr(lib, name)
+    except AttributeError as exc:
+        raise TideNativeError(
+            f"{name} is not built; stencil {stencil} is not part of this build"
+        ) from exc
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    """Raw pointer of a contiguous tensor (None for absent/empty)."""
+    if t is None or t.numel() == 0:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("native arguments must be contiguous tensors")
+    return t.data_ptr()
+
+
+def stream_handle(device: torch.device) -> int | None:
+    """The HIP stream of torch's current stream on ``device``."""
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+__all__ = [
+    "TideNativeError",
+    "check",
+    "dtype_name",
+    "get_function",
+    "library_path",
+    "load",
+    "ptr",
+    "stream_handle",
+]

@@ -1,0 +1,31 @@
+from .contracts import (
+    CPML,
+    Acquisition,
+    Discretization,
+    EMDirection,
+    EMModel,
+    ExecutionOptions,
+    Experiment,
+    Observers,
+    SourceConvention,
+)
+from .operators import Maxwell3D, MaxwellTM
+from .results import EM3DState, EMGradient, ForwardResult, TMState
+
+__all__ = [
+    "Acquisition",
+    "CPML",
+    "Discretization",
+    "EM3DState",
+    "EMDirection",
+    "EMGradient",
+    "EMModel",
+    "ExecutionOptions",
+    "Experiment",
+    "ForwardResult",
+    "Maxwell3D",
+    "MaxwellTM",
+    "Observers",
+    "SourceConvention",
+    "TMState",
+]

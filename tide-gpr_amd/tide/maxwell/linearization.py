@@ -1,0 +1,101 @@
+"""Adjoint Jacobian of the Maxwell operators (reference
+``src/tide/maxwell/linearization.py:36-118``).
+
+``op.linearize(model)`` runs the primal once with a differentiable model and
+``.vjp(cotangent)`` applies ``J(model)^T`` to a receiver cotangent through
+the hand-written adjoint.  The tangent (``jvp``) and second-order products
+belong to the Born family, which is not part of this build.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Iterable
+
+import torch
+
+from ..core import StorageOptions
+from .contracts import EMModel
+from .results import EMGradient, ForwardResult
+
+_TARGETS = ("epsilon", "sigma", "mu")
+
+
+class _LinearizedMaxwell:
+    def __init__(self, operator, model: EMModel, *, storage: StorageOptions,
+                 targets: Iterable[str] = ("epsilon", "sigma")) -> None:
+        target_set = frozenset(targets)
+        unknown = target_set - set(_TARGETS)
+        if unknown:
+            raise ValueError(f"Unknown linearization targets: {sorted(unknown)!r}.")
+        if not target_set:
+            raise ValueError("At least one linearization target is required.")
+        if "mu" in target_set:
+            raise NotImplementedError("gradients with respect to mu are not part of this build")
+        if storage != operator.storage:
+            operator = type(operator)(
+                operator.discretization, operator.experiment,
+                execution=operator.execution, storage=storage,
+                model_gradient_sampling_interval=operator.model_gradient_sampling_interval,
+            )
+        self.operator = operator
+        self.model = EMModel(
+            epsilon=model.epsilon.detach().requires_grad_("epsilon" in target_set),
+            sigma=model.sigma.detach().requires_grad_("sigma" in target_set),
+            mu=model.mu.detach(),
+            dispersion=model.dispersion,
+        )
+        self.storage = storage
+        self.targets = target_set
+        self._primal: ForwardResult | None = None
+        self._closed = False
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def close(self) -> None:
+        self._primal = None
+        self._closed = True
+
+    @property
+    def primal(self) -> ForwardResult:
+        if self._closed:
+            raise RuntimeError("The linearized Maxwell operator is closed.")
+        if self._primal is None:
+            self._primal = self.operator.forward(self.model)
+        return self._primal
+
+    def _parameters(self) -> tuple[torch.Tensor, ...]:
+        return tuple(getattr(self.model, n) for n in _TARGETS if n in self.targets)
+
+    def vjp(self, cotangent: torch.Tensor) -> EMGradient:
+        """``J(model)^T cotangent`` as an :class:`EMGradient`."""
+        receiver_data = self.primal.receiver_data
+        if cotangent.shape != receiver_data.shape:
+            raise ValueError("Receiver cotangent must match primal receiver data shape.")
+        grads = torch.autograd.grad(
+            receiver_data, self._parameters(), cotangent, retain_graph=True, allow_unused=True
+        )
+        values = {n: None for n in _TARGETS}
+        for n, g in zip((n for n in _TARGETS if n in self.targets), grads):
+            values[n] = g
+        return EMGradient(**values)
+
+    def jvp(self, direction):
+        raise NotImplementedError("Born tangent products are not part of this build")
+
+    def second_vjp(self, *args, **kwargs):
+        raise NotImplementedError("second-order products are not part of this build")
+
+
+class LinearizedMaxwellTM(_LinearizedMaxwell):
+    pass
+
+
+class LinearizedMaxwell3D(_LinearizedMaxwell):
+    pass
+
+
+__all__ = ["LinearizedMaxwell3D", "LinearizedMaxwellTM"]

@@ -1,0 +1,497 @@
+"""2-D TM propagation on the MI355X HIP kernels.
+
+``maxwelltm`` keeps the reference entry point's signature and semantics
+(reference ``src/tide/maxwell/tm2d.py:258-745`` and the native adapter
+``tm2d_cuda.py:29-823``):
+
+* the model is replicate-padded by ``fd_pad + pml`` per side
+  (``fd_pad = [s/2, s/2-1, s/2, s/2-1]``) so its gradient folds back onto
+  the edge cells;
+* ``ca, cb, cq`` come from ``utils.prepare_parameters``; the source term is
+  ``f[t, shot, k] = amplitude * cb[src] * (-1/(dy dx))``, differentiable in
+  both factors;
+* :class:`TM2DPropagate` runs the forward kernels and, in its backward, the
+  exact adjoint, returning ``dJ/dca``, ``dJ/dcb`` (FP64-accumulated) and
+  ``dJ/df``; torch's autograd then applies the coefficient, padding and
+  source-scaling chain rules exactly as in the reference;
+* outputs are the unpadded final state and receivers ``[nt, shots, n_rec]``.
+
+Only the HIP device path exists.  CPU tensors, the Python reference
+backend, callbacks, dispersion, stencils other than 2, host-staged storage
+and internal resampling (step_ratio > 1) raise ``NotImplementedError``.
+"""
+
+from __future__ import annotations
+
+import math
+from collections.abc import Sequence
+from dataclasses import dataclass, field
+from typing import Any
+
+import torch
+
+from .. import _native, staggered
+from ..cfl import cfl_condition
+from ..grid_utils import _normalize_grid_spacing_2d, _normalize_pml_width_2d
+from ..padding import create_or_pad, zero_interior
+from ..storage import (
+    STORAGE_DEVICE,
+    STORAGE_FORMAT_BF16,
+    STORAGE_NONE,
+    _resolve_storage_compression,
+    resolve_storage_mode,
+)
+from ..utils import C0, compile_material_coefficients
+
+_FIELD_NAMES = ("Ey", "Hx", "Hz", "m_Ey_x", "m_Ey_z", "m_Hx_z", "m_Hz_x")
+
+
+@dataclass
+class TM2DSetup:
+    """Everything the kernels read for one propagation, in padded layout."""
+
+    ca: torch.Tensor  # [1|N, ny, nx]
+    cb: torch.Tensor
+    cq: torch.Tensor
+    f: torch.Tensor  # [nt, N, n_src] (empty when no sources)
+    profiles: list[torch.Tensor]  # ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh
+    sources_i: torch.Tensor  # [N, n_src] int64, CPU
+    receivers_i: torch.Tensor  # [N, n_rec] int64, CPU
+    fields: list[torch.Tensor]  # 7 x [N, ny, nx]
+    rdy: float
+    rdx: float
+    dt: float
+    nt: int
+    n_shots: int
+    ny: int
+    nx: int
+    n_src: int
+    n_rec: int
+    pml: tuple[int, int, int, int]  # pml_y0, pml_x0, pml_y1, pml_x1
+    fd_pad: list[int]
+    pml_width: list[int]
+    batched: bool
+    extra: dict[str, Any] = field(default_factory=dict)
+
+    def unpad(self, t: torch.Tensor) -> torch.Tensor:
+        p = self.fd_pad
+        return t[:, p[0] : self.ny - p[1] if p[1] > 0 else None, p[2] : self.nx - p[3] if p[3] > 0 else None]
+
+
+def _flat_indices(loc: torch.Tensor | None, pad_y: int, pad_x: int, nx_p: int) -> tuple[torch.Tensor, int]:
+    if loc is None or loc.numel() == 0:
+        return torch.empty(0, dtype=torch.long), 0
+    loc = loc.detach().to("cpu")
+    idx = ((loc[..., 0] + pad_y) * nx_p + (loc[..., 1] + pad_x)).long().contiguous()
+    return idx, int(loc.shape[1])
+
+
+def prepare_tm2d(
+    epsilon: torch.Tensor,
+    sigma: torch.Tensor,
+    mu: torch.Tensor,
+    grid_spacing: float | Sequence[float],
+    dt: float,
+    source_amplitude: torch.Tensor | None,
+    source_location: torch.Tensor | None,
+    receiver_location: torch.Tensor | None,
+    *,
+    stencil: int = 2,
+    pml_width: int | Sequence[int] = 20,
+    nt: int | None = None,
+    initial: Sequence[torch.Tensor | None] = (None,) * 7,
+    validate_material_inputs: bool = True,
+) -> TM2DSetup:
+    """Pad the model, compile coefficients and profiles, map the points.
+
+    ``epsilon`` is ``[ny, nx]`` (shared by all shots) or ``[N, ny, nx]`` (one
+    model per shot).  Device-agnostic: runs on CPU tensors too, so the host
+    numerics can be tested without a GPU.
+    """
+    if epsilon.ndim not in (2, 3):
+        raise RuntimeError("epsilon must be 2D or batched 3D")
+    if sigma.shape != epsilon.shape or mu.shape != epsilon.shape:
+        raise RuntimeError("sigma and mu must have the same shape as epsilon")
+    device, dtype = epsilon.device, epsilon.dtype
+    batched = epsilon.ndim == 3
+    dy, dx = _normalize_grid_spacing_2d(grid_spacing)
+    widths = _normalize_pml_width_2d(pml_width)
+    if nt is None:
+        if source_amplitude is None:
+            raise ValueError("Either nt or source_amplitude must be provided")
+        nt = int(source_amplitude.shape[-1])
+    nt = int(nt)
+    if source_amplitude is not None and source_amplitude.numel() > 0:
+        n_shots = int(source_amplitude.shape[0])
+    elif source_location is not None and source_location.numel() > 0:
+        n_shots = int(source_location.shape[0])
+    elif receiver_location is not None and receiver_location.numel() > 0:
+        n_shots = int(receiver_location.shape[0])
+    else:
+        n_shots = 1
+    if batched and int(epsilon.shape[0]) != n_shots:
+        raise RuntimeError("Batched model count must match the effective shot count.")
+
+    half = stencil // 2
+    fd_pad = [half, half - 1, half, half - 1]
+    total = [a + b for a, b in zip(fd_pad, widths)]
+    ny = int(epsilon.shape[-2]) + total[0] + total[1]
+    nx = int(epsilon.shape[-1]) + total[2] + total[3]
+    size = (n_shots, ny, nx) if batched else (ny, nx)
+    eps_p = create_or_pad(epsilon, total, device, dtype, size, mode="replicate")
+    sig_p = create_or_pad(sigma, total, device, dtype, size, mode="replicate")
+    mu_p = create_or_pad(mu, total, device, dtype, size, mode="replicate")
+    coef = compile_material_coefficients(eps_p, sig_p, mu_p, dt, validate_inputs=validate_material_inputs)
+    shape3 = (lambda t: t.contiguous()) if batched else (lambda t: t[None].contiguous())
+    ca, cb, cq = shape3(coef["ca"]), shape3(coef["cb"]), shape3(coef["cq"])
+
+    sources_i, n_src = _flat_indices(source_location, total[0], total[2], nx)
+    receivers_i, n_rec = _flat_indices(receiver_location, total[0], total[2], nx)
+
+    source_coeff = -1.0 / (dx * dy)
+    if n_src > 0 and source_amplitude is not None and source_amplitude.numel() > 0:
+        flat_cb = cb.reshape(cb.shape[0], ny * nx).expand(n_shots, -1)
+        cb_src = flat_cb.gather(1, sources_i.to(device))
+        amp = source_amplitude.permute(2, 0, 1).contiguous().to(dtype)
+        f = amp * cb_src.to(dtype).unsqueeze(0)
+        f = (f * source_coeff).contiguous()
+    else:
+        f = torch.empty(0, device=device, dtype=dtype)
+
+    fields = []
+    for t0 in initial:
+        if t0 is None:
+            fields.append(torch.zeros(n_shots, ny, nx, device=device, dtype=dtype))
+            continue
+        t0 = t0.to(device=device, dtype=dtype)
+        if t0.ndim == 2:
+            t0 = t0[None].expand(n_shots, -1, -1)
+        if tuple(t0.shape) == (n_shots, ny, nx):
+            fields.append(t0.detach().clone().contiguous())
+        else:
+            fields.append(
+                create_or_pad(t0.detach(), fd_pad, device, dtype, (n_shots, ny, nx)).contiguous()
+            )
+    for wf, dim in zip(fields[3:], (1, 0, 0, 1)):
+        zero_interior(wf, fd_pad, widths, dim)
+
+    max_vel = float((1.0 / torch.sqrt(epsilon.detach() * mu.detach())).max().item()) * C0
+    prof = staggered.set_pml_profiles(
+        pml_width=widths,
+        accuracy=stencil,
+        fd_pad=fd_pad,
+        dt=dt,
+        grid_spacing=[dy, dx],
+        max_vel=max_vel,
+        dtype=dtype,
+        device=device,
+        pml_freq=0.5 / dt,
+        ny=ny,
+        nx=nx,
+    )
+    ay, ayh, ax, axh, by, byh, bx, bxh, ky, kyh, kx, kxh = [p.reshape(-1).contiguous() for p in prof]
+    pml = (total[0], total[2], ny - fd_pad[1] - widths[1], nx - fd_pad[3] - widths[3])
+    return TM2DSetup(
+        ca=ca, cb=cb, cq=cq, f=f,
+        profiles=[ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh],
+        sources_i=sources_i, receivers_i=receivers_i, fields=fields,
+        rdy=1.0 / dy, rdx=1.0 / dx, dt=float(dt), nt=nt, n_shots=n_shots,
+        ny=ny, nx=nx, n_src=n_src, n_rec=n_rec, pml=pml, fd_pad=fd_pad,
+        pml_width=widths, batched=batched,
+    )
+
+
+def _profile_ptrs(profiles: list[torch.Tensor]) -> list[int | None]:
+    return [_native.ptr(p) for p in profiles]
+
+
+class TM2DPropagate(torch.autograd.Function):
+    """Forward kernels + exact adjoint behind torch autograd.
+
+    Inputs: ``ca, cb, cq`` ``[1|N, ny, nx]``, source term ``f``
+    ``[nt, N, n_src]``; the setup object and storage policy ride along.
+    Returns the 7 padded final fields and receivers ``[nt, N, n_rec]``.
+    Gradients flow to ``ca``, ``cb`` and ``f`` from the receiver
+    cotangent; cotangents of the final state are ignored, as in the
+    reference (``tm2d_autograd.py:519-547``).
+    """
+
+    @staticmethod
+    def forward(ctx, ca, cb, cq, f, setup: TM2DSetup, storage: dict[str, Any]):
+        dev = ca.device
+        dtype = ca.dtype
+        fields = [t.contiguous() for t in setup.fields]
+        r = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
+        f_c = f.detach().contiguous()
+        stream = _native.stream_handle(dev)
+        dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        interval = storage["interval"]
+        want_ca = ctx.needs_input_grad[0]
+        want_cb = ctx.needs_input_grad[1]
+        store = storage["mode"] == "device" and (want_ca or want_cb)
+        common_tail = (
+            _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
+            setup.rdy, setup.rdx, setup.dt,
+            setup.nt, setup.n_shots, setup.ny, setup.nx, setup.n_src, setup.n_rec,
+        )
+        ey_store = curl_store = None
+        if store:
+            n_store = (setup.nt + interval - 1) // interval
+            sdtype = torch.int16 if storage["format"] == STORAGE_FORMAT_BF16 else dtype
+            shape = (n_store, setup.n_shots, setup.ny, setup.nx)
+            ey_store = torch.empty(shape, device=dev, dtype=sdtype) if want_ca else None
+            curl_store = torch.empty(shape, device=dev, dtype=sdtype) if want_cb else None
+            fn = _native.get_function("maxwell_tm", "forward_with_storage", 2, dtype)
+            shot_bytes = setup.ny * setup.nx * (2 if storage["format"] == STORAGE_FORMAT_BF16 else dtype.itemsize)
+            status = fn(
+                _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f_c),
+                *[_native.ptr(t) for t in fields], _native.ptr(r),
+                _native.ptr(ey_store), None, None, _native.ptr(curl_store), None, None,
+                *_profile_ptrs(setup.profiles),
+                *common_tail[:5], *common_tail[5:], interval,
+                STORAGE_DEVICE, storage["format"], shot_bytes, want_ca, want_cb,
+                setup.batched, setup.batched, setup.batched, 0,
+                *setup.pml, 0, dev_idx, stream, stream,
+            )
+        else:
+            fn = _native.get_function("maxwell_tm", "forward", 2, dtype)
+            status = fn(
+                _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f_c),
+                *[_native.ptr(t) for t in fields],
+                None, None, None, None, None, _native.ptr(r), 0,
+                *_profile_ptrs(setup.profiles),
+                *common_tail, interval, False,
+                setup.batched, setup.batched, setup.batched, 0,
+                *setup.pml, 0, dev_idx, stream,
+            )
+        _native.check(status)
+        ctx.setup = setup
+        ctx.storage = storage
+        ctx.stored = store
+        ctx.save_for_backward(ca, cb, cq, ey_store, curl_store)
+        ctx.mark_non_differentiable(*fields)
+        return (*fields, r)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        grad_r = grads[-1]
+        setup: TM2DSetup = ctx.setup
+        ca, cb, cq, ey_store, curl_store = ctx.saved_tensors
+        want_ca = ctx.needs_input_grad[0]
+        want_cb = ctx.needs_input_grad[1]
+        want_f = ctx.needs_input_grad[3] and setup.n_src > 0
+        if (want_ca or want_cb) and not ctx.stored:
+            raise RuntimeError(
+                "model gradients need stored snapshots; storage_mode='none' was used"
+            )
+        dev, dtype = ca.device, ca.dtype
+        shape = (setup.n_shots, setup.ny, setup.nx)
+        if grad_r is None:
+            grad_r = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
+        grad_r = grad_r.to(dtype).contiguous()
+        lam = [torch.zeros(shape, device=dev, dtype=dtype) for _ in range(7)]
+        grad_f = (
+            torch.zeros(setup.nt, setup.n_shots, setup.n_src, device=dev, dtype=dtype)
+            if want_f else None
+        )
+        gshape = shape if setup.batched else (setup.ny, setup.nx)
+        gca = torch.zeros(gshape, device=dev, dtype=torch.float64) if want_ca else None
+        gcb = torch.zeros(gshape, device=dev, dtype=torch.float64) if want_cb else None
+        fmt = ctx.storage["format"]
+        shot_bytes = setup.ny * setup.nx * (2 if fmt == STORAGE_FORMAT_BF16 else dtype.itemsize)
+        fn = getattr(_native.load(), f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
+        stream = _native.stream_handle(dev)
+        dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        status = fn(
+            _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(grad_r),
+            *[_native.ptr(t) for t in lam],
+            _native.ptr(ey_store) if want_ca else None,
+            _native.ptr(curl_store) if want_cb else None,
+            _native.ptr(grad_f), _native.ptr(gca), _native.ptr(gcb),
+            *_profile_ptrs(setup.profiles),
+            _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
+            setup.rdy, setup.rdx, setup.nt, setup.n_shots, setup.ny, setup.nx,
+            setup.n_src, setup.n_rec, ctx.storage["interval"], fmt, shot_bytes,
+            setup.batched, setup.batched, setup.batched, setup.nt, *setup.pml,
+            dev_idx, stream,
+        )
+        _native.check(status)
+        out_ca = gca.to(dtype).reshape(ca.shape) if want_ca else None
+        out_cb = gcb.to(dtype).reshape(cb.shape) if want_cb else None
+        return out_ca, out_cb, None, grad_f, None, None
+
+
+def _reject_unsupported(
+    python_backend, stencil, forward_callback, backward_callback, dispersion, device
+) -> None:
+    if python_backend not in (False, "native"):
+        raise NotImplementedError(
+            "this build has one backend (HIP on MI355X); python_backend is not available"
+        )
+    if stencil != 2:
+        raise NotImplementedError(f"stencil {stencil} is not part of this build (stencil 2 is)")
+    if forward_callback is not None or backward_callback is not None:
+        raise NotImplementedError("wavefield callbacks are not part of this build")
+    if dispersion is not None:
+        raise NotImplementedError("Debye dispersion is not part of this build")
+    if device.type != "cuda":
+        raise NotImplementedError(
+            f"the MI355X build runs on HIP devices only; got tensors on {device}"
+        )
+
+
+def _normalize_batch(epsilon, sigma, mu, shot_tensors: dict[str, torch.Tensor | None]):
+    """Structured batches: ``[B, ny, nx]`` models with ``[B, S, ...]`` shot
+    tensors run as ``N = B*S`` shots with per-shot models (reference
+    ``maxwell/common.py:534-603``)."""
+    if epsilon.ndim == 2:
+        return epsilon, sigma, mu, shot_tensors, None
+    B = int(epsilon.shape[0])
+    S = None
+    for name, t in shot_tensors.items():
+        if t is None or t.numel() == 0:
+            continue
+        if t.ndim == 4:
+            if int(t.shape[0]) != B:
+                raise RuntimeError(f"{name} batch dimension must be {B}")
+            s = int(t.shape[1])
+        elif t.ndim == 3 and int(t.shape[0]) == B:
+            s = 1
+        else:
+            raise RuntimeError(f"{name} must be [B, S, ...] for a batched model")
+        if S is not None and s != S:
+            raise RuntimeError("All shot-batched tensors must agree on S")
+        S = s
+    S = S or 1
+    out = {}
+    for name, t in shot_tensors.items():
+        if t is None or t.numel() == 0:
+            out[name] = t
+        elif t.ndim == 4:
+            out[name] = t.reshape(B * S, *t.shape[2:])
+        else:
+            out[name] = t
+    rep = lambda m: m.repeat_interleave(S, dim=0).contiguous()  # noqa: E731
+    return rep(epsilon), rep(sigma), rep(mu), out, (B, S)
+
+
+def maxwelltm(
+    epsilon: torch.Tensor,
+    sigma: torch.Tensor,
+    mu: torch.Tensor,
+    grid_spacing: float | Sequence[float],
+    dt: float,
+    source_amplitude: torch.Tensor | None,
+    source_location: torch.Tensor | None,
+    receiver_location: torch.Tensor | None,
+    stencil: int = 2,
+    pml_width: int | Sequence[int] = 20,
+    max_vel: float | None = None,
+    Ey_0: torch.Tensor | None = None,
+    Hx_0: torch.Tensor | None = None,
+    Hz_0: torch.Tensor | None = None,
+    m_Ey_x: torch.Tensor | None = None,
+    m_Ey_z: torch.Tensor | None = None,
+    m_Hx_z: torch.Tensor | None = None,
+    m_Hz_x: torch.Tensor | None = None,
+    nt: int | None = None,
+    model_gradient_sampling_interval: int = 1,
+    freq_taper_frac: float = 0.0,
+    time_pad_frac: float = 0.0,
+    time_taper: bool = False,
+    save_snapshots: bool | None = None,
+    forward_callback: Any = None,
+    backward_callback: Any = None,
+    callback_frequency: int = 1,
+    python_backend: bool | str = False,
+    storage_mode: str = "device",
+    storage_path: str = ".",
+    storage_compression: bool | str = False,
+    storage_bytes_limit_device: int | None = None,
+    storage_bytes_limit_host: int | None = None,
+    storage_chunk_steps: int = 0,
+    n_threads: int | None = None,
+    dispersion: Any = None,
+    fallback: str = "reference",
+):
+    """2-D TM solver.  Returns ``(Ey, Hx, Hz, m_Ey_x, m_Ey_z, m_Hx_z,
+    m_Hz_x, receiver_amplitudes)``; fields unpadded ``[shots, ny, nx]``,
+    receivers ``[nt, shots, n_rec]`` (``[nt, B, S, n_rec]`` for a batch of
+    several models)."""
+    del freq_taper_frac, time_pad_frac, time_taper, storage_path, storage_chunk_steps
+    del n_threads, fallback, callback_frequency
+    _reject_unsupported(
+        python_backend, stencil, forward_callback, backward_callback, dispersion, epsilon.device
+    )
+    epsilon_in, sigma_in = epsilon, sigma
+    epsilon, sigma, mu, shots, bs = _normalize_batch(
+        epsilon, sigma, mu,
+        {
+            "source_amplitude": source_amplitude,
+            "source_location": source_location,
+            "receiver_location": receiver_location,
+        },
+    )
+    source_amplitude = shots["source_amplitude"]
+    source_location = shots["source_location"]
+    receiver_location = shots["receiver_location"]
+
+    spacing = _normalize_grid_spacing_2d(grid_spacing)
+    if max_vel is None:
+        max_vel = float((1.0 / torch.sqrt(epsilon.detach() * mu.detach())).max().item()) * C0
+    _, step_ratio = cfl_condition(spacing, dt, max_vel)
+    if step_ratio > 1:
+        raise NotImplementedError(
+            f"dt={dt} exceeds the CFL limit (needs {step_ratio} internal steps); "
+            "internal resampling is not part of this build, pass a stable dt"
+        )
+    interval = max(1, int(model_gradient_sampling_interval))
+    for name, t in (("Source", source_location), ("Receiver", receiver_location)):
+        if t is not None and t.numel() > 0:
+            hi = torch.tensor(epsilon.shape[-2:], device=t.device)
+            if bool((t >= hi).any()):
+                raise RuntimeError(f"{name} location out of bounds for model shape {tuple(epsilon.shape[-2:])}")
+
+    setup = prepare_tm2d(
+        epsilon, sigma, mu, spacing, dt, source_amplitude, source_location, receiver_location,
+        stencil=stencil, pml_width=pml_width, nt=nt,
+        initial=(Ey_0, Hx_0, Hz_0, m_Ey_x, m_Ey_z, m_Hx_z, m_Hz_x),
+    )
+    if setup.nt > 0:
+        interval = min(interval, setup.nt)
+
+    model_rg = epsilon_in.requires_grad or sigma_in.requires_grad
+    do_save = model_rg if save_snapshots is None else bool(save_snapshots)
+    kind, _, bpe, fmt = _resolve_storage_compression(
+        storage_compression, epsilon.dtype, epsilon.device, context="storage_compression"
+    )
+    n_store = (setup.nt + interval - 1) // interval
+    total_bytes = n_store * setup.n_shots * setup.ny * setup.nx * bpe * 2
+    mode = resolve_storage_mode(
+        storage_mode,
+        needs_storage=do_save and model_rg,
+        total_bytes=total_bytes,
+        bytes_limit_device=storage_bytes_limit_device,
+        bytes_limit_host=storage_bytes_limit_host,
+    )
+    if mode in ("cpu", "disk"):
+        raise NotImplementedError(
+            f"storage_mode='{mode}' (host-staged snapshots) is not part of this build; "
+            "use 'device' and split the shots into chunks that fit in HBM"
+        )
+    storage = {"mode": mode, "format": fmt, "interval": interval, "kind": kind}
+    f = setup.f if setup.f.numel() > 0 else torch.empty(0, device=epsilon.device, dtype=epsilon.dtype)
+    out = TM2DPropagate.apply(setup.ca, setup.cb, setup.cq, f, setup, storage)
+    fields = [setup.unpad(t) for t in out[:7]]
+    r = out[7]
+    if setup.n_rec == 0:
+        r = torch.empty(0, device=epsilon.device, dtype=epsilon.dtype)
+    if bs is not None and bs[0] > 1:
+        B, S = bs
+        fields = [t.reshape(B, S, *t.shape[1:]) for t in fields]
+        if r.numel() > 0:
+            r = r.reshape(r.shape[0], B, S, *r.shape[2:])
+    return (*fields, r)
+
+
+__all__ = ["TM2DPropagate", "TM2DSetup", "maxwelltm", "prepare_tm2d"]
