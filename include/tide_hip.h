@@ -1,0 +1,217 @@
+/*
+ * C ABI of the MI355X (gfx950) TIDE propagator library, libtide_hip.so.
+ *
+ * Conventions shared by every entry point:
+ *  - all array arguments are device pointers to contiguous, caller-owned
+ *    buffers that stay valid until the enqueued work completes; the library
+ *    keeps no pointer after returning.  sources_i / receivers_i may be host
+ *    or device pointers.
+ *  - fields are [n_shots, ny, nx] with x fastest (padded grid); ca/cb/cq are
+ *    [ny, nx], or [n_shots, ny, nx] when the matching *_batched flag is set.
+ *  - work is enqueued on `compute_stream` (a hipStream_t; NULL = the null
+ *    stream) and the call returns without waiting for it.
+ *  - the return value is 0 on success or a nonzero status; the message of
+ *    the last failure on the calling thread is returned by tide_last_error().
+ *    (The reference functions return void and exit the process on error.)
+ *  - storage_mode: 0 device, 3 none (1 cpu and 2 disk return status 2);
+ *    storage_format: 0 full precision, 1 BF16 (round-to-nearest-even).
+ * Only stencil order 2 is built; {dtype} is float or double.
+ */
+#ifndef TIDE_HIP_H
+#define TIDE_HIP_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *tide_last_error(void);
+
+/*
+ * Forward propagation without snapshots.
+ * Replaces maxwell_tm_2_{dtype}_forward_cuda: reference ABI
+ * src/tide/backend_utils.py:256-269 (_TM_FORWARD_SPEC), CUDA implementation
+ * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1281-1368.  The Debye
+ * arguments are accepted and must be unused (has_dispersion false).
+ */
+int maxwell_tm_2_float_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float const *debye_a, float const *debye_b,
+    float const *debye_cp, float *polarization, float *ey_prev, float *r,
+    int64_t n_poles, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+int maxwell_tm_2_double_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double const *debye_a,
+    double const *debye_b, double const *debye_cp, double *polarization,
+    double *ey_prev, double *r, int64_t n_poles, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax,
+    double const *bx, double const *axh, double const *bxh, double const *ky,
+    double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+/*
+ * Forward propagation storing, every step_ratio-th step t, the pre-update Ey
+ * and the CPML curl of H at ey_store_1 / curl_store_1 +
+ * (t / step_ratio) * shot_bytes_uncomp * n_shots.
+ * Replaces maxwell_tm_2_{dtype}_forward_with_storage_cuda: reference ABI
+ * src/tide/backend_utils.py:271-287, CUDA implementation
+ * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1371-1609.  ey_store_3,
+ * curl_store_3 and the filename arrays (host staging) are unused.
+ */
+int maxwell_tm_2_float_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float *r, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_2_double_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double *r, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax,
+    double const *bx, double const *axh, double const *bxh, double const *ky,
+    double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+/*
+ * Exact adjoint over nt steps ending at start_t: injects grad_r, writes
+ * grad_f, and adds dJ/dca, dJ/dcb (accumulated in FP64, summed over shots
+ * unless batched) into grad_ca / grad_cb.  grad_*_shot and
+ * adjoint_memory_scratch are not needed (pass NULL).
+ * Replaces maxwell_tm_2_{dtype}_backward_cuda: reference ABI
+ * src/tide/backend_utils.py:289-308, CUDA implementation
+ * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:2405-2783.
+ */
+int maxwell_tm_2_float_backward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *grad_r,
+    float *lambda_ey, float *lambda_hx, float *lambda_hz, float *m_lambda_ey_x,
+    float *m_lambda_ey_z, float *m_lambda_hx_z, float *m_lambda_hz_x,
+    float *adjoint_memory_scratch, void *ey_store_1, void *ey_store_3,
+    char const *const *ey_filenames, void *curl_store_1, void *curl_store_3,
+    char const *const *curl_filenames, float *grad_f, float *grad_ca,
+    float *grad_cb, float *grad_ca_shot, float *grad_cb_shot, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_2_double_backward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *grad_r,
+    double *lambda_ey, double *lambda_hx, double *lambda_hz,
+    double *m_lambda_ey_x, double *m_lambda_ey_z, double *m_lambda_hx_z,
+    double *m_lambda_hz_x, double *adjoint_memory_scratch, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double *grad_f,
+    double *grad_ca, double *grad_cb, double *grad_ca_shot,
+    double *grad_cb_shot, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh,
+    double const *kx, double const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, double rdy, double rdx, double dt, int64_t nt,
+    int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+/*
+ * Same adjoint as maxwell_tm_2_{dtype}_backward_cuda, with grad_ca / grad_cb
+ * as FP64 accumulators that are added to (so a caller running shot chunks
+ * keeps one FP64 sum across them).  Storage arguments are the device
+ * snapshot buffers written by forward_with_storage (NULL when that
+ * coefficient's gradient is not wanted).  No reference counterpart; the
+ * reference keeps fp32 per-shot accumulators
+ * (src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:834-871).
+ */
+int tide_tm2d_float_backward_f64acc(
+    float const *ca, float const *cb, float const *cq, float const *grad_r,
+    float *lambda_ey, float *lambda_hx, float *lambda_hz, float *m_lambda_ey_x,
+    float *m_lambda_ey_z, float *m_lambda_hx_z, float *m_lambda_hz_x,
+    void const *ey_store, void const *curl_store, float *grad_f,
+    double *grad_ca, double *grad_cb, float const *ay, float const *by,
+    float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh,
+    float const *kx, float const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, float rdy, float rdx, int64_t nt,
+    int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t device, void *compute_stream);
+
+int tide_tm2d_double_backward_f64acc(
+    double const *ca, double const *cb, double const *cq,
+    double const *grad_r, double *lambda_ey, double *lambda_hx,
+    double *lambda_hz, double *m_lambda_ey_x, double *m_lambda_ey_z,
+    double *m_lambda_hx_z, double *m_lambda_hz_x, void const *ey_store,
+    void const *curl_store, double *grad_f, double *grad_ca, double *grad_cb,
+    double const *ay, double const *by, double const *ayh, double const *byh,
+    double const *ax, double const *bx, double const *axh, double const *bxh,
+    double const *ky, double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t device, void *compute_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TIDE_HIP_H */

@@ -1,0 +1,122 @@
+"""Direct C-ABI calls of the HIP library on oracle-style numpy cases.
+
+The parity tests feed the kernels exactly the arrays the oracle sees
+(``oracle.TMCase``), so any difference is the kernels' own.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from oracle.oracle import TMCase
+from tide import _native
+from tide.storage import STORAGE_DEVICE
+
+
+def _dev(a, device, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device).contiguous()
+
+
+def gpu_forward(case: TMCase, *, store: str | None = None, interval: int = 1,
+                device="cuda:0", nt: int | None = None, start_t: int = 0,
+                fields: list | None = None):
+    """Run ``maxwell_tm_2_<dtype>_forward[_with_storage]_cuda``.
+
+    Returns numpy ``(fields, r, ey_store, curl_store)``."""
+    dev = torch.device(device)
+    dtype = torch.float64 if case.dtype == np.float64 else torch.float32
+    nt = case.nt if nt is None else nt
+    ca, cb, cq = (_dev(a, dev) for a in (case.ca, case.cb, case.cq))
+    f = _dev(case.f, dev)
+    prof = [_dev(p, dev) for p in case.profiles]
+    flds = [_dev(x, dev) for x in (fields if fields is not None else case.fields)]
+    N, ny, nx = case.n_shots, case.ny, case.nx
+    n_src, n_rec = case.src.shape[1], case.rec.shape[1]
+    r = torch.zeros(start_t + nt, N, n_rec, device=dev, dtype=dtype)
+    src = torch.from_numpy(np.ascontiguousarray(case.src, np.int64))
+    rec = torch.from_numpy(np.ascontiguousarray(case.rec, np.int64))
+    batched = case.ca.shape[0] != 1
+    stream = _native.stream_handle(dev)
+    P = _native.ptr
+    es = cs = None
+    if store is None:
+        fn = _native.get_function("maxwell_tm", "forward", 2, dtype)
+        st = fn(P(ca), P(cb), P(cq), P(f), *[P(x) for x in flds],
+                None, None, None, None, None, P(r), 0, *[P(p) for p in prof],
+                P(src), P(rec), case.rdy, case.rdx, case.dt, nt, N, ny, nx, n_src, n_rec,
+                interval, False, batched, batched, batched, start_t, *case.pml, 0,
+                dev.index or 0, stream)
+    else:
+        fmt = 1 if store == "bf16" else 0
+        n_store = (start_t + nt + interval - 1) // interval
+        sdt = torch.int16 if fmt == 1 else dtype
+        es = torch.zeros(n_store, N, ny, nx, device=dev, dtype=sdt)
+        cs = torch.zeros_like(es)
+        shot_bytes = ny * nx * (2 if fmt == 1 else case.dtype.itemsize)
+        fn = _native.get_function("maxwell_tm", "forward_with_storage", 2, dtype)
+        st = fn(P(ca), P(cb), P(cq), P(f), *[P(x) for x in flds], P(r),
+                P(es), None, None, P(cs), None, None, *[P(p) for p in prof],
+                P(src), P(rec), case.rdy, case.rdx, case.dt, nt, N, ny, nx, n_src, n_rec,
+                interval, STORAGE_DEVICE, fmt, shot_bytes, True, True,
+                batched, batched, batched, start_t, *case.pml, 0, dev.index or 0, stream, stream)
+    _native.check(st)
+    torch.cuda.synchronize(dev)
+    to_np = lambda t: None if t is None else t.cpu().numpy()  # noqa: E731
+    es_np, cs_np = to_np(es), to_np(cs)
+    if store == "bf16":
+        es_np, cs_np = es_np.view(np.uint16), cs_np.view(np.uint16)
+    return [to_np(x) for x in flds], to_np(r), es_np, cs_np
+
+
+def gpu_backward(case: TMCase, grad_r: np.ndarray, ey_store: np.ndarray | None,
+                 curl_store: np.ndarray | None, *, store: str = "full", interval: int = 1,
+                 device="cuda:0", reference_abi: bool = False):
+    """Run the adjoint.  ``reference_abi`` selects
+    ``maxwell_tm_2_<dtype>_backward_cuda`` (fp32/fp64 grad out) instead of
+    the FP64-accumulator entry.  Returns ``(grad_ca, grad_cb, grad_f, lam)``."""
+    dev = torch.device(device)
+    dtype = torch.float64 if case.dtype == np.float64 else torch.float32
+    ca, cb, cq = (_dev(a, dev) for a in (case.ca, case.cb, case.cq))
+    prof = [_dev(p, dev) for p in case.profiles]
+    N, ny, nx, nt = case.n_shots, case.ny, case.nx, case.nt
+    n_src, n_rec = case.src.shape[1], case.rec.shape[1]
+    lam = [torch.zeros(N, ny, nx, device=dev, dtype=dtype) for _ in range(7)]
+    gr = _dev(grad_r, dev, dtype)
+    gf = torch.zeros(nt, N, n_src, device=dev, dtype=dtype)
+    src = torch.from_numpy(np.ascontiguousarray(case.src, np.int64))
+    rec = torch.from_numpy(np.ascontiguousarray(case.rec, np.int64))
+    batched = case.ca.shape[0] != 1
+    gshape = (N, ny, nx) if batched else (ny, nx)
+    fmt = 1 if store == "bf16" else 0
+    shot_bytes = ny * nx * (2 if fmt == 1 else case.dtype.itemsize)
+    sdt = torch.int16 if fmt == 1 else dtype
+    es = None if ey_store is None else torch.from_numpy(np.ascontiguousarray(ey_store).view(np.int16) if fmt == 1 else np.ascontiguousarray(ey_store)).to(dev)
+    cs = None if curl_store is None else torch.from_numpy(np.ascontiguousarray(curl_store).view(np.int16) if fmt == 1 else np.ascontiguousarray(curl_store)).to(dev)
+    stream = _native.stream_handle(dev)
+    P = _native.ptr
+    if reference_abi:
+        gca = torch.zeros(gshape, device=dev, dtype=dtype)
+        gcb = torch.zeros(gshape, device=dev, dtype=dtype)
+        fn = _native.get_function("maxwell_tm", "backward", 2, dtype)
+        st = fn(P(ca), P(cb), P(cq), P(gr), *[P(x) for x in lam], None,
+                P(es), None, None, P(cs), None, None, P(gf), P(gca), P(gcb), None, None,
+                *[P(p) for p in prof], P(src), P(rec), case.rdy, case.rdx, case.dt,
+                nt, N, ny, nx, n_src, n_rec, interval, STORAGE_DEVICE, fmt, shot_bytes,
+                es is not None, cs is not None, batched, batched, batched, nt, *case.pml,
+                0, dev.index or 0, stream, stream)
+    else:
+        gca = torch.zeros(gshape, device=dev, dtype=torch.float64)
+        gcb = torch.zeros(gshape, device=dev, dtype=torch.float64)
+        fn = getattr(_native.load(), f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
+        st = fn(P(ca), P(cb), P(cq), P(gr), *[P(x) for x in lam], P(es), P(cs), P(gf),
+                P(gca) if es is not None else None, P(gcb) if cs is not None else None,
+                *[P(p) for p in prof], P(src), P(rec), case.rdy, case.rdx, nt, N, ny, nx,
+                n_src, n_rec, interval, fmt, shot_bytes, batched, batched, batched, nt,
+                *case.pml, dev.index or 0, stream)
+    _native.check(st)
+    torch.cuda.synchronize(dev)
+    return gca.cpu().numpy(), gcb.cpu().numpy(), gf.cpu().numpy(), [x.cpu().numpy() for x in lam]

@@ -1,0 +1,68 @@
+"""Shared test helpers: golden fixtures, example problems, norms."""
+
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import torch
+
+import tide
+from oracle import oracle as O
+from tide.maxwell.tm2d import prepare_tm2d
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+PROFILE_NAMES = ("ay", "by", "ayh", "byh", "ax", "bx", "axh", "bxh", "ky", "kyh", "kx", "kxh")
+FIELD_NAMES = ("Ey", "Hx", "Hz", "m_Ey_x", "m_Ey_z", "m_Hx_z", "m_Hz_x")
+
+
+def load_golden(name: str) -> tuple[O.TMCase, dict]:
+    z = np.load(GOLDEN / f"{name}.npz")
+    ca = z["ca"]
+    n_shots = z["src"].shape[0] if z["src"].size else z["rec"].shape[0]
+    ny, nx = ca.shape[-2:]
+    nt, *pml = (int(v) for v in z["ints"])
+    rdy, rdx, dt = (float(v) for v in z["scalars"])
+    case = O.TMCase(
+        ca=ca, cb=z["cb"], cq=z["cq"], f=z["f"],
+        profiles=[z[f"prof_{n}"] for n in PROFILE_NAMES],
+        src=z["src"], rec=z["rec"], rdy=rdy, rdx=rdx, dt=dt, nt=nt, pml=tuple(pml),
+        fields=[np.zeros((n_shots, ny, nx), ca.dtype) for _ in range(7)],
+    )
+    expected = {"r": z["r"], **{k: z["out_" + k] for k in FIELD_NAMES}}
+    return case, expected
+
+
+def relative_l2(actual, reference) -> float:
+    a = np.asarray(actual, dtype=np.float64)
+    b = np.asarray(reference, dtype=np.float64)
+    den = np.linalg.norm(b)
+    assert den > 0
+    return float(np.linalg.norm(a - b) / den)
+
+
+def config1_setup(dtype=torch.float32, device="cpu"):
+    eps = torch.full((96, 96), 4.0, dtype=dtype, device=device)
+    amp = tide.ricker(8e8, 300, 4e-11, dtype=dtype, device=device).reshape(1, 1, -1)
+    return prepare_tm2d(eps, torch.zeros_like(eps), torch.ones_like(eps), 0.02, 4e-11, amp,
+                        torch.tensor([[[20, 48]]]), torch.tensor([[[20, 60]]]), pml_width=10)
+
+
+def random_model(ny, nx, *, seed, dtype=torch.float32, device="cpu", eps_range=(1.0, 9.0), sigma_max=0.01):
+    g = torch.Generator().manual_seed(seed)
+    eps = eps_range[0] + (eps_range[1] - eps_range[0]) * torch.rand(ny, nx, generator=g, dtype=torch.float64)
+    sig = sigma_max * torch.rand(ny, nx, generator=g, dtype=torch.float64)
+    return eps.to(dtype=dtype, device=device), sig.to(dtype=dtype, device=device)
+
+
+def hetero_problem(*, ny=40, nx=52, nt=160, shots=3, seed=1234, dtype=torch.float32,
+                   device="cpu", pml=(5, 6, 4, 7), dx=0.01):
+    eps, sig = random_model(ny, nx, seed=seed, dtype=dtype, device=device)
+    dt = 0.9 * dx / (2**0.5 * tide.C0)
+    amp = tide.ricker(6e8, nt, dt, dtype=dtype, device=device).reshape(1, 1, -1).repeat(shots, 2, 1)
+    amp[:, 1] *= 0.5
+    src = torch.stack([torch.tensor([[3, 5 + 7 * s], [ny - 4, 9 + 5 * s]]) for s in range(shots)])
+    rec = torch.stack([torch.tensor([[2, x] for x in range(1, nx - 1, 5)]) for _ in range(shots)])
+    return dict(epsilon=eps, sigma=sig, mu=torch.ones_like(eps), grid_spacing=dx, dt=dt,
+                source_amplitude=amp, source_location=src, receiver_location=rec,
+                pml_width=list(pml))

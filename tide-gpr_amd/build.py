@@ -16,6 +16,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
+INCLUDE = HERE.parent / "include"
 OUT = HERE / "tide" / "_lib" / "libtide_hip.so"
 SOURCES = ["tide_common.hip", "tm2d.hip"]
 OPTIONAL_SOURCES = ["em3d.hip", "tm2d_batch.hip"]
@@ -38,7 +39,7 @@ def build(verbose: bool = False) -> Path:
         hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
         "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-I", str(CSRC), "-o", str(tmp), *srcs,
+        "-I", str(CSRC), "-I", str(INCLUDE), "-o", str(tmp), *srcs,
     ]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -23,6 +23,7 @@
 // Stencil order 2 (the reference default) is the one built here.
 
 #include "tide_common.h"
+#include "tide_hip.h"
 
 #include <algorithm>
 #include <cstring>
