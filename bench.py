@@ -1,0 +1,262 @@
+"""Throughput of the 2-D TM forward + adjoint gradient on one MI355X.
+
+Workload (one step = one full gradient): 1024x1024 model (1065x1065 padded,
+CPML 20, stencil 2), dx 0.01 m, dt at CFL 0.9, nt 4000, 64 shots along the
+surface row at 16-cell spacing (200 MHz Ricker), 256 receivers per shot,
+seeded layered eps_r 1-25 with 12 Gaussian anomalies, sigma U[0, 0.01].
+Each step computes the L2 misfit (FP64) against data from a perturbed model
+(one extra anomaly, computed before timing) and its eps/sigma gradient
+through the public ``tide.MaxwellTM`` operator, shot chunk by shot chunk so
+that fp32 snapshots fit in HBM.  Useful work counted: 2 sweeps x 1065^2 x 64
+x 4000 = 5.807e11 cell-updates per step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--nt NT]
+                    [--compression bf16] [--no-cpu-baseline]
+
+Prints one JSON line (rank 0).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT / "tide-gpr_amd", ROOT):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    """Progress on stderr; stdout carries only the JSON line."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+METRIC = "fwd+adj Gcell-updates/s & %HBM roofline (2D TM); p99 served-solve latency"
+
+
+def parse() -> argparse.Namespace:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--compression", default="none", choices=["none", "bf16"])
+    ap.add_argument("--chunk", type=int, default=0, help="shots per chunk (0: fit HBM)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-nt", type=int, default=3000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 1 GiB device copy (profiler byte-counter calibration)")
+    return ap.parse_args()
+
+
+def bytes_per_shot(ny: int, nx: int, nt: int, snap_bytes: int) -> int:
+    numel = ny * nx
+    snapshots = 2 * nt * numel * snap_bytes
+    fields = (7 + 5 + 7 + 7 + 4) * numel * 4  # forward state + scratch, adjoint state + scratch
+    return snapshots + fields
+
+
+def cpu_baseline(args, setup_kwargs) -> dict:
+    """The reference's own CPU forward_with_storage + backward (compiled from
+    its sources by oracle/Makefile) on a bounded sample, all host cores."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as O
+    from tide.maxwell.tm2d import prepare_tm2d
+
+    cores = len(os.sched_getaffinity(0))
+    n_shots, nt = 2, args.cpu_sample_nt
+    kw = dict(setup_kwargs)
+    kw["source_amplitude"] = kw["source_amplitude"][:n_shots, :, :nt].cpu()
+    kw["source_location"] = kw["source_location"][:n_shots].cpu()
+    kw["receiver_location"] = kw["receiver_location"][:n_shots].cpu()
+    for k in ("epsilon", "sigma", "mu"):
+        kw[k] = kw[k].detach().cpu()
+    st = prepare_tm2d(**kw, nt=nt)
+    case = O.case_from_setup(st)
+    grad_r = np.zeros((nt, n_shots, case.rec.shape[1]), np.float32)
+    grad_r[nt // 2 :] = 1.0
+    if O.ref_available():
+        kind = "reference"
+        t0 = time.perf_counter()
+        O.ref_tm2d_forward_with_storage_and_backward(case, grad_r, n_threads=cores)
+        secs = time.perf_counter() - t0
+        what = "reference CPU forward_with_storage + backward (OpenMP, -O2)"
+    else:
+        kind = "port"
+        t0 = time.perf_counter()
+        _, _, es, cs = O.tm2d_forward(case, store="full")
+        O.tm2d_backward(case, grad_r, es, cs)
+        secs = time.perf_counter() - t0
+        what = "CPU oracle restatement, forward with storage + exact adjoint"
+    updates = 2.0 * n_shots * case.ny * case.nx * nt
+    return {
+        "value": round(updates / secs / 1e9, 4),
+        "unit": "Gcell-updates/s",
+        "cores": cores,
+        "kind": kind,
+        "sample": f"{what}: {n_shots} shots x {case.ny}x{case.nx} x nt {nt} "
+                  f"({updates:.3e} cell-updates, {secs:.1f} s)",
+    }
+
+
+def main() -> None:
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
+              "(shots shard across GPUs with one FP64 gradient all-reduce)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import tide
+    from tide.maxwell.tm2d import KernelTimer, set_kernel_timer
+    from tide.synthetic import add_anomaly, cfl_dt, layered_model, surface_acquisition
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n = args.size
+    pml, dx = 20, 0.01
+    dt = cfl_dt(dx, 2)
+    nt, n_shots = args.nt, args.shots
+    eps, sig, mu = layered_model(n, n, seed=1234, device=dev)
+    # the perturbation sits where its reflection reaches the surface within nt
+    eps_obs = add_anomaly(eps, n * 0.12, n * 0.45, 30.0, 2.0)
+    src, rec = surface_acquisition(n_shots, 256, device=dev)
+    amp = tide.ricker(2e8, nt, dt, dtype=torch.float32, device=dev).reshape(1, 1, -1).repeat(n_shots, 1, 1)
+    ny_p, nx_p = n + 2 * pml + 1, n + 2 * pml + 1
+
+    snap_bytes = 2 if args.compression == "bf16" else 4
+    per_shot = bytes_per_shot(ny_p, nx_p, nt, snap_bytes)
+    free, total_mem = torch.cuda.mem_get_info(dev)
+    cap = args.chunk or max(1, int(0.90 * free // per_shot))
+    n_chunks = math.ceil(n_shots / cap)
+    base, extra = divmod(n_shots, n_chunks)
+    sizes = [base + (1 if i < extra else 0) for i in range(n_chunks)]
+    bounds = [sum(sizes[:i]) for i in range(n_chunks + 1)]
+
+    disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
+    storage = tide.StorageOptions(compression=args.compression == "bf16")
+
+    def operator(lo, hi):
+        exp = tide.Experiment(tide.Acquisition(src[lo:hi], rec[lo:hi]), amp[lo:hi])
+        return tide.MaxwellTM(disc, exp, storage=storage)
+
+    ops = [operator(bounds[i], bounds[i + 1]) for i in range(n_chunks)]
+    with torch.no_grad():
+        observed = [op(tide.EMModel(eps_obs, sig, mu)).receiver_data for op in ops]
+    torch.cuda.synchronize()
+    log(f"observed data ready ({n_chunks} shot chunks of {sizes})")
+
+    e = eps.clone().requires_grad_(True)
+    s = sig.clone().requires_grad_(True)
+
+    def step():
+        e.grad = None
+        s.grad = None
+        misfit = torch.zeros((), device=dev, dtype=torch.float64)
+        for op, d_obs in zip(ops, observed):
+            r = op(tide.EMModel(e, s, mu)).receiver_data
+            loss = 0.5 * ((r.double() - d_obs.double()) ** 2).sum()
+            loss.backward()
+            misfit = misfit + loss.detach()
+        return misfit
+
+    for k in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"warmup step {k + 1}/{args.warmup} done")
+    timer = KernelTimer()
+    set_kernel_timer(timer)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        misfit = step()
+        log(f"timed step {k + 1}/{args.steps} enqueued")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    log(f"timed region {elapsed:.2f} s")
+    set_kernel_timer(None)
+
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, device=dev, dtype=torch.float32)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+
+    updates_per_step = 2.0 * ny_p * nx_p * n_shots * nt
+    gcells = updates_per_step * args.steps / elapsed / 1e9
+    summ = timer.summary()
+    b_snap = 2 * snap_bytes
+    b_alg = {"forward_store": 24 + b_snap, "adjoint": 24 + b_snap, "forward": 24}
+    kernels = {}
+    for name, d in summ.items():
+        per_launch_ms = d["ms"] / d["launches"]
+        achieved = b_alg[name] * d["cell_updates"] / (d["ms"] * 1e-3) / 1e9
+        kernels[name] = {
+            "kernel": "adj_step" if name == "adjoint" else "fwd_step",
+            "avg_launch_us": round(per_launch_ms * 1e3, 2),
+            "bytes_per_cell": b_alg[name],
+            "achieved_GBs": round(achieved, 1),
+            "gcell_per_s": round(d["cell_updates"] / (d["ms"] * 1e-3) / 1e9, 2),
+        }
+    dom = max(summ, key=lambda k: summ[k]["ms"])
+    dom_k = kernels[dom]
+    roofline = {
+        "bound": "hbm",
+        "achieved": dom_k["achieved_GBs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(dom_k["achieved_GBs"] / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": dom_k["kernel"],
+        "kernels": kernels,
+    }
+    setup_kwargs = dict(
+        epsilon=eps, sigma=sig, mu=mu, grid_spacing=dx, dt=dt, source_amplitude=amp,
+        source_location=src, receiver_location=rec, pml_width=pml,
+    )
+    log("timing the CPU baseline" if not args.no_cpu_baseline else "CPU baseline skipped")
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, setup_kwargs)
+    out = {
+        "metric": METRIC,
+        "value": round(gcells, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D TM fwd+adj gradient, {n}x{n} model ({ny_p}x{nx_p} padded, CPML {pml}), "
+                        f"{n_shots} shots x 256 receivers, nt {nt}, stencil 2, "
+                        f"{'bf16' if snap_bytes == 2 else 'fp32'} snapshots",
+            "shot_chunks": sizes,
+            "cell_updates_per_step": updates_per_step,
+            "hbm_frac_whole_job": round(gcells * 1e9 * (24 + b_snap) / 1e9 / HBM_PEAK_GBS, 4),
+            "misfit": float(misfit),
+            "seed": 1234,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -210,6 +210,16 @@ int tide_tm2d_double_backward_f64acc(
     int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
     int64_t device, void *compute_stream);
 
+/*
+ * Launch shape of the TM2D step kernels: rows marched per wavefront
+ * (forward 4/8/16, adjoint 2/4/8; halved for fp64) and shots per workgroup
+ * (1..8), process-wide.  Set by the autotuner.  No reference counterpart
+ * (the reference fixes a 32x8 block at build time,
+ * src/tide/csrc/CMakeLists.txt:24-25).
+ */
+int tide_tm2d_set_tuning(int fwd_rows, int fwd_group, int adj_rows, int adj_group);
+int tide_tm2d_get_tuning(int *out4);
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,14 +176,20 @@ def tm2d_forward(case: TMCase, *, store: str | None = None, interval: int = 1,
 
 def tm2d_backward(case: TMCase, grad_r: np.ndarray, ey_store, curl_store, *,
                   store: str = "full", interval: int = 1, want_ca: bool = True,
-                  want_cb: bool = True):
+                  want_cb: bool = True, lam0=None):
     """Oracle exact adjoint.  Returns ``(grad_ca, grad_cb, grad_f, lam)`` with
-    FP64 coefficient gradients (``[ny, nx]``, or ``[N, ny, nx]`` batched)."""
+    FP64 coefficient gradients (``[ny, nx]``, or ``[N, ny, nx]`` batched).
+    ``lam0`` optionally gives the initial adjoint state (7 arrays, order
+    lambda_ey, lambda_hx, lambda_hz, m_lambda_ey_x, m_lambda_ey_z,
+    m_lambda_hx_z, m_lambda_hz_x); zero otherwise."""
     lib = oracle_lib()
     dt_ = case.dtype
     f64 = dt_ == np.float64
     shape = (case.n_shots, case.ny, case.nx)
-    lam = [np.zeros(shape, dt_) for _ in range(7)]
+    if lam0 is None:
+        lam = [np.zeros(shape, dt_) for _ in range(7)]
+    else:
+        lam = [np.array(x, dtype=dt_, copy=True).reshape(shape) for x in lam0]
     n_src = case.src.shape[1]
     grad_f = np.zeros((case.nt, case.n_shots, n_src), dt_)
     batched = case.ca.shape[0] != 1

@@ -74,7 +74,7 @@ def gpu_forward(case: TMCase, *, store: str | None = None, interval: int = 1,
 
 def gpu_backward(case: TMCase, grad_r: np.ndarray, ey_store: np.ndarray | None,
                  curl_store: np.ndarray | None, *, store: str = "full", interval: int = 1,
-                 device="cuda:0", reference_abi: bool = False):
+                 device="cuda:0", reference_abi: bool = False, lam0=None):
     """Run the adjoint.  ``reference_abi`` selects
     ``maxwell_tm_2_<dtype>_backward_cuda`` (fp32/fp64 grad out) instead of
     the FP64-accumulator entry.  Returns ``(grad_ca, grad_cb, grad_f, lam)``."""
@@ -84,7 +84,10 @@ def gpu_backward(case: TMCase, grad_r: np.ndarray, ey_store: np.ndarray | None,
     prof = [_dev(p, dev) for p in case.profiles]
     N, ny, nx, nt = case.n_shots, case.ny, case.nx, case.nt
     n_src, n_rec = case.src.shape[1], case.rec.shape[1]
-    lam = [torch.zeros(N, ny, nx, device=dev, dtype=dtype) for _ in range(7)]
+    if lam0 is None:
+        lam = [torch.zeros(N, ny, nx, device=dev, dtype=dtype) for _ in range(7)]
+    else:
+        lam = [_dev(np.ascontiguousarray(x).reshape(N, ny, nx), dev, dtype) for x in lam0]
     gr = _dev(grad_r, dev, dtype)
     gf = torch.zeros(nt, N, n_src, device=dev, dtype=dtype)
     src = torch.from_numpy(np.ascontiguousarray(case.src, np.int64))

@@ -1,0 +1,216 @@
+"""CPU tests of the host-side numerics that feed the HIP kernels.
+
+Coverage mirrors the reference's unit tests for the same helpers
+(reference ``tests/unit/test_grid.py:101-285``): CPML profile shapes and
+ranges, the CFL sub-stepping rule, padding, wavelets.  On top of that the
+closed forms the kernels rely on are pinned: CPML coefficients at known
+depths, the (ca, cb) partials against autograd, the padded geometry and flat
+point indices, the source scaling and the snapshot-storage policy.  None of
+these calls touch a GPU.
+"""
+
+from __future__ import annotations
+
+import math
+
+import pytest
+import torch
+
+from helpers import config1_setup, hetero_problem, random_model
+from tide import cfl_condition, ricker, gaussian, gaussian_derivative
+from tide import staggered, storage, utils
+from tide.maxwell.tm2d import maxwelltm, prepare_tm2d
+from tide.padding import create_or_pad, reverse_pad, zero_interior
+
+
+def _profiles(widths, size=32, dtype=torch.float64):
+    return staggered.set_pml_profiles(
+        pml_width=widths, accuracy=2, fd_pad=[1, 0, 1, 0], dt=1e-11,
+        grid_spacing=[0.02, 0.025], max_vel=3e8, dtype=dtype,
+        device=torch.device("cpu"), pml_freq=5e10, ny=size, nx=size,
+    )
+
+
+def test_pml_profile_shapes():
+    prof = _profiles([4] * 4)
+    assert len(prof) == 12
+    ay, ayh, ax, axh = prof[:4]
+    assert ay.shape == (1, 32, 1) and ayh.shape == (1, 32, 1)
+    assert ax.shape == (1, 1, 32) and axh.shape == (1, 1, 32)
+
+
+def test_pml_profile_zero_width_is_identity():
+    prof = _profiles([0] * 4, size=16)
+    assert all(torch.count_nonzero(p) == 0 for p in prof[:8])
+    assert all(torch.all(p == 1.0) for p in prof[8:])
+
+
+def test_pml_profile_ranges_and_interior():
+    widths = [8, 6, 5, 7]
+    prof = _profiles(widths)
+    ay, ayh, ax, axh, by, byh, bx, bxh, ky, kyh, kx, kxh = [p.reshape(-1) for p in prof]
+    for b in (by, byh, bx, bxh):
+        assert torch.all((b >= 0) & (b <= 1))
+    for k in (ky, kyh, kx, kxh):
+        assert torch.all(k >= 1) and k.max() > 1
+    for a in (ay, ayh, ax, axh):
+        assert torch.all(a <= 0)
+    # Strictly inside the model (between the strips) the update is plain.
+    y0, y1 = 1 + widths[0], 32 - widths[1]
+    assert torch.all(by[y0 + 1 : y1 - 1] == 0) and torch.all(ky[y0 + 1 : y1 - 1] == 1)
+
+
+def test_pml_profile_closed_form_at_outer_edge():
+    """At the outermost cell (depth = width) the grading is 1: sigma =
+    sigma0 = 5/(150 pi h), kappa = 5, alpha = 0.1 alpha_max,
+    b = exp(-(sigma/kappa + alpha) dt/eps0), a = sigma (b-1)/(kappa (sigma + kappa alpha))."""
+    dt, h, w = 1e-11, 0.02, 8
+    a, b, k = utils.setup_pml([w, w], [float(w), 40.0 - w], float(w) * h, dt, 41, 3e8,
+                              torch.float64, torch.device("cpu"), 5e10, grid_spacing=h)
+    sigma0 = 5.0 / (150.0 * math.pi * h)
+    alpha = 0.1 * 0.008
+    bb = math.exp(-(sigma0 / 5.0 + alpha) * dt / utils.EP0)
+    aa = sigma0 * (bb - 1.0) / (5.0 * (sigma0 + 5.0 * alpha) + 1e-9)
+    assert k[0].item() == pytest.approx(5.0, rel=1e-15)
+    assert b[0].item() == pytest.approx(bb, rel=1e-13)
+    assert a[0].item() == pytest.approx(aa, rel=1e-13)
+    # Symmetric strips give mirror-image profiles on the integer grid.
+    torch.testing.assert_close(b, b.flip(0), rtol=0, atol=0)
+    torch.testing.assert_close(a, a.flip(0), rtol=0, atol=0)
+
+
+def test_cfl_condition_substeps():
+    with pytest.warns(UserWarning):
+        inner, ratio = cfl_condition([0.1, 0.1], dt=0.1, max_vel=1.0)
+    assert ratio >= 2 and math.isclose(inner * ratio, 0.1)
+    # The largest stable step of the 2-D Yee scheme: 1/(v sqrt(1/dy^2 + 1/dx^2)).
+    inner, ratio = cfl_condition([0.02, 0.02], dt=1e-12, max_vel=utils.C0)
+    assert ratio == 1 and inner == 1e-12
+    dt_max = 1.0 / (utils.C0 * math.sqrt(2.0) / 0.02)
+    _, ratio = cfl_condition([0.02, 0.02], dt=dt_max * 2.5, max_vel=utils.C0)
+    assert ratio == 3
+
+
+def test_padding_helpers():
+    assert reverse_pad([1, 2, 3, 4]) == [3, 4, 1, 2]
+    z = create_or_pad(torch.empty(0), 1, torch.device("cpu"), torch.float32, (2, 5, 6))
+    assert z.shape == (2, 5, 6) and torch.count_nonzero(z) == 0
+    base = torch.arange(4.0).reshape(2, 2)
+    p = create_or_pad(base, [1, 1, 1, 1], torch.device("cpu"), torch.float32, (4, 4))
+    assert p.shape == (4, 4) and torch.equal(p[1:3, 1:3], base) and p[0, 0] == 0
+    r = create_or_pad(base, [1, 0, 2, 1], torch.device("cpu"), torch.float32, (3, 5), mode="replicate")
+    expected = torch.tensor([[0, 0, 0, 1, 1], [0, 0, 0, 1, 1], [2, 2, 2, 3, 3]], dtype=torch.float32)
+    assert torch.equal(r, expected)
+    t = torch.ones(1, 6, 6)
+    zero_interior(t, [1, 1, 1, 1], [1, 1, 1, 1], 0)
+    assert torch.count_nonzero(t[:, 2:4, :]) == 0 and torch.all(t[:, :2] == 1) and torch.all(t[:, 4:] == 1)
+
+
+def test_replicate_pad_gradient_folds_onto_edges():
+    """The gradient of a padded cell lands on its nearest model cell."""
+    m = torch.rand(3, 4, dtype=torch.float64, requires_grad=True)
+    p = create_or_pad(m, [2, 1, 1, 3], torch.device("cpu"), torch.float64, (6, 8), mode="replicate")
+    g = torch.rand_like(p)
+    (p * g).sum().backward()
+    expected = torch.zeros(3, 4, dtype=torch.float64)
+    for i in range(6):
+        for j in range(8):
+            expected[min(max(i - 2, 0), 2), min(max(j - 1, 0), 3)] += g[i, j]
+    torch.testing.assert_close(m.grad, expected, rtol=1e-14, atol=1e-14)
+
+
+def test_wavelets():
+    w = ricker(2.0, 50, 0.1, dtype=torch.float32)
+    assert w.shape == (50,) and w.dtype == torch.float32
+    assert abs(int(w.abs().argmax()) - 5) <= 1
+    with pytest.raises(ValueError):
+        ricker(0.0, 50, 0.1)
+    with pytest.raises(ValueError):
+        ricker(2.0, 50, 0.0)
+    for fn in (ricker, gaussian):
+        c = fn(2.0, 21, 0.05, peak_time=0.5, dtype=torch.float64)
+        assert c[10].item() == pytest.approx(1.0)
+    gd = gaussian_derivative(2.0, 2001, 0.0005, peak_time=0.5)
+    torch.testing.assert_close(gd, -gd.flip(0), atol=2e-5, rtol=0.0)
+    assert gd.abs().max().item() == pytest.approx(1.0, rel=1e-5)
+
+
+def test_coefficient_partials_match_autograd():
+    dt = 4e-12
+    eps = (torch.rand(5, 6, dtype=torch.float64) * 8 + 1).requires_grad_()
+    sig = (torch.rand(5, 6, dtype=torch.float64) * 0.05).requires_grad_()
+    mu = torch.ones(5, 6, dtype=torch.float64)
+    ca, cb, _ = utils.prepare_parameters(eps, sig, mu, dt)
+    dca_de, dcb_de, dca_ds, dcb_ds = utils.coefficient_partials(ca.detach(), cb.detach(), dt)
+    for out, de, ds in ((ca, dca_de, dca_ds), (cb, dcb_de, dcb_ds)):
+        ge, gs = torch.autograd.grad(out.sum(), (eps, sig), retain_graph=True)
+        torch.testing.assert_close(ge, de, rtol=1e-10, atol=0)
+        torch.testing.assert_close(gs, ds, rtol=1e-10, atol=0)
+
+
+def test_material_coefficients_lossless_limit():
+    """sigma = 0 gives ca = 1, cb = dt/(eps_r eps0), cq = dt/(mu_r mu0)."""
+    dt = 1e-11
+    eps = torch.tensor([[4.0, 9.0]], dtype=torch.float64)
+    ca, cb, cq = utils.prepare_parameters(eps, torch.zeros_like(eps), torch.ones_like(eps), dt)
+    assert torch.all(ca == 1)
+    torch.testing.assert_close(cb, dt / (eps * utils.EP0), rtol=1e-15, atol=0)
+    torch.testing.assert_close(cq, torch.full_like(eps, dt / utils.MU0), rtol=1e-15, atol=0)
+    with pytest.raises(ValueError):
+        utils.compile_material_coefficients(-eps, eps, eps, dt)
+
+
+def test_prepare_tm2d_geometry_and_points():
+    ny, nx, pml = 30, 26, [3, 4, 5, 2]
+    eps, sig = random_model(ny, nx, seed=3, dtype=torch.float64)
+    mu = torch.ones_like(eps)
+    src_loc = torch.tensor([[[2, 3]], [[10, 20]]])
+    rec_loc = torch.tensor([[[0, 0], [29, 25]], [[5, 7], [6, 8]]])
+    amp = torch.randn(2, 1, 17, dtype=torch.float64)
+    dy, dx, dt = 0.02, 0.025, 2e-11
+    st = prepare_tm2d(eps, sig, mu, [dy, dx], dt, amp, src_loc, rec_loc, pml_width=pml)
+    assert (st.ny, st.nx) == (ny + 1 + 3 + 4, nx + 1 + 5 + 2)
+    assert st.pml == (1 + 3, 1 + 5, st.ny - 4, st.nx - 2)
+    assert st.nt == 17 and st.n_shots == 2 and st.n_src == 1 and st.n_rec == 2
+    oy, ox = 1 + pml[0], 1 + pml[2]
+    assert st.receivers_i[0, 1].item() == (29 + oy) * st.nx + (25 + ox)
+    assert st.sources_i[1, 0].item() == (10 + oy) * st.nx + (20 + ox)
+    # Replicate padding: the padded corner carries the model's corner.
+    _, cb0, _ = utils.prepare_parameters(eps, sig, mu, dt)
+    assert st.cb[0, 0, 0].item() == cb0[0, 0].item()
+    assert st.cb[0, -1, -1].item() == cb0[-1, -1].item()
+    # Source term: f = amplitude * cb(source) * (-1/(dx dy)).
+    i = st.sources_i[1, 0].item()
+    expected = amp[1, 0] * st.cb.reshape(-1)[i] * (-1.0 / (dx * dy))
+    torch.testing.assert_close(st.f[:, 1, 0], expected, rtol=1e-15, atol=0)
+    assert all(tuple(f.shape) == (2, st.ny, st.nx) for f in st.fields)
+
+
+def test_prepare_tm2d_config1_sizes():
+    st = config1_setup(dtype=torch.float32)
+    assert (st.ny, st.nx) == (117, 117) and st.n_shots == 1
+
+
+def test_storage_policy():
+    kw = dict(needs_storage=True, total_bytes=100, bytes_limit_device=50, bytes_limit_host=200)
+    assert storage.resolve_storage_mode("auto", **kw) == "cpu"
+    assert storage.resolve_storage_mode("auto", **{**kw, "bytes_limit_device": 100}) == "device"
+    assert storage.resolve_storage_mode("auto", **{**kw, "bytes_limit_host": 10}) == "disk"
+    assert storage.resolve_storage_mode("auto", **{**kw, "needs_storage": False}) == "none"
+    with pytest.raises(ValueError):
+        storage.resolve_storage_mode("none", **kw)
+    with pytest.raises(ValueError):
+        storage.resolve_storage_mode("tape", **kw)
+    kind, dt, nbytes, fmt = storage._resolve_storage_compression(
+        "bf16", torch.float32, torch.device("cpu"), context="x")
+    assert (kind, dt, nbytes, fmt) == ("bf16", torch.bfloat16, 2, storage.STORAGE_FORMAT_BF16)
+    with pytest.raises(NotImplementedError):
+        storage._resolve_storage_compression(True, torch.float64, torch.device("cpu"), context="x")
+    with pytest.raises(ValueError):
+        storage._resolve_storage_compression("zstd", torch.float32, torch.device("cpu"), context="x")
+
+
+def test_cpu_tensors_are_rejected_not_emulated():
+    """The operator has one compute path (the HIP library); CPU tensors raise."""
+    with pytest.raises(NotImplementedError, match="HIP"):
+        maxwelltm(**hetero_problem(nt=4, shots=1))

@@ -47,12 +47,14 @@ __host__ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
 // -------------------------------------------------------------- point table
 // Sources and receivers are applied inside the step kernels by the thread
 // that owns the cell.  The table groups the (shot, point) entries by padded
-// row, shot and 64-column tile so that a wavefront walking row y reads only
-// the handful of entries that fall in its columns:
+// row, shot and column tile (tile_w columns, tile xt starting at column
+// xt*tile_w) so that a wavefront walking row y reads only the handful of
+// entries that fall in its columns:
 //   row_slot[y]                      -1 or the index of row y among point rows
 //   seg[(slot*n_shots+shot)*n_xt+xt] first entry of that (row, shot, tile);
 //                                     seg[last+1] is the end
-//   ent[e] = {xloc | kind<<8, q}      kind 0 = source (q indexes f[t]),
+//   ent[e] = {lane | kind<<8, q}      lane = column - xt*tile_w + lane_offset;
+//                                     kind 0 = source (q indexes f[t]),
 //                                     kind 1 = receiver (q indexes r[t]);
 //                                     sources precede receivers in a segment,
 //                                     each kind in ascending point order.
@@ -69,7 +71,7 @@ struct PointTable {
 // skipped).  Returns false (with set_error) on an index outside the plane.
 bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
                        int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
-                       int tile_w, PointTable &out);
+                       int tile_w, int lane_offset, PointTable &out);
 
 // Device copy of a PointTable; owns three device allocations.
 struct DevicePointTable {

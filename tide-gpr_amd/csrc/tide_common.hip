@@ -16,7 +16,7 @@ const char *last_error() { return g_last_error.c_str(); }
 
 bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
                        int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
-                       int tile_w, PointTable &out) {
+                       int tile_w, int lane_offset, PointTable &out) {
   const int64_t numel = ny * nx;
   // (y, shot, xt, kind, k, xloc, q)
   struct Entry {
@@ -41,7 +41,7 @@ bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
           return false;
         }
         const int64_t y = v / nx, x = v % nx;
-        entries.push_back({y, s, x / tile_w, kind, k, (int32_t)(x % tile_w),
+        entries.push_back({y, s, x / tile_w, kind, k, (int32_t)(x % tile_w + lane_offset),
                            (int32_t)(s * n + k)});
       }
     }

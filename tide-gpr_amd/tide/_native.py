@@ -174,6 +174,10 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, f"tide_tm2d_{name}_backward_f64acc")
         fn.restype = ctypes.c_int
         fn.argtypes = _tm_backward_f64acc_types(ctype)
+    lib.tide_tm2d_set_tuning.restype = ctypes.c_int
+    lib.tide_tm2d_set_tuning.argtypes = [ctypes.c_int] * 4
+    lib.tide_tm2d_get_tuning.restype = ctypes.c_int
+    lib.tide_tm2d_get_tuning.argtypes = [ctypes.POINTER(ctypes.c_int)]
     _lib = lib
     return lib
 
@@ -213,6 +217,17 @@ def ptr(t: torch.Tensor | None) -> int | None:
     return t.data_ptr()
 
 
+def set_tm2d_tuning(fwd_rows: int, fwd_group: int, adj_rows: int, adj_group: int) -> None:
+    """Launch shape of the TM2D step kernels (see include/tide_hip.h)."""
+    check(load().tide_tm2d_set_tuning(fwd_rows, fwd_group, adj_rows, adj_group))
+
+
+def get_tm2d_tuning() -> tuple[int, int, int, int]:
+    out = (ctypes.c_int * 4)()
+    check(load().tide_tm2d_get_tuning(out))
+    return tuple(out)  # type: ignore[return-value]
+
+
 def stream_handle(device: torch.device) -> int | None:
     """The HIP stream of torch's current stream on ``device``."""
     return torch.cuda.current_stream(device).cuda_stream
@@ -222,6 +237,8 @@ __all__ = [
     "TideNativeError",
     "check",
     "dtype_name",
+    "get_tm2d_tuning",
+    "set_tm2d_tuning",
     "get_function",
     "library_path",
     "load",

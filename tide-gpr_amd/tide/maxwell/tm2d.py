@@ -46,6 +46,47 @@ from ..utils import C0, compile_material_coefficients
 _FIELD_NAMES = ("Ey", "Hx", "Hz", "m_Ey_x", "m_Ey_z", "m_Hx_z", "m_Hz_x")
 
 
+class KernelTimer:
+    """Optional HIP-event timing of the native calls (one sweep each).
+
+    When installed with :func:`set_kernel_timer`, every forward and adjoint
+    sweep is bracketed by events on the stream the kernels run on, and the
+    record ``(pass, n_steps, cells_per_step, start, end)`` is kept so a
+    caller can derive per-launch kernel durations without a profiler.
+    """
+
+    def __init__(self) -> None:
+        self.records: list[tuple[str, int, int, torch.cuda.Event, torch.cuda.Event]] = []
+
+    def start(self) -> torch.cuda.Event:
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return ev
+
+    def stop(self, name: str, steps: int, cells: int, start: torch.cuda.Event) -> None:
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self.records.append((name, steps, cells, start, ev))
+
+    def summary(self) -> dict[str, dict[str, float]]:
+        """Per pass: total ms, launches, and cell-updates (after a sync)."""
+        out: dict[str, dict[str, float]] = {}
+        for name, steps, cells, a, b in self.records:
+            d = out.setdefault(name, {"ms": 0.0, "launches": 0, "cell_updates": 0.0})
+            d["ms"] += a.elapsed_time(b)
+            d["launches"] += steps
+            d["cell_updates"] += float(steps) * cells
+        return out
+
+
+_timer: KernelTimer | None = None
+
+
+def set_kernel_timer(timer: KernelTimer | None) -> None:
+    global _timer
+    _timer = timer
+
+
 @dataclass
 class TM2DSetup:
     """Everything the kernels read for one propagation, in padded layout."""
@@ -235,6 +276,7 @@ class TM2DPropagate(torch.autograd.Function):
             setup.nt, setup.n_shots, setup.ny, setup.nx, setup.n_src, setup.n_rec,
         )
         ey_store = curl_store = None
+        t0 = _timer.start() if _timer is not None else None
         if store:
             n_store = (setup.nt + interval - 1) // interval
             sdtype = torch.int16 if storage["format"] == STORAGE_FORMAT_BF16 else dtype
@@ -265,6 +307,9 @@ class TM2DPropagate(torch.autograd.Function):
                 *setup.pml, 0, dev_idx, stream,
             )
         _native.check(status)
+        if t0 is not None:
+            _timer.stop("forward_store" if store else "forward", setup.nt,
+                        setup.n_shots * setup.ny * setup.nx, t0)
         ctx.setup = setup
         ctx.storage = storage
         ctx.stored = store
@@ -302,6 +347,7 @@ class TM2DPropagate(torch.autograd.Function):
         fn = getattr(_native.load(), f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
         stream = _native.stream_handle(dev)
         dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        t0 = _timer.start() if _timer is not None else None
         status = fn(
             _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(grad_r),
             *[_native.ptr(t) for t in lam],
@@ -316,6 +362,8 @@ class TM2DPropagate(torch.autograd.Function):
             dev_idx, stream,
         )
         _native.check(status)
+        if t0 is not None:
+            _timer.stop("adjoint", setup.nt, setup.n_shots * setup.ny * setup.nx, t0)
         out_ca = gca.to(dtype).reshape(ca.shape) if want_ca else None
         out_cb = gcb.to(dtype).reshape(cb.shape) if want_cb else None
         return out_ca, out_cb, None, grad_f, None, None
@@ -494,4 +542,4 @@ def maxwelltm(
     return (*fields, r)
 
 
-__all__ = ["TM2DPropagate", "TM2DSetup", "maxwelltm", "prepare_tm2d"]
+__all__ = ["KernelTimer", "TM2DPropagate", "TM2DSetup", "maxwelltm", "prepare_tm2d", "set_kernel_timer"]
