@@ -220,6 +220,98 @@ int tide_tm2d_double_backward_f64acc(
 int tide_tm2d_set_tuning(int fwd_rows, int fwd_group, int adj_rows, int adj_group);
 int tide_tm2d_get_tuning(int *out4);
 
+/* ------------------------------------------------------------------- 3-D
+ * Fields are [n_shots, nz, ny, nx] (x fastest).  `fields` / `lambda` point
+ * to 18 device arrays in the reference's state order:
+ *   ex, ey, ez, hx, hy, hz,
+ *   m_hz_y, m_hy_z, m_hx_z, m_hz_x, m_hy_x, m_hx_y,   (E-update memories)
+ *   m_ey_z, m_ez_y, m_ez_x, m_ex_z, m_ex_y, m_ey_x    (H-update memories)
+ * (the adjoint uses the same slots for the lambda counterparts).
+ * `prof` points to 18 profile arrays in the reference's order
+ *   az, bz, azh, bzh, ay, by, ayh, byh, ax, bx, axh, bxh,
+ *   kz, kzh, ky, kyh, kx, kxh;
+ * `pml` to int64 {z0, y0, x0, z1, y1, x1}; components are 0 x, 1 y, 2 z.
+ * `stores` points to 6 snapshot buffers (Ex, Ey, Ez, curl x, y, z; NULL
+ * entries are not written); the snapshot of step t (t a multiple of
+ * step_ratio) sits at (t - store_t0) / step_ratio * store_step_bytes, so a
+ * caller can recompute one checkpoint segment into a segment-sized buffer.
+ * storage_format 0 = T payload, 1 = BF16 (fp32 fields only).
+ */
+
+/*
+ * Forward propagation over [start_t, start_t + nt), in place.
+ * Replaces maxwell_3d_2_{dtype}_forward_cuda and
+ * maxwell_3d_2_{dtype}_forward_with_storage_cuda: reference ABI
+ * src/tide/backend_utils.py:487-521 (_3D_FORWARD_SPEC,
+ * _3D_FORWARD_WITH_STORAGE_SPEC), CUDA implementation
+ * src/tide/csrc/em3d/maxwell_3d_cuda_inst.cu:3012-3765; pointer arrays
+ * replace the 18 + 18 individual pointers and store_t0 is added.
+ */
+int tide_em3d_float_forward(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *const *fields, float *r, float const *const *prof,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdz, float rdy,
+    float rdx, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t const *pml, int64_t source_component,
+    int64_t receiver_component, void *const *stores, int64_t storage_format,
+    int64_t store_step_bytes, int64_t store_t0, int64_t device,
+    void *compute_stream);
+
+int tide_em3d_double_forward(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *const *fields, double *r, double const *const *prof,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdz,
+    double rdy, double rdx, int64_t nt, int64_t n_shots, int64_t nz,
+    int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, bool ca_batched,
+    bool cb_batched, bool cq_batched, int64_t start_t, int64_t const *pml,
+    int64_t source_component, int64_t receiver_component,
+    void *const *stores, int64_t storage_format, int64_t store_step_bytes,
+    int64_t store_t0, int64_t device, void *compute_stream);
+
+/*
+ * Exact adjoint over steps start_t - 1 down to start_t - nt (receivers
+ * inject grad_r[t] into lambda of the receiver component, sources read
+ * grad_f[t]), accumulating dJ/dca and dJ/dcb into FP64 arrays (added to;
+ * [nz, ny, nx] summed over shots, or per shot when batched; NULL when not
+ * wanted).  Replaces maxwell_3d_2_{dtype}_backward_cuda: reference ABI
+ * src/tide/backend_utils.py:523-555 (_3D_BACKWARD_SPEC), CUDA adjoint
+ * src/tide/csrc/em3d/maxwell_3d_cuda_inst.cu:1420-2187, 2300-2450.
+ */
+int tide_em3d_float_backward_f64acc(
+    float const *ca, float const *cb, float const *cq, float const *grad_r,
+    float *const *lambda, void const *const *stores, int64_t storage_format,
+    int64_t store_step_bytes, int64_t store_t0, float *grad_f,
+    double *grad_ca, double *grad_cb, float const *const *prof,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdz, float rdy,
+    float rdx, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t const *pml, int64_t source_component,
+    int64_t receiver_component, int64_t device, void *compute_stream);
+
+int tide_em3d_double_backward_f64acc(
+    double const *ca, double const *cb, double const *cq,
+    double const *grad_r, double *const *lambda, void const *const *stores,
+    int64_t storage_format, int64_t store_step_bytes, int64_t store_t0,
+    double *grad_f, double *grad_ca, double *grad_cb,
+    double const *const *prof, int64_t const *sources_i,
+    int64_t const *receivers_i, double rdz, double rdy, double rdx,
+    int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t const *pml, int64_t source_component,
+    int64_t receiver_component, int64_t device, void *compute_stream);
+
+/*
+ * Launch shape of the 3-D step kernels: z planes marched per wavefront
+ * (forward 4/8/16, adjoint 4/8) and shots per workgroup (1..8).  No
+ * reference counterpart.
+ */
+int tide_em3d_set_tuning(int fwd_z, int adj_z, int group);
+
 #ifdef __cplusplus
 }
 #endif

@@ -316,3 +316,197 @@ def ref_tm2d_forward_with_storage_and_backward(case: TMCase, grad_r: np.ndarray,
         nt, N, ny, nx, n_src, n_rec, 1, 0, 0, shot_bytes, True, True, False, False, False, nt,
         *case.pml, n_threads, 0, None, None)
     return r, gca, gcb
+
+
+# ====================================================================== 3-D
+@dataclass
+class EMCase:
+    """One 3-D Maxwell propagation in padded layout (mirrors ``EM3DSetup``)."""
+
+    ca: np.ndarray  # [1|N, nz, ny, nx]
+    cb: np.ndarray
+    cq: np.ndarray
+    f: np.ndarray  # [nt, N, n_src]
+    profiles: list  # az bz azh bzh ay by ayh byh ax bx axh bxh kz kzh ky kyh kx kxh
+    src: np.ndarray  # [N, n_src] int64
+    rec: np.ndarray  # [N, n_rec] int64
+    rdz: float
+    rdy: float
+    rdx: float
+    dt: float
+    nt: int
+    pml: tuple  # z0 y0 x0 z1 y1 x1
+    src_comp: int = 1
+    rec_comp: int = 1
+    fields: list = field(default_factory=list)  # 18 initial state arrays
+
+    @property
+    def n_shots(self) -> int:
+        return int(self.fields[0].shape[0])
+
+    @property
+    def shape(self) -> tuple:
+        return tuple(int(v) for v in self.fields[0].shape[1:])
+
+    @property
+    def dtype(self):
+        return self.ca.dtype
+
+    def astype(self, dtype) -> "EMCase":
+        c = lambda a: np.ascontiguousarray(a, dtype=dtype)  # noqa: E731
+        return EMCase(c(self.ca), c(self.cb), c(self.cq), c(self.f),
+                      [c(p) for p in self.profiles], self.src, self.rec, self.rdz,
+                      self.rdy, self.rdx, self.dt, self.nt, self.pml, self.src_comp,
+                      self.rec_comp, [c(x) for x in self.fields])
+
+
+def em_case_from_setup(setup) -> EMCase:
+    """Convert a ``tide.maxwell.maxwell3d.EM3DSetup`` (any device) to numpy."""
+    n = lambda t: np.ascontiguousarray(t.detach().cpu().numpy())  # noqa: E731
+    dt_ = n(setup.ca).dtype
+    f = (n(setup.f).reshape(setup.nt, setup.n_shots, setup.n_src) if setup.n_src
+         else np.zeros((setup.nt, setup.n_shots, 0), dt_))
+    return EMCase(
+        ca=n(setup.ca), cb=n(setup.cb), cq=n(setup.cq), f=f,
+        profiles=[n(p) for p in setup.profiles],
+        src=n(setup.sources_i).reshape(setup.n_shots, setup.n_src).astype(np.int64),
+        rec=n(setup.receivers_i).reshape(setup.n_shots, setup.n_rec).astype(np.int64),
+        rdz=setup.rdz, rdy=setup.rdy, rdx=setup.rdx, dt=setup.dt, nt=setup.nt,
+        pml=tuple(setup.pml), src_comp=setup.source_component,
+        rec_comp=setup.receiver_component, fields=[n(t) for t in setup.fields],
+    )
+
+
+def _em_common(case: EMCase):
+    f64 = case.dtype == np.float64
+    ct = ctypes.c_double if f64 else ctypes.c_float
+    b = case.ca.shape[0] != 1
+    return f64, ct, (ctypes.c_bool * 3)(b, b, b)
+
+
+def em3d_forward(case: EMCase, *, store: str | None = None, interval: int = 1,
+                 start_t: int = 0, nt: int | None = None):
+    """Oracle 3-D forward.  ``store`` None / 'full' / 'bf16'.
+
+    Returns ``(fields, r, stores)`` with ``stores`` = six arrays
+    ``[n_stored, N, nz, ny, nx]`` (Ex, Ey, Ez, curl x, y, z) or None."""
+    lib = oracle_lib()
+    f64, ct, batched = _em_common(case)
+    nt = case.nt if nt is None else nt
+    fields = [np.ascontiguousarray(x.copy()) for x in case.fields]
+    n_rec = case.rec.shape[1]
+    r = np.zeros((start_t + nt, case.n_shots, n_rec), case.dtype)
+    stores, fmt = None, 0
+    if store is not None:
+        n_store = (start_t + nt + interval - 1) // interval
+        fmt = 1 if store == "bf16" else 0
+        sdt = np.uint16 if fmt == 1 else case.dtype
+        stores = [np.zeros((n_store, case.n_shots, *case.shape), sdt) for _ in range(6)]
+    step_bytes = case.n_shots * int(np.prod(case.shape)) * (2 if fmt == 1 else case.dtype.itemsize)
+    f = case.f if case.f.shape[0] >= start_t + nt else np.zeros((start_t + nt, case.n_shots, case.src.shape[1]), case.dtype)
+    fn = lib.oracle_em3d_forward_f64 if f64 else lib.oracle_em3d_forward_f32
+    fn.restype = ctypes.c_int
+    nz, ny, nx = case.shape
+    rc = fn(_p(case.ca), _p(case.cb), _p(case.cq), _p(f), _parr(fields), _p(r),
+            _parr(case.profiles), _p(case.src), _p(case.rec), ct(case.rdz), ct(case.rdy),
+            ct(case.rdx), ctypes.c_int64(nt), ctypes.c_int64(case.n_shots), ctypes.c_int64(nz),
+            ctypes.c_int64(ny), ctypes.c_int64(nx), ctypes.c_int64(case.src.shape[1]),
+            ctypes.c_int64(n_rec), ctypes.c_int64(interval), ctypes.c_int64(start_t),
+            (ctypes.c_int64 * 6)(*case.pml), batched, ctypes.c_int(case.src_comp),
+            ctypes.c_int(case.rec_comp),
+            _parr(stores) if stores is not None else (ctypes.c_void_p * 6)(),
+            ctypes.c_int(fmt), ctypes.c_int64(step_bytes))
+    if rc != 0:
+        raise RuntimeError(f"oracle 3-D forward failed ({rc})")
+    return fields, r, stores
+
+
+def em3d_backward(case: EMCase, grad_r: np.ndarray, stores, *, store: str = "full",
+                  interval: int = 1, want_ca: bool = True, want_cb: bool = True, lam0=None,
+                  start_t: int | None = None, nt: int | None = None):
+    """Oracle exact 3-D adjoint over steps ``start_t-1`` down to
+    ``start_t-nt`` (default: the whole run).  Returns ``(grad_ca, grad_cb,
+    grad_f, lam)`` with FP64 coefficient gradients; ``lam0`` optionally gives
+    the initial 18-array adjoint state (same slots as the forward state)."""
+    start_t = case.nt if start_t is None else start_t
+    nt = start_t if nt is None else nt
+    lib = oracle_lib()
+    f64, ct, batched = _em_common(case)
+    shape = (case.n_shots, *case.shape)
+    lam = ([np.zeros(shape, case.dtype) for _ in range(18)] if lam0 is None
+           else [np.array(x, dtype=case.dtype, copy=True).reshape(shape) for x in lam0])
+    n_src = case.src.shape[1]
+    grad_f = np.zeros((case.nt, case.n_shots, n_src), case.dtype)
+    gshape = shape if case.ca.shape[0] != 1 else case.shape
+    gca = np.zeros(gshape, np.float64) if want_ca else None
+    gcb = np.zeros(gshape, np.float64) if want_cb else None
+    fmt = 1 if store == "bf16" else 0
+    step_bytes = int(np.prod(shape)) * (2 if fmt == 1 else case.dtype.itemsize)
+    fn = lib.oracle_em3d_backward_f64 if f64 else lib.oracle_em3d_backward_f32
+    fn.restype = ctypes.c_int
+    nz, ny, nx = case.shape
+    grad_r = np.ascontiguousarray(grad_r, dtype=case.dtype)
+    rc = fn(_p(case.ca), _p(case.cb), _p(case.cq), _p(grad_r), _parr(lam),
+            _parr(stores) if stores is not None else (ctypes.c_void_p * 6)(),
+            ctypes.c_int(fmt), ctypes.c_int64(step_bytes), _p(grad_f) if n_src else None,
+            _p(gca), _p(gcb), _parr(case.profiles), _p(case.src), _p(case.rec),
+            ct(case.rdz), ct(case.rdy), ct(case.rdx), ctypes.c_int64(nt),
+            ctypes.c_int64(case.n_shots), ctypes.c_int64(nz), ctypes.c_int64(ny),
+            ctypes.c_int64(nx), ctypes.c_int64(n_src), ctypes.c_int64(case.rec.shape[1]),
+            ctypes.c_int64(interval), ctypes.c_int64(start_t), (ctypes.c_int64 * 6)(*case.pml),
+            batched, ctypes.c_int(case.src_comp), ctypes.c_int(case.rec_comp))
+    if rc != 0:
+        raise RuntimeError(f"oracle 3-D backward failed ({rc})")
+    return gca, gcb, grad_f, lam
+
+
+def em3d_tangent(case: EMCase, dca: np.ndarray, dcb: np.ndarray, df: np.ndarray):
+    """FP64 tangent-linear 3-D forward: returns ``(r, dr)``."""
+    lib = oracle_lib()
+    c = case.astype(np.float64)
+    fields = [x.copy() for x in c.fields]
+    dfields = [np.zeros_like(x) for x in c.fields]
+    n_rec = c.rec.shape[1]
+    r = np.zeros((c.nt, c.n_shots, n_rec))
+    dr = np.zeros_like(r)
+    nz, ny, nx = c.shape
+    fn = lib.oracle_em3d_tangent_f64
+    fn.restype = ctypes.c_int
+    D = ctypes.c_double
+    rc = fn(_p(c.ca), _p(c.cb), _p(c.cq), _p(c.f), _p(np.ascontiguousarray(dca, np.float64)),
+            _p(np.ascontiguousarray(dcb, np.float64)), _p(np.ascontiguousarray(df, np.float64)),
+            _parr(fields), _parr(dfields), _p(r), _p(dr), _parr(c.profiles), _p(c.src),
+            _p(c.rec), D(c.rdz), D(c.rdy), D(c.rdx), ctypes.c_int64(c.nt),
+            ctypes.c_int64(c.n_shots), ctypes.c_int64(nz), ctypes.c_int64(ny),
+            ctypes.c_int64(nx), ctypes.c_int64(c.src.shape[1]), ctypes.c_int64(n_rec),
+            (ctypes.c_int64 * 6)(*c.pml), ctypes.c_int(c.src_comp), ctypes.c_int(c.rec_comp))
+    if rc != 0:
+        raise RuntimeError(f"oracle 3-D tangent failed ({rc})")
+    return r, dr
+
+
+def ref_em3d_forward(case: EMCase, *, n_threads: int = 0, nt: int | None = None):
+    """The reference's own CPU 3-D forward (``maxwell_3d_2_<dtype>_forward_cpu``,
+    argument order of the reference ``backend_utils.py:487-503``).
+    Returns ``(fields, r)``."""
+    lib = ref_lib()
+    dt_ = case.dtype
+    name = "double" if dt_ == np.float64 else "float"
+    fn = getattr(lib, f"maxwell_3d_2_{name}_forward_cpu")
+    ct = ctypes.c_double if dt_ == np.float64 else ctypes.c_float
+    I, P, B = ctypes.c_int64, ctypes.c_void_p, ctypes.c_bool
+    fn.restype = None
+    fn.argtypes = ([P] * 4 + [P] * 18 + [P] * 9 + [P] + [I] + [P] * 20 + [ct] * 4 + [I] * 8
+                   + [B] + [B] * 3 + [I] + [I] * 6 + [I] * 2 + [I] * 3 + [P])
+    nt = case.nt if nt is None else nt
+    fields = [np.ascontiguousarray(x.copy()) for x in case.fields]
+    n_rec = case.rec.shape[1]
+    r = np.zeros((nt, case.n_shots, n_rec), dt_)
+    b = case.ca.shape[0] != 1
+    nz, ny, nx = case.shape
+    fn(_p(case.ca), _p(case.cb), _p(case.cq), _p(case.f), *[_p(x) for x in fields],
+       *([None] * 9), _p(r), 0, *[_p(p) for p in case.profiles], _p(case.src), _p(case.rec),
+       case.rdz, case.rdy, case.rdx, case.dt, nt, case.n_shots, nz, ny, nx,
+       case.src.shape[1], n_rec, 1, False, b, b, b, 0, *case.pml, case.src_comp,
+       case.rec_comp, n_threads, 0, 0, None)
+    return fields, r

@@ -1,4 +1,4 @@
-"""Writes the golden TM2D fixtures from the reference's own CPU library.
+"""Writes the golden TM2D and 3-D fixtures from the reference's own CPU library.
 
 Run in a container where the reference sources exist:
 
@@ -8,7 +8,11 @@ Each fixture holds the padded inputs (built by this project's host code,
 ``tide.maxwell.tm2d.prepare_tm2d``) and the outputs of the reference's
 compiled CPU forward ``maxwell_tm_2_<dtype>_forward_cpu`` (reference
 ``src/tide/csrc/tm2d/maxwell_tm_inst.cpp:727-825``): the receiver traces and
-the final 7 padded fields.  The reference ships no golden vectors of its own
+the final 7 padded fields; likewise for the 3-D forward
+``maxwell_3d_2_<dtype>_forward_cpu`` (reference
+``src/tide/csrc/em3d/maxwell_3d_inst.cpp:446-761``, inputs from
+``tide.maxwell.maxwell3d.prepare_em3d``), receivers and the six final E/H
+fields.  The reference ships no golden vectors of its own
 (its tests pin closed forms and cross-backend agreement), so these vectors
 are produced from its compiled CPU forward path.
 """
@@ -27,6 +31,7 @@ sys.path.insert(0, str(ROOT))
 
 import tide  # noqa: E402
 from oracle import oracle as O  # noqa: E402
+from tide.maxwell.maxwell3d import prepare_em3d  # noqa: E402
 from tide.maxwell.tm2d import prepare_tm2d  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
@@ -74,6 +79,43 @@ def save(name, setup):
     print(name, {k: v.shape for k, v in arrays.items() if k in ("r", "ca")})
 
 
+PROFILE_NAMES_3D = ("az", "bz", "azh", "bzh", "ay", "by", "ayh", "byh", "ax", "bx", "axh",
+                    "bxh", "kz", "kzh", "ky", "kyh", "kx", "kxh")
+
+
+def heterogeneous_3d(dtype):
+    """Seeded random eps/sigma on 8x9x10, uneven CPML, 2 shots, Ex source,
+    Ez receivers; exercises every CPML strip of every axis."""
+    g = torch.Generator().manual_seed(4321)
+    nz, ny, nx, nt, shots = 8, 9, 10, 60, 2
+    eps = (1 + 8 * torch.rand(nz, ny, nx, generator=g)).to(dtype)
+    sig = (0.01 * torch.rand(nz, ny, nx, generator=g)).to(dtype)
+    dt = 0.9 * 0.02 / (3**0.5 * tide.C0)
+    amp = tide.ricker(3e8, nt, dt, dtype=dtype).reshape(1, 1, -1).repeat(shots, 1, 1)
+    src = torch.tensor([[[3, 4, 5]], [[5, 2, 6]]])
+    rec = torch.tensor([[[1, 2, 3], [6, 7, 8], [4, 4, 4]], [[2, 2, 2], [4, 6, 7], [7, 8, 9]]])
+    return prepare_em3d(eps, sig, torch.ones_like(eps), 0.02, dt, amp, src, rec,
+                        pml_width=[2, 3, 2, 2, 3, 2], source_component="ex",
+                        receiver_component="ez")
+
+
+def save_3d(name, setup):
+    case = O.em_case_from_setup(setup)
+    fields, r = O.ref_em3d_forward(case)
+    arrays = {
+        "ca": case.ca, "cb": case.cb, "cq": case.cq, "f": case.f,
+        **{f"prof_{n}": v for n, v in zip(PROFILE_NAMES_3D, case.profiles)},
+        "src": case.src, "rec": case.rec,
+        "scalars": np.array([case.rdz, case.rdy, case.rdx, case.dt]),
+        "ints": np.array([case.nt, *case.pml, case.src_comp, case.rec_comp], dtype=np.int64),
+        "r": r,
+    }
+    for k, v in zip(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"), fields[:6]):
+        arrays["out_" + k] = v
+    np.savez_compressed(OUT / f"{name}.npz", **arrays)
+    print(name, {k: v.shape for k, v in arrays.items() if k in ("r", "ca")})
+
+
 def main():
     if not O.ref_available():
         raise SystemExit("oracle/_ref/libtide_ref.so missing: run `make -C oracle ref` first")
@@ -81,6 +123,8 @@ def main():
     save("tm2d_config1_f64", config1(torch.float64))
     save("tm2d_hetero_f32", heterogeneous(torch.float32))
     save("tm2d_hetero_f64", heterogeneous(torch.float64))
+    save_3d("em3d_hetero_f32", heterogeneous_3d(torch.float32))
+    save_3d("em3d_hetero_f64", heterogeneous_3d(torch.float64))
 
 
 if __name__ == "__main__":

@@ -123,3 +123,90 @@ def gpu_backward(case: TMCase, grad_r: np.ndarray, ey_store: np.ndarray | None,
     _native.check(st)
     torch.cuda.synchronize(dev)
     return gca.cpu().numpy(), gcb.cpu().numpy(), gf.cpu().numpy(), [x.cpu().numpy() for x in lam]
+
+
+# ---------------------------------------------------------------------- 3-D
+def gpu_em3d_forward(case, *, store: str | None = None, interval: int = 1, device="cuda:0",
+                     nt: int | None = None, start_t: int = 0, fields=None):
+    """Run ``tide_em3d_<dtype>_forward`` on an ``oracle.EMCase``.
+
+    Returns numpy ``(fields, r, stores)`` (stores: six arrays or None)."""
+    dev = torch.device(device)
+    dtype = torch.float64 if case.dtype == np.float64 else torch.float32
+    nt = case.nt if nt is None else nt
+    ca, cb, cq = (_dev(a, dev) for a in (case.ca, case.cb, case.cq))
+    f = _dev(case.f, dev)
+    prof = [_dev(p, dev) for p in case.profiles]
+    flds = [_dev(x, dev) for x in (fields if fields is not None else case.fields)]
+    N = case.n_shots
+    nz, ny, nx = case.shape
+    n_src, n_rec = case.src.shape[1], case.rec.shape[1]
+    r = torch.zeros(start_t + nt, N, n_rec, device=dev, dtype=dtype)
+    src = torch.from_numpy(np.ascontiguousarray(case.src, np.int64))
+    rec = torch.from_numpy(np.ascontiguousarray(case.rec, np.int64))
+    batched = case.ca.shape[0] != 1
+    stores = None
+    fmt = 1 if store == "bf16" else 0
+    if store is not None:
+        n_store = (start_t + nt + interval - 1) // interval
+        sdt = torch.int16 if fmt == 1 else dtype
+        stores = [torch.zeros(n_store, N, nz, ny, nx, device=dev, dtype=sdt) for _ in range(6)]
+    step_bytes = N * nz * ny * nx * (2 if fmt == 1 else case.dtype.itemsize)
+    fn = getattr(_native.load(), f"tide_em3d_{_native.dtype_name(dtype)}_forward")
+    st = fn(_native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
+            _native.ptr_array(flds), _native.ptr(r), _native.ptr_array(prof),
+            _native.ptr(src), _native.ptr(rec), case.rdz, case.rdy, case.rdx, nt, N, nz, ny, nx,
+            n_src, n_rec, interval, batched, batched, batched, start_t,
+            _native.int64_array(case.pml), case.src_comp, case.rec_comp,
+            _native.ptr_array(stores if stores is not None else [None] * 6), fmt, step_bytes, 0,
+            dev.index or 0, _native.stream_handle(dev))
+    _native.check(st)
+    torch.cuda.synchronize(dev)
+    out_st = None
+    if stores is not None:
+        out_st = [s.cpu().numpy() for s in stores]
+        if fmt == 1:
+            out_st = [s.view(np.uint16) for s in out_st]
+    return [x.cpu().numpy() for x in flds], r.cpu().numpy(), out_st
+
+
+def gpu_em3d_backward(case, grad_r, stores, *, store: str = "full", interval: int = 1,
+                      device="cuda:0", lam0=None, want_ca: bool = True, want_cb: bool = True):
+    """Run ``tide_em3d_<dtype>_backward_f64acc``.  Returns
+    ``(grad_ca, grad_cb, grad_f, lam)`` as numpy (FP64 gradients)."""
+    dev = torch.device(device)
+    dtype = torch.float64 if case.dtype == np.float64 else torch.float32
+    ca, cb, cq = (_dev(a, dev) for a in (case.ca, case.cb, case.cq))
+    prof = [_dev(p, dev) for p in case.profiles]
+    N = case.n_shots
+    nz, ny, nx = case.shape
+    n_src, n_rec = case.src.shape[1], case.rec.shape[1]
+    shape = (N, nz, ny, nx)
+    lam = ([torch.zeros(shape, device=dev, dtype=dtype) for _ in range(18)] if lam0 is None
+           else [_dev(np.ascontiguousarray(x).reshape(shape), dev, dtype) for x in lam0])
+    gr = _dev(grad_r, dev, dtype)
+    gf = torch.zeros(case.nt, N, n_src, device=dev, dtype=dtype)
+    gshape = shape if case.ca.shape[0] != 1 else shape[1:]
+    gca = torch.zeros(gshape, device=dev, dtype=torch.float64) if want_ca else None
+    gcb = torch.zeros(gshape, device=dev, dtype=torch.float64) if want_cb else None
+    fmt = 1 if store == "bf16" else 0
+    st_t = None
+    if stores is not None:
+        st_t = [torch.from_numpy(np.ascontiguousarray(s).view(np.int16) if fmt == 1
+                                 else np.ascontiguousarray(s)).to(dev) for s in stores]
+    src = torch.from_numpy(np.ascontiguousarray(case.src, np.int64))
+    rec = torch.from_numpy(np.ascontiguousarray(case.rec, np.int64))
+    batched = case.ca.shape[0] != 1
+    step_bytes = N * nz * ny * nx * (2 if fmt == 1 else case.dtype.itemsize)
+    fn = getattr(_native.load(), f"tide_em3d_{_native.dtype_name(dtype)}_backward_f64acc")
+    st = fn(_native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(gr),
+            _native.ptr_array(lam), _native.ptr_array(st_t if st_t is not None else [None] * 6),
+            fmt, step_bytes, 0, _native.ptr(gf), _native.ptr(gca), _native.ptr(gcb),
+            _native.ptr_array(prof), _native.ptr(src), _native.ptr(rec), case.rdz, case.rdy,
+            case.rdx, case.nt, N, nz, ny, nx, n_src, n_rec, interval, batched, batched, batched,
+            case.nt, _native.int64_array(case.pml), case.src_comp, case.rec_comp,
+            dev.index or 0, _native.stream_handle(dev))
+    _native.check(st)
+    torch.cuda.synchronize(dev)
+    to_np = lambda t: None if t is None else t.cpu().numpy()  # noqa: E731
+    return to_np(gca), to_np(gcb), gf.cpu().numpy(), [x.cpu().numpy() for x in lam]

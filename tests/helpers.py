@@ -66,3 +66,43 @@ def hetero_problem(*, ny=40, nx=52, nt=160, shots=3, seed=1234, dtype=torch.floa
     return dict(epsilon=eps, sigma=sig, mu=torch.ones_like(eps), grid_spacing=dx, dt=dt,
                 source_amplitude=amp, source_location=src, receiver_location=rec,
                 pml_width=list(pml))
+
+
+def hetero3d_problem(*, nz=10, ny=11, nx=13, nt=40, shots=2, seed=77, dtype=torch.float32,
+                     device="cpu", pml=(3, 2, 4, 3, 2, 3), dx=0.02, src_comp="ey", rec_comp="ey"):
+    """Small heterogeneous 3-D problem with uneven CPML for parity tests."""
+    g = torch.Generator().manual_seed(seed)
+    eps = 1.0 + 8.0 * torch.rand(nz, ny, nx, generator=g, dtype=torch.float64)
+    sig = 0.01 * torch.rand(nz, ny, nx, generator=g, dtype=torch.float64)
+    dt = 0.9 * dx / (3**0.5 * tide.C0)
+    amp = tide.ricker(3e8, nt, dt, dtype=torch.float64).reshape(1, 1, -1).repeat(shots, 1, 1)
+    src = torch.stack([torch.tensor([[2 + s % (nz - 4), 3, 2 + 3 * s % (nx - 4)]]) for s in range(shots)])
+    rec = torch.stack([torch.tensor([[1, 1 + k, nx - 2 - k] for k in range(0, ny - 2, 3)])
+                       for _ in range(shots)])
+    c = lambda t: t.to(dtype=dtype, device=device)  # noqa: E731
+    return dict(epsilon=c(eps), sigma=c(sig), mu=c(torch.ones_like(eps)), grid_spacing=dx, dt=dt,
+                source_amplitude=c(amp), source_location=src, receiver_location=rec,
+                pml_width=list(pml), source_component=src_comp, receiver_component=rec_comp)
+
+
+PROFILE_NAMES_3D = ("az", "bz", "azh", "bzh", "ay", "by", "ayh", "byh", "ax", "bx", "axh",
+                    "bxh", "kz", "kzh", "ky", "kyh", "kx", "kxh")
+FIELD_NAMES_3D = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def load_golden3d(name: str) -> tuple[O.EMCase, dict]:
+    z = np.load(GOLDEN / f"{name}.npz")
+    ca = z["ca"]
+    n_shots = z["src"].shape[0] if z["src"].size else z["rec"].shape[0]
+    shape = ca.shape[-3:]
+    nt, z0, y0, x0, z1, y1, x1, sc, rc = (int(v) for v in z["ints"])
+    rdz, rdy, rdx, dt = (float(v) for v in z["scalars"])
+    case = O.EMCase(
+        ca=ca, cb=z["cb"], cq=z["cq"], f=z["f"],
+        profiles=[z[f"prof_{n}"] for n in PROFILE_NAMES_3D], src=z["src"], rec=z["rec"],
+        rdz=rdz, rdy=rdy, rdx=rdx, dt=dt, nt=nt, pml=(z0, y0, x0, z1, y1, x1),
+        src_comp=sc, rec_comp=rc,
+        fields=[np.zeros((n_shots, *shape), ca.dtype) for _ in range(18)],
+    )
+    expected = {"r": z["r"], **{k: z["out_" + k] for k in FIELD_NAMES_3D}}
+    return case, expected

@@ -1,0 +1,1111 @@
+// 3-D Maxwell FDTD propagator for MI355X (gfx950), stencil order 2.
+//
+// Two kernels per forward step (H update, then E update) and two per
+// adjoint step (transpose of the E update, then of the H update).  Each
+// kernel only writes the field family it updates and reads the other, so
+// all of them work in place; the CPML adjoint memories, which neighbours
+// read while the owner writes, ping-pong between two buffers.
+//
+// Layout: fields are [n_shots, nz, ny, nx] (x fastest).  A workgroup is one
+// (62-column x tile, y row, z block) with one wavefront per shot; the 64
+// lanes cover columns x0-1 .. x0+62 (lanes 1..62 own their cells), the
+// x-neighbour comes from one cross-lane shuffle, the z-neighbour is carried
+// in registers while the wavefront marches through its z block, and the
+// y-neighbour row is read directly (it is the neighbouring workgroup's own
+// row, so it comes from L2).  Coefficients are read by every wavefront of a
+// workgroup from the same lines.  Sources, receivers, the adjoint injection
+// and the FP64 coefficient gradient (summed over the workgroup's shots in
+// LDS in a fixed order) are handled by the thread that owns the cell.
+//
+// Semantics follow the reference propagator (paths relative to src/tide/):
+//   H / E updates     csrc/em3d/maxwell_3d_inst.cpp:541-670 (CPU loops),
+//                     storing E and curl H before the update :1000-1061
+//   point ops         csrc/em3d/maxwell_3d_inst.cpp:300-348
+//   gradient          csrc/em3d/maxwell_3d_inst.cpp:221-298
+//   exact adjoint     csrc/em3d/maxwell_3d_cuda_inst.cu:1420-2187
+// with the same floating-point operation order (no FMA contraction).
+
+#include "tide_common.h"
+#include "tide_hip.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace tide_hip {
+namespace em3d {
+
+constexpr int kOwn = 62;
+constexpr int kMaxGroup = 8;
+enum { kA = 0, kB = 1, kK = 2, kAh = 3, kBh = 4, kKh = 5 };
+enum { EX = 0, EY, EZ, HX, HY, HZ,
+       M_HZ_Y, M_HY_Z, M_HX_Z, M_HZ_X, M_HY_X, M_HX_Y,
+       M_EY_Z, M_EZ_Y, M_EZ_X, M_EX_Z, M_EX_Y, M_EY_X, N_SLOTS };
+
+// CPML profiles packed per axis as [a, b, k, ah, bh, kh] x n.
+template <typename T> struct Geom3 {
+  const T *pz, *py, *px;
+  T rdz, rdy, rdx;
+  int32_t nz, ny, nx, numel;
+  int32_t z0, z1, z1h, y0, y1, y1h, x0, x1, x1h;
+  int32_t ca_b, cb_b, cq_b;
+  __device__ __forceinline__ T Z(int k, int z) const { return pz[k * nz + z]; }
+  __device__ __forceinline__ T Y(int k, int y) const { return py[k * ny + y]; }
+  __device__ __forceinline__ T X(int k, int x) const { return px[k * nx + x]; }
+};
+
+struct Points {
+  const int32_t *row_slot;  // indexed by line z*ny + y
+  const int32_t *seg;
+  const int2 *ent;
+  int32_t n_xt;
+};
+
+template <typename T> struct Args3 {
+  Geom3<T> g;
+  Points pts;
+  const T *ca, *cb, *cq;
+  T *f[N_SLOTS];        // forward state, or adjoint state (same slots)
+  T *mn[N_SLOTS];       // adjoint: next CPML memories (memory slots only)
+  void *st[6];          // E x/y/z, curl x/y/z snapshots of this step
+  const T *src_amp;     // f[t] (forward) or grad_r[t] (adjoint)
+  T *rec_out;           // r[t] (forward) or grad_f[t] (adjoint)
+  double *gca, *gcb;
+  double step_scale;
+  int32_t per_shot_grad, use_atomics;
+  int32_t n_shots, n_zb, src_comp, rec_comp;
+};
+
+template <typename T, int FMT>
+__device__ __forceinline__ void put_store(void *base, int64_t i, T v) {
+  if constexpr (FMT == 2) {
+    ((uint16_t *)base)[i] = f32_to_bf16((float)v);
+  } else {
+    ((T *)base)[i] = v;
+  }
+}
+
+template <typename T, int FMT>
+__device__ __forceinline__ double get_store(const void *base, int64_t i) {
+  if constexpr (FMT == 2) {
+    return (double)bf16_to_f32(((const uint16_t *)base)[i]);
+  } else {
+    return (double)((const T *)base)[i];
+  }
+}
+
+struct Tile3 {
+  int xt, y, zb, grp;
+};
+
+__device__ __forceinline__ Tile3 decode3(int n_xt, int ny, int n_zb, int n_groups) {
+  const int64_t n_tiles = (int64_t)n_xt * ny * n_zb;
+  const int64_t flat = xcd_remap((int64_t)blockIdx.x, n_tiles * n_groups);
+  Tile3 c;
+  c.grp = (int)(flat / n_tiles);
+  int64_t t = flat - (int64_t)c.grp * n_tiles;
+  c.zb = (int)(t / ((int64_t)n_xt * ny));
+  t -= (int64_t)c.zb * n_xt * ny;
+  c.y = (int)(t / n_xt);
+  c.xt = (int)(t - (int64_t)c.y * n_xt);
+  return c;
+}
+
+// Reads p[z, y, x] of one shot plane; zero outside the grid.
+template <typename T> struct Ld3 {
+  int xc, nx, ny, nz;
+  bool xv;
+  __device__ __forceinline__ Ld3(int x, int nx_, int ny_, int nz_) : nx(nx_), ny(ny_), nz(nz_) {
+    xv = x >= 0 && x < nx_;
+    xc = min(max(x, 0), nx_ - 1);
+  }
+  __device__ __forceinline__ T operator()(const T *__restrict__ p, int z, int y) const {
+    if (z < 0 || z >= nz || y < 0 || y >= ny) return (T)0;  // uniform
+    const T v = p[(z * ny + y) * nx + xc];
+    return xv ? v : (T)0;
+  }
+};
+
+// CPML correction of a raw difference d with memory m (updated in place):
+// m = b m + a d; d = d / k + m.
+template <typename T>
+__device__ __forceinline__ T cpml(T d, T *__restrict__ m, int j, T a, T b, T k, bool own) {
+  const T mv = b * m[j] + a * d;
+  if (own) m[j] = mv;
+  return d / k + mv;
+}
+
+// ----------------------------------------------------------- forward: H
+template <typename T, int TZ>
+__global__ __launch_bounds__(512) void h3_step(Args3<T> a) {
+  const Geom3<T> &g = a.g;
+  const int G = (int)blockDim.y;
+  const int n_groups = (a.n_shots + G - 1) / G;
+  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
+  const int s = tc.grp * G + (int)threadIdx.y;
+  if (s >= a.n_shots) return;
+  const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
+  const int lane = threadIdx.x;
+  const int x = tc.xt * kOwn - 1 + lane;
+  const bool own = lane >= 1 && lane <= kOwn && x >= 1 && x < nx && y >= 1;
+  const int64_t base = (int64_t)s * g.numel;
+  const T *__restrict__ ex = a.f[EX] + base;
+  const T *__restrict__ ey = a.f[EY] + base;
+  const T *__restrict__ ez = a.f[EZ] + base;
+  T *__restrict__ hx = a.f[HX] + base;
+  T *__restrict__ hy = a.f[HY] + base;
+  T *__restrict__ hz = a.f[HZ] + base;
+  const T *__restrict__ cq = a.cq + (g.cq_b ? base : 0);
+  const Ld3<T> L(x, nx, ny, nz);
+  const bool has_dy = y < ny - 1, has_dx = x < nx - 1;
+  const bool pyh = y < g.y0 || y >= g.y1h;
+  const bool pxh = x < g.x0 || x >= g.x1h;
+  const int zs = tc.zb * TZ;
+
+  T exc = L(ex, zs, y), eyc = L(ey, zs, y), ezc = L(ez, zs, y);
+#pragma unroll
+  for (int r = 0; r < TZ; ++r) {
+    const int z = zs + r;
+    if (z >= nz) break;  // uniform
+    const T exn = L(ex, z + 1, y), eyn = L(ey, z + 1, y), ezn = L(ez, z + 1, y);
+    const T ex_yp = L(ex, z, y + 1), ez_yp = L(ez, z, y + 1);
+    const T ez_xp = __shfl_down(ezc, 1, 64), ey_xp = __shfl_down(eyc, 1, 64);
+    if (own && z >= 1) {
+      const int j = (z * ny + y) * nx + x;
+      const T cqv = cq[j];
+      const bool has_dz = z < nz - 1;
+      const bool pzh = z < g.z0 || z >= g.z1h;
+      T dey_dz = (T)0, dez_dy = (T)0, dez_dx = (T)0, dex_dz = (T)0, dex_dy = (T)0, dey_dx = (T)0;
+      if (has_dz) {
+        dey_dz = (eyn - eyc) * g.rdz;
+        dex_dz = (exn - exc) * g.rdz;
+        if (pzh) {
+          const T az = g.Z(kAh, z), bz = g.Z(kBh, z), kz = g.Z(kKh, z);
+          dey_dz = cpml(dey_dz, a.f[M_EY_Z] + base, j, az, bz, kz, true);
+          dex_dz = cpml(dex_dz, a.f[M_EX_Z] + base, j, az, bz, kz, true);
+        }
+      }
+      if (has_dy) {
+        dez_dy = (ez_yp - ezc) * g.rdy;
+        dex_dy = (ex_yp - exc) * g.rdy;
+        if (pyh) {
+          const T ay = g.Y(kAh, y), by = g.Y(kBh, y), ky = g.Y(kKh, y);
+          dez_dy = cpml(dez_dy, a.f[M_EZ_Y] + base, j, ay, by, ky, true);
+          dex_dy = cpml(dex_dy, a.f[M_EX_Y] + base, j, ay, by, ky, true);
+        }
+      }
+      if (has_dx) {
+        dez_dx = (ez_xp - ezc) * g.rdx;
+        dey_dx = (ey_xp - eyc) * g.rdx;
+        if (pxh) {
+          const T ax = g.X(kAh, x), bx = g.X(kBh, x), kx = g.X(kKh, x);
+          dez_dx = cpml(dez_dx, a.f[M_EZ_X] + base, j, ax, bx, kx, true);
+          dey_dx = cpml(dey_dx, a.f[M_EY_X] + base, j, ax, bx, kx, true);
+        }
+      }
+      hx[j] = hx[j] - cqv * (dey_dz - dez_dy);
+      hy[j] = hy[j] - cqv * (dez_dx - dex_dz);
+      hz[j] = hz[j] - cqv * (dex_dy - dey_dx);
+    }
+    exc = exn;
+    eyc = eyn;
+    ezc = ezn;
+  }
+}
+
+// ----------------------------------------------------------- forward: E
+template <typename T, int TZ, int FMT>
+__global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
+  const Geom3<T> &g = a.g;
+  const int G = (int)blockDim.y;
+  const int n_groups = (a.n_shots + G - 1) / G;
+  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
+  const int s = tc.grp * G + (int)threadIdx.y;
+  if (s >= a.n_shots) return;
+  const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
+  const int lane = threadIdx.x;
+  const int x = tc.xt * kOwn - 1 + lane;
+  const bool own = lane >= 1 && lane <= kOwn && x >= 1 && x < nx && y >= 1;
+  const int64_t base = (int64_t)s * g.numel;
+  T *__restrict__ ex = a.f[EX] + base;
+  T *__restrict__ ey = a.f[EY] + base;
+  T *__restrict__ ez = a.f[EZ] + base;
+  const T *__restrict__ hx = a.f[HX] + base;
+  const T *__restrict__ hy = a.f[HY] + base;
+  const T *__restrict__ hz = a.f[HZ] + base;
+  const T *__restrict__ ca = a.ca + (g.ca_b ? base : 0);
+  const T *__restrict__ cb = a.cb + (g.cb_b ? base : 0);
+  const Ld3<T> L(x, nx, ny, nz);
+  const bool py = y < g.y0 || y >= g.y1;
+  const bool px = x < g.x0 || x >= g.x1;
+  const int zs = tc.zb * TZ;
+
+  T hx_zm = L(hx, zs - 1, y), hy_zm = L(hy, zs - 1, y);
+#pragma unroll
+  for (int r = 0; r < TZ; ++r) {
+    const int z = zs + r;
+    if (z >= nz) break;
+    const T hxc = L(hx, z, y), hyc = L(hy, z, y), hzc = L(hz, z, y);
+    const T hz_ym = L(hz, z, y - 1), hx_ym = L(hx, z, y - 1);
+    const T hz_xm = __shfl_up(hzc, 1, 64), hy_xm = __shfl_up(hyc, 1, 64);
+    if (own && z >= 1) {
+      const int j = (z * ny + y) * nx + x;
+      const bool pz = z < g.z0 || z >= g.z1;
+      T dhy_dz = (hyc - hy_zm) * g.rdz;
+      T dhz_dy = (hzc - hz_ym) * g.rdy;
+      T dhz_dx = (hzc - hz_xm) * g.rdx;
+      T dhx_dz = (hxc - hx_zm) * g.rdz;
+      T dhx_dy = (hxc - hx_ym) * g.rdy;
+      T dhy_dx = (hyc - hy_xm) * g.rdx;
+      if (pz) {
+        const T az = g.Z(kA, z), bz = g.Z(kB, z), kz = g.Z(kK, z);
+        dhy_dz = cpml(dhy_dz, a.f[M_HY_Z] + base, j, az, bz, kz, true);
+        dhx_dz = cpml(dhx_dz, a.f[M_HX_Z] + base, j, az, bz, kz, true);
+      }
+      if (py) {
+        const T ay = g.Y(kA, y), by = g.Y(kB, y), ky = g.Y(kK, y);
+        dhz_dy = cpml(dhz_dy, a.f[M_HZ_Y] + base, j, ay, by, ky, true);
+        dhx_dy = cpml(dhx_dy, a.f[M_HX_Y] + base, j, ay, by, ky, true);
+      }
+      if (px) {
+        const T ax = g.X(kA, x), bx = g.X(kB, x), kx = g.X(kK, x);
+        dhz_dx = cpml(dhz_dx, a.f[M_HZ_X] + base, j, ax, bx, kx, true);
+        dhy_dx = cpml(dhy_dx, a.f[M_HY_X] + base, j, ax, bx, kx, true);
+      }
+      const T cx = dhy_dz - dhz_dy;
+      const T cy = dhz_dx - dhx_dz;
+      const T cz = dhx_dy - dhy_dx;
+      T e0 = ex[j], e1 = ey[j], e2 = ez[j];
+      if constexpr (FMT != 0) {
+        const int64_t i = base + j;
+        if (a.st[0]) {
+          put_store<T, FMT>(a.st[0], i, e0);
+          put_store<T, FMT>(a.st[1], i, e1);
+          put_store<T, FMT>(a.st[2], i, e2);
+        }
+        if (a.st[3]) {
+          put_store<T, FMT>(a.st[3], i, cx);
+          put_store<T, FMT>(a.st[4], i, cy);
+          put_store<T, FMT>(a.st[5], i, cz);
+        }
+      }
+      const T cav = ca[j], cbv = cb[j];
+      e0 = cav * e0 + cbv * cx;
+      e1 = cav * e1 + cbv * cy;
+      e2 = cav * e2 + cbv * cz;
+      // Sources of this shot at this cell, then receivers.
+      const int slot = a.pts.row_slot[z * ny + y];
+      if (slot >= 0) {
+        const int64_t si = ((int64_t)slot * a.n_shots + s) * a.pts.n_xt + tc.xt;
+        const int e1_ = a.pts.seg[si + 1];
+        for (int e = a.pts.seg[si]; e < e1_; ++e) {
+          const int2 en = a.pts.ent[e];
+          if ((en.x & 0xff) != lane) continue;
+          if ((en.x >> 8) == 0) {
+            const T v = a.src_amp[en.y];
+            if (a.src_comp == 0) e0 += v;
+            else if (a.src_comp == 1) e1 += v;
+            else e2 += v;
+          } else {
+            a.rec_out[en.y] = a.rec_comp == 0 ? e0 : (a.rec_comp == 1 ? e1 : e2);
+          }
+        }
+      }
+      ex[j] = e0;
+      ey[j] = e1;
+      ez[j] = e2;
+    }
+    hx_zm = hxc;
+    hy_zm = hyc;
+  }
+}
+
+// ------------------------------------------------------------- adjoint
+// Adjoint of a CPML-corrected derivative whose adjoint is q:
+// strip: a (b M + q) + q / k, with the next memory b M + q; else q.
+template <typename T>
+__device__ __forceinline__ T adj_raw(T q, T m, T a, T b, T k, bool strip) {
+  if (!strip) return q;
+  const T mn = b * m + q;
+  return a * mn + q / k;
+}
+
+// Transpose of the E update: lambda_H += D^T (raw E-curl adjoints); the
+// E-phase adjoint memories advance into a.mn.
+template <typename T, int TZ>
+__global__ __launch_bounds__(512) void adj_e3(Args3<T> a) {
+  const Geom3<T> &g = a.g;
+  const int G = (int)blockDim.y;
+  const int n_groups = (a.n_shots + G - 1) / G;
+  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
+  const int s = tc.grp * G + (int)threadIdx.y;
+  if (s >= a.n_shots) return;
+  const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
+  const int lane = threadIdx.x;
+  const int x = tc.xt * kOwn - 1 + lane;
+  const bool own = lane >= 1 && lane <= kOwn && x >= 1 && x < nx && y >= 1;
+  const int64_t base = (int64_t)s * g.numel;
+  const T *__restrict__ lex = a.f[EX] + base;
+  const T *__restrict__ ley = a.f[EY] + base;
+  const T *__restrict__ lez = a.f[EZ] + base;
+  T *__restrict__ lhx = a.f[HX] + base;
+  T *__restrict__ lhy = a.f[HY] + base;
+  T *__restrict__ lhz = a.f[HZ] + base;
+  const T *__restrict__ m_hy_z = a.f[M_HY_Z] + base;
+  const T *__restrict__ m_hx_z = a.f[M_HX_Z] + base;
+  const T *__restrict__ m_hz_y = a.f[M_HZ_Y] + base;
+  const T *__restrict__ m_hx_y = a.f[M_HX_Y] + base;
+  const T *__restrict__ m_hz_x = a.f[M_HZ_X] + base;
+  const T *__restrict__ m_hy_x = a.f[M_HY_X] + base;
+  const T *__restrict__ cb = a.cb + (g.cb_b ? base : 0);
+  const Ld3<T> L(x, nx, ny, nz);
+  const bool xdom = x >= 1 && x < nx;
+  const bool py = y < g.y0 || y >= g.y1, py1 = y + 1 < g.y0 || y + 1 >= g.y1;
+  const bool px = x < g.x0 || x >= g.x1;
+  const int zs = tc.zb * TZ;
+  // y profiles at y and y + 1; x profiles at x (clamped for halo lanes)
+  const int xc = min(max(x, 0), nx - 1);
+  const T ay = g.Y(kA, y), by = g.Y(kB, y), ky = g.Y(kK, y);
+  const int yp = min(y + 1, ny - 1);
+  const T ay1 = g.Y(kA, yp), by1 = g.Y(kB, yp), ky1 = g.Y(kK, yp);
+  const T ax = g.X(kA, xc), bx = g.X(kB, xc), kx = g.X(kK, xc);
+
+  // z-derivative raws at plane zz: (dHy/dz from Ex, +) and (dHx/dz from Ey, -)
+  auto z_raws = [&](int zz, T &r_yz, T &r_xz) {
+    r_yz = (T)0;
+    r_xz = (T)0;
+    if (zz < 1 || zz >= nz || y < 1 || !xdom) return;
+    const int j = (zz * ny + y) * nx + x;
+    const T cbv = cb[j];
+    const T q_yz = cbv * lex[j];
+    const T q_xz = -(cbv * ley[j]);
+    const bool pz = zz < g.z0 || zz >= g.z1;
+    const T az = g.Z(kA, zz), bz = g.Z(kB, zz), kz = g.Z(kK, zz);
+    r_yz = adj_raw(q_yz, pz ? m_hy_z[j] : (T)0, az, bz, kz, pz);
+    r_xz = adj_raw(q_xz, pz ? m_hx_z[j] : (T)0, az, bz, kz, pz);
+  };
+
+  T ryz_c, rxz_c;
+  z_raws(zs, ryz_c, rxz_c);
+#pragma unroll
+  for (int r = 0; r < TZ; ++r) {
+    const int z = zs + r;
+    if (z >= nz) break;
+    T ryz_n, rxz_n;
+    z_raws(z + 1, ryz_n, rxz_n);
+    const bool zdom = z >= 1;
+    const int j = (z * ny + y) * nx + x;
+    // x-derivative raws at (z, y, x) for every lane (the left lane needs ours)
+    T r_zx = (T)0, r_yx = (T)0;
+    const T cbv = L(cb, z, y);
+    const T l0 = L(lex, z, y), l1 = L(ley, z, y), l2 = L(lez, z, y);
+    if (zdom && y >= 1 && xdom) {
+      const T q_zx = cbv * l1;
+      const T q_yx = -(cbv * l2);
+      r_zx = adj_raw(q_zx, px ? m_hz_x[j] : (T)0, ax, bx, kx, px);
+      r_yx = adj_raw(q_yx, px ? m_hy_x[j] : (T)0, ax, bx, kx, px);
+    }
+    const T r_zx_xp = __shfl_down(r_zx, 1, 64);
+    const T r_yx_xp = __shfl_down(r_yx, 1, 64);
+    if (own && zdom) {
+      // y-derivative raws at y and y + 1: (dHz/dy from Ex, -) and (dHx/dy from Ez, +)
+      const T q_zy = -(cbv * l0);
+      const T q_xy = cbv * l2;
+      const T r_zy = adj_raw(q_zy, py ? m_hz_y[j] : (T)0, ay, by, ky, py);
+      const T r_xy = adj_raw(q_xy, py ? m_hx_y[j] : (T)0, ay, by, ky, py);
+      T r_zy_yp = (T)0, r_xy_yp = (T)0;
+      if (y + 1 < ny) {
+        const int jp = j + nx;
+        const T cbp = cb[jp];
+        const T q_zy1 = -(cbp * lex[jp]);
+        const T q_xy1 = cbp * lez[jp];
+        r_zy_yp = adj_raw(q_zy1, py1 ? m_hz_y[jp] : (T)0, ay1, by1, ky1, py1);
+        r_xy_yp = adj_raw(q_xy1, py1 ? m_hx_y[jp] : (T)0, ay1, by1, ky1, py1);
+      }
+      // next E-phase memories at this cell
+      const bool pz = z < g.z0 || z >= g.z1;
+      if (pz) {
+        const T bz = g.Z(kB, z);
+        a.mn[M_HY_Z][base + j] = bz * m_hy_z[j] + cbv * l0;
+        a.mn[M_HX_Z][base + j] = bz * m_hx_z[j] + -(cbv * l1);
+      }
+      if (py) {
+        a.mn[M_HZ_Y][base + j] = by * m_hz_y[j] + -(cbv * l0);
+        a.mn[M_HX_Y][base + j] = by * m_hx_y[j] + cbv * l2;
+      }
+      if (px) {
+        a.mn[M_HZ_X][base + j] = bx * m_hz_x[j] + cbv * l1;
+        a.mn[M_HY_X][base + j] = bx * m_hy_x[j] + -(cbv * l2);
+      }
+      T vx = lhx[j];
+      vx = vx + (rxz_c - rxz_n) * g.rdz;
+      vx = vx + (r_xy - r_xy_yp) * g.rdy;
+      T vy = lhy[j];
+      vy = vy + (ryz_c - ryz_n) * g.rdz;
+      vy = vy + (r_yx - r_yx_xp) * g.rdx;
+      T vz = lhz[j];
+      vz = vz + (r_zy - r_zy_yp) * g.rdy;
+      vz = vz + (r_zx - r_zx_xp) * g.rdx;
+      lhx[j] = vx;
+      lhy[j] = vy;
+      lhz[j] = vz;
+    }
+    ryz_c = ryz_n;
+    rxz_c = rxz_n;
+  }
+}
+
+// Transpose of the H update: lambda_E = ca lambda_E + D^T (raw H-term
+// adjoints); H-phase adjoint memories advance into a.mn; then receivers
+// inject grad_r[t], sources read grad_f[t], and the coefficient gradient
+// accumulates against the stored forward state.
+template <typename T, int TZ, int FMT>
+__global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
+  extern __shared__ double red[];  // [2][G][TZ][64] when reducing
+  const Geom3<T> &g = a.g;
+  const int G = (int)blockDim.y;
+  const int w = (int)threadIdx.y;
+  const int n_groups = (a.n_shots + G - 1) / G;
+  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
+  const int s = tc.grp * G + w;
+  const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
+  const int lane = threadIdx.x;
+  const int x = tc.xt * kOwn - 1 + lane;
+  const bool own = lane >= 1 && lane <= kOwn && x >= 1 && x < nx && y >= 1;
+  const bool want_grad = (a.gca != nullptr || a.gcb != nullptr) && FMT != 0;
+  const bool reduce = want_grad && !a.per_shot_grad;
+  const int zs = tc.zb * TZ;
+  double *red_ca = red + (int64_t)w * TZ * 64;
+  double *red_cb = red + ((int64_t)G + w) * TZ * 64;
+  if (reduce) {
+#pragma unroll
+    for (int r = 0; r < TZ; ++r) {
+      red_ca[r * 64 + lane] = 0.0;
+      red_cb[r * 64 + lane] = 0.0;
+    }
+  }
+  if (s < a.n_shots) {  // inactive wavefronts still join the barrier below
+    const int64_t base = (int64_t)s * g.numel;
+    T *__restrict__ lex = a.f[EX] + base;
+    T *__restrict__ ley = a.f[EY] + base;
+    T *__restrict__ lez = a.f[EZ] + base;
+    const T *__restrict__ lhx = a.f[HX] + base;
+    const T *__restrict__ lhy = a.f[HY] + base;
+    const T *__restrict__ lhz = a.f[HZ] + base;
+    const T *__restrict__ m_ey_z = a.f[M_EY_Z] + base;
+    const T *__restrict__ m_ex_z = a.f[M_EX_Z] + base;
+    const T *__restrict__ m_ez_y = a.f[M_EZ_Y] + base;
+    const T *__restrict__ m_ex_y = a.f[M_EX_Y] + base;
+    const T *__restrict__ m_ez_x = a.f[M_EZ_X] + base;
+    const T *__restrict__ m_ey_x = a.f[M_EY_X] + base;
+    const T *__restrict__ ca = a.ca + (g.ca_b ? base : 0);
+    const T *__restrict__ cq = a.cq + (g.cq_b ? base : 0);
+    const Ld3<T> L(x, nx, ny, nz);
+    const bool xdom = x >= 1 && x < nx;
+    const bool has_dx = x < nx - 1;
+    const bool pyh = y < g.y0 || y >= g.y1h;
+    const int ym = y - 1;
+    const bool pyh_m = ym < g.y0 || ym >= g.y1h;
+    const int xc = min(max(x, 0), nx - 1);
+    const int ymc = max(ym, 0);
+    const T ay = g.Y(kAh, y), by = g.Y(kBh, y), ky = g.Y(kKh, y);
+    const T aym = g.Y(kAh, ymc), bym = g.Y(kBh, ymc), kym = g.Y(kKh, ymc);
+    const bool pxh = x < g.x0 || x >= g.x1h;
+    const T ax = g.X(kAh, xc), bx = g.X(kBh, xc), kx = g.X(kKh, xc);
+
+    // z-derivative raws at plane zz (cell (zz, y, x)):
+    //   Hx term dEy/dz (first, -cq lHx) and Hy term dEx/dz (second, +cq lHy)
+    auto z_raws = [&](int zz, T &r_yz, T &r_xz) {
+      r_yz = (T)0;
+      r_xz = (T)0;
+      if (zz < 1 || zz >= nz - 1 || y < 1 || !xdom) return;
+      const int j = (zz * ny + y) * nx + x;
+      const T cqv = cq[j];
+      const T adj_yz = -cqv * lhx[j];
+      const T adj_xz = cqv * lhy[j];
+      const bool pz = zz < g.z0 || zz >= g.z1h;
+      const T az = g.Z(kAh, zz), bz = g.Z(kBh, zz), kz = g.Z(kKh, zz);
+      r_yz = adj_raw(adj_yz, pz ? m_ey_z[j] : (T)0, az, bz, kz, pz);
+      r_xz = adj_raw(adj_xz, pz ? m_ex_z[j] : (T)0, az, bz, kz, pz);
+    };
+
+    T ryz_p, rxz_p;
+    z_raws(zs - 1, ryz_p, rxz_p);
+    float se[3][TZ], sc[3][TZ];
+#pragma unroll
+    for (int r = 0; r < TZ; ++r) {
+      const int z = zs + r;
+      if (z >= nz) break;
+      const bool zdom = z >= 1;
+      const int j = (z * ny + y) * nx + x;
+      T ryz_c, rxz_c;
+      z_raws(z, ryz_c, rxz_c);
+      // x-derivative raws at (z, y, x): Hy term dEz/dx (first, -cq lHy) and
+      // Hz term dEy/dx (second, +cq lHz); the right lane needs ours
+      T r_zx = (T)0, r_yx = (T)0;
+      const T cqv = L(cq, z, y);
+      const T h0 = L(lhx, z, y), h1 = L(lhy, z, y), h2 = L(lhz, z, y);
+      if (zdom && y >= 1 && xdom && has_dx) {
+        const T adj_zx = -cqv * h1;
+        const T adj_yx = cqv * h2;
+        r_zx = adj_raw(adj_zx, pxh ? m_ez_x[j] : (T)0, ax, bx, kx, pxh);
+        r_yx = adj_raw(adj_yx, pxh ? m_ey_x[j] : (T)0, ax, bx, kx, pxh);
+      }
+      const T r_zx_xm = __shfl_up(r_zx, 1, 64);
+      const T r_yx_xm = __shfl_up(r_yx, 1, 64);
+      if (own && zdom) {
+        // y-derivative raws at y and y - 1: Hx term dEz/dy (second, +cq lHx)
+        // and Hz term dEx/dy (first, -cq lHz); defined for y < ny - 1
+        T r_zy = (T)0, r_xy = (T)0;
+        if (y < ny - 1) {
+          const T adj_zy = cqv * h0;
+          const T adj_xy = -cqv * h2;
+          r_zy = adj_raw(adj_zy, pyh ? m_ez_y[j] : (T)0, ay, by, ky, pyh);
+          r_xy = adj_raw(adj_xy, pyh ? m_ex_y[j] : (T)0, ay, by, ky, pyh);
+        }
+        T r_zy_ym = (T)0, r_xy_ym = (T)0;
+        if (ym >= 1) {
+          const int jm = j - nx;
+          const T cqm = cq[jm];
+          const T adj_zy1 = cqm * lhx[jm];
+          const T adj_xy1 = -cqm * lhz[jm];
+          r_zy_ym = adj_raw(adj_zy1, pyh_m ? m_ez_y[jm] : (T)0, aym, bym, kym, pyh_m);
+          r_xy_ym = adj_raw(adj_xy1, pyh_m ? m_ex_y[jm] : (T)0, aym, bym, kym, pyh_m);
+        }
+        const T rzx_m = x - 1 >= 1 ? r_zx_xm : (T)0;
+        const T ryx_m = x - 1 >= 1 ? r_yx_xm : (T)0;
+        // next H-phase memories at this cell
+        const bool pzh = z < g.z0 || z >= g.z1h;
+        if (z < nz - 1 && pzh) {
+          const T bz = g.Z(kBh, z);
+          a.mn[M_EY_Z][base + j] = bz * m_ey_z[j] + -cqv * h0;
+          a.mn[M_EX_Z][base + j] = bz * m_ex_z[j] + cqv * h1;
+        }
+        if (y < ny - 1 && pyh) {
+          a.mn[M_EZ_Y][base + j] = by * m_ez_y[j] + cqv * h0;
+          a.mn[M_EX_Y][base + j] = by * m_ex_y[j] + -cqv * h2;
+        }
+        if (has_dx && pxh) {
+          a.mn[M_EZ_X][base + j] = bx * m_ez_x[j] + -cqv * h1;
+          a.mn[M_EY_X][base + j] = bx * m_ey_x[j] + cqv * h2;
+        }
+        const T cav = ca[j];
+        const T tx = (rxz_p - rxz_c) * g.rdz + (r_xy_ym - r_xy) * g.rdy;
+        const T ty = (ryz_p - ryz_c) * g.rdz + (ryx_m - r_yx) * g.rdx;
+        const T tz = (r_zy_ym - r_zy) * g.rdy + (rzx_m - r_zx) * g.rdx;
+        T v0 = cav * lex[j] + tx;
+        T v1 = cav * ley[j] + ty;
+        T v2 = cav * lez[j] + tz;
+        // receivers inject grad_r[t]; sources read grad_f[t]
+        const int slot = a.pts.row_slot[z * ny + y];
+        if (slot >= 0) {
+          const int64_t si = ((int64_t)slot * a.n_shots + s) * a.pts.n_xt + tc.xt;
+          const int e0 = a.pts.seg[si], e1 = a.pts.seg[si + 1];
+          for (int e = e0; e < e1; ++e) {
+            const int2 en = a.pts.ent[e];
+            if ((en.x >> 8) != 1 || (en.x & 0xff) != lane) continue;
+            const T v = a.src_amp[en.y];
+            if (a.rec_comp == 0) v0 += v;
+            else if (a.rec_comp == 1) v1 += v;
+            else v2 += v;
+          }
+          if (a.rec_out != nullptr) {
+            for (int e = e0; e < e1; ++e) {
+              const int2 en = a.pts.ent[e];
+              if ((en.x >> 8) != 0 || (en.x & 0xff) != lane) continue;
+              a.rec_out[en.y] = a.src_comp == 0 ? v0 : (a.src_comp == 1 ? v1 : v2);
+            }
+          }
+        }
+        lex[j] = v0;
+        ley[j] = v1;
+        lez[j] = v2;
+        if (want_grad) {
+          const int64_t i = base + j;
+          double pa = 0.0, pb = 0.0;
+          if (a.gca)
+            pa = ((double)v0 * get_store<T, FMT>(a.st[0], i) +
+                  (double)v1 * get_store<T, FMT>(a.st[1], i) +
+                  (double)v2 * get_store<T, FMT>(a.st[2], i)) * a.step_scale;
+          if (a.gcb)
+            pb = ((double)v0 * get_store<T, FMT>(a.st[3], i) +
+                  (double)v1 * get_store<T, FMT>(a.st[4], i) +
+                  (double)v2 * get_store<T, FMT>(a.st[5], i)) * a.step_scale;
+          if (reduce) {
+            red_ca[r * 64 + lane] = pa;
+            red_cb[r * 64 + lane] = pb;
+          } else {
+            if (a.gca) a.gca[i] += pa;
+            if (a.gcb) a.gcb[i] += pb;
+          }
+        }
+      }
+      ryz_p = ryz_c;
+      rxz_p = rxz_c;
+    }
+    (void)se;
+    (void)sc;
+  }
+  if (!reduce) return;  // uniform over the workgroup
+  __syncthreads();
+  const int x_ = tc.xt * kOwn - 1 + lane;
+  if (lane < 1 || lane > kOwn || x_ < 1 || x_ >= nx || y < 1) return;
+  for (int pass = 0; pass < 2; ++pass) {
+    double *out = pass == 0 ? a.gca : a.gcb;
+    if (out == nullptr) continue;
+    const double *slots = red + (int64_t)pass * G * TZ * 64;
+    for (int r = w; r < TZ; r += G) {
+      const int z = zs + r;
+      if (z >= 1 && z < nz) {
+        double sum = 0.0;
+        for (int k = 0; k < G; ++k) sum += slots[(k * TZ + r) * 64 + lane];
+        const int64_t j = ((int64_t)z * ny + y) * nx + x_;
+        if (a.use_atomics)
+          atomicAdd(&out[j], sum);
+        else
+          out[j] += sum;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ driver
+
+struct Tuning3 {
+  int fwd_z = 8, adj_z = 8, group = 8;
+};
+Tuning3 g_tuning3;
+
+template <typename T> struct Problem3 {
+  Geom3<T> g;
+  const T *ca, *cb, *cq;
+  const T *prof[18];  // az bz azh bzh ay by ayh byh ax bx axh bxh kz kzh ky kyh kx kxh
+  int32_t n_shots;
+  int64_t n_src, n_rec;
+  int32_t src_comp, rec_comp;
+};
+
+#define TIDE_CHECK(expr)                    \
+  do {                                      \
+    const int _rc = (expr);                 \
+    if (_rc != ::tide_hip::kOk) return _rc; \
+  } while (0)
+
+struct Scratch {
+  std::vector<void *> bufs;
+  hipStream_t stream;
+  explicit Scratch(hipStream_t s) : stream(s) {}
+  ~Scratch() {
+    for (void *p : bufs) (void)hipFreeAsync(p, stream);
+  }
+  template <typename U> int alloc(U **out, size_t count) {
+    void *p = nullptr;
+    TIDE_HIP_TRY(hipMallocAsync(&p, std::max<size_t>(count, 1) * sizeof(U), stream));
+    bufs.push_back(p);
+    *out = (U *)p;
+    return kOk;
+  }
+};
+
+inline int fetch_indices(const int64_t *p, int64_t n, std::vector<int64_t> &out,
+                         hipStream_t stream) {
+  out.assign((size_t)std::max<int64_t>(n, 0), 0);
+  if (n <= 0 || p == nullptr) return kOk;
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  const hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e == hipSuccess && attr.type == hipMemoryTypeDevice) {
+    TIDE_HIP_TRY(hipMemcpyAsync(out.data(), p, (size_t)n * sizeof(int64_t),
+                                hipMemcpyDeviceToHost, stream));
+    TIDE_HIP_TRY(hipStreamSynchronize(stream));
+  } else {
+    (void)hipGetLastError();
+    std::memcpy(out.data(), p, (size_t)n * sizeof(int64_t));
+  }
+  return kOk;
+}
+
+template <typename T>
+int prepare(Problem3<T> &p, const int64_t *src, const int64_t *rec, Scratch &scr,
+            DevicePointTable &dpt, hipStream_t stream) {
+  Geom3<T> &g = p.g;
+  if (p.n_shots <= 0 || g.nz < 3 || g.ny < 3 || g.nx < 3) {
+    set_error("n_shots must be positive and the padded grid at least 3x3x3");
+    return kInvalidArgument;
+  }
+  if ((int64_t)g.nz * g.ny * g.nx >= ((int64_t)1 << 31)) {
+    set_error("a shot volume must hold fewer than 2^31 cells");
+    return kInvalidArgument;
+  }
+  if (p.src_comp < 0 || p.src_comp > 2 || p.rec_comp < 0 || p.rec_comp > 2) {
+    set_error("source/receiver component must be 0 (x), 1 (y) or 2 (z)");
+    return kInvalidArgument;
+  }
+  // profiles: [a, b, k, ah, bh, kh] per axis
+  const int n[3] = {g.nz, g.ny, g.nx};
+  T *buf;
+  TIDE_CHECK(scr.alloc(&buf, (size_t)6 * (n[0] + n[1] + n[2])));
+  T *axis_buf[3] = {buf, buf + 6 * n[0], buf + 6 * (n[0] + n[1])};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int src_k[6] = {4 * ax + 0, 4 * ax + 1, 12 + 2 * ax, 4 * ax + 2, 4 * ax + 3, 13 + 2 * ax};
+    for (int k = 0; k < 6; ++k)
+      TIDE_HIP_TRY(hipMemcpyAsync(axis_buf[ax] + (size_t)k * n[ax], p.prof[src_k[k]],
+                                  (size_t)n[ax] * sizeof(T), hipMemcpyDefault, stream));
+  }
+  g.pz = axis_buf[0];
+  g.py = axis_buf[1];
+  g.px = axis_buf[2];
+  std::vector<int64_t> hs, hr;
+  TIDE_CHECK(fetch_indices(src, (int64_t)p.n_shots * p.n_src, hs, stream));
+  TIDE_CHECK(fetch_indices(rec, (int64_t)p.n_shots * p.n_rec, hr, stream));
+  PointTable table;
+  if (!build_point_table(hs.data(), p.n_src, hr.data(), p.n_rec, p.n_shots,
+                         (int64_t)g.nz * g.ny, g.nx, kOwn, 1, table))
+    return kInvalidArgument;
+  return dpt.upload(table, stream);
+}
+
+template <typename T>
+void fill_common(Args3<T> &a, const Problem3<T> &p, const DevicePointTable &dpt, int tz) {
+  a.g = p.g;
+  a.pts = Points{dpt.row_slot, dpt.seg, dpt.ent, dpt.n_xt};
+  a.ca = p.ca;
+  a.cb = p.cb;
+  a.cq = p.cq;
+  a.n_shots = p.n_shots;
+  a.n_zb = (p.g.nz + tz - 1) / tz;
+  a.src_comp = p.src_comp;
+  a.rec_comp = p.rec_comp;
+  a.gca = a.gcb = nullptr;
+  a.per_shot_grad = 0;
+  a.use_atomics = 0;
+  a.step_scale = 1.0;
+  for (int k = 0; k < 6; ++k) a.st[k] = nullptr;
+  for (int k = 0; k < N_SLOTS; ++k) a.mn[k] = nullptr;
+}
+
+template <typename T>
+void launch_h3(int tz, dim3 grid, dim3 block, hipStream_t st, const Args3<T> &a) {
+  switch (tz) {
+    case 4: h3_step<T, 4><<<grid, block, 0, st>>>(a); break;
+    case 16: h3_step<T, 16><<<grid, block, 0, st>>>(a); break;
+    default: h3_step<T, 8><<<grid, block, 0, st>>>(a); break;
+  }
+}
+
+template <typename T, int FMT>
+void launch_e3(int tz, dim3 grid, dim3 block, hipStream_t st, const Args3<T> &a) {
+  switch (tz) {
+    case 4: e3_step<T, 4, FMT><<<grid, block, 0, st>>>(a); break;
+    case 16: e3_step<T, 16, FMT><<<grid, block, 0, st>>>(a); break;
+    default: e3_step<T, 8, FMT><<<grid, block, 0, st>>>(a); break;
+  }
+}
+
+template <typename T>
+void launch_adj_e3(int tz, dim3 grid, dim3 block, hipStream_t st, const Args3<T> &a) {
+  switch (tz) {
+    case 4: adj_e3<T, 4><<<grid, block, 0, st>>>(a); break;
+    default: adj_e3<T, 8><<<grid, block, 0, st>>>(a); break;
+  }
+}
+
+template <typename T, int FMT>
+void launch_adj_h3(int tz, dim3 grid, dim3 block, size_t lds, hipStream_t st,
+                   const Args3<T> &a) {
+  switch (tz) {
+    case 4: adj_h3<T, 4, FMT><<<grid, block, lds, st>>>(a); break;
+    default: adj_h3<T, 8, FMT><<<grid, block, lds, st>>>(a); break;
+  }
+}
+
+// Forward over [start_t, start_t + nt).  Snapshot slot of step t is
+// (t - store_t0) / step_ratio (t a multiple of step_ratio).
+template <typename T>
+int forward_impl(Problem3<T> p, T *const *fields, const T *f, T *r, const int64_t *src,
+                 const int64_t *rec, int64_t nt, int64_t start_t, int64_t step_ratio,
+                 void *const *stores, int fmt, size_t store_step_bytes, int64_t store_t0,
+                 hipStream_t stream) {
+  if (nt <= 0) return kOk;
+  if (step_ratio <= 0) {
+    set_error("step_ratio must be positive");
+    return kInvalidArgument;
+  }
+  if (fmt == 2 && sizeof(T) != 4) {
+    set_error("BF16 snapshots require fp32 fields");
+    return kUnsupported;
+  }
+  Scratch scr(stream);
+  DevicePointTable dpt;
+  TIDE_CHECK(prepare(p, src, rec, scr, dpt, stream));
+  const int tz = g_tuning3.fwd_z;
+  const int group = std::min<int>(p.n_shots, g_tuning3.group);
+  Args3<T> a;
+  fill_common(a, p, dpt, tz);
+  for (int k = 0; k < N_SLOTS; ++k) a.f[k] = fields[k];
+  const int64_t n_groups = (p.n_shots + group - 1) / group;
+  const dim3 block(64, (unsigned)group);
+  const dim3 grid((unsigned)((int64_t)dpt.n_xt * p.g.ny * a.n_zb * n_groups));
+  bool storing = false;
+  for (int k = 0; k < 6; ++k) storing = storing || (stores && stores[k]);
+  for (int64_t t = start_t; t < start_t + nt; ++t) {
+    a.src_amp = p.n_src > 0 ? f + t * p.n_shots * p.n_src : nullptr;
+    a.rec_out = p.n_rec > 0 ? r + t * p.n_shots * p.n_rec : nullptr;
+    const bool st = storing && (t % step_ratio) == 0;
+    for (int k = 0; k < 6; ++k)
+      a.st[k] = st && stores[k]
+                    ? (char *)stores[k] + (size_t)((t - store_t0) / step_ratio) * store_step_bytes
+                    : nullptr;
+    launch_h3<T>(tz, grid, block, stream, a);
+    if (!st) {
+      launch_e3<T, 0>(tz, grid, block, stream, a);
+    } else if (fmt == 2) {
+      if constexpr (sizeof(T) == 4) launch_e3<T, 2>(tz, grid, block, stream, a);
+    } else {
+      launch_e3<T, 1>(tz, grid, block, stream, a);
+    }
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  dpt.release(stream);
+  return kOk;
+}
+
+// Adjoint over steps start_t-1 down to start_t-nt.  lam holds the 18
+// adjoint arrays (advanced in place); gca/gcb are FP64 accumulators
+// ([nz,ny,nx] summed over shots, or [n_shots,...] when per shot), added to.
+template <typename T>
+int backward_impl(Problem3<T> p, const T *grad_r, T *const *lam, const void *const *stores,
+                  int fmt, size_t store_step_bytes, int64_t store_t0, T *grad_f, double *gca,
+                  double *gcb, const int64_t *src, const int64_t *rec, int64_t nt,
+                  int64_t start_t, int64_t step_ratio, hipStream_t stream) {
+  if (nt <= 0) return kOk;
+  if (step_ratio <= 0) {
+    set_error("step_ratio must be positive");
+    return kInvalidArgument;
+  }
+  if (fmt == 2 && sizeof(T) != 4) {
+    set_error("BF16 snapshots require fp32 fields");
+    return kUnsupported;
+  }
+  if ((gca && (!stores || !stores[0] || !stores[1] || !stores[2])) ||
+      (gcb && (!stores || !stores[3] || !stores[4] || !stores[5]))) {
+    set_error("coefficient gradients require the stored forward snapshots");
+    return kInvalidArgument;
+  }
+  Scratch scr(stream);
+  DevicePointTable dpt;
+  TIDE_CHECK(prepare(p, src, rec, scr, dpt, stream));
+  const int64_t total = (int64_t)p.n_shots * p.g.numel;
+  const size_t bytes = (size_t)total * sizeof(T);
+  T *cur[N_SLOTS], *nxt[N_SLOTS];
+  for (int k = 0; k < N_SLOTS; ++k) {
+    cur[k] = lam[k];
+    nxt[k] = nullptr;
+  }
+  for (int k = M_HZ_Y; k < N_SLOTS; ++k) {
+    TIDE_CHECK(scr.alloc(&nxt[k], (size_t)total));
+    // Memories are only written inside their strips; both copies start
+    // equal so untouched cells agree whichever buffer ends up current.
+    TIDE_HIP_TRY(hipMemcpyAsync(nxt[k], cur[k], bytes, hipMemcpyDeviceToDevice, stream));
+  }
+  const int tz = g_tuning3.adj_z;
+  const int group = std::min<int>(p.n_shots, g_tuning3.group);
+  Args3<T> a;
+  fill_common(a, p, dpt, tz);
+  a.per_shot_grad = ((gca && p.g.ca_b) || (gcb && p.g.cb_b)) ? 1 : 0;
+  a.step_scale = (double)step_ratio;
+  const int64_t n_groups = (p.n_shots + group - 1) / group;
+  a.use_atomics = n_groups > 1 ? 1 : 0;
+  const dim3 block(64, (unsigned)group);
+  const dim3 grid((unsigned)((int64_t)dpt.n_xt * p.g.ny * a.n_zb * n_groups));
+  const size_t lds = 2 * (size_t)group * tz * 64 * sizeof(double);
+  for (int64_t t = start_t - 1; t >= start_t - nt; --t) {
+    for (int k = 0; k < N_SLOTS; ++k) {
+      a.f[k] = cur[k];
+      a.mn[k] = nxt[k];
+    }
+    launch_adj_e3<T>(tz, grid, block, stream, a);
+    for (int k = M_HZ_Y; k <= M_HX_Y; ++k) std::swap(cur[k], nxt[k]);
+    for (int k = 0; k < N_SLOTS; ++k) {
+      a.f[k] = cur[k];
+      a.mn[k] = nxt[k];
+    }
+    a.src_amp = p.n_rec > 0 ? grad_r + t * p.n_shots * p.n_rec : nullptr;
+    a.rec_out = (p.n_src > 0 && grad_f) ? grad_f + t * p.n_shots * p.n_src : nullptr;
+    const bool g_step = (gca || gcb) && (t % step_ratio) == 0;
+    for (int k = 0; k < 6; ++k)
+      a.st[k] = g_step && stores[k]
+                    ? (void *)((const char *)stores[k] +
+                               (size_t)((t - store_t0) / step_ratio) * store_step_bytes)
+                    : nullptr;
+    a.gca = g_step ? gca : nullptr;
+    a.gcb = g_step ? gcb : nullptr;
+    if (!g_step) {
+      launch_adj_h3<T, 0>(tz, grid, block, lds, stream, a);
+    } else if (fmt == 2) {
+      if constexpr (sizeof(T) == 4) launch_adj_h3<T, 2>(tz, grid, block, lds, stream, a);
+    } else {
+      launch_adj_h3<T, 1>(tz, grid, block, lds, stream, a);
+    }
+    for (int k = M_EY_Z; k < N_SLOTS; ++k) std::swap(cur[k], nxt[k]);
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  for (int k = M_HZ_Y; k < N_SLOTS; ++k)
+    if (cur[k] != lam[k])
+      TIDE_HIP_TRY(hipMemcpyAsync(lam[k], cur[k], bytes, hipMemcpyDeviceToDevice, stream));
+  dpt.release(stream);
+  return kOk;
+}
+
+template <typename T>
+__global__ void fold_grad3(T *__restrict__ out, const double *__restrict__ acc, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (T)((double)out[i] + acc[i]);
+}
+
+template <typename T>
+Problem3<T> make_problem(const T *ca, const T *cb, const T *cq, const T *const *prof, T rdz,
+                         T rdy, T rdx, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+                         int64_t n_src, int64_t n_rec, bool ca_b, bool cb_b, bool cq_b,
+                         const int64_t *pml, int64_t src_comp, int64_t rec_comp) {
+  Problem3<T> p;
+  Geom3<T> &g = p.g;
+  g.pz = g.py = g.px = nullptr;
+  g.rdz = rdz;
+  g.rdy = rdy;
+  g.rdx = rdx;
+  g.nz = (int32_t)nz;
+  g.ny = (int32_t)ny;
+  g.nx = (int32_t)nx;
+  g.numel = (int32_t)(nz * ny * nx);
+  auto h1 = [](int64_t p0, int64_t p1) { return (int32_t)(p1 > p0 ? p1 - 1 : p0); };
+  g.z0 = (int32_t)pml[0];
+  g.y0 = (int32_t)pml[1];
+  g.x0 = (int32_t)pml[2];
+  g.z1 = (int32_t)pml[3];
+  g.y1 = (int32_t)pml[4];
+  g.x1 = (int32_t)pml[5];
+  g.z1h = h1(pml[0], pml[3]);
+  g.y1h = h1(pml[1], pml[4]);
+  g.x1h = h1(pml[2], pml[5]);
+  g.ca_b = ca_b;
+  g.cb_b = cb_b;
+  g.cq_b = cq_b;
+  p.ca = ca;
+  p.cb = cb;
+  p.cq = cq;
+  for (int k = 0; k < 18; ++k) p.prof[k] = prof[k];
+  p.n_shots = (int32_t)n_shots;
+  p.n_src = n_src;
+  p.n_rec = n_rec;
+  p.src_comp = (int32_t)src_comp;
+  p.rec_comp = (int32_t)rec_comp;
+  return p;
+}
+
+}  // namespace em3d
+}  // namespace tide_hip
+
+// ===================================================================== C ABI
+
+using namespace tide_hip;
+using namespace tide_hip::em3d;
+
+namespace {
+
+inline int set_device(int64_t device) {
+  if (hipSetDevice((int)device) != hipSuccess) {
+    set_error("hipSetDevice failed");
+    return kHipError;
+  }
+  return kOk;
+}
+
+template <typename T>
+int em3d_forward(const T *ca, const T *cb, const T *cq, const T *f, T *const *fields, T *r,
+                 const T *const *prof, const int64_t *src, const int64_t *rec, T rdz, T rdy,
+                 T rdx, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+                 int64_t n_src, int64_t n_rec, int64_t step_ratio, bool ca_b, bool cb_b,
+                 bool cq_b, int64_t start_t, const int64_t *pml, int64_t src_comp,
+                 int64_t rec_comp, void *const *stores, int64_t storage_format,
+                 int64_t store_step_bytes, int64_t store_t0, int64_t device, void *stream) {
+  TIDE_CHECK(set_device(device));
+  if (fields == nullptr || prof == nullptr || pml == nullptr) {
+    set_error("fields, profiles and pml bounds are required");
+    return kInvalidArgument;
+  }
+  Problem3<T> p = make_problem<T>(ca, cb, cq, prof, rdz, rdy, rdx, n_shots, nz, ny, nx, n_src,
+                                  n_rec, ca_b, cb_b, cq_b, pml, src_comp, rec_comp);
+  const int fmt = storage_format == 1 ? 2 : 1;
+  return forward_impl<T>(p, fields, f, r, src, rec, nt, start_t, step_ratio, stores, fmt,
+                         (size_t)store_step_bytes, store_t0, (hipStream_t)stream);
+}
+
+template <typename T>
+int em3d_backward(const T *ca, const T *cb, const T *cq, const T *grad_r, T *const *lam,
+                  const void *const *stores, int64_t storage_format, int64_t store_step_bytes,
+                  int64_t store_t0, T *grad_f, double *gca, double *gcb, const T *const *prof,
+                  const int64_t *src, const int64_t *rec, T rdz, T rdy, T rdx, int64_t nt,
+                  int64_t n_shots, int64_t nz, int64_t ny, int64_t nx, int64_t n_src,
+                  int64_t n_rec, int64_t step_ratio, bool ca_b, bool cb_b, bool cq_b,
+                  int64_t start_t, const int64_t *pml, int64_t src_comp, int64_t rec_comp,
+                  int64_t device, void *stream) {
+  TIDE_CHECK(set_device(device));
+  if (lam == nullptr || prof == nullptr || pml == nullptr) {
+    set_error("adjoint fields, profiles and pml bounds are required");
+    return kInvalidArgument;
+  }
+  Problem3<T> p = make_problem<T>(ca, cb, cq, prof, rdz, rdy, rdx, n_shots, nz, ny, nx, n_src,
+                                  n_rec, ca_b, cb_b, cq_b, pml, src_comp, rec_comp);
+  const int fmt = storage_format == 1 ? 2 : 1;
+  return backward_impl<T>(p, grad_r, lam, stores, fmt, (size_t)store_step_bytes, store_t0,
+                          grad_f, gca, gcb, src, rec, nt, start_t, step_ratio,
+                          (hipStream_t)stream);
+}
+
+}  // namespace
+
+#define TIDE_EM3D_ENTRY_POINTS(T, NAME)                                                     \
+  extern "C" int tide_em3d_##NAME##_forward(                                               \
+      T const *ca, T const *cb, T const *cq, T const *f, T *const *fields, T *r,           \
+      T const *const *prof, int64_t const *sources_i, int64_t const *receivers_i, T rdz,   \
+      T rdy, T rdx, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,       \
+      int64_t n_src, int64_t n_rec, int64_t step_ratio, bool ca_b, bool cb_b, bool cq_b,   \
+      int64_t start_t, int64_t const *pml, int64_t source_component,                      \
+      int64_t receiver_component, void *const *stores, int64_t storage_format,             \
+      int64_t store_step_bytes, int64_t store_t0, int64_t device, void *compute_stream) {  \
+    return em3d_forward<T>(ca, cb, cq, f, fields, r, prof, sources_i, receivers_i, rdz,    \
+                           rdy, rdx, nt, n_shots, nz, ny, nx, n_src, n_rec, step_ratio,    \
+                           ca_b, cb_b, cq_b, start_t, pml, source_component,               \
+                           receiver_component, stores, storage_format, store_step_bytes,   \
+                           store_t0, device, compute_stream);                              \
+  }                                                                                        \
+  extern "C" int tide_em3d_##NAME##_backward_f64acc(                                       \
+      T const *ca, T const *cb, T const *cq, T const *grad_r, T *const *lam,               \
+      void const *const *stores, int64_t storage_format, int64_t store_step_bytes,         \
+      int64_t store_t0, T *grad_f, double *grad_ca, double *grad_cb, T const *const *prof, \
+      int64_t const *sources_i, int64_t const *receivers_i, T rdz, T rdy, T rdx,           \
+      int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx, int64_t n_src,      \
+      int64_t n_rec, int64_t step_ratio, bool ca_b, bool cb_b, bool cq_b, int64_t start_t, \
+      int64_t const *pml, int64_t source_component, int64_t receiver_component,            \
+      int64_t device, void *compute_stream) {                                              \
+    return em3d_backward<T>(ca, cb, cq, grad_r, lam, stores, storage_format,               \
+                            store_step_bytes, store_t0, grad_f, grad_ca, grad_cb, prof,    \
+                            sources_i, receivers_i, rdz, rdy, rdx, nt, n_shots, nz, ny,    \
+                            nx, n_src, n_rec, step_ratio, ca_b, cb_b, cq_b, start_t, pml,  \
+                            source_component, receiver_component, device,                  \
+                            compute_stream);                                               \
+  }
+
+TIDE_EM3D_ENTRY_POINTS(float, float)
+TIDE_EM3D_ENTRY_POINTS(double, double)
+
+extern "C" int tide_em3d_set_tuning(int fwd_z, int adj_z, int group) {
+  const bool ok = (fwd_z == 4 || fwd_z == 8 || fwd_z == 16) && (adj_z == 4 || adj_z == 8) &&
+                  group >= 1 && group <= kMaxGroup;
+  if (!ok) {
+    set_error("tuning: fwd_z in {4,8,16}, adj_z in {4,8}, group 1..8");
+    return kInvalidArgument;
+  }
+  g_tuning3 = Tuning3{fwd_z, adj_z, group};
+  return kOk;
+}

@@ -148,6 +148,46 @@ def _tm_backward_f64acc_types(f: Any) -> list[Any]:
     ]
 
 
+def _em3d_forward_types(f: Any) -> list[Any]:
+    return [
+        *([_P] * 4),  # ca, cb, cq, f
+        _P,  # fields: void*[18]
+        _P,  # r
+        _P,  # profiles: void*[18]
+        _P, _P,  # sources_i, receivers_i
+        f, f, f,  # rdz, rdy, rdx
+        *([_I] * 8),  # nt, n_shots, nz, ny, nx, n_src, n_rec, step_ratio
+        *([_B] * 3),  # ca/cb/cq batched
+        _I,  # start_t
+        _P,  # pml: int64[6]
+        _I, _I,  # source_component, receiver_component
+        _P,  # stores: void*[6]
+        _I, _I, _I,  # storage_format, store_step_bytes, store_t0
+        _I,  # device
+        _P,  # compute_stream
+    ]
+
+
+def _em3d_backward_types(f: Any) -> list[Any]:
+    return [
+        *([_P] * 4),  # ca, cb, cq, grad_r
+        _P,  # lambda state: void*[18]
+        _P,  # stores: void*[6]
+        _I, _I, _I,  # storage_format, store_step_bytes, store_t0
+        _P, _P, _P,  # grad_f, grad_ca (fp64), grad_cb (fp64)
+        _P,  # profiles
+        _P, _P,  # sources_i, receivers_i
+        f, f, f,
+        *([_I] * 8),
+        *([_B] * 3),
+        _I,  # start_t
+        _P,  # pml
+        _I, _I,  # components
+        _I,  # device
+        _P,  # compute_stream
+    ]
+
+
 def load() -> ctypes.CDLL:
     """Load the HIP library, raising if it has not been built."""
     global _lib
@@ -174,6 +214,15 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, f"tide_tm2d_{name}_backward_f64acc")
         fn.restype = ctypes.c_int
         fn.argtypes = _tm_backward_f64acc_types(ctype)
+    for name, ctype in (("float", ctypes.c_float), ("double", ctypes.c_double)):
+        fn = getattr(lib, f"tide_em3d_{name}_forward")
+        fn.restype = ctypes.c_int
+        fn.argtypes = _em3d_forward_types(ctype)
+        fn = getattr(lib, f"tide_em3d_{name}_backward_f64acc")
+        fn.restype = ctypes.c_int
+        fn.argtypes = _em3d_backward_types(ctype)
+    lib.tide_em3d_set_tuning.restype = ctypes.c_int
+    lib.tide_em3d_set_tuning.argtypes = [ctypes.c_int] * 3
     lib.tide_tm2d_set_tuning.restype = ctypes.c_int
     lib.tide_tm2d_set_tuning.argtypes = [ctypes.c_int] * 4
     lib.tide_tm2d_get_tuning.restype = ctypes.c_int
@@ -222,6 +271,21 @@ def set_tm2d_tuning(fwd_rows: int, fwd_group: int, adj_rows: int, adj_group: int
     check(load().tide_tm2d_set_tuning(fwd_rows, fwd_group, adj_rows, adj_group))
 
 
+def set_em3d_tuning(fwd_z: int, adj_z: int, group: int) -> None:
+    """z planes marched per wavefront (forward, adjoint) and shots per
+    workgroup of the 3-D kernels (see include/tide_hip.h)."""
+    check(load().tide_em3d_set_tuning(fwd_z, adj_z, group))
+
+
+def ptr_array(tensors) -> ctypes.Array:
+    """``void*[n]`` of tensor pointers (None for absent entries)."""
+    return (ctypes.c_void_p * len(tensors))(*[ptr(t) for t in tensors])
+
+
+def int64_array(values) -> ctypes.Array:
+    return (ctypes.c_int64 * len(values))(*[int(v) for v in values])
+
+
 def get_tm2d_tuning() -> tuple[int, int, int, int]:
     out = (ctypes.c_int * 4)()
     check(load().tide_tm2d_get_tuning(out))
@@ -238,6 +302,9 @@ __all__ = [
     "check",
     "dtype_name",
     "get_tm2d_tuning",
+    "int64_array",
+    "ptr_array",
+    "set_em3d_tuning",
     "set_tm2d_tuning",
     "get_function",
     "library_path",
