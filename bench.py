@@ -32,6 +32,10 @@ for _p in (ROOT / "tide-gpr_amd", ROOT):
         sys.path.insert(0, str(_p))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per cell-update measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled by a 1 GiB copy's counters) for the kernels as built; see
+# profiles/tm2d_config2_hbm_counters.md.  Reported as roofline.traffic.
+MEASURED_TRAFFIC_B_PER_CELL = {"fwd_step": 37.25, "adj_step": 41.58}
 
 
 def log(msg: str) -> None:
@@ -213,13 +217,19 @@ def main() -> None:
         }
     dom = max(summ, key=lambda k: summ[k]["ms"])
     dom_k = kernels[dom]
+    shots_per_launch = n_shots / n_chunks
+    measured = MEASURED_TRAFFIC_B_PER_CELL.get(dom_k["kernel"]) if snap_bytes == 4 else None
+    traffic = (None if measured is None
+               else round(measured * ny_p * nx_p * shots_per_launch))
     roofline = {
         "bound": "hbm",
         "achieved": dom_k["achieved_GBs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(dom_k["achieved_GBs"] / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
+        "algorithmic_bytes_per_launch": round(b_alg[dom] * ny_p * nx_p * shots_per_launch),
         "kernel": dom_k["kernel"],
         "kernels": kernels,
     }
