@@ -55,6 +55,8 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--compression", default="none", choices=["none", "bf16"])
     ap.add_argument("--chunk", type=int, default=0, help="shots per chunk (0: fit HBM)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shapes")
     ap.add_argument("--cpu-sample-nt", type=int, default=3000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device copy (profiler byte-counter calibration)")
@@ -148,6 +150,15 @@ def main() -> None:
     base, extra = divmod(n_shots, n_chunks)
     sizes = [base + (1 if i < extra else 0) for i in range(n_chunks)]
     bounds = [sum(sizes[:i]) for i in range(n_chunks + 1)]
+
+    tuning = None
+    if not args.no_autotune:
+        from tide.autotune import tune_tm2d
+        tuning = tune_tm2d(ny_p, nx_p, max(sizes), device=dev)
+        log(f"autotuned launch shapes {tuning['tuning']} in {tuning.get('seconds', 0)} s")
+    else:
+        from tide import _native
+        tuning = {"tuning": _native.get_tm2d_tuning(), "cached": False}
 
     disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
     storage = tide.StorageOptions(compression=args.compression == "bf16")
@@ -260,6 +271,8 @@ def main() -> None:
             "cell_updates_per_step": updates_per_step,
             "hbm_frac_whole_job": round(gcells * 1e9 * (24 + b_snap) / 1e9 / HBM_PEAK_GBS, 4),
             "misfit": float(misfit),
+            "launch_shape": {"fwd_rows_x_shots": list(tuning["tuning"][:2]),
+                             "adj_rows_x_shots": list(tuning["tuning"][2:])},
             "seed": 1234,
         },
         "roofline": roofline,

@@ -212,13 +212,55 @@ int tide_tm2d_double_backward_f64acc(
 
 /*
  * Launch shape of the TM2D step kernels: rows marched per wavefront
- * (forward 4/8/16, adjoint 2/4/8; halved for fp64) and shots per workgroup
+ * (forward 2/4/8/16, adjoint 2/4/8; halved for fp64) and shots per workgroup
  * (1..8), process-wide.  Set by the autotuner.  No reference counterpart
  * (the reference fixes a 32x8 block at build time,
  * src/tide/csrc/CMakeLists.txt:24-25).
  */
 int tide_tm2d_set_tuning(int fwd_rows, int fwd_group, int adj_rows, int adj_group);
 int tide_tm2d_get_tuning(int *out4);
+
+/* ------------------------------------------------------- batched 2-D plans
+ * Many independent fp32 TM2D problems (different grid sizes, shot counts
+ * and nt) advanced together: one launch per time step covers every problem
+ * still running, and the forward and adjoint time loops are captured once
+ * into HIP graphs and replayed by each run.  All pointers are device
+ * pointers owned by the caller and must stay valid for the plan's life
+ * (sources_i / receivers_i and the profiles are read once, at creation);
+ * the kernels read ca/cb/cq/f/grad_r through them at every run, so a
+ * caller updates models or sources in place between runs.  The forward
+ * starts from `fields` (the caller's initial state, normally zero); the
+ * backward zeroes `lambda`, grad_ca/grad_cb (FP64, [ny, nx], summed over
+ * shots) and grad_f itself.  No reference counterpart (the reference runs
+ * one problem per call, maxwell_tm_cuda_inst.cu:1371-1609, 2405-2734).
+ */
+typedef struct tide_tm2d_problem {
+  float const *ca, *cb, *cq;          /* [ny, nx] */
+  float const *f;                     /* [nt, n_shots, n_src] */
+  int64_t const *sources_i;           /* [n_shots, n_src] (host or device) */
+  int64_t const *receivers_i;         /* [n_shots, n_rec] */
+  float const *prof[12];              /* ay by ayh byh ax bx axh bxh ky kyh kx kxh */
+  float rdy, rdx;
+  int64_t nt, n_shots, ny, nx, n_src, n_rec;
+  int64_t interval;                   /* snapshot / gradient sampling (step_ratio) */
+  int64_t pml[4];                     /* pml_y0, pml_x0, pml_y1, pml_x1 */
+  float *fields[7];                   /* ey hx hz m_ey_x m_ey_z m_hx_z m_hz_x */
+  float *r;                           /* [nt, n_shots, n_rec] */
+  void *ey_store, *curl_store;        /* [ceil(nt/interval), n_shots, ny, nx] or NULL */
+  float const *grad_r;                /* [nt, n_shots, n_rec] */
+  float *lambda[7];                   /* adjoint state, same order as fields; NULL = forward only */
+  double *grad_ca, *grad_cb;          /* [ny, nx] FP64, or NULL */
+  float *grad_f;                      /* [nt, n_shots, n_src] or NULL */
+} tide_tm2d_problem;
+
+/* storage_format 0 = fp32 snapshots, 1 = BF16; use_graph 0 launches each
+ * run eagerly (same kernels).  Launch shape from tide_tm2d_set_tuning. */
+int tide_tm2d_plan_create(const tide_tm2d_problem *problems, int64_t n,
+                          int64_t storage_format, int64_t use_graph, int64_t device,
+                          void **plan);
+int tide_tm2d_plan_forward(void *plan, void *stream);
+int tide_tm2d_plan_backward(void *plan, void *stream);
+int tide_tm2d_plan_destroy(void *plan);
 
 /* ------------------------------------------------------------------- 3-D
  * Fields are [n_shots, nz, ny, nx] (x fastest).  `fields` / `lambda` point

@@ -214,3 +214,14 @@ def test_cpu_tensors_are_rejected_not_emulated():
     """The operator has one compute path (the HIP library); CPU tensors raise."""
     with pytest.raises(NotImplementedError, match="HIP"):
         maxwelltm(**hetero_problem(nt=4, shots=1))
+
+
+def test_autotune_size_classes():
+    from tide.autotune import ADJ_CANDIDATES, FWD_CANDIDATES, _size_class
+    a = _size_class(1065, 1065, 7, torch.float32)
+    assert a == _size_class(1100, 1050, 6, torch.float32)  # similar size, same shot bucket
+    assert a != _size_class(1065, 1065, 16, torch.float32)
+    assert a != _size_class(300, 300, 7, torch.float32)
+    assert a != _size_class(1065, 1065, 7, torch.float64)
+    assert all(r in (2, 4, 8, 16) and 1 <= g <= 8 for r, g in FWD_CANDIDATES)
+    assert all(r in (2, 4, 8) and 1 <= g <= 8 for r, g in ADJ_CANDIDATES)

@@ -287,3 +287,30 @@ def test_repeat_runs_are_bitwise_identical():
     g2 = gpu_backward(case, grad_r, es1, cs1)
     for a, b in zip(g1[:3], g2[:3]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("tuning", [(2, 8, 2, 8), (4, 4, 2, 4), (8, 8, 4, 8), (16, 2, 8, 2), (4, 1, 4, 3)])
+def test_every_launch_shape_matches_oracle(tuning):
+    """Any launch shape the autotuner may pick gives the oracle's forward
+    bitwise and its adjoint fields bitwise (gradients to FP64 rounding)."""
+    case, _ = load_golden("tm2d_hetero_f32")
+    print(f"tuning {tuning}", flush=True)
+    before = tide._native.get_tm2d_tuning()
+    tide._native.set_tm2d_tuning(*tuning)
+    try:
+        f_o, r_o, es, cs = O.tm2d_forward(case, store="full")
+        f_g, r_g, _, _ = gpu_forward(case, store="full")
+        assert np.array_equal(r_g, r_o)
+        for a, b in zip(f_g, f_o):
+            assert np.array_equal(a, b)
+        rng = np.random.default_rng(3)
+        w = rng.standard_normal(r_o.shape).astype(np.float32)
+        go = O.tm2d_backward(case, w, es, cs)
+        gg = gpu_backward(case, w, es, cs)
+        for a, b in zip(gg[3], go[3]):
+            assert np.array_equal(a, b)
+        assert np.array_equal(gg[2], go[2])
+        assert relative_l2(gg[0], go[0]) < 1e-12
+        assert relative_l2(gg[1], go[1]) < 1e-12
+    finally:
+        tide._native.set_tm2d_tuning(*before)

@@ -75,21 +75,24 @@ template <typename T> struct Args3 {
   int32_t n_shots, n_zb, src_comp, rec_comp;
 };
 
+// Snapshots are written once and read once, much later: non-temporal stores
+// keep them from evicting the fields.
 template <typename T, int FMT>
 __device__ __forceinline__ void put_store(void *base, int64_t i, T v) {
   if constexpr (FMT == 2) {
-    ((uint16_t *)base)[i] = f32_to_bf16((float)v);
+    __builtin_nontemporal_store(f32_to_bf16((float)v), (uint16_t *)base + i);
   } else {
-    ((T *)base)[i] = v;
+    __builtin_nontemporal_store(v, (T *)base + i);
   }
 }
 
+// The adjoint reads each snapshot once: non-temporal loads.
 template <typename T, int FMT>
 __device__ __forceinline__ double get_store(const void *base, int64_t i) {
   if constexpr (FMT == 2) {
-    return (double)bf16_to_f32(((const uint16_t *)base)[i]);
+    return (double)bf16_to_f32(__builtin_nontemporal_load((const uint16_t *)base + i));
   } else {
-    return (double)((const T *)base)[i];
+    return (double)__builtin_nontemporal_load((const T *)base + i);
   }
 }
 
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(512) void h3_step(Args3<T> a) {
     if (z >= nz) break;  // uniform
     const T exn = L(ex, z + 1, y), eyn = L(ey, z + 1, y), ezn = L(ez, z + 1, y);
     const T ex_yp = L(ex, z, y + 1), ez_yp = L(ez, z, y + 1);
-    const T ez_xp = __shfl_down(ezc, 1, 64), ey_xp = __shfl_down(eyc, 1, 64);
+    const T ez_xp = shfl_dn1(ezc), ey_xp = shfl_dn1(eyc);
     if (own && z >= 1) {
       const int j = (z * ny + y) * nx + x;
       const T cqv = cq[j];
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
     if (z >= nz) break;
     const T hxc = L(hx, z, y), hyc = L(hy, z, y), hzc = L(hz, z, y);
     const T hz_ym = L(hz, z, y - 1), hx_ym = L(hx, z, y - 1);
-    const T hz_xm = __shfl_up(hzc, 1, 64), hy_xm = __shfl_up(hyc, 1, 64);
+    const T hz_xm = shfl_up1(hzc), hy_xm = shfl_up1(hyc);
     if (own && z >= 1) {
       const int j = (z * ny + y) * nx + x;
       const bool pz = z < g.z0 || z >= g.z1;
@@ -404,8 +407,8 @@ __global__ __launch_bounds__(512) void adj_e3(Args3<T> a) {
       r_zx = adj_raw(q_zx, px ? m_hz_x[j] : (T)0, ax, bx, kx, px);
       r_yx = adj_raw(q_yx, px ? m_hy_x[j] : (T)0, ax, bx, kx, px);
     }
-    const T r_zx_xp = __shfl_down(r_zx, 1, 64);
-    const T r_yx_xp = __shfl_down(r_yx, 1, 64);
+    const T r_zx_xp = shfl_dn1(r_zx);
+    const T r_yx_xp = shfl_dn1(r_yx);
     if (own && zdom) {
       // y-derivative raws at y and y + 1: (dHz/dy from Ex, -) and (dHx/dy from Ez, +)
       const T q_zy = -(cbv * l0);
@@ -550,8 +553,8 @@ __global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
         r_zx = adj_raw(adj_zx, pxh ? m_ez_x[j] : (T)0, ax, bx, kx, pxh);
         r_yx = adj_raw(adj_yx, pxh ? m_ey_x[j] : (T)0, ax, bx, kx, pxh);
       }
-      const T r_zx_xm = __shfl_up(r_zx, 1, 64);
-      const T r_yx_xm = __shfl_up(r_yx, 1, 64);
+      const T r_zx_xm = shfl_up1(r_zx);
+      const T r_yx_xm = shfl_up1(r_yx);
       if (own && zdom) {
         // y-derivative raws at y and y - 1: Hx term dEz/dy (second, +cq lHx)
         // and Hz term dEx/dy (first, -cq lHz); defined for y < ny - 1

@@ -83,6 +83,34 @@ struct DevicePointTable {
   void release(hipStream_t stream);
 };
 
+// ----------------------------------------------------------- lane moves
+// Cross-lane moves by one lane through full-wave DPP shifts (a VALU move, no
+// LDS round trip): shfl_dn1 gives lane i the value of lane i+1, shfl_up1 that
+// of lane i-1; the value at the wave edge (lane 63 resp. 0) is unspecified
+// and never used (those lanes are halo lanes).
+__device__ __forceinline__ int dpp_dn1_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+}
+__device__ __forceinline__ int dpp_up1_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+}
+__device__ __forceinline__ float shfl_dn1(float v) {
+  return __builtin_bit_cast(float, dpp_dn1_i(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float shfl_up1(float v) {
+  return __builtin_bit_cast(float, dpp_up1_i(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double shfl_dn1(double v) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const int lo = dpp_dn1_i((int)u), hi = dpp_dn1_i((int)(u >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double shfl_up1(double v) {
+  const long long u = __builtin_bit_cast(long long, v);
+  const int lo = dpp_up1_i((int)u), hi = dpp_up1_i((int)(u >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
 // ----------------------------------------------------------- tile mapping
 // Maps a flat workgroup id onto (tile, group) so that the 8 XCDs of the
 // MI355X each receive a contiguous band of tiles.  Workgroups are dispatched

@@ -1,0 +1,580 @@
+// Batched 2-D TM plans: many independent problems (different grid sizes,
+// shot counts and nt) advanced together, one launch per time step.
+//
+// Every problem keeps its own arrays; a plan holds a device table with one
+// descriptor per problem and a tile -> problem map.  Problems are sorted by
+// nt (longest first), so the tiles still running at step k form a prefix
+// of the tile list: step k launches exactly that prefix.  Forward step k
+// advances every problem with nt > k by its step k; adjoint iteration k
+// applies every such problem's step nt-1-k.  A workgroup is a tile of one
+// problem, so the tile kernels are the single-problem ones (tm2d_device.h)
+// fed from the descriptor.
+//
+// The time loops are captured once into HIP graphs when the plan is built
+// and replayed by every run: the per-run resets are fill kernels and
+// device copies (no memset nodes), the per-step kernels read the
+// problems' inputs through fixed pointers, so a caller updates
+// coefficients or sources in place between runs.  With graphs off, the same
+// sequence is launched eagerly.
+
+#include "tm2d_device.h"
+#include "tide_hip.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace tide_hip {
+namespace tm2d {
+
+template <typename T> struct BatchItem {
+  Geom<T> g;
+  Points pts;
+  const T *ca, *cb, *cq;
+  T *fwd[2][5];  // [parity]: ey, hx, hz, m_ey_x, m_ey_z (parity 0: caller's arrays)
+  T *mhzx, *mhxz;
+  const T *f;
+  T *r;
+  void *ey_store, *curl_store;
+  int64_t store_step_bytes;
+  T *lam[2][7];  // [parity]: ley, lhx, lhz, m_hz_x, m_hx_z, m_ey_x, m_ey_z
+  const T *grad_r;
+  T *grad_f;
+  double *gca, *gcb;
+  int32_t n_shots, n_src, n_rec, n_yt, nt, interval, use_atomics, tile0;
+};
+
+__device__ __forceinline__ TileCoord batch_tile(int local, int n_xt, int n_yt) {
+  const int n_tiles = n_xt * n_yt;
+  TileCoord tc;
+  tc.grp = local / n_tiles;
+  const int tile = local - tc.grp * n_tiles;
+  tc.yt = tile / n_xt;
+  tc.xt = tile - tc.yt * n_xt;
+  return tc;
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(512) void fwd_batch(const BatchItem<T> *__restrict__ items,
+                                                 const int32_t *__restrict__ tile_item, int k) {
+  const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  const BatchItem<T> &it = items[tile_item[flat]];
+  const TileCoord tc = batch_tile((int)(flat - it.tile0), it.pts.n_xt, it.n_yt);
+  const int p = k & 1;
+  FwdArgs<T> a;
+  a.g = it.g;
+  a.pts = it.pts;
+  a.ca = it.ca;
+  a.cb = it.cb;
+  a.cq = it.cq;
+  a.ey_i = it.fwd[p][0];
+  a.hx_i = it.fwd[p][1];
+  a.hz_i = it.fwd[p][2];
+  a.mex_i = it.fwd[p][3];
+  a.mez_i = it.fwd[p][4];
+  a.ey_o = it.fwd[p ^ 1][0];
+  a.hx_o = it.fwd[p ^ 1][1];
+  a.hz_o = it.fwd[p ^ 1][2];
+  a.mex_o = it.fwd[p ^ 1][3];
+  a.mez_o = it.fwd[p ^ 1][4];
+  a.mhzx = it.mhzx;
+  a.mhxz = it.mhxz;
+  a.f_t = it.n_src > 0 ? it.f + (int64_t)k * it.n_shots * it.n_src : nullptr;
+  a.r_t = it.n_rec > 0 ? it.r + (int64_t)k * it.n_shots * it.n_rec : nullptr;
+  const bool st = (k % it.interval) == 0;
+  const int64_t off = (int64_t)(k / it.interval) * it.store_step_bytes;
+  a.ey_store = st && it.ey_store ? (char *)it.ey_store + off : nullptr;
+  a.curl_store = st && it.curl_store ? (char *)it.curl_store + off : nullptr;
+  a.n_shots = it.n_shots;
+  a.n_yt = it.n_yt;
+  fwd_workgroup<T, TY, FMT>(a, tc);
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(512) void adj_batch(const BatchItem<T> *__restrict__ items,
+                                                 const int32_t *__restrict__ tile_item, int k) {
+  extern __shared__ double red[];
+  const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  const BatchItem<T> &it = items[tile_item[flat]];
+  const TileCoord tc = batch_tile((int)(flat - it.tile0), it.pts.n_xt, it.n_yt);
+  const int p = k & 1;
+  const int t = it.nt - 1 - k;
+  AdjArgs<T> a;
+  a.g = it.g;
+  a.pts = it.pts;
+  a.ca = it.ca;
+  a.cb = it.cb;
+  a.cq = it.cq;
+  a.ley_i = it.lam[p][0];
+  a.lhx_i = it.lam[p][1];
+  a.lhz_i = it.lam[p][2];
+  a.mx_i = it.lam[p][3];
+  a.my_i = it.lam[p][4];
+  a.nx_i = it.lam[p][5];
+  a.nz_i = it.lam[p][6];
+  a.ley_o = it.lam[p ^ 1][0];
+  a.lhx_o = it.lam[p ^ 1][1];
+  a.lhz_o = it.lam[p ^ 1][2];
+  a.mx_o = it.lam[p ^ 1][3];
+  a.my_o = it.lam[p ^ 1][4];
+  a.nx_o = it.lam[p ^ 1][5];
+  a.nz_o = it.lam[p ^ 1][6];
+  const bool g_step = (it.gca != nullptr || it.gcb != nullptr) && (t % it.interval) == 0;
+  const int64_t off = (int64_t)(t / it.interval) * it.store_step_bytes;
+  a.ey_store = g_step && it.gca ? (const char *)it.ey_store + off : nullptr;
+  a.curl_store = g_step && it.gcb ? (const char *)it.curl_store + off : nullptr;
+  a.gca = g_step ? it.gca : nullptr;
+  a.gcb = g_step ? it.gcb : nullptr;
+  a.per_shot_grad = 0;
+  a.step_scale = (double)it.interval;
+  a.grad_r_t = it.n_rec > 0 ? it.grad_r + (int64_t)t * it.n_shots * it.n_rec : nullptr;
+  a.grad_f_t = (it.n_src > 0 && it.grad_f) ? it.grad_f + (int64_t)t * it.n_shots * it.n_src
+                                           : nullptr;
+  a.n_shots = it.n_shots;
+  a.n_yt = it.n_yt;
+  a.use_atomics = it.use_atomics;
+  adj_workgroup<T, TY, FMT>(a, tc, red);
+}
+
+template <typename T>
+__global__ void fill_zero(T *__restrict__ p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (T)0;
+}
+
+// ------------------------------------------------------------------ plan
+
+struct Plan {
+  int device = 0;
+  int64_t n = 0;
+  int fmt_fwd = 0, fmt_adj = 0;  // 0 none, 1 T payload, 2 BF16
+  int rows_f = 8, rows_a = 4, group = 8;
+  int max_nt = 0;
+  std::vector<tide_tm2d_problem> probs;  // sorted by nt, descending
+  std::vector<void *> allocs;            // plan-owned device memory
+  BatchItem<float> *d_fwd = nullptr, *d_adj = nullptr;
+  int32_t *d_tiles_fwd = nullptr, *d_tiles_adj = nullptr;
+  std::vector<int64_t> active_fwd, active_adj;  // tiles running at step k
+  std::vector<BatchItem<float>> h_fwd, h_adj;
+  std::vector<DevicePointTable> dpts;
+  hipStream_t capture = nullptr;
+  hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;
+  hipGraph_t gr_fwd = nullptr, gr_adj = nullptr;
+  bool use_graph = false;
+};
+
+namespace {
+
+template <typename U> int plan_alloc(Plan &p, U **out, size_t count) {
+  void *ptr = nullptr;
+  TIDE_HIP_TRY(hipMalloc(&ptr, std::max<size_t>(count, 1) * sizeof(U)));
+  p.allocs.push_back(ptr);
+  *out = (U *)ptr;
+  return kOk;
+}
+
+int fetch_host(const int64_t *p, int64_t n, std::vector<int64_t> &out) {
+  out.assign((size_t)std::max<int64_t>(n, 0), 0);
+  if (n <= 0 || p == nullptr) return kOk;
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  if (hipPointerGetAttributes(&attr, p) == hipSuccess && attr.type == hipMemoryTypeDevice) {
+    TIDE_HIP_TRY(hipMemcpy(out.data(), p, (size_t)n * sizeof(int64_t), hipMemcpyDeviceToHost));
+  } else {
+    (void)hipGetLastError();
+    std::memcpy(out.data(), p, (size_t)n * sizeof(int64_t));
+  }
+  return kOk;
+}
+
+#define PLAN_CHECK(expr)                    \
+  do {                                      \
+    const int _rc = (expr);                 \
+    if (_rc != ::tide_hip::kOk) return _rc; \
+  } while (0)
+
+template <typename T>
+void launch_zero(T *p, int64_t n, hipStream_t s) {
+  if (p == nullptr || n <= 0) return;
+  const int threads = 256;
+  const int64_t blocks = std::min<int64_t>((n + threads - 1) / threads, 4096);
+  fill_zero<T><<<(unsigned)blocks, threads, 0, s>>>(p, n);
+}
+
+template <int FMT>
+void launch_fwd_b(int rows, int64_t grid, int group, hipStream_t s, const BatchItem<float> *d,
+                  const int32_t *tiles, int k) {
+  const dim3 block(64, (unsigned)group);
+  switch (rows) {
+    case 2: fwd_batch<float, 2, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k); break;
+    case 4: fwd_batch<float, 4, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k); break;
+    case 16: fwd_batch<float, 16, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k); break;
+    default: fwd_batch<float, 8, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k); break;
+  }
+}
+
+template <int FMT>
+void launch_adj_b(int rows, int64_t grid, int group, hipStream_t s, const BatchItem<float> *d,
+                  const int32_t *tiles, int k) {
+  const dim3 block(64, (unsigned)group);
+  const size_t lds = 2 * (size_t)group * rows * 64 * sizeof(double);
+  switch (rows) {
+    case 2: adj_batch<float, 2, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k); break;
+    case 8: adj_batch<float, 8, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k); break;
+    default: adj_batch<float, 4, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k); break;
+  }
+}
+
+// Enqueues one whole forward run on `s`.
+int enqueue_forward(Plan &p, hipStream_t s) {
+  const size_t fb = sizeof(float);
+  for (size_t m = 0; m < p.probs.size(); ++m) {
+    const tide_tm2d_problem &q = p.probs[m];
+    const BatchItem<float> &it = p.h_fwd[m];
+    const int64_t total = q.n_shots * q.ny * q.nx;
+    // the second copy of the H-phase memories starts equal to the first
+    TIDE_HIP_TRY(hipMemcpyAsync(it.fwd[1][3], it.fwd[0][3], total * fb, hipMemcpyDeviceToDevice, s));
+    TIDE_HIP_TRY(hipMemcpyAsync(it.fwd[1][4], it.fwd[0][4], total * fb, hipMemcpyDeviceToDevice, s));
+    if (q.n_rec > 0) launch_zero(q.r, q.nt * q.n_shots * q.n_rec, s);
+  }
+  for (int k = 0; k < p.max_nt; ++k) {
+    const int64_t grid = p.active_fwd[(size_t)k];
+    if (grid <= 0) break;
+    if (p.fmt_fwd == 0) launch_fwd_b<0>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k);
+    else if (p.fmt_fwd == 2) launch_fwd_b<2>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k);
+    else launch_fwd_b<1>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k);
+  }
+  for (size_t m = 0; m < p.probs.size(); ++m) {
+    const tide_tm2d_problem &q = p.probs[m];
+    if ((q.nt & 1) == 0) continue;
+    const BatchItem<float> &it = p.h_fwd[m];
+    const int64_t total = q.n_shots * q.ny * q.nx;
+    for (int c = 0; c < 5; ++c)
+      TIDE_HIP_TRY(hipMemcpyAsync(it.fwd[0][c], it.fwd[1][c], total * fb, hipMemcpyDeviceToDevice, s));
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+// Enqueues one whole adjoint run on `s` (adjoint state and gradients start at zero).
+int enqueue_backward(Plan &p, hipStream_t s) {
+  const size_t fb = sizeof(float);
+  for (size_t m = 0; m < p.probs.size(); ++m) {
+    const tide_tm2d_problem &q = p.probs[m];
+    const BatchItem<float> &it = p.h_adj[m];
+    const int64_t total = q.n_shots * q.ny * q.nx;
+    for (int par = 0; par < 2; ++par)
+      for (int c = 0; c < 7; ++c) launch_zero(it.lam[par][c], total, s);
+    launch_zero(q.grad_ca, q.ny * q.nx, s);
+    launch_zero(q.grad_cb, q.ny * q.nx, s);
+    if (q.grad_f && q.n_src > 0) launch_zero(q.grad_f, q.nt * q.n_shots * q.n_src, s);
+  }
+  for (int k = 0; k < p.max_nt; ++k) {
+    const int64_t grid = p.active_adj[(size_t)k];
+    if (grid <= 0) break;
+    if (p.fmt_adj == 0) launch_adj_b<0>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
+    else if (p.fmt_adj == 2) launch_adj_b<2>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
+    else launch_adj_b<1>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
+  }
+  for (size_t m = 0; m < p.probs.size(); ++m) {
+    const tide_tm2d_problem &q = p.probs[m];
+    if ((q.nt & 1) == 0) continue;
+    const BatchItem<float> &it = p.h_adj[m];
+    const int64_t total = q.n_shots * q.ny * q.nx;
+    for (int c = 0; c < 7; ++c)
+      TIDE_HIP_TRY(hipMemcpyAsync(it.lam[0][c], it.lam[1][c], total * fb, hipMemcpyDeviceToDevice, s));
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+int capture(Plan &p, bool forward, hipGraph_t *graph, hipGraphExec_t *exec) {
+  TIDE_HIP_TRY(hipStreamBeginCapture(p.capture, hipStreamCaptureModeRelaxed));
+  const int rc = forward ? enqueue_forward(p, p.capture) : enqueue_backward(p, p.capture);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p.capture, &g);
+  if (rc != kOk) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  TIDE_HIP_TRY(e);
+  *graph = g;
+  TIDE_HIP_TRY(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return kOk;
+}
+
+void destroy(Plan *p) {
+  if (p == nullptr) return;
+  (void)hipSetDevice(p->device);
+  (void)hipDeviceSynchronize();
+  if (p->g_fwd) (void)hipGraphExecDestroy(p->g_fwd);
+  if (p->g_adj) (void)hipGraphExecDestroy(p->g_adj);
+  if (p->gr_fwd) (void)hipGraphDestroy(p->gr_fwd);
+  if (p->gr_adj) (void)hipGraphDestroy(p->gr_adj);
+  for (DevicePointTable &d : p->dpts) d.release(nullptr);
+  (void)hipDeviceSynchronize();
+  for (void *a : p->allocs) (void)hipFree(a);
+  if (p->capture) (void)hipStreamDestroy(p->capture);
+  delete p;
+}
+
+// Builds descriptors for one tile shape; returns the tile -> item map.
+int build_items(Plan &p, int rows, std::vector<BatchItem<float>> &items,
+                std::vector<int32_t> &tile_item, std::vector<int64_t> &active,
+                const std::vector<BatchItem<float>> &proto) {
+  items = proto;
+  tile_item.clear();
+  int64_t tile0 = 0;
+  std::vector<int64_t> tiles_of(p.probs.size());
+  for (size_t m = 0; m < p.probs.size(); ++m) {
+    BatchItem<float> &it = items[m];
+    it.n_yt = (int32_t)((p.probs[m].ny + rows - 1) / rows);
+    const int64_t n_groups = (it.n_shots + p.group - 1) / p.group;
+    it.use_atomics = n_groups > 1 ? 1 : 0;
+    const int64_t tiles = (int64_t)it.pts.n_xt * it.n_yt * n_groups;
+    if (tile0 + tiles >= ((int64_t)1 << 31)) {
+      set_error("batch: too many tiles for one launch");
+      return kInvalidArgument;
+    }
+    it.tile0 = (int32_t)tile0;
+    tiles_of[m] = tiles;
+    tile_item.insert(tile_item.end(), (size_t)tiles, (int32_t)m);
+    tile0 += tiles;
+  }
+  active.assign((size_t)p.max_nt, 0);
+  // problems are sorted by nt descending: those with nt > k are a prefix
+  int64_t run = 0;
+  size_t m = p.probs.size();
+  std::vector<int64_t> prefix(p.probs.size() + 1, 0);
+  for (size_t i = 0; i < p.probs.size(); ++i) prefix[i + 1] = prefix[i] + tiles_of[i];
+  for (int k = 0; k < p.max_nt; ++k) {
+    while (m > 0 && p.probs[m - 1].nt <= k) --m;
+    run = prefix[m];
+    active[(size_t)k] = run;
+  }
+  return kOk;
+}
+
+}  // namespace
+}  // namespace tm2d
+}  // namespace tide_hip
+
+using namespace tide_hip;
+using namespace tide_hip::tm2d;
+
+namespace {
+int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t storage_format,
+               int64_t use_graph, int64_t device);
+}  // namespace
+
+extern "C" int tide_tm2d_plan_create(const tide_tm2d_problem *problems, int64_t n,
+                                     int64_t storage_format, int64_t use_graph, int64_t device,
+                                     void **plan_out) {
+  if (plan_out == nullptr || problems == nullptr || n <= 0) {
+    set_error("plan: problems and output handle are required");
+    return kInvalidArgument;
+  }
+  *plan_out = nullptr;
+  if (hipSetDevice((int)device) != hipSuccess) {
+    set_error("hipSetDevice failed");
+    return kHipError;
+  }
+  Plan *p = new Plan();
+  const int rc = build_plan(p, problems, n, storage_format, use_graph, device);
+  if (rc != kOk) {
+    destroy(p);
+    return rc;
+  }
+  *plan_out = p;
+  return kOk;
+}
+
+namespace {
+int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t storage_format,
+               int64_t use_graph, int64_t device) {
+  p->device = (int)device;
+  p->n = n;
+  p->use_graph = use_graph != 0;
+  p->rows_f = rows_for<float>(g_tuning.fwd_rows);
+  p->rows_a = rows_for<float>(g_tuning.adj_rows);
+  p->group = g_tuning.adj_group;  // one group size for both passes
+  p->probs.assign(problems, problems + n);
+  std::stable_sort(p->probs.begin(), p->probs.end(),
+                   [](const tide_tm2d_problem &a, const tide_tm2d_problem &b) { return a.nt > b.nt; });
+  const int fmt = storage_format == 1 ? 2 : 1;
+  bool any_store = false, any_grad = false;
+  auto fail = [](int rc) { return rc; };
+  std::vector<BatchItem<float>> proto(p->probs.size());
+  p->dpts.reserve(p->probs.size());
+  for (size_t m = 0; m < p->probs.size(); ++m) {
+    const tide_tm2d_problem &q = p->probs[m];
+    if (q.n_shots <= 0 || q.ny < 3 || q.nx < 3 || q.nt < 0 || q.interval <= 0 ||
+        q.ny * q.nx >= ((int64_t)1 << 29) || q.n_shots > 4096) {
+      set_error("plan: each problem needs n_shots in 1..4096, ny, nx >= 3, nt >= 0, interval >= 1");
+      return fail(kInvalidArgument);
+    }
+    if (q.nt >= ((int64_t)1 << 30)) {
+      set_error("plan: nt too large");
+      return fail(kInvalidArgument);
+    }
+    p->max_nt = std::max<int>(p->max_nt, (int)q.nt);
+    any_store = any_store || q.ey_store || q.curl_store;
+    any_grad = any_grad || q.grad_ca || q.grad_cb;
+    if ((q.grad_ca && !q.ey_store) || (q.grad_cb && !q.curl_store)) {
+      set_error("plan: coefficient gradients require the snapshot buffers");
+      return fail(kInvalidArgument);
+    }
+    BatchItem<float> &it = proto[m];
+    std::memset(&it, 0, sizeof(it));
+    Geom<float> &g = it.g;
+    g.rdy = q.rdy;
+    g.rdx = q.rdx;
+    g.ny = (int32_t)q.ny;
+    g.nx = (int32_t)q.nx;
+    g.numel = (int32_t)(q.ny * q.nx);
+    g.pml_y0 = (int32_t)q.pml[0];
+    g.pml_x0 = (int32_t)q.pml[1];
+    g.pml_y1 = (int32_t)q.pml[2];
+    g.pml_x1 = (int32_t)q.pml[3];
+    g.pml_y1h = (int32_t)(q.pml[2] > q.pml[0] ? q.pml[2] - 1 : q.pml[0]);
+    g.pml_x1h = (int32_t)(q.pml[3] > q.pml[1] ? q.pml[3] - 1 : q.pml[1]);
+    g.ca_b = g.cb_b = g.cq_b = 0;
+    // profiles packed [a, b, k, ah, bh, kh] per axis
+    float *pbuf;
+    PLAN_CHECK(plan_alloc(*p, &pbuf, (size_t)6 * (q.ny + q.nx)));
+    const int ysrc[6] = {0, 1, 8, 2, 3, 9}, xsrc[6] = {4, 5, 10, 6, 7, 11};
+    for (int k = 0; k < 6; ++k) {
+      TIDE_HIP_TRY(hipMemcpy(pbuf + (size_t)k * q.ny, q.prof[ysrc[k]], (size_t)q.ny * sizeof(float),
+                             hipMemcpyDefault));
+      TIDE_HIP_TRY(hipMemcpy(pbuf + (size_t)6 * q.ny + (size_t)k * q.nx, q.prof[xsrc[k]],
+                             (size_t)q.nx * sizeof(float), hipMemcpyDefault));
+    }
+    g.py = pbuf;
+    g.px = pbuf + (size_t)6 * q.ny;
+    // point table
+    std::vector<int64_t> hs, hr;
+    PLAN_CHECK(fetch_host(q.sources_i, q.n_shots * q.n_src, hs));
+    PLAN_CHECK(fetch_host(q.receivers_i, q.n_shots * q.n_rec, hr));
+    PointTable table;
+    if (!build_point_table(hs.data(), q.n_src, hr.data(), q.n_rec, q.n_shots, q.ny, q.nx, kOwn,
+                           1, table))
+      return fail(kInvalidArgument);
+    p->dpts.emplace_back();
+    DevicePointTable &dpt = p->dpts.back();
+    PLAN_CHECK(dpt.upload(table, nullptr));
+    it.pts = Points{dpt.row_slot, dpt.seg, dpt.ent, dpt.n_xt};
+    it.ca = q.ca;
+    it.cb = q.cb;
+    it.cq = q.cq;
+    const int64_t total = q.n_shots * q.ny * q.nx;
+    it.fwd[0][0] = q.fields[0];
+    it.fwd[0][1] = q.fields[1];
+    it.fwd[0][2] = q.fields[2];
+    it.fwd[0][3] = q.fields[3];
+    it.fwd[0][4] = q.fields[4];
+    for (int c = 0; c < 5; ++c) PLAN_CHECK(plan_alloc(*p, &it.fwd[1][c], (size_t)total));
+    for (int c = 3; c < 5; ++c) TIDE_HIP_TRY(hipMemset(it.fwd[1][c], 0, (size_t)total * sizeof(float)));
+    it.mhzx = q.fields[6];
+    it.mhxz = q.fields[5];
+    it.f = q.f;
+    it.r = q.r;
+    it.ey_store = q.ey_store;
+    it.curl_store = q.curl_store;
+    it.store_step_bytes = total * (fmt == 2 ? 2 : (int64_t)sizeof(float));
+    // adjoint state: caller's order ley lhx lhz ml_ey_x ml_ey_z ml_hx_z ml_hz_x;
+    // kernel order ley lhx lhz m_hz_x m_hx_z m_ey_x m_ey_z
+    if (q.lambda[0] != nullptr) {
+      const int perm[7] = {0, 1, 2, 6, 5, 3, 4};
+      for (int c = 0; c < 7; ++c) {
+        it.lam[0][c] = q.lambda[perm[c]];
+        PLAN_CHECK(plan_alloc(*p, &it.lam[1][c], (size_t)total));
+      }
+    }
+    it.grad_r = q.grad_r;
+    it.grad_f = q.grad_f;
+    it.gca = q.grad_ca;
+    it.gcb = q.grad_cb;
+    it.n_shots = (int32_t)q.n_shots;
+    it.n_src = (int32_t)q.n_src;
+    it.n_rec = (int32_t)q.n_rec;
+    it.nt = (int32_t)q.nt;
+    it.interval = (int32_t)q.interval;
+    if (q.n_src > 0 && q.f == nullptr) {
+      set_error("plan: a problem with sources needs its source term f");
+      return fail(kInvalidArgument);
+    }
+    if (q.n_rec > 0 && q.r == nullptr) {
+      set_error("plan: a problem with receivers needs its receiver buffer r");
+      return fail(kInvalidArgument);
+    }
+  }
+  p->fmt_fwd = any_store ? fmt : 0;
+  p->fmt_adj = any_grad ? fmt : 0;
+  std::vector<int32_t> tiles_f, tiles_a;
+  PLAN_CHECK(build_items(*p, p->rows_f, p->h_fwd, tiles_f, p->active_fwd, proto));
+  PLAN_CHECK(build_items(*p, p->rows_a, p->h_adj, tiles_a, p->active_adj, proto));
+  PLAN_CHECK(plan_alloc(*p, &p->d_fwd, p->h_fwd.size()));
+  PLAN_CHECK(plan_alloc(*p, &p->d_adj, p->h_adj.size()));
+  PLAN_CHECK(plan_alloc(*p, &p->d_tiles_fwd, tiles_f.size()));
+  PLAN_CHECK(plan_alloc(*p, &p->d_tiles_adj, tiles_a.size()));
+  TIDE_HIP_TRY(hipMemcpy(p->d_fwd, p->h_fwd.data(), p->h_fwd.size() * sizeof(BatchItem<float>),
+                         hipMemcpyHostToDevice));
+  TIDE_HIP_TRY(hipMemcpy(p->d_adj, p->h_adj.data(), p->h_adj.size() * sizeof(BatchItem<float>),
+                         hipMemcpyHostToDevice));
+  TIDE_HIP_TRY(hipMemcpy(p->d_tiles_fwd, tiles_f.data(), tiles_f.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+  TIDE_HIP_TRY(hipMemcpy(p->d_tiles_adj, tiles_a.data(), tiles_a.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice));
+  TIDE_HIP_TRY(hipDeviceSynchronize());
+  if (p->use_graph) {
+    if (hipStreamCreateWithFlags(&p->capture, hipStreamNonBlocking) != hipSuccess) {
+      set_error("plan: stream creation failed");
+      return fail(kHipError);
+    }
+    PLAN_CHECK(capture(*p, true, &p->gr_fwd, &p->g_fwd));
+    const bool any_adj = p->probs[0].lambda[0] != nullptr;
+    if (any_adj) PLAN_CHECK(capture(*p, false, &p->gr_adj, &p->g_adj));
+  }
+  return kOk;
+}
+}  // namespace
+
+extern "C" int tide_tm2d_plan_forward(void *plan, void *stream) {
+  Plan *p = (Plan *)plan;
+  if (p == nullptr) {
+    set_error("plan: null handle");
+    return kInvalidArgument;
+  }
+  TIDE_HIP_TRY(hipSetDevice(p->device));
+  if (p->use_graph) {
+    TIDE_HIP_TRY(hipGraphLaunch(p->g_fwd, (hipStream_t)stream));
+    return kOk;
+  }
+  return enqueue_forward(*p, (hipStream_t)stream);
+}
+
+extern "C" int tide_tm2d_plan_backward(void *plan, void *stream) {
+  Plan *p = (Plan *)plan;
+  if (p == nullptr) {
+    set_error("plan: null handle");
+    return kInvalidArgument;
+  }
+  for (const tide_tm2d_problem &q : p->probs)
+    if (q.lambda[0] == nullptr) {
+      set_error("plan: built without adjoint state; pass lambda arrays to run the adjoint");
+      return kInvalidArgument;
+    }
+  TIDE_HIP_TRY(hipSetDevice(p->device));
+  if (p->use_graph) {
+    TIDE_HIP_TRY(hipGraphLaunch(p->g_adj, (hipStream_t)stream));
+    return kOk;
+  }
+  return enqueue_backward(*p, (hipStream_t)stream);
+}
+
+extern "C" int tide_tm2d_plan_destroy(void *plan) {
+  destroy((Plan *)plan);
+  return kOk;
+}

@@ -1,0 +1,594 @@
+// Device code of the 2-D TM step kernels (tile functions and the
+// single-problem kernels), shared by the single-problem driver (tm2d.hip)
+// and the batched plan (tm2d_batch.hip).  See tm2d.hip for the design.
+#pragma once
+
+#include "tide_common.h"
+
+namespace tide_hip {
+namespace tm2d {
+
+constexpr int kOwn = 62;  // columns owned by a wavefront (64 lanes - 2 halo)
+
+
+// CPML profiles packed per axis: py = [a, b, k, ah, bh, kh] x ny and
+// px = [a, b, k, ah, bh, kh] x nx.
+enum { kA = 0, kB = 1, kK = 2, kAh = 3, kBh = 4, kKh = 5 };
+
+template <typename T> struct Geom {
+  const T *py, *px;
+  T rdy, rdx;
+  int32_t ny, nx, numel;
+  int32_t pml_y0, pml_y1, pml_x0, pml_x1, pml_y1h, pml_x1h;
+  int32_t ca_b, cb_b, cq_b;  // coefficient is per shot
+  __device__ __forceinline__ T Y(int k, int y) const { return py[k * ny + y]; }
+  __device__ __forceinline__ T X(int k, int x) const { return px[k * nx + x]; }
+};
+
+struct Points {
+  const int32_t *row_slot;
+  const int32_t *seg;
+  const int2 *ent;
+  int32_t n_xt;
+};
+
+template <typename T> struct FwdArgs {
+  Geom<T> g;
+  Points pts;
+  const T *ca, *cb, *cq;
+  const T *ey_i, *hx_i, *hz_i;
+  T *ey_o, *hx_o, *hz_o;
+  const T *mex_i, *mez_i;  // m_Ey_x, m_Ey_z: H-phase memories (ping-pong)
+  T *mex_o, *mez_o;
+  T *mhzx, *mhxz;          // m_Hz_x, m_Hx_z: E-phase memories (in place)
+  void *ey_store, *curl_store;
+  const T *f_t;
+  T *r_t;
+  int32_t n_shots, n_yt;
+};
+
+template <typename T> struct AdjArgs {
+  Geom<T> g;
+  Points pts;
+  const T *ca, *cb, *cq;
+  const T *ley_i, *lhx_i, *lhz_i;
+  T *ley_o, *lhx_o, *lhz_o;
+  // adjoint CPML memories: mx = m_lambda_hz_x, my = m_lambda_hx_z,
+  // nx = m_lambda_ey_x, nz = m_lambda_ey_z (all ping-pong)
+  const T *mx_i, *my_i, *nx_i, *nz_i;
+  T *mx_o, *my_o, *nx_o, *nz_o;
+  const void *ey_store, *curl_store;
+  double *gca, *gcb;
+  int32_t per_shot_grad;  // gradient arrays are [n_shots, ny, nx]
+  double step_scale;
+  const T *grad_r_t;
+  T *grad_f_t;
+  int32_t n_shots, n_yt;
+  int32_t use_atomics;
+};
+
+// Snapshots are written once and read once, much later: non-temporal stores
+// keep them from evicting the fields, which the next step reads again.
+template <typename T, int FMT>
+__device__ __forceinline__ void put_store(void *base, int64_t i, T v) {
+  if constexpr (FMT == 2) {
+    __builtin_nontemporal_store(f32_to_bf16((float)v), (uint16_t *)base + i);
+  } else {
+    __builtin_nontemporal_store(v, (T *)base + i);
+  }
+}
+
+// The adjoint reads each snapshot once: non-temporal loads keep the adjoint
+// fields cached.
+template <typename T, int FMT>
+__device__ __forceinline__ double get_store(const void *base, int64_t i) {
+  if constexpr (FMT == 2) {
+    return (double)bf16_to_f32(__builtin_nontemporal_load((const uint16_t *)base + i));
+  } else {
+    return (double)__builtin_nontemporal_load((const T *)base + i);
+  }
+}
+
+struct TileCoord {
+  int xt, yt, grp;
+};
+
+__device__ __forceinline__ TileCoord decode_tile(int n_xt, int n_yt, int n_groups) {
+  const int64_t n_tiles = (int64_t)n_xt * n_yt;
+  const int64_t flat = xcd_remap((int64_t)blockIdx.x, n_tiles * n_groups);
+  TileCoord c;
+  c.grp = (int)(flat / n_tiles);
+  const int64_t tile = flat - (int64_t)c.grp * n_tiles;
+  c.yt = (int)(tile / n_xt);
+  c.xt = (int)(tile - (int64_t)c.yt * n_xt);
+  return c;
+}
+
+// Row loader for one wavefront.  Interior tiles (EDGE false) need no bounds
+// checks: every row and column they touch exists.  Edge tiles read a
+// clamped column and zero it outside the grid.
+template <typename T, bool EDGE> struct RowLoader {
+  int x, xc, nx, ny;
+  bool xv;
+  __device__ __forceinline__ RowLoader(int x_, int nx_, int ny_) : x(x_), nx(nx_), ny(ny_) {
+    xv = x_ >= 0 && x_ < nx_;
+    xc = min(max(x_, 0), nx_ - 1);
+  }
+  __device__ __forceinline__ T operator()(const T *__restrict__ p, int y) const {
+    if constexpr (EDGE) {
+      if (y < 0 || y >= ny) return (T)0;  // uniform
+      const T v = p[y * nx + xc];
+      return xv ? v : (T)0;
+    } else {
+      return p[y * nx + x];
+    }
+  }
+};
+
+// ----------------------------------------------------------------- forward
+// FMT: 0 = no snapshot, 1 = T payload, 2 = BF16 payload.
+template <typename T, int TY, int FMT, bool EDGE>
+__device__ __forceinline__ void fwd_tile(const FwdArgs<T> &a, int s, int xt, int yt) {
+  const Geom<T> &g = a.g;
+  const int ny = g.ny, nx = g.nx;
+  const int lane = threadIdx.x;
+  const int x = xt * kOwn - 1 + lane;
+  const int y0 = yt * TY;
+  const bool own = lane >= 1 && lane <= kOwn && x < nx;
+  // Update domains for stencil 2 (FD_PAD = 1); all true inside interior tiles.
+  const bool e_col = !EDGE || (x >= 1 && x < nx);       // Ey and Hx columns
+  const bool hz_col = !EDGE || (x >= 1 && x < nx - 1);  // Hz columns
+  const int x0 = xt * kOwn - 1;
+  const bool tile_pml_x = x0 < g.pml_x0 || x0 + 63 >= g.pml_x1h;
+  const bool tile_pml_y = y0 - 1 < g.pml_y0 || y0 + TY - 1 >= g.pml_y1h;
+  const bool pml_xe = tile_pml_x && (x < g.pml_x0 || x >= g.pml_x1);
+  const bool pml_xh = tile_pml_x && (x < g.pml_x0 || x >= g.pml_x1h);
+
+  const int64_t base = (int64_t)s * g.numel;
+  const T *__restrict__ ey_i = a.ey_i + base;
+  const T *__restrict__ hx_i = a.hx_i + base;
+  const T *__restrict__ hz_i = a.hz_i + base;
+  const T *__restrict__ mex_i = a.mex_i + base;
+  const T *__restrict__ mez_i = a.mez_i + base;
+  T *__restrict__ ey_o = a.ey_o + base;
+  T *__restrict__ hx_o = a.hx_o + base;
+  T *__restrict__ hz_o = a.hz_o + base;
+  T *__restrict__ mex_o = a.mex_o + base;
+  T *__restrict__ mez_o = a.mez_o + base;
+  T *__restrict__ mhzx = a.mhzx + base;
+  T *__restrict__ mhxz = a.mhxz + base;
+  const T *__restrict__ ca = a.ca + (g.ca_b ? base : 0);
+  const T *__restrict__ cb = a.cb + (g.cb_b ? base : 0);
+  const T *__restrict__ cq = a.cq + (g.cq_b ? base : 0);
+  const RowLoader<T, EDGE> L(x, nx, ny);
+
+  // ---- loads: Ey rows y0-1..y0+TY, Hx and cq rows y0-1..y0+TY-1, the rest
+  // rows y0..y0+TY-1.
+  T ey[TY + 2], hx[TY + 1], cqv[TY + 1], hz[TY], cav[TY], cbv[TY];
+#pragma unroll
+  for (int k = 0; k < TY + 2; ++k) ey[k] = L(ey_i, y0 - 1 + k);
+#pragma unroll
+  for (int k = 0; k < TY + 1; ++k) {
+    hx[k] = L(hx_i, y0 - 1 + k);
+    cqv[k] = L(cq, y0 - 1 + k);
+  }
+#pragma unroll
+  for (int k = 0; k < TY; ++k) {
+    hz[k] = L(hz_i, y0 + k);
+    cav[k] = L(ca, y0 + k);
+    cbv[k] = L(cb, y0 + k);
+  }
+
+  // Hx^{n+1/2}(y0-1, x): recomputed here, written by the tile above.
+  T hx_up = hx[0];
+  {
+    const int y = y0 - 1;
+    if (e_col && (!EDGE || (y >= 1 && y < ny - 1))) {
+      T d = (ey[1] - ey[0]) * g.rdy;
+      if (tile_pml_y && (y < g.pml_y0 || y >= g.pml_y1h)) {
+        const T m = g.Y(kBh, y) * mez_i[y * nx + x] + g.Y(kAh, y) * d;
+        d = d / g.Y(kKh, y) + m;
+      }
+      hx_up = hx_up - cqv[0] * d;
+    }
+  }
+
+#pragma unroll
+  for (int r = 0; r < TY; ++r) {
+    const int y = y0 + r;
+    if (EDGE && y >= ny) break;  // uniform
+    const int j = y * nx + x;
+    const int64_t i = base + j;
+    const bool e_row = !EDGE || y >= 1;
+    const bool hx_row = !EDGE || (y >= 1 && y < ny - 1);
+    const bool pml_ye = tile_pml_y && (y < g.pml_y0 || y >= g.pml_y1);
+    const bool pml_yh = tile_pml_y && (y < g.pml_y0 || y >= g.pml_y1h);
+    const T ey_c = ey[r + 1];
+    const T ey_r = shfl_dn1(ey_c);  // Ey(y, x+1); unused at lane 63
+
+    // ---- H half-step (every lane: lanes 0 and 63 are halo copies).
+    T hx_n = hx[r + 1];
+    if (e_col && hx_row) {
+      T d = (ey[r + 2] - ey_c) * g.rdy;
+      if (pml_yh) {
+        const T m = g.Y(kBh, y) * mez_i[j] + g.Y(kAh, y) * d;
+        if (own) mez_o[j] = m;
+        d = d / g.Y(kKh, y) + m;
+      }
+      hx_n = hx_n - cqv[r + 1] * d;
+    }
+    T hz_n = hz[r];
+    if (hz_col && e_row) {
+      T d = (ey_r - ey_c) * g.rdx;
+      if (pml_xh) {
+        const T m = g.X(kBh, x) * mex_i[j] + g.X(kAh, x) * d;
+        if (own) mex_o[j] = m;
+        d = d / g.X(kKh, x) + m;
+      }
+      hz_n = hz_n + cqv[r + 1] * d;
+    }
+    const T hz_l = shfl_up1(hz_n);  // Hz(y, x-1); unused at lane 0
+
+    // ---- E step for owned cells.
+    T ey_n = ey_c;
+    if (own && e_col && e_row) {
+      T dhz = (hz_n - hz_l) * g.rdx;
+      T dhx = (hx_n - hx_up) * g.rdy;
+      if (pml_xe) {
+        const T m = g.X(kB, x) * mhzx[j] + g.X(kA, x) * dhz;
+        mhzx[j] = m;
+        dhz = dhz / g.X(kK, x) + m;
+      }
+      if (pml_ye) {
+        const T m = g.Y(kB, y) * mhxz[j] + g.Y(kA, y) * dhx;
+        mhxz[j] = m;
+        dhx = dhx / g.Y(kK, y) + m;
+      }
+      const T curl = dhz - dhx;
+      if constexpr (FMT != 0) {
+        if (a.ey_store) put_store<T, FMT>(a.ey_store, i, ey_c);
+        if (a.curl_store) put_store<T, FMT>(a.curl_store, i, curl);
+      }
+      ey_n = cav[r] * ey_c + cbv[r] * curl;
+    }
+
+    // ---- sources, then receivers, of this shot in this row segment.
+    const int slot = a.pts.row_slot[y];
+    if (slot >= 0) {
+      const int64_t si = ((int64_t)slot * a.n_shots + s) * a.pts.n_xt + xt;
+      const int e1 = a.pts.seg[si + 1];
+      for (int e = a.pts.seg[si]; e < e1; ++e) {
+        const int2 en = a.pts.ent[e];
+        if ((en.x & 0xff) == lane) {
+          if ((en.x >> 8) == 0)
+            ey_n += a.f_t[en.y];
+          else
+            a.r_t[en.y] = ey_n;
+        }
+      }
+    }
+
+    if (own) {
+      hx_o[j] = hx_n;
+      hz_o[j] = hz_n;
+      ey_o[j] = ey_n;
+    }
+    hx_up = hx_n;
+  }
+}
+
+template <typename T, int TY>
+__device__ __forceinline__ bool interior_tile(const Geom<T> &g, int xt, int yt) {
+  const int x0 = xt * kOwn - 1, y0 = yt * TY;
+  return x0 >= 1 && x0 + 63 <= g.nx - 2 && y0 >= 2 && y0 + TY + 1 <= g.ny - 1;
+}
+
+// One forward workgroup: wavefront threadIdx.y handles shot grp*G + y.
+template <typename T, int TY, int FMT>
+__device__ __forceinline__ void fwd_workgroup(const FwdArgs<T> &a, const TileCoord &tc) {
+  const int G = (int)blockDim.y;
+  const int s = tc.grp * G + (int)threadIdx.y;
+  if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
+  if (interior_tile<T, TY>(a.g, tc.xt, tc.yt))
+    fwd_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt);
+  else
+    fwd_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt);
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(512) void fwd_step(FwdArgs<T> a) {
+  const int G = (int)blockDim.y;
+  const int n_groups = (a.n_shots + G - 1) / G;
+  fwd_workgroup<T, TY, FMT>(a, decode_tile(a.pts.n_xt, a.n_yt, n_groups));
+}
+
+// ----------------------------------------------------------------- adjoint
+// One reverse iteration: transpose of the E step, transpose of the H
+// half-step, receiver injection, source read-out, coefficient gradient.
+template <typename T, int TY, int FMT, bool EDGE>
+__device__ __forceinline__ void adj_tile(const AdjArgs<T> &a, int s, int xt, int yt,
+                                         double *red_ca, double *red_cb, bool reduce) {
+  const Geom<T> &g = a.g;
+  const int ny = g.ny, nx = g.nx;
+  const int lane = threadIdx.x;
+  const int x = xt * kOwn - 1 + lane;
+  const int y0 = yt * TY;
+  const bool own = lane >= 1 && lane <= kOwn && x < nx;
+  const bool e_col = !EDGE || (x >= 1 && x < nx);
+  const int x0 = xt * kOwn - 1;
+  const bool tile_pml_x = x0 - 1 < g.pml_x0 || x0 + 64 >= g.pml_x1h;
+  const bool tile_pml_y = y0 - 1 < g.pml_y0 || y0 + TY >= g.pml_y1h;
+  const bool pml_xe = tile_pml_x && (x < g.pml_x0 || x >= g.pml_x1);
+  const bool pml_xh = tile_pml_x && (x < g.pml_x0 || x >= g.pml_x1h);
+  const bool want_grad = (a.gca != nullptr || a.gcb != nullptr) && FMT != 0;
+
+  const int64_t base = (int64_t)s * g.numel;
+  const T *__restrict__ ley_i = a.ley_i + base;
+  const T *__restrict__ lhx_i = a.lhx_i + base;
+  const T *__restrict__ lhz_i = a.lhz_i + base;
+  const T *__restrict__ mx_i = a.mx_i + base;
+  const T *__restrict__ my_i = a.my_i + base;
+  const T *__restrict__ nx_i = a.nx_i + base;
+  const T *__restrict__ nz_i = a.nz_i + base;
+  T *__restrict__ ley_o = a.ley_o + base;
+  T *__restrict__ lhx_o = a.lhx_o + base;
+  T *__restrict__ lhz_o = a.lhz_o + base;
+  const T *__restrict__ ca = a.ca + (g.ca_b ? base : 0);
+  const T *__restrict__ cb = a.cb + (g.cb_b ? base : 0);
+  const T *__restrict__ cq = a.cq + (g.cq_b ? base : 0);
+  const RowLoader<T, EDGE> L(x, nx, ny);
+
+  // ---- loads.  lam/cbv rows y0-1..y0+TY; lhx/cqv rows y0-1..y0+TY-1; the
+  // rest rows y0..y0+TY-1.
+  T lam[TY + 2], cbv[TY + 2], lhx[TY + 1], cqv[TY + 1], lhz[TY], cav[TY];
+  float se[TY], sc[TY];  // snapshot values (exact in fp32 for fp32/bf16 payloads)
+#pragma unroll
+  for (int k = 0; k < TY + 2; ++k) {
+    lam[k] = L(ley_i, y0 - 1 + k);
+    cbv[k] = L(cb, y0 - 1 + k);
+  }
+#pragma unroll
+  for (int k = 0; k < TY + 1; ++k) {
+    lhx[k] = L(lhx_i, y0 - 1 + k);
+    cqv[k] = L(cq, y0 - 1 + k);
+  }
+#pragma unroll
+  for (int k = 0; k < TY; ++k) {
+    const int y = y0 + k;
+    lhz[k] = L(lhz_i, y);
+    cav[k] = L(ca, y);
+    se[k] = 0.0f;
+    sc[k] = 0.0f;
+    if constexpr (sizeof(T) == 4) {
+      if (want_grad && own && (!EDGE || (y >= 1 && y < ny && x >= 1))) {
+        const int64_t i = base + y * nx + x;
+        if (a.gca) se[k] = (float)get_store<T, FMT>(a.ey_store, i);
+        if (a.gcb) sc[k] = (float)get_store<T, FMT>(a.curl_store, i);
+      }
+    }
+  }
+
+  // Adjoint of dHz/dx at (y, x) from lambda_Ey and cb there.
+  auto raw_x = [&](int y, T lm, T cbx) -> T {
+    if (EDGE && (y < 1 || y >= ny || x < 1 || x >= nx)) return (T)0;
+    const T q = cbx * lm;
+    if (!pml_xe) return q;
+    const T mn = g.X(kB, x) * mx_i[y * nx + x] + q;
+    return g.X(kA, x) * mn + q / g.X(kK, x);
+  };
+  // Negated adjoint of dHx/dz at (y, x): q off the strips, q/k - a*mn in
+  // them.  The reference's term is -q resp. a*mn - q/k; IEEE subtraction
+  // is sign-symmetric, so carrying the negation and swapping the operands
+  // of the differences below gives bit-identical results.  (Returning -q
+  // from a branch of this lambda was miscompiled by the gfx950 backend of
+  // ROCm 7.2: inside tiles that touch a y strip, the interior-row path
+  // skipped the negation.)
+  auto nraw_y = [&](int y, T lm, T cbx) -> T {
+    if (EDGE && (y < 1 || y >= ny || !e_col)) return (T)0;
+    const T q = cbx * lm;
+    if (!(tile_pml_y && (y < g.pml_y0 || y >= g.pml_y1))) return q;
+    const T mn = g.Y(kB, y) * my_i[y * nx + x] - q;
+    return q / g.Y(kK, y) - g.Y(kA, y) * mn;
+  };
+  // Adjoint of the Hx update's dEy/dz at half row y.
+  auto raw_z = [&](int y, T lhxv, T cqx) -> T {
+    if (EDGE && (y < 1 || y >= ny - 1 || !e_col)) return (T)0;
+    const T adj = -cqx * lhxv;
+    if (!(tile_pml_y && (y < g.pml_y0 || y >= g.pml_y1h))) return adj;
+    const T mn = g.Y(kBh, y) * nz_i[y * nx + x] + adj;
+    return g.Y(kAh, y) * mn + adj / g.Y(kKh, y);
+  };
+  // Adjoint of the Hz update's dEy/dx at half column x.
+  auto raw_xh = [&](int y, T lhzv, T cqx) -> T {
+    if (EDGE && (y < 1 || y >= ny || x < 1 || x >= nx - 1)) return (T)0;
+    const T adj = cqx * lhzv;
+    if (!pml_xh) return adj;
+    const T mn = g.X(kBh, x) * nx_i[y * nx + x] + adj;
+    return g.X(kAh, x) * mn + adj / g.X(kKh, x);
+  };
+
+  T nrawy_c = nraw_y(y0, lam[1], cbv[1]);
+  T rawz_up = (T)0;
+  if (!EDGE || y0 >= 1) {
+    const int y = y0 - 1;
+    T v = lhx[0];
+    if (e_col && (!EDGE || y >= 1)) v = v + (nrawy_c - nraw_y(y, lam[0], cbv[0])) * g.rdy;
+    rawz_up = raw_z(y, v, cqv[0]);
+  }
+
+#pragma unroll
+  for (int r = 0; r < TY; ++r) {
+    const int y = y0 + r;
+    if (EDGE && y >= ny) break;
+    const int j = y * nx + x;
+    const bool in_e = e_col && (!EDGE || y >= 1);
+    const T ley_c = lam[r + 1];
+
+    // ---- transpose of the E step.
+    const T rawx_c = raw_x(y, ley_c, cbv[r + 1]);
+    const T rawx_r = shfl_dn1(rawx_c);  // unused at lane 63
+    T lhz_n = lhz[r];
+    if (in_e) lhz_n = lhz_n + (rawx_c - rawx_r) * g.rdx;
+    const T nrawy_dn = nraw_y(y + 1, lam[r + 2], cbv[r + 2]);
+    T lhx_n = lhx[r + 1];
+    if (in_e) lhx_n = lhx_n + (nrawy_dn - nrawy_c) * g.rdy;
+    if (own && in_e) {
+      const T q = cbv[r + 1] * ley_c;
+      if (pml_xe) a.mx_o[base + j] = g.X(kB, x) * mx_i[j] + q;
+      if (tile_pml_y && (y < g.pml_y0 || y >= g.pml_y1))
+        a.my_o[base + j] = g.Y(kB, y) * my_i[j] - q;
+    }
+
+    // ---- transpose of the H half-step.
+    const T rawz_c = raw_z(y, lhx_n, cqv[r + 1]);
+    const T rawxh_c = raw_xh(y, lhz_n, cqv[r + 1]);
+    const T rawxh_l = shfl_up1(rawxh_c);  // unused at lane 0
+    T ley_n = ley_c;
+    if (own && in_e) {
+      const T cqx = cqv[r + 1];
+      const T adj_y = -cqx * lhx_n;
+      const T adj_x = cqx * lhz_n;
+      if ((!EDGE || y < ny - 1) && tile_pml_y && (y < g.pml_y0 || y >= g.pml_y1h))
+        a.nz_o[base + j] = g.Y(kBh, y) * nz_i[j] + adj_y;
+      if ((!EDGE || x < nx - 1) && pml_xh) a.nx_o[base + j] = g.X(kBh, x) * nx_i[j] + adj_x;
+      const T t1 = cav[r] * ley_c;
+      const T t2 = (rawz_up - rawz_c) * g.rdy + (rawxh_l - rawxh_c) * g.rdx;
+      ley_n = t1 + t2;
+    }
+
+    // ---- receivers inject grad_r[t]; sources read grad_f[t].
+    const int slot = a.pts.row_slot[y];
+    if (slot >= 0) {
+      const int64_t si = ((int64_t)slot * a.n_shots + s) * a.pts.n_xt + xt;
+      const int e0 = a.pts.seg[si], e1 = a.pts.seg[si + 1];
+      for (int e = e0; e < e1; ++e) {
+        const int2 en = a.pts.ent[e];
+        if ((en.x >> 8) == 1 && (en.x & 0xff) == lane) ley_n += a.grad_r_t[en.y];
+      }
+      if (a.grad_f_t != nullptr) {
+        for (int e = e0; e < e1; ++e) {
+          const int2 en = a.pts.ent[e];
+          if ((en.x >> 8) == 0 && (en.x & 0xff) == lane) a.grad_f_t[en.y] = ley_n;
+        }
+      }
+    }
+
+    // ---- coefficient gradients against the stored forward state.
+    if (want_grad && own && in_e) {
+      const double lm = (double)ley_n;
+      double ve, vc;
+      if constexpr (sizeof(T) == 4) {
+        ve = (double)se[r];
+        vc = (double)sc[r];
+      } else {
+        ve = a.gca ? get_store<T, FMT>(a.ey_store, base + j) : 0.0;
+        vc = a.gcb ? get_store<T, FMT>(a.curl_store, base + j) : 0.0;
+      }
+      const double pa = lm * ve * a.step_scale;
+      const double pb = lm * vc * a.step_scale;
+      if (reduce) {
+        red_ca[r * 64 + lane] = pa;
+        red_cb[r * 64 + lane] = pb;
+      } else {
+        const int64_t gi = (a.per_shot_grad ? base : 0) + j;
+        if (a.gca) a.gca[gi] += pa;
+        if (a.gcb) a.gcb[gi] += pb;
+      }
+    }
+
+    if (own) {
+      ley_o[j] = ley_n;
+      lhx_o[j] = lhx_n;
+      lhz_o[j] = lhz_n;
+    }
+    nrawy_c = nrawy_dn;
+    rawz_up = rawz_c;
+  }
+}
+
+// The per-shot FP64 gradient terms of a tile are summed across its
+// wavefronts (shots) through LDS in a fixed order, bitwise reproducibly,
+// and added once per cell.
+// One adjoint workgroup; `red` is the dynamic LDS block [2][G][TY][64].
+template <typename T, int TY, int FMT>
+__device__ __forceinline__ void adj_workgroup(const AdjArgs<T> &a, const TileCoord &tc,
+                                              double *red) {
+  const int lane = threadIdx.x;
+  const int w = (int)threadIdx.y;
+  const int G = (int)blockDim.y;
+  const int s = tc.grp * G + w;
+  const bool want_grad = (a.gca != nullptr || a.gcb != nullptr) && FMT != 0;
+  const bool reduce = want_grad && !a.per_shot_grad;
+  double *red_ca = red + (int64_t)w * TY * 64;
+  double *red_cb = red + ((int64_t)G + w) * TY * 64;
+  if (reduce) {
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+      red_ca[r * 64 + lane] = 0.0;
+      red_cb[r * 64 + lane] = 0.0;
+    }
+  }
+  if (s < a.n_shots) {  // inactive wavefronts still join the barrier below
+    if (interior_tile<T, TY>(a.g, tc.xt, tc.yt))
+      adj_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt, red_ca, red_cb, reduce);
+    else
+      adj_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt, red_ca, red_cb, reduce);
+  }
+  if (!reduce) return;  // uniform over the workgroup
+  __syncthreads();
+  const int x = tc.xt * kOwn - 1 + lane;
+  const int y0 = tc.yt * TY;
+  if (lane < 1 || lane > kOwn || x < 1 || x >= a.g.nx) return;
+  for (int pass = 0; pass < 2; ++pass) {
+    double *out = pass == 0 ? a.gca : a.gcb;
+    if (out == nullptr) continue;
+    const double *slots = red + (int64_t)pass * G * TY * 64;
+    for (int r = w; r < TY; r += G) {
+      const int y = y0 + r;
+      if (y >= 1 && y < a.g.ny) {
+        double sum = 0.0;
+        for (int k = 0; k < G; ++k) sum += slots[(k * TY + r) * 64 + lane];
+        const int64_t j = (int64_t)y * a.g.nx + x;
+        if (a.use_atomics)
+          atomicAdd(&out[j], sum);
+        else
+          out[j] += sum;
+      }
+    }
+  }
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(512) void adj_step(AdjArgs<T> a) {
+  extern __shared__ double red[];
+  const int G = (int)blockDim.y;
+  const int n_groups = (a.n_shots + G - 1) / G;
+  adj_workgroup<T, TY, FMT>(a, decode_tile(a.pts.n_xt, a.n_yt, n_groups), red);
+}
+
+// Adds an FP64 accumulator into the caller's gradient array (reference
+// combine_grad semantics: grad += sum over shots).
+template <typename T>
+__global__ void fold_grad(T *__restrict__ out, const double *__restrict__ acc, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (T)((double)out[i] + acc[i]);
+}
+
+constexpr int kMaxGroup = 8;  // shots (wavefronts) per workgroup
+
+// Launch shape: rows marched per wavefront and shots (wavefronts) per
+// workgroup for each kernel.  Process-wide (defined in tm2d.hip); set by the
+// autotuner through tide_tm2d_set_tuning().  fp64 fields use half the rows
+// (twice the registers per value).
+struct Tuning {
+  int fwd_rows = 4, fwd_group = 4, adj_rows = 2, adj_group = 4;
+};
+extern Tuning g_tuning;
+
+template <typename T> inline int rows_for(int rows) {
+  if (sizeof(T) == 8) rows = rows / 2 > 2 ? rows / 2 : 2;
+  return rows;
+}
+
+}  // namespace tm2d
+}  // namespace tide_hip

@@ -1,0 +1,146 @@
+"""Launch-shape autotuner for the TM2D step kernels.
+
+The forward and adjoint kernels take two launch parameters: rows marched
+per wavefront (the tile height) and shots per workgroup (wavefronts that
+share a tile's coefficient lines and reduce the FP64 gradient in LDS).  The
+best pair depends on the grid, the shot count and the pass (the adjoint
+carries more state per row).  :func:`tune_tm2d` times every candidate on a
+synthetic problem of the caller's size with HIP events, installs the
+fastest pair per pass (``tide_tm2d_set_tuning``) and caches it per size
+class, so later calls with a similar problem reuse the choice.
+"""
+
+from __future__ import annotations
+
+import time
+from collections.abc import Sequence
+
+import torch
+
+from . import _native
+from .maxwell.tm2d import prepare_tm2d
+from .utils import C0
+
+FWD_CANDIDATES: tuple[tuple[int, int], ...] = ((2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (8, 4), (8, 8), (16, 4))
+ADJ_CANDIDATES: tuple[tuple[int, int], ...] = ((2, 2), (2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (8, 2))
+
+_cache: dict[tuple, tuple[int, int, int, int]] = {}
+
+
+def _size_class(ny: int, nx: int, n_shots: int, dtype: torch.dtype) -> tuple:
+    """Grids within a factor sqrt(2) in cells and shot counts in the same
+    power-of-two bucket share a tuning."""
+    cells = ny * nx
+    return (int(round(2 * torch.log2(torch.tensor(float(cells))).item())),
+            max(0, int(n_shots).bit_length() - 1), str(dtype))
+
+
+def _problem(ny: int, nx: int, n_shots: int, nt: int, dtype: torch.dtype, device: torch.device):
+    pml = 20 if min(ny, nx) > 80 else 5
+    my, mx = max(ny - 2 * pml - 1, 4), max(nx - 2 * pml - 1, 4)
+    g = torch.Generator().manual_seed(11)
+    eps = (1 + 8 * torch.rand(my, mx, generator=g)).to(dtype=dtype, device=device)
+    sig = (0.01 * torch.rand(my, mx, generator=g)).to(dtype=dtype, device=device)
+    dx = 0.01
+    dt = 0.9 * dx / (2**0.5 * C0)
+    amp = torch.zeros(n_shots, 1, nt, dtype=dtype, device=device)
+    amp[:, :, 1] = 1.0
+    src = torch.stack([torch.tensor([[1, (3 + 7 * s) % mx]]) for s in range(n_shots)]).to(device)
+    rec = torch.stack([torch.tensor([[1, x] for x in range(0, mx, max(1, mx // 32))])
+                       for _ in range(n_shots)]).to(device)
+    return prepare_tm2d(eps, sig, torch.ones_like(eps), dx, dt, amp, src, rec, pml_width=pml)
+
+
+def _time(fn, reps: int = 2) -> float:
+    best = float("inf")
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    return best
+
+
+def tune_tm2d(
+    ny: int,
+    nx: int,
+    n_shots: int,
+    *,
+    dtype: torch.dtype = torch.float32,
+    nt: int = 48,
+    device: torch.device | str = "cuda:0",
+    fwd_candidates: Sequence[tuple[int, int]] = FWD_CANDIDATES,
+    adj_candidates: Sequence[tuple[int, int]] = ADJ_CANDIDATES,
+    use_cache: bool = True,
+) -> dict:
+    """Pick and install the fastest (rows, shots-per-workgroup) for the
+    forward-with-snapshots and adjoint kernels on a padded ``ny x nx`` grid
+    with ``n_shots`` shots per call.  Returns the choice and the timings."""
+    device = torch.device(device)
+    key = _size_class(ny, nx, n_shots, dtype)
+    if use_cache and key in _cache:
+        _native.set_tm2d_tuning(*_cache[key])
+        return {"tuning": _cache[key], "cached": True}
+    t_start = time.perf_counter()
+    st = _problem(ny, nx, n_shots, nt, dtype, device)
+    lib = _native.load()
+    P = _native.ptr
+    S, pny, pnx = st.n_shots, st.ny, st.nx
+    stores = [torch.empty(nt, S, pny, pnx, device=device, dtype=dtype) for _ in range(2)]
+    r = torch.zeros(nt, S, st.n_rec, device=device, dtype=dtype)
+    gr = torch.ones_like(r)
+    gf = torch.zeros(nt, S, st.n_src, device=device, dtype=dtype)
+    gca = torch.zeros(pny, pnx, device=device, dtype=torch.float64)
+    gcb = torch.zeros_like(gca)
+    stream = _native.stream_handle(device)
+    fwd = _native.get_function("maxwell_tm", "forward_with_storage", 2, dtype)
+    bwd = getattr(lib, f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
+    bpe = dtype.itemsize
+
+    def run_fwd():
+        fields = [t.clone() for t in st.fields]
+        _native.check(fwd(
+            P(st.ca), P(st.cb), P(st.cq), P(st.f), *[P(t) for t in fields], P(r),
+            P(stores[0]), None, None, P(stores[1]), None, None, *[P(p) for p in st.profiles],
+            P(st.sources_i), P(st.receivers_i), st.rdy, st.rdx, st.dt, nt, S, pny, pnx,
+            st.n_src, st.n_rec, 1, 0, 0, pny * pnx * bpe, True, True, False, False, False, 0,
+            *st.pml, 0, device.index or 0, stream, stream))
+
+    def run_bwd():
+        lam = [torch.zeros(S, pny, pnx, device=device, dtype=dtype) for _ in range(7)]
+        _native.check(bwd(
+            P(st.ca), P(st.cb), P(st.cq), P(gr), *[P(t) for t in lam], P(stores[0]),
+            P(stores[1]), P(gf), P(gca), P(gcb), *[P(p) for p in st.profiles],
+            P(st.sources_i), P(st.receivers_i), st.rdy, st.rdx, nt, S, pny, pnx, st.n_src,
+            st.n_rec, 1, 0, pny * pnx * bpe, False, False, False, nt, *st.pml,
+            device.index or 0, stream))
+
+    before = _native.get_tm2d_tuning()
+    run_fwd()
+    run_bwd()
+    torch.cuda.synchronize()
+    f_times, a_times = {}, {}
+    for rows, grp in fwd_candidates:
+        _native.set_tm2d_tuning(rows, grp, before[2], before[3])
+        f_times[(rows, grp)] = _time(run_fwd)
+    best_f = min(f_times, key=f_times.get)
+    for rows, grp in adj_candidates:
+        _native.set_tm2d_tuning(best_f[0], best_f[1], rows, grp)
+        a_times[(rows, grp)] = _time(run_bwd)
+    best_a = min(a_times, key=a_times.get)
+    choice = (best_f[0], best_f[1], best_a[0], best_a[1])
+    _native.set_tm2d_tuning(*choice)
+    _cache[key] = choice
+    return {
+        "tuning": choice,
+        "cached": False,
+        "forward_us_per_step": {f"{k[0]}x{k[1]}": round(v * 1e3 / nt, 2) for k, v in f_times.items()},
+        "adjoint_us_per_step": {f"{k[0]}x{k[1]}": round(v * 1e3 / nt, 2) for k, v in a_times.items()},
+        "seconds": round(time.perf_counter() - t_start, 2),
+    }
+
+
+__all__ = ["tune_tm2d", "FWD_CANDIDATES", "ADJ_CANDIDATES"]
