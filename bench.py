@@ -155,10 +155,12 @@ def main() -> None:
     if not args.no_autotune:
         from tide.autotune import tune_tm2d
         tuning = tune_tm2d(ny_p, nx_p, max(sizes), device=dev)
-        log(f"autotuned launch shapes {tuning['tuning']} in {tuning.get('seconds', 0)} s")
+        log(f"autotuned launch shapes {tuning['tuning']} shot loop {tuning['shot_loop']} "
+            f"in {tuning.get('seconds', 0)} s")
     else:
         from tide import _native
-        tuning = {"tuning": _native.get_tm2d_tuning(), "cached": False}
+        tuning = {"tuning": _native.get_tm2d_tuning(), "shot_loop": _native.get_tm2d_shot_loop(),
+                  "cached": False}
 
     disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
     storage = tide.StorageOptions(compression=args.compression == "bf16")
@@ -271,8 +273,9 @@ def main() -> None:
             "cell_updates_per_step": updates_per_step,
             "hbm_frac_whole_job": round(gcells * 1e9 * (24 + b_snap) / 1e9 / HBM_PEAK_GBS, 4),
             "misfit": float(misfit),
-            "launch_shape": {"fwd_rows_x_shots": list(tuning["tuning"][:2]),
-                             "adj_rows_x_shots": list(tuning["tuning"][2:])},
+            "launch_shape": {"fwd_rows_x_group": list(tuning["tuning"][:2]),
+                             "adj_rows_x_group": list(tuning["tuning"][2:]),
+                             "shot_loop": [int(v) for v in tuning["shot_loop"]]},
             "seed": 1234,
         },
         "roofline": roofline,

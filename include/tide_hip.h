@@ -220,6 +220,20 @@ int tide_tm2d_double_backward_f64acc(
 int tide_tm2d_set_tuning(int fwd_rows, int fwd_group, int adj_rows, int adj_group);
 int tide_tm2d_get_tuning(int *out4);
 
+/*
+ * Kernel family of the single-problem TM2D step, per pass (0 or 1): 0 puts
+ * one shot per wavefront and sums a tile's FP64 gradient terms across the
+ * workgroup in LDS; 1 gives each wavefront one tile and walks every shot of
+ * the call, summing gradient terms in registers (one read-modify-write per
+ * cell and step).  With 1 the group of tide_tm2d_set_tuning is the number
+ * of tiles per workgroup (clamped to 4).  Results are identical in both
+ * families up to the order of the FP64 shot sum.  No reference counterpart
+ * (the reference launches one thread per cell and shot,
+ * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:263-270).
+ */
+int tide_tm2d_set_shot_loop(int fwd_loop, int adj_loop);
+int tide_tm2d_get_shot_loop(int *out2);
+
 /* ------------------------------------------------------- batched 2-D plans
  * Many independent fp32 TM2D problems (different grid sizes, shot counts
  * and nt) advanced together: one launch per time step covers every problem

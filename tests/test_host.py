@@ -223,5 +223,7 @@ def test_autotune_size_classes():
     assert a != _size_class(1065, 1065, 16, torch.float32)
     assert a != _size_class(300, 300, 7, torch.float32)
     assert a != _size_class(1065, 1065, 7, torch.float64)
-    assert all(r in (2, 4, 8, 16) and 1 <= g <= 8 for r, g in FWD_CANDIDATES)
-    assert all(r in (2, 4, 8) and 1 <= g <= 8 for r, g in ADJ_CANDIDATES)
+    assert all(r in (2, 4, 8, 16) and 1 <= g <= (4 if lp else 8) and lp in (0, 1)
+               for r, g, lp in FWD_CANDIDATES)
+    assert all(r in (2, 4, 8) and 1 <= g <= (4 if lp else 8) and lp in (0, 1)
+               for r, g, lp in ADJ_CANDIDATES)

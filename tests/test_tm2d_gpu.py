@@ -10,6 +10,8 @@ oracle's exact adjoint followed by the same torch chain rule on the CPU.
 
 from __future__ import annotations
 
+import contextlib
+
 import numpy as np
 import pytest
 import torch
@@ -172,7 +174,13 @@ def test_operator_bf16_gradient():
     assert e_e < 1e-2 and e_s < 1e-2 and cos > 0.999
 
 
-def test_batched_models_equal_separate_runs():
+@pytest.mark.parametrize("loop", [0, 1])
+def test_batched_models_equal_separate_runs(loop):
+    with shot_loop(loop, loop):
+        _check_batched_models()
+
+
+def _check_batched_models():
     dev = torch.device("cuda:0")
     p0 = hetero_problem(seed=1, shots=2, ny=30, nx=34, nt=90)
     p1 = hetero_problem(seed=2, shots=2, ny=30, nx=34, nt=90)
@@ -260,10 +268,27 @@ def test_linearity_at_config2_size():
     assert relative_l2(r2.cpu().numpy(), 2.5 * r1.cpu().numpy()) < 2e-6
 
 
-@pytest.mark.parametrize("shots", [1, 5, 12])
-def test_shot_groups_match_oracle(shots):
-    """One workgroup holds up to 8 shots; 12 shots exercise two groups (the
-    FP64 gradient sums of the groups meet in atomics)."""
+@contextlib.contextmanager
+def shot_loop(fwd, adj):
+    before = tide._native.get_tm2d_shot_loop()
+    tide._native.set_tm2d_shot_loop(fwd, adj)
+    try:
+        yield
+    finally:
+        tide._native.set_tm2d_shot_loop(*before)
+
+
+@pytest.mark.parametrize("shots,loop", [(1, 0), (5, 0), (12, 0), (5, 1), (12, 1)])
+def test_shot_groups_match_oracle(shots, loop):
+    """One workgroup holds up to 8 shots; 12 shots exercise two groups (each
+    group sums into its own FP64 slice, folded afterwards).  The shot-looping
+    kernels sum every shot of a cell in registers instead."""
+    print(f"shots {shots} shot loop {loop}", flush=True)
+    with shot_loop(loop, loop):
+        _check_shot_groups(shots)
+
+
+def _check_shot_groups(shots):
     st = prepare_tm2d(**hetero_problem(shots=shots, ny=30, nx=70, nt=60, dtype=torch.float64, seed=shots))
     case = O.case_from_setup(st)
     _, r_o, es, cs = O.tm2d_forward(case, store="full")
@@ -289,14 +314,30 @@ def test_repeat_runs_are_bitwise_identical():
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("tuning", [(2, 8, 2, 8), (4, 4, 2, 4), (8, 8, 4, 8), (16, 2, 8, 2), (4, 1, 4, 3)])
-def test_every_launch_shape_matches_oracle(tuning):
-    """Any launch shape the autotuner may pick gives the oracle's forward
-    bitwise and its adjoint fields bitwise (gradients to FP64 rounding)."""
-    case, _ = load_golden("tm2d_hetero_f32")
-    print(f"tuning {tuning}", flush=True)
+@pytest.mark.parametrize("tuning,loop,name", [
+    ((2, 8, 2, 8), (0, 0), "tm2d_hetero_f32"),
+    ((4, 4, 2, 4), (0, 0), "tm2d_hetero_f32"),
+    ((8, 8, 4, 8), (0, 0), "tm2d_hetero_f32"),
+    ((16, 2, 8, 2), (0, 0), "tm2d_hetero_f32"),
+    ((4, 1, 4, 3), (0, 0), "tm2d_hetero_f32"),
+    ((2, 4, 2, 4), (1, 1), "tm2d_hetero_f32"),
+    ((4, 2, 4, 1), (1, 1), "tm2d_hetero_f32"),
+    ((8, 3, 8, 3), (1, 1), "tm2d_hetero_f32"),
+    ((16, 4, 2, 2), (1, 0), "tm2d_hetero_f32"),
+    ((4, 4, 2, 8), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 4, 4), (1, 1), "tm2d_hetero_f64"),
+])
+def test_every_launch_shape_matches_oracle(tuning, loop, name):
+    """Any launch shape the autotuner may pick, in either kernel family
+    (one shot per wavefront, or one tile per wavefront walking the shots),
+    gives the oracle's forward bitwise and its adjoint fields bitwise
+    (gradients to FP64 rounding)."""
+    case, _ = load_golden(name)
+    print(f"tuning {tuning} shot loop {loop} {name}", flush=True)
     before = tide._native.get_tm2d_tuning()
+    before_loop = tide._native.get_tm2d_shot_loop()
     tide._native.set_tm2d_tuning(*tuning)
+    tide._native.set_tm2d_shot_loop(*loop)
     try:
         f_o, r_o, es, cs = O.tm2d_forward(case, store="full")
         f_g, r_g, _, _ = gpu_forward(case, store="full")
@@ -314,3 +355,4 @@ def test_every_launch_shape_matches_oracle(tuning):
         assert relative_l2(gg[1], go[1]) < 1e-12
     finally:
         tide._native.set_tm2d_tuning(*before)
+        tide._native.set_tm2d_shot_loop(*before_loop)

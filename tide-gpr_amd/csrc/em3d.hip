@@ -71,8 +71,9 @@ template <typename T> struct Args3 {
   T *rec_out;           // r[t] (forward) or grad_f[t] (adjoint)
   double *gca, *gcb;
   double step_scale;
-  int32_t per_shot_grad, use_atomics;
+  int32_t per_shot_grad;
   int32_t n_shots, n_zb, src_comp, rec_comp;
+  int64_t group_stride;  // shot group g adds into gca/gcb + g*group_stride
 };
 
 // Snapshots are written once and read once, much later: non-temporal stores
@@ -661,14 +662,21 @@ __global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
       if (z >= 1 && z < nz) {
         double sum = 0.0;
         for (int k = 0; k < G; ++k) sum += slots[(k * TZ + r) * 64 + lane];
-        const int64_t j = ((int64_t)z * ny + y) * nx + x_;
-        if (a.use_atomics)
-          atomicAdd(&out[j], sum);
-        else
-          out[j] += sum;
+        const int64_t j = ((int64_t)z * ny + y) * nx + x_ + tc.grp * a.group_stride;
+        out[j] += sum;
       }
     }
   }
+}
+
+// out[i] += sum over g of acc[g*n + i], summed in group order.
+__global__ void fold_groups3(double *__restrict__ out, const double *__restrict__ acc, int64_t n,
+                             int groups) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sum = acc[i];
+  for (int g = 1; g < groups; ++g) sum += acc[(int64_t)g * n + i];
+  out[i] += sum;
 }
 
 // ------------------------------------------------------------------ driver
@@ -780,7 +788,7 @@ void fill_common(Args3<T> &a, const Problem3<T> &p, const DevicePointTable &dpt,
   a.rec_comp = p.rec_comp;
   a.gca = a.gcb = nullptr;
   a.per_shot_grad = 0;
-  a.use_atomics = 0;
+  a.group_stride = 0;
   a.step_scale = 1.0;
   for (int k = 0; k < 6; ++k) a.st[k] = nullptr;
   for (int k = 0; k < N_SLOTS; ++k) a.mn[k] = nullptr;
@@ -917,7 +925,21 @@ int backward_impl(Problem3<T> p, const T *grad_r, T *const *lam, const void *con
   a.per_shot_grad = ((gca && p.g.ca_b) || (gcb && p.g.cb_b)) ? 1 : 0;
   a.step_scale = (double)step_ratio;
   const int64_t n_groups = (p.n_shots + group - 1) / group;
-  a.use_atomics = n_groups > 1 ? 1 : 0;
+  // several shot groups: per-group FP64 slices, folded in group order below
+  double *gacc = nullptr, *gbcc = nullptr;
+  const int64_t numel = p.g.numel;
+  if (n_groups > 1 && !a.per_shot_grad && (gca || gcb)) {
+    a.group_stride = numel;
+    if (gca) {
+      TIDE_CHECK(scr.alloc(&gacc, (size_t)(n_groups * numel)));
+      TIDE_HIP_TRY(hipMemsetAsync(gacc, 0, (size_t)(n_groups * numel) * sizeof(double), stream));
+    }
+    if (gcb) {
+      TIDE_CHECK(scr.alloc(&gbcc, (size_t)(n_groups * numel)));
+      TIDE_HIP_TRY(hipMemsetAsync(gbcc, 0, (size_t)(n_groups * numel) * sizeof(double), stream));
+    }
+  }
+  double *const ga_k = gacc ? gacc : gca, *const gb_k = gbcc ? gbcc : gcb;
   const dim3 block(64, (unsigned)group);
   const dim3 grid((unsigned)((int64_t)dpt.n_xt * p.g.ny * a.n_zb * n_groups));
   const size_t lds = 2 * (size_t)group * tz * 64 * sizeof(double);
@@ -940,8 +962,8 @@ int backward_impl(Problem3<T> p, const T *grad_r, T *const *lam, const void *con
                     ? (void *)((const char *)stores[k] +
                                (size_t)((t - store_t0) / step_ratio) * store_step_bytes)
                     : nullptr;
-    a.gca = g_step ? gca : nullptr;
-    a.gcb = g_step ? gcb : nullptr;
+    a.gca = g_step ? ga_k : nullptr;
+    a.gcb = g_step ? gb_k : nullptr;
     if (!g_step) {
       launch_adj_h3<T, 0>(tz, grid, block, lds, stream, a);
     } else if (fmt == 2) {
@@ -952,6 +974,12 @@ int backward_impl(Problem3<T> p, const T *grad_r, T *const *lam, const void *con
     for (int k = M_EY_Z; k < N_SLOTS; ++k) std::swap(cur[k], nxt[k]);
   }
   TIDE_HIP_TRY(hipGetLastError());
+  if (gacc || gbcc) {
+    const unsigned blocks = (unsigned)((numel + 255) / 256);
+    if (gacc) fold_groups3<<<blocks, 256, 0, stream>>>(gca, gacc, numel, (int)n_groups);
+    if (gbcc) fold_groups3<<<blocks, 256, 0, stream>>>(gcb, gbcc, numel, (int)n_groups);
+    TIDE_HIP_TRY(hipGetLastError());
+  }
   for (int k = M_HZ_Y; k < N_SLOTS; ++k)
     if (cur[k] != lam[k])
       TIDE_HIP_TRY(hipMemcpyAsync(lam[k], cur[k], bytes, hipMemcpyDeviceToDevice, stream));

@@ -47,8 +47,20 @@ template <typename T> struct Problem {
 
 Tuning g_tuning;
 
+constexpr int kMaxLoopTiles = 4;  // wavefronts per workgroup of the shot-looping kernels
+
 template <typename T, int FMT>
-void launch_fwd(int rows, dim3 grid, dim3 block, hipStream_t stream, const FwdArgs<T> &a) {
+void launch_fwd(int rows, bool loop, dim3 grid, dim3 block, hipStream_t stream,
+                const FwdArgs<T> &a) {
+  if (loop) {
+    switch (rows) {
+      case 2: fwd_loop<T, 2, FMT><<<grid, block, 0, stream>>>(a); break;
+      case 4: fwd_loop<T, 4, FMT><<<grid, block, 0, stream>>>(a); break;
+      case 16: fwd_loop<T, 16, FMT><<<grid, block, 0, stream>>>(a); break;
+      default: fwd_loop<T, 8, FMT><<<grid, block, 0, stream>>>(a); break;
+    }
+    return;
+  }
   switch (rows) {
     case 2: fwd_step<T, 2, FMT><<<grid, block, 0, stream>>>(a); break;
     case 4: fwd_step<T, 4, FMT><<<grid, block, 0, stream>>>(a); break;
@@ -58,8 +70,16 @@ void launch_fwd(int rows, dim3 grid, dim3 block, hipStream_t stream, const FwdAr
 }
 
 template <typename T, int FMT>
-void launch_adj(int rows, dim3 grid, dim3 block, size_t lds, hipStream_t stream,
+void launch_adj(int rows, bool loop, dim3 grid, dim3 block, size_t lds, hipStream_t stream,
                 const AdjArgs<T> &a) {
+  if (loop) {
+    switch (rows) {
+      case 2: adj_loop<T, 2, FMT><<<grid, block, 0, stream>>>(a); break;
+      case 8: adj_loop<T, 8, FMT><<<grid, block, 0, stream>>>(a); break;
+      default: adj_loop<T, 4, FMT><<<grid, block, 0, stream>>>(a); break;
+    }
+    return;
+  }
   switch (rows) {
     case 2: adj_step<T, 2, FMT><<<grid, block, lds, stream>>>(a); break;
     case 8: adj_step<T, 8, FMT><<<grid, block, lds, stream>>>(a); break;
@@ -195,11 +215,14 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
   a.mhxz = m_hx_z;
   a.n_shots = p.n_shots;
   const int rows = rows_for<T>(g_tuning.fwd_rows);
-  const int group = std::min<int>(p.n_shots, g_tuning.fwd_group);
+  const bool loop = g_tuning.fwd_loop != 0;
+  const int group = loop ? std::min<int>(g_tuning.fwd_group, kMaxLoopTiles)
+                         : std::min<int>(p.n_shots, g_tuning.fwd_group);
   a.n_yt = (p.g.ny + rows - 1) / rows;
+  const int64_t n_tiles = (int64_t)dpt.n_xt * a.n_yt;
   const int64_t n_groups = (p.n_shots + group - 1) / group;
   const dim3 block(64, (unsigned)group);
-  const dim3 grid((unsigned)((int64_t)dpt.n_xt * a.n_yt * n_groups));
+  const dim3 grid((unsigned)(loop ? (n_tiles + group - 1) / group : n_tiles * n_groups));
 
   T *cur[5] = {ey, hx, hz, m_ey_x, m_ey_z};
   T *nxt[5] = {ey2, hx2, hz2, mex2, mez2};
@@ -216,11 +239,11 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     a.ey_store = st && ey_store ? (char *)ey_store + off : nullptr;
     a.curl_store = st && curl_store ? (char *)curl_store + off : nullptr;
     if (!st) {
-      launch_fwd<T, 0>(rows, grid, block, stream, a);
+      launch_fwd<T, 0>(rows, loop, grid, block, stream, a);
     } else if (fmt == 2) {
-      if constexpr (sizeof(T) == 4) launch_fwd<T, 2>(rows, grid, block, stream, a);
+      if constexpr (sizeof(T) == 4) launch_fwd<T, 2>(rows, loop, grid, block, stream, a);
     } else {
-      launch_fwd<T, 1>(rows, grid, block, stream, a);
+      launch_fwd<T, 1>(rows, loop, grid, block, stream, a);
     }
     for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
   }
@@ -283,15 +306,32 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
   a.step_scale = (double)step_ratio;
   a.n_shots = p.n_shots;
   const int rows = rows_for<T>(g_tuning.adj_rows);
-  const int group = std::min<int>(p.n_shots, g_tuning.adj_group);
+  const bool loop = g_tuning.adj_loop != 0;
+  const int group = loop ? std::min<int>(g_tuning.adj_group, kMaxLoopTiles)
+                         : std::min<int>(p.n_shots, g_tuning.adj_group);
   a.n_yt = (p.g.ny + rows - 1) / rows;
-  const int64_t n_groups = (p.n_shots + group - 1) / group;
-  // More shots than one workgroup holds: the per-group sums meet in FP64
-  // atomics (order-dependent in the last bit); otherwise one plain add.
-  a.use_atomics = n_groups > 1 ? 1 : 0;
+  const int64_t n_tiles = (int64_t)dpt.n_xt * a.n_yt;
+  const int64_t n_groups = loop ? 1 : (p.n_shots + group - 1) / group;
+  // More shots than one workgroup holds: each shot group accumulates into
+  // its own FP64 slice, folded into gca/gcb in group order at the end.
+  a.group_stride = 0;
+  double *gacc = nullptr, *gbcc = nullptr;
+  const int64_t numel = p.g.numel;
+  if (n_groups > 1 && !a.per_shot_grad && (gca || gcb)) {
+    a.group_stride = numel;
+    if (gca) {
+      TIDE_CHECK(scr.alloc(&gacc, (size_t)(n_groups * numel)));
+      TIDE_HIP_TRY(hipMemsetAsync(gacc, 0, (size_t)(n_groups * numel) * sizeof(double), stream));
+    }
+    if (gcb) {
+      TIDE_CHECK(scr.alloc(&gbcc, (size_t)(n_groups * numel)));
+      TIDE_HIP_TRY(hipMemsetAsync(gbcc, 0, (size_t)(n_groups * numel) * sizeof(double), stream));
+    }
+  }
+  double *const ga_k = gacc ? gacc : gca, *const gb_k = gbcc ? gbcc : gcb;
   const dim3 block(64, (unsigned)group);
-  const dim3 grid((unsigned)((int64_t)dpt.n_xt * a.n_yt * n_groups));
-  const size_t lds = 2 * (size_t)group * rows * 64 * sizeof(double);
+  const dim3 grid((unsigned)(loop ? (n_tiles + group - 1) / group : n_tiles * n_groups));
+  const size_t lds = loop ? 0 : 2 * (size_t)group * rows * 64 * sizeof(double);
 
   // order: ley, lhx, lhz, mx (hz_x), my (hx_z), nx (ey_x), nz (ey_z)
   T *cur[7] = {l_ey, l_hx, l_hz, ml_hz_x, ml_hx_z, ml_ey_x, ml_ey_z};
@@ -307,17 +347,21 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     const size_t off = (size_t)(t / step_ratio) * store_step_bytes;
     a.ey_store = g_step && gca ? (const char *)ey_store + off : nullptr;
     a.curl_store = g_step && gcb ? (const char *)curl_store + off : nullptr;
-    a.gca = g_step ? gca : nullptr;
-    a.gcb = g_step ? gcb : nullptr;
+    a.gca = g_step ? ga_k : nullptr;
+    a.gcb = g_step ? gb_k : nullptr;
     if (!g_step) {
-      launch_adj<T, 0>(rows, grid, block, lds, stream, a);
+      launch_adj<T, 0>(rows, loop, grid, block, lds, stream, a);
     } else if (fmt == 2) {
-      if constexpr (sizeof(T) == 4) launch_adj<T, 2>(rows, grid, block, lds, stream, a);
+      if constexpr (sizeof(T) == 4) launch_adj<T, 2>(rows, loop, grid, block, lds, stream, a);
     } else {
-      launch_adj<T, 1>(rows, grid, block, lds, stream, a);
+      launch_adj<T, 1>(rows, loop, grid, block, lds, stream, a);
     }
     for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
   }
+  TIDE_HIP_TRY(hipGetLastError());
+  const unsigned fold_blocks = (unsigned)((numel + 255) / 256);
+  if (gacc) fold_groups<<<fold_blocks, 256, 0, stream>>>(gca, gacc, numel, (int)n_groups);
+  if (gbcc) fold_groups<<<fold_blocks, 256, 0, stream>>>(gcb, gbcc, numel, (int)n_groups);
   TIDE_HIP_TRY(hipGetLastError());
   T *orig[7] = {l_ey, l_hx, l_hz, ml_hz_x, ml_hx_z, ml_ey_x, ml_ey_z};
   for (int k = 0; k < 7; ++k)
@@ -603,7 +647,30 @@ extern "C" int tide_tm2d_set_tuning(int fwd_rows, int fwd_group, int adj_rows, i
     set_error("tuning: rows must be fwd {2,4,8,16}, adj {2,4,8}; groups 1..8");
     return kInvalidArgument;
   }
-  g_tuning = Tuning{fwd_rows, fwd_group, adj_rows, adj_group};
+  g_tuning.fwd_rows = fwd_rows;
+  g_tuning.fwd_group = fwd_group;
+  g_tuning.adj_rows = adj_rows;
+  g_tuning.adj_group = adj_group;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_set_shot_loop(int fwd_loop, int adj_loop) {
+  if ((fwd_loop != 0 && fwd_loop != 1) || (adj_loop != 0 && adj_loop != 1)) {
+    set_error("shot loop flags must be 0 or 1");
+    return kInvalidArgument;
+  }
+  g_tuning.fwd_loop = fwd_loop;
+  g_tuning.adj_loop = adj_loop;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_get_shot_loop(int *out2) {
+  if (out2 == nullptr) {
+    set_error("tuning: null output");
+    return kInvalidArgument;
+  }
+  out2[0] = g_tuning.fwd_loop;
+  out2[1] = g_tuning.adj_loop;
   return kOk;
 }
 

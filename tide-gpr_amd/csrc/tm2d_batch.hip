@@ -41,8 +41,9 @@ template <typename T> struct BatchItem {
   T *lam[2][7];  // [parity]: ley, lhx, lhz, m_hz_x, m_hx_z, m_ey_x, m_ey_z
   const T *grad_r;
   T *grad_f;
-  double *gca, *gcb;
-  int32_t n_shots, n_src, n_rec, n_yt, nt, interval, use_atomics, tile0;
+  double *gca, *gcb;           // what the kernels add to (group slices when n_groups > 1)
+  int64_t group_stride;        // elements between group slices (0: one group)
+  int32_t n_shots, n_src, n_rec, n_yt, nt, interval, n_groups, tile0;
 };
 
 __device__ __forceinline__ TileCoord batch_tile(int local, int n_xt, int n_yt) {
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(512) void adj_batch(const BatchItem<T> *__restrict_
                                            : nullptr;
   a.n_shots = it.n_shots;
   a.n_yt = it.n_yt;
-  a.use_atomics = it.use_atomics;
+  a.group_stride = it.group_stride;
   adj_workgroup<T, TY, FMT>(a, tc, red);
 }
 
@@ -269,6 +270,10 @@ int enqueue_backward(Plan &p, hipStream_t s) {
       for (int c = 0; c < 7; ++c) launch_zero(it.lam[par][c], total, s);
     launch_zero(q.grad_ca, q.ny * q.nx, s);
     launch_zero(q.grad_cb, q.ny * q.nx, s);
+    if (it.group_stride > 0) {
+      if (it.gca) launch_zero(it.gca, it.n_groups * it.group_stride, s);
+      if (it.gcb) launch_zero(it.gcb, it.n_groups * it.group_stride, s);
+    }
     if (q.grad_f && q.n_src > 0) launch_zero(q.grad_f, q.nt * q.n_shots * q.n_src, s);
   }
   for (int k = 0; k < p.max_nt; ++k) {
@@ -277,6 +282,15 @@ int enqueue_backward(Plan &p, hipStream_t s) {
     if (p.fmt_adj == 0) launch_adj_b<0>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
     else if (p.fmt_adj == 2) launch_adj_b<2>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
     else launch_adj_b<1>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
+  }
+  for (size_t m = 0; m < p.probs.size(); ++m) {
+    const BatchItem<float> &it = p.h_adj[m];
+    if (it.group_stride == 0) continue;
+    const tide_tm2d_problem &q = p.probs[m];
+    const int64_t numel = q.ny * q.nx;
+    const unsigned blocks = (unsigned)((numel + 255) / 256);
+    if (it.gca) fold_groups<<<blocks, 256, 0, s>>>(q.grad_ca, it.gca, numel, it.n_groups);
+    if (it.gcb) fold_groups<<<blocks, 256, 0, s>>>(q.grad_cb, it.gcb, numel, it.n_groups);
   }
   for (size_t m = 0; m < p.probs.size(); ++m) {
     const tide_tm2d_problem &q = p.probs[m];
@@ -332,7 +346,7 @@ int build_items(Plan &p, int rows, std::vector<BatchItem<float>> &items,
     BatchItem<float> &it = items[m];
     it.n_yt = (int32_t)((p.probs[m].ny + rows - 1) / rows);
     const int64_t n_groups = (it.n_shots + p.group - 1) / p.group;
-    it.use_atomics = n_groups > 1 ? 1 : 0;
+    it.n_groups = (int32_t)n_groups;
     const int64_t tiles = (int64_t)it.pts.n_xt * it.n_yt * n_groups;
     if (tile0 + tiles >= ((int64_t)1 << 31)) {
       set_error("batch: too many tiles for one launch");
@@ -515,6 +529,17 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
   std::vector<int32_t> tiles_f, tiles_a;
   PLAN_CHECK(build_items(*p, p->rows_f, p->h_fwd, tiles_f, p->active_fwd, proto));
   PLAN_CHECK(build_items(*p, p->rows_a, p->h_adj, tiles_a, p->active_adj, proto));
+  // problems with several shot groups accumulate into per-group FP64 slices
+  for (size_t m = 0; m < p->probs.size(); ++m) {
+    BatchItem<float> &it = p->h_adj[m];
+    it.group_stride = 0;
+    if (it.n_groups > 1 && (it.gca || it.gcb)) {
+      const int64_t numel = p->probs[m].ny * p->probs[m].nx;
+      it.group_stride = numel;
+      if (it.gca) PLAN_CHECK(plan_alloc(*p, &it.gca, (size_t)(it.n_groups * numel)));
+      if (it.gcb) PLAN_CHECK(plan_alloc(*p, &it.gcb, (size_t)(it.n_groups * numel)));
+    }
+  }
   PLAN_CHECK(plan_alloc(*p, &p->d_fwd, p->h_fwd.size()));
   PLAN_CHECK(plan_alloc(*p, &p->d_adj, p->h_adj.size()));
   PLAN_CHECK(plan_alloc(*p, &p->d_tiles_fwd, tiles_f.size()));

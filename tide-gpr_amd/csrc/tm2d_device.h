@@ -64,7 +64,10 @@ template <typename T> struct AdjArgs {
   const T *grad_r_t;
   T *grad_f_t;
   int32_t n_shots, n_yt;
-  int32_t use_atomics;
+  // With several shot groups, group g adds its sums to gca/gcb + g*group_stride
+  // (separate FP64 slices, summed in group order after the time loop), so
+  // no two workgroups update the same cell: no atomics, bitwise repeatable.
+  int64_t group_stride;
 };
 
 // Snapshots are written once and read once, much later: non-temporal stores
@@ -305,7 +308,10 @@ __global__ __launch_bounds__(512) void fwd_step(FwdArgs<T> a) {
 // ----------------------------------------------------------------- adjoint
 // One reverse iteration: transpose of the E step, transpose of the H
 // half-step, receiver injection, source read-out, coefficient gradient.
-template <typename T, int TY, int FMT, bool EDGE>
+// SINK 0: the gradient terms of shot s go to the LDS slots red_*[r*64+lane]
+// (summed over the workgroup's shots afterwards); SINK 1: they are added to
+// the per-thread accumulators red_*[r] (the caller walks the shots).
+template <typename T, int TY, int FMT, bool EDGE, int SINK = 0>
 __device__ __forceinline__ void adj_tile(const AdjArgs<T> &a, int s, int xt, int yt,
                                          double *red_ca, double *red_cb, bool reduce) {
   const Geom<T> &g = a.g;
@@ -487,8 +493,13 @@ __device__ __forceinline__ void adj_tile(const AdjArgs<T> &a, int s, int xt, int
       const double pa = lm * ve * a.step_scale;
       const double pb = lm * vc * a.step_scale;
       if (reduce) {
-        red_ca[r * 64 + lane] = pa;
-        red_cb[r * 64 + lane] = pb;
+        if constexpr (SINK == 0) {
+          red_ca[r * 64 + lane] = pa;
+          red_cb[r * 64 + lane] = pb;
+        } else {
+          red_ca[r] += pa;
+          red_cb[r] += pb;
+        }
       } else {
         const int64_t gi = (a.per_shot_grad ? base : 0) + j;
         if (a.gca) a.gca[gi] += pa;
@@ -542,6 +553,7 @@ __device__ __forceinline__ void adj_workgroup(const AdjArgs<T> &a, const TileCoo
   for (int pass = 0; pass < 2; ++pass) {
     double *out = pass == 0 ? a.gca : a.gcb;
     if (out == nullptr) continue;
+    out += tc.grp * a.group_stride;
     const double *slots = red + (int64_t)pass * G * TY * 64;
     for (int r = w; r < TY; r += G) {
       const int y = y0 + r;
@@ -549,10 +561,7 @@ __device__ __forceinline__ void adj_workgroup(const AdjArgs<T> &a, const TileCoo
         double sum = 0.0;
         for (int k = 0; k < G; ++k) sum += slots[(k * TY + r) * 64 + lane];
         const int64_t j = (int64_t)y * a.g.nx + x;
-        if (a.use_atomics)
-          atomicAdd(&out[j], sum);
-        else
-          out[j] += sum;
+        out[j] += sum;
       }
     }
   }
@@ -564,6 +573,79 @@ __global__ __launch_bounds__(512) void adj_step(AdjArgs<T> a) {
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
   adj_workgroup<T, TY, FMT>(a, decode_tile(a.pts.n_xt, a.n_yt, n_groups), red);
+}
+
+// ------------------------------------------------------ shot-looping kernels
+// One wavefront per tile walking every shot of the call: workgroup b takes
+// tiles b*W .. b*W+W-1 (W = blockDim.y), no LDS and no barrier.  The
+// adjoint sums each cell's FP64 gradient terms over the shots in registers
+// (shot order) and adds them once per cell and step.
+__device__ __forceinline__ int loop_tile(int n_tiles) {
+  const int64_t blk = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  const int64_t tile = blk * blockDim.y + threadIdx.y;
+  return tile < n_tiles ? (int)tile : -1;
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(256) void fwd_loop(FwdArgs<T> a) {
+  const int n_xt = a.pts.n_xt;
+  const int tile = loop_tile(n_xt * a.n_yt);
+  if (tile < 0) return;
+  const int yt = tile / n_xt, xt = tile - yt * n_xt;
+  const bool interior = interior_tile<T, TY>(a.g, xt, yt);
+  for (int s = 0; s < a.n_shots; ++s) {
+    if (interior)
+      fwd_tile<T, TY, FMT, false>(a, s, xt, yt);
+    else
+      fwd_tile<T, TY, FMT, true>(a, s, xt, yt);
+  }
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(256) void adj_loop(AdjArgs<T> a) {
+  const int n_xt = a.pts.n_xt;
+  const int tile = loop_tile(n_xt * a.n_yt);
+  if (tile < 0) return;
+  const int yt = tile / n_xt, xt = tile - yt * n_xt;
+  const bool want_grad = (a.gca != nullptr || a.gcb != nullptr) && FMT != 0;
+  const bool reduce = want_grad && !a.per_shot_grad;
+  double acc_ca[TY], acc_cb[TY];
+#pragma unroll
+  for (int r = 0; r < TY; ++r) {
+    acc_ca[r] = 0.0;
+    acc_cb[r] = 0.0;
+  }
+  const bool interior = interior_tile<T, TY>(a.g, xt, yt);
+  for (int s = 0; s < a.n_shots; ++s) {
+    if (interior)
+      adj_tile<T, TY, FMT, false, 1>(a, s, xt, yt, acc_ca, acc_cb, reduce);
+    else
+      adj_tile<T, TY, FMT, true, 1>(a, s, xt, yt, acc_ca, acc_cb, reduce);
+  }
+  if (!reduce) return;
+  const int lane = threadIdx.x;
+  const int x = xt * kOwn - 1 + lane;
+  if (lane < 1 || lane > kOwn || x < 1 || x >= a.g.nx) return;
+  const int y0 = yt * TY;
+#pragma unroll
+  for (int r = 0; r < TY; ++r) {
+    const int y = y0 + r;
+    if (y >= 1 && y < a.g.ny) {
+      const int64_t j = (int64_t)y * a.g.nx + x;
+      if (a.gca) a.gca[j] += acc_ca[r];
+      if (a.gcb) a.gcb[j] += acc_cb[r];
+    }
+  }
+}
+
+// out[i] += sum over g of acc[g*n + i], summed in group order.
+static __global__ void fold_groups(double *__restrict__ out, const double *__restrict__ acc, int64_t n,
+                            int groups) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sum = acc[i];
+  for (int g = 1; g < groups; ++g) sum += acc[(int64_t)g * n + i];
+  out[i] += sum;
 }
 
 // Adds an FP64 accumulator into the caller's gradient array (reference
@@ -580,8 +662,11 @@ constexpr int kMaxGroup = 8;  // shots (wavefronts) per workgroup
 // workgroup for each kernel.  Process-wide (defined in tm2d.hip); set by the
 // autotuner through tide_tm2d_set_tuning().  fp64 fields use half the rows
 // (twice the registers per value).
+// fwd_loop / adj_loop select the shot-looping kernels; their group is then
+// the number of tiles (wavefronts) per workgroup.
 struct Tuning {
   int fwd_rows = 4, fwd_group = 4, adj_rows = 2, adj_group = 4;
+  int fwd_loop = 0, adj_loop = 0;
 };
 extern Tuning g_tuning;
 

@@ -227,6 +227,10 @@ def load() -> ctypes.CDLL:
     lib.tide_tm2d_set_tuning.argtypes = [ctypes.c_int] * 4
     lib.tide_tm2d_get_tuning.restype = ctypes.c_int
     lib.tide_tm2d_get_tuning.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    lib.tide_tm2d_set_shot_loop.restype = ctypes.c_int
+    lib.tide_tm2d_set_shot_loop.argtypes = [ctypes.c_int] * 2
+    lib.tide_tm2d_get_shot_loop.restype = ctypes.c_int
+    lib.tide_tm2d_get_shot_loop.argtypes = [ctypes.POINTER(ctypes.c_int)]
     _lib = lib
     return lib
 
@@ -292,6 +296,17 @@ def get_tm2d_tuning() -> tuple[int, int, int, int]:
     return tuple(out)  # type: ignore[return-value]
 
 
+def set_tm2d_shot_loop(fwd_loop: bool, adj_loop: bool) -> None:
+    """Select the shot-looping TM2D kernels per pass (see include/tide_hip.h)."""
+    check(load().tide_tm2d_set_shot_loop(int(bool(fwd_loop)), int(bool(adj_loop))))
+
+
+def get_tm2d_shot_loop() -> tuple[bool, bool]:
+    out = (ctypes.c_int * 2)()
+    check(load().tide_tm2d_get_shot_loop(out))
+    return bool(out[0]), bool(out[1])
+
+
 def stream_handle(device: torch.device) -> int | None:
     """The HIP stream of torch's current stream on ``device``."""
     return torch.cuda.current_stream(device).cuda_stream
@@ -301,10 +316,12 @@ __all__ = [
     "TideNativeError",
     "check",
     "dtype_name",
+    "get_tm2d_shot_loop",
     "get_tm2d_tuning",
     "int64_array",
     "ptr_array",
     "set_em3d_tuning",
+    "set_tm2d_shot_loop",
     "set_tm2d_tuning",
     "get_function",
     "library_path",

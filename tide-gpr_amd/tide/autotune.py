@@ -1,13 +1,17 @@
 """Launch-shape autotuner for the TM2D step kernels.
 
-The forward and adjoint kernels take two launch parameters: rows marched
-per wavefront (the tile height) and shots per workgroup (wavefronts that
-share a tile's coefficient lines and reduce the FP64 gradient in LDS).  The
-best pair depends on the grid, the shot count and the pass (the adjoint
-carries more state per row).  :func:`tune_tm2d` times every candidate on a
-synthetic problem of the caller's size with HIP events, installs the
-fastest pair per pass (``tide_tm2d_set_tuning``) and caches it per size
-class, so later calls with a similar problem reuse the choice.
+The forward and adjoint kernels come in two families and take two launch
+parameters.  Family 0 puts one shot on each wavefront: a workgroup of G
+wavefronts shares a tile (its coefficient lines) across G shots and sums the
+FP64 gradient in LDS.  Family 1 (shot loop) gives each wavefront one tile and
+walks every shot of the call, summing the gradient in registers; G is then
+tiles per workgroup.  Both take the rows marched per wavefront (the tile
+height).  The best (rows, G, family) depends on the grid, the shot count and
+the pass (the adjoint carries more state per row).  :func:`tune_tm2d` times
+every candidate on a synthetic problem of the caller's size with HIP events,
+installs the fastest per pass (``tide_tm2d_set_tuning``,
+``tide_tm2d_set_shot_loop``) and caches it per size class, so later calls
+with a similar problem reuse the choice.
 """
 
 from __future__ import annotations
@@ -21,10 +25,20 @@ from . import _native
 from .maxwell.tm2d import prepare_tm2d
 from .utils import C0
 
-FWD_CANDIDATES: tuple[tuple[int, int], ...] = ((2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (8, 4), (8, 8), (16, 4))
-ADJ_CANDIDATES: tuple[tuple[int, int], ...] = ((2, 2), (2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (8, 2))
+# (rows, group, shot loop)
+FWD_CANDIDATES: tuple[tuple[int, int, int], ...] = (
+    (2, 4, 0), (2, 8, 0), (4, 2, 0), (4, 4, 0), (4, 8, 0), (8, 4, 0), (8, 8, 0), (16, 4, 0),
+    (2, 4, 1), (4, 2, 1), (4, 4, 1), (8, 2, 1))
+ADJ_CANDIDATES: tuple[tuple[int, int, int], ...] = (
+    (2, 2, 0), (2, 4, 0), (2, 8, 0), (4, 2, 0), (4, 4, 0), (4, 8, 0), (8, 2, 0),
+    (2, 2, 1), (2, 4, 1), (4, 2, 1), (4, 4, 1))
 
-_cache: dict[tuple, tuple[int, int, int, int]] = {}
+_cache: dict[tuple, tuple[tuple[int, int, int, int], tuple[int, int]]] = {}
+
+
+def _install(tuning: tuple[int, int, int, int], loop: tuple[int, int]) -> None:
+    _native.set_tm2d_tuning(*tuning)
+    _native.set_tm2d_shot_loop(*loop)
 
 
 def _size_class(ny: int, nx: int, n_shots: int, dtype: torch.dtype) -> tuple:
@@ -72,18 +86,21 @@ def tune_tm2d(
     dtype: torch.dtype = torch.float32,
     nt: int = 48,
     device: torch.device | str = "cuda:0",
-    fwd_candidates: Sequence[tuple[int, int]] = FWD_CANDIDATES,
-    adj_candidates: Sequence[tuple[int, int]] = ADJ_CANDIDATES,
+    fwd_candidates: Sequence[tuple[int, int, int]] = FWD_CANDIDATES,
+    adj_candidates: Sequence[tuple[int, int, int]] = ADJ_CANDIDATES,
     use_cache: bool = True,
 ) -> dict:
-    """Pick and install the fastest (rows, shots-per-workgroup) for the
+    """Pick and install the fastest (rows, group, shot loop) for the
     forward-with-snapshots and adjoint kernels on a padded ``ny x nx`` grid
-    with ``n_shots`` shots per call.  Returns the choice and the timings."""
+    with ``n_shots`` shots per call.  Returns the choice (``tuning`` =
+    (fwd rows, fwd group, adj rows, adj group), ``shot_loop`` = (fwd, adj))
+    and the timings."""
     device = torch.device(device)
     key = _size_class(ny, nx, n_shots, dtype)
     if use_cache and key in _cache:
-        _native.set_tm2d_tuning(*_cache[key])
-        return {"tuning": _cache[key], "cached": True}
+        tuning, loop = _cache[key]
+        _install(tuning, loop)
+        return {"tuning": tuning, "shot_loop": loop, "cached": True}
     t_start = time.perf_counter()
     st = _problem(ny, nx, n_shots, nt, dtype, device)
     lib = _native.load()
@@ -119,26 +136,33 @@ def tune_tm2d(
             device.index or 0, stream))
 
     before = _native.get_tm2d_tuning()
+    before_loop = _native.get_tm2d_shot_loop()
     run_fwd()
     run_bwd()
     torch.cuda.synchronize()
     f_times, a_times = {}, {}
-    for rows, grp in fwd_candidates:
-        _native.set_tm2d_tuning(rows, grp, before[2], before[3])
-        f_times[(rows, grp)] = _time(run_fwd)
+    for rows, grp, loop in fwd_candidates:
+        _install((rows, grp, before[2], before[3]), (loop, before_loop[1]))
+        f_times[(rows, grp, loop)] = _time(run_fwd)
     best_f = min(f_times, key=f_times.get)
-    for rows, grp in adj_candidates:
-        _native.set_tm2d_tuning(best_f[0], best_f[1], rows, grp)
-        a_times[(rows, grp)] = _time(run_bwd)
+    for rows, grp, loop in adj_candidates:
+        _install((best_f[0], best_f[1], rows, grp), (best_f[2], loop))
+        a_times[(rows, grp, loop)] = _time(run_bwd)
     best_a = min(a_times, key=a_times.get)
     choice = (best_f[0], best_f[1], best_a[0], best_a[1])
-    _native.set_tm2d_tuning(*choice)
-    _cache[key] = choice
+    loop_choice = (best_f[2], best_a[2])
+    _install(choice, loop_choice)
+    _cache[key] = (choice, loop_choice)
+
+    def label(k):
+        return f"{k[0]}x{k[1]}{'L' if k[2] else ''}"
+
     return {
         "tuning": choice,
+        "shot_loop": loop_choice,
         "cached": False,
-        "forward_us_per_step": {f"{k[0]}x{k[1]}": round(v * 1e3 / nt, 2) for k, v in f_times.items()},
-        "adjoint_us_per_step": {f"{k[0]}x{k[1]}": round(v * 1e3 / nt, 2) for k, v in a_times.items()},
+        "forward_us_per_step": {label(k): round(v * 1e3 / nt, 2) for k, v in f_times.items()},
+        "adjoint_us_per_step": {label(k): round(v * 1e3 / nt, 2) for k, v in a_times.items()},
         "seconds": round(time.perf_counter() - t_start, 2),
     }
 
