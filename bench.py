@@ -60,6 +60,11 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--cpu-sample-nt", type=int, default=3000)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device copy (profiler byte-counter calibration)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="served-solve trace length (0: skip the serving measurement)")
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-seed", type=int, default=2024)
+    ap.add_argument("--serve-tick", type=int, default=32, help="steps per engine tick")
     return ap.parse_args()
 
 
@@ -112,6 +117,107 @@ def cpu_baseline(args, setup_kwargs) -> dict:
         "kind": kind,
         "sample": f"{what}: {n_shots} shots x {case.ny}x{case.nx} x nt {nt} "
                   f"({updates:.3e} cell-updates, {secs:.1f} s)",
+    }
+
+
+def _request_updates(kw) -> float:
+    """Useful cell-updates of one served request (forward: one sweep;
+    gradient: forward + adjoint)."""
+    from tide.serve import padded_shape
+
+    p = kw["problem"]
+    ny_p, nx_p = padded_shape(*p["epsilon"].shape, p.get("pml_width", 20))
+    shots, _, nt = p["source_amplitude"].shape
+    return (2.0 if kw["kind"] == "gradient" else 1.0) * ny_p * nx_p * shots * nt
+
+
+def _plain_solve(kw, dev, budget: int) -> None:
+    """One request through the public operator, one shot chunk at a time
+    (chunks sized by the same HBM rule as the server)."""
+    import torch
+
+    from tide.maxwell.tm2d import maxwelltm
+    from tide.serve import job_bytes, padded_shape, plan_chunks
+
+    p = {k: (v.to(dev) if hasattr(v, "to") and k not in ("source_location", "receiver_location") else v)
+         for k, v in kw["problem"].items()}
+    amp = p["source_amplitude"]
+    shots, n_src, nt = amp.shape
+    n_rec = p["receiver_location"].shape[1]
+    ny_p, nx_p = padded_shape(*p["epsilon"].shape, p.get("pml_width", 20))
+    grad = kw["kind"] == "gradient"
+    bf16 = kw.get("storage_compression") in (True, "bf16")
+    chunks = plan_chunks(shots, lambda c: job_bytes(ny_p, nx_p, c, nt, n_src, n_rec, gradient=grad,
+                                                    bf16=bf16), budget)
+    if grad:
+        obs = kw["observed"].to(dev)
+        e = p["epsilon"].clone().requires_grad_(True)
+        s = p["sigma"].clone().requires_grad_(True)
+    for s0, s1 in chunks:
+        sub = dict(p, source_amplitude=amp[s0:s1], source_location=p["source_location"][s0:s1],
+                   receiver_location=p["receiver_location"][s0:s1])
+        if not grad:
+            with torch.no_grad():
+                maxwelltm(**sub)
+            continue
+        r = maxwelltm(**dict(sub, epsilon=e, sigma=s),
+                      storage_compression="bf16" if bf16 else False)[-1]
+        loss = 0.5 * ((r.double() - obs[:, s0:s1].double()) ** 2).sum()
+        loss.backward()
+
+
+def serve_bench(args, dev) -> dict:
+    """Served solves: a seeded Poisson trace of forward (70%) and gradient
+    (30%) requests replayed in real time through the continuous-batching
+    server, then through the plain per-request path (one request at a time,
+    first come first served, on the same arrival times)."""
+    import torch
+
+    from tide.serve import TM2DServer, latency_stats, served_trace
+
+    trace = served_trace(args.serve_requests, args.serve_rate, seed=args.serve_seed)
+    work = sum(_request_updates(kw) for _, kw in trace)
+    log(f"serving trace: {len(trace)} requests, {work:.3e} cell-updates")
+    torch.cuda.empty_cache()
+    server = TM2DServer(dev, steps_per_tick=args.serve_tick)
+    budget = server.budget
+    # first contact with the kernels and allocator outside the measurement
+    server.serve([(0.0, dict(trace[0][1]))])
+    tickets = server.serve(trace)
+    torch.cuda.synchronize()
+    served = latency_stats(tickets)
+    served["gcell_per_s"] = work / served["span_s"] / 1e9
+    log(f"served: {served} in {server.ticks} ticks")
+    server.close()
+    del server
+    torch.cuda.empty_cache()
+
+    lat, t_free, first = [], 0.0, trace[0][0]
+    for arrival, kw in trace:
+        t0 = time.perf_counter()
+        _plain_solve(kw, dev, budget)
+        torch.cuda.synchronize()
+        svc = time.perf_counter() - t0
+        start = max(arrival, t_free)
+        t_free = start + svc
+        lat.append(t_free - arrival)
+    lat.sort()
+    pct = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]  # noqa: E731
+    plain = {"solves": len(lat), "solves_per_s": len(lat) / (t_free - first), "p50_s": pct(0.5),
+             "p99_s": pct(0.99), "mean_s": sum(lat) / len(lat), "span_s": t_free - first,
+             "gcell_per_s": work / (t_free - first) / 1e9}
+    log(f"plain: {plain}")
+    r3 = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
+    return {
+        "workload": f"{len(trace)} Poisson arrivals at {args.serve_rate}/s (seed {args.serve_seed}), "
+                    "70% forward / 30% forward+gradient, sides log-uniform 128-1536, nt 500-6000, "
+                    "1-32 shots, 64 receivers, fp32 snapshots",
+        "cell_updates": work,
+        "hbm_budget_bytes": budget,
+        "served": r3(served),
+        "plain_one_at_a_time": r3(plain),
+        "solves_per_s_ratio": round(served["solves_per_s"] / plain["solves_per_s"], 3),
+        "p99_ratio_plain_over_served": round(plain["p99_s"] / served["p99_s"], 3),
     }
 
 
@@ -252,6 +358,8 @@ def main() -> None:
     )
     log("timing the CPU baseline" if not args.no_cpu_baseline else "CPU baseline skipped")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, setup_kwargs)
+    del ops, observed, e, s, setup_kwargs
+    serving = serve_bench(args, dev) if args.serve_requests > 0 else None
     out = {
         "metric": METRIC,
         "value": round(gcells, 3),
@@ -280,6 +388,7 @@ def main() -> None:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

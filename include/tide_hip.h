@@ -276,6 +276,49 @@ int tide_tm2d_plan_forward(void *plan, void *stream);
 int tide_tm2d_plan_backward(void *plan, void *stream);
 int tide_tm2d_plan_destroy(void *plan);
 
+/* ------------------------------------------------- 2-D serving engine
+ * Continuous batching of independent fp32 TM2D solves that arrive and
+ * leave at any time.  A job is one problem (or one shot chunk of a request)
+ * whose device arrays the caller allocates and keeps alive until the job is
+ * removed; the engine itself allocates per job only its packed CPML
+ * profiles and source/receiver table (stream-ordered, released at removal).
+ * sources_i / receivers_i must be host arrays here.  All calls of one
+ * engine use one stream.
+ * Every tick advances every running job by up to `max_steps` steps of its
+ * current phase with ONE launch per step: each workgroup reads its job's
+ * descriptor and runs that job's forward step or adjoint iteration, so
+ * forward-only jobs, forward passes of gradient jobs and adjoints share the
+ * same launches.  Job phases: 0 forward running, 1 forward done, 2 adjoint
+ * running, 3 adjoint done.  A job with lambda[0] == NULL is forward-only.
+ * Results are the kernels of the single-problem path with the same shot
+ * grouping: receivers and gradients do not depend on what else runs.
+ * No reference counterpart (the reference solves one request per call,
+ * src/tide/maxwell/tm2d.py:258-745); the calls are not thread-safe.
+ */
+typedef struct tide_tm2d_job {
+  tide_tm2d_problem p;       /* fields[0..4] / lambda: parity-0 buffers, zero at start */
+  float *fields_b[5];        /* parity-1 ey hx hz m_ey_x m_ey_z, zero at start */
+  float *lambda_b[7];        /* parity-1 adjoint state, same order as lambda, zero at start */
+  double *grad_ca_groups;    /* [ceil(n_shots/group), ny, nx] zeroed FP64, when n_shots > group */
+  double *grad_cb_groups;
+  int64_t storage_format;    /* 0 fp32 snapshots, 1 BF16 */
+} tide_tm2d_job;
+
+int tide_tm2d_engine_create(int64_t device, int64_t group, int64_t max_jobs, void **engine);
+/* shots per workgroup (sizes grad_*_groups) */
+int tide_tm2d_engine_group(void *engine, int64_t *group);
+int tide_tm2d_engine_add(void *engine, const tide_tm2d_job *job, int64_t *job_id);
+/* Enqueues up to max_steps steps of every running job on `stream`;
+ * *n_running = jobs still running after it. */
+int tide_tm2d_engine_tick(void *engine, int64_t max_steps, void *stream, int64_t *n_running);
+int tide_tm2d_engine_query(void *engine, int64_t job_id, int64_t *phase, int64_t *step);
+/* phase 1 -> 2; grad_r must hold the receiver cotangents by then (stream order). */
+int tide_tm2d_engine_begin_adjoint(void *engine, int64_t job_id);
+/* Forgets the job; after a finished adjoint enqueues the fold of the group
+ * slices into grad_ca / grad_cb on `stream`. */
+int tide_tm2d_engine_remove(void *engine, int64_t job_id, void *stream);
+int tide_tm2d_engine_destroy(void *engine);
+
 /* ------------------------------------------------------------------- 3-D
  * Fields are [n_shots, nz, ny, nx] (x fastest).  `fields` / `lambda` point
  * to 18 device arrays in the reference's state order:

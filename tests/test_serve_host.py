@@ -1,0 +1,102 @@
+"""Host logic of the serving path (CPU): the FP64 closed-form model-gradient
+chain rule against torch autograd through the operator's own preparation,
+shot chunking under an HBM budget, the Poisson request trace and the
+latency statistics."""
+
+from __future__ import annotations
+
+import math
+
+import pytest
+import torch
+
+import tide
+from tide.maxwell.tm2d import prepare_tm2d
+from tide.serve import Ticket, job_bytes, latency_stats, model_gradient, padded_shape, plan_chunks, served_trace
+
+
+@pytest.mark.parametrize("pml", [[3, 4, 2, 5], 6])
+def test_model_gradient_equals_autograd_of_the_preparation(pml):
+    """dJ/deps, dJ/dsigma from (dJ/dca, dJ/dcb, dJ/df) equal autograd through
+    padding, coefficient compilation and source scaling (FP64)."""
+    g = torch.Generator().manual_seed(0)
+    ny, nx, S, nt = 17, 23, 3, 11
+    eps = (1 + 8 * torch.rand(ny, nx, generator=g, dtype=torch.float64)).requires_grad_(True)
+    sig = (0.01 * torch.rand(ny, nx, generator=g, dtype=torch.float64)).requires_grad_(True)
+    mu = torch.ones(ny, nx, dtype=torch.float64)
+    dx = 0.01
+    dt = 0.9 * dx / (2**0.5 * tide.C0)
+    amp = torch.randn(S, 2, nt, generator=g, dtype=torch.float64)
+    src = torch.tensor([[[2, 3], [5, 7]], [[1, 1], [4, 9]], [[3, 3], [0, 0]]])
+    rec = torch.tensor([[[1, 2]]] * S)
+    st = prepare_tm2d(eps, sig, mu, dx, dt, amp, src, rec, pml_width=pml, nt=nt)
+    gca = torch.randn(st.ny, st.nx, generator=g, dtype=torch.float64)
+    gcb = torch.randn(st.ny, st.nx, generator=g, dtype=torch.float64)
+    gf = torch.randn(nt, S, 2, generator=g, dtype=torch.float64)
+    torch.autograd.backward([st.ca, st.cb, st.f], [gca.reshape(st.ca.shape), gcb.reshape(st.cb.shape), gf])
+    st2 = prepare_tm2d(eps.detach(), sig.detach(), mu, dx, dt, amp, src, rec, pml_width=pml, nt=nt)
+    ge, gs = model_gradient(gca, gcb, gf, st2, (ny, nx), amp, dx)
+    assert torch.allclose(ge, eps.grad, rtol=1e-13, atol=0)
+    assert torch.allclose(gs, sig.grad, rtol=1e-13, atol=0)
+    # a dropped source term is caught
+    ge0, _ = model_gradient(gca, gcb, None, st2, (ny, nx), amp, dx)
+    assert not torch.allclose(ge0, eps.grad, rtol=1e-6, atol=0)
+
+
+def test_padded_shape_and_job_bytes():
+    assert padded_shape(1024, 1024, 20) == (1065, 1065)
+    assert padded_shape(10, 12, [1, 2, 3, 4]) == (14, 20)
+    f = job_bytes(100, 120, 4, 50, 1, 8, gradient=False)
+    g = job_bytes(100, 120, 4, 50, 1, 8, gradient=True)
+    h = job_bytes(100, 120, 4, 50, 1, 8, gradient=True, bf16=True)
+    assert f < h < g
+    # fields + memories dominate a forward job: 12 arrays per shot
+    assert f >= 12 * 4 * 100 * 120 * 4
+    # snapshots dominate a long gradient job: 2 x nt x shots x cells x 4 B
+    assert job_bytes(100, 120, 4, 4000, 1, 8, gradient=True) >= 2 * 4000 * 4 * 100 * 120 * 4
+
+
+def test_plan_chunks():
+    per_shot = lambda n: 10 * n  # noqa: E731
+    assert plan_chunks(7, per_shot, 1000) == [(0, 7)]
+    assert plan_chunks(7, per_shot, 40) == [(0, 2), (2, 4), (4, 6), (6, 7)]
+    assert plan_chunks(3, per_shot, 15) == [(0, 1), (1, 2), (2, 3)]  # one shot may use it all
+    with pytest.raises(ValueError, match="HBM budget"):
+        plan_chunks(3, per_shot, 9)
+
+
+def test_served_trace_is_seeded_and_in_range():
+    a = served_trace(12, rate=5.0, seed=7, side_range=(128, 256), nt_range=(20, 40), shot_range=(1, 4))
+    b = served_trace(12, rate=5.0, seed=7, side_range=(128, 256), nt_range=(20, 40), shot_range=(1, 4))
+    times = [t for t, _ in a]
+    assert times == sorted(times) and times == [t for t, _ in b]
+    for (_, kw), (_, kw2) in zip(a, b):
+        p = kw["problem"]
+        ny, nx = p["epsilon"].shape
+        assert 128 <= ny <= 256 and 128 <= nx <= 256
+        nt = p["source_amplitude"].shape[-1]
+        shots = p["source_amplitude"].shape[0]
+        assert 20 <= nt <= 40 and 1 <= shots <= 4
+        assert torch.equal(p["epsilon"], kw2["problem"]["epsilon"])
+        assert p["receiver_location"].shape[:2] == (shots, min(64, nx - 2))
+        if kw["kind"] == "gradient":
+            assert kw["observed"].shape == (nt, shots, min(64, nx - 2))
+    kinds = [kw["kind"] for _, kw in served_trace(200, rate=1.0, seed=3, side_range=(8, 9), nt_range=(2, 3))]
+    assert 0.2 < kinds.count("gradient") / 200 < 0.4
+    gaps = [t1 - t0 for (t0, _), (t1, _) in zip(served_trace(400, 2.0, seed=1, side_range=(8, 9),
+                                                              nt_range=(2, 3))[:-1],
+                                                 served_trace(400, 2.0, seed=1, side_range=(8, 9),
+                                                              nt_range=(2, 3))[1:])]
+    assert abs(sum(gaps) / len(gaps) - 0.5) < 0.1  # exponential gaps, mean 1/rate
+
+
+def test_latency_stats():
+    tickets = []
+    for i in range(101):
+        t = Ticket(id=i, kind="forward", arrival=float(i))
+        t.completed = float(i) + 1.0 + i / 100.0
+        tickets.append(t)
+    s = latency_stats(tickets)
+    assert s["solves"] == 101
+    assert math.isclose(s["p50_s"], 1.5) and math.isclose(s["p99_s"], 1.99)
+    assert math.isclose(s["span_s"], 102.0) and math.isclose(s["solves_per_s"], 101 / 102.0)

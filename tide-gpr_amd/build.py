@@ -12,6 +12,8 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
@@ -19,7 +21,7 @@ CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 OUT = HERE / "tide" / "_lib" / "libtide_hip.so"
 SOURCES = ["tide_common.hip", "tm2d.hip"]
-OPTIONAL_SOURCES = ["em3d.hip", "tm2d_batch.hip"]
+OPTIONAL_SOURCES = ["em3d.hip", "tm2d_batch.hip", "tm2d_serve.hip"]
 ARCH = os.environ.get("TIDE_HIP_ARCH", "gfx950")
 
 
@@ -30,23 +32,36 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> Path:
-    OUT.parent.mkdir(parents=True, exist_ok=True)
-    srcs = [str(CSRC / s) for s in SOURCES]
-    srcs += [str(CSRC / s) for s in OPTIONAL_SOURCES if (CSRC / s).exists()]
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [
-        hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-        "-ffp-contract=off",
-        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-I", str(CSRC), "-I", str(INCLUDE), "-o", str(tmp), *srcs,
-    ]
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(proc.stdout + proc.stderr)
         raise RuntimeError(f"hipcc failed with status {proc.returncode}")
+
+
+def build(verbose: bool = False) -> Path:
+    """Compiles every source to an object in parallel, then links the
+    shared library in place (atomically replaced)."""
+    OUT.parent.mkdir(parents=True, exist_ok=True)
+    srcs = [CSRC / s for s in SOURCES]
+    srcs += [CSRC / s for s in OPTIONAL_SOURCES if (CSRC / s).exists()]
+    flags = [
+        f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-I", str(CSRC), "-I", str(INCLUDE),
+    ]
+    tmp = OUT.with_suffix(".so.tmp")
+    with tempfile.TemporaryDirectory(prefix="tide_hip_build_") as work:
+        objs = [str(Path(work) / (src.stem + ".o")) for src in srcs]
+        jobs = max(1, min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            futures = [pool.submit(_run, [hipcc(), *flags, "-c", str(src), "-o", obj], verbose)
+                       for src, obj in zip(srcs, objs)]
+            for fut in futures:
+                fut.result()
+        _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs], verbose)
     os.replace(tmp, OUT)
     return OUT
 

@@ -1,0 +1,570 @@
+// Serving engine for 2-D TM solves: continuous batching of independent
+// problems that arrive and leave at any time.
+//
+// The engine keeps one descriptor per admitted job (its geometry, its
+// arrays, its current phase and step).  A tick advances every running job
+// by up to K steps of its phase with one launch per step: the jobs are
+// sorted by the steps they take in this tick (longest first), so the jobs
+// still running at step j of the tick own a prefix of the flat tile range,
+// and a workgroup finds its job by a binary search over the jobs' first
+// tiles.  The job's phase picks the forward step or the adjoint iteration
+// (the single-problem tile code), so forward-only solves, forward passes of
+// gradient solves and adjoints all share the same launches.
+//
+// Host uploads go through two pinned staging slots, each reused only after
+// the copies last issued from it have completed (an event per slot), so the
+// host never waits on the device except to recycle a slot.  Per-job device
+// memory (packed CPML profiles and the source/receiver table) comes from the
+// stream-ordered allocator and is released in stream order.
+
+#include "tide_hip.h"
+#include "tm2d_items.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+namespace tide_hip {
+namespace tm2d {
+
+struct ServeItem {
+  BatchItem<float> b;
+  int32_t phase;  // 0 forward, 2 adjoint
+  int32_t fmt;    // snapshot payload: 0 none, 1 fp32, 2 BF16
+  int32_t k0;     // step of the phase at the start of the tick
+  int32_t pad;
+};
+
+// n: jobs active at this step (a prefix of the tick's job list).
+template <int TYF, int TYA>
+__global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ items,
+                                                  const int32_t *__restrict__ start, int n, int j) {
+  extern __shared__ double red[];
+  const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  int lo = 0, hi = n - 1;  // largest i with start[i] <= flat
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= flat)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const ServeItem &it = items[lo];
+  const int k = it.k0 + j;
+  const TileCoord tc = batch_tile((int)(flat - start[lo]), it.b.pts.n_xt, it.b.n_yt);
+  if (it.phase == 0) {
+    const FwdArgs<float> a = item_fwd_args(it.b, k);
+    if (it.fmt == 0)
+      fwd_workgroup<float, TYF, 0>(a, tc);
+    else if (it.fmt == 2)
+      fwd_workgroup<float, TYF, 2>(a, tc);
+    else
+      fwd_workgroup<float, TYF, 1>(a, tc);
+  } else {
+    const AdjArgs<float> a = item_adj_args(it.b, k);
+    if (it.fmt == 2)
+      adj_workgroup<float, TYA, 2>(a, tc, red);
+    else
+      adj_workgroup<float, TYA, 1>(a, tc, red);
+  }
+}
+
+namespace {
+
+constexpr size_t kAlign = 256;
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+struct EJob {
+  tide_tm2d_job spec;
+  BatchItem<float> item;
+  PointTable table;
+  char *block = nullptr;  // packed profiles + point table (device)
+  size_t block_bytes = 0;
+  bool uploaded = false;
+  bool grad = false;
+  int fmt = 1;            // snapshot payload when storing
+  int phase = 0;          // 0 forward, 1 forward done, 2 adjoint, 3 adjoint done
+  int64_t k = 0;
+  int64_t nt = 0;
+  int n_groups = 1;
+};
+
+struct Slot {
+  char *host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+};
+
+struct Engine {
+  int device = 0;
+  int group = 4;
+  int rows_f = 4, rows_a = 2;
+  int64_t max_jobs = 64;
+  std::map<int64_t, EJob> jobs;
+  int64_t next_id = 1;
+  ServeItem *d_items = nullptr;
+  int32_t *d_start = nullptr;
+  Slot slots[2];
+  int cur = 0;
+};
+
+// The (forward rows, adjoint rows) pairs the step kernel is built for; the
+// tile counts of a tick use exactly the pair the launch uses.
+void pick_rows(int rows_f, int rows_a, int *out_f, int *out_a) {
+  if (rows_f >= 8) {
+    *out_f = 8;
+    *out_a = 4;
+  } else if (rows_f <= 2) {
+    *out_f = 2;
+    *out_a = 2;
+  } else {
+    *out_f = 4;
+    *out_a = rows_a >= 4 ? 4 : 2;
+  }
+}
+
+void launch_step(const Engine &e, int64_t grid, int n, int j, hipStream_t s) {
+  const dim3 block(64, (unsigned)e.group);
+  const size_t lds = 2 * (size_t)e.group * e.rows_a * 64 * sizeof(double);
+  if (e.rows_f == 8 && e.rows_a == 4)
+    serve_step<8, 4><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j);
+  else if (e.rows_f == 2 && e.rows_a == 2)
+    serve_step<2, 2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j);
+  else if (e.rows_f == 4 && e.rows_a == 4)
+    serve_step<4, 4><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j);
+  else
+    serve_step<4, 2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j);
+}
+
+int slot_reserve(Slot &sl, size_t bytes) {
+  if (sl.pending) {
+    TIDE_HIP_TRY(hipEventSynchronize(sl.ev));
+    sl.pending = false;
+  }
+  if (bytes <= sl.cap) return kOk;
+  if (sl.host) TIDE_HIP_TRY(hipHostFree(sl.host));
+  sl.host = nullptr;
+  sl.cap = 0;
+  const size_t cap = std::max<size_t>(bytes * 2, (size_t)1 << 20);
+  TIDE_HIP_TRY(hipHostMalloc((void **)&sl.host, cap, hipHostMallocDefault));
+  sl.cap = cap;
+  return kOk;
+}
+
+size_t table_bytes(const EJob &j) {
+  const int64_t ny = j.spec.p.ny, nx = j.spec.p.nx;
+  return align_up(6 * (size_t)(ny + nx) * sizeof(float)) +
+         align_up(j.table.row_slot.size() * sizeof(int32_t)) +
+         align_up(j.table.seg.size() * sizeof(int32_t)) +
+         align_up(std::max<size_t>(1, j.table.ent.size()) * sizeof(int2));
+}
+
+// Allocates the job's device block and enqueues its profile and table
+// copies; host bytes are staged at `stage`.  Returns the staged bytes.
+int upload_job(EJob &j, char *stage, size_t *staged, hipStream_t s) {
+  const tide_tm2d_problem &q = j.spec.p;
+  j.block_bytes = table_bytes(j);
+  TIDE_HIP_TRY(hipMallocAsync((void **)&j.block, j.block_bytes, s));
+  char *d = j.block;
+  float *prof = (float *)d;
+  const int ysrc[6] = {0, 1, 8, 2, 3, 9}, xsrc[6] = {4, 5, 10, 6, 7, 11};
+  for (int k = 0; k < 6; ++k) {
+    TIDE_HIP_TRY(hipMemcpyAsync(prof + (size_t)k * q.ny, q.prof[ysrc[k]], (size_t)q.ny * sizeof(float),
+                                hipMemcpyDeviceToDevice, s));
+    TIDE_HIP_TRY(hipMemcpyAsync(prof + (size_t)6 * q.ny + (size_t)k * q.nx, q.prof[xsrc[k]],
+                                (size_t)q.nx * sizeof(float), hipMemcpyDeviceToDevice, s));
+  }
+  d += align_up(6 * (size_t)(q.ny + q.nx) * sizeof(float));
+  size_t off = 0;
+  auto put = [&](const void *src, size_t bytes, char **dst) -> int {
+    std::memcpy(stage + off, src, bytes);
+    const hipError_t e = hipMemcpyAsync(d, stage + off, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      set_error(std::string("serve upload: ") + hipGetErrorString(e));
+      return kHipError;
+    }
+    *dst = d;
+    d += align_up(bytes);
+    off += align_up(bytes);
+    return kOk;
+  };
+  char *row_slot, *seg, *ent;
+  int rc = put(j.table.row_slot.data(), j.table.row_slot.size() * sizeof(int32_t), &row_slot);
+  if (rc == kOk) rc = put(j.table.seg.data(), j.table.seg.size() * sizeof(int32_t), &seg);
+  const int2 zero = make_int2(0, 0);
+  if (rc == kOk)
+    rc = j.table.ent.empty() ? put(&zero, sizeof(int2), &ent)
+                             : put(j.table.ent.data(), j.table.ent.size() * sizeof(int2), &ent);
+  if (rc != kOk) return rc;
+  j.item.g.py = prof;
+  j.item.g.px = prof + (size_t)6 * q.ny;
+  j.item.pts = Points{(const int32_t *)row_slot, (const int32_t *)seg, (const int2 *)ent, j.table.n_xt};
+  j.uploaded = true;
+  j.table = PointTable();  // host copy no longer needed
+  *staged = off;
+  return kOk;
+}
+
+int validate(const tide_tm2d_job &job, int group, EJob &out) {
+  const tide_tm2d_problem &q = job.p;
+  if (q.n_shots <= 0 || q.ny < 3 || q.nx < 3 || q.nt < 0 || q.interval <= 0 ||
+      q.ny * q.nx >= ((int64_t)1 << 29) || q.n_shots > 4096 || q.nt >= ((int64_t)1 << 30)) {
+    set_error("engine: a job needs n_shots in 1..4096, 3 <= ny, nx, ny*nx < 2^29, 0 <= nt < 2^30, interval >= 1");
+    return kInvalidArgument;
+  }
+  if (!q.ca || !q.cb || !q.cq) {
+    set_error("engine: coefficients are required");
+    return kInvalidArgument;
+  }
+  for (int c = 0; c < 12; ++c)
+    if (!q.prof[c]) {
+      set_error("engine: all 12 CPML profiles are required");
+      return kInvalidArgument;
+    }
+  for (int c = 0; c < 7; ++c)
+    if (!q.fields[c] || (c < 5 && !job.fields_b[c])) {
+      set_error("engine: forward state needs fields[7] and fields_b[5]");
+      return kInvalidArgument;
+    }
+  if ((q.n_src > 0 && (!q.f || !q.sources_i)) || (q.n_rec > 0 && (!q.r || !q.receivers_i))) {
+    set_error("engine: sources need f and sources_i, receivers need r and receivers_i");
+    return kInvalidArgument;
+  }
+  if (job.storage_format != 0 && job.storage_format != 1) {
+    set_error("engine: storage_format must be 0 (fp32) or 1 (BF16)");
+    return kInvalidArgument;
+  }
+  out.grad = q.lambda[0] != nullptr;
+  out.n_groups = (int)((q.n_shots + group - 1) / group);
+  if (out.grad) {
+    for (int c = 0; c < 7; ++c)
+      if (!q.lambda[c] || !job.lambda_b[c]) {
+        set_error("engine: a gradient job needs lambda[7] and lambda_b[7]");
+        return kInvalidArgument;
+      }
+    if (!q.ey_store || !q.curl_store || !q.grad_ca || !q.grad_cb || (q.n_rec > 0 && !q.grad_r)) {
+      set_error("engine: a gradient job needs both snapshot buffers, grad_ca, grad_cb and grad_r");
+      return kInvalidArgument;
+    }
+    if (out.n_groups > 1 && (!job.grad_ca_groups || !job.grad_cb_groups)) {
+      set_error("engine: a gradient job with more shots than a workgroup holds needs grad_*_groups");
+      return kInvalidArgument;
+    }
+  }
+  return kOk;
+}
+
+void fill_item(EJob &j) {
+  const tide_tm2d_problem &q = j.spec.p;
+  BatchItem<float> &it = j.item;
+  std::memset(&it, 0, sizeof(it));
+  Geom<float> &g = it.g;
+  g.rdy = q.rdy;
+  g.rdx = q.rdx;
+  g.ny = (int32_t)q.ny;
+  g.nx = (int32_t)q.nx;
+  g.numel = (int32_t)(q.ny * q.nx);
+  g.pml_y0 = (int32_t)q.pml[0];
+  g.pml_x0 = (int32_t)q.pml[1];
+  g.pml_y1 = (int32_t)q.pml[2];
+  g.pml_x1 = (int32_t)q.pml[3];
+  g.pml_y1h = (int32_t)(q.pml[2] > q.pml[0] ? q.pml[2] - 1 : q.pml[0]);
+  g.pml_x1h = (int32_t)(q.pml[3] > q.pml[1] ? q.pml[3] - 1 : q.pml[1]);
+  it.ca = q.ca;
+  it.cb = q.cb;
+  it.cq = q.cq;
+  for (int c = 0; c < 5; ++c) {
+    it.fwd[0][c] = q.fields[c];
+    it.fwd[1][c] = j.spec.fields_b[c];
+  }
+  it.mhzx = q.fields[6];
+  it.mhxz = q.fields[5];
+  it.f = q.f;
+  it.r = q.r;
+  j.fmt = j.spec.storage_format == 1 ? 2 : 1;
+  const int64_t total = q.n_shots * q.ny * q.nx;
+  it.store_step_bytes = total * (j.fmt == 2 ? 2 : (int64_t)sizeof(float));
+  if (j.grad) {
+    it.ey_store = q.ey_store;
+    it.curl_store = q.curl_store;
+    // caller's order ley lhx lhz ml_ey_x ml_ey_z ml_hx_z ml_hz_x;
+    // kernel order ley lhx lhz m_hz_x m_hx_z m_ey_x m_ey_z
+    const int perm[7] = {0, 1, 2, 6, 5, 3, 4};
+    for (int c = 0; c < 7; ++c) {
+      it.lam[0][c] = q.lambda[perm[c]];
+      it.lam[1][c] = j.spec.lambda_b[perm[c]];
+    }
+    it.grad_r = q.grad_r;
+    it.grad_f = q.grad_f;
+    if (j.n_groups > 1) {
+      it.gca = j.spec.grad_ca_groups;
+      it.gcb = j.spec.grad_cb_groups;
+      it.group_stride = q.ny * q.nx;
+    } else {
+      it.gca = q.grad_ca;
+      it.gcb = q.grad_cb;
+      it.group_stride = 0;
+    }
+  }
+  it.n_shots = (int32_t)q.n_shots;
+  it.n_src = (int32_t)q.n_src;
+  it.n_rec = (int32_t)q.n_rec;
+  it.nt = (int32_t)q.nt;
+  it.interval = (int32_t)q.interval;
+  it.n_groups = (int32_t)j.n_groups;
+}
+
+int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running) {
+  std::vector<std::pair<int64_t, EJob *>> run;  // (steps this tick, job)
+  for (auto &kv : e.jobs) {
+    EJob &j = kv.second;
+    if (j.phase != 0 && j.phase != 2) continue;
+    const int64_t left = j.nt - j.k;
+    if (left <= 0) {
+      j.phase += 1;
+      continue;
+    }
+    run.emplace_back(std::min<int64_t>(left, max_steps), &j);
+  }
+  if (run.empty()) {
+    *n_running = 0;
+    return kOk;
+  }
+  std::stable_sort(run.begin(), run.end(),
+                   [](const std::pair<int64_t, EJob *> &a, const std::pair<int64_t, EJob *> &b) {
+                     return a.first > b.first;
+                   });
+  const int n = (int)run.size();
+  size_t need = align_up((size_t)n * sizeof(ServeItem)) + align_up((size_t)(n + 1) * sizeof(int32_t));
+  for (auto &r : run)
+    if (!r.second->uploaded) need += table_bytes(*r.second);
+  Slot &sl = e.slots[e.cur];
+  if (int rc = slot_reserve(sl, need); rc != kOk) return rc;
+  size_t off = 0;
+  for (auto &r : run) {
+    if (r.second->uploaded) continue;
+    size_t used = 0;
+    if (int rc = upload_job(*r.second, sl.host + off, &used, s); rc != kOk) return rc;
+    off += used;
+  }
+  ServeItem *items = (ServeItem *)(sl.host + off);
+  int32_t *start = (int32_t *)(sl.host + off + align_up((size_t)n * sizeof(ServeItem)));
+  int64_t tile = 0;
+  for (int i = 0; i < n; ++i) {
+    EJob &j = *run[(size_t)i].second;
+    ServeItem &si = items[i];
+    si.b = j.item;
+    const int rows = j.phase == 0 ? e.rows_f : e.rows_a;
+    si.b.n_yt = (int32_t)((j.spec.p.ny + rows - 1) / rows);
+    si.b.tile0 = (int32_t)tile;
+    si.phase = j.phase;
+    si.fmt = j.phase == 0 ? (j.grad ? j.fmt : 0) : j.fmt;
+    si.k0 = (int32_t)j.k;
+    si.pad = 0;
+    start[i] = (int32_t)tile;
+    tile += (int64_t)si.b.pts.n_xt * si.b.n_yt * j.n_groups;
+    if (tile >= ((int64_t)1 << 31)) {
+      set_error("engine: too many tiles in one tick");
+      return kInvalidArgument;
+    }
+  }
+  start[n] = (int32_t)tile;
+  TIDE_HIP_TRY(hipMemcpyAsync(e.d_items, items, (size_t)n * sizeof(ServeItem), hipMemcpyHostToDevice, s));
+  TIDE_HIP_TRY(hipMemcpyAsync(e.d_start, start, (size_t)(n + 1) * sizeof(int32_t),
+                              hipMemcpyHostToDevice, s));
+  TIDE_HIP_TRY(hipEventRecord(sl.ev, s));
+  sl.pending = true;
+  e.cur ^= 1;
+  const int64_t steps = run[0].first;
+  int m = n;
+  for (int64_t jstep = 0; jstep < steps; ++jstep) {
+    while (m > 0 && run[(size_t)m - 1].first <= jstep) --m;
+    launch_step(e, start[m], m, (int)jstep, s);
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  int64_t still = 0;
+  for (auto &r : run) {
+    EJob &j = *r.second;
+    j.k += r.first;
+    if (j.k >= j.nt)
+      j.phase += 1;
+    else
+      ++still;
+  }
+  *n_running = still;
+  return kOk;
+}
+
+void engine_destroy(Engine *e) {
+  if (e == nullptr) return;
+  (void)hipSetDevice(e->device);
+  (void)hipDeviceSynchronize();
+  for (auto &kv : e->jobs)
+    if (kv.second.block) (void)hipFreeAsync(kv.second.block, nullptr);
+  (void)hipDeviceSynchronize();
+  if (e->d_items) (void)hipFree(e->d_items);
+  if (e->d_start) (void)hipFree(e->d_start);
+  for (Slot &sl : e->slots) {
+    if (sl.host) (void)hipHostFree(sl.host);
+    if (sl.ev) (void)hipEventDestroy(sl.ev);
+  }
+  delete e;
+}
+
+Engine *as_engine(void *p) {
+  if (p == nullptr) set_error("engine: null handle");
+  return (Engine *)p;
+}
+
+}  // namespace
+}  // namespace tm2d
+}  // namespace tide_hip
+
+using namespace tide_hip;
+using namespace tide_hip::tm2d;
+
+extern "C" int tide_tm2d_engine_create(int64_t device, int64_t group, int64_t max_jobs, void **engine) {
+  if (engine == nullptr || group < 1 || group > kMaxGroup || max_jobs < 1 || max_jobs > 65536) {
+    set_error("engine: need an output handle, group in 1..8 and max_jobs in 1..65536");
+    return kInvalidArgument;
+  }
+  *engine = nullptr;
+  TIDE_HIP_TRY(hipSetDevice((int)device));
+  Engine *e = new Engine();
+  e->device = (int)device;
+  e->group = (int)group;
+  pick_rows(rows_for<float>(g_tuning.fwd_rows), rows_for<float>(g_tuning.adj_rows), &e->rows_f, &e->rows_a);
+  e->max_jobs = max_jobs;
+  hipError_t err = hipMalloc((void **)&e->d_items, (size_t)max_jobs * sizeof(ServeItem));
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_start, (size_t)(max_jobs + 1) * sizeof(int32_t));
+  for (Slot &sl : e->slots)
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
+  if (err != hipSuccess) {
+    set_error(std::string("engine: ") + hipGetErrorString(err));
+    engine_destroy(e);
+    return kHipError;
+  }
+  *engine = e;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_engine_group(void *engine, int64_t *group) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr || group == nullptr) return kInvalidArgument;
+  *group = e->group;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_engine_add(void *engine, const tide_tm2d_job *job, int64_t *job_id) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr) return kInvalidArgument;
+  if (job == nullptr || job_id == nullptr) {
+    set_error("engine: job and id output are required");
+    return kInvalidArgument;
+  }
+  if ((int64_t)e->jobs.size() >= e->max_jobs) {
+    set_error("engine: job table full (max_jobs)");
+    return kInvalidArgument;
+  }
+  EJob j;
+  j.spec = *job;
+  if (int rc = validate(*job, e->group, j); rc != kOk) return rc;
+  const tide_tm2d_problem &q = job->p;
+  std::vector<int64_t> hs((size_t)(q.n_shots * q.n_src)), hr((size_t)(q.n_shots * q.n_rec));
+  // the index arrays must be host memory: a device read here would order
+  // this call behind all queued work
+  hipPointerAttribute_t attr;
+  for (const int64_t *p : {q.sources_i, q.receivers_i}) {
+    if (p == nullptr) continue;
+    std::memset(&attr, 0, sizeof(attr));
+    if (hipPointerGetAttributes(&attr, p) == hipSuccess && attr.type == hipMemoryTypeDevice) {
+      set_error("engine: sources_i / receivers_i must be host arrays");
+      return kInvalidArgument;
+    }
+    (void)hipGetLastError();
+  }
+  if (!hs.empty()) std::memcpy(hs.data(), q.sources_i, hs.size() * sizeof(int64_t));
+  if (!hr.empty()) std::memcpy(hr.data(), q.receivers_i, hr.size() * sizeof(int64_t));
+  if (!build_point_table(hs.data(), q.n_src, hr.data(), q.n_rec, q.n_shots, q.ny, q.nx, kOwn, 1, j.table))
+    return kInvalidArgument;
+  fill_item(j);
+  j.nt = q.nt;
+  j.phase = 0;
+  j.k = 0;
+  const int64_t id = e->next_id++;
+  e->jobs.emplace(id, std::move(j));
+  *job_id = id;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_engine_tick(void *engine, int64_t max_steps, void *stream, int64_t *n_running) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr) return kInvalidArgument;
+  if (max_steps < 1 || n_running == nullptr) {
+    set_error("engine: max_steps >= 1 and a running-count output are required");
+    return kInvalidArgument;
+  }
+  TIDE_HIP_TRY(hipSetDevice(e->device));
+  return engine_tick(*e, max_steps, (hipStream_t)stream, n_running);
+}
+
+extern "C" int tide_tm2d_engine_query(void *engine, int64_t job_id, int64_t *phase, int64_t *step) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr) return kInvalidArgument;
+  auto it = e->jobs.find(job_id);
+  if (it == e->jobs.end()) {
+    set_error("engine: unknown job id");
+    return kInvalidArgument;
+  }
+  if (phase) *phase = it->second.phase;
+  if (step) *step = it->second.k;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_engine_begin_adjoint(void *engine, int64_t job_id) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr) return kInvalidArgument;
+  auto it = e->jobs.find(job_id);
+  if (it == e->jobs.end()) {
+    set_error("engine: unknown job id");
+    return kInvalidArgument;
+  }
+  EJob &j = it->second;
+  if (!j.grad || j.phase != 1) {
+    set_error("engine: the adjoint starts after the forward of a gradient job");
+    return kInvalidArgument;
+  }
+  j.phase = 2;
+  j.k = 0;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_engine_remove(void *engine, int64_t job_id, void *stream) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr) return kInvalidArgument;
+  auto it = e->jobs.find(job_id);
+  if (it == e->jobs.end()) {
+    set_error("engine: unknown job id");
+    return kInvalidArgument;
+  }
+  TIDE_HIP_TRY(hipSetDevice(e->device));
+  EJob &j = it->second;
+  hipStream_t s = (hipStream_t)stream;
+  if (j.phase == 3 && j.n_groups > 1) {
+    const int64_t numel = j.spec.p.ny * j.spec.p.nx;
+    const unsigned blocks = (unsigned)((numel + 255) / 256);
+    fold_groups<<<blocks, 256, 0, s>>>(j.spec.p.grad_ca, j.spec.grad_ca_groups, numel, j.n_groups);
+    fold_groups<<<blocks, 256, 0, s>>>(j.spec.p.grad_cb, j.spec.grad_cb_groups, numel, j.n_groups);
+    TIDE_HIP_TRY(hipGetLastError());
+  }
+  if (j.block) TIDE_HIP_TRY(hipFreeAsync(j.block, s));
+  e->jobs.erase(it);
+  return kOk;
+}
+
+extern "C" int tide_tm2d_engine_destroy(void *engine) {
+  engine_destroy((Engine *)engine);
+  return kOk;
+}
