@@ -234,6 +234,18 @@ int tide_tm2d_get_tuning(int *out4);
 int tide_tm2d_set_shot_loop(int fwd_loop, int adj_loop);
 int tide_tm2d_get_shot_loop(int *out2);
 
+/*
+ * Time steps per launch of the single-problem TM2D path, per pass (1 or
+ * 2).  With 2, pairs of steps run in one launch of the two-step kernels
+ * (60 owned columns per wavefront, a recomputed halo of two), which read
+ * and write the wavefield state once per two steps; results are bitwise
+ * those of single steps.  An odd remainder runs as a single step.  No
+ * reference counterpart (one launch per kernel and step,
+ * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1371-1609).
+ */
+int tide_tm2d_set_time_block(int fwd_steps, int adj_steps);
+int tide_tm2d_get_time_block(int *out2);
+
 /* ------------------------------------------------------- batched 2-D plans
  * Many independent fp32 TM2D problems (different grid sizes, shot counts
  * and nt) advanced together: one launch per time step covers every problem

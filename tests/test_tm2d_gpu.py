@@ -174,10 +174,15 @@ def test_operator_bf16_gradient():
     assert e_e < 1e-2 and e_s < 1e-2 and cos > 0.999
 
 
-@pytest.mark.parametrize("loop", [0, 1])
-def test_batched_models_equal_separate_runs(loop):
-    with shot_loop(loop, loop):
-        _check_batched_models()
+@pytest.mark.parametrize("loop,steps", [(0, 1), (1, 1), (0, 2)])
+def test_batched_models_equal_separate_runs(loop, steps):
+    before = tide._native.get_tm2d_time_block()
+    tide._native.set_tm2d_time_block(steps, steps)
+    try:
+        with shot_loop(loop, loop):
+            _check_batched_models()
+    finally:
+        tide._native.set_tm2d_time_block(*before)
 
 
 def _check_batched_models():
@@ -191,15 +196,21 @@ def _check_batched_models():
     rec = torch.stack([p0["receiver_location"], p1["receiver_location"]]).to(dev)
     from tide.maxwell.tm2d import maxwelltm
 
+    eps = eps.requires_grad_(True)
     out = maxwelltm(eps, sig, torch.ones_like(eps), p0["grid_spacing"], p0["dt"], amp, src, rec,
                     pml_width=p0["pml_width"])
     r_b = out[-1]
     assert r_b.shape[1:3] == (2, 2)
+    (r_b.double() ** 2).sum().backward()  # per-model gradients (per-shot accumulators)
     for b, p in enumerate((p0, p1)):
-        single = maxwelltm(p["epsilon"].to(dev), p["sigma"].to(dev), p["mu"].to(dev), p["grid_spacing"],
+        e1 = p["epsilon"].to(dev).requires_grad_(True)
+        single = maxwelltm(e1, p["sigma"].to(dev), p["mu"].to(dev), p["grid_spacing"],
                            p["dt"], p["source_amplitude"].to(dev), p["source_location"].to(dev),
                            p["receiver_location"].to(dev), pml_width=p["pml_width"])[-1]
-        assert torch.equal(r_b[:, b], single)
+        assert torch.equal(r_b[:, b].detach(), single.detach())
+        (single.double() ** 2).sum().backward()
+        err = relative_l2(eps.grad[b].cpu().numpy(), e1.grad.cpu().numpy())
+        assert err < 1e-6, f"model {b} gradient {err:.2e}"
 
 
 def test_full_size_single_shot_matches_oracle():
@@ -278,14 +289,21 @@ def shot_loop(fwd, adj):
         tide._native.set_tm2d_shot_loop(*before)
 
 
-@pytest.mark.parametrize("shots,loop", [(1, 0), (5, 0), (12, 0), (5, 1), (12, 1)])
-def test_shot_groups_match_oracle(shots, loop):
+@pytest.mark.parametrize("shots,loop,steps", [(1, 0, 1), (5, 0, 1), (12, 0, 1), (5, 1, 1), (12, 1, 1),
+                                              (5, 0, 2), (12, 0, 2)])
+def test_shot_groups_match_oracle(shots, loop, steps):
     """One workgroup holds up to 8 shots; 12 shots exercise two groups (each
     group sums into its own FP64 slice, folded afterwards).  The shot-looping
-    kernels sum every shot of a cell in registers instead."""
-    print(f"shots {shots} shot loop {loop}", flush=True)
-    with shot_loop(loop, loop):
-        _check_shot_groups(shots)
+    kernels sum every shot of a cell in registers instead; the two-step
+    kernels reduce both steps' terms per launch."""
+    print(f"shots {shots} shot loop {loop} steps {steps}", flush=True)
+    before = tide._native.get_tm2d_time_block()
+    tide._native.set_tm2d_time_block(steps, steps)
+    try:
+        with shot_loop(loop, loop):
+            _check_shot_groups(shots)
+    finally:
+        tide._native.set_tm2d_time_block(*before)
 
 
 def _check_shot_groups(shots):
@@ -314,34 +332,63 @@ def test_repeat_runs_are_bitwise_identical():
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("tuning,loop,name", [
-    ((2, 8, 2, 8), (0, 0), "tm2d_hetero_f32"),
-    ((4, 4, 2, 4), (0, 0), "tm2d_hetero_f32"),
-    ((8, 8, 4, 8), (0, 0), "tm2d_hetero_f32"),
-    ((16, 2, 8, 2), (0, 0), "tm2d_hetero_f32"),
-    ((4, 1, 4, 3), (0, 0), "tm2d_hetero_f32"),
-    ((2, 4, 2, 4), (1, 1), "tm2d_hetero_f32"),
-    ((4, 2, 4, 1), (1, 1), "tm2d_hetero_f32"),
-    ((8, 3, 8, 3), (1, 1), "tm2d_hetero_f32"),
-    ((16, 4, 2, 2), (1, 0), "tm2d_hetero_f32"),
-    ((4, 4, 2, 8), (0, 1), "tm2d_hetero_f32"),
-    ((4, 4, 4, 4), (1, 1), "tm2d_hetero_f64"),
-])
-def test_every_launch_shape_matches_oracle(tuning, loop, name):
-    """Any launch shape the autotuner may pick, in either kernel family
-    (one shot per wavefront, or one tile per wavefront walking the shots),
-    gives the oracle's forward bitwise and its adjoint fields bitwise
-    (gradients to FP64 rounding)."""
-    case, _ = load_golden(name)
-    print(f"tuning {tuning} shot loop {loop} {name}", flush=True)
-    before = tide._native.get_tm2d_tuning()
-    before_loop = tide._native.get_tm2d_shot_loop()
+LAUNCH_SHAPES = [
+    ((2, 8, 2, 8), (0, 0), (1, 1), "tm2d_hetero_f32"),
+    ((4, 4, 2, 4), (0, 0), (1, 1), "tm2d_hetero_f32"),
+    ((8, 8, 4, 8), (0, 0), (1, 1), "tm2d_hetero_f32"),
+    ((16, 2, 8, 2), (0, 0), (1, 1), "tm2d_hetero_f32"),
+    ((4, 1, 4, 3), (0, 0), (1, 1), "tm2d_hetero_f32"),
+    ((2, 4, 2, 4), (1, 1), (1, 1), "tm2d_hetero_f32"),
+    ((4, 2, 4, 1), (1, 1), (1, 1), "tm2d_hetero_f32"),
+    ((8, 3, 8, 3), (1, 1), (1, 1), "tm2d_hetero_f32"),
+    ((16, 4, 2, 2), (1, 0), (1, 1), "tm2d_hetero_f32"),
+    ((4, 4, 2, 8), (0, 1), (1, 1), "tm2d_hetero_f32"),
+    ((4, 4, 4, 4), (1, 1), (1, 1), "tm2d_hetero_f64"),
+    # two time steps per launch
+    ((4, 4, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f32"),
+    ((2, 3, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f32"),
+    ((8, 2, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f32"),
+    ((16, 1, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f32"),
+    ((4, 4, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f64"),
+    ((2, 2, 2, 4), (0, 0), (2, 1), "tm2d_config1_f32"),
+    ((4, 4, 2, 4), (0, 0), (2, 2), "tm2d_hetero_f32"),
+    ((4, 4, 4, 2), (0, 0), (1, 2), "tm2d_hetero_f32"),
+    ((4, 4, 2, 3), (1, 1), (2, 2), "tm2d_hetero_f32"),
+    ((4, 4, 4, 8), (0, 0), (2, 2), "tm2d_hetero_f32"),
+    ((4, 4, 2, 1), (0, 0), (2, 2), "tm2d_hetero_f64"),
+    ((4, 4, 4, 4), (0, 0), (2, 2), "tm2d_hetero_f64"),
+    ((2, 2, 2, 4), (0, 0), (2, 2), "tm2d_config1_f32"),
+]
+
+
+@contextlib.contextmanager
+def launch_shape(tuning, loop, steps):
+    before = (tide._native.get_tm2d_tuning(), tide._native.get_tm2d_shot_loop(),
+              tide._native.get_tm2d_time_block())
     tide._native.set_tm2d_tuning(*tuning)
     tide._native.set_tm2d_shot_loop(*loop)
+    tide._native.set_tm2d_time_block(*steps)
     try:
+        yield
+    finally:
+        tide._native.set_tm2d_tuning(*before[0])
+        tide._native.set_tm2d_shot_loop(*before[1])
+        tide._native.set_tm2d_time_block(*before[2])
+
+
+@pytest.mark.parametrize("tuning,loop,steps,name", LAUNCH_SHAPES)
+def test_every_launch_shape_matches_oracle(tuning, loop, steps, name):
+    """Any launch shape the autotuner may pick, in every kernel family (one
+    shot per wavefront, one tile per wavefront walking the shots, one or two
+    time steps per launch), gives the oracle's forward bitwise and its
+    adjoint fields bitwise (gradients to FP64 rounding)."""
+    case, _ = load_golden(name)
+    print(f"tuning {tuning} shot loop {loop} steps {steps} {name}", flush=True)
+    with launch_shape(tuning, loop, steps):
         f_o, r_o, es, cs = O.tm2d_forward(case, store="full")
-        f_g, r_g, _, _ = gpu_forward(case, store="full")
+        f_g, r_g, es_g, cs_g = gpu_forward(case, store="full")
         assert np.array_equal(r_g, r_o)
+        assert np.array_equal(es_g, es) and np.array_equal(cs_g, cs)
         for a, b in zip(f_g, f_o):
             assert np.array_equal(a, b)
         rng = np.random.default_rng(3)
@@ -353,6 +400,3 @@ def test_every_launch_shape_matches_oracle(tuning, loop, name):
         assert np.array_equal(gg[2], go[2])
         assert relative_l2(gg[0], go[0]) < 1e-12
         assert relative_l2(gg[1], go[1]) < 1e-12
-    finally:
-        tide._native.set_tm2d_tuning(*before)
-        tide._native.set_tm2d_shot_loop(*before_loop)

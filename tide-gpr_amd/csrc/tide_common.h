@@ -73,6 +73,14 @@ bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
                        int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
                        int tile_w, int lane_offset, PointTable &out);
 
+// As above, and additionally every point is listed in the neighbouring
+// tiles whose 64-lane window (columns xt*tile_w - lane_offset ...
+// + 63) holds its column, as a copy: ent.x = lane | kind<<8 | 1<<9.  The
+// multi-step kernels recompute halo cells and apply sources there too.
+bool build_point_table_halo(const int64_t *src, int64_t n_src, const int64_t *rec,
+                            int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
+                            int tile_w, int lane_offset, PointTable &out);
+
 // Device copy of a PointTable; owns three device allocations.
 struct DevicePointTable {
   int32_t *row_slot = nullptr;

@@ -14,11 +14,13 @@ thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 const char *last_error() { return g_last_error.c_str(); }
 
-bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
-                       int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
-                       int tile_w, int lane_offset, PointTable &out) {
+namespace {
+bool build_table(const int64_t *src, int64_t n_src, const int64_t *rec, int64_t n_rec,
+                 int64_t n_shots, int64_t ny, int64_t nx, int tile_w, int lane_offset,
+                 bool halo, PointTable &out) {
   const int64_t numel = ny * nx;
-  // (y, shot, xt, kind, k, xloc, q)
+  const int64_t n_xt = (nx + tile_w - 1) / tile_w;
+  // (y, shot, xt, kind, k, xloc, q); kind bit 1 marks a halo copy
   struct Entry {
     int64_t y, shot, xt, kind, k;
     int32_t xloc, q;
@@ -41,8 +43,15 @@ bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
           return false;
         }
         const int64_t y = v / nx, x = v % nx;
-        entries.push_back({y, s, x / tile_w, kind, k, (int32_t)(x % tile_w + lane_offset),
-                           (int32_t)(s * n + k)});
+        const int64_t xt = x / tile_w;
+        const int32_t lane = (int32_t)(x % tile_w + lane_offset);
+        entries.push_back({y, s, xt, kind, k, lane, (int32_t)(s * n + k)});
+        if (halo) {
+          if (lane + tile_w <= 63 && xt > 0)
+            entries.push_back({y, s, xt - 1, kind | 2, k, lane + tile_w, (int32_t)(s * n + k)});
+          if (lane - tile_w >= 0 && xt + 1 < n_xt)
+            entries.push_back({y, s, xt + 1, kind | 2, k, lane - tile_w, (int32_t)(s * n + k)});
+        }
       }
     }
     return true;
@@ -52,7 +61,7 @@ bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
     return std::tie(a.y, a.shot, a.xt, a.kind, a.k) <
            std::tie(b.y, b.shot, b.xt, b.kind, b.k);
   });
-  out.n_xt = (int32_t)((nx + tile_w - 1) / tile_w);
+  out.n_xt = (int32_t)n_xt;
   out.row_slot.assign((size_t)ny, -1);
   std::vector<int64_t> rows;
   for (const Entry &e : entries)
@@ -70,8 +79,23 @@ bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
   out.ent.resize(entries.size());
   // entries are already in (row, shot, tile, kind, k) order == segment order.
   for (size_t i = 0; i < entries.size(); ++i)
-    out.ent[i] = make_int2(entries[i].xloc | ((int32_t)entries[i].kind << 8), entries[i].q);
+    out.ent[i] = make_int2(entries[i].xloc | ((int32_t)(entries[i].kind & 1) << 8) |
+                               ((int32_t)(entries[i].kind >> 1) << 9),
+                           entries[i].q);
   return true;
+}
+}  // namespace
+
+bool build_point_table(const int64_t *src, int64_t n_src, const int64_t *rec,
+                       int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
+                       int tile_w, int lane_offset, PointTable &out) {
+  return build_table(src, n_src, rec, n_rec, n_shots, ny, nx, tile_w, lane_offset, false, out);
+}
+
+bool build_point_table_halo(const int64_t *src, int64_t n_src, const int64_t *rec,
+                            int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
+                            int tile_w, int lane_offset, PointTable &out) {
+  return build_table(src, n_src, rec, n_rec, n_shots, ny, nx, tile_w, lane_offset, true, out);
 }
 
 int DevicePointTable::upload(const PointTable &t, hipStream_t stream) {

@@ -27,6 +27,7 @@
 // reference default) is the one built here.
 
 #include "tm2d_device.h"
+#include "tm2d_fused.h"
 #include "tide_hip.h"
 
 #include <algorithm>
@@ -67,6 +68,31 @@ void launch_fwd(int rows, bool loop, dim3 grid, dim3 block, hipStream_t stream,
     case 16: fwd_step<T, 16, FMT><<<grid, block, 0, stream>>>(a); break;
     default: fwd_step<T, 8, FMT><<<grid, block, 0, stream>>>(a); break;
   }
+}
+
+template <typename T, int FMT>
+void launch_fwd2(int rows, dim3 grid, dim3 block, hipStream_t stream, const Fwd2Args<T> &a) {
+  if (rows == 2) {
+    fwd2_step<T, 2, FMT><<<grid, block, 0, stream>>>(a);
+    return;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (rows == 8) {
+      fwd2_step<T, 8, FMT><<<grid, block, 0, stream>>>(a);
+      return;
+    }
+  }
+  fwd2_step<T, 4, FMT><<<grid, block, 0, stream>>>(a);
+}
+
+template <typename T, int FMT>
+void launch_adj2(int rows, dim3 grid, dim3 block, size_t lds, hipStream_t stream,
+                 const Adj2Args<T> &a) {
+  if (rows == 2) {
+    adj2_step<T, 2, FMT><<<grid, block, lds, stream>>>(a);
+    return;
+  }
+  adj2_step<T, 4, FMT><<<grid, block, lds, stream>>>(a);
 }
 
 template <typename T, int FMT>
@@ -160,9 +186,11 @@ int pack_profiles(Problem<T> &p, Scratch &scr, hipStream_t stream) {
   return kOk;
 }
 
+// halo2: the table of the two-step kernels (60-column tiles, lane offset 2,
+// halo copies) in `dpt2`.
 template <typename T>
 int upload_points(const Problem<T> &p, const int64_t *src, const int64_t *rec,
-                  DevicePointTable &dpt, hipStream_t stream) {
+                  DevicePointTable &dpt, hipStream_t stream, DevicePointTable *dpt2 = nullptr) {
   std::vector<int64_t> hsrc, hrec;
   TIDE_CHECK(fetch_indices(src, (int64_t)p.n_shots * p.n_src, hsrc, stream));
   TIDE_CHECK(fetch_indices(rec, (int64_t)p.n_shots * p.n_rec, hrec, stream));
@@ -170,7 +198,13 @@ int upload_points(const Problem<T> &p, const int64_t *src, const int64_t *rec,
   if (!build_point_table(hsrc.data(), p.n_src, hrec.data(), p.n_rec, p.n_shots, p.g.ny,
                          p.g.nx, kOwn, 1, table))
     return kInvalidArgument;
-  return dpt.upload(table, stream);
+  TIDE_CHECK(dpt.upload(table, stream));
+  if (dpt2 == nullptr) return kOk;
+  PointTable table2;
+  if (!build_point_table_halo(hsrc.data(), p.n_src, hrec.data(), p.n_rec, p.n_shots, p.g.ny,
+                              p.g.nx, kOwn2, 2, table2))
+    return kInvalidArgument;
+  return dpt2->upload(table2, stream);
 }
 
 template <typename T>
@@ -189,21 +223,28 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     return kUnsupported;
   }
   const int64_t n_shots = p.n_shots, total = n_shots * p.g.numel;
-  DevicePointTable dpt;
-  TIDE_CHECK(upload_points(p, src, rec, dpt, stream));
+  const bool fused = g_tuning.fwd_steps == 2 && nt >= 2;
+  DevicePointTable dpt, dpt2;
+  TIDE_CHECK(upload_points(p, src, rec, dpt, stream, fused ? &dpt2 : nullptr));
   Scratch scr(stream);
   TIDE_CHECK(pack_profiles(p, scr, stream));
-  T *ey2, *hx2, *hz2, *mex2, *mez2;
+  T *ey2, *hx2, *hz2, *mex2, *mez2, *mhzx2 = nullptr, *mhxz2 = nullptr;
   TIDE_CHECK(scr.alloc(&ey2, (size_t)total));
   TIDE_CHECK(scr.alloc(&hx2, (size_t)total));
   TIDE_CHECK(scr.alloc(&hz2, (size_t)total));
   TIDE_CHECK(scr.alloc(&mex2, (size_t)total));
   TIDE_CHECK(scr.alloc(&mez2, (size_t)total));
   const size_t bytes = (size_t)total * sizeof(T);
-  // H-phase memories are only written inside the strips; both copies start
+  // CPML memories are only written inside the strips; both copies start
   // equal so the untouched cells agree whichever buffer ends up current.
   TIDE_HIP_TRY(hipMemcpyAsync(mex2, m_ey_x, bytes, hipMemcpyDeviceToDevice, stream));
   TIDE_HIP_TRY(hipMemcpyAsync(mez2, m_ey_z, bytes, hipMemcpyDeviceToDevice, stream));
+  if (fused) {  // the two-step kernels also ping-pong the E-phase memories
+    TIDE_CHECK(scr.alloc(&mhzx2, (size_t)total));
+    TIDE_CHECK(scr.alloc(&mhxz2, (size_t)total));
+    TIDE_HIP_TRY(hipMemcpyAsync(mhzx2, m_hz_x, bytes, hipMemcpyDeviceToDevice, stream));
+    TIDE_HIP_TRY(hipMemcpyAsync(mhxz2, m_hx_z, bytes, hipMemcpyDeviceToDevice, stream));
+  }
 
   FwdArgs<T> a;
   a.g = p.g;
@@ -215,7 +256,7 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
   a.mhxz = m_hx_z;
   a.n_shots = p.n_shots;
   const int rows = rows_for<T>(g_tuning.fwd_rows);
-  const bool loop = g_tuning.fwd_loop != 0;
+  const bool loop = g_tuning.fwd_loop != 0 && !fused;
   const int group = loop ? std::min<int>(g_tuning.fwd_group, kMaxLoopTiles)
                          : std::min<int>(p.n_shots, g_tuning.fwd_group);
   a.n_yt = (p.g.ny + rows - 1) / rows;
@@ -226,8 +267,59 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
 
   T *cur[5] = {ey, hx, hz, m_ey_x, m_ey_z};
   T *nxt[5] = {ey2, hx2, hz2, mex2, mez2};
+  T *cur_e[2] = {m_hz_x, m_hx_z};  // E-phase memories (swapped by two-step launches)
+  T *nxt_e[2] = {mhzx2, mhxz2};
   const bool storing = ey_store != nullptr || curl_store != nullptr;
-  for (int64_t t = start_t; t < start_t + nt; ++t) {
+  auto store_at = [&](int64_t t, void *buf) -> void * {
+    if (buf == nullptr || !storing || (t % step_ratio) != 0) return nullptr;
+    return (char *)buf + (size_t)(t / step_ratio) * store_step_bytes;
+  };
+  Fwd2Args<T> a2;
+  dim3 grid2(1);
+  const dim3 block2(64, (unsigned)std::min<int>(p.n_shots, g_tuning.fwd_group));
+  const int rows2 = std::min<int>(rows_for<T>(g_tuning.fwd_rows), sizeof(T) == 4 ? 8 : 4);
+  if (fused) {
+    std::memset(&a2, 0, sizeof(a2));
+    a2.g = p.g;
+    a2.pts = Points{dpt2.row_slot, dpt2.seg, dpt2.ent, dpt2.n_xt};
+    a2.ca = p.ca;
+    a2.cb = p.cb;
+    a2.cq = p.cq;
+    a2.n_shots = p.n_shots;
+    a2.n_yt = (p.g.ny + rows2 - 1) / rows2;
+    const int64_t groups2 = (p.n_shots + block2.y - 1) / block2.y;
+    grid2 = dim3((unsigned)((int64_t)dpt2.n_xt * a2.n_yt * groups2));
+  }
+  int64_t t = start_t;
+  for (; fused && t + 1 < start_t + nt; t += 2) {
+    a2.ey_i = cur[0]; a2.hx_i = cur[1]; a2.hz_i = cur[2];
+    a2.mex_i = cur[3]; a2.mez_i = cur[4];
+    a2.mhzx_i = cur_e[0]; a2.mhxz_i = cur_e[1];
+    a2.ey_o = nxt[0]; a2.hx_o = nxt[1]; a2.hz_o = nxt[2];
+    a2.mex_o = nxt[3]; a2.mez_o = nxt[4];
+    a2.mhzx_o = nxt_e[0]; a2.mhxz_o = nxt_e[1];
+    a2.f0 = p.n_src > 0 ? f + t * n_shots * p.n_src : nullptr;
+    a2.f1 = p.n_src > 0 ? f + (t + 1) * n_shots * p.n_src : nullptr;
+    a2.r0 = p.n_rec > 0 ? r + t * n_shots * p.n_rec : nullptr;
+    a2.r1 = p.n_rec > 0 ? r + (t + 1) * n_shots * p.n_rec : nullptr;
+    a2.ey_store0 = store_at(t, ey_store);
+    a2.curl_store0 = store_at(t, curl_store);
+    a2.ey_store1 = store_at(t + 1, ey_store);
+    a2.curl_store1 = store_at(t + 1, curl_store);
+    const bool st = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
+    if (!st) {
+      launch_fwd2<T, 0>(rows2, grid2, block2, stream, a2);
+    } else if (fmt == 2) {
+      if constexpr (sizeof(T) == 4) launch_fwd2<T, 2>(rows2, grid2, block2, stream, a2);
+    } else {
+      launch_fwd2<T, 1>(rows2, grid2, block2, stream, a2);
+    }
+    for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
+    for (int k = 0; k < 2; ++k) std::swap(cur_e[k], nxt_e[k]);
+  }
+  a.mhzx = cur_e[0];
+  a.mhxz = cur_e[1];
+  for (; t < start_t + nt; ++t) {
     a.ey_i = cur[0]; a.hx_i = cur[1]; a.hz_i = cur[2];
     a.mex_i = cur[3]; a.mez_i = cur[4];
     a.ey_o = nxt[0]; a.hx_o = nxt[1]; a.hz_o = nxt[2];
@@ -255,7 +347,12 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     TIDE_HIP_TRY(hipMemcpyAsync(m_ey_x, cur[3], bytes, hipMemcpyDeviceToDevice, stream));
     TIDE_HIP_TRY(hipMemcpyAsync(m_ey_z, cur[4], bytes, hipMemcpyDeviceToDevice, stream));
   }
+  if (cur_e[0] != m_hz_x) {
+    TIDE_HIP_TRY(hipMemcpyAsync(m_hz_x, cur_e[0], bytes, hipMemcpyDeviceToDevice, stream));
+    TIDE_HIP_TRY(hipMemcpyAsync(m_hx_z, cur_e[1], bytes, hipMemcpyDeviceToDevice, stream));
+  }
   dpt.release(stream);
+  if (fused) dpt2.release(stream);
   return kOk;
 }
 
@@ -282,8 +379,9 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     return kInvalidArgument;
   }
   const int64_t n_shots = p.n_shots, total = n_shots * p.g.numel;
-  DevicePointTable dpt;
-  TIDE_CHECK(upload_points(p, src, rec, dpt, stream));
+  const bool fused = g_tuning.adj_steps == 2 && nt >= 2;
+  DevicePointTable dpt, dpt2;
+  TIDE_CHECK(upload_points(p, src, rec, dpt, stream, fused ? &dpt2 : nullptr));
   Scratch scr(stream);
   TIDE_CHECK(pack_profiles(p, scr, stream));
   T *ley2, *lhx2, *lhz2, *m2[4];
@@ -306,7 +404,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
   a.step_scale = (double)step_ratio;
   a.n_shots = p.n_shots;
   const int rows = rows_for<T>(g_tuning.adj_rows);
-  const bool loop = g_tuning.adj_loop != 0;
+  const bool loop = g_tuning.adj_loop != 0 && !fused;  // two-step launches group shots
   const int group = loop ? std::min<int>(g_tuning.adj_group, kMaxLoopTiles)
                          : std::min<int>(p.n_shots, g_tuning.adj_group);
   a.n_yt = (p.g.ny + rows - 1) / rows;
@@ -336,7 +434,55 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
   // order: ley, lhx, lhz, mx (hz_x), my (hx_z), nx (ey_x), nz (ey_z)
   T *cur[7] = {l_ey, l_hx, l_hz, ml_hz_x, ml_hx_z, ml_ey_x, ml_ey_z};
   T *nxt[7] = {ley2, lhx2, lhz2, m2[0], m2[1], m2[2], m2[3]};
-  for (int64_t t = start_t - 1; t >= start_t - nt; --t) {
+  auto snap = [&](int64_t t, const void *buf, const double *acc) -> const void * {
+    if (acc == nullptr || (t % step_ratio) != 0) return nullptr;
+    return (const char *)buf + (size_t)(t / step_ratio) * store_step_bytes;
+  };
+  int64_t t = start_t - 1;
+  if (fused) {
+    const int rows2 = std::min<int>(rows_for<T>(g_tuning.adj_rows), 4);
+    Adj2Args<T> a2;
+    std::memset(&a2, 0, sizeof(a2));
+    a2.g = p.g;
+    a2.pts = Points{dpt2.row_slot, dpt2.seg, dpt2.ent, dpt2.n_xt};
+    a2.ca = p.ca;
+    a2.cb = p.cb;
+    a2.cq = p.cq;
+    a2.per_shot_grad = a.per_shot_grad;
+    a2.step_scale = a.step_scale;
+    a2.n_shots = p.n_shots;
+    a2.n_yt = (p.g.ny + rows2 - 1) / rows2;
+    a2.group_stride = a.group_stride;
+    a2.gca = ga_k;
+    a2.gcb = gb_k;
+    const dim3 block2 = block;  // same shot groups as the single-step launches
+    const dim3 grid2((unsigned)((int64_t)dpt2.n_xt * a2.n_yt * n_groups));
+    const size_t lds2 = 4 * (size_t)block2.y * rows2 * 64 * sizeof(double);
+    for (; t - 1 >= start_t - nt; t -= 2) {
+      a2.ley_i = cur[0]; a2.lhx_i = cur[1]; a2.lhz_i = cur[2];
+      a2.mx_i = cur[3]; a2.my_i = cur[4]; a2.nx_i = cur[5]; a2.nz_i = cur[6];
+      a2.ley_o = nxt[0]; a2.lhx_o = nxt[1]; a2.lhz_o = nxt[2];
+      a2.mx_o = nxt[3]; a2.my_o = nxt[4]; a2.nx_o = nxt[5]; a2.nz_o = nxt[6];
+      a2.grad_r0 = p.n_rec > 0 ? grad_r + t * n_shots * p.n_rec : nullptr;
+      a2.grad_r1 = p.n_rec > 0 ? grad_r + (t - 1) * n_shots * p.n_rec : nullptr;
+      a2.grad_f0 = (p.n_src > 0 && grad_f) ? grad_f + t * n_shots * p.n_src : nullptr;
+      a2.grad_f1 = (p.n_src > 0 && grad_f) ? grad_f + (t - 1) * n_shots * p.n_src : nullptr;
+      a2.ey_store0 = snap(t, ey_store, gca);
+      a2.curl_store0 = snap(t, curl_store, gcb);
+      a2.ey_store1 = snap(t - 1, ey_store, gca);
+      a2.curl_store1 = snap(t - 1, curl_store, gcb);
+      const bool g_any = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
+      if (!g_any) {
+        launch_adj2<T, 0>(rows2, grid2, block2, lds2, stream, a2);
+      } else if (fmt == 2) {
+        if constexpr (sizeof(T) == 4) launch_adj2<T, 2>(rows2, grid2, block2, lds2, stream, a2);
+      } else {
+        launch_adj2<T, 1>(rows2, grid2, block2, lds2, stream, a2);
+      }
+      for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
+    }
+  }
+  for (; t >= start_t - nt; --t) {
     a.ley_i = cur[0]; a.lhx_i = cur[1]; a.lhz_i = cur[2];
     a.mx_i = cur[3]; a.my_i = cur[4]; a.nx_i = cur[5]; a.nz_i = cur[6];
     a.ley_o = nxt[0]; a.lhx_o = nxt[1]; a.lhz_o = nxt[2];
@@ -368,6 +514,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     if (cur[k] != orig[k])
       TIDE_HIP_TRY(hipMemcpyAsync(orig[k], cur[k], bytes, hipMemcpyDeviceToDevice, stream));
   dpt.release(stream);
+  if (fused) dpt2.release(stream);
   return kOk;
 }
 
@@ -671,6 +818,26 @@ extern "C" int tide_tm2d_get_shot_loop(int *out2) {
   }
   out2[0] = g_tuning.fwd_loop;
   out2[1] = g_tuning.adj_loop;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_set_time_block(int fwd_steps, int adj_steps) {
+  if (fwd_steps < 1 || fwd_steps > 2 || adj_steps < 1 || adj_steps > 2) {
+    set_error("time steps per launch must be 1 or 2");
+    return kInvalidArgument;
+  }
+  g_tuning.fwd_steps = fwd_steps;
+  g_tuning.adj_steps = adj_steps;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_get_time_block(int *out2) {
+  if (out2 == nullptr) {
+    set_error("tuning: null output");
+    return kInvalidArgument;
+  }
+  out2[0] = g_tuning.fwd_steps;
+  out2[1] = g_tuning.adj_steps;
   return kOk;
 }
 

@@ -1,0 +1,662 @@
+// Two-step (temporally blocked) 2-D TM step kernels.
+//
+// A wavefront owns kOwn2 = 60 columns (lanes 2..61) of TY rows of one shot
+// and advances them by two time steps in one launch: it loads the step-t
+// state of its rows and columns plus a halo of two, computes step t+1 on the
+// owned cells and a halo of one (recomputing what the neighbouring tiles
+// own), and step t+2 on the owned cells.  HBM sees the field state once per
+// two steps; the snapshots of both steps are written (forward) or read
+// (adjoint) as before.  Every cell update is the single-step kernel's
+// expression in the same operation order, so the results are bit-identical
+// to two single steps.
+//
+// CPML memories ping-pong between an input and an output buffer (the halo
+// cells of a tile read the memories its neighbours update).  Tiles whose
+// rows and columns (halo included) touch neither a CPML strip nor the grid
+// edge take a branch-free path.
+#pragma once
+
+#include "tm2d_device.h"
+
+namespace tide_hip {
+namespace tm2d {
+
+constexpr int kOwn2 = 60;  // owned columns per wavefront (lanes 2..61)
+
+template <typename T> struct Fwd2Args {
+  Geom<T> g;
+  Points pts;  // built with build_point_table_halo(tile_w 60, lane offset 2)
+  const T *ca, *cb, *cq;
+  const T *ey_i, *hx_i, *hz_i;
+  T *ey_o, *hx_o, *hz_o;
+  const T *mex_i, *mez_i, *mhzx_i, *mhxz_i;  // m_ey_x m_ey_z m_hz_x m_hx_z, step t
+  T *mex_o, *mez_o, *mhzx_o, *mhxz_o;        // step t+2
+  const T *f0, *f1;                          // source terms of steps t, t+1
+  T *r0, *r1;                                // receiver rows of steps t, t+1
+  void *ey_store0, *curl_store0;             // snapshots of step t (null: none)
+  void *ey_store1, *curl_store1;             // snapshots of step t+1
+  int32_t n_shots, n_yt;
+};
+
+// Tiles whose loads and updates (halo of two included) stay inside the
+// grid and outside every CPML strip.
+template <typename T, int TY>
+__device__ __forceinline__ bool fused_interior(const Geom<T> &g, int xt, int yt) {
+  const int x0 = xt * kOwn2 - 2, y0 = yt * TY;
+  return x0 >= 1 && x0 + 63 <= g.nx - 2 && x0 >= g.pml_x0 && x0 + 63 < g.pml_x1h &&
+         y0 - 2 >= 1 && y0 + TY + 1 <= g.ny - 2 && y0 - 2 >= g.pml_y0 && y0 + TY < g.pml_y1h;
+}
+
+// Applies the point entries of (row y, shot s, tile xt) to `ey`: sources of
+// this lane (owner entries and halo copies) add f[q]; receivers (owner
+// entries only, and only when `record`) store ey to r[q].
+template <typename T>
+__device__ __forceinline__ void fused_points(const Points &pts, int n_shots, int s, int xt, int y,
+                                             int lane, const T *f, T *r, bool record, T &ey) {
+  const int slot = pts.row_slot[y];
+  if (slot < 0) return;
+  const int64_t si = ((int64_t)slot * n_shots + s) * pts.n_xt + xt;
+  const int e1 = pts.seg[si + 1];
+  for (int e = pts.seg[si]; e < e1; ++e) {
+    const int2 en = pts.ent[e];
+    if ((en.x & 0xff) != lane) continue;
+    const int kind = (en.x >> 8) & 1;
+    if (kind == 0) {
+      if (f != nullptr) ey += f[en.y];
+    } else if (record && (en.x >> 9) == 0 && r != nullptr) {
+      r[en.y] = ey;
+    }
+  }
+}
+
+// GEN: tile touches the grid edge or a CPML strip (per-cell checks);
+// otherwise the branch-free interior path.
+template <typename T, int TY, int FMT, bool GEN>
+__device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, int yt) {
+  const Geom<T> &g = a.g;
+  const int ny = g.ny, nx = g.nx;
+  const int lane = threadIdx.x;
+  const int x = xt * kOwn2 - 2 + lane;
+  const int y0 = yt * TY;
+  const bool own = lane >= 2 && lane <= 61 && x < nx;
+  const bool e_col = !GEN || (x >= 1 && x < nx);       // Ey and Hx columns
+  const bool hz_col = !GEN || (x >= 1 && x < nx - 1);  // Hz columns
+  const bool pml_xe = GEN && (x < g.pml_x0 || x >= g.pml_x1);
+  const bool pml_xh = GEN && (x < g.pml_x0 || x >= g.pml_x1h);
+
+  const int64_t base = (int64_t)s * g.numel;
+  const T *__restrict__ ey_i = a.ey_i + base;
+  const T *__restrict__ hx_i = a.hx_i + base;
+  const T *__restrict__ hz_i = a.hz_i + base;
+  const T *__restrict__ ca = a.ca + (g.ca_b ? base : 0);
+  const T *__restrict__ cb = a.cb + (g.cb_b ? base : 0);
+  const T *__restrict__ cq = a.cq + (g.cq_b ? base : 0);
+  const RowLoader<T, GEN> L(x, nx, ny);
+
+  // Rows: Ey y0-2..y0+TY+1, Hx and cq y0-2..y0+TY, Hz, ca and cb y0-1..y0+TY.
+  constexpr int NE = TY + 4, NX = TY + 3, NZ = TY + 2;
+  T ey0[NE], hx0[NX], cqv[NX], hz0[NZ], cav[NZ], cbv[NZ];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) ey0[k] = L(ey_i, y0 - 2 + k);
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    hx0[k] = L(hx_i, y0 - 2 + k);
+    cqv[k] = L(cq, y0 - 2 + k);
+  }
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    hz0[k] = L(hz_i, y0 - 1 + k);
+    cav[k] = L(ca, y0 - 1 + k);
+    cbv[k] = L(cb, y0 - 1 + k);
+  }
+
+  // CPML memories of step t (GEN only): m_ey_z at the Hx rows, m_ey_x at the
+  // Hz rows, m_hz_x and m_hx_z at the Ey rows of step t+1.
+  T mez[NX], mex[NZ], mhzx[NZ], mhxz[NZ];
+  if constexpr (GEN) {
+    const T *__restrict__ mez_i = a.mez_i + base;
+    const T *__restrict__ mex_i = a.mex_i + base;
+    const T *__restrict__ mhzx_i = a.mhzx_i + base;
+    const T *__restrict__ mhxz_i = a.mhxz_i + base;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      const int y = y0 - 2 + k;
+      const bool pml_yh = y < g.pml_y0 || y >= g.pml_y1h;
+      mez[k] = pml_yh ? L(mez_i, y) : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      const int y = y0 - 1 + k;
+      const bool pml_ye = y < g.pml_y0 || y >= g.pml_y1;
+      mex[k] = pml_xh ? L(mex_i, y) : (T)0;
+      mhzx[k] = pml_xe ? L(mhzx_i, y) : (T)0;
+      mhxz[k] = pml_ye ? L(mhxz_i, y) : (T)0;
+    }
+  }
+
+  // ------------------------------------------------------------- step t
+  // Hx^{t+1/2} rows y0-2..y0+TY.
+  T hx1[NX];
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    const int y = y0 - 2 + k;
+    hx1[k] = hx0[k];
+    if (!GEN || (e_col && y >= 1 && y < ny - 1)) {
+      T d = (ey0[k + 1] - ey0[k]) * g.rdy;
+      if (GEN && (y < g.pml_y0 || y >= g.pml_y1h)) {
+        const T m = g.Y(kBh, y) * mez[k] + g.Y(kAh, y) * d;
+        mez[k] = m;
+        d = d / g.Y(kKh, y) + m;
+      }
+      hx1[k] = hx0[k] - cqv[k] * d;
+    }
+  }
+  // Hz^{t+1/2} and Ey^{t+1} rows y0-1..y0+TY.
+  T hz1[NZ], ey1[NZ];
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    const int y = y0 - 1 + k;
+    const bool row = !GEN || (y >= 1 && y < ny);
+    const T ey_c = ey0[k + 1];
+    const T ey_r = shfl_dn1(ey_c);  // unused at lane 63
+    hz1[k] = hz0[k];
+    if (!GEN || (hz_col && row)) {
+      T d = (ey_r - ey_c) * g.rdx;
+      if (GEN && pml_xh) {
+        const T m = g.X(kBh, x) * mex[k] + g.X(kAh, x) * d;
+        mex[k] = m;
+        d = d / g.X(kKh, x) + m;
+      }
+      hz1[k] = hz0[k] + cqv[k + 1] * d;
+    }
+    const T hz_l = shfl_up1(hz1[k]);  // unused at lane 0
+    T ey_n = ey_c;
+    if (!GEN || (e_col && row)) {
+      T dhz = (hz1[k] - hz_l) * g.rdx;
+      T dhx = (hx1[k + 1] - hx1[k]) * g.rdy;
+      if (GEN && pml_xe) {
+        const T m = g.X(kB, x) * mhzx[k] + g.X(kA, x) * dhz;
+        mhzx[k] = m;
+        dhz = dhz / g.X(kK, x) + m;
+      }
+      if (GEN && (y < g.pml_y0 || y >= g.pml_y1)) {
+        const T m = g.Y(kB, y) * mhxz[k] + g.Y(kA, y) * dhx;
+        mhxz[k] = m;
+        dhx = dhx / g.Y(kK, y) + m;
+      }
+      const T curl = dhz - dhx;
+      if constexpr (FMT != 0) {
+        if (own && k >= 1 && k <= TY) {
+          const int64_t i = base + (int64_t)y * nx + x;
+          if (a.ey_store0) put_store<T, FMT>(a.ey_store0, i, ey_c);
+          if (a.curl_store0) put_store<T, FMT>(a.curl_store0, i, curl);
+        }
+      }
+      ey_n = cav[k] * ey_c + cbv[k] * curl;
+    }
+    if (!GEN || (y >= 0 && y < ny))
+      fused_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.f0, a.r0, own && k >= 1 && k <= TY, ey_n);
+    ey1[k] = ey_n;
+  }
+
+  // ----------------------------------------------------------- step t+1
+  // Hx^{t+3/2} rows y0-1..y0+TY-1 (index k <-> row y0-1+k, from Hx^{t+1/2}
+  // row index k+1).
+  T hx2[TY + 1];
+#pragma unroll
+  for (int k = 0; k < TY + 1; ++k) {
+    const int y = y0 - 1 + k;
+    hx2[k] = hx1[k + 1];
+    if (!GEN || (e_col && y >= 1 && y < ny - 1)) {
+      T d = (ey1[k + 1] - ey1[k]) * g.rdy;
+      if (GEN && (y < g.pml_y0 || y >= g.pml_y1h)) {
+        const T m = g.Y(kBh, y) * mez[k + 1] + g.Y(kAh, y) * d;
+        mez[k + 1] = m;
+        d = d / g.Y(kKh, y) + m;
+      }
+      hx2[k] = hx1[k + 1] - cqv[k + 1] * d;
+    }
+  }
+  T *__restrict__ ey_o = a.ey_o + base;
+  T *__restrict__ hx_o = a.hx_o + base;
+  T *__restrict__ hz_o = a.hz_o + base;
+#pragma unroll
+  for (int k = 0; k < TY; ++k) {
+    const int y = y0 + k;
+    if (GEN && y >= ny) break;  // uniform
+    const bool row = !GEN || y >= 1;
+    const int j = y * nx + x;
+    const T ey_c = ey1[k + 1];
+    const T ey_r = shfl_dn1(ey_c);
+    T hz_n = hz1[k + 1];
+    if (!GEN || (hz_col && row)) {
+      T d = (ey_r - ey_c) * g.rdx;
+      if (GEN && pml_xh) {
+        const T m = g.X(kBh, x) * mex[k + 1] + g.X(kAh, x) * d;
+        mex[k + 1] = m;
+        d = d / g.X(kKh, x) + m;
+      }
+      hz_n = hz1[k + 1] + cqv[k + 2] * d;
+    }
+    const T hz_l = shfl_up1(hz_n);
+    T ey_n = ey_c;
+    if (!GEN || (e_col && row)) {
+      T dhz = (hz_n - hz_l) * g.rdx;
+      T dhx = (hx2[k + 1] - hx2[k]) * g.rdy;
+      if (GEN && pml_xe) {
+        const T m = g.X(kB, x) * mhzx[k + 1] + g.X(kA, x) * dhz;
+        mhzx[k + 1] = m;
+        dhz = dhz / g.X(kK, x) + m;
+      }
+      if (GEN && (y < g.pml_y0 || y >= g.pml_y1)) {
+        const T m = g.Y(kB, y) * mhxz[k + 1] + g.Y(kA, y) * dhx;
+        mhxz[k + 1] = m;
+        dhx = dhx / g.Y(kK, y) + m;
+      }
+      const T curl = dhz - dhx;
+      if constexpr (FMT != 0) {
+        if (own) {
+          const int64_t i = base + j;
+          if (a.ey_store1) put_store<T, FMT>(a.ey_store1, i, ey_c);
+          if (a.curl_store1) put_store<T, FMT>(a.curl_store1, i, curl);
+        }
+      }
+      ey_n = cav[k + 1] * ey_c + cbv[k + 1] * curl;
+    }
+    fused_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.f1, a.r1, own, ey_n);
+    if (own) {
+      ey_o[j] = ey_n;
+      hx_o[j] = hx2[k + 1];
+      hz_o[j] = hz_n;
+      if constexpr (GEN) {
+        if (y < g.pml_y0 || y >= g.pml_y1h) a.mez_o[base + j] = mez[k + 2];
+        if (pml_xh) a.mex_o[base + j] = mex[k + 1];
+        if (pml_xe) a.mhzx_o[base + j] = mhzx[k + 1];
+        if (y < g.pml_y0 || y >= g.pml_y1) a.mhxz_o[base + j] = mhxz[k + 1];
+      }
+    }
+  }
+}
+
+template <typename T, int TY, int FMT>
+__device__ __forceinline__ void fwd2_workgroup(const Fwd2Args<T> &a, const TileCoord &tc) {
+  const int G = (int)blockDim.y;
+  const int s = tc.grp * G + (int)threadIdx.y;
+  if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
+  if (fused_interior<T, TY>(a.g, tc.xt, tc.yt))
+    fwd2_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt);
+  else
+    fwd2_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt);
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(512) void fwd2_step(Fwd2Args<T> a) {
+  const int G = (int)blockDim.y;
+  const int n_groups = (a.n_shots + G - 1) / G;
+  fwd2_workgroup<T, TY, FMT>(a, decode_tile(a.pts.n_xt, a.n_yt, n_groups));
+}
+
+// ----------------------------------------------------------------- adjoint
+template <typename T> struct Adj2Args {
+  Geom<T> g;
+  Points pts;  // built with build_point_table_halo(tile_w 60, lane offset 2)
+  const T *ca, *cb, *cq;
+  const T *ley_i, *lhx_i, *lhz_i;
+  T *ley_o, *lhx_o, *lhz_o;
+  // adjoint CPML memories (mx: m_hz_x, my: m_hx_z, nx: m_ey_x, nz: m_ey_z)
+  const T *mx_i, *my_i, *nx_i, *nz_i;
+  T *mx_o, *my_o, *nx_o, *nz_o;
+  // Iteration A is the transpose of forward step t, iteration B of step t-1.
+  const void *ey_store0, *curl_store0;  // snapshots of step t (null: no gradient term)
+  const void *ey_store1, *curl_store1;  // snapshots of step t-1
+  double *gca, *gcb;
+  int32_t per_shot_grad;
+  double step_scale;
+  const T *grad_r0, *grad_r1;  // receiver cotangents of steps t, t-1
+  T *grad_f0, *grad_f1;        // source-term gradients of steps t, t-1
+  int32_t n_shots, n_yt;
+  int64_t group_stride;
+};
+
+// Applies the point entries of (row y, shot s, tile xt) to `ley`: receivers
+// of this lane (owner entries and halo copies) inject grad_r[q]; sources
+// (owner entries only, and only when `record`) read grad_f[q] = ley.
+template <typename T>
+__device__ __forceinline__ void fused_adj_points(const Points &pts, int n_shots, int s, int xt, int y,
+                                                 int lane, const T *gr, T *gf, bool record, T &ley) {
+  const int slot = pts.row_slot[y];
+  if (slot < 0) return;
+  const int64_t si = ((int64_t)slot * n_shots + s) * pts.n_xt + xt;
+  const int e0 = pts.seg[si], e1 = pts.seg[si + 1];
+  if (gr != nullptr) {
+    for (int e = e0; e < e1; ++e) {
+      const int2 en = pts.ent[e];
+      if ((en.x & 0xff) == lane && ((en.x >> 8) & 1) == 1) ley += gr[en.y];
+    }
+  }
+  if (record && gf != nullptr) {
+    for (int e = e0; e < e1; ++e) {
+      const int2 en = pts.ent[e];
+      if ((en.x & 0xff) == lane && (en.x >> 8) == 0) gf[en.y] = ley;
+    }
+  }
+}
+
+// Two reverse iterations of TY rows of one shot.  The per-shot FP64
+// gradient terms of iteration A / B go to red_a / red_b ([TY][64] LDS
+// slots each, per coefficient) when `reduce`, else straight to the
+// per-shot gradient arrays.
+template <typename T, int TY, int FMT, bool GEN>
+__device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, int yt,
+                                          double *red_ca_a, double *red_cb_a, double *red_ca_b,
+                                          double *red_cb_b, bool reduce) {
+  const Geom<T> &g = a.g;
+  const int ny = g.ny, nx = g.nx;
+  const int lane = threadIdx.x;
+  const int x = xt * kOwn2 - 2 + lane;
+  const int y0 = yt * TY;
+  const bool own = lane >= 2 && lane <= 61 && x < nx;
+  const bool e_col = !GEN || (x >= 1 && x < nx);
+  const bool pml_xe = GEN && (x < g.pml_x0 || x >= g.pml_x1);
+  const bool pml_xh = GEN && (x < g.pml_x0 || x >= g.pml_x1h);
+  const bool grad_a = FMT != 0 && (a.ey_store0 != nullptr || a.curl_store0 != nullptr);
+  const bool grad_b = FMT != 0 && (a.ey_store1 != nullptr || a.curl_store1 != nullptr);
+
+  const int64_t base = (int64_t)s * g.numel;
+  const T *__restrict__ ca = a.ca + (g.ca_b ? base : 0);
+  const T *__restrict__ cb = a.cb + (g.cb_b ? base : 0);
+  const T *__restrict__ cq = a.cq + (g.cq_b ? base : 0);
+  const RowLoader<T, GEN> L(x, nx, ny);
+
+  // Rows: lambda_Ey and cb y0-2..y0+TY+1, lambda_Hx and cq y0-2..y0+TY,
+  // lambda_Hz and ca y0-1..y0+TY.
+  constexpr int NE = TY + 4, NX = TY + 3, NZ = TY + 2;
+  T ley0[NE], cbv[NE], lhx0[NX], cqv[NX], lhz0[NZ], cav[NZ];
+  {
+    const T *__restrict__ ley_i = a.ley_i + base;
+    const T *__restrict__ lhx_i = a.lhx_i + base;
+    const T *__restrict__ lhz_i = a.lhz_i + base;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      ley0[k] = L(ley_i, y0 - 2 + k);
+      cbv[k] = L(cb, y0 - 2 + k);
+    }
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      lhx0[k] = L(lhx_i, y0 - 2 + k);
+      cqv[k] = L(cq, y0 - 2 + k);
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      lhz0[k] = L(lhz_i, y0 - 1 + k);
+      cav[k] = L(ca, y0 - 1 + k);
+    }
+  }
+  // Snapshots of the owned rows (exact in fp32 for fp32 / BF16 payloads).
+  float se0[TY], sc0[TY], se1[TY], sc1[TY];
+#pragma unroll
+  for (int k = 0; k < TY; ++k) {
+    se0[k] = sc0[k] = se1[k] = sc1[k] = 0.0f;
+    if constexpr (sizeof(T) == 4 && FMT != 0) {
+      const int y = y0 + k;
+      if (own && (!GEN || (y >= 1 && y < ny && x >= 1))) {
+        const int64_t i = base + (int64_t)y * nx + x;
+        if (grad_a && a.ey_store0) se0[k] = (float)get_store<T, FMT>(a.ey_store0, i);
+        if (grad_a && a.curl_store0) sc0[k] = (float)get_store<T, FMT>(a.curl_store0, i);
+        if (grad_b && a.ey_store1) se1[k] = (float)get_store<T, FMT>(a.ey_store1, i);
+        if (grad_b && a.curl_store1) sc1[k] = (float)get_store<T, FMT>(a.curl_store1, i);
+      }
+    }
+  }
+  // CPML memories (GEN only): my and nz indexed from row y0-2, mx and nx
+  // from row y0-1.
+  T my[NE], nz[NX], mx[NZ], nxm[NZ];
+  if constexpr (GEN) {
+    const T *__restrict__ mx_i = a.mx_i + base;
+    const T *__restrict__ my_i = a.my_i + base;
+    const T *__restrict__ nx_i = a.nx_i + base;
+    const T *__restrict__ nz_i = a.nz_i + base;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int y = y0 - 2 + k;
+      my[k] = (y < g.pml_y0 || y >= g.pml_y1) ? L(my_i, y) : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+      const int y = y0 - 2 + k;
+      nz[k] = (y < g.pml_y0 || y >= g.pml_y1h) ? L(nz_i, y) : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      const int y = y0 - 1 + k;
+      mx[k] = pml_xe ? L(mx_i, y) : (T)0;
+      nxm[k] = pml_xh ? L(nx_i, y) : (T)0;
+    }
+  }
+
+  // The single-step kernel's terms (adj_tile), each also advancing its CPML
+  // memory `m` where it is updated.
+  auto raw_x = [&](int y, T lm, T cbx, T &m) -> T {  // adjoint of dHz/dx
+    if (GEN && (y < 1 || y >= ny || x < 1 || x >= nx)) return (T)0;
+    const T q = cbx * lm;
+    if (!pml_xe) return q;
+    const T mn = g.X(kB, x) * m + q;
+    m = mn;
+    return g.X(kA, x) * mn + q / g.X(kK, x);
+  };
+  auto nraw_y = [&](int y, T lm, T cbx, T &m) -> T {  // negated adjoint of dHx/dz
+    if (GEN && (y < 1 || y >= ny || !e_col)) return (T)0;
+    const T q = cbx * lm;
+    if (!(GEN && (y < g.pml_y0 || y >= g.pml_y1))) return q;
+    const T mn = g.Y(kB, y) * m - q;
+    m = mn;
+    return q / g.Y(kK, y) - g.Y(kA, y) * mn;
+  };
+  auto raw_z = [&](int y, T lhxv, T cqx, T &m) -> T {  // adjoint of dEy/dz (Hx update)
+    if (GEN && (y < 1 || y >= ny - 1 || !e_col)) return (T)0;
+    const T adj = -cqx * lhxv;
+    if (!(GEN && (y < g.pml_y0 || y >= g.pml_y1h))) return adj;
+    const T mn = g.Y(kBh, y) * m + adj;
+    m = mn;
+    return g.Y(kAh, y) * mn + adj / g.Y(kKh, y);
+  };
+  auto raw_xh = [&](int y, T lhzv, T cqx, T &m) -> T {  // adjoint of dEy/dx (Hz update)
+    if (GEN && (y < 1 || y >= ny || x < 1 || x >= nx - 1)) return (T)0;
+    const T adj = cqx * lhzv;
+    if (!pml_xh) return adj;
+    const T mn = g.X(kBh, x) * m + adj;
+    m = mn;
+    return g.X(kAh, x) * mn + adj / g.X(kKh, x);
+  };
+  auto in_e = [&](int y) -> bool { return e_col && (!GEN || (y >= 1 && y < ny)); };
+  auto grad_terms = [&](bool on, int y, T lm_t, float se, float sc, const void *es, const void *cs,
+                        double *ra, double *rb, int k) {
+    if (!on || !own || !in_e(y)) return;
+    const double lm = (double)lm_t;
+    double ve, vc;
+    if constexpr (sizeof(T) == 4) {
+      ve = (double)se;
+      vc = (double)sc;
+    } else {
+      const int64_t i = base + (int64_t)y * nx + x;
+      ve = es ? get_store<T, FMT>(es, i) : 0.0;
+      vc = cs ? get_store<T, FMT>(cs, i) : 0.0;
+    }
+    const double pa = lm * ve * a.step_scale;
+    const double pb = lm * vc * a.step_scale;
+    if (reduce) {
+      ra[k * 64 + lane] = pa;
+      rb[k * 64 + lane] = pb;
+    } else {
+      const int64_t gi = (a.per_shot_grad ? base : 0) + (int64_t)y * nx + x;
+      if (a.gca && es) a.gca[gi] += pa;
+      if (a.gcb && cs) a.gcb[gi] += pb;
+    }
+  };
+
+  // ---------------------------------------------------- iteration A (t)
+  T nrA[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) nrA[k] = nraw_y(y0 - 2 + k, ley0[k], cbv[k], my[k]);
+  T lhxA[NX], rzA[NX];
+#pragma unroll
+  for (int k = 0; k < NX; ++k) {
+    const int y = y0 - 2 + k;
+    lhxA[k] = lhx0[k];
+    if (in_e(y)) lhxA[k] = lhx0[k] + (nrA[k + 1] - nrA[k]) * g.rdy;
+    rzA[k] = raw_z(y, lhxA[k], cqv[k], nz[k]);
+  }
+  T lhzA[NZ], leyA[NZ];
+#pragma unroll
+  for (int k = 0; k < NZ; ++k) {
+    const int y = y0 - 1 + k;
+    const T ley_c = ley0[k + 1];
+    const T rx = raw_x(y, ley_c, cbv[k + 1], mx[k]);
+    const T rx_r = shfl_dn1(rx);  // unused at lane 63
+    lhzA[k] = lhz0[k];
+    if (in_e(y)) lhzA[k] = lhz0[k] + (rx - rx_r) * g.rdx;
+    const T rxh = raw_xh(y, lhzA[k], cqv[k + 1], nxm[k]);
+    const T rxh_l = shfl_up1(rxh);  // unused at lane 0
+    T ley_n = ley_c;
+    if (in_e(y)) {
+      const T t1 = cav[k] * ley_c;
+      const T t2 = (rzA[k] - rzA[k + 1]) * g.rdy + (rxh_l - rxh) * g.rdx;
+      ley_n = t1 + t2;
+    }
+    const bool owned_row = k >= 1 && k <= TY;
+    if (!GEN || (y >= 0 && y < ny))
+      fused_adj_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.grad_r0, a.grad_f0, own && owned_row, ley_n);
+    if (owned_row)
+      grad_terms(grad_a, y, ley_n, se0[k - 1], sc0[k - 1], a.ey_store0, a.curl_store0, red_ca_a,
+                 red_cb_a, k - 1);
+    leyA[k] = ley_n;
+  }
+
+  // -------------------------------------------------- iteration B (t-1)
+  T nrB[TY + 2];
+#pragma unroll
+  for (int k = 0; k < TY + 2; ++k) nrB[k] = nraw_y(y0 - 1 + k, leyA[k], cbv[k + 1], my[k + 1]);
+  T lhxB[TY + 1], rzB[TY + 1];
+#pragma unroll
+  for (int k = 0; k < TY + 1; ++k) {
+    const int y = y0 - 1 + k;
+    lhxB[k] = lhxA[k + 1];
+    if (in_e(y)) lhxB[k] = lhxA[k + 1] + (nrB[k + 1] - nrB[k]) * g.rdy;
+    rzB[k] = raw_z(y, lhxB[k], cqv[k + 1], nz[k + 1]);
+  }
+  T *__restrict__ ley_o = a.ley_o + base;
+  T *__restrict__ lhx_o = a.lhx_o + base;
+  T *__restrict__ lhz_o = a.lhz_o + base;
+#pragma unroll
+  for (int k = 0; k < TY; ++k) {
+    const int y = y0 + k;
+    if (GEN && y >= ny) break;  // uniform
+    const int j = y * nx + x;
+    const T ley_c = leyA[k + 1];
+    const T rx = raw_x(y, ley_c, cbv[k + 2], mx[k + 1]);
+    const T rx_r = shfl_dn1(rx);
+    T lhz_n = lhzA[k + 1];
+    if (in_e(y)) lhz_n = lhzA[k + 1] + (rx - rx_r) * g.rdx;
+    const T rxh = raw_xh(y, lhz_n, cqv[k + 2], nxm[k + 1]);
+    const T rxh_l = shfl_up1(rxh);
+    T ley_n = ley_c;
+    if (own && in_e(y)) {
+      const T t1 = cav[k + 1] * ley_c;
+      const T t2 = (rzB[k] - rzB[k + 1]) * g.rdy + (rxh_l - rxh) * g.rdx;
+      ley_n = t1 + t2;
+    }
+    fused_adj_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.grad_r1, a.grad_f1, own, ley_n);
+    grad_terms(grad_b, y, ley_n, se1[k], sc1[k], a.ey_store1, a.curl_store1, red_ca_b, red_cb_b, k);
+    if (own) {
+      ley_o[j] = ley_n;
+      lhx_o[j] = lhxB[k + 1];
+      lhz_o[j] = lhz_n;
+      if constexpr (GEN) {
+        if (in_e(y)) {
+          if (pml_xe) a.mx_o[base + j] = mx[k + 1];
+          if (y < g.pml_y0 || y >= g.pml_y1) a.my_o[base + j] = my[k + 2];
+          if (y < ny - 1 && (y < g.pml_y0 || y >= g.pml_y1h)) a.nz_o[base + j] = nz[k + 2];
+          if (x < nx - 1 && pml_xh) a.nx_o[base + j] = nxm[k + 1];
+        }
+      }
+    }
+  }
+}
+
+// One two-step adjoint workgroup; `red` is the dynamic LDS block
+// [2 coefficients][2 iterations][G][TY][64].  Each iteration's shot sum is
+// added to the cell separately, in iteration order (as two single steps do).
+template <typename T, int TY, int FMT>
+__device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileCoord &tc, double *red) {
+  const int lane = threadIdx.x;
+  const int w = (int)threadIdx.y;
+  const int G = (int)blockDim.y;
+  const int s = tc.grp * G + w;
+  const bool any_grad = FMT != 0 && (a.gca != nullptr || a.gcb != nullptr);
+  const bool reduce = any_grad && !a.per_shot_grad;
+  const int64_t slab = (int64_t)G * TY * 64;  // one [G][TY][64] block
+  double *ca_a = red + (int64_t)w * TY * 64;
+  double *ca_b = ca_a + slab;
+  double *cb_a = ca_a + 2 * slab;
+  double *cb_b = ca_a + 3 * slab;
+  if (reduce) {
+#pragma unroll
+    for (int r = 0; r < TY; ++r) {
+      ca_a[r * 64 + lane] = 0.0;
+      ca_b[r * 64 + lane] = 0.0;
+      cb_a[r * 64 + lane] = 0.0;
+      cb_b[r * 64 + lane] = 0.0;
+    }
+  }
+  if (s < a.n_shots) {  // inactive wavefronts still join the barrier below
+    if (fused_interior<T, TY>(a.g, tc.xt, tc.yt))
+      adj2_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+    else
+      adj2_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+  }
+  if (!reduce) return;  // uniform over the workgroup
+  __syncthreads();
+  const int x = tc.xt * kOwn2 - 2 + lane;
+  const int y0 = tc.yt * TY;
+  if (lane < 2 || lane > 61 || x < 1 || x >= a.g.nx) return;
+  const bool step_a = a.ey_store0 != nullptr || a.curl_store0 != nullptr;
+  const bool step_b = a.ey_store1 != nullptr || a.curl_store1 != nullptr;
+  for (int pass = 0; pass < 2; ++pass) {
+    double *out = pass == 0 ? a.gca : a.gcb;
+    const void *sa = pass == 0 ? a.ey_store0 : a.curl_store0;
+    const void *sb = pass == 0 ? a.ey_store1 : a.curl_store1;
+    if (out == nullptr || (!sa && !sb)) continue;
+    out += tc.grp * a.group_stride;
+    const double *slots_a = red + (int64_t)(2 * pass) * slab;
+    const double *slots_b = slots_a + slab;
+    for (int r = w; r < TY; r += G) {
+      const int y = y0 + r;
+      if (y >= 1 && y < a.g.ny) {
+        const int64_t j = (int64_t)y * a.g.nx + x;
+        double v = out[j];
+        if (step_a && sa) {
+          double sum = 0.0;
+          for (int k = 0; k < G; ++k) sum += slots_a[(k * TY + r) * 64 + lane];
+          v += sum;
+        }
+        if (step_b && sb) {
+          double sum = 0.0;
+          for (int k = 0; k < G; ++k) sum += slots_b[(k * TY + r) * 64 + lane];
+          v += sum;
+        }
+        out[j] = v;
+      }
+    }
+  }
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(512) void adj2_step(Adj2Args<T> a) {
+  extern __shared__ double red[];
+  const int G = (int)blockDim.y;
+  const int n_groups = (a.n_shots + G - 1) / G;
+  adj2_workgroup<T, TY, FMT>(a, decode_tile(a.pts.n_xt, a.n_yt, n_groups), red);
+}
+
+}  // namespace tm2d
+}  // namespace tide_hip

@@ -231,6 +231,10 @@ def load() -> ctypes.CDLL:
     lib.tide_tm2d_set_shot_loop.argtypes = [ctypes.c_int] * 2
     lib.tide_tm2d_get_shot_loop.restype = ctypes.c_int
     lib.tide_tm2d_get_shot_loop.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    lib.tide_tm2d_set_time_block.restype = ctypes.c_int
+    lib.tide_tm2d_set_time_block.argtypes = [ctypes.c_int] * 2
+    lib.tide_tm2d_get_time_block.restype = ctypes.c_int
+    lib.tide_tm2d_get_time_block.argtypes = [ctypes.POINTER(ctypes.c_int)]
     _lib = lib
     return lib
 
@@ -307,6 +311,17 @@ def get_tm2d_shot_loop() -> tuple[bool, bool]:
     return bool(out[0]), bool(out[1])
 
 
+def set_tm2d_time_block(fwd_steps: int, adj_steps: int) -> None:
+    """Time steps per launch (1 or 2) of the TM2D kernels per pass (see include/tide_hip.h)."""
+    check(load().tide_tm2d_set_time_block(int(fwd_steps), int(adj_steps)))
+
+
+def get_tm2d_time_block() -> tuple[int, int]:
+    out = (ctypes.c_int * 2)()
+    check(load().tide_tm2d_get_time_block(out))
+    return int(out[0]), int(out[1])
+
+
 def stream_handle(device: torch.device) -> int | None:
     """The HIP stream of torch's current stream on ``device``."""
     return torch.cuda.current_stream(device).cuda_stream
@@ -317,11 +332,13 @@ __all__ = [
     "check",
     "dtype_name",
     "get_tm2d_shot_loop",
+    "get_tm2d_time_block",
     "get_tm2d_tuning",
     "int64_array",
     "ptr_array",
     "set_em3d_tuning",
     "set_tm2d_shot_loop",
+    "set_tm2d_time_block",
     "set_tm2d_tuning",
     "get_function",
     "library_path",
