@@ -64,7 +64,7 @@ def parse() -> argparse.Namespace:
                     help="served-solve trace length (0: skip the serving measurement)")
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-seed", type=int, default=2024)
-    ap.add_argument("--serve-tick", type=int, default=32, help="steps per engine tick")
+    ap.add_argument("--serve-tick", type=int, default=64, help="steps per engine tick")
     return ap.parse_args()
 
 
@@ -261,12 +261,12 @@ def main() -> None:
     if not args.no_autotune:
         from tide.autotune import tune_tm2d
         tuning = tune_tm2d(ny_p, nx_p, max(sizes), device=dev)
-        log(f"autotuned launch shapes {tuning['tuning']} shot loop {tuning['shot_loop']} "
+        log(f"autotuned launch shapes forward {tuning['forward']} adjoint {tuning['adjoint']} "
             f"in {tuning.get('seconds', 0)} s")
     else:
-        from tide import _native
-        tuning = {"tuning": _native.get_tm2d_tuning(), "shot_loop": _native.get_tm2d_shot_loop(),
-                  "cached": False}
+        from tide.autotune import current
+        fwd, adj = current()
+        tuning = {"forward": fwd, "adjoint": adj, "cached": False}
 
     disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
     storage = tide.StorageOptions(compression=args.compression == "bf16")
@@ -325,12 +325,16 @@ def main() -> None:
     b_alg = {"forward_store": 24 + b_snap, "adjoint": 24 + b_snap, "forward": 24}
     kernels = {}
     for name, d in summ.items():
+        spl = int(d["steps_per_launch"])
         per_launch_ms = d["ms"] / d["launches"]
         achieved = b_alg[name] * d["cell_updates"] / (d["ms"] * 1e-3) / 1e9
+        two = spl == 2
         kernels[name] = {
-            "kernel": "adj_step" if name == "adjoint" else "fwd_step",
+            "kernel": ("adj2_step" if two else "adj_step") if name == "adjoint"
+                      else ("fwd2_step" if two else "fwd_step"),
+            "steps_per_launch": spl,
             "avg_launch_us": round(per_launch_ms * 1e3, 2),
-            "bytes_per_cell": b_alg[name],
+            "bytes_per_cell_update": b_alg[name],
             "achieved_GBs": round(achieved, 1),
             "gcell_per_s": round(d["cell_updates"] / (d["ms"] * 1e-3) / 1e9, 2),
         }
@@ -338,8 +342,8 @@ def main() -> None:
     dom_k = kernels[dom]
     shots_per_launch = n_shots / n_chunks
     measured = MEASURED_TRAFFIC_B_PER_CELL.get(dom_k["kernel"]) if snap_bytes == 4 else None
-    traffic = (None if measured is None
-               else round(measured * ny_p * nx_p * shots_per_launch))
+    cells_per_launch = ny_p * nx_p * shots_per_launch * dom_k["steps_per_launch"]
+    traffic = None if measured is None else round(measured * cells_per_launch)
     roofline = {
         "bound": "hbm",
         "achieved": dom_k["achieved_GBs"],
@@ -348,7 +352,7 @@ def main() -> None:
         "frac": round(dom_k["achieved_GBs"] / HBM_PEAK_GBS, 4),
         "traffic": traffic,
         "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
-        "algorithmic_bytes_per_launch": round(b_alg[dom] * ny_p * nx_p * shots_per_launch),
+        "algorithmic_bytes_per_launch": round(b_alg[dom] * cells_per_launch),
         "kernel": dom_k["kernel"],
         "kernels": kernels,
     }
@@ -381,9 +385,11 @@ def main() -> None:
             "cell_updates_per_step": updates_per_step,
             "hbm_frac_whole_job": round(gcells * 1e9 * (24 + b_snap) / 1e9 / HBM_PEAK_GBS, 4),
             "misfit": float(misfit),
-            "launch_shape": {"fwd_rows_x_group": list(tuning["tuning"][:2]),
-                             "adj_rows_x_group": list(tuning["tuning"][2:]),
-                             "shot_loop": [int(v) for v in tuning["shot_loop"]]},
+            "launch_shape": {
+                "fields": ["rows", "group", "shot_loop", "steps_per_launch", "tile_split"],
+                "forward": list(tuning["forward"]),
+                "adjoint": list(tuning["adjoint"]),
+            },
             "seed": 1234,
         },
         "roofline": roofline,

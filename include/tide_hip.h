@@ -246,6 +246,15 @@ int tide_tm2d_get_shot_loop(int *out2);
 int tide_tm2d_set_time_block(int fwd_steps, int adj_steps);
 int tide_tm2d_get_time_block(int *out2);
 
+/*
+ * Two-step launches per pass: 1 runs the interior tiles (no CPML strip, no
+ * grid edge within the halo) and the border tiles as two kernels, each with
+ * its own register budget; 0 runs one kernel over all tiles.  Results are
+ * identical.  No reference counterpart.
+ */
+int tide_tm2d_set_tile_split(int fwd_split, int adj_split);
+int tide_tm2d_get_tile_split(int *out2);
+
 /* ------------------------------------------------------- batched 2-D plans
  * Many independent fp32 TM2D problems (different grid sizes, shot counts
  * and nt) advanced together: one launch per time step covers every problem
@@ -324,6 +333,9 @@ int tide_tm2d_engine_add(void *engine, const tide_tm2d_job *job, int64_t *job_id
  * *n_running = jobs still running after it. */
 int tide_tm2d_engine_tick(void *engine, int64_t max_steps, void *stream, int64_t *n_running);
 int tide_tm2d_engine_query(void *engine, int64_t job_id, int64_t *phase, int64_t *step);
+/* Ids (up to cap) of the jobs whose phase ended in the ticks since the last
+ * call, in the order they ended; *n = how many were written. */
+int tide_tm2d_engine_finished(void *engine, int64_t *ids, int64_t cap, int64_t *n);
 /* phase 1 -> 2; grad_r must hold the receiver cotangents by then (stream order). */
 int tide_tm2d_engine_begin_adjoint(void *engine, int64_t job_id);
 /* Forgets the job; after a finished adjoint enqueues the fold of the group

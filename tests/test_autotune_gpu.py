@@ -17,22 +17,19 @@ pytestmark = pytest.mark.gpu
 
 
 def test_autotune_installs_a_candidate_and_caches_it():
-    before = _native.get_tm2d_tuning()
-    before_loop = _native.get_tm2d_shot_loop()
+    from tide.autotune import _install, current
+
+    before = current()
     try:
         info = tune_tm2d(180, 220, 3, nt=16)
         print(info, flush=True)
-        fr, fg, ar, ag = info["tuning"]
-        fl, al = info["shot_loop"]
-        assert (fr, fg, fl) in FWD_CANDIDATES and (ar, ag, al) in ADJ_CANDIDATES
-        assert _native.get_tm2d_tuning() == info["tuning"]
-        assert _native.get_tm2d_shot_loop() == (bool(fl), bool(al))
+        assert info["forward"] in FWD_CANDIDATES and info["adjoint"] in ADJ_CANDIDATES
+        assert current() == (info["forward"], info["adjoint"])
         again = tune_tm2d(181, 219, 3, nt=16)
-        assert again["cached"] and again["tuning"] == info["tuning"]
-        assert again["shot_loop"] == info["shot_loop"]
+        assert again["cached"] and again["forward"] == info["forward"]
+        assert again["adjoint"] == info["adjoint"]
         case, expected = load_golden("tm2d_hetero_f32")
         _, r, _, _ = gpu_forward(case)
         assert np.array_equal(r, O.tm2d_forward(case)[1])
     finally:
-        _native.set_tm2d_tuning(*before)
-        _native.set_tm2d_shot_loop(*before_loop)
+        _install(*before)

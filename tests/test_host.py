@@ -223,7 +223,10 @@ def test_autotune_size_classes():
     assert a != _size_class(1065, 1065, 16, torch.float32)
     assert a != _size_class(300, 300, 7, torch.float32)
     assert a != _size_class(1065, 1065, 7, torch.float64)
-    assert all(r in (2, 4, 8, 16) and 1 <= g <= (4 if lp else 8) and lp in (0, 1)
-               for r, g, lp in FWD_CANDIDATES)
-    assert all(r in (2, 4, 8) and 1 <= g <= (4 if lp else 8) and lp in (0, 1)
-               for r, g, lp in ADJ_CANDIDATES)
+    for cands, rows in ((FWD_CANDIDATES, (2, 4, 8, 16)), (ADJ_CANDIDATES, (2, 4, 8))):
+        for r, g, lp, steps, split in cands:
+            assert r in rows and 1 <= g <= (4 if lp else 8) and lp in (0, 1)
+            assert steps in (1, 2) and split in (0, 1)
+            assert not (lp and steps == 2)  # two-step launches group shots per wavefront
+            if steps == 2:
+                assert r <= 8

@@ -26,6 +26,7 @@ from tide.serve import TM2DServer, latency_stats
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
+BUDGET = 2 << 30  # device pool of each test server
 
 SPECS = [
     dict(kind="forward", ny=40, nx=52, nt=60, shots=3, seed=1),
@@ -79,7 +80,7 @@ def problems():
 @pytest.fixture(scope="module")
 def mixed(problems):
     """All solves through one server, arriving 2 ms apart."""
-    server = TM2DServer(DEV, steps_per_tick=16)
+    server = TM2DServer(DEV, hbm_budget=BUDGET, steps_per_tick=16)
     trace = [(0.002 * i, dict(problem=p, **_submit_kwargs(s, p))) for i, (s, p) in enumerate(zip(SPECS, problems))]
     tickets = server.serve(trace)
     torch.cuda.synchronize()
@@ -107,7 +108,7 @@ def test_served_solves_match_the_operator(problems, mixed):
 def test_each_solve_is_the_same_alone_and_in_the_mix(problems, mixed):
     for i, (spec, p, t_mix) in enumerate(zip(SPECS, problems, mixed)):
         print(f"request {i} alone", flush=True)
-        server = TM2DServer(DEV, steps_per_tick=7)
+        server = TM2DServer(DEV, hbm_budget=BUDGET, steps_per_tick=7)
         t = server.submit(p, **_submit_kwargs(spec, p))
         server.drain()
         torch.cuda.synchronize()
@@ -121,12 +122,12 @@ def test_each_solve_is_the_same_alone_and_in_the_mix(problems, mixed):
 
 def test_small_budget_chunks_shots_and_queues(problems, mixed):
     spec, p, t_mix = SPECS[3], problems[3], mixed[3]
-    from tide.serve import job_bytes, padded_shape
+    from tide.serve import padded_shape, spec_bytes, state_spec
 
     ny_p, nx_p = padded_shape(spec["ny"], spec["nx"], p["pml_width"])
     n_rec = p["receiver_location"].shape[1]
-    one = job_bytes(ny_p, nx_p, 2, spec["nt"], 2, n_rec, gradient=True)
-    server = TM2DServer(DEV, hbm_budget=2 * one + 1, steps_per_tick=9)
+    one = spec_bytes(state_spec(ny_p, nx_p, 2, spec["nt"], 2, n_rec, gradient=True, group=2))
+    server = TM2DServer(DEV, hbm_budget=2 * one + 1024, steps_per_tick=9)
     tickets = [server.submit(p, **_submit_kwargs(spec, p)) for _ in range(2)]
     assert tickets[0].n_chunks == 5  # 9 shots in chunks of 2
     server.drain()
@@ -157,18 +158,17 @@ def test_engine_rejects_device_point_indices(problems):
     from tide import _native
     from tide.serve import _I64
 
-    server = TM2DServer(DEV)
+    server = TM2DServer(DEV, hbm_budget=BUDGET)
     spec, p = SPECS[0], problems[0]
     server.submit(p)
-    ch = server._waiting[0]
-    server._start(ch)  # admitted normally
+    server._admit()  # admitted normally
+    ch = next(iter(server._jobs.values()))
     job = ch.job
     dev_idx = torch.zeros(4, dtype=torch.int64, device=DEV)
     job.p.sources_i = dev_idx.data_ptr()
     jid = _I64()
     rc = server._lib.tide_tm2d_engine_add(server._engine, ctypes.byref(job), ctypes.byref(jid))
     assert rc != 0 and "host arrays" in _native.load().tide_last_error().decode()
-    server._waiting.clear()
     server.drain()
     server.close()
 
@@ -176,7 +176,7 @@ def test_engine_rejects_device_point_indices(problems):
 def test_many_small_solves_share_launches():
     """Forty single-shot 24x24 solves submitted at once finish in far fewer
     ticks than one after another would need."""
-    server = TM2DServer(DEV, steps_per_tick=32)
+    server = TM2DServer(DEV, hbm_budget=BUDGET, steps_per_tick=32)
     specs = [dict(kind="forward", ny=24, nx=24, nt=64, shots=1, seed=200 + i) for i in range(40)]
     ps = [_problem(s) for s in specs]
     tickets = [server.submit(p) for p in ps]

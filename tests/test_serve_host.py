@@ -56,6 +56,35 @@ def test_padded_shape_and_job_bytes():
     assert job_bytes(100, 120, 4, 4000, 1, 8, gradient=True) >= 2 * 4000 * 4 * 100 * 120 * 4
 
 
+def test_state_spec_and_pool():
+    from tide.serve import DevicePool, spec_bytes, state_spec
+
+    f = state_spec(30, 40, 3, 20, 1, 5, gradient=False)
+    g = state_spec(30, 40, 3, 20, 1, 5, gradient=True, group=2)
+    assert {n for n, *_ in f} == {"fields", "fields_b", "r"}
+    assert {"ey_store", "lambda", "gca_groups"} <= {n for n, *_ in g}
+    assert spec_bytes(f) < spec_bytes(g)
+    assert spec_bytes(f) % 256 == 0 and spec_bytes(f) >= 12 * 3 * 30 * 40 * 4
+    pool = DevicePool(1 << 20, torch.device("cpu"))
+    a = pool.alloc(1000)
+    b = pool.alloc(300_000)
+    c = pool.alloc(1000)
+    assert (a, b, c) == (0, 1024, 1024 + 300_032)
+    assert pool.alloc(1 << 20) is None
+    pool.release(b, 300_000)
+    assert pool.alloc(200_000) == 1024  # first fit reuses the hole
+    pool.release(a, 1000)
+    pool.release(c, 1000)
+    pool.release(1024, 200_000)
+    assert pool.free == [(0, pool.size)] and pool.available == pool.size
+    t = pool.carve(0, f)
+    assert len(t["fields"]) == 7 and t["fields"][0].shape == (3, 30, 40)
+    assert t["r"].shape == (20, 3, 5) and t["r"].dtype == torch.float32
+    views = t["fields"] + t["fields_b"] + [t["r"]]
+    spans = sorted((v.data_ptr(), v.data_ptr() + v.numel() * v.element_size()) for v in views)
+    assert all(e0 <= s1 for (_, e0), (s1, _) in zip(spans, spans[1:]))  # disjoint
+
+
 def test_plan_chunks():
     per_shot = lambda n: 10 * n  # noqa: E731
     assert plan_chunks(7, per_shot, 1000) == [(0, 7)]

@@ -333,58 +333,64 @@ def test_repeat_runs_are_bitwise_identical():
 
 
 LAUNCH_SHAPES = [
-    ((2, 8, 2, 8), (0, 0), (1, 1), "tm2d_hetero_f32"),
-    ((4, 4, 2, 4), (0, 0), (1, 1), "tm2d_hetero_f32"),
-    ((8, 8, 4, 8), (0, 0), (1, 1), "tm2d_hetero_f32"),
-    ((16, 2, 8, 2), (0, 0), (1, 1), "tm2d_hetero_f32"),
-    ((4, 1, 4, 3), (0, 0), (1, 1), "tm2d_hetero_f32"),
-    ((2, 4, 2, 4), (1, 1), (1, 1), "tm2d_hetero_f32"),
-    ((4, 2, 4, 1), (1, 1), (1, 1), "tm2d_hetero_f32"),
-    ((8, 3, 8, 3), (1, 1), (1, 1), "tm2d_hetero_f32"),
-    ((16, 4, 2, 2), (1, 0), (1, 1), "tm2d_hetero_f32"),
-    ((4, 4, 2, 8), (0, 1), (1, 1), "tm2d_hetero_f32"),
-    ((4, 4, 4, 4), (1, 1), (1, 1), "tm2d_hetero_f64"),
+    ((2, 8, 2, 8), (0, 0), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 2, 4), (0, 0), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((8, 8, 4, 8), (0, 0), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((16, 2, 8, 2), (0, 0), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((4, 1, 4, 3), (0, 0), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((2, 4, 2, 4), (1, 1), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((4, 2, 4, 1), (1, 1), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((8, 3, 8, 3), (1, 1), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((16, 4, 2, 2), (1, 0), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 2, 8), (0, 1), (1, 1), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 4, 4), (1, 1), (1, 1), (0, 1), "tm2d_hetero_f64"),
     # two time steps per launch
-    ((4, 4, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f32"),
-    ((2, 3, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f32"),
-    ((8, 2, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f32"),
-    ((16, 1, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f32"),
-    ((4, 4, 2, 4), (0, 0), (2, 1), "tm2d_hetero_f64"),
-    ((2, 2, 2, 4), (0, 0), (2, 1), "tm2d_config1_f32"),
-    ((4, 4, 2, 4), (0, 0), (2, 2), "tm2d_hetero_f32"),
-    ((4, 4, 4, 2), (0, 0), (1, 2), "tm2d_hetero_f32"),
-    ((4, 4, 2, 3), (1, 1), (2, 2), "tm2d_hetero_f32"),
-    ((4, 4, 4, 8), (0, 0), (2, 2), "tm2d_hetero_f32"),
-    ((4, 4, 2, 1), (0, 0), (2, 2), "tm2d_hetero_f64"),
-    ((4, 4, 4, 4), (0, 0), (2, 2), "tm2d_hetero_f64"),
-    ((2, 2, 2, 4), (0, 0), (2, 2), "tm2d_config1_f32"),
+    ((4, 4, 2, 4), (0, 0), (2, 1), (0, 1), "tm2d_hetero_f32"),
+    ((2, 3, 2, 4), (0, 0), (2, 1), (0, 1), "tm2d_hetero_f32"),
+    ((8, 2, 2, 4), (0, 0), (2, 1), (0, 1), "tm2d_hetero_f32"),
+    ((16, 1, 2, 4), (0, 0), (2, 1), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 2, 4), (0, 0), (2, 1), (0, 1), "tm2d_hetero_f64"),
+    ((2, 2, 2, 4), (0, 0), (2, 1), (0, 1), "tm2d_config1_f32"),
+    ((4, 4, 2, 4), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 4, 2), (0, 0), (1, 2), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 2, 3), (1, 1), (2, 2), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 4, 8), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f32"),
+    ((4, 4, 2, 1), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
+    ((4, 4, 4, 4), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
+    ((2, 2, 2, 4), (0, 0), (2, 2), (0, 1), "tm2d_config1_f32"),
+    # interior and border tiles as one kernel or two
+    ((4, 2, 4, 4), (0, 0), (2, 2), (1, 0), "tm2d_hetero_f32"),
+    ((2, 2, 2, 2), (0, 0), (2, 2), (1, 0), "tm2d_config1_f32"),
+    ((4, 2, 2, 4), (0, 0), (2, 2), (1, 1), "tm2d_hetero_f64"),
 ]
 
 
 @contextlib.contextmanager
-def launch_shape(tuning, loop, steps):
+def launch_shape(tuning, loop, steps, split=(0, 1)):
     before = (tide._native.get_tm2d_tuning(), tide._native.get_tm2d_shot_loop(),
-              tide._native.get_tm2d_time_block())
+              tide._native.get_tm2d_time_block(), tide._native.get_tm2d_tile_split())
     tide._native.set_tm2d_tuning(*tuning)
     tide._native.set_tm2d_shot_loop(*loop)
     tide._native.set_tm2d_time_block(*steps)
+    tide._native.set_tm2d_tile_split(*split)
     try:
         yield
     finally:
         tide._native.set_tm2d_tuning(*before[0])
         tide._native.set_tm2d_shot_loop(*before[1])
         tide._native.set_tm2d_time_block(*before[2])
+        tide._native.set_tm2d_tile_split(*before[3])
 
 
-@pytest.mark.parametrize("tuning,loop,steps,name", LAUNCH_SHAPES)
-def test_every_launch_shape_matches_oracle(tuning, loop, steps, name):
+@pytest.mark.parametrize("tuning,loop,steps,split,name", LAUNCH_SHAPES)
+def test_every_launch_shape_matches_oracle(tuning, loop, steps, split, name):
     """Any launch shape the autotuner may pick, in every kernel family (one
     shot per wavefront, one tile per wavefront walking the shots, one or two
     time steps per launch), gives the oracle's forward bitwise and its
     adjoint fields bitwise (gradients to FP64 rounding)."""
     case, _ = load_golden(name)
-    print(f"tuning {tuning} shot loop {loop} steps {steps} {name}", flush=True)
-    with launch_shape(tuning, loop, steps):
+    print(f"tuning {tuning} shot loop {loop} steps {steps} split {split} {name}", flush=True)
+    with launch_shape(tuning, loop, steps, split):
         f_o, r_o, es, cs = O.tm2d_forward(case, store="full")
         f_g, r_g, es_g, cs_g = gpu_forward(case, store="full")
         assert np.array_equal(r_g, r_o)

@@ -70,29 +70,56 @@ void launch_fwd(int rows, bool loop, dim3 grid, dim3 block, hipStream_t stream,
   }
 }
 
-template <typename T, int FMT>
-void launch_fwd2(int rows, dim3 grid, dim3 block, hipStream_t stream, const Fwd2Args<T> &a) {
-  if (rows == 2) {
-    fwd2_step<T, 2, FMT><<<grid, block, 0, stream>>>(a);
+// Number of interior and border tiles of a split.
+inline void split_counts(const TileSplit &sp, int n_xt, int n_yt, int64_t *inner, int64_t *border) {
+  *inner = (int64_t)(sp.xb - sp.xa) * (sp.yb - sp.ya);
+  *border = (int64_t)n_xt * n_yt - *inner;
+}
+
+// Two-step launches: split (interior and border tile sets as two kernels,
+// each with its own register budget) or one kernel over all tiles.
+template <typename T, int TY, int FMT>
+void launch_fwd2_parts(int64_t groups, dim3 block, hipStream_t stream, bool split,
+                       const Fwd2Args<T> &a) {
+  int64_t inner, border;
+  split_counts(a.split, a.pts.n_xt, a.n_yt, &inner, &border);
+  if (!split) {
+    fwd2_step<T, TY, FMT, kAllTiles><<<(unsigned)((inner + border) * groups), block, 0, stream>>>(a);
     return;
   }
-  if constexpr (sizeof(T) == 4) {
-    if (rows == 8) {
-      fwd2_step<T, 8, FMT><<<grid, block, 0, stream>>>(a);
-      return;
-    }
-  }
-  fwd2_step<T, 4, FMT><<<grid, block, 0, stream>>>(a);
+  if (inner > 0) fwd2_step<T, TY, FMT, kInnerTiles><<<(unsigned)(inner * groups), block, 0, stream>>>(a);
+  if (border > 0) fwd2_step<T, TY, FMT, kBorderTiles><<<(unsigned)(border * groups), block, 0, stream>>>(a);
 }
 
 template <typename T, int FMT>
-void launch_adj2(int rows, dim3 grid, dim3 block, size_t lds, hipStream_t stream,
-                 const Adj2Args<T> &a) {
-  if (rows == 2) {
-    adj2_step<T, 2, FMT><<<grid, block, lds, stream>>>(a);
+void launch_fwd2(int rows, int64_t groups, dim3 block, hipStream_t stream, bool split,
+                 const Fwd2Args<T> &a) {
+  if (rows == 2) return launch_fwd2_parts<T, 2, FMT>(groups, block, stream, split, a);
+  if constexpr (sizeof(T) == 4) {
+    if (rows == 8) return launch_fwd2_parts<T, 8, FMT>(groups, block, stream, split, a);
+  }
+  launch_fwd2_parts<T, 4, FMT>(groups, block, stream, split, a);
+}
+
+template <typename T, int TY, int FMT>
+void launch_adj2_parts(int64_t groups, dim3 block, size_t lds, hipStream_t stream, bool split,
+                       const Adj2Args<T> &a) {
+  int64_t inner, border;
+  split_counts(a.split, a.pts.n_xt, a.n_yt, &inner, &border);
+  if (!split) {
+    adj2_step<T, TY, FMT, kAllTiles><<<(unsigned)((inner + border) * groups), block, lds, stream>>>(a);
     return;
   }
-  adj2_step<T, 4, FMT><<<grid, block, lds, stream>>>(a);
+  if (inner > 0) adj2_step<T, TY, FMT, kInnerTiles><<<(unsigned)(inner * groups), block, lds, stream>>>(a);
+  if (border > 0)
+    adj2_step<T, TY, FMT, kBorderTiles><<<(unsigned)(border * groups), block, lds, stream>>>(a);
+}
+
+template <typename T, int FMT>
+void launch_adj2(int rows, int64_t groups, dim3 block, size_t lds, hipStream_t stream, bool split,
+                 const Adj2Args<T> &a) {
+  if (rows == 2) return launch_adj2_parts<T, 2, FMT>(groups, block, lds, stream, split, a);
+  launch_adj2_parts<T, 4, FMT>(groups, block, lds, stream, split, a);
 }
 
 template <typename T, int FMT>
@@ -275,7 +302,8 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     return (char *)buf + (size_t)(t / step_ratio) * store_step_bytes;
   };
   Fwd2Args<T> a2;
-  dim3 grid2(1);
+  const bool split2 = g_tuning.fwd_split != 0;
+  int64_t groups2 = 1;
   const dim3 block2(64, (unsigned)std::min<int>(p.n_shots, g_tuning.fwd_group));
   const int rows2 = std::min<int>(rows_for<T>(g_tuning.fwd_rows), sizeof(T) == 4 ? 8 : 4);
   if (fused) {
@@ -287,8 +315,8 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     a2.cq = p.cq;
     a2.n_shots = p.n_shots;
     a2.n_yt = (p.g.ny + rows2 - 1) / rows2;
-    const int64_t groups2 = (p.n_shots + block2.y - 1) / block2.y;
-    grid2 = dim3((unsigned)((int64_t)dpt2.n_xt * a2.n_yt * groups2));
+    a2.split = make_split(p.g, dpt2.n_xt, a2.n_yt, rows2);
+    groups2 = (p.n_shots + block2.y - 1) / block2.y;
   }
   int64_t t = start_t;
   for (; fused && t + 1 < start_t + nt; t += 2) {
@@ -308,11 +336,11 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     a2.curl_store1 = store_at(t + 1, curl_store);
     const bool st = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
     if (!st) {
-      launch_fwd2<T, 0>(rows2, grid2, block2, stream, a2);
+      launch_fwd2<T, 0>(rows2, groups2, block2, stream, split2, a2);
     } else if (fmt == 2) {
-      if constexpr (sizeof(T) == 4) launch_fwd2<T, 2>(rows2, grid2, block2, stream, a2);
+      if constexpr (sizeof(T) == 4) launch_fwd2<T, 2>(rows2, groups2, block2, stream, split2, a2);
     } else {
-      launch_fwd2<T, 1>(rows2, grid2, block2, stream, a2);
+      launch_fwd2<T, 1>(rows2, groups2, block2, stream, split2, a2);
     }
     for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
     for (int k = 0; k < 2; ++k) std::swap(cur_e[k], nxt_e[k]);
@@ -440,6 +468,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
   };
   int64_t t = start_t - 1;
   if (fused) {
+    const bool split2 = g_tuning.adj_split != 0;
     const int rows2 = std::min<int>(rows_for<T>(g_tuning.adj_rows), 4);
     Adj2Args<T> a2;
     std::memset(&a2, 0, sizeof(a2));
@@ -456,7 +485,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     a2.gca = ga_k;
     a2.gcb = gb_k;
     const dim3 block2 = block;  // same shot groups as the single-step launches
-    const dim3 grid2((unsigned)((int64_t)dpt2.n_xt * a2.n_yt * n_groups));
+    a2.split = make_split(p.g, dpt2.n_xt, a2.n_yt, rows2);
     const size_t lds2 = 4 * (size_t)block2.y * rows2 * 64 * sizeof(double);
     for (; t - 1 >= start_t - nt; t -= 2) {
       a2.ley_i = cur[0]; a2.lhx_i = cur[1]; a2.lhz_i = cur[2];
@@ -473,11 +502,11 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
       a2.curl_store1 = snap(t - 1, curl_store, gcb);
       const bool g_any = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
       if (!g_any) {
-        launch_adj2<T, 0>(rows2, grid2, block2, lds2, stream, a2);
+        launch_adj2<T, 0>(rows2, n_groups, block2, lds2, stream, split2, a2);
       } else if (fmt == 2) {
-        if constexpr (sizeof(T) == 4) launch_adj2<T, 2>(rows2, grid2, block2, lds2, stream, a2);
+        if constexpr (sizeof(T) == 4) launch_adj2<T, 2>(rows2, n_groups, block2, lds2, stream, split2, a2);
       } else {
-        launch_adj2<T, 1>(rows2, grid2, block2, lds2, stream, a2);
+        launch_adj2<T, 1>(rows2, n_groups, block2, lds2, stream, split2, a2);
       }
       for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
     }
@@ -838,6 +867,26 @@ extern "C" int tide_tm2d_get_time_block(int *out2) {
   }
   out2[0] = g_tuning.fwd_steps;
   out2[1] = g_tuning.adj_steps;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_set_tile_split(int fwd_split, int adj_split) {
+  if ((fwd_split != 0 && fwd_split != 1) || (adj_split != 0 && adj_split != 1)) {
+    set_error("tile split flags must be 0 or 1");
+    return kInvalidArgument;
+  }
+  g_tuning.fwd_split = fwd_split;
+  g_tuning.adj_split = adj_split;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_get_tile_split(int *out2) {
+  if (out2 == nullptr) {
+    set_error("tuning: null output");
+    return kInvalidArgument;
+  }
+  out2[0] = g_tuning.fwd_split;
+  out2[1] = g_tuning.adj_split;
   return kOk;
 }
 

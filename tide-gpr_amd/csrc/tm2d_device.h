@@ -665,11 +665,13 @@ constexpr int kMaxGroup = 8;  // shots (wavefronts) per workgroup
 // fwd_loop / adj_loop select the shot-looping kernels; their group is then
 // the number of tiles (wavefronts) per workgroup.
 // fwd_steps / adj_steps (1 or 2) are the time steps per launch: 2 selects
-// the two-step kernels (tm2d_fused.h) for pairs of steps.
+// the two-step kernels (tm2d_fused.h) for pairs of steps; fwd_split /
+// adj_split run their interior and border tiles as two kernels.
 struct Tuning {
-  int fwd_rows = 4, fwd_group = 4, adj_rows = 2, adj_group = 4;
+  int fwd_rows = 4, fwd_group = 1, adj_rows = 4, adj_group = 4;
   int fwd_loop = 0, adj_loop = 0;
-  int fwd_steps = 1, adj_steps = 1;
+  int fwd_steps = 2, adj_steps = 2;
+  int fwd_split = 0, adj_split = 1;
 };
 extern Tuning g_tuning;
 

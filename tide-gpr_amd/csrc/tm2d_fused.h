@@ -23,6 +23,57 @@ namespace tm2d {
 
 constexpr int kOwn2 = 60;  // owned columns per wavefront (lanes 2..61)
 
+// The interior tiles -- those whose loads and updates (halo of two
+// included) stay inside the grid and outside every CPML strip, see
+// make_split -- form the rectangle [xa, xb) x [ya, yb) of the n_xt x n_yt
+// tile grid; interior and border tiles run as
+// separate launches, so the interior kernel carries none of the CPML and
+// edge state (fewer live scalar registers, no spills).
+struct TileSplit {
+  int32_t xa, xb, ya, yb;
+};
+
+// Tile sets of a two-step launch.
+constexpr int kInnerTiles = 0;   // the interior rectangle (branch-free kernel)
+constexpr int kBorderTiles = 1;  // every other tile (CPML / edge kernel)
+constexpr int kAllTiles = 2;     // all tiles, the kind chosen per tile
+
+__device__ __forceinline__ bool in_split(const TileSplit &sp, int xt, int yt) {
+  return xt >= sp.xa && xt < sp.xb && yt >= sp.ya && yt < sp.yb;
+}
+
+// Tile and shot group of this workgroup within the interior (GEN false) or
+// border (GEN true) tile set.
+template <bool GEN>
+__device__ __forceinline__ TileCoord split_tile(const TileSplit &sp, int n_xt, int n_yt, int n_groups) {
+  const int iw = sp.xb - sp.xa, ih = sp.yb - sp.ya;
+  const int64_t n_tiles = GEN ? (int64_t)n_xt * n_yt - (int64_t)iw * ih : (int64_t)iw * ih;
+  const int64_t flat = xcd_remap((int64_t)blockIdx.x, n_tiles * n_groups);
+  TileCoord c;
+  c.grp = (int)(flat / n_tiles);
+  int i = (int)(flat - (int64_t)c.grp * n_tiles);
+  if (!GEN) {
+    c.yt = sp.ya + i / iw;
+    c.xt = sp.xa + i % iw;
+    return c;
+  }
+  const int top = sp.ya * n_xt, w = n_xt - iw, mid = ih * w;
+  if (i < top) {
+    c.yt = i / n_xt;
+    c.xt = i % n_xt;
+  } else if (i < top + mid) {
+    i -= top;
+    c.yt = sp.ya + i / w;
+    const int col = i % w;
+    c.xt = col < sp.xa ? col : sp.xb + (col - sp.xa);
+  } else {
+    i -= top + mid;
+    c.yt = sp.yb + i / n_xt;
+    c.xt = i % n_xt;
+  }
+  return c;
+}
+
 template <typename T> struct Fwd2Args {
   Geom<T> g;
   Points pts;  // built with build_point_table_halo(tile_w 60, lane offset 2)
@@ -36,16 +87,8 @@ template <typename T> struct Fwd2Args {
   void *ey_store0, *curl_store0;             // snapshots of step t (null: none)
   void *ey_store1, *curl_store1;             // snapshots of step t+1
   int32_t n_shots, n_yt;
+  TileSplit split;
 };
-
-// Tiles whose loads and updates (halo of two included) stay inside the
-// grid and outside every CPML strip.
-template <typename T, int TY>
-__device__ __forceinline__ bool fused_interior(const Geom<T> &g, int xt, int yt) {
-  const int x0 = xt * kOwn2 - 2, y0 = yt * TY;
-  return x0 >= 1 && x0 + 63 <= g.nx - 2 && x0 >= g.pml_x0 && x0 + 63 < g.pml_x1h &&
-         y0 - 2 >= 1 && y0 + TY + 1 <= g.ny - 2 && y0 - 2 >= g.pml_y0 && y0 + TY < g.pml_y1h;
-}
 
 // Applies the point entries of (row y, shot s, tile xt) to `ey`: sources of
 // this lane (owner entries and halo copies) add f[q]; receivers (owner
@@ -278,22 +321,19 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   }
 }
 
-template <typename T, int TY, int FMT>
-__device__ __forceinline__ void fwd2_workgroup(const Fwd2Args<T> &a, const TileCoord &tc) {
-  const int G = (int)blockDim.y;
-  const int s = tc.grp * G + (int)threadIdx.y;
-  if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
-  if (fused_interior<T, TY>(a.g, tc.xt, tc.yt))
-    fwd2_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt);
-  else
-    fwd2_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt);
-}
-
-template <typename T, int TY, int FMT>
+// SET selects the tiles: kInnerTiles, kBorderTiles or kAllTiles.
+template <typename T, int TY, int FMT, int SET>
 __global__ __launch_bounds__(512) void fwd2_step(Fwd2Args<T> a) {
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
-  fwd2_workgroup<T, TY, FMT>(a, decode_tile(a.pts.n_xt, a.n_yt, n_groups));
+  const TileCoord tc = SET == kAllTiles ? decode_tile(a.pts.n_xt, a.n_yt, n_groups)
+                                        : split_tile<SET == kBorderTiles>(a.split, a.pts.n_xt, a.n_yt, n_groups);
+  const int s = tc.grp * G + (int)threadIdx.y;
+  if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
+  if (SET == kInnerTiles || (SET == kAllTiles && in_split(a.split, tc.xt, tc.yt)))
+    fwd2_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt);
+  else
+    fwd2_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt);
 }
 
 // ----------------------------------------------------------------- adjoint
@@ -316,6 +356,7 @@ template <typename T> struct Adj2Args {
   T *grad_f0, *grad_f1;        // source-term gradients of steps t, t-1
   int32_t n_shots, n_yt;
   int64_t group_stride;
+  TileSplit split;
 };
 
 // Applies the point entries of (row y, shot s, tile xt) to `ley`: receivers
@@ -586,7 +627,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
 // One two-step adjoint workgroup; `red` is the dynamic LDS block
 // [2 coefficients][2 iterations][G][TY][64].  Each iteration's shot sum is
 // added to the cell separately, in iteration order (as two single steps do).
-template <typename T, int TY, int FMT>
+template <typename T, int TY, int FMT, int SET>
 __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileCoord &tc, double *red) {
   const int lane = threadIdx.x;
   const int w = (int)threadIdx.y;
@@ -609,7 +650,7 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
     }
   }
   if (s < a.n_shots) {  // inactive wavefronts still join the barrier below
-    if (fused_interior<T, TY>(a.g, tc.xt, tc.yt))
+    if (SET == kInnerTiles || (SET == kAllTiles && in_split(a.split, tc.xt, tc.yt)))
       adj2_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
     else
       adj2_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
@@ -650,12 +691,42 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
   }
 }
 
-template <typename T, int TY, int FMT>
+template <typename T, int TY, int FMT, int SET>
 __global__ __launch_bounds__(512) void adj2_step(Adj2Args<T> a) {
   extern __shared__ double red[];
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
-  adj2_workgroup<T, TY, FMT>(a, decode_tile(a.pts.n_xt, a.n_yt, n_groups), red);
+  const TileCoord tc = SET == kAllTiles ? decode_tile(a.pts.n_xt, a.n_yt, n_groups)
+                                        : split_tile<SET == kBorderTiles>(a.split, a.pts.n_xt, a.n_yt, n_groups);
+  adj2_workgroup<T, TY, FMT, SET>(a, tc, red);
+}
+
+// Host side of the split: the interior rectangle for TY-row tiles.  A tile
+// is interior when its columns x0-2..x0+61 and rows y0-2..y0+TY+1 lie in
+// [1, n-2] and outside the CPML strips (H-phase bounds, which contain the
+// E-phase ones); the test separates into a column and a row condition.
+// Empty when no tile qualifies.
+template <typename T>
+inline TileSplit make_split(const Geom<T> &g, int n_xt, int n_yt, int ty) {
+  auto col_ok = [&](int xt) {
+    const int x0 = xt * kOwn2 - 2;
+    return x0 >= 1 && x0 + 63 <= g.nx - 2 && x0 >= g.pml_x0 && x0 + 63 < g.pml_x1h;
+  };
+  auto row_ok = [&](int yt) {
+    const int y0 = yt * ty;
+    return y0 - 2 >= 1 && y0 + ty + 1 <= g.ny - 2 && y0 - 2 >= g.pml_y0 && y0 + ty < g.pml_y1h;
+  };
+  TileSplit sp{0, 0, 0, 0};
+  int xa = 0;
+  while (xa < n_xt && !col_ok(xa)) ++xa;
+  int xb = xa;
+  while (xb < n_xt && col_ok(xb)) ++xb;
+  int ya = 0;
+  while (ya < n_yt && !row_ok(ya)) ++ya;
+  int yb = ya;
+  while (yb < n_yt && row_ok(yb)) ++yb;
+  if (xb > xa && yb > ya) sp = TileSplit{xa, xb, ya, yb};
+  return sp;
 }
 
 }  // namespace tm2d

@@ -52,7 +52,43 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
   }
   const ServeItem &it = items[lo];
   const int k = it.k0 + j;
-  const TileCoord tc = batch_tile((int)(flat - start[lo]), it.b.pts.n_xt, it.b.n_yt);
+  const int local = (int)(flat - start[lo]);
+  if (it.b.n_shots == 1 && blockDim.y > 1) {
+    // A single-shot job gives each wavefront of the workgroup its own tile
+    // (no idle wavefronts, no cross-wavefront gradient sum).
+    const int n_xt = it.b.pts.n_xt;
+    const int tile = local * (int)blockDim.y + (int)threadIdx.y;
+    if (it.phase == 0) {
+      if (tile >= n_xt * it.b.n_yt) return;  // uniform over the wavefront; no barriers here
+      const int yt = tile / n_xt, xt = tile - yt * n_xt;
+      const FwdArgs<float> a = item_fwd_args(it.b, k);
+      const bool inner = interior_tile<float, TYF>(a.g, xt, yt);
+      if (it.fmt == 0)
+        inner ? fwd_tile<float, TYF, 0, false>(a, 0, xt, yt) : fwd_tile<float, TYF, 0, true>(a, 0, xt, yt);
+      else if (it.fmt == 2)
+        inner ? fwd_tile<float, TYF, 2, false>(a, 0, xt, yt) : fwd_tile<float, TYF, 2, true>(a, 0, xt, yt);
+      else
+        inner ? fwd_tile<float, TYF, 1, false>(a, 0, xt, yt) : fwd_tile<float, TYF, 1, true>(a, 0, xt, yt);
+    } else {
+      if (tile >= n_xt * it.b.n_yt) return;
+      const int yt = tile / n_xt, xt = tile - yt * n_xt;
+      const AdjArgs<float> a = item_adj_args(it.b, k);
+      const bool inner = interior_tile<float, TYA>(a.g, xt, yt);
+      if (it.fmt == 2) {
+        if (inner)
+          adj_tile<float, TYA, 2, false>(a, 0, xt, yt, nullptr, nullptr, false);
+        else
+          adj_tile<float, TYA, 2, true>(a, 0, xt, yt, nullptr, nullptr, false);
+      } else {
+        if (inner)
+          adj_tile<float, TYA, 1, false>(a, 0, xt, yt, nullptr, nullptr, false);
+        else
+          adj_tile<float, TYA, 1, true>(a, 0, xt, yt, nullptr, nullptr, false);
+      }
+    }
+    return;
+  }
+  const TileCoord tc = batch_tile(local, it.b.pts.n_xt, it.b.n_yt);
   if (it.phase == 0) {
     const FwdArgs<float> a = item_fwd_args(it.b, k);
     if (it.fmt == 0)
@@ -76,6 +112,7 @@ constexpr size_t kAlign = 256;
 size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
 struct EJob {
+  int64_t id = 0;
   tide_tm2d_job spec;
   BatchItem<float> item;
   PointTable table;
@@ -108,6 +145,7 @@ struct Engine {
   int32_t *d_start = nullptr;
   Slot slots[2];
   int cur = 0;
+  std::vector<int64_t> done;  // jobs whose phase ended since the last report
 };
 
 // The (forward rows, adjoint rows) pairs the step kernel is built for; the
@@ -324,6 +362,7 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
     const int64_t left = j.nt - j.k;
     if (left <= 0) {
       j.phase += 1;
+      e.done.push_back(j.id);
       continue;
     }
     run.emplace_back(std::min<int64_t>(left, max_steps), &j);
@@ -364,7 +403,9 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
     si.k0 = (int32_t)j.k;
     si.pad = 0;
     start[i] = (int32_t)tile;
-    tile += (int64_t)si.b.pts.n_xt * si.b.n_yt * j.n_groups;
+    const int64_t tiles = (int64_t)si.b.pts.n_xt * si.b.n_yt;
+    // single-shot jobs pack `group` tiles per workgroup (see serve_step)
+    tile += j.spec.p.n_shots == 1 ? (tiles + e.group - 1) / e.group : tiles * j.n_groups;
     if (tile >= ((int64_t)1 << 31)) {
       set_error("engine: too many tiles in one tick");
       return kInvalidArgument;
@@ -388,10 +429,12 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
   for (auto &r : run) {
     EJob &j = *r.second;
     j.k += r.first;
-    if (j.k >= j.nt)
+    if (j.k >= j.nt) {
       j.phase += 1;
-    else
+      e.done.push_back(j.id);
+    } else {
       ++still;
+    }
   }
   *n_running = still;
   return kOk;
@@ -494,6 +537,7 @@ extern "C" int tide_tm2d_engine_add(void *engine, const tide_tm2d_job *job, int6
   j.phase = 0;
   j.k = 0;
   const int64_t id = e->next_id++;
+  j.id = id;
   e->jobs.emplace(id, std::move(j));
   *job_id = id;
   return kOk;
@@ -508,6 +552,20 @@ extern "C" int tide_tm2d_engine_tick(void *engine, int64_t max_steps, void *stre
   }
   TIDE_HIP_TRY(hipSetDevice(e->device));
   return engine_tick(*e, max_steps, (hipStream_t)stream, n_running);
+}
+
+extern "C" int tide_tm2d_engine_finished(void *engine, int64_t *ids, int64_t cap, int64_t *n) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr) return kInvalidArgument;
+  if (n == nullptr || (cap > 0 && ids == nullptr)) {
+    set_error("engine: output array and count are required");
+    return kInvalidArgument;
+  }
+  const int64_t m = std::min<int64_t>(cap, (int64_t)e->done.size());
+  for (int64_t i = 0; i < m; ++i) ids[i] = e->done[(size_t)i];
+  e->done.erase(e->done.begin(), e->done.begin() + m);
+  *n = m;
+  return kOk;
 }
 
 extern "C" int tide_tm2d_engine_query(void *engine, int64_t job_id, int64_t *phase, int64_t *step) {

@@ -235,6 +235,10 @@ def load() -> ctypes.CDLL:
     lib.tide_tm2d_set_time_block.argtypes = [ctypes.c_int] * 2
     lib.tide_tm2d_get_time_block.restype = ctypes.c_int
     lib.tide_tm2d_get_time_block.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    lib.tide_tm2d_set_tile_split.restype = ctypes.c_int
+    lib.tide_tm2d_set_tile_split.argtypes = [ctypes.c_int] * 2
+    lib.tide_tm2d_get_tile_split.restype = ctypes.c_int
+    lib.tide_tm2d_get_tile_split.argtypes = [ctypes.POINTER(ctypes.c_int)]
     _lib = lib
     return lib
 
@@ -322,6 +326,17 @@ def get_tm2d_time_block() -> tuple[int, int]:
     return int(out[0]), int(out[1])
 
 
+def set_tm2d_tile_split(fwd_split: bool, adj_split: bool) -> None:
+    """Interior / border tiles as separate two-step kernels, per pass (see include/tide_hip.h)."""
+    check(load().tide_tm2d_set_tile_split(int(bool(fwd_split)), int(bool(adj_split))))
+
+
+def get_tm2d_tile_split() -> tuple[bool, bool]:
+    out = (ctypes.c_int * 2)()
+    check(load().tide_tm2d_get_tile_split(out))
+    return bool(out[0]), bool(out[1])
+
+
 def stream_handle(device: torch.device) -> int | None:
     """The HIP stream of torch's current stream on ``device``."""
     return torch.cuda.current_stream(device).cuda_stream
@@ -332,12 +347,14 @@ __all__ = [
     "check",
     "dtype_name",
     "get_tm2d_shot_loop",
+    "get_tm2d_tile_split",
     "get_tm2d_time_block",
     "get_tm2d_tuning",
     "int64_array",
     "ptr_array",
     "set_em3d_tuning",
     "set_tm2d_shot_loop",
+    "set_tm2d_tile_split",
     "set_tm2d_time_block",
     "set_tm2d_tuning",
     "get_function",

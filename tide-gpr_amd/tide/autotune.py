@@ -1,16 +1,22 @@
 """Launch-shape autotuner for the TM2D step kernels.
 
-The forward and adjoint kernels come in two families and take two launch
-parameters.  Family 0 puts one shot on each wavefront: a workgroup of G
-wavefronts shares a tile (its coefficient lines) across G shots and sums the
-FP64 gradient in LDS.  Family 1 (shot loop) gives each wavefront one tile and
-walks every shot of the call, summing the gradient in registers; G is then
-tiles per workgroup.  Both take the rows marched per wavefront (the tile
-height).  The best (rows, G, family) depends on the grid, the shot count and
-the pass (the adjoint carries more state per row).  :func:`tune_tm2d` times
-every candidate on a synthetic problem of the caller's size with HIP events,
-installs the fastest per pass (``tide_tm2d_set_tuning``,
-``tide_tm2d_set_shot_loop``) and caches it per size class, so later calls
+The forward and adjoint kernels come in several families and take two
+launch parameters:
+
+* shots per wavefront: family 0 puts one shot on each wavefront, a
+  workgroup of G wavefronts sharing a tile (its coefficient lines) across G
+  shots and summing the FP64 gradient in LDS; the shot-loop family gives
+  each wavefront one tile and walks every shot of the call (G is then tiles
+  per workgroup);
+* time steps per launch: 1, or 2 (two-step kernels that recompute a halo of
+  two and touch the wavefield state once per two steps), the latter with
+  the interior and border tiles as one kernel or two;
+* rows marched per wavefront (the tile height).
+
+The best combination depends on the grid, the shot count and the pass (the
+adjoint carries more state per row).  :func:`tune_tm2d` times every
+candidate on a synthetic problem of the caller's size with HIP events,
+installs the fastest per pass and caches it per size class, so later calls
 with a similar problem reuse the choice.
 """
 
@@ -25,20 +31,36 @@ from . import _native
 from .maxwell.tm2d import prepare_tm2d
 from .utils import C0
 
-# (rows, group, shot loop)
-FWD_CANDIDATES: tuple[tuple[int, int, int], ...] = (
-    (2, 4, 0), (2, 8, 0), (4, 2, 0), (4, 4, 0), (4, 8, 0), (8, 4, 0), (8, 8, 0), (16, 4, 0),
-    (2, 4, 1), (4, 2, 1), (4, 4, 1), (8, 2, 1))
-ADJ_CANDIDATES: tuple[tuple[int, int, int], ...] = (
-    (2, 2, 0), (2, 4, 0), (2, 8, 0), (4, 2, 0), (4, 4, 0), (4, 8, 0), (8, 2, 0),
-    (2, 2, 1), (2, 4, 1), (4, 2, 1), (4, 4, 1))
+# (rows, group, shot loop, steps per launch, interior/border split)
+FWD_CANDIDATES: tuple[tuple[int, int, int, int, int], ...] = (
+    (2, 4, 0, 1, 0), (4, 2, 0, 1, 0), (4, 4, 0, 1, 0), (8, 4, 0, 1, 0), (16, 4, 0, 1, 0),
+    (2, 4, 1, 1, 0), (4, 2, 1, 1, 0),
+    (2, 2, 0, 2, 0), (4, 1, 0, 2, 0), (4, 2, 0, 2, 0), (4, 4, 0, 2, 0), (8, 2, 0, 2, 0),
+    (4, 1, 0, 2, 1), (4, 2, 0, 2, 1))
+ADJ_CANDIDATES: tuple[tuple[int, int, int, int, int], ...] = (
+    (2, 2, 0, 1, 0), (2, 4, 0, 1, 0), (2, 8, 0, 1, 0), (4, 4, 0, 1, 0), (8, 2, 0, 1, 0),
+    (2, 4, 1, 1, 0), (4, 2, 1, 1, 0),
+    (2, 4, 0, 2, 1), (4, 2, 0, 2, 1), (4, 4, 0, 2, 1), (4, 8, 0, 2, 1),
+    (2, 4, 0, 2, 0), (4, 4, 0, 2, 0))
 
-_cache: dict[tuple, tuple[tuple[int, int, int, int], tuple[int, int]]] = {}
+_cache: dict[tuple, tuple[tuple[int, ...], tuple[int, ...]]] = {}
 
 
-def _install(tuning: tuple[int, int, int, int], loop: tuple[int, int]) -> None:
-    _native.set_tm2d_tuning(*tuning)
-    _native.set_tm2d_shot_loop(*loop)
+def _install(fwd: tuple[int, ...], adj: tuple[int, ...]) -> None:
+    """Install one forward and one adjoint candidate."""
+    _native.set_tm2d_tuning(fwd[0], fwd[1], adj[0], adj[1])
+    _native.set_tm2d_shot_loop(fwd[2], adj[2])
+    _native.set_tm2d_time_block(fwd[3], adj[3])
+    _native.set_tm2d_tile_split(fwd[4], adj[4])
+
+
+def current() -> tuple[tuple[int, ...], tuple[int, ...]]:
+    """The installed (forward, adjoint) candidates."""
+    t = _native.get_tm2d_tuning()
+    lp = _native.get_tm2d_shot_loop()
+    st = _native.get_tm2d_time_block()
+    sp = _native.get_tm2d_tile_split()
+    return ((t[0], t[1], int(lp[0]), st[0], int(sp[0])), (t[2], t[3], int(lp[1]), st[1], int(sp[1])))
 
 
 def _size_class(ny: int, nx: int, n_shots: int, dtype: torch.dtype) -> tuple:
@@ -86,21 +108,20 @@ def tune_tm2d(
     dtype: torch.dtype = torch.float32,
     nt: int = 48,
     device: torch.device | str = "cuda:0",
-    fwd_candidates: Sequence[tuple[int, int, int]] = FWD_CANDIDATES,
-    adj_candidates: Sequence[tuple[int, int, int]] = ADJ_CANDIDATES,
+    fwd_candidates: Sequence[tuple[int, ...]] = FWD_CANDIDATES,
+    adj_candidates: Sequence[tuple[int, ...]] = ADJ_CANDIDATES,
     use_cache: bool = True,
 ) -> dict:
-    """Pick and install the fastest (rows, group, shot loop) for the
-    forward-with-snapshots and adjoint kernels on a padded ``ny x nx`` grid
-    with ``n_shots`` shots per call.  Returns the choice (``tuning`` =
-    (fwd rows, fwd group, adj rows, adj group), ``shot_loop`` = (fwd, adj))
-    and the timings."""
+    """Pick and install the fastest (rows, group, shot loop, steps, split)
+    for the forward-with-snapshots and adjoint kernels on a padded
+    ``ny x nx`` grid with ``n_shots`` shots per call.  Returns the choice
+    (``forward`` and ``adjoint`` candidates) and the timings."""
     device = torch.device(device)
     key = _size_class(ny, nx, n_shots, dtype)
     if use_cache and key in _cache:
-        tuning, loop = _cache[key]
-        _install(tuning, loop)
-        return {"tuning": tuning, "shot_loop": loop, "cached": True}
+        fwd, adj = _cache[key]
+        _install(fwd, adj)
+        return {"forward": fwd, "adjoint": adj, "cached": True}
     t_start = time.perf_counter()
     st = _problem(ny, nx, n_shots, nt, dtype, device)
     lib = _native.load()
@@ -135,31 +156,28 @@ def tune_tm2d(
             st.n_rec, 1, 0, pny * pnx * bpe, False, False, False, nt, *st.pml,
             device.index or 0, stream))
 
-    before = _native.get_tm2d_tuning()
-    before_loop = _native.get_tm2d_shot_loop()
+    _, adj_before = current()
     run_fwd()
     run_bwd()
     torch.cuda.synchronize()
     f_times, a_times = {}, {}
-    for rows, grp, loop in fwd_candidates:
-        _install((rows, grp, before[2], before[3]), (loop, before_loop[1]))
-        f_times[(rows, grp, loop)] = _time(run_fwd)
+    for cand in fwd_candidates:
+        _install(tuple(cand), adj_before)
+        f_times[tuple(cand)] = _time(run_fwd)
     best_f = min(f_times, key=f_times.get)
-    for rows, grp, loop in adj_candidates:
-        _install((best_f[0], best_f[1], rows, grp), (best_f[2], loop))
-        a_times[(rows, grp, loop)] = _time(run_bwd)
+    for cand in adj_candidates:
+        _install(best_f, tuple(cand))
+        a_times[tuple(cand)] = _time(run_bwd)
     best_a = min(a_times, key=a_times.get)
-    choice = (best_f[0], best_f[1], best_a[0], best_a[1])
-    loop_choice = (best_f[2], best_a[2])
-    _install(choice, loop_choice)
-    _cache[key] = (choice, loop_choice)
+    _install(best_f, best_a)
+    _cache[key] = (best_f, best_a)
 
     def label(k):
-        return f"{k[0]}x{k[1]}{'L' if k[2] else ''}"
+        return f"{k[0]}x{k[1]}{'L' if k[2] else ''}{'/2' if k[3] == 2 else ''}{'s' if k[4] else ''}"
 
     return {
-        "tuning": choice,
-        "shot_loop": loop_choice,
+        "forward": best_f,
+        "adjoint": best_a,
         "cached": False,
         "forward_us_per_step": {label(k): round(v * 1e3 / nt, 2) for k, v in f_times.items()},
         "adjoint_us_per_step": {label(k): round(v * 1e3 / nt, 2) for k, v in a_times.items()},
@@ -167,4 +185,4 @@ def tune_tm2d(
     }
 
 
-__all__ = ["tune_tm2d", "FWD_CANDIDATES", "ADJ_CANDIDATES"]
+__all__ = ["tune_tm2d", "current", "FWD_CANDIDATES", "ADJ_CANDIDATES"]

@@ -56,25 +56,29 @@ class KernelTimer:
     """
 
     def __init__(self) -> None:
-        self.records: list[tuple[str, int, int, torch.cuda.Event, torch.cuda.Event]] = []
+        self.records: list[tuple[str, int, int, int, torch.cuda.Event, torch.cuda.Event]] = []
 
     def start(self) -> torch.cuda.Event:
         ev = torch.cuda.Event(enable_timing=True)
         ev.record()
         return ev
 
-    def stop(self, name: str, steps: int, cells: int, start: torch.cuda.Event) -> None:
+    def stop(self, name: str, steps: int, cells: int, start: torch.cuda.Event,
+             steps_per_launch: int = 1) -> None:
         ev = torch.cuda.Event(enable_timing=True)
         ev.record()
-        self.records.append((name, steps, cells, start, ev))
+        self.records.append((name, steps, cells, steps_per_launch, start, ev))
 
     def summary(self) -> dict[str, dict[str, float]]:
-        """Per pass: total ms, launches, and cell-updates (after a sync)."""
+        """Per pass: total ms, launches (time steps / steps per launch), time
+        steps, and cell-updates (after a sync)."""
         out: dict[str, dict[str, float]] = {}
-        for name, steps, cells, a, b in self.records:
-            d = out.setdefault(name, {"ms": 0.0, "launches": 0, "cell_updates": 0.0})
+        for name, steps, cells, spl, a, b in self.records:
+            d = out.setdefault(name, {"ms": 0.0, "launches": 0, "steps": 0, "cell_updates": 0.0,
+                                      "steps_per_launch": spl})
             d["ms"] += a.elapsed_time(b)
-            d["launches"] += steps
+            d["launches"] += (steps + spl - 1) // spl
+            d["steps"] += steps
             d["cell_updates"] += float(steps) * cells
         return out
 
@@ -142,12 +146,14 @@ def prepare_tm2d(
     nt: int | None = None,
     initial: Sequence[torch.Tensor | None] = (None,) * 7,
     validate_material_inputs: bool = True,
+    allocate_fields: bool = True,
 ) -> TM2DSetup:
     """Pad the model, compile coefficients and profiles, map the points.
 
     ``epsilon`` is ``[ny, nx]`` (shared by all shots) or ``[N, ny, nx]`` (one
     model per shot).  Device-agnostic: runs on CPU tensors too, so the host
-    numerics can be tested without a GPU.
+    numerics can be tested without a GPU.  ``allocate_fields=False`` leaves
+    ``fields`` empty for callers that bring their own (zeroed) state.
     """
     if epsilon.ndim not in (2, 3):
         raise RuntimeError("epsilon must be 2D or batched 3D")
@@ -200,7 +206,7 @@ def prepare_tm2d(
         f = torch.empty(0, device=device, dtype=dtype)
 
     fields = []
-    for t0 in initial:
+    for t0 in initial if allocate_fields else ():
         if t0 is None:
             fields.append(torch.zeros(n_shots, ny, nx, device=device, dtype=dtype))
             continue
@@ -309,7 +315,7 @@ class TM2DPropagate(torch.autograd.Function):
         _native.check(status)
         if t0 is not None:
             _timer.stop("forward_store" if store else "forward", setup.nt,
-                        setup.n_shots * setup.ny * setup.nx, t0)
+                        setup.n_shots * setup.ny * setup.nx, t0, _native.get_tm2d_time_block()[0])
         ctx.setup = setup
         ctx.storage = storage
         ctx.stored = store
@@ -363,7 +369,8 @@ class TM2DPropagate(torch.autograd.Function):
         )
         _native.check(status)
         if t0 is not None:
-            _timer.stop("adjoint", setup.nt, setup.n_shots * setup.ny * setup.nx, t0)
+            _timer.stop("adjoint", setup.nt, setup.n_shots * setup.ny * setup.nx, t0,
+                        _native.get_tm2d_time_block()[1])
         out_ca = gca.to(dtype).reshape(ca.shape) if want_ca else None
         out_cb = gcb.to(dtype).reshape(cb.shape) if want_cb else None
         return out_ca, out_cb, None, grad_f, None, None

@@ -76,6 +76,8 @@ def _bind(lib: ctypes.CDLL) -> None:
     lib.tide_tm2d_engine_add.argtypes = [_P, ctypes.POINTER(_Job), ctypes.POINTER(_I64)]
     lib.tide_tm2d_engine_tick.restype = ctypes.c_int
     lib.tide_tm2d_engine_tick.argtypes = [_P, _I64, _P, ctypes.POINTER(_I64)]
+    lib.tide_tm2d_engine_finished.restype = ctypes.c_int
+    lib.tide_tm2d_engine_finished.argtypes = [_P, ctypes.POINTER(_I64), _I64, ctypes.POINTER(_I64)]
     lib.tide_tm2d_engine_query.restype = ctypes.c_int
     lib.tide_tm2d_engine_query.argtypes = [_P, _I64, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]
     lib.tide_tm2d_engine_begin_adjoint.restype = ctypes.c_int
@@ -126,6 +128,109 @@ def plan_chunks(n_shots: int, shot_bytes: Any, budget: int) -> list[tuple[int, i
     while per < n_shots and shot_bytes(per + 1) <= limit:
         per += 1
     return [(s, min(n_shots, s + per)) for s in range(0, n_shots, per)]
+
+
+_ALIGN = 256
+
+
+def _aligned(n: int) -> int:
+    return (int(n) + _ALIGN - 1) // _ALIGN * _ALIGN
+
+
+def state_spec(
+    ny_p: int, nx_p: int, n_shots: int, nt: int, n_src: int, n_rec: int, *,
+    gradient: bool, bf16: bool = False, interval: int = 1, group: int = 2,
+) -> list[tuple[str, torch.dtype, tuple[int, ...], int, bool]]:
+    """The device arrays of one engine job carved from the server's pool:
+    ``(name, dtype, shape, count, zeroed)``."""
+    f32, f64 = torch.float32, torch.float64
+    shape = (n_shots, ny_p, nx_p)
+    spec = [
+        ("fields", f32, shape, 7, True),
+        ("fields_b", f32, shape, 5, True),
+        ("r", f32, (nt, n_shots, max(n_rec, 1)), 1, False),
+    ]
+    if gradient:
+        n_store = (nt + interval - 1) // interval if nt > 0 else 0
+        sdt = torch.int16 if bf16 else f32
+        spec += [
+            ("ey_store", sdt, (n_store, *shape), 1, False),
+            ("curl_store", sdt, (n_store, *shape), 1, False),
+            ("lambda", f32, shape, 7, True),
+            ("lambda_b", f32, shape, 7, True),
+            ("grad_r", f32, (nt, n_shots, max(n_rec, 1)), 1, False),
+            ("grad_f", f32, (nt, n_shots, max(n_src, 1)), 1, False),
+            ("grad_ca", f64, (ny_p, nx_p), 1, True),
+            ("grad_cb", f64, (ny_p, nx_p), 1, True),
+        ]
+        groups = (n_shots + group - 1) // group
+        if groups > 1:
+            spec += [("gca_groups", f64, (groups, ny_p, nx_p), 1, True),
+                     ("gcb_groups", f64, (groups, ny_p, nx_p), 1, True)]
+    return spec
+
+
+def spec_bytes(spec) -> int:
+    total = 0
+    for _, dt, shape, count, _ in spec:
+        n = dt.itemsize
+        for d in shape:
+            n *= d
+        total += count * _aligned(n)
+    return total
+
+
+class DevicePool:
+    """One device allocation made up front and carved into job state
+    (first fit, 256-byte aligned, neighbours merged on release): admitting
+    a solve costs no hipMalloc, whatever its size."""
+
+    def __init__(self, nbytes: int, device: torch.device) -> None:
+        self.size = _aligned(nbytes) - _ALIGN
+        self.buf = torch.empty(self.size, dtype=torch.uint8, device=device)
+        self.free: list[tuple[int, int]] = [(0, self.size)]
+
+    @property
+    def available(self) -> int:
+        return sum(n for _, n in self.free)
+
+    def alloc(self, nbytes: int) -> int | None:
+        n = _aligned(nbytes)
+        for i, (off, size) in enumerate(self.free):
+            if size >= n:
+                if size == n:
+                    del self.free[i]
+                else:
+                    self.free[i] = (off + n, size - n)
+                return off
+        return None
+
+    def release(self, off: int, nbytes: int) -> None:
+        n = _aligned(nbytes)
+        self.free.append((off, n))
+        self.free.sort()
+        merged: list[tuple[int, int]] = []
+        for o, m in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + m)
+            else:
+                merged.append((o, m))
+        self.free = merged
+
+    def carve(self, off: int, spec) -> dict[str, Any]:
+        """Tensors for ``spec`` laid out from byte ``off`` (lists for counts > 1)."""
+        out: dict[str, Any] = {}
+        pos = off
+        for name, dt, shape, count, _ in spec:
+            n = dt.itemsize
+            for d in shape:
+                n *= d
+            ts = []
+            for _ in range(count):
+                ts.append(self.buf[pos : pos + n].view(dt).view(shape))
+                pos += _aligned(n)
+            out[name] = ts if count > 1 else ts[0]
+        return out
 
 
 # ------------------------------------------------------------ chain rule
@@ -224,6 +329,8 @@ class _Chunk:
     bf16: bool
     interval: int
     inputs_ready: Any = None
+    spec: Any = None
+    block: int = -1
     bypassed: int = 0
     job_id: int = -1
     st: Any = None
@@ -245,15 +352,16 @@ class TM2DServer:
     device:
         HIP device (default: the current one).
     hbm_budget:
-        Bytes the admitted jobs may hold (default 70% of the free memory at
-        construction).
+        Size of the device pool the admitted jobs' state is carved from,
+        allocated once at construction (default 70% of the free memory).
     max_jobs:
         Jobs resident at once.
     steps_per_tick:
         Time steps each running job advances per engine tick (admission and
         completion happen at tick boundaries).
     group:
-        Shots per workgroup (default: the tuned adjoint group).
+        Shots per workgroup, the same for every job of a launch (default 2:
+        requests with few shots leave fewer idle wavefronts).
     max_bypass:
         Admissions that may overtake a waiting chunk before it blocks the
         queue.
@@ -265,7 +373,7 @@ class TM2DServer:
         *,
         hbm_budget: int | None = None,
         max_jobs: int = 64,
-        steps_per_tick: int = 32,
+        steps_per_tick: int = 64,
         group: int | None = None,
         max_bypass: int = 8,
     ) -> None:
@@ -277,9 +385,7 @@ class TM2DServer:
         self.device = dev
         self._lib = _native.load()
         _bind(self._lib)
-        if group is None:
-            group = _native.get_tm2d_tuning()[3]
-        self.group = max(1, min(8, int(group)))
+        self.group = max(1, min(8, int(2 if group is None else group)))
         self.max_jobs = int(max_jobs)
         self.steps_per_tick = max(1, int(steps_per_tick))
         self.max_bypass = int(max_bypass)
@@ -292,7 +398,8 @@ class TM2DServer:
             free, _ = torch.cuda.mem_get_info(dev)
             hbm_budget = int(0.7 * free)
         self.budget = int(hbm_budget)
-        self.used = 0
+        self.pool = DevicePool(self.budget, dev)
+        self._released = None  # engine-stream event after the last pool release
         self._waiting: deque[_Chunk] = deque()
         self._jobs: dict[int, _Chunk] = {}
         self._open: list[Ticket] = []
@@ -346,11 +453,14 @@ class TM2DServer:
             interval = min(interval, nt)
         grad = kind == "gradient"
 
-        def shot_bytes(count: int) -> int:
-            return job_bytes(ny_p, nx_p, count, nt, n_src, n_rec, gradient=grad, bf16=bf16,
-                             interval=interval, group=self.group)
+        def spec_of(count: int):
+            return state_spec(ny_p, nx_p, count, nt, n_src, n_rec, gradient=grad, bf16=bf16,
+                              interval=interval, group=self.group)
 
-        chunks = plan_chunks(shots, shot_bytes, self.budget)
+        def shot_bytes(count: int) -> int:
+            return spec_bytes(spec_of(count))
+
+        chunks = plan_chunks(shots, shot_bytes, self.pool.size)
         # inputs produced on the caller's stream are read by the preparation stream
         inputs_ready = torch.cuda.Event()
         inputs_ready.record(torch.cuda.current_stream(self.device))
@@ -365,8 +475,9 @@ class TM2DServer:
             sub["source_location"] = _slice_shots(src, s0, s1)
             sub["receiver_location"] = _slice_shots(rec, s0, s1)
             obs = observed[:, s0:s1] if observed is not None else None
-            self._waiting.append(_Chunk(ticket, i, (s0, s1), sub, obs, shot_bytes(s1 - s0), bf16, interval,
-                                        inputs_ready))
+            spec = spec_of(s1 - s0)
+            self._waiting.append(_Chunk(ticket, i, (s0, s1), sub, obs, spec_bytes(spec), bf16, interval,
+                                        inputs_ready, spec))
         self._open.append(ticket)
         return ticket
 
@@ -376,14 +487,16 @@ class TM2DServer:
         for ch in list(self._waiting):
             if len(self._jobs) >= self.max_jobs:
                 break
-            if self.used + ch.bytes > self.budget:
+            if blocked is not None and blocked.bypassed >= self.max_bypass:
+                break
+            off = self.pool.alloc(ch.bytes)
+            if off is None:
                 if blocked is None:
                     blocked = ch
                 continue
             if blocked is not None:
-                if blocked.bypassed >= self.max_bypass:
-                    break
                 blocked.bypassed += 1
+            ch.block = off
             self._waiting.remove(ch)
             self._start(ch)
 
@@ -407,9 +520,15 @@ class TM2DServer:
             amp = amp.detach().to(dev) if amp is not None else None
             st = prepare_tm2d(eps, sig, mu, spacing, kw["dt"], amp, kw.get("source_location"),
                               kw.get("receiver_location"), pml_width=kw.get("pml_width", 20),
-                              nt=kw.get("nt"), validate_material_inputs=False)
-            shape = (st.n_shots, st.ny, st.nx)
-            b: dict[str, Any] = {}
+                              nt=kw.get("nt"), validate_material_inputs=False, allocate_fields=False)
+            # the pool block may still be in use by work queued before its release
+            if self._released is not None:
+                self._prep.wait_event(self._released)
+            b: dict[str, Any] = self.pool.carve(ch.block, ch.spec)
+            for name, _, _, _, zeroed in ch.spec:
+                if zeroed:
+                    for t in b[name] if isinstance(b[name], list) else [b[name]]:
+                        t.zero_()
             b["ca"], b["cb"], b["cq"] = (t.detach().contiguous() for t in (st.ca, st.cb, st.cq))
             b["f"] = st.f.detach().contiguous() if st.n_src > 0 else None
             b["src"] = st.sources_i.contiguous()
@@ -417,25 +536,7 @@ class TM2DServer:
             b["amp"] = amp
             b["rec"] = st.receivers_i.contiguous()
             b["prof"] = list(st.profiles)
-            b["fields"] = st.fields
-            b["fields_b"] = [torch.zeros(shape, device=dev, dtype=f32) for _ in range(5)]
-            b["r"] = torch.empty(st.nt, st.n_shots, max(st.n_rec, 1), device=dev, dtype=f32)
-            grad = ch.ticket.kind == "gradient"
-            if grad:
-                n_store = (st.nt + ch.interval - 1) // ch.interval
-                sdt = torch.int16 if ch.bf16 else f32
-                b["ey_store"] = torch.empty(n_store, *shape, device=dev, dtype=sdt)
-                b["curl_store"] = torch.empty(n_store, *shape, device=dev, dtype=sdt)
-                b["lambda"] = [torch.zeros(shape, device=dev, dtype=f32) for _ in range(7)]
-                b["lambda_b"] = [torch.zeros(shape, device=dev, dtype=f32) for _ in range(7)]
-                b["grad_r"] = torch.empty(st.nt, st.n_shots, max(st.n_rec, 1), device=dev, dtype=f32)
-                b["grad_f"] = torch.empty(st.nt, st.n_shots, max(st.n_src, 1), device=dev, dtype=f32)
-                b["grad_ca"] = torch.zeros(st.ny, st.nx, device=dev, dtype=torch.float64)
-                b["grad_cb"] = torch.zeros(st.ny, st.nx, device=dev, dtype=torch.float64)
-                groups = (st.n_shots + self.group - 1) // self.group
-                if groups > 1:
-                    b["gca_groups"] = torch.zeros(groups, st.ny, st.nx, device=dev, dtype=torch.float64)
-                    b["gcb_groups"] = torch.zeros(groups, st.ny, st.nx, device=dev, dtype=torch.float64)
+            if ch.ticket.kind == "gradient":
                 obs = ch.observed
                 b["observed"] = obs.detach().to(device=dev, dtype=f32) if obs is not None else None
             ready = torch.cuda.Event()
@@ -449,7 +550,6 @@ class TM2DServer:
         _native.check(self._lib.tide_tm2d_engine_add(self._engine, ctypes.byref(ch.job), ctypes.byref(jid)))
         ch.job_id = int(jid.value)
         self._jobs[ch.job_id] = ch
-        self.used += ch.bytes
         if ch.ticket.admitted is None:
             ch.ticket.admitted = time.perf_counter()
 
@@ -508,13 +608,18 @@ class TM2DServer:
         return self._collect()
 
     def _after_tick(self) -> None:
+        cap = max(1, len(self._jobs))
+        ids = (_I64 * cap)()
+        n = _I64()
+        _native.check(self._lib.tide_tm2d_engine_finished(self._engine, ids, cap, ctypes.byref(n)))
         phase, k = _I64(), _I64()
         with torch.cuda.stream(self.stream):
-            for jid, ch in list(self._jobs.items()):
+            for jid in ids[: n.value]:
+                ch = self._jobs[jid]
                 _native.check(self._lib.tide_tm2d_engine_query(self._engine, jid, ctypes.byref(phase), ctypes.byref(k)))
                 if phase.value == 1 and ch.ticket.kind == "gradient":
                     self._begin_adjoint(ch)
-                elif phase.value == 1 or phase.value == 3:
+                else:
                     self._finish(ch)
 
     def _begin_adjoint(self, ch: _Chunk) -> None:
@@ -531,14 +636,17 @@ class TM2DServer:
         st, b = ch.st, ch.bufs
         _native.check(self._lib.tide_tm2d_engine_remove(self._engine, ch.job_id, self.stream.cuda_stream))
         del self._jobs[ch.job_id]
-        self.used -= ch.bytes
-        r = b["r"][:, :, : st.n_rec]
-        part: dict[str, Any] = {"r": r}
+        part: dict[str, Any] = {"r": b["r"][:, :, : st.n_rec].clone()}
         if ch.ticket.kind == "gradient":
             gf = b["grad_f"][:, :, : st.n_src] if st.n_src > 0 else None
             ge, gs = model_gradient(b["grad_ca"], b["grad_cb"], gf, st, tuple(ch.problem["epsilon"].shape),
                                     b["amp"], ch.problem["grid_spacing"], b["src_dev"])
             part.update(misfit=b["misfit"], ge=ge, gs=gs)
+        # the block returns to the pool once the work queued so far is done
+        self.pool.release(ch.block, ch.bytes)
+        ch.block = -1
+        self._released = torch.cuda.Event()
+        self._released.record(self.stream)
         t = ch.ticket
         t._parts[ch.index] = part
         t._left -= 1
@@ -570,6 +678,11 @@ class TM2DServer:
         return done
 
     # ------------------------------------------------------------------ run
+    @property
+    def used(self) -> int:
+        """Pool bytes held by admitted jobs."""
+        return self.pool.size - self.pool.available
+
     @property
     def idle(self) -> bool:
         return not self._open
@@ -614,6 +727,7 @@ class TM2DServer:
             torch.cuda.synchronize(self.device)
             self._lib.tide_tm2d_engine_destroy(self._engine)
             self._engine = None
+            self.pool = None
 
     def __del__(self) -> None:  # pragma: no cover - best effort
         try:
@@ -720,6 +834,6 @@ def latency_stats(tickets: Sequence[Ticket]) -> dict[str, float]:
 
 
 __all__ = [
-    "SolveResult", "TM2DServer", "Ticket", "job_bytes", "latency_stats", "model_gradient",
-    "padded_shape", "plan_chunks", "served_trace",
+    "DevicePool", "SolveResult", "TM2DServer", "Ticket", "job_bytes", "latency_stats", "model_gradient",
+    "padded_shape", "plan_chunks", "served_trace", "spec_bytes", "state_spec",
 ]
