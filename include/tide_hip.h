@@ -254,6 +254,9 @@ int tide_tm2d_get_time_block(int *out2);
  */
 int tide_tm2d_set_tile_split(int fwd_split, int adj_split);
 int tide_tm2d_get_tile_split(int *out2);
+/* Tile height (1 or 2 rows) of the split border kernels. */
+int tide_tm2d_set_border_rows(int rows);
+int tide_tm2d_get_border_rows(void);
 
 /* ------------------------------------------------------- batched 2-D plans
  * Many independent fp32 TM2D problems (different grid sizes, shot counts

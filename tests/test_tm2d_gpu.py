@@ -358,8 +358,10 @@ LAUNCH_SHAPES = [
     ((4, 4, 2, 1), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
     ((4, 4, 4, 4), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
     ((2, 2, 2, 4), (0, 0), (2, 2), (0, 1), "tm2d_config1_f32"),
-    # interior and border tiles as one kernel or two
+    # interior and border tiles as one kernel or three (border rows from the
+    # library default, 2; the test below sweeps both heights)
     ((4, 2, 4, 4), (0, 0), (2, 2), (1, 0), "tm2d_hetero_f32"),
+    ((8, 2, 2, 4), (0, 0), (2, 2), (1, 1), "tm2d_hetero_f32"),
     ((2, 2, 2, 2), (0, 0), (2, 2), (1, 0), "tm2d_config1_f32"),
     ((4, 2, 2, 4), (0, 0), (2, 2), (1, 1), "tm2d_hetero_f64"),
 ]
@@ -406,3 +408,29 @@ def test_every_launch_shape_matches_oracle(tuning, loop, steps, split, name):
         assert np.array_equal(gg[2], go[2])
         assert relative_l2(gg[0], go[0]) < 1e-12
         assert relative_l2(gg[1], go[1]) < 1e-12
+
+
+@pytest.mark.parametrize("border_rows", [1, 2])
+def test_border_tile_heights_match_oracle(border_rows):
+    """The split border kernels give the same bits with 1- or 2-row tiles."""
+    case, _ = load_golden("tm2d_hetero_f32")
+    before = tide._native.get_tm2d_border_rows()
+    tide._native.set_tm2d_border_rows(border_rows)
+    try:
+        with launch_shape((4, 2, 4, 4), (0, 0), (2, 2), (1, 1)):
+            f_o, r_o, es, cs = O.tm2d_forward(case, store="full")
+            f_g, r_g, es_g, cs_g = gpu_forward(case, store="full")
+            assert np.array_equal(r_g, r_o)
+            assert np.array_equal(es_g, es) and np.array_equal(cs_g, cs)
+            for a, b in zip(f_g, f_o):
+                assert np.array_equal(a, b)
+            w = np.random.default_rng(4).standard_normal(r_o.shape).astype(np.float32)
+            go = O.tm2d_backward(case, w, es, cs)
+            gg = gpu_backward(case, w, es, cs)
+            for a, b in zip(gg[3], go[3]):
+                assert np.array_equal(a, b)
+            assert np.array_equal(gg[2], go[2])
+            assert relative_l2(gg[0], go[0]) < 1e-12
+            assert relative_l2(gg[1], go[1]) < 1e-12
+    finally:
+        tide._native.set_tm2d_border_rows(before)

@@ -70,56 +70,72 @@ void launch_fwd(int rows, bool loop, dim3 grid, dim3 block, hipStream_t stream,
   }
 }
 
-// Number of interior and border tiles of a split.
-inline void split_counts(const TileSplit &sp, int n_xt, int n_yt, int64_t *inner, int64_t *border) {
-  *inner = (int64_t)(sp.xb - sp.xa) * (sp.yb - sp.ya);
-  *border = (int64_t)n_xt * n_yt - *inner;
-}
-
-// Two-step launches: split (interior and border tile sets as two kernels,
-// each with its own register budget) or one kernel over all tiles.
-template <typename T, int TY, int FMT>
+// Two-step launches: split (interior tiles of height TY; the border tiles
+// beside the interior rows and those above and below it, of height TYB; three
+// kernels with their own register budgets) or one kernel over all tiles of
+// height TY.
+template <typename T, int TY, int TYB, int FMT>
 void launch_fwd2_parts(int64_t groups, dim3 block, hipStream_t stream, bool split,
                        const Fwd2Args<T> &a) {
-  int64_t inner, border;
-  split_counts(a.split, a.pts.n_xt, a.n_yt, &inner, &border);
   if (!split) {
-    fwd2_step<T, TY, FMT, kAllTiles><<<(unsigned)((inner + border) * groups), block, 0, stream>>>(a);
+    const int64_t all = (int64_t)a.pts.n_xt * a.n_yt;
+    fwd2_step<T, TY, FMT, kOrderedTiles><<<(unsigned)(all * groups), block, 0, stream>>>(a);
     return;
   }
+  const int64_t inner = split_tiles<TY>(a.split, a.pts.n_xt, a.g.ny, kInnerTiles);
+  const int64_t sides = split_tiles<TYB>(a.split, a.pts.n_xt, a.g.ny, kSideTiles);
+  const int64_t bands = split_tiles<TYB>(a.split, a.pts.n_xt, a.g.ny, kBandTiles);
   if (inner > 0) fwd2_step<T, TY, FMT, kInnerTiles><<<(unsigned)(inner * groups), block, 0, stream>>>(a);
-  if (border > 0) fwd2_step<T, TY, FMT, kBorderTiles><<<(unsigned)(border * groups), block, 0, stream>>>(a);
+  if (sides > 0) fwd2_step<T, TYB, FMT, kSideTiles><<<(unsigned)(sides * groups), block, 0, stream>>>(a);
+  if (bands > 0) fwd2_step<T, TYB, FMT, kBandTiles><<<(unsigned)(bands * groups), block, 0, stream>>>(a);
 }
 
 template <typename T, int FMT>
-void launch_fwd2(int rows, int64_t groups, dim3 block, hipStream_t stream, bool split,
-                 const Fwd2Args<T> &a) {
-  if (rows == 2) return launch_fwd2_parts<T, 2, FMT>(groups, block, stream, split, a);
-  if constexpr (sizeof(T) == 4) {
-    if (rows == 8) return launch_fwd2_parts<T, 8, FMT>(groups, block, stream, split, a);
+void launch_fwd2(int rows, int border_rows, int64_t groups, dim3 block, hipStream_t stream,
+                 bool split, const Fwd2Args<T> &a) {
+  const bool b1 = border_rows == 1;
+  if (rows == 2) {
+    return b1 ? launch_fwd2_parts<T, 2, 1, FMT>(groups, block, stream, split, a)
+              : launch_fwd2_parts<T, 2, 2, FMT>(groups, block, stream, split, a);
   }
-  launch_fwd2_parts<T, 4, FMT>(groups, block, stream, split, a);
+  if constexpr (sizeof(T) == 4) {
+    if (rows == 8)
+      return b1 ? launch_fwd2_parts<T, 8, 1, FMT>(groups, block, stream, split, a)
+                : launch_fwd2_parts<T, 8, 2, FMT>(groups, block, stream, split, a);
+  }
+  b1 ? launch_fwd2_parts<T, 4, 1, FMT>(groups, block, stream, split, a)
+     : launch_fwd2_parts<T, 4, 2, FMT>(groups, block, stream, split, a);
 }
 
-template <typename T, int TY, int FMT>
-void launch_adj2_parts(int64_t groups, dim3 block, size_t lds, hipStream_t stream, bool split,
+template <typename T, int TY, int TYB, int FMT>
+void launch_adj2_parts(int64_t groups, dim3 block, hipStream_t stream, bool split,
                        const Adj2Args<T> &a) {
-  int64_t inner, border;
-  split_counts(a.split, a.pts.n_xt, a.n_yt, &inner, &border);
+  const size_t lds = 4 * (size_t)block.y * TY * 64 * sizeof(double);
+  const size_t lds_b = 4 * (size_t)block.y * TYB * 64 * sizeof(double);
   if (!split) {
-    adj2_step<T, TY, FMT, kAllTiles><<<(unsigned)((inner + border) * groups), block, lds, stream>>>(a);
+    const int64_t all = (int64_t)a.pts.n_xt * a.n_yt;
+    adj2_step<T, TY, FMT, kOrderedTiles><<<(unsigned)(all * groups), block, lds, stream>>>(a);
     return;
   }
+  const int64_t inner = split_tiles<TY>(a.split, a.pts.n_xt, a.g.ny, kInnerTiles);
+  const int64_t sides = split_tiles<TYB>(a.split, a.pts.n_xt, a.g.ny, kSideTiles);
+  const int64_t bands = split_tiles<TYB>(a.split, a.pts.n_xt, a.g.ny, kBandTiles);
   if (inner > 0) adj2_step<T, TY, FMT, kInnerTiles><<<(unsigned)(inner * groups), block, lds, stream>>>(a);
-  if (border > 0)
-    adj2_step<T, TY, FMT, kBorderTiles><<<(unsigned)(border * groups), block, lds, stream>>>(a);
+  if (sides > 0)
+    adj2_step<T, TYB, FMT, kSideTiles><<<(unsigned)(sides * groups), block, lds_b, stream>>>(a);
+  if (bands > 0)
+    adj2_step<T, TYB, FMT, kBandTiles><<<(unsigned)(bands * groups), block, lds_b, stream>>>(a);
 }
 
 template <typename T, int FMT>
-void launch_adj2(int rows, int64_t groups, dim3 block, size_t lds, hipStream_t stream, bool split,
-                 const Adj2Args<T> &a) {
-  if (rows == 2) return launch_adj2_parts<T, 2, FMT>(groups, block, lds, stream, split, a);
-  launch_adj2_parts<T, 4, FMT>(groups, block, lds, stream, split, a);
+void launch_adj2(int rows, int border_rows, int64_t groups, dim3 block, hipStream_t stream,
+                 bool split, const Adj2Args<T> &a) {
+  const bool b1 = border_rows == 1;
+  if (rows == 2)
+    return b1 ? launch_adj2_parts<T, 2, 1, FMT>(groups, block, stream, split, a)
+              : launch_adj2_parts<T, 2, 2, FMT>(groups, block, stream, split, a);
+  b1 ? launch_adj2_parts<T, 4, 1, FMT>(groups, block, stream, split, a)
+     : launch_adj2_parts<T, 4, 2, FMT>(groups, block, stream, split, a);
 }
 
 template <typename T, int FMT>
@@ -336,11 +352,11 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     a2.curl_store1 = store_at(t + 1, curl_store);
     const bool st = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
     if (!st) {
-      launch_fwd2<T, 0>(rows2, groups2, block2, stream, split2, a2);
+      launch_fwd2<T, 0>(rows2, g_tuning.border_rows, groups2, block2, stream, split2, a2);
     } else if (fmt == 2) {
-      if constexpr (sizeof(T) == 4) launch_fwd2<T, 2>(rows2, groups2, block2, stream, split2, a2);
+      if constexpr (sizeof(T) == 4) launch_fwd2<T, 2>(rows2, g_tuning.border_rows, groups2, block2, stream, split2, a2);
     } else {
-      launch_fwd2<T, 1>(rows2, groups2, block2, stream, split2, a2);
+      launch_fwd2<T, 1>(rows2, g_tuning.border_rows, groups2, block2, stream, split2, a2);
     }
     for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
     for (int k = 0; k < 2; ++k) std::swap(cur_e[k], nxt_e[k]);
@@ -486,7 +502,6 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     a2.gcb = gb_k;
     const dim3 block2 = block;  // same shot groups as the single-step launches
     a2.split = make_split(p.g, dpt2.n_xt, a2.n_yt, rows2);
-    const size_t lds2 = 4 * (size_t)block2.y * rows2 * 64 * sizeof(double);
     for (; t - 1 >= start_t - nt; t -= 2) {
       a2.ley_i = cur[0]; a2.lhx_i = cur[1]; a2.lhz_i = cur[2];
       a2.mx_i = cur[3]; a2.my_i = cur[4]; a2.nx_i = cur[5]; a2.nz_i = cur[6];
@@ -502,11 +517,11 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
       a2.curl_store1 = snap(t - 1, curl_store, gcb);
       const bool g_any = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
       if (!g_any) {
-        launch_adj2<T, 0>(rows2, n_groups, block2, lds2, stream, split2, a2);
+        launch_adj2<T, 0>(rows2, g_tuning.border_rows, n_groups, block2, stream, split2, a2);
       } else if (fmt == 2) {
-        if constexpr (sizeof(T) == 4) launch_adj2<T, 2>(rows2, n_groups, block2, lds2, stream, split2, a2);
+        if constexpr (sizeof(T) == 4) launch_adj2<T, 2>(rows2, g_tuning.border_rows, n_groups, block2, stream, split2, a2);
       } else {
-        launch_adj2<T, 1>(rows2, n_groups, block2, lds2, stream, split2, a2);
+        launch_adj2<T, 1>(rows2, g_tuning.border_rows, n_groups, block2, stream, split2, a2);
       }
       for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
     }
@@ -879,6 +894,17 @@ extern "C" int tide_tm2d_set_tile_split(int fwd_split, int adj_split) {
   g_tuning.adj_split = adj_split;
   return kOk;
 }
+
+extern "C" int tide_tm2d_set_border_rows(int rows) {
+  if (rows != 1 && rows != 2) {
+    set_error("border tile height must be 1 or 2");
+    return kInvalidArgument;
+  }
+  g_tuning.border_rows = rows;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_get_border_rows(void) { return g_tuning.border_rows; }
 
 extern "C" int tide_tm2d_get_tile_split(int *out2) {
   if (out2 == nullptr) {

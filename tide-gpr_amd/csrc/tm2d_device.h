@@ -671,7 +671,8 @@ struct Tuning {
   int fwd_rows = 4, fwd_group = 1, adj_rows = 4, adj_group = 4;
   int fwd_loop = 0, adj_loop = 0;
   int fwd_steps = 2, adj_steps = 2;
-  int fwd_split = 0, adj_split = 1;
+  int fwd_split = 0, adj_split = 0;
+  int border_rows = 2;  // tile height of the split border kernels (1 or 2)
 };
 extern Tuning g_tuning;
 

@@ -25,53 +25,141 @@ constexpr int kOwn2 = 60;  // owned columns per wavefront (lanes 2..61)
 
 // The interior tiles -- those whose loads and updates (halo of two
 // included) stay inside the grid and outside every CPML strip, see
-// make_split -- form the rectangle [xa, xb) x [ya, yb) of the n_xt x n_yt
-// tile grid; interior and border tiles run as
-// separate launches, so the interior kernel carries none of the CPML and
-// edge state (fewer live scalar registers, no spills).
+// make_split -- form the rectangle of tile columns [xa, xb) and rows
+// [ya, yb) (cell rows, multiples of the interior tile height).  Interior and
+// border tiles run as separate launches, so the interior kernel carries none
+// of the CPML and edge state (fewer live scalar registers, no spills), and
+// the border kernel may use lower tiles (more wavefronts for the short, heavy
+// border work).
 struct TileSplit {
   int32_t xa, xb, ya, yb;
 };
 
 // Tile sets of a two-step launch.
 constexpr int kInnerTiles = 0;   // the interior rectangle (branch-free kernel)
-constexpr int kBorderTiles = 1;  // every other tile (CPML / edge kernel)
+constexpr int kBorderTiles = 1;  // every other tile (x and y checks)
 constexpr int kAllTiles = 2;     // all tiles, the kind chosen per tile
+constexpr int kSideTiles = 3;    // border tiles beside the interior rows (x checks only)
+constexpr int kBandTiles = 4;    // border tiles above and below the interior rows
+constexpr int kOrderedTiles = 5; // all tiles, border tiles first (one kernel)
 
+template <int TY>
 __device__ __forceinline__ bool in_split(const TileSplit &sp, int xt, int yt) {
-  return xt >= sp.xa && xt < sp.xb && yt >= sp.ya && yt < sp.yb;
+  return xt >= sp.xa && xt < sp.xb && yt * TY >= sp.ya && yt * TY < sp.yb;
 }
 
-// Tile and shot group of this workgroup within the interior (GEN false) or
-// border (GEN true) tile set.
-template <bool GEN>
-__device__ __forceinline__ TileCoord split_tile(const TileSplit &sp, int n_xt, int n_yt, int n_groups) {
-  const int iw = sp.xb - sp.xa, ih = sp.yb - sp.ya;
-  const int64_t n_tiles = GEN ? (int64_t)n_xt * n_yt - (int64_t)iw * ih : (int64_t)iw * ih;
+// Number of TY-row tiles of a tile set (TY divides the interior tile height).
+template <int TY>
+__host__ __device__ __forceinline__ int64_t split_tiles(const TileSplit &sp, int n_xt, int ny, int set) {
+  const int64_t iw = sp.xb - sp.xa, ih = (sp.yb - sp.ya) / TY;
+  const int64_t top = (int64_t)(sp.ya / TY) * n_xt, mid = ih * (n_xt - iw);
+  const int64_t bottom = (int64_t)((ny - sp.yb + TY - 1) / TY) * n_xt;
+  switch (set) {
+    case kInnerTiles: return iw * ih;
+    case kSideTiles: return mid;
+    case kBandTiles: return top + bottom;
+    default: return top + mid + bottom;  // kBorderTiles
+  }
+}
+
+// Tile (of height TY) and shot group of this workgroup within tile set SET
+// (kInnerTiles, kBorderTiles, kSideTiles or kBandTiles).
+template <int SET, int TY>
+__device__ __forceinline__ TileCoord split_tile(const TileSplit &sp, int n_xt, int ny, int n_groups) {
+  const int64_t n_tiles = split_tiles<TY>(sp, n_xt, ny, SET);
   const int64_t flat = xcd_remap((int64_t)blockIdx.x, n_tiles * n_groups);
   TileCoord c;
   c.grp = (int)(flat / n_tiles);
   int i = (int)(flat - (int64_t)c.grp * n_tiles);
-  if (!GEN) {
-    c.yt = sp.ya + i / iw;
+  const int iw = sp.xb - sp.xa;
+  if (SET == kInnerTiles) {
+    c.yt = sp.ya / TY + i / iw;
     c.xt = sp.xa + i % iw;
     return c;
   }
-  const int top = sp.ya * n_xt, w = n_xt - iw, mid = ih * w;
+  const int top = (sp.ya / TY) * n_xt, w = n_xt - iw, mid = ((sp.yb - sp.ya) / TY) * w;
+  if (SET == kSideTiles) i += top;                       // skip the top band
+  if (SET == kBandTiles && i >= top) i += mid;           // skip the side tiles
   if (i < top) {
     c.yt = i / n_xt;
     c.xt = i % n_xt;
   } else if (i < top + mid) {
     i -= top;
-    c.yt = sp.ya + i / w;
+    c.yt = sp.ya / TY + i / w;
     const int col = i % w;
     c.xt = col < sp.xa ? col : sp.xb + (col - sp.xa);
   } else {
     i -= top + mid;
-    c.yt = sp.yb + i / n_xt;
+    c.yt = sp.yb / TY + i / n_xt;
     c.xt = i % n_xt;
   }
   return c;
+}
+
+// Edge handling of the tiles of a set: 0 interior, 1 x only, 3 x and y.
+template <int SET>
+constexpr int set_edge() {
+  return SET == kInnerTiles ? 0 : (SET == kSideTiles ? 1 : 3);
+}
+
+// This workgroup's tile for tile set SET; for the one-kernel sets also the
+// edge handling the tile needs.  kOrderedTiles hands out the border tiles
+// (their long per-wavefront latency) before the interior ones, so the
+// interior work overlaps them.
+template <int SET, int TY>
+__device__ __forceinline__ TileCoord pick_tile(const TileSplit &sp, int n_xt, int n_yt, int ny,
+                                               int n_groups, int *edge) {
+  if constexpr (SET == kAllTiles) {
+    const TileCoord tc = decode_tile(n_xt, n_yt, n_groups);
+    *edge = in_split<TY>(sp, tc.xt, tc.yt) ? 0 : 3;
+    return tc;
+  } else if constexpr (SET == kOrderedTiles) {
+    const int64_t nb = split_tiles<TY>(sp, n_xt, ny, kBorderTiles);
+    const int64_t ni = split_tiles<TY>(sp, n_xt, ny, kInnerTiles);
+    // blocks 0 .. nb*n_groups-1 (dispatched first, spread over the XCDs):
+    // border tiles; the rest: the interior, XCD-grouped
+    const int64_t b = blockIdx.x, nbg = nb * n_groups;
+    TileCoord c;
+    int64_t i;
+    const bool border = b < nbg;
+    if (border) {
+      c.grp = (int)(b / nb);
+      i = b - (int64_t)c.grp * nb;
+    } else {
+      const int64_t f = xcd_remap(b - nbg, ni * n_groups);
+      c.grp = (int)(f / ni);
+      i = f - (int64_t)c.grp * ni;
+    }
+    const int iw = sp.xb - sp.xa;
+    if (!border) {
+      c.yt = sp.ya / TY + (int)(i / iw);
+      c.xt = sp.xa + (int)(i % iw);
+      *edge = 0;
+      return c;
+    }
+    const int top = (sp.ya / TY) * n_xt, w = n_xt - iw, mid = ((sp.yb - sp.ya) / TY) * w;
+    int k = (int)i;
+    if (k < top) {
+      c.yt = k / n_xt;
+      c.xt = k % n_xt;
+      *edge = 3;
+    } else if (k < top + mid) {
+      k -= top;
+      c.yt = sp.ya / TY + k / w;
+      const int col = k % w;
+      c.xt = col < sp.xa ? col : sp.xb + (col - sp.xa);
+      *edge = 1;
+    } else {
+      k -= top + mid;
+      c.yt = sp.yb / TY + k / n_xt;
+      c.xt = k % n_xt;
+      *edge = 3;
+    }
+    return c;
+  } else {
+    *edge = set_edge<SET>();
+    return split_tile<SET, TY>(sp, n_xt, ny, n_groups);
+  }
 }
 
 template <typename T> struct Fwd2Args {
@@ -112,20 +200,30 @@ __device__ __forceinline__ void fused_points(const Points &pts, int n_shots, int
   }
 }
 
-// GEN: tile touches the grid edge or a CPML strip (per-cell checks);
-// otherwise the branch-free interior path.
-template <typename T, int TY, int FMT, bool GEN>
+// EDGE bit 0: the tile's columns touch the grid edge or an x CPML strip;
+// bit 1: its rows touch the grid edge or a y strip (per-cell checks for
+// that axis).  EDGE 0 is the branch-free interior path.
+template <typename T, int TY, int FMT, int EDGE>
 __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, int yt) {
+  constexpr bool GX = (EDGE & 1) != 0, GY = (EDGE & 2) != 0, GEN = EDGE != 0;
   const Geom<T> &g = a.g;
   const int ny = g.ny, nx = g.nx;
   const int lane = threadIdx.x;
   const int x = xt * kOwn2 - 2 + lane;
   const int y0 = yt * TY;
   const bool own = lane >= 2 && lane <= 61 && x < nx;
-  const bool e_col = !GEN || (x >= 1 && x < nx);       // Ey and Hx columns
-  const bool hz_col = !GEN || (x >= 1 && x < nx - 1);  // Hz columns
-  const bool pml_xe = GEN && (x < g.pml_x0 || x >= g.pml_x1);
-  const bool pml_xh = GEN && (x < g.pml_x0 || x >= g.pml_x1h);
+  const bool e_col = !GX || (x >= 1 && x < nx);       // Ey and Hx columns
+  const bool hz_col = !GX || (x >= 1 && x < nx - 1);  // Hz columns
+  const bool pml_xe = GX && (x < g.pml_x0 || x >= g.pml_x1);
+  const bool pml_xh = GX && (x < g.pml_x0 || x >= g.pml_x1h);
+  // x profiles of this lane's column, loaded once (they do not change with
+  // the row; the clamped column keeps halo lanes in bounds)
+  T xb = 0, xa = 0, xk = 1, xbh = 0, xah = 0, xkh = 1;
+  if constexpr (GX) {
+    const int xc = min(max(x, 0), nx - 1);
+    xb = g.X(kB, xc), xa = g.X(kA, xc), xk = g.X(kK, xc);
+    xbh = g.X(kBh, xc), xah = g.X(kAh, xc), xkh = g.X(kKh, xc);
+  }
 
   const int64_t base = (int64_t)s * g.numel;
   const T *__restrict__ ey_i = a.ey_i + base;
@@ -156,10 +254,8 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   // CPML memories of step t (GEN only): m_ey_z at the Hx rows, m_ey_x at the
   // Hz rows, m_hz_x and m_hx_z at the Ey rows of step t+1.
   T mez[NX], mex[NZ], mhzx[NZ], mhxz[NZ];
-  if constexpr (GEN) {
+  if constexpr (GY) {
     const T *__restrict__ mez_i = a.mez_i + base;
-    const T *__restrict__ mex_i = a.mex_i + base;
-    const T *__restrict__ mhzx_i = a.mhzx_i + base;
     const T *__restrict__ mhxz_i = a.mhxz_i + base;
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
@@ -171,9 +267,17 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
     for (int k = 0; k < NZ; ++k) {
       const int y = y0 - 1 + k;
       const bool pml_ye = y < g.pml_y0 || y >= g.pml_y1;
+      mhxz[k] = pml_ye ? L(mhxz_i, y) : (T)0;
+    }
+  }
+  if constexpr (GX) {
+    const T *__restrict__ mex_i = a.mex_i + base;
+    const T *__restrict__ mhzx_i = a.mhzx_i + base;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+      const int y = y0 - 1 + k;
       mex[k] = pml_xh ? L(mex_i, y) : (T)0;
       mhzx[k] = pml_xe ? L(mhzx_i, y) : (T)0;
-      mhxz[k] = pml_ye ? L(mhxz_i, y) : (T)0;
     }
   }
 
@@ -184,9 +288,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   for (int k = 0; k < NX; ++k) {
     const int y = y0 - 2 + k;
     hx1[k] = hx0[k];
-    if (!GEN || (e_col && y >= 1 && y < ny - 1)) {
+    if (e_col && (!GY || (y >= 1 && y < ny - 1))) {
       T d = (ey0[k + 1] - ey0[k]) * g.rdy;
-      if (GEN && (y < g.pml_y0 || y >= g.pml_y1h)) {
+      if (GY && (y < g.pml_y0 || y >= g.pml_y1h)) {
         const T m = g.Y(kBh, y) * mez[k] + g.Y(kAh, y) * d;
         mez[k] = m;
         d = d / g.Y(kKh, y) + m;
@@ -199,30 +303,30 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
 #pragma unroll
   for (int k = 0; k < NZ; ++k) {
     const int y = y0 - 1 + k;
-    const bool row = !GEN || (y >= 1 && y < ny);
+    const bool row = !GY || (y >= 1 && y < ny);
     const T ey_c = ey0[k + 1];
     const T ey_r = shfl_dn1(ey_c);  // unused at lane 63
     hz1[k] = hz0[k];
-    if (!GEN || (hz_col && row)) {
+    if (hz_col && row) {
       T d = (ey_r - ey_c) * g.rdx;
-      if (GEN && pml_xh) {
-        const T m = g.X(kBh, x) * mex[k] + g.X(kAh, x) * d;
+      if (pml_xh) {
+        const T m = xbh * mex[k] + xah * d;
         mex[k] = m;
-        d = d / g.X(kKh, x) + m;
+        d = d / xkh + m;
       }
       hz1[k] = hz0[k] + cqv[k + 1] * d;
     }
     const T hz_l = shfl_up1(hz1[k]);  // unused at lane 0
     T ey_n = ey_c;
-    if (!GEN || (e_col && row)) {
+    if (e_col && row) {
       T dhz = (hz1[k] - hz_l) * g.rdx;
       T dhx = (hx1[k + 1] - hx1[k]) * g.rdy;
-      if (GEN && pml_xe) {
-        const T m = g.X(kB, x) * mhzx[k] + g.X(kA, x) * dhz;
+      if (pml_xe) {
+        const T m = xb * mhzx[k] + xa * dhz;
         mhzx[k] = m;
-        dhz = dhz / g.X(kK, x) + m;
+        dhz = dhz / xk + m;
       }
-      if (GEN && (y < g.pml_y0 || y >= g.pml_y1)) {
+      if (GY && (y < g.pml_y0 || y >= g.pml_y1)) {
         const T m = g.Y(kB, y) * mhxz[k] + g.Y(kA, y) * dhx;
         mhxz[k] = m;
         dhx = dhx / g.Y(kK, y) + m;
@@ -237,7 +341,7 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
       }
       ey_n = cav[k] * ey_c + cbv[k] * curl;
     }
-    if (!GEN || (y >= 0 && y < ny))
+    if (!GY || (y >= 0 && y < ny))
       fused_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.f0, a.r0, own && k >= 1 && k <= TY, ey_n);
     ey1[k] = ey_n;
   }
@@ -250,9 +354,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   for (int k = 0; k < TY + 1; ++k) {
     const int y = y0 - 1 + k;
     hx2[k] = hx1[k + 1];
-    if (!GEN || (e_col && y >= 1 && y < ny - 1)) {
+    if (e_col && (!GY || (y >= 1 && y < ny - 1))) {
       T d = (ey1[k + 1] - ey1[k]) * g.rdy;
-      if (GEN && (y < g.pml_y0 || y >= g.pml_y1h)) {
+      if (GY && (y < g.pml_y0 || y >= g.pml_y1h)) {
         const T m = g.Y(kBh, y) * mez[k + 1] + g.Y(kAh, y) * d;
         mez[k + 1] = m;
         d = d / g.Y(kKh, y) + m;
@@ -266,32 +370,32 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
 #pragma unroll
   for (int k = 0; k < TY; ++k) {
     const int y = y0 + k;
-    if (GEN && y >= ny) break;  // uniform
-    const bool row = !GEN || y >= 1;
+    if (GY && y >= ny) break;  // uniform
+    const bool row = !GY || y >= 1;
     const int j = y * nx + x;
     const T ey_c = ey1[k + 1];
     const T ey_r = shfl_dn1(ey_c);
     T hz_n = hz1[k + 1];
-    if (!GEN || (hz_col && row)) {
+    if (hz_col && row) {
       T d = (ey_r - ey_c) * g.rdx;
-      if (GEN && pml_xh) {
-        const T m = g.X(kBh, x) * mex[k + 1] + g.X(kAh, x) * d;
+      if (pml_xh) {
+        const T m = xbh * mex[k + 1] + xah * d;
         mex[k + 1] = m;
-        d = d / g.X(kKh, x) + m;
+        d = d / xkh + m;
       }
       hz_n = hz1[k + 1] + cqv[k + 2] * d;
     }
     const T hz_l = shfl_up1(hz_n);
     T ey_n = ey_c;
-    if (!GEN || (e_col && row)) {
+    if (e_col && row) {
       T dhz = (hz_n - hz_l) * g.rdx;
       T dhx = (hx2[k + 1] - hx2[k]) * g.rdy;
-      if (GEN && pml_xe) {
-        const T m = g.X(kB, x) * mhzx[k + 1] + g.X(kA, x) * dhz;
+      if (pml_xe) {
+        const T m = xb * mhzx[k + 1] + xa * dhz;
         mhzx[k + 1] = m;
-        dhz = dhz / g.X(kK, x) + m;
+        dhz = dhz / xk + m;
       }
-      if (GEN && (y < g.pml_y0 || y >= g.pml_y1)) {
+      if (GY && (y < g.pml_y0 || y >= g.pml_y1)) {
         const T m = g.Y(kB, y) * mhxz[k + 1] + g.Y(kA, y) * dhx;
         mhxz[k + 1] = m;
         dhx = dhx / g.Y(kK, y) + m;
@@ -311,11 +415,13 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
       ey_o[j] = ey_n;
       hx_o[j] = hx2[k + 1];
       hz_o[j] = hz_n;
-      if constexpr (GEN) {
+      if constexpr (GY) {
         if (y < g.pml_y0 || y >= g.pml_y1h) a.mez_o[base + j] = mez[k + 2];
+        if (y < g.pml_y0 || y >= g.pml_y1) a.mhxz_o[base + j] = mhxz[k + 1];
+      }
+      if constexpr (GX) {
         if (pml_xh) a.mex_o[base + j] = mex[k + 1];
         if (pml_xe) a.mhzx_o[base + j] = mhzx[k + 1];
-        if (y < g.pml_y0 || y >= g.pml_y1) a.mhxz_o[base + j] = mhxz[k + 1];
       }
     }
   }
@@ -326,14 +432,20 @@ template <typename T, int TY, int FMT, int SET>
 __global__ __launch_bounds__(512) void fwd2_step(Fwd2Args<T> a) {
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
-  const TileCoord tc = SET == kAllTiles ? decode_tile(a.pts.n_xt, a.n_yt, n_groups)
-                                        : split_tile<SET == kBorderTiles>(a.split, a.pts.n_xt, a.n_yt, n_groups);
+  int edge = 3;
+  const TileCoord tc = pick_tile<SET, TY>(a.split, a.pts.n_xt, a.n_yt, a.g.ny, n_groups, &edge);
   const int s = tc.grp * G + (int)threadIdx.y;
   if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
-  if (SET == kInnerTiles || (SET == kAllTiles && in_split(a.split, tc.xt, tc.yt)))
-    fwd2_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt);
-  else
-    fwd2_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt);
+  if constexpr (SET == kAllTiles || SET == kOrderedTiles) {
+    if (edge == 0)
+      fwd2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt);
+    else if (edge == 1)
+      fwd2_tile<T, TY, FMT, 1>(a, s, tc.xt, tc.yt);
+    else
+      fwd2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt);
+  } else {
+    fwd2_tile<T, TY, FMT, set_edge<SET>()>(a, s, tc.xt, tc.yt);
+  }
 }
 
 // ----------------------------------------------------------------- adjoint
@@ -387,19 +499,26 @@ __device__ __forceinline__ void fused_adj_points(const Points &pts, int n_shots,
 // gradient terms of iteration A / B go to red_a / red_b ([TY][64] LDS
 // slots each, per coefficient) when `reduce`, else straight to the
 // per-shot gradient arrays.
-template <typename T, int TY, int FMT, bool GEN>
+template <typename T, int TY, int FMT, int EDGE>
 __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, int yt,
                                           double *red_ca_a, double *red_cb_a, double *red_ca_b,
                                           double *red_cb_b, bool reduce) {
+  constexpr bool GX = (EDGE & 1) != 0, GY = (EDGE & 2) != 0, GEN = EDGE != 0;
   const Geom<T> &g = a.g;
   const int ny = g.ny, nx = g.nx;
   const int lane = threadIdx.x;
   const int x = xt * kOwn2 - 2 + lane;
   const int y0 = yt * TY;
   const bool own = lane >= 2 && lane <= 61 && x < nx;
-  const bool e_col = !GEN || (x >= 1 && x < nx);
-  const bool pml_xe = GEN && (x < g.pml_x0 || x >= g.pml_x1);
-  const bool pml_xh = GEN && (x < g.pml_x0 || x >= g.pml_x1h);
+  const bool e_col = !GX || (x >= 1 && x < nx);
+  const bool pml_xe = GX && (x < g.pml_x0 || x >= g.pml_x1);
+  const bool pml_xh = GX && (x < g.pml_x0 || x >= g.pml_x1h);
+  T xb = 0, xa = 0, xk = 1, xbh = 0, xah = 0, xkh = 1;  // this column's x profiles
+  if constexpr (GX) {
+    const int xc = min(max(x, 0), nx - 1);
+    xb = g.X(kB, xc), xa = g.X(kA, xc), xk = g.X(kK, xc);
+    xbh = g.X(kBh, xc), xah = g.X(kAh, xc), xkh = g.X(kKh, xc);
+  }
   const bool grad_a = FMT != 0 && (a.ey_store0 != nullptr || a.curl_store0 != nullptr);
   const bool grad_b = FMT != 0 && (a.ey_store1 != nullptr || a.curl_store1 != nullptr);
 
@@ -440,7 +559,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
     se0[k] = sc0[k] = se1[k] = sc1[k] = 0.0f;
     if constexpr (sizeof(T) == 4 && FMT != 0) {
       const int y = y0 + k;
-      if (own && (!GEN || (y >= 1 && y < ny && x >= 1))) {
+      if (own && (!GY || (y >= 1 && y < ny)) && (!GX || x >= 1)) {
         const int64_t i = base + (int64_t)y * nx + x;
         if (grad_a && a.ey_store0) se0[k] = (float)get_store<T, FMT>(a.ey_store0, i);
         if (grad_a && a.curl_store0) sc0[k] = (float)get_store<T, FMT>(a.curl_store0, i);
@@ -452,10 +571,8 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
   // CPML memories (GEN only): my and nz indexed from row y0-2, mx and nx
   // from row y0-1.
   T my[NE], nz[NX], mx[NZ], nxm[NZ];
-  if constexpr (GEN) {
-    const T *__restrict__ mx_i = a.mx_i + base;
+  if constexpr (GY) {
     const T *__restrict__ my_i = a.my_i + base;
-    const T *__restrict__ nx_i = a.nx_i + base;
     const T *__restrict__ nz_i = a.nz_i + base;
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
@@ -467,6 +584,10 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
       const int y = y0 - 2 + k;
       nz[k] = (y < g.pml_y0 || y >= g.pml_y1h) ? L(nz_i, y) : (T)0;
     }
+  }
+  if constexpr (GX) {
+    const T *__restrict__ mx_i = a.mx_i + base;
+    const T *__restrict__ nx_i = a.nx_i + base;
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
       const int y = y0 - 1 + k;
@@ -478,38 +599,38 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
   // The single-step kernel's terms (adj_tile), each also advancing its CPML
   // memory `m` where it is updated.
   auto raw_x = [&](int y, T lm, T cbx, T &m) -> T {  // adjoint of dHz/dx
-    if (GEN && (y < 1 || y >= ny || x < 1 || x >= nx)) return (T)0;
+    if ((GY && (y < 1 || y >= ny)) || (GX && (x < 1 || x >= nx))) return (T)0;
     const T q = cbx * lm;
     if (!pml_xe) return q;
-    const T mn = g.X(kB, x) * m + q;
+    const T mn = xb * m + q;
     m = mn;
-    return g.X(kA, x) * mn + q / g.X(kK, x);
+    return xa * mn + q / xk;
   };
   auto nraw_y = [&](int y, T lm, T cbx, T &m) -> T {  // negated adjoint of dHx/dz
-    if (GEN && (y < 1 || y >= ny || !e_col)) return (T)0;
+    if ((GY && (y < 1 || y >= ny)) || !e_col) return (T)0;
     const T q = cbx * lm;
-    if (!(GEN && (y < g.pml_y0 || y >= g.pml_y1))) return q;
+    if (!(GY && (y < g.pml_y0 || y >= g.pml_y1))) return q;
     const T mn = g.Y(kB, y) * m - q;
     m = mn;
     return q / g.Y(kK, y) - g.Y(kA, y) * mn;
   };
   auto raw_z = [&](int y, T lhxv, T cqx, T &m) -> T {  // adjoint of dEy/dz (Hx update)
-    if (GEN && (y < 1 || y >= ny - 1 || !e_col)) return (T)0;
+    if ((GY && (y < 1 || y >= ny - 1)) || !e_col) return (T)0;
     const T adj = -cqx * lhxv;
-    if (!(GEN && (y < g.pml_y0 || y >= g.pml_y1h))) return adj;
+    if (!(GY && (y < g.pml_y0 || y >= g.pml_y1h))) return adj;
     const T mn = g.Y(kBh, y) * m + adj;
     m = mn;
     return g.Y(kAh, y) * mn + adj / g.Y(kKh, y);
   };
   auto raw_xh = [&](int y, T lhzv, T cqx, T &m) -> T {  // adjoint of dEy/dx (Hz update)
-    if (GEN && (y < 1 || y >= ny || x < 1 || x >= nx - 1)) return (T)0;
+    if ((GY && (y < 1 || y >= ny)) || (GX && (x < 1 || x >= nx - 1))) return (T)0;
     const T adj = cqx * lhzv;
     if (!pml_xh) return adj;
-    const T mn = g.X(kBh, x) * m + adj;
+    const T mn = xbh * m + adj;
     m = mn;
-    return g.X(kAh, x) * mn + adj / g.X(kKh, x);
+    return xah * mn + adj / xkh;
   };
-  auto in_e = [&](int y) -> bool { return e_col && (!GEN || (y >= 1 && y < ny)); };
+  auto in_e = [&](int y) -> bool { return e_col && (!GY || (y >= 1 && y < ny)); };
   auto grad_terms = [&](bool on, int y, T lm_t, float se, float sc, const void *es, const void *cs,
                         double *ra, double *rb, int k) {
     if (!on || !own || !in_e(y)) return;
@@ -565,7 +686,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
       ley_n = t1 + t2;
     }
     const bool owned_row = k >= 1 && k <= TY;
-    if (!GEN || (y >= 0 && y < ny))
+    if (!GY || (y >= 0 && y < ny))
       fused_adj_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.grad_r0, a.grad_f0, own && owned_row, ley_n);
     if (owned_row)
       grad_terms(grad_a, y, ley_n, se0[k - 1], sc0[k - 1], a.ey_store0, a.curl_store0, red_ca_a,
@@ -591,7 +712,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
 #pragma unroll
   for (int k = 0; k < TY; ++k) {
     const int y = y0 + k;
-    if (GEN && y >= ny) break;  // uniform
+    if (GY && y >= ny) break;  // uniform
     const int j = y * nx + x;
     const T ley_c = leyA[k + 1];
     const T rx = raw_x(y, ley_c, cbv[k + 2], mx[k + 1]);
@@ -614,10 +735,14 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
       lhz_o[j] = lhz_n;
       if constexpr (GEN) {
         if (in_e(y)) {
-          if (pml_xe) a.mx_o[base + j] = mx[k + 1];
-          if (y < g.pml_y0 || y >= g.pml_y1) a.my_o[base + j] = my[k + 2];
-          if (y < ny - 1 && (y < g.pml_y0 || y >= g.pml_y1h)) a.nz_o[base + j] = nz[k + 2];
-          if (x < nx - 1 && pml_xh) a.nx_o[base + j] = nxm[k + 1];
+          if constexpr (GX) {
+            if (pml_xe) a.mx_o[base + j] = mx[k + 1];
+            if (x < nx - 1 && pml_xh) a.nx_o[base + j] = nxm[k + 1];
+          }
+          if constexpr (GY) {
+            if (y < g.pml_y0 || y >= g.pml_y1) a.my_o[base + j] = my[k + 2];
+            if (y < ny - 1 && (y < g.pml_y0 || y >= g.pml_y1h)) a.nz_o[base + j] = nz[k + 2];
+          }
         }
       }
     }
@@ -628,7 +753,8 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
 // [2 coefficients][2 iterations][G][TY][64].  Each iteration's shot sum is
 // added to the cell separately, in iteration order (as two single steps do).
 template <typename T, int TY, int FMT, int SET>
-__device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileCoord &tc, double *red) {
+__device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileCoord &tc, int edge,
+                                               double *red) {
   const int lane = threadIdx.x;
   const int w = (int)threadIdx.y;
   const int G = (int)blockDim.y;
@@ -650,10 +776,16 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
     }
   }
   if (s < a.n_shots) {  // inactive wavefronts still join the barrier below
-    if (SET == kInnerTiles || (SET == kAllTiles && in_split(a.split, tc.xt, tc.yt)))
-      adj2_tile<T, TY, FMT, false>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
-    else
-      adj2_tile<T, TY, FMT, true>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+    if constexpr (SET == kAllTiles || SET == kOrderedTiles) {
+      if (edge == 0)
+        adj2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+      else if (edge == 1)
+        adj2_tile<T, TY, FMT, 1>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+      else
+        adj2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+    } else {
+      adj2_tile<T, TY, FMT, set_edge<SET>()>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+    }
   }
   if (!reduce) return;  // uniform over the workgroup
   __syncthreads();
@@ -696,12 +828,13 @@ __global__ __launch_bounds__(512) void adj2_step(Adj2Args<T> a) {
   extern __shared__ double red[];
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
-  const TileCoord tc = SET == kAllTiles ? decode_tile(a.pts.n_xt, a.n_yt, n_groups)
-                                        : split_tile<SET == kBorderTiles>(a.split, a.pts.n_xt, a.n_yt, n_groups);
-  adj2_workgroup<T, TY, FMT, SET>(a, tc, red);
+  int edge = 3;
+  const TileCoord tc = pick_tile<SET, TY>(a.split, a.pts.n_xt, a.n_yt, a.g.ny, n_groups, &edge);
+  adj2_workgroup<T, TY, FMT, SET>(a, tc, edge, red);
 }
 
-// Host side of the split: the interior rectangle for TY-row tiles.  A tile
+// Host side of the split: the interior rectangle for TY-row tiles (rows
+// returned in cells).  A tile
 // is interior when its columns x0-2..x0+61 and rows y0-2..y0+TY+1 lie in
 // [1, n-2] and outside the CPML strips (H-phase bounds, which contain the
 // E-phase ones); the test separates into a column and a row condition.
@@ -725,7 +858,7 @@ inline TileSplit make_split(const Geom<T> &g, int n_xt, int n_yt, int ty) {
   while (ya < n_yt && !row_ok(ya)) ++ya;
   int yb = ya;
   while (yb < n_yt && row_ok(yb)) ++yb;
-  if (xb > xa && yb > ya) sp = TileSplit{xa, xb, ya, yb};
+  if (xb > xa && yb > ya) sp = TileSplit{xa, xb, ya * ty, yb * ty};
   return sp;
 }
 

@@ -239,6 +239,10 @@ def load() -> ctypes.CDLL:
     lib.tide_tm2d_set_tile_split.argtypes = [ctypes.c_int] * 2
     lib.tide_tm2d_get_tile_split.restype = ctypes.c_int
     lib.tide_tm2d_get_tile_split.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    lib.tide_tm2d_set_border_rows.restype = ctypes.c_int
+    lib.tide_tm2d_set_border_rows.argtypes = [ctypes.c_int]
+    lib.tide_tm2d_get_border_rows.restype = ctypes.c_int
+    lib.tide_tm2d_get_border_rows.argtypes = []
     _lib = lib
     return lib
 
@@ -337,6 +341,15 @@ def get_tm2d_tile_split() -> tuple[bool, bool]:
     return bool(out[0]), bool(out[1])
 
 
+def set_tm2d_border_rows(rows: int) -> None:
+    """Tile height of the split border kernels (1 or 2)."""
+    check(load().tide_tm2d_set_border_rows(int(rows)))
+
+
+def get_tm2d_border_rows() -> int:
+    return int(load().tide_tm2d_get_border_rows())
+
+
 def stream_handle(device: torch.device) -> int | None:
     """The HIP stream of torch's current stream on ``device``."""
     return torch.cuda.current_stream(device).cuda_stream
@@ -346,6 +359,7 @@ __all__ = [
     "TideNativeError",
     "check",
     "dtype_name",
+    "get_tm2d_border_rows",
     "get_tm2d_shot_loop",
     "get_tm2d_tile_split",
     "get_tm2d_time_block",
@@ -353,6 +367,7 @@ __all__ = [
     "int64_array",
     "ptr_array",
     "set_em3d_tuning",
+    "set_tm2d_border_rows",
     "set_tm2d_shot_loop",
     "set_tm2d_tile_split",
     "set_tm2d_time_block",

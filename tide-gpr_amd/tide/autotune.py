@@ -40,8 +40,8 @@ FWD_CANDIDATES: tuple[tuple[int, int, int, int, int], ...] = (
 ADJ_CANDIDATES: tuple[tuple[int, int, int, int, int], ...] = (
     (2, 2, 0, 1, 0), (2, 4, 0, 1, 0), (2, 8, 0, 1, 0), (4, 4, 0, 1, 0), (8, 2, 0, 1, 0),
     (2, 4, 1, 1, 0), (4, 2, 1, 1, 0),
-    (2, 4, 0, 2, 1), (4, 2, 0, 2, 1), (4, 4, 0, 2, 1), (4, 8, 0, 2, 1),
-    (2, 4, 0, 2, 0), (4, 4, 0, 2, 0))
+    (4, 4, 0, 2, 1), (4, 8, 0, 2, 1),
+    (2, 4, 0, 2, 0), (4, 2, 0, 2, 0), (4, 4, 0, 2, 0), (4, 8, 0, 2, 0))
 
 _cache: dict[tuple, tuple[tuple[int, ...], tuple[int, ...]]] = {}
 
