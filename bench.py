@@ -33,9 +33,10 @@ for _p in (ROOT / "tide-gpr_amd", ROOT):
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per cell-update measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-# (scaled by a 1 GiB copy's counters) for the kernels as built; see
-# profiles/tm2d_config2_hbm_counters.md.  Reported as roofline.traffic.
-MEASURED_TRAFFIC_B_PER_CELL = {"fwd_step": 37.25, "adj_step": 41.58}
+# (scaled by a 1 GiB copy's counters) for the kernels as built, default
+# launch shapes; see profiles/tm2d_config2_hbm_counters.md.  Reported as
+# roofline.traffic.
+MEASURED_TRAFFIC_B_PER_CELL = {"fwd2_step": 31.02, "adj2_step": 32.53}
 
 
 def log(msg: str) -> None:
@@ -299,8 +300,11 @@ def main() -> None:
         step()
         torch.cuda.synchronize()
         log(f"warmup step {k + 1}/{args.warmup} done")
+    from tide import _native
+
     timer = KernelTimer()
     set_kernel_timer(timer)
+    _native.set_tm2d_loop_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -310,6 +314,11 @@ def main() -> None:
     elapsed = time.perf_counter() - t0
     log(f"timed region {elapsed:.2f} s")
     set_kernel_timer(None)
+    # HIP events around the kernel launches of every time loop (no set-up,
+    # gradient fold or copies), on the stream the kernels run on
+    loop_ms = {"forward_store": _native.take_tm2d_loop_timing(0),
+               "adjoint": _native.take_tm2d_loop_timing(1)}
+    _native.set_tm2d_loop_timing(False)
 
     if args.calibrate_copy:
         a = torch.empty(1 << 28, device=dev, dtype=torch.float32)
@@ -326,17 +335,19 @@ def main() -> None:
     kernels = {}
     for name, d in summ.items():
         spl = int(d["steps_per_launch"])
-        per_launch_ms = d["ms"] / d["launches"]
-        achieved = b_alg[name] * d["cell_updates"] / (d["ms"] * 1e-3) / 1e9
+        k_ms, k_launches, _ = loop_ms.get(name, (d["ms"], d["launches"], 0))
+        achieved = b_alg[name] * d["cell_updates"] / (k_ms * 1e-3) / 1e9
         two = spl == 2
         kernels[name] = {
             "kernel": ("adj2_step" if two else "adj_step") if name == "adjoint"
                       else ("fwd2_step" if two else "fwd_step"),
             "steps_per_launch": spl,
-            "avg_launch_us": round(per_launch_ms * 1e3, 2),
+            "launches": k_launches,
+            "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
+            "call_avg_launch_us": round(d["ms"] / d["launches"] * 1e3, 2),
             "bytes_per_cell_update": b_alg[name],
             "achieved_GBs": round(achieved, 1),
-            "gcell_per_s": round(d["cell_updates"] / (d["ms"] * 1e-3) / 1e9, 2),
+            "gcell_per_s": round(d["cell_updates"] / (k_ms * 1e-3) / 1e9, 2),
         }
     dom = max(summ, key=lambda k: summ[k]["ms"])
     dom_k = kernels[dom]

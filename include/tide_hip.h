@@ -258,6 +258,16 @@ int tide_tm2d_get_tile_split(int *out2);
 int tide_tm2d_set_border_rows(int rows);
 int tide_tm2d_get_border_rows(void);
 
+/*
+ * Kernel timing without a profiler: when enabled, every single-problem
+ * TM2D time loop is bracketed by HIP events on its stream (the launches
+ * only).  take_loop_timing waits for the recorded loops of a pass (0
+ * forward, 1 adjoint), returns their summed milliseconds, launch count and
+ * loop count, and clears them.  No reference counterpart.
+ */
+int tide_tm2d_set_loop_timing(int enable);
+int tide_tm2d_take_loop_timing(int pass, double *ms, int64_t *launches, int64_t *loops);
+
 /* ------------------------------------------------------- batched 2-D plans
  * Many independent fp32 TM2D problems (different grid sizes, shot counts
  * and nt) advanced together: one launch per time step covers every problem

@@ -48,6 +48,40 @@ template <typename T> struct Problem {
 
 Tuning g_tuning;
 
+// Optional HIP-event timing of the time loops (launches only, no set-up,
+// fold or copies): a caller turns it on, runs, then collects the summed
+// milliseconds and launch counts per pass.
+struct LoopTimes {
+  bool enabled = false;
+  struct Rec {
+    hipEvent_t start, stop;
+    int64_t launches;
+  };
+  std::vector<Rec> recs[2];  // 0 forward, 1 adjoint
+};
+LoopTimes g_loop_times;
+
+struct LoopTimer {
+  int pass;
+  hipStream_t stream;
+  hipEvent_t start = nullptr, stop = nullptr;
+  LoopTimer(int pass_, hipStream_t s) : pass(pass_), stream(s) {
+    if (!g_loop_times.enabled) return;
+    if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
+      (void)hipGetLastError();
+      start = stop = nullptr;
+      return;
+    }
+    (void)hipEventRecord(start, stream);
+  }
+  void finish(int64_t launches) {
+    if (start == nullptr) return;
+    (void)hipEventRecord(stop, stream);
+    g_loop_times.recs[pass].push_back({start, stop, launches});
+    start = stop = nullptr;
+  }
+};
+
 constexpr int kMaxLoopTiles = 4;  // wavefronts per workgroup of the shot-looping kernels
 
 template <typename T, int FMT>
@@ -335,7 +369,9 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     groups2 = (p.n_shots + block2.y - 1) / block2.y;
   }
   int64_t t = start_t;
-  for (; fused && t + 1 < start_t + nt; t += 2) {
+  LoopTimer timer(0, stream);
+  int64_t launches = 0;
+  for (; fused && t + 1 < start_t + nt; t += 2, ++launches) {
     a2.ey_i = cur[0]; a2.hx_i = cur[1]; a2.hz_i = cur[2];
     a2.mex_i = cur[3]; a2.mez_i = cur[4];
     a2.mhzx_i = cur_e[0]; a2.mhxz_i = cur_e[1];
@@ -363,7 +399,7 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
   }
   a.mhzx = cur_e[0];
   a.mhxz = cur_e[1];
-  for (; t < start_t + nt; ++t) {
+  for (; t < start_t + nt; ++t, ++launches) {
     a.ey_i = cur[0]; a.hx_i = cur[1]; a.hz_i = cur[2];
     a.mex_i = cur[3]; a.mez_i = cur[4];
     a.ey_o = nxt[0]; a.hx_o = nxt[1]; a.hz_o = nxt[2];
@@ -383,6 +419,7 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     }
     for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
   }
+  timer.finish(launches);
   TIDE_HIP_TRY(hipGetLastError());
   if (cur[0] != ey) {
     TIDE_HIP_TRY(hipMemcpyAsync(ey, cur[0], bytes, hipMemcpyDeviceToDevice, stream));
@@ -483,6 +520,8 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     return (const char *)buf + (size_t)(t / step_ratio) * store_step_bytes;
   };
   int64_t t = start_t - 1;
+  LoopTimer timer(1, stream);
+  int64_t launches = 0;
   if (fused) {
     const bool split2 = g_tuning.adj_split != 0;
     const int rows2 = std::min<int>(rows_for<T>(g_tuning.adj_rows), 4);
@@ -502,7 +541,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     a2.gcb = gb_k;
     const dim3 block2 = block;  // same shot groups as the single-step launches
     a2.split = make_split(p.g, dpt2.n_xt, a2.n_yt, rows2);
-    for (; t - 1 >= start_t - nt; t -= 2) {
+    for (; t - 1 >= start_t - nt; t -= 2, ++launches) {
       a2.ley_i = cur[0]; a2.lhx_i = cur[1]; a2.lhz_i = cur[2];
       a2.mx_i = cur[3]; a2.my_i = cur[4]; a2.nx_i = cur[5]; a2.nz_i = cur[6];
       a2.ley_o = nxt[0]; a2.lhx_o = nxt[1]; a2.lhz_o = nxt[2];
@@ -526,7 +565,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
       for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
     }
   }
-  for (; t >= start_t - nt; --t) {
+  for (; t >= start_t - nt; --t, ++launches) {
     a.ley_i = cur[0]; a.lhx_i = cur[1]; a.lhz_i = cur[2];
     a.mx_i = cur[3]; a.my_i = cur[4]; a.nx_i = cur[5]; a.nz_i = cur[6];
     a.ley_o = nxt[0]; a.lhx_o = nxt[1]; a.lhz_o = nxt[2];
@@ -548,6 +587,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     }
     for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
   }
+  timer.finish(launches);
   TIDE_HIP_TRY(hipGetLastError());
   const unsigned fold_blocks = (unsigned)((numel + 255) / 256);
   if (gacc) fold_groups<<<fold_blocks, 256, 0, stream>>>(gca, gacc, numel, (int)n_groups);
@@ -914,6 +954,40 @@ extern "C" int tide_tm2d_get_tile_split(int *out2) {
   out2[0] = g_tuning.fwd_split;
   out2[1] = g_tuning.adj_split;
   return kOk;
+}
+
+extern "C" int tide_tm2d_set_loop_timing(int enable) {
+  g_loop_times.enabled = enable != 0;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_take_loop_timing(int pass, double *ms, int64_t *launches, int64_t *loops) {
+  if (pass < 0 || pass > 1 || ms == nullptr || launches == nullptr || loops == nullptr) {
+    set_error("loop timing: pass 0 or 1 and three outputs are required");
+    return kInvalidArgument;
+  }
+  double total = 0.0;
+  int64_t n = 0;
+  auto &recs = g_loop_times.recs[pass];
+  int rc = kOk;
+  for (auto &r : recs) {
+    float e = 0.0f;
+    if (rc == kOk && hipEventSynchronize(r.stop) == hipSuccess &&
+        hipEventElapsedTime(&e, r.start, r.stop) == hipSuccess) {
+      total += e;
+      n += r.launches;
+    } else if (rc == kOk) {
+      set_error("loop timing: event query failed");
+      rc = kHipError;
+    }
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  *ms = total;
+  *launches = n;
+  *loops = (int64_t)recs.size();
+  recs.clear();
+  return rc;
 }
 
 extern "C" int tide_tm2d_get_tuning(int *out4) {

@@ -239,6 +239,12 @@ def load() -> ctypes.CDLL:
     lib.tide_tm2d_set_tile_split.argtypes = [ctypes.c_int] * 2
     lib.tide_tm2d_get_tile_split.restype = ctypes.c_int
     lib.tide_tm2d_get_tile_split.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    lib.tide_tm2d_set_loop_timing.restype = ctypes.c_int
+    lib.tide_tm2d_set_loop_timing.argtypes = [ctypes.c_int]
+    lib.tide_tm2d_take_loop_timing.restype = ctypes.c_int
+    lib.tide_tm2d_take_loop_timing.argtypes = [
+        ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+        ctypes.POINTER(ctypes.c_int64)]
     lib.tide_tm2d_set_border_rows.restype = ctypes.c_int
     lib.tide_tm2d_set_border_rows.argtypes = [ctypes.c_int]
     lib.tide_tm2d_get_border_rows.restype = ctypes.c_int
@@ -350,6 +356,22 @@ def get_tm2d_border_rows() -> int:
     return int(load().tide_tm2d_get_border_rows())
 
 
+def set_tm2d_loop_timing(enable: bool) -> None:
+    """HIP-event timing of every TM2D time loop (see include/tide_hip.h)."""
+    check(load().tide_tm2d_set_loop_timing(int(bool(enable))))
+
+
+def take_tm2d_loop_timing(pass_: int) -> tuple[float, int, int]:
+    """(milliseconds, launches, loops) recorded for pass 0 (forward) or 1
+    (adjoint) since the last call."""
+    ms = ctypes.c_double()
+    launches = ctypes.c_int64()
+    loops = ctypes.c_int64()
+    check(load().tide_tm2d_take_loop_timing(int(pass_), ctypes.byref(ms), ctypes.byref(launches),
+                                            ctypes.byref(loops)))
+    return float(ms.value), int(launches.value), int(loops.value)
+
+
 def stream_handle(device: torch.device) -> int | None:
     """The HIP stream of torch's current stream on ``device``."""
     return torch.cuda.current_stream(device).cuda_stream
@@ -368,6 +390,7 @@ __all__ = [
     "ptr_array",
     "set_em3d_tuning",
     "set_tm2d_border_rows",
+    "set_tm2d_loop_timing",
     "set_tm2d_shot_loop",
     "set_tm2d_tile_split",
     "set_tm2d_time_block",
@@ -377,4 +400,5 @@ __all__ = [
     "load",
     "ptr",
     "stream_handle",
+    "take_tm2d_loop_timing",
 ]
