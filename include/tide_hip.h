@@ -254,7 +254,8 @@ int tide_tm2d_get_time_block(int *out2);
  */
 int tide_tm2d_set_tile_split(int fwd_split, int adj_split);
 int tide_tm2d_get_tile_split(int *out2);
-/* Tile height (1 or 2 rows) of the split border kernels. */
+/* Height of the two-step border tiles: 1 or 2 rows, or 0 (the default) for
+ * the interior tile height. */
 int tide_tm2d_set_border_rows(int rows);
 int tide_tm2d_get_border_rows(void);
 

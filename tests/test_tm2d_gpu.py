@@ -358,8 +358,8 @@ LAUNCH_SHAPES = [
     ((4, 4, 2, 1), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
     ((4, 4, 4, 4), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
     ((2, 2, 2, 4), (0, 0), (2, 2), (0, 1), "tm2d_config1_f32"),
-    # interior and border tiles as one kernel or three (border rows from the
-    # library default, 2; the test below sweeps both heights)
+    # interior and border tiles as one kernel or three (border tiles of the
+    # interior height, the library default; the test below sweeps the heights)
     ((4, 2, 4, 4), (0, 0), (2, 2), (1, 0), "tm2d_hetero_f32"),
     ((8, 2, 2, 4), (0, 0), (2, 2), (1, 1), "tm2d_hetero_f32"),
     ((2, 2, 2, 2), (0, 0), (2, 2), (1, 0), "tm2d_config1_f32"),
@@ -410,14 +410,18 @@ def test_every_launch_shape_matches_oracle(tuning, loop, steps, split, name):
         assert relative_l2(gg[1], go[1]) < 1e-12
 
 
-@pytest.mark.parametrize("border_rows", [1, 2])
-def test_border_tile_heights_match_oracle(border_rows):
-    """The split border kernels give the same bits with 1- or 2-row tiles."""
+@pytest.mark.parametrize("split", [(1, 1), (0, 0)])
+@pytest.mark.parametrize("border_rows", [0, 1, 2])
+def test_border_tile_heights_match_oracle(border_rows, split):
+    """Border tiles of 1, 2 or 4 (border_rows 0: the interior height) rows
+    beside 4-row interior tiles, in their own kernels or in one kernel with
+    the interior, give the oracle's bits."""
     case, _ = load_golden("tm2d_hetero_f32")
     before = tide._native.get_tm2d_border_rows()
     tide._native.set_tm2d_border_rows(border_rows)
+    print(f"border rows {border_rows} split {split}", flush=True)
     try:
-        with launch_shape((4, 2, 4, 4), (0, 0), (2, 2), (1, 1)):
+        with launch_shape((4, 2, 4, 4), (0, 0), (2, 2), split):
             f_o, r_o, es, cs = O.tm2d_forward(case, store="full")
             f_g, r_g, es_g, cs_g = gpu_forward(case, store="full")
             assert np.array_equal(r_g, r_o)

@@ -104,16 +104,17 @@ void launch_fwd(int rows, bool loop, dim3 grid, dim3 block, hipStream_t stream,
   }
 }
 
-// Two-step launches: split (interior tiles of height TY; the border tiles
-// beside the interior rows and those above and below it, of height TYB; three
-// kernels with their own register budgets) or one kernel over all tiles of
-// height TY.
+// Two-step launches: interior tiles of height TY, border tiles of height TYB,
+// either split (the interior, the border tiles beside the interior rows and
+// those above and below it: three kernels with their own register budgets) or
+// in one kernel that takes the border tiles first.
 template <typename T, int TY, int TYB, int FMT>
 void launch_fwd2_parts(int64_t groups, dim3 block, hipStream_t stream, bool split,
                        const Fwd2Args<T> &a) {
   if (!split) {
-    const int64_t all = (int64_t)a.pts.n_xt * a.n_yt;
-    fwd2_step<T, TY, FMT, kOrderedTiles><<<(unsigned)(all * groups), block, 0, stream>>>(a);
+    const int64_t tiles = split_tiles<TY>(a.split, a.pts.n_xt, a.g.ny, kInnerTiles) +
+                          split_tiles<TYB>(a.split, a.pts.n_xt, a.g.ny, kBorderTiles);
+    fwd2_step<T, TY, FMT, kOrderedTiles, TYB><<<(unsigned)(tiles * groups), block, 0, stream>>>(a);
     return;
   }
   const int64_t inner = split_tiles<TY>(a.split, a.pts.n_xt, a.g.ny, kInnerTiles);
@@ -124,21 +125,24 @@ void launch_fwd2_parts(int64_t groups, dim3 block, hipStream_t stream, bool spli
   if (bands > 0) fwd2_step<T, TYB, FMT, kBandTiles><<<(unsigned)(bands * groups), block, 0, stream>>>(a);
 }
 
+// border_rows 0 gives the border tiles the interior height.
 template <typename T, int FMT>
 void launch_fwd2(int rows, int border_rows, int64_t groups, dim3 block, hipStream_t stream,
                  bool split, const Fwd2Args<T> &a) {
-  const bool b1 = border_rows == 1;
+  const int b = border_rows == 0 ? rows : border_rows;
   if (rows == 2) {
-    return b1 ? launch_fwd2_parts<T, 2, 1, FMT>(groups, block, stream, split, a)
-              : launch_fwd2_parts<T, 2, 2, FMT>(groups, block, stream, split, a);
+    return b == 1 ? launch_fwd2_parts<T, 2, 1, FMT>(groups, block, stream, split, a)
+                  : launch_fwd2_parts<T, 2, 2, FMT>(groups, block, stream, split, a);
   }
   if constexpr (sizeof(T) == 4) {
     if (rows == 8)
-      return b1 ? launch_fwd2_parts<T, 8, 1, FMT>(groups, block, stream, split, a)
-                : launch_fwd2_parts<T, 8, 2, FMT>(groups, block, stream, split, a);
+      return b == 1   ? launch_fwd2_parts<T, 8, 1, FMT>(groups, block, stream, split, a)
+             : b == 2 ? launch_fwd2_parts<T, 8, 2, FMT>(groups, block, stream, split, a)
+                      : launch_fwd2_parts<T, 8, 8, FMT>(groups, block, stream, split, a);
   }
-  b1 ? launch_fwd2_parts<T, 4, 1, FMT>(groups, block, stream, split, a)
-     : launch_fwd2_parts<T, 4, 2, FMT>(groups, block, stream, split, a);
+  b == 1   ? launch_fwd2_parts<T, 4, 1, FMT>(groups, block, stream, split, a)
+  : b == 2 ? launch_fwd2_parts<T, 4, 2, FMT>(groups, block, stream, split, a)
+           : launch_fwd2_parts<T, 4, 4, FMT>(groups, block, stream, split, a);
 }
 
 template <typename T, int TY, int TYB, int FMT>
@@ -147,8 +151,9 @@ void launch_adj2_parts(int64_t groups, dim3 block, hipStream_t stream, bool spli
   const size_t lds = 4 * (size_t)block.y * TY * 64 * sizeof(double);
   const size_t lds_b = 4 * (size_t)block.y * TYB * 64 * sizeof(double);
   if (!split) {
-    const int64_t all = (int64_t)a.pts.n_xt * a.n_yt;
-    adj2_step<T, TY, FMT, kOrderedTiles><<<(unsigned)(all * groups), block, lds, stream>>>(a);
+    const int64_t tiles = split_tiles<TY>(a.split, a.pts.n_xt, a.g.ny, kInnerTiles) +
+                          split_tiles<TYB>(a.split, a.pts.n_xt, a.g.ny, kBorderTiles);
+    adj2_step<T, TY, FMT, kOrderedTiles, TYB><<<(unsigned)(tiles * groups), block, lds, stream>>>(a);
     return;
   }
   const int64_t inner = split_tiles<TY>(a.split, a.pts.n_xt, a.g.ny, kInnerTiles);
@@ -164,12 +169,13 @@ void launch_adj2_parts(int64_t groups, dim3 block, hipStream_t stream, bool spli
 template <typename T, int FMT>
 void launch_adj2(int rows, int border_rows, int64_t groups, dim3 block, hipStream_t stream,
                  bool split, const Adj2Args<T> &a) {
-  const bool b1 = border_rows == 1;
+  const int b = border_rows == 0 ? rows : border_rows;
   if (rows == 2)
-    return b1 ? launch_adj2_parts<T, 2, 1, FMT>(groups, block, stream, split, a)
-              : launch_adj2_parts<T, 2, 2, FMT>(groups, block, stream, split, a);
-  b1 ? launch_adj2_parts<T, 4, 1, FMT>(groups, block, stream, split, a)
-     : launch_adj2_parts<T, 4, 2, FMT>(groups, block, stream, split, a);
+    return b == 1 ? launch_adj2_parts<T, 2, 1, FMT>(groups, block, stream, split, a)
+                  : launch_adj2_parts<T, 2, 2, FMT>(groups, block, stream, split, a);
+  b == 1   ? launch_adj2_parts<T, 4, 1, FMT>(groups, block, stream, split, a)
+  : b == 2 ? launch_adj2_parts<T, 4, 2, FMT>(groups, block, stream, split, a)
+           : launch_adj2_parts<T, 4, 4, FMT>(groups, block, stream, split, a);
 }
 
 template <typename T, int FMT>
@@ -936,8 +942,8 @@ extern "C" int tide_tm2d_set_tile_split(int fwd_split, int adj_split) {
 }
 
 extern "C" int tide_tm2d_set_border_rows(int rows) {
-  if (rows != 1 && rows != 2) {
-    set_error("border tile height must be 1 or 2");
+  if (rows < 0 || rows > 2) {
+    set_error("border tile height must be 0 (the interior height), 1 or 2");
     return kInvalidArgument;
   }
   g_tuning.border_rows = rows;

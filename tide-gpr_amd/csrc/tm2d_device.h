@@ -672,7 +672,7 @@ struct Tuning {
   int fwd_loop = 0, adj_loop = 0;
   int fwd_steps = 2, adj_steps = 2;
   int fwd_split = 0, adj_split = 0;
-  int border_rows = 2;  // tile height of the split border kernels (1 or 2)
+  int border_rows = 0;  // border tile height: 1, 2 or 0 (the interior height)
 };
 extern Tuning g_tuning;
 

@@ -106,7 +106,8 @@ constexpr int set_edge() {
 // edge handling the tile needs.  kOrderedTiles hands out the border tiles
 // (their long per-wavefront latency) before the interior ones, so the
 // interior work overlaps them.
-template <int SET, int TY>
+// For kOrderedTiles the border tiles are TYB rows high (TYB divides TY).
+template <int SET, int TY, int TYB = TY>
 __device__ __forceinline__ TileCoord pick_tile(const TileSplit &sp, int n_xt, int n_yt, int ny,
                                                int n_groups, int *edge) {
   if constexpr (SET == kAllTiles) {
@@ -114,7 +115,7 @@ __device__ __forceinline__ TileCoord pick_tile(const TileSplit &sp, int n_xt, in
     *edge = in_split<TY>(sp, tc.xt, tc.yt) ? 0 : 3;
     return tc;
   } else if constexpr (SET == kOrderedTiles) {
-    const int64_t nb = split_tiles<TY>(sp, n_xt, ny, kBorderTiles);
+    const int64_t nb = split_tiles<TYB>(sp, n_xt, ny, kBorderTiles);
     const int64_t ni = split_tiles<TY>(sp, n_xt, ny, kInnerTiles);
     // blocks 0 .. nb*n_groups-1 (dispatched first, spread over the XCDs):
     // border tiles; the rest: the interior, XCD-grouped
@@ -137,7 +138,7 @@ __device__ __forceinline__ TileCoord pick_tile(const TileSplit &sp, int n_xt, in
       *edge = 0;
       return c;
     }
-    const int top = (sp.ya / TY) * n_xt, w = n_xt - iw, mid = ((sp.yb - sp.ya) / TY) * w;
+    const int top = (sp.ya / TYB) * n_xt, w = n_xt - iw, mid = ((sp.yb - sp.ya) / TYB) * w;
     int k = (int)i;
     if (k < top) {
       c.yt = k / n_xt;
@@ -145,13 +146,13 @@ __device__ __forceinline__ TileCoord pick_tile(const TileSplit &sp, int n_xt, in
       *edge = 3;
     } else if (k < top + mid) {
       k -= top;
-      c.yt = sp.ya / TY + k / w;
+      c.yt = sp.ya / TYB + k / w;
       const int col = k % w;
       c.xt = col < sp.xa ? col : sp.xb + (col - sp.xa);
       *edge = 1;
     } else {
       k -= top + mid;
-      c.yt = sp.yb / TY + k / n_xt;
+      c.yt = sp.yb / TYB + k / n_xt;
       c.xt = k % n_xt;
       *edge = 3;
     }
@@ -427,22 +428,28 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   }
 }
 
-// SET selects the tiles: kInnerTiles, kBorderTiles or kAllTiles.
-template <typename T, int TY, int FMT, int SET>
+// SET selects the tiles (kInnerTiles, kBorderTiles, kSideTiles, kBandTiles,
+// kAllTiles, kOrderedTiles); TYB is the border tile height of kOrderedTiles.
+template <typename T, int TY, int FMT, int SET, int TYB = TY>
 __global__ __launch_bounds__(512) void fwd2_step(Fwd2Args<T> a) {
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
   int edge = 3;
-  const TileCoord tc = pick_tile<SET, TY>(a.split, a.pts.n_xt, a.n_yt, a.g.ny, n_groups, &edge);
+  const TileCoord tc = pick_tile<SET, TY, TYB>(a.split, a.pts.n_xt, a.n_yt, a.g.ny, n_groups, &edge);
   const int s = tc.grp * G + (int)threadIdx.y;
   if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
-  if constexpr (SET == kAllTiles || SET == kOrderedTiles) {
+  if constexpr (SET == kAllTiles) {
+    if (edge == 0)
+      fwd2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt);
+    else
+      fwd2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt);
+  } else if constexpr (SET == kOrderedTiles) {
     if (edge == 0)
       fwd2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt);
     else if (edge == 1)
-      fwd2_tile<T, TY, FMT, 1>(a, s, tc.xt, tc.yt);
+      fwd2_tile<T, TYB, FMT, 1>(a, s, tc.xt, tc.yt);
     else
-      fwd2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt);
+      fwd2_tile<T, TYB, FMT, 3>(a, s, tc.xt, tc.yt);
   } else {
     fwd2_tile<T, TY, FMT, set_edge<SET>()>(a, s, tc.xt, tc.yt);
   }
@@ -752,7 +759,9 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
 // One two-step adjoint workgroup; `red` is the dynamic LDS block
 // [2 coefficients][2 iterations][G][TY][64].  Each iteration's shot sum is
 // added to the cell separately, in iteration order (as two single steps do).
-template <typename T, int TY, int FMT, int SET>
+// ALLOW: bit k set = edge variant k (0 interior, 1 x checks, 3 x and y
+// checks: bit 3) may occur; `edge` picks one at run time.
+template <typename T, int TY, int FMT, int ALLOW>
 __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileCoord &tc, int edge,
                                                double *red) {
   const int lane = threadIdx.x;
@@ -775,25 +784,33 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
       cb_b[r * 64 + lane] = 0.0;
     }
   }
+  // The group's accumulators of this wavefront's first reduction row (row
+  // w) are read now, so the final add after the barrier waits on no load.
+  // Only this workgroup writes them.
+  const int x = tc.xt * kOwn2 - 2 + lane;
+  const int y0 = tc.yt * TY;
+  const bool red_lane = lane >= 2 && lane <= 61 && x >= 1 && x < a.g.nx;
+  const bool step_a = a.ey_store0 != nullptr || a.curl_store0 != nullptr;
+  const bool step_b = a.ey_store1 != nullptr || a.curl_store1 != nullptr;
+  double pre[2] = {0.0, 0.0};
+  if (reduce && red_lane && w < TY && y0 + w >= 1 && y0 + w < a.g.ny) {
+    const int64_t j = (int64_t)(y0 + w) * a.g.nx + x;
+    if (a.gca && (a.ey_store0 || a.ey_store1)) pre[0] = (a.gca + tc.grp * a.group_stride)[j];
+    if (a.gcb && (a.curl_store0 || a.curl_store1)) pre[1] = (a.gcb + tc.grp * a.group_stride)[j];
+  }
   if (s < a.n_shots) {  // inactive wavefronts still join the barrier below
-    if constexpr (SET == kAllTiles || SET == kOrderedTiles) {
-      if (edge == 0)
-        adj2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
-      else if (edge == 1)
-        adj2_tile<T, TY, FMT, 1>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
-      else
-        adj2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+    constexpr bool a0 = ALLOW & 1, a1 = ALLOW & 2, a3 = ALLOW & 8;
+    if (a0 && (!(a1 || a3) || edge == 0)) {
+      if constexpr (a0) adj2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+    } else if (a1 && (!a3 || edge == 1)) {
+      if constexpr (a1) adj2_tile<T, TY, FMT, 1>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
     } else {
-      adj2_tile<T, TY, FMT, set_edge<SET>()>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+      if constexpr (a3) adj2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
     }
   }
   if (!reduce) return;  // uniform over the workgroup
   __syncthreads();
-  const int x = tc.xt * kOwn2 - 2 + lane;
-  const int y0 = tc.yt * TY;
-  if (lane < 2 || lane > 61 || x < 1 || x >= a.g.nx) return;
-  const bool step_a = a.ey_store0 != nullptr || a.curl_store0 != nullptr;
-  const bool step_b = a.ey_store1 != nullptr || a.curl_store1 != nullptr;
+  if (!red_lane) return;
   for (int pass = 0; pass < 2; ++pass) {
     double *out = pass == 0 ? a.gca : a.gcb;
     const void *sa = pass == 0 ? a.ey_store0 : a.curl_store0;
@@ -806,7 +823,7 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
       const int y = y0 + r;
       if (y >= 1 && y < a.g.ny) {
         const int64_t j = (int64_t)y * a.g.nx + x;
-        double v = out[j];
+        double v = r == w ? pre[pass] : out[j];
         if (step_a && sa) {
           double sum = 0.0;
           for (int k = 0; k < G; ++k) sum += slots_a[(k * TY + r) * 64 + lane];
@@ -823,14 +840,26 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
   }
 }
 
-template <typename T, int TY, int FMT, int SET>
+template <typename T, int TY, int FMT, int SET, int TYB = TY>
 __global__ __launch_bounds__(512) void adj2_step(Adj2Args<T> a) {
   extern __shared__ double red[];
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
   int edge = 3;
-  const TileCoord tc = pick_tile<SET, TY>(a.split, a.pts.n_xt, a.n_yt, a.g.ny, n_groups, &edge);
-  adj2_workgroup<T, TY, FMT, SET>(a, tc, edge, red);
+  const TileCoord tc = pick_tile<SET, TY, TYB>(a.split, a.pts.n_xt, a.n_yt, a.g.ny, n_groups, &edge);
+  if constexpr (SET == kAllTiles) {
+    adj2_workgroup<T, TY, FMT, 1 | 8>(a, tc, edge, red);
+  } else if constexpr (SET == kOrderedTiles && TYB == TY) {
+    adj2_workgroup<T, TY, FMT, 1 | 2 | 8>(a, tc, edge, red);
+  } else if constexpr (SET == kOrderedTiles) {
+    if (edge == 0)  // uniform over the workgroup
+      adj2_workgroup<T, TY, FMT, 1>(a, tc, edge, red);
+    else
+      adj2_workgroup<T, TYB, FMT, 2 | 8>(a, tc, edge, red);
+  } else {
+    constexpr int e = set_edge<SET>();
+    adj2_workgroup<T, TY, FMT, e == 0 ? 1 : (e == 1 ? 2 : 8)>(a, tc, edge, red);
+  }
 }
 
 // Host side of the split: the interior rectangle for TY-row tiles (rows

@@ -348,7 +348,7 @@ def get_tm2d_tile_split() -> tuple[bool, bool]:
 
 
 def set_tm2d_border_rows(rows: int) -> None:
-    """Tile height of the split border kernels (1 or 2)."""
+    """Height of the two-step border tiles: 1, 2 or 0 (the interior height)."""
     check(load().tide_tm2d_set_border_rows(int(rows)))
 
 
