@@ -5,6 +5,8 @@
 
 #include "tide_common.h"
 
+#include <type_traits>
+
 namespace tide_hip {
 namespace tm2d {
 
@@ -92,6 +94,47 @@ __device__ __forceinline__ double get_store(const void *base, int64_t i) {
   }
 }
 
+// Element i of an array through a base pointer that is the same in every
+// lane of the wavefront (a shot's plane, say).  The byte offset is formed in
+// 32 bits, so the access compiles to the scalar-base plus 32-bit
+// vector-offset address form instead of 64-bit address arithmetic per access.
+// A plane holds fewer than 2^29 cells, so its byte offsets fit (fp64
+// included).
+template <typename T>
+__device__ __forceinline__ T *elem32(T *p, uint32_t i) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T *>(reinterpret_cast<B *>(p) + i * (uint32_t)sizeof(T));
+}
+
+// Shot plane of a snapshot buffer (payload T, or BF16 for FMT 2) starting at
+// element `base`; null stays null.
+template <typename T, int FMT, typename V>
+__device__ __forceinline__ V *snap_plane(V *p, int64_t base) {
+  using B = typename std::conditional<std::is_const<V>::value, const char, char>::type;
+  constexpr int64_t bytes = FMT == 2 ? 2 : (int64_t)sizeof(T);
+  return p == nullptr ? nullptr : reinterpret_cast<V *>(reinterpret_cast<B *>(p) + base * bytes);
+}
+
+// put_store / get_store through a snapshot plane (snap_plane) and a 32-bit
+// in-plane index.
+template <typename T, int FMT>
+__device__ __forceinline__ void put_store32(void *plane, uint32_t j, T v) {
+  if constexpr (FMT == 2) {
+    __builtin_nontemporal_store(f32_to_bf16((float)v), elem32((uint16_t *)plane, j));
+  } else {
+    __builtin_nontemporal_store(v, elem32((T *)plane, j));
+  }
+}
+
+template <typename T, int FMT>
+__device__ __forceinline__ double get_store32(const void *plane, uint32_t j) {
+  if constexpr (FMT == 2) {
+    return (double)bf16_to_f32(__builtin_nontemporal_load(elem32((const uint16_t *)plane, j)));
+  } else {
+    return (double)__builtin_nontemporal_load(elem32((const T *)plane, j));
+  }
+}
+
 struct TileCoord {
   int xt, yt, grp;
 };
@@ -120,10 +163,10 @@ template <typename T, bool EDGE> struct RowLoader {
   __device__ __forceinline__ T operator()(const T *__restrict__ p, int y) const {
     if constexpr (EDGE) {
       if (y < 0 || y >= ny) return (T)0;  // uniform
-      const T v = p[y * nx + xc];
+      const T v = *elem32(p, (uint32_t)(y * nx + xc));
       return xv ? v : (T)0;
     } else {
-      return p[y * nx + x];
+      return *elem32(p, (uint32_t)(y * nx + x));
     }
   }
 };

@@ -179,13 +179,24 @@ template <typename T> struct Fwd2Args {
   TileSplit split;
 };
 
-// Applies the point entries of (row y, shot s, tile xt) to `ey`: sources of
-// this lane (owner entries and halo copies) add f[q]; receivers (owner
-// entries only, and only when `record`) store ey to r[q].
+// Point-table slots of the N rows y_first .. y_first+N-1 (-1: no points, or
+// a row outside the grid), read at the start of a tile so that these scalar
+// loads overlap the row loads instead of stalling each row's update.
+template <int N, bool GY>
+__device__ __forceinline__ void tile_slots(const Points &pts, int y_first, int ny, int (&slot)[N]) {
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int y = y_first + k;
+    slot[k] = (GY && (y < 0 || y >= ny)) ? -1 : pts.row_slot[y];
+  }
+}
+
+// Applies the point entries of (row slot `slot`, shot s, tile xt) to `ey`:
+// sources of this lane (owner entries and halo copies) add f[q]; receivers
+// (owner entries only, and only when `record`) store ey to r[q].
 template <typename T>
-__device__ __forceinline__ void fused_points(const Points &pts, int n_shots, int s, int xt, int y,
+__device__ __forceinline__ void fused_points(const Points &pts, int slot, int n_shots, int s, int xt,
                                              int lane, const T *f, T *r, bool record, T &ey) {
-  const int slot = pts.row_slot[y];
   if (slot < 0) return;
   const int64_t si = ((int64_t)slot * n_shots + s) * pts.n_xt + xt;
   const int e1 = pts.seg[si + 1];
@@ -234,6 +245,12 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   const T *__restrict__ cb = a.cb + (g.cb_b ? base : 0);
   const T *__restrict__ cq = a.cq + (g.cq_b ? base : 0);
   const RowLoader<T, GEN> L(x, nx, ny);
+  void *const es0 = snap_plane<T, FMT>(a.ey_store0, base);
+  void *const cs0 = snap_plane<T, FMT>(a.curl_store0, base);
+  void *const es1 = snap_plane<T, FMT>(a.ey_store1, base);
+  void *const cs1 = snap_plane<T, FMT>(a.curl_store1, base);
+  int slot[TY + 2];  // rows y0-1 .. y0+TY
+  tile_slots<TY + 2, GY>(a.pts, y0 - 1, ny, slot);
 
   // Rows: Ey y0-2..y0+TY+1, Hx and cq y0-2..y0+TY, Hz, ca and cb y0-1..y0+TY.
   constexpr int NE = TY + 4, NX = TY + 3, NZ = TY + 2;
@@ -335,15 +352,14 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
       const T curl = dhz - dhx;
       if constexpr (FMT != 0) {
         if (own && k >= 1 && k <= TY) {
-          const int64_t i = base + (int64_t)y * nx + x;
-          if (a.ey_store0) put_store<T, FMT>(a.ey_store0, i, ey_c);
-          if (a.curl_store0) put_store<T, FMT>(a.curl_store0, i, curl);
+          const uint32_t i = (uint32_t)(y * nx + x);
+          if (es0) put_store32<T, FMT>(es0, i, ey_c);
+          if (cs0) put_store32<T, FMT>(cs0, i, curl);
         }
       }
       ey_n = cav[k] * ey_c + cbv[k] * curl;
     }
-    if (!GY || (y >= 0 && y < ny))
-      fused_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.f0, a.r0, own && k >= 1 && k <= TY, ey_n);
+    fused_points<T>(a.pts, slot[k], a.n_shots, s, xt, lane, a.f0, a.r0, own && k >= 1 && k <= TY, ey_n);
     ey1[k] = ey_n;
   }
 
@@ -404,25 +420,25 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
       const T curl = dhz - dhx;
       if constexpr (FMT != 0) {
         if (own) {
-          const int64_t i = base + j;
-          if (a.ey_store1) put_store<T, FMT>(a.ey_store1, i, ey_c);
-          if (a.curl_store1) put_store<T, FMT>(a.curl_store1, i, curl);
+          if (es1) put_store32<T, FMT>(es1, (uint32_t)j, ey_c);
+          if (cs1) put_store32<T, FMT>(cs1, (uint32_t)j, curl);
         }
       }
       ey_n = cav[k + 1] * ey_c + cbv[k + 1] * curl;
     }
-    fused_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.f1, a.r1, own, ey_n);
+    fused_points<T>(a.pts, slot[k + 1], a.n_shots, s, xt, lane, a.f1, a.r1, own, ey_n);
     if (own) {
-      ey_o[j] = ey_n;
-      hx_o[j] = hx2[k + 1];
-      hz_o[j] = hz_n;
+      const uint32_t o = (uint32_t)j;
+      *elem32(ey_o, o) = ey_n;
+      *elem32(hx_o, o) = hx2[k + 1];
+      *elem32(hz_o, o) = hz_n;
       if constexpr (GY) {
-        if (y < g.pml_y0 || y >= g.pml_y1h) a.mez_o[base + j] = mez[k + 2];
-        if (y < g.pml_y0 || y >= g.pml_y1) a.mhxz_o[base + j] = mhxz[k + 1];
+        if (y < g.pml_y0 || y >= g.pml_y1h) *elem32(a.mez_o + base, o) = mez[k + 2];
+        if (y < g.pml_y0 || y >= g.pml_y1) *elem32(a.mhxz_o + base, o) = mhxz[k + 1];
       }
       if constexpr (GX) {
-        if (pml_xh) a.mex_o[base + j] = mex[k + 1];
-        if (pml_xe) a.mhzx_o[base + j] = mhzx[k + 1];
+        if (pml_xh) *elem32(a.mex_o + base, o) = mex[k + 1];
+        if (pml_xe) *elem32(a.mhzx_o + base, o) = mhzx[k + 1];
       }
     }
   }
@@ -436,7 +452,9 @@ __global__ __launch_bounds__(512) void fwd2_step(Fwd2Args<T> a) {
   const int n_groups = (a.n_shots + G - 1) / G;
   int edge = 3;
   const TileCoord tc = pick_tile<SET, TY, TYB>(a.split, a.pts.n_xt, a.n_yt, a.g.ny, n_groups, &edge);
-  const int s = tc.grp * G + (int)threadIdx.y;
+  // threadIdx.y is one value per wavefront (64-wide rows): made explicit so
+  // the shot's plane pointers stay scalar.
+  const int s = tc.grp * G + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
   if constexpr (SET == kAllTiles) {
     if (edge == 0)
@@ -478,13 +496,12 @@ template <typename T> struct Adj2Args {
   TileSplit split;
 };
 
-// Applies the point entries of (row y, shot s, tile xt) to `ley`: receivers
+// Applies the point entries of (row slot `slot`, shot s, tile xt) to `ley`: receivers
 // of this lane (owner entries and halo copies) inject grad_r[q]; sources
 // (owner entries only, and only when `record`) read grad_f[q] = ley.
 template <typename T>
-__device__ __forceinline__ void fused_adj_points(const Points &pts, int n_shots, int s, int xt, int y,
+__device__ __forceinline__ void fused_adj_points(const Points &pts, int slot, int n_shots, int s, int xt,
                                                  int lane, const T *gr, T *gf, bool record, T &ley) {
-  const int slot = pts.row_slot[y];
   if (slot < 0) return;
   const int64_t si = ((int64_t)slot * n_shots + s) * pts.n_xt + xt;
   const int e0 = pts.seg[si], e1 = pts.seg[si + 1];
@@ -534,6 +551,12 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
   const T *__restrict__ cb = a.cb + (g.cb_b ? base : 0);
   const T *__restrict__ cq = a.cq + (g.cq_b ? base : 0);
   const RowLoader<T, GEN> L(x, nx, ny);
+  const void *const es0 = snap_plane<T, FMT>(a.ey_store0, base);
+  const void *const cs0 = snap_plane<T, FMT>(a.curl_store0, base);
+  const void *const es1 = snap_plane<T, FMT>(a.ey_store1, base);
+  const void *const cs1 = snap_plane<T, FMT>(a.curl_store1, base);
+  int slot[TY + 2];  // rows y0-1 .. y0+TY
+  tile_slots<TY + 2, GY>(a.pts, y0 - 1, ny, slot);
 
   // Rows: lambda_Ey and cb y0-2..y0+TY+1, lambda_Hx and cq y0-2..y0+TY,
   // lambda_Hz and ca y0-1..y0+TY.
@@ -560,18 +583,29 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
     }
   }
   // Snapshots of the owned rows (exact in fp32 for fp32 / BF16 payloads).
+  // They are loaded without per-lane or per-buffer branches (an absent
+  // buffer reads this shot's lambda_Ey plane, a lane outside the grid its
+  // clamped column; those values are replaced by 0) so that the loads issue
+  // with the row loads above and complete under one wait.
   float se0[TY], sc0[TY], se1[TY], sc1[TY];
 #pragma unroll
   for (int k = 0; k < TY; ++k) {
     se0[k] = sc0[k] = se1[k] = sc1[k] = 0.0f;
     if constexpr (sizeof(T) == 4 && FMT != 0) {
       const int y = y0 + k;
-      if (own && (!GY || (y >= 1 && y < ny)) && (!GX || x >= 1)) {
-        const int64_t i = base + (int64_t)y * nx + x;
-        if (grad_a && a.ey_store0) se0[k] = (float)get_store<T, FMT>(a.ey_store0, i);
-        if (grad_a && a.curl_store0) sc0[k] = (float)get_store<T, FMT>(a.curl_store0, i);
-        if (grad_b && a.ey_store1) se1[k] = (float)get_store<T, FMT>(a.ey_store1, i);
-        if (grad_b && a.curl_store1) sc1[k] = (float)get_store<T, FMT>(a.curl_store1, i);
+      if (!GY || y < ny) {  // uniform
+        const void *const dflt = a.ley_i + base;
+        const int xc = GX ? min(max(x, 0), nx - 1) : x;
+        const uint32_t i = (uint32_t)(y * nx + xc);
+        const float v0 = (float)get_store32<T, FMT>(es0 ? es0 : dflt, i);
+        const float v1 = (float)get_store32<T, FMT>(cs0 ? cs0 : dflt, i);
+        const float v2 = (float)get_store32<T, FMT>(es1 ? es1 : dflt, i);
+        const float v3 = (float)get_store32<T, FMT>(cs1 ? cs1 : dflt, i);
+        const bool in = !GX || (x >= 1 && x < nx);
+        se0[k] = (grad_a && es0 && in) ? v0 : 0.0f;
+        sc0[k] = (grad_a && cs0 && in) ? v1 : 0.0f;
+        se1[k] = (grad_b && es1 && in) ? v2 : 0.0f;
+        sc1[k] = (grad_b && cs1 && in) ? v3 : 0.0f;
       }
     }
   }
@@ -647,9 +681,9 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
       ve = (double)se;
       vc = (double)sc;
     } else {
-      const int64_t i = base + (int64_t)y * nx + x;
-      ve = es ? get_store<T, FMT>(es, i) : 0.0;
-      vc = cs ? get_store<T, FMT>(cs, i) : 0.0;
+      const uint32_t i = (uint32_t)(y * nx + x);
+      ve = es ? get_store32<T, FMT>(es, i) : 0.0;
+      vc = cs ? get_store32<T, FMT>(cs, i) : 0.0;
     }
     const double pa = lm * ve * a.step_scale;
     const double pb = lm * vc * a.step_scale;
@@ -657,9 +691,10 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
       ra[k * 64 + lane] = pa;
       rb[k * 64 + lane] = pb;
     } else {
-      const int64_t gi = (a.per_shot_grad ? base : 0) + (int64_t)y * nx + x;
-      if (a.gca && es) a.gca[gi] += pa;
-      if (a.gcb && cs) a.gcb[gi] += pb;
+      const int64_t gb = a.per_shot_grad ? base : 0;
+      const uint32_t gi = (uint32_t)(y * nx + x);
+      if (a.gca && es) *elem32(a.gca + gb, gi) += pa;
+      if (a.gcb && cs) *elem32(a.gcb + gb, gi) += pb;
     }
   };
 
@@ -693,11 +728,9 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
       ley_n = t1 + t2;
     }
     const bool owned_row = k >= 1 && k <= TY;
-    if (!GY || (y >= 0 && y < ny))
-      fused_adj_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.grad_r0, a.grad_f0, own && owned_row, ley_n);
+    fused_adj_points<T>(a.pts, slot[k], a.n_shots, s, xt, lane, a.grad_r0, a.grad_f0, own && owned_row, ley_n);
     if (owned_row)
-      grad_terms(grad_a, y, ley_n, se0[k - 1], sc0[k - 1], a.ey_store0, a.curl_store0, red_ca_a,
-                 red_cb_a, k - 1);
+      grad_terms(grad_a, y, ley_n, se0[k - 1], sc0[k - 1], es0, cs0, red_ca_a, red_cb_a, k - 1);
     leyA[k] = ley_n;
   }
 
@@ -734,21 +767,22 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
       const T t2 = (rzB[k] - rzB[k + 1]) * g.rdy + (rxh_l - rxh) * g.rdx;
       ley_n = t1 + t2;
     }
-    fused_adj_points<T>(a.pts, a.n_shots, s, xt, y, lane, a.grad_r1, a.grad_f1, own, ley_n);
-    grad_terms(grad_b, y, ley_n, se1[k], sc1[k], a.ey_store1, a.curl_store1, red_ca_b, red_cb_b, k);
+    fused_adj_points<T>(a.pts, slot[k + 1], a.n_shots, s, xt, lane, a.grad_r1, a.grad_f1, own, ley_n);
+    grad_terms(grad_b, y, ley_n, se1[k], sc1[k], es1, cs1, red_ca_b, red_cb_b, k);
     if (own) {
-      ley_o[j] = ley_n;
-      lhx_o[j] = lhxB[k + 1];
-      lhz_o[j] = lhz_n;
+      const uint32_t o = (uint32_t)j;
+      *elem32(ley_o, o) = ley_n;
+      *elem32(lhx_o, o) = lhxB[k + 1];
+      *elem32(lhz_o, o) = lhz_n;
       if constexpr (GEN) {
         if (in_e(y)) {
           if constexpr (GX) {
-            if (pml_xe) a.mx_o[base + j] = mx[k + 1];
-            if (x < nx - 1 && pml_xh) a.nx_o[base + j] = nxm[k + 1];
+            if (pml_xe) *elem32(a.mx_o + base, o) = mx[k + 1];
+            if (x < nx - 1 && pml_xh) *elem32(a.nx_o + base, o) = nxm[k + 1];
           }
           if constexpr (GY) {
-            if (y < g.pml_y0 || y >= g.pml_y1) a.my_o[base + j] = my[k + 2];
-            if (y < ny - 1 && (y < g.pml_y0 || y >= g.pml_y1h)) a.nz_o[base + j] = nz[k + 2];
+            if (y < g.pml_y0 || y >= g.pml_y1) *elem32(a.my_o + base, o) = my[k + 2];
+            if (y < ny - 1 && (y < g.pml_y0 || y >= g.pml_y1h)) *elem32(a.nz_o + base, o) = nz[k + 2];
           }
         }
       }
@@ -765,7 +799,7 @@ template <typename T, int TY, int FMT, int ALLOW>
 __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileCoord &tc, int edge,
                                                double *red) {
   const int lane = threadIdx.x;
-  const int w = (int)threadIdx.y;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one per wavefront
   const int G = (int)blockDim.y;
   const int s = tc.grp * G + w;
   const bool any_grad = FMT != 0 && (a.gca != nullptr || a.gcb != nullptr);
@@ -794,9 +828,9 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
   const bool step_b = a.ey_store1 != nullptr || a.curl_store1 != nullptr;
   double pre[2] = {0.0, 0.0};
   if (reduce && red_lane && w < TY && y0 + w >= 1 && y0 + w < a.g.ny) {
-    const int64_t j = (int64_t)(y0 + w) * a.g.nx + x;
-    if (a.gca && (a.ey_store0 || a.ey_store1)) pre[0] = (a.gca + tc.grp * a.group_stride)[j];
-    if (a.gcb && (a.curl_store0 || a.curl_store1)) pre[1] = (a.gcb + tc.grp * a.group_stride)[j];
+    const uint32_t j = (uint32_t)((y0 + w) * a.g.nx + x);
+    if (a.gca && (a.ey_store0 || a.ey_store1)) pre[0] = *elem32(a.gca + tc.grp * a.group_stride, j);
+    if (a.gcb && (a.curl_store0 || a.curl_store1)) pre[1] = *elem32(a.gcb + tc.grp * a.group_stride, j);
   }
   if (s < a.n_shots) {  // inactive wavefronts still join the barrier below
     constexpr bool a0 = ALLOW & 1, a1 = ALLOW & 2, a3 = ALLOW & 8;
@@ -822,8 +856,8 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
     for (int r = w; r < TY; r += G) {
       const int y = y0 + r;
       if (y >= 1 && y < a.g.ny) {
-        const int64_t j = (int64_t)y * a.g.nx + x;
-        double v = r == w ? pre[pass] : out[j];
+        const uint32_t j = (uint32_t)(y * a.g.nx + x);
+        double v = r == w ? pre[pass] : *elem32(out, j);
         if (step_a && sa) {
           double sum = 0.0;
           for (int k = 0; k < G; ++k) sum += slots_a[(k * TY + r) * 64 + lane];
@@ -834,7 +868,7 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
           for (int k = 0; k < G; ++k) sum += slots_b[(k * TY + r) * 64 + lane];
           v += sum;
         }
-        out[j] = v;
+        *elem32(out, j) = v;
       }
     }
   }
