@@ -254,10 +254,10 @@ int tide_tm2d_get_time_block(int *out2);
  */
 int tide_tm2d_set_tile_split(int fwd_split, int adj_split);
 int tide_tm2d_get_tile_split(int *out2);
-/* Height of the two-step border tiles: 1 or 2 rows, or 0 (the default) for
- * the interior tile height. */
-int tide_tm2d_set_border_rows(int rows);
-int tide_tm2d_get_border_rows(void);
+/* Height of the two-step border tiles per pass: 1 or 2 rows, or 0 for the
+ * interior tile height (defaults: forward 0, adjoint 2). */
+int tide_tm2d_set_border_rows(int fwd_rows, int adj_rows);
+int tide_tm2d_get_border_rows(int *out2);
 
 /*
  * Kernel timing without a profiler: when enabled, every single-problem

@@ -47,3 +47,25 @@ def test_missing_library_raises(monkeypatch, tmp_path):
     with pytest.raises(_native.TideNativeError):
         _native.load()
     monkeypatch.setattr(_native, "_lib", None)
+
+
+def test_launch_shape_setters_round_trip_and_reject_bad_values():
+    """Host-only calls: the launch-shape settings round-trip, and values the
+    kernels do not support are refused with a message (state unchanged)."""
+    before = (_native.get_tm2d_time_block(), _native.get_tm2d_tile_split(),
+              _native.get_tm2d_border_rows())
+    try:
+        _native.set_tm2d_border_rows(1, 0)
+        assert _native.get_tm2d_border_rows() == (1, 0)
+        for bad in ((3, 0), (0, -1)):
+            with pytest.raises(_native.TideNativeError, match="border tile height"):
+                _native.set_tm2d_border_rows(*bad)
+        assert _native.get_tm2d_border_rows() == (1, 0)
+        _native.set_tm2d_time_block(2, 1)
+        assert _native.get_tm2d_time_block() == (2, 1)
+        _native.set_tm2d_tile_split(True, False)
+        assert _native.get_tm2d_tile_split() == (True, False)
+    finally:
+        _native.set_tm2d_time_block(*before[0])
+        _native.set_tm2d_tile_split(*before[1])
+        _native.set_tm2d_border_rows(*before[2])

@@ -358,8 +358,8 @@ LAUNCH_SHAPES = [
     ((4, 4, 2, 1), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
     ((4, 4, 4, 4), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
     ((2, 2, 2, 4), (0, 0), (2, 2), (0, 1), "tm2d_config1_f32"),
-    # interior and border tiles as one kernel or three (border tiles of the
-    # interior height, the library default; the test below sweeps the heights)
+    # interior and border tiles as one kernel or three (library default
+    # border tile heights; the test below sweeps them)
     ((4, 2, 4, 4), (0, 0), (2, 2), (1, 0), "tm2d_hetero_f32"),
     ((8, 2, 2, 4), (0, 0), (2, 2), (1, 1), "tm2d_hetero_f32"),
     ((2, 2, 2, 2), (0, 0), (2, 2), (1, 0), "tm2d_config1_f32"),
@@ -418,7 +418,7 @@ def test_border_tile_heights_match_oracle(border_rows, split):
     the interior, give the oracle's bits."""
     case, _ = load_golden("tm2d_hetero_f32")
     before = tide._native.get_tm2d_border_rows()
-    tide._native.set_tm2d_border_rows(border_rows)
+    tide._native.set_tm2d_border_rows(border_rows, border_rows)
     print(f"border rows {border_rows} split {split}", flush=True)
     try:
         with launch_shape((4, 2, 4, 4), (0, 0), (2, 2), split):
@@ -437,4 +437,4 @@ def test_border_tile_heights_match_oracle(border_rows, split):
             assert relative_l2(gg[0], go[0]) < 1e-12
             assert relative_l2(gg[1], go[1]) < 1e-12
     finally:
-        tide._native.set_tm2d_border_rows(before)
+        tide._native.set_tm2d_border_rows(*before)

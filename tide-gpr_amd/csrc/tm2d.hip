@@ -283,11 +283,11 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     a2.curl_store1 = store_at(t + 1, curl_store);
     const bool st = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
     if (!st) {
-      launch_fwd2<T, 0>(rows2, g_tuning.border_rows, groups2, block2, stream, split2, a2);
+      launch_fwd2<T, 0>(rows2, g_tuning.fwd_border_rows, groups2, block2, stream, split2, a2);
     } else if (fmt == 2) {
-      if constexpr (sizeof(T) == 4) launch_fwd2<T, 2>(rows2, g_tuning.border_rows, groups2, block2, stream, split2, a2);
+      if constexpr (sizeof(T) == 4) launch_fwd2<T, 2>(rows2, g_tuning.fwd_border_rows, groups2, block2, stream, split2, a2);
     } else {
-      launch_fwd2<T, 1>(rows2, g_tuning.border_rows, groups2, block2, stream, split2, a2);
+      launch_fwd2<T, 1>(rows2, g_tuning.fwd_border_rows, groups2, block2, stream, split2, a2);
     }
     for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
     for (int k = 0; k < 2; ++k) std::swap(cur_e[k], nxt_e[k]);
@@ -451,11 +451,11 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
       a2.curl_store1 = snap(t - 1, curl_store, gcb);
       const bool g_any = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
       if (!g_any) {
-        launch_adj2<T, 0>(rows2, g_tuning.border_rows, n_groups, block2, stream, split2, a2);
+        launch_adj2<T, 0>(rows2, g_tuning.adj_border_rows, n_groups, block2, stream, split2, a2);
       } else if (fmt == 2) {
-        if constexpr (sizeof(T) == 4) launch_adj2<T, 2>(rows2, g_tuning.border_rows, n_groups, block2, stream, split2, a2);
+        if constexpr (sizeof(T) == 4) launch_adj2<T, 2>(rows2, g_tuning.adj_border_rows, n_groups, block2, stream, split2, a2);
       } else {
-        launch_adj2<T, 1>(rows2, g_tuning.border_rows, n_groups, block2, stream, split2, a2);
+        launch_adj2<T, 1>(rows2, g_tuning.adj_border_rows, n_groups, block2, stream, split2, a2);
       }
       for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
     }
@@ -830,16 +830,25 @@ extern "C" int tide_tm2d_set_tile_split(int fwd_split, int adj_split) {
   return kOk;
 }
 
-extern "C" int tide_tm2d_set_border_rows(int rows) {
-  if (rows < 0 || rows > 2) {
+extern "C" int tide_tm2d_set_border_rows(int fwd_rows, int adj_rows) {
+  if (fwd_rows < 0 || fwd_rows > 2 || adj_rows < 0 || adj_rows > 2) {
     set_error("border tile height must be 0 (the interior height), 1 or 2");
     return kInvalidArgument;
   }
-  g_tuning.border_rows = rows;
+  g_tuning.fwd_border_rows = fwd_rows;
+  g_tuning.adj_border_rows = adj_rows;
   return kOk;
 }
 
-extern "C" int tide_tm2d_get_border_rows(void) { return g_tuning.border_rows; }
+extern "C" int tide_tm2d_get_border_rows(int *out2) {
+  if (out2 == nullptr) {
+    set_error("null output");
+    return kInvalidArgument;
+  }
+  out2[0] = g_tuning.fwd_border_rows;
+  out2[1] = g_tuning.adj_border_rows;
+  return kOk;
+}
 
 extern "C" int tide_tm2d_get_tile_split(int *out2) {
   if (out2 == nullptr) {

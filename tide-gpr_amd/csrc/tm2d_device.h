@@ -150,6 +150,18 @@ __device__ __forceinline__ TileCoord decode_tile(int n_xt, int n_yt, int n_group
   return c;
 }
 
+// v where `keep`, else +0, without a select: a select with a loaded operand
+// may be turned into a branch around the load, and the wavefront then waits
+// for that load on the spot instead of with the other loads in flight.  The
+// empty asm hides where the mask comes from, so the AND stays an AND.
+template <typename T>
+__device__ __forceinline__ T keep_or_zero(T v, bool keep) {
+  using U = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
+  U m = keep ? ~U(0) : U(0);
+  asm("" : "+v"(m));
+  return __builtin_bit_cast(T, __builtin_bit_cast(U, v) & m);
+}
+
 // Row loader for one wavefront.  Interior tiles (EDGE false) need no bounds
 // checks: every row and column they touch exists.  Edge tiles read a
 // clamped column and zero it outside the grid.
@@ -162,9 +174,11 @@ template <typename T, bool EDGE> struct RowLoader {
   }
   __device__ __forceinline__ T operator()(const T *__restrict__ p, int y) const {
     if constexpr (EDGE) {
-      if (y < 0 || y >= ny) return (T)0;  // uniform
-      const T v = *elem32(p, (uint32_t)(y * nx + xc));
-      return xv ? v : (T)0;
+      // Branch-free (clamped row and column, then a select): a branch around
+      // the load would make the wavefront wait for it before the next load.
+      const int yc = min(max(y, 0), ny - 1);
+      const T v = *elem32(p, (uint32_t)(yc * nx + xc));
+      return keep_or_zero(v, xv && y == yc);
     } else {
       return *elem32(p, (uint32_t)(y * nx + x));
     }
@@ -715,7 +729,10 @@ struct Tuning {
   int fwd_loop = 0, adj_loop = 0;
   int fwd_steps = 2, adj_steps = 2;
   int fwd_split = 0, adj_split = 0;
-  int border_rows = 0;  // border tile height: 1, 2 or 0 (the interior height)
+  // Border tile height of the two-step kernels: 1, 2 or 0 (the interior
+  // height).  The forward is fastest with interior-height border tiles, the
+  // adjoint (more live state per border tile) with 2-row ones.
+  int fwd_border_rows = 0, adj_border_rows = 2;
 };
 extern Tuning g_tuning;
 

@@ -270,7 +270,10 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   }
 
   // CPML memories of step t (GEN only): m_ey_z at the Hx rows, m_ey_x at the
-  // Hz rows, m_hz_x and m_hx_z at the Ey rows of step t+1.
+  // Hz rows, m_hz_x and m_hx_z at the Ey rows of step t+1.  The memory
+  // arrays cover the whole grid, so every row is loaded (no per-lane or
+  // per-row branch around the loads, which would wait for all loads in
+  // flight) and the values outside the strips are replaced by 0.
   T mez[NX], mex[NZ], mhzx[NZ], mhxz[NZ];
   if constexpr (GY) {
     const T *__restrict__ mez_i = a.mez_i + base;
@@ -278,14 +281,14 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
       const int y = y0 - 2 + k;
-      const bool pml_yh = y < g.pml_y0 || y >= g.pml_y1h;
-      mez[k] = pml_yh ? L(mez_i, y) : (T)0;
+      const T v = L(mez_i, y);
+      mez[k] = keep_or_zero(v, y < g.pml_y0 || y >= g.pml_y1h);
     }
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
       const int y = y0 - 1 + k;
-      const bool pml_ye = y < g.pml_y0 || y >= g.pml_y1;
-      mhxz[k] = pml_ye ? L(mhxz_i, y) : (T)0;
+      const T v = L(mhxz_i, y);
+      mhxz[k] = keep_or_zero(v, y < g.pml_y0 || y >= g.pml_y1);
     }
   }
   if constexpr (GX) {
@@ -294,8 +297,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
       const int y = y0 - 1 + k;
-      mex[k] = pml_xh ? L(mex_i, y) : (T)0;
-      mhzx[k] = pml_xe ? L(mhzx_i, y) : (T)0;
+      const T v = L(mex_i, y), w = L(mhzx_i, y);
+      mex[k] = keep_or_zero(v, pml_xh);
+      mhzx[k] = keep_or_zero(w, pml_xe);
     }
   }
 
@@ -585,8 +589,8 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
   // Snapshots of the owned rows (exact in fp32 for fp32 / BF16 payloads).
   // They are loaded without per-lane or per-buffer branches (an absent
   // buffer reads this shot's lambda_Ey plane, a lane outside the grid its
-  // clamped column; those values are replaced by 0) so that the loads issue
-  // with the row loads above and complete under one wait.
+  // clamped column; keep_or_zero clears those values) so that the loads
+  // issue with the row loads above and complete under one wait.
   float se0[TY], sc0[TY], se1[TY], sc1[TY];
 #pragma unroll
   for (int k = 0; k < TY; ++k) {
@@ -602,15 +606,16 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
         const float v2 = (float)get_store32<T, FMT>(es1 ? es1 : dflt, i);
         const float v3 = (float)get_store32<T, FMT>(cs1 ? cs1 : dflt, i);
         const bool in = !GX || (x >= 1 && x < nx);
-        se0[k] = (grad_a && es0 && in) ? v0 : 0.0f;
-        sc0[k] = (grad_a && cs0 && in) ? v1 : 0.0f;
-        se1[k] = (grad_b && es1 && in) ? v2 : 0.0f;
-        sc1[k] = (grad_b && cs1 && in) ? v3 : 0.0f;
+        se0[k] = keep_or_zero(v0, grad_a && es0 && in);
+        sc0[k] = keep_or_zero(v1, grad_a && cs0 && in);
+        se1[k] = keep_or_zero(v2, grad_b && es1 && in);
+        sc1[k] = keep_or_zero(v3, grad_b && cs1 && in);
       }
     }
   }
   // CPML memories (GEN only): my and nz indexed from row y0-2, mx and nx
-  // from row y0-1.
+  // from row y0-1.  Loaded for every row and lane (the arrays cover the
+  // grid; no branch around the loads), zero outside the strips.
   T my[NE], nz[NX], mx[NZ], nxm[NZ];
   if constexpr (GY) {
     const T *__restrict__ my_i = a.my_i + base;
@@ -618,12 +623,14 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
       const int y = y0 - 2 + k;
-      my[k] = (y < g.pml_y0 || y >= g.pml_y1) ? L(my_i, y) : (T)0;
+      const T v = L(my_i, y);
+      my[k] = keep_or_zero(v, y < g.pml_y0 || y >= g.pml_y1);
     }
 #pragma unroll
     for (int k = 0; k < NX; ++k) {
       const int y = y0 - 2 + k;
-      nz[k] = (y < g.pml_y0 || y >= g.pml_y1h) ? L(nz_i, y) : (T)0;
+      const T v = L(nz_i, y);
+      nz[k] = keep_or_zero(v, y < g.pml_y0 || y >= g.pml_y1h);
     }
   }
   if constexpr (GX) {
@@ -632,8 +639,9 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
 #pragma unroll
     for (int k = 0; k < NZ; ++k) {
       const int y = y0 - 1 + k;
-      mx[k] = pml_xe ? L(mx_i, y) : (T)0;
-      nxm[k] = pml_xh ? L(nx_i, y) : (T)0;
+      const T v = L(mx_i, y), w = L(nx_i, y);
+      mx[k] = keep_or_zero(v, pml_xe);
+      nxm[k] = keep_or_zero(w, pml_xh);
     }
   }
 

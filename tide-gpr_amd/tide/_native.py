@@ -246,9 +246,9 @@ def load() -> ctypes.CDLL:
         ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
         ctypes.POINTER(ctypes.c_int64)]
     lib.tide_tm2d_set_border_rows.restype = ctypes.c_int
-    lib.tide_tm2d_set_border_rows.argtypes = [ctypes.c_int]
+    lib.tide_tm2d_set_border_rows.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.tide_tm2d_get_border_rows.restype = ctypes.c_int
-    lib.tide_tm2d_get_border_rows.argtypes = []
+    lib.tide_tm2d_get_border_rows.argtypes = [ctypes.POINTER(ctypes.c_int)]
     _lib = lib
     return lib
 
@@ -347,13 +347,15 @@ def get_tm2d_tile_split() -> tuple[bool, bool]:
     return bool(out[0]), bool(out[1])
 
 
-def set_tm2d_border_rows(rows: int) -> None:
-    """Height of the two-step border tiles: 1, 2 or 0 (the interior height)."""
-    check(load().tide_tm2d_set_border_rows(int(rows)))
+def set_tm2d_border_rows(fwd_rows: int, adj_rows: int) -> None:
+    """Height of the two-step border tiles per pass: 1, 2 or 0 (the interior height)."""
+    check(load().tide_tm2d_set_border_rows(int(fwd_rows), int(adj_rows)))
 
 
-def get_tm2d_border_rows() -> int:
-    return int(load().tide_tm2d_get_border_rows())
+def get_tm2d_border_rows() -> tuple[int, int]:
+    out = (ctypes.c_int * 2)()
+    check(load().tide_tm2d_get_border_rows(out))
+    return int(out[0]), int(out[1])
 
 
 def set_tm2d_loop_timing(enable: bool) -> None:
