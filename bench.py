@@ -397,7 +397,8 @@ def main() -> None:
             "hbm_frac_whole_job": round(gcells * 1e9 * (24 + b_snap) / 1e9 / HBM_PEAK_GBS, 4),
             "misfit": float(misfit),
             "launch_shape": {
-                "fields": ["rows", "group", "shot_loop", "steps_per_launch", "tile_split"],
+                "fields": ["rows", "group", "shot_loop", "steps_per_launch", "tile_split",
+                           "border_rows"],
                 "forward": list(tuning["forward"]),
                 "adjoint": list(tuning["adjoint"]),
             },

@@ -224,9 +224,10 @@ def test_autotune_size_classes():
     assert a != _size_class(300, 300, 7, torch.float32)
     assert a != _size_class(1065, 1065, 7, torch.float64)
     for cands, rows in ((FWD_CANDIDATES, (2, 4, 8, 16)), (ADJ_CANDIDATES, (2, 4, 8))):
-        for r, g, lp, steps, split in cands:
+        for r, g, lp, steps, split, border in cands:
             assert r in rows and 1 <= g <= (4 if lp else 8) and lp in (0, 1)
-            assert steps in (1, 2) and split in (0, 1)
+            assert steps in (1, 2) and split in (0, 1) and border in (0, 1, 2)
+            assert border == 0 or steps == 2  # only two-step launches have border tiles
             assert not (lp and steps == 2)  # two-step launches group shots per wavefront
             if steps == 2:
                 assert r <= 8

@@ -523,14 +523,13 @@ __device__ __forceinline__ void fused_adj_points(const Points &pts, int slot, in
   }
 }
 
-// Two reverse iterations of TY rows of one shot.  The per-shot FP64
-// gradient terms of iteration A / B go to red_a / red_b ([TY][64] LDS
-// slots each, per coefficient) when `reduce`, else straight to the
-// per-shot gradient arrays.
+// Two reverse iterations of TY rows of one shot.  When `reduce`, the
+// per-shot FP64 gradient terms of iteration A are stored to red_ca / red_cb
+// ([TY][64] LDS slots per coefficient) and those of iteration B added to
+// them; otherwise both go straight to the per-shot gradient arrays, A then B.
 template <typename T, int TY, int FMT, int EDGE>
 __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, int yt,
-                                          double *red_ca_a, double *red_cb_a, double *red_ca_b,
-                                          double *red_cb_b, bool reduce) {
+                                          double *red_ca, double *red_cb, bool reduce) {
   constexpr bool GX = (EDGE & 1) != 0, GY = (EDGE & 2) != 0, GEN = EDGE != 0;
   const Geom<T> &g = a.g;
   const int ny = g.ny, nx = g.nx;
@@ -681,7 +680,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
   };
   auto in_e = [&](int y) -> bool { return e_col && (!GY || (y >= 1 && y < ny)); };
   auto grad_terms = [&](bool on, int y, T lm_t, float se, float sc, const void *es, const void *cs,
-                        double *ra, double *rb, int k) {
+                        int k, bool add) {
     if (!on || !own || !in_e(y)) return;
     const double lm = (double)lm_t;
     double ve, vc;
@@ -696,8 +695,10 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
     const double pa = lm * ve * a.step_scale;
     const double pb = lm * vc * a.step_scale;
     if (reduce) {
-      ra[k * 64 + lane] = pa;
-      rb[k * 64 + lane] = pb;
+      double *const ra = red_ca + k * 64 + lane;
+      double *const rb = red_cb + k * 64 + lane;
+      *ra = add ? *ra + pa : pa;
+      *rb = add ? *rb + pb : pb;
     } else {
       const int64_t gb = a.per_shot_grad ? base : 0;
       const uint32_t gi = (uint32_t)(y * nx + x);
@@ -738,7 +739,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
     const bool owned_row = k >= 1 && k <= TY;
     fused_adj_points<T>(a.pts, slot[k], a.n_shots, s, xt, lane, a.grad_r0, a.grad_f0, own && owned_row, ley_n);
     if (owned_row)
-      grad_terms(grad_a, y, ley_n, se0[k - 1], sc0[k - 1], es0, cs0, red_ca_a, red_cb_a, k - 1);
+      grad_terms(grad_a, y, ley_n, se0[k - 1], sc0[k - 1], es0, cs0, k - 1, false);
     leyA[k] = ley_n;
   }
 
@@ -776,7 +777,7 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
       ley_n = t1 + t2;
     }
     fused_adj_points<T>(a.pts, slot[k + 1], a.n_shots, s, xt, lane, a.grad_r1, a.grad_f1, own, ley_n);
-    grad_terms(grad_b, y, ley_n, se1[k], sc1[k], es1, cs1, red_ca_b, red_cb_b, k);
+    grad_terms(grad_b, y, ley_n, se1[k], sc1[k], es1, cs1, k, true);
     if (own) {
       const uint32_t o = (uint32_t)j;
       *elem32(ley_o, o) = ley_n;
@@ -799,8 +800,10 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
 }
 
 // One two-step adjoint workgroup; `red` is the dynamic LDS block
-// [2 coefficients][2 iterations][G][TY][64].  Each iteration's shot sum is
-// added to the cell separately, in iteration order (as two single steps do).
+// [2 coefficients][G][TY][64].  Each slot holds one shot's terms of both
+// iterations; the shots' slots are summed in shot order and added to the
+// cell once per launch (FP64 throughout, so the result differs from two
+// single steps only by the rounding of the summation order).
 // ALLOW: bit k set = edge variant k (0 interior, 1 x checks, 3 x and y
 // checks: bit 3) may occur; `edge` picks one at run time.
 template <typename T, int TY, int FMT, int ALLOW>
@@ -813,17 +816,13 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
   const bool any_grad = FMT != 0 && (a.gca != nullptr || a.gcb != nullptr);
   const bool reduce = any_grad && !a.per_shot_grad;
   const int64_t slab = (int64_t)G * TY * 64;  // one [G][TY][64] block
-  double *ca_a = red + (int64_t)w * TY * 64;
-  double *ca_b = ca_a + slab;
-  double *cb_a = ca_a + 2 * slab;
-  double *cb_b = ca_a + 3 * slab;
+  double *red_ca = red + (int64_t)w * TY * 64;
+  double *red_cb = red_ca + slab;
   if (reduce) {
 #pragma unroll
     for (int r = 0; r < TY; ++r) {
-      ca_a[r * 64 + lane] = 0.0;
-      ca_b[r * 64 + lane] = 0.0;
-      cb_a[r * 64 + lane] = 0.0;
-      cb_b[r * 64 + lane] = 0.0;
+      red_ca[r * 64 + lane] = 0.0;
+      red_cb[r * 64 + lane] = 0.0;
     }
   }
   // The group's accumulators of this wavefront's first reduction row (row
@@ -843,11 +842,11 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
   if (s < a.n_shots) {  // inactive wavefronts still join the barrier below
     constexpr bool a0 = ALLOW & 1, a1 = ALLOW & 2, a3 = ALLOW & 8;
     if (a0 && (!(a1 || a3) || edge == 0)) {
-      if constexpr (a0) adj2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+      if constexpr (a0) adj2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt, red_ca, red_cb, reduce);
     } else if (a1 && (!a3 || edge == 1)) {
-      if constexpr (a1) adj2_tile<T, TY, FMT, 1>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+      if constexpr (a1) adj2_tile<T, TY, FMT, 1>(a, s, tc.xt, tc.yt, red_ca, red_cb, reduce);
     } else {
-      if constexpr (a3) adj2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt, ca_a, cb_a, ca_b, cb_b, reduce);
+      if constexpr (a3) adj2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt, red_ca, red_cb, reduce);
     }
   }
   if (!reduce) return;  // uniform over the workgroup
@@ -859,24 +858,15 @@ __device__ __forceinline__ void adj2_workgroup(const Adj2Args<T> &a, const TileC
     const void *sb = pass == 0 ? a.ey_store1 : a.curl_store1;
     if (out == nullptr || (!sa && !sb)) continue;
     out += tc.grp * a.group_stride;
-    const double *slots_a = red + (int64_t)(2 * pass) * slab;
-    const double *slots_b = slots_a + slab;
+    if (!((step_a && sa) || (step_b && sb))) continue;
+    const double *slots = red + (int64_t)pass * slab;
     for (int r = w; r < TY; r += G) {
       const int y = y0 + r;
       if (y >= 1 && y < a.g.ny) {
         const uint32_t j = (uint32_t)(y * a.g.nx + x);
-        double v = r == w ? pre[pass] : *elem32(out, j);
-        if (step_a && sa) {
-          double sum = 0.0;
-          for (int k = 0; k < G; ++k) sum += slots_a[(k * TY + r) * 64 + lane];
-          v += sum;
-        }
-        if (step_b && sb) {
-          double sum = 0.0;
-          for (int k = 0; k < G; ++k) sum += slots_b[(k * TY + r) * 64 + lane];
-          v += sum;
-        }
-        *elem32(out, j) = v;
+        double sum = 0.0;
+        for (int k = 0; k < G; ++k) sum += slots[(k * TY + r) * 64 + lane];
+        *elem32(out, j) = (r == w ? pre[pass] : *elem32(out, j)) + sum;
       }
     }
   }

@@ -90,8 +90,8 @@ void launch_fwd2(int rows, int border_rows, int64_t groups, dim3 block, hipStrea
 template <typename T, int TY, int TYB, int FMT>
 void launch_adj2_parts(int64_t groups, dim3 block, hipStream_t stream, bool split,
                        const Adj2Args<T> &a) {
-  const size_t lds = 4 * (size_t)block.y * TY * 64 * sizeof(double);
-  const size_t lds_b = 4 * (size_t)block.y * TYB * 64 * sizeof(double);
+  const size_t lds = 2 * (size_t)block.y * TY * 64 * sizeof(double);
+  const size_t lds_b = 2 * (size_t)block.y * TYB * 64 * sizeof(double);
   if (!split) {
     const int64_t tiles = split_tiles<TY>(a.split, a.pts.n_xt, a.g.ny, kInnerTiles) +
                           split_tiles<TYB>(a.split, a.pts.n_xt, a.g.ny, kBorderTiles);

@@ -11,7 +11,8 @@ launch parameters:
 * time steps per launch: 1, or 2 (two-step kernels that recompute a halo of
   two and touch the wavefield state once per two steps), the latter with
   the interior and border tiles as one kernel or two;
-* rows marched per wavefront (the tile height).
+* rows marched per wavefront (the tile height), and for two-step launches
+  the height of the border tiles (0: the interior height, or 1 or 2 rows).
 
 The best combination depends on the grid, the shot count and the pass (the
 adjoint carries more state per row).  :func:`tune_tm2d` times every
@@ -31,17 +32,20 @@ from . import _native
 from .maxwell.tm2d import prepare_tm2d
 from .utils import C0
 
-# (rows, group, shot loop, steps per launch, interior/border split)
-FWD_CANDIDATES: tuple[tuple[int, int, int, int, int], ...] = (
-    (2, 4, 0, 1, 0), (4, 2, 0, 1, 0), (4, 4, 0, 1, 0), (8, 4, 0, 1, 0), (16, 4, 0, 1, 0),
-    (2, 4, 1, 1, 0), (4, 2, 1, 1, 0),
-    (2, 2, 0, 2, 0), (4, 1, 0, 2, 0), (4, 2, 0, 2, 0), (4, 4, 0, 2, 0), (8, 2, 0, 2, 0),
-    (4, 1, 0, 2, 1), (4, 2, 0, 2, 1))
-ADJ_CANDIDATES: tuple[tuple[int, int, int, int, int], ...] = (
-    (2, 2, 0, 1, 0), (2, 4, 0, 1, 0), (2, 8, 0, 1, 0), (4, 4, 0, 1, 0), (8, 2, 0, 1, 0),
-    (2, 4, 1, 1, 0), (4, 2, 1, 1, 0),
-    (4, 4, 0, 2, 1), (4, 8, 0, 2, 1),
-    (2, 4, 0, 2, 0), (4, 2, 0, 2, 0), (4, 4, 0, 2, 0), (4, 8, 0, 2, 0))
+# (rows, group, shot loop, steps per launch, interior/border split,
+#  border tile rows)
+FWD_CANDIDATES: tuple[tuple[int, int, int, int, int, int], ...] = (
+    (2, 4, 0, 1, 0, 0), (4, 2, 0, 1, 0, 0), (4, 4, 0, 1, 0, 0), (8, 4, 0, 1, 0, 0),
+    (16, 4, 0, 1, 0, 0), (2, 4, 1, 1, 0, 0), (4, 2, 1, 1, 0, 0),
+    (2, 2, 0, 2, 0, 0), (4, 1, 0, 2, 0, 0), (4, 2, 0, 2, 0, 0), (4, 4, 0, 2, 0, 0),
+    (8, 2, 0, 2, 0, 0), (4, 1, 0, 2, 0, 2), (4, 2, 0, 2, 0, 2),
+    (4, 1, 0, 2, 1, 0), (4, 2, 0, 2, 1, 0))
+ADJ_CANDIDATES: tuple[tuple[int, int, int, int, int, int], ...] = (
+    (2, 2, 0, 1, 0, 0), (2, 4, 0, 1, 0, 0), (2, 8, 0, 1, 0, 0), (4, 4, 0, 1, 0, 0),
+    (8, 2, 0, 1, 0, 0), (2, 4, 1, 1, 0, 0), (4, 2, 1, 1, 0, 0),
+    (4, 4, 0, 2, 1, 0), (4, 8, 0, 2, 1, 0), (4, 4, 0, 2, 1, 2), (4, 8, 0, 2, 1, 2),
+    (2, 4, 0, 2, 0, 0), (4, 2, 0, 2, 0, 0), (4, 4, 0, 2, 0, 0), (4, 8, 0, 2, 0, 0),
+    (4, 4, 0, 2, 0, 2), (4, 8, 0, 2, 0, 2))
 
 _cache: dict[tuple, tuple[tuple[int, ...], tuple[int, ...]]] = {}
 
@@ -52,6 +56,7 @@ def _install(fwd: tuple[int, ...], adj: tuple[int, ...]) -> None:
     _native.set_tm2d_shot_loop(fwd[2], adj[2])
     _native.set_tm2d_time_block(fwd[3], adj[3])
     _native.set_tm2d_tile_split(fwd[4], adj[4])
+    _native.set_tm2d_border_rows(fwd[5], adj[5])
 
 
 def current() -> tuple[tuple[int, ...], tuple[int, ...]]:
@@ -60,7 +65,9 @@ def current() -> tuple[tuple[int, ...], tuple[int, ...]]:
     lp = _native.get_tm2d_shot_loop()
     st = _native.get_tm2d_time_block()
     sp = _native.get_tm2d_tile_split()
-    return ((t[0], t[1], int(lp[0]), st[0], int(sp[0])), (t[2], t[3], int(lp[1]), st[1], int(sp[1])))
+    br = _native.get_tm2d_border_rows()
+    return ((t[0], t[1], int(lp[0]), st[0], int(sp[0]), br[0]),
+            (t[2], t[3], int(lp[1]), st[1], int(sp[1]), br[1]))
 
 
 def _size_class(ny: int, nx: int, n_shots: int, dtype: torch.dtype) -> tuple:
@@ -112,8 +119,8 @@ def tune_tm2d(
     adj_candidates: Sequence[tuple[int, ...]] = ADJ_CANDIDATES,
     use_cache: bool = True,
 ) -> dict:
-    """Pick and install the fastest (rows, group, shot loop, steps, split)
-    for the forward-with-snapshots and adjoint kernels on a padded
+    """Pick and install the fastest (rows, group, shot loop, steps, split,
+    border rows) for the forward-with-snapshots and adjoint kernels on a padded
     ``ny x nx`` grid with ``n_shots`` shots per call.  Returns the choice
     (``forward`` and ``adjoint`` candidates) and the timings."""
     device = torch.device(device)
@@ -173,7 +180,8 @@ def tune_tm2d(
     _cache[key] = (best_f, best_a)
 
     def label(k):
-        return f"{k[0]}x{k[1]}{'L' if k[2] else ''}{'/2' if k[3] == 2 else ''}{'s' if k[4] else ''}"
+        return (f"{k[0]}x{k[1]}{'L' if k[2] else ''}{'/2' if k[3] == 2 else ''}"
+                f"{'s' if k[4] else ''}{f'b{k[5]}' if k[5] else ''}")
 
     return {
         "forward": best_f,
