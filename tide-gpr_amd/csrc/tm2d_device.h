@@ -162,6 +162,36 @@ __device__ __forceinline__ T keep_or_zero(T v, bool keep) {
   return __builtin_bit_cast(T, __builtin_bit_cast(U, v) & m);
 }
 
+// Value of v in lane `lane` (wave-uniform index), as a scalar.
+template <typename T>
+__device__ __forceinline__ T lane_read(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  } else {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  }
+}
+
+// The six y profiles (kA .. kKh) of up to 64 consecutive rows, held across
+// the lanes of a wavefront (lane k: row y_first + k, clamped into the grid).
+// A tile loads them with one vector load per profile up front and reads a
+// row's value back with a cross-lane read, instead of a dependent scalar load
+// (and wait) at every use.
+template <typename T> struct LaneRows {
+  T v[6];
+  int y_first;
+  __device__ __forceinline__ void load(const Geom<T> &g, int y_first_, int lane) {
+    y_first = y_first_;
+    const int yc = min(max(y_first_ + lane, 0), g.ny - 1);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] = g.Y(k, yc);
+  }
+  __device__ __forceinline__ T operator()(int k, int y) const { return lane_read(v[k], y - y_first); }
+};
+
 // Row loader for one wavefront.  Interior tiles (EDGE false) need no bounds
 // checks: every row and column they touch exists.  Edge tiles read a
 // clamped column and zero it outside the grid.

@@ -251,6 +251,8 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   void *const cs1 = snap_plane<T, FMT>(a.curl_store1, base);
   int slot[TY + 2];  // rows y0-1 .. y0+TY
   tile_slots<TY + 2, GY>(a.pts, y0 - 1, ny, slot);
+  LaneRows<T> yp;  // y profiles of rows y0-2 .. y0+TY+1
+  if constexpr (GY) yp.load(g, y0 - 2, lane);
 
   // Rows: Ey y0-2..y0+TY+1, Hx and cq y0-2..y0+TY, Hz, ca and cb y0-1..y0+TY.
   constexpr int NE = TY + 4, NX = TY + 3, NZ = TY + 2;
@@ -313,9 +315,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
     if (e_col && (!GY || (y >= 1 && y < ny - 1))) {
       T d = (ey0[k + 1] - ey0[k]) * g.rdy;
       if (GY && (y < g.pml_y0 || y >= g.pml_y1h)) {
-        const T m = g.Y(kBh, y) * mez[k] + g.Y(kAh, y) * d;
+        const T m = yp(kBh, y) * mez[k] + yp(kAh, y) * d;
         mez[k] = m;
-        d = d / g.Y(kKh, y) + m;
+        d = d / yp(kKh, y) + m;
       }
       hx1[k] = hx0[k] - cqv[k] * d;
     }
@@ -349,9 +351,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
         dhz = dhz / xk + m;
       }
       if (GY && (y < g.pml_y0 || y >= g.pml_y1)) {
-        const T m = g.Y(kB, y) * mhxz[k] + g.Y(kA, y) * dhx;
+        const T m = yp(kB, y) * mhxz[k] + yp(kA, y) * dhx;
         mhxz[k] = m;
-        dhx = dhx / g.Y(kK, y) + m;
+        dhx = dhx / yp(kK, y) + m;
       }
       const T curl = dhz - dhx;
       if constexpr (FMT != 0) {
@@ -378,9 +380,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
     if (e_col && (!GY || (y >= 1 && y < ny - 1))) {
       T d = (ey1[k + 1] - ey1[k]) * g.rdy;
       if (GY && (y < g.pml_y0 || y >= g.pml_y1h)) {
-        const T m = g.Y(kBh, y) * mez[k + 1] + g.Y(kAh, y) * d;
+        const T m = yp(kBh, y) * mez[k + 1] + yp(kAh, y) * d;
         mez[k + 1] = m;
-        d = d / g.Y(kKh, y) + m;
+        d = d / yp(kKh, y) + m;
       }
       hx2[k] = hx1[k + 1] - cqv[k + 1] * d;
     }
@@ -417,9 +419,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
         dhz = dhz / xk + m;
       }
       if (GY && (y < g.pml_y0 || y >= g.pml_y1)) {
-        const T m = g.Y(kB, y) * mhxz[k + 1] + g.Y(kA, y) * dhx;
+        const T m = yp(kB, y) * mhxz[k + 1] + yp(kA, y) * dhx;
         mhxz[k + 1] = m;
-        dhx = dhx / g.Y(kK, y) + m;
+        dhx = dhx / yp(kK, y) + m;
       }
       const T curl = dhz - dhx;
       if constexpr (FMT != 0) {
@@ -560,6 +562,8 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
   const void *const cs1 = snap_plane<T, FMT>(a.curl_store1, base);
   int slot[TY + 2];  // rows y0-1 .. y0+TY
   tile_slots<TY + 2, GY>(a.pts, y0 - 1, ny, slot);
+  LaneRows<T> yp;  // y profiles of rows y0-2 .. y0+TY+1
+  if constexpr (GY) yp.load(g, y0 - 2, lane);
 
   // Rows: lambda_Ey and cb y0-2..y0+TY+1, lambda_Hx and cq y0-2..y0+TY,
   // lambda_Hz and ca y0-1..y0+TY.
@@ -596,20 +600,19 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
     se0[k] = sc0[k] = se1[k] = sc1[k] = 0.0f;
     if constexpr (sizeof(T) == 4 && FMT != 0) {
       const int y = y0 + k;
-      if (!GY || y < ny) {  // uniform
-        const void *const dflt = a.ley_i + base;
-        const int xc = GX ? min(max(x, 0), nx - 1) : x;
-        const uint32_t i = (uint32_t)(y * nx + xc);
-        const float v0 = (float)get_store32<T, FMT>(es0 ? es0 : dflt, i);
-        const float v1 = (float)get_store32<T, FMT>(cs0 ? cs0 : dflt, i);
-        const float v2 = (float)get_store32<T, FMT>(es1 ? es1 : dflt, i);
-        const float v3 = (float)get_store32<T, FMT>(cs1 ? cs1 : dflt, i);
-        const bool in = !GX || (x >= 1 && x < nx);
-        se0[k] = keep_or_zero(v0, grad_a && es0 && in);
-        sc0[k] = keep_or_zero(v1, grad_a && cs0 && in);
-        se1[k] = keep_or_zero(v2, grad_b && es1 && in);
-        sc1[k] = keep_or_zero(v3, grad_b && cs1 && in);
-      }
+      const void *const dflt = a.ley_i + base;
+      const int xc = GX ? min(max(x, 0), nx - 1) : x;
+      const int yc = GY ? min(y, ny - 1) : y;
+      const uint32_t i = (uint32_t)(yc * nx + xc);
+      const float v0 = (float)get_store32<T, FMT>(es0 ? es0 : dflt, i);
+      const float v1 = (float)get_store32<T, FMT>(cs0 ? cs0 : dflt, i);
+      const float v2 = (float)get_store32<T, FMT>(es1 ? es1 : dflt, i);
+      const float v3 = (float)get_store32<T, FMT>(cs1 ? cs1 : dflt, i);
+      const bool in = (!GX || (x >= 1 && x < nx)) && (!GY || y < ny);
+      se0[k] = keep_or_zero(v0, grad_a && es0 && in);
+      sc0[k] = keep_or_zero(v1, grad_a && cs0 && in);
+      se1[k] = keep_or_zero(v2, grad_b && es1 && in);
+      sc1[k] = keep_or_zero(v3, grad_b && cs1 && in);
     }
   }
   // CPML memories (GEN only): my and nz indexed from row y0-2, mx and nx
@@ -658,17 +661,17 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
     if ((GY && (y < 1 || y >= ny)) || !e_col) return (T)0;
     const T q = cbx * lm;
     if (!(GY && (y < g.pml_y0 || y >= g.pml_y1))) return q;
-    const T mn = g.Y(kB, y) * m - q;
+    const T mn = yp(kB, y) * m - q;
     m = mn;
-    return q / g.Y(kK, y) - g.Y(kA, y) * mn;
+    return q / yp(kK, y) - yp(kA, y) * mn;
   };
   auto raw_z = [&](int y, T lhxv, T cqx, T &m) -> T {  // adjoint of dEy/dz (Hx update)
     if ((GY && (y < 1 || y >= ny - 1)) || !e_col) return (T)0;
     const T adj = -cqx * lhxv;
     if (!(GY && (y < g.pml_y0 || y >= g.pml_y1h))) return adj;
-    const T mn = g.Y(kBh, y) * m + adj;
+    const T mn = yp(kBh, y) * m + adj;
     m = mn;
-    return g.Y(kAh, y) * mn + adj / g.Y(kKh, y);
+    return yp(kAh, y) * mn + adj / yp(kKh, y);
   };
   auto raw_xh = [&](int y, T lhzv, T cqx, T &m) -> T {  // adjoint of dEy/dx (Hz update)
     if ((GY && (y < 1 || y >= ny)) || (GX && (x < 1 || x >= nx - 1))) return (T)0;
