@@ -63,9 +63,15 @@ def parse() -> argparse.Namespace:
                     help="also run one 1 GiB device copy (profiler byte-counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="served-solve trace length (0: skip the serving measurement)")
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-rates", default="2,inf",
+                    help="Poisson arrival rates (per second) of the serving sweep; inf = all at once")
     ap.add_argument("--serve-seed", type=int, default=2024)
     ap.add_argument("--serve-tick", type=int, default=64, help="steps per engine tick")
+    ap.add_argument("--batched-models", type=int, default=32,
+                    help="models of the batched uneven-model measurement (0: skip)")
+    ap.add_argument("--batched-seed", type=int, default=3030)
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the small launch-latency-bound forward measurement")
     return ap.parse_args()
 
 
@@ -167,16 +173,18 @@ def _plain_solve(kw, dev, budget: int) -> None:
         loss.backward()
 
 
-def serve_bench(args, dev) -> dict:
+def serve_bench(args, dev, rate: float) -> dict:
     """Served solves: a seeded Poisson trace of forward (70%) and gradient
     (30%) requests replayed in real time through the continuous-batching
     server, then through the plain per-request path (one request at a time,
-    first come first served, on the same arrival times)."""
+    first come first served, on the same arrival times).  The same requests
+    at every rate (only the arrival times scale); rate inf puts every
+    arrival at 0, so solves/s is each path's capacity."""
     import torch
 
     from tide.serve import TM2DServer, latency_stats, served_trace
 
-    trace = served_trace(args.serve_requests, args.serve_rate, seed=args.serve_seed)
+    trace = served_trace(args.serve_requests, rate, seed=args.serve_seed)
     work = sum(_request_updates(kw) for _, kw in trace)
     log(f"serving trace: {len(trace)} requests, {work:.3e} cell-updates")
     torch.cuda.empty_cache()
@@ -210,7 +218,8 @@ def serve_bench(args, dev) -> dict:
     log(f"plain: {plain}")
     r3 = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
     return {
-        "workload": f"{len(trace)} Poisson arrivals at {args.serve_rate}/s (seed {args.serve_seed}), "
+        "rate_per_s": rate if math.isfinite(rate) else "inf",
+        "workload": f"{len(trace)} Poisson arrivals at {rate}/s (seed {args.serve_seed}), "
                     "70% forward / 30% forward+gradient, sides log-uniform 128-1536, nt 500-6000, "
                     "1-32 shots, 64 receivers, fp32 snapshots",
         "cell_updates": work,
@@ -219,6 +228,138 @@ def serve_bench(args, dev) -> dict:
         "plain_one_at_a_time": r3(plain),
         "solves_per_s_ratio": round(served["solves_per_s"] / plain["solves_per_s"], 3),
         "p99_ratio_plain_over_served": round(plain["p99_s"] / served["p99_s"], 3),
+    }
+
+
+def batched_bench(args, dev) -> dict:
+    """Batched uneven models (the 256-model distribution, fewer models):
+    sides log-uniform 128-1024 per axis, nt 500-4000, 1-16 shots, seeded
+    layered + anomaly models, fp32, forward + adjoint gradient each.  The
+    models are packed in arrival order into batched plans that fit the HBM
+    budget (one launch per step over every model of a plan, graph-replayed
+    time loops).  Building a plan (preparation, point tables, graph capture)
+    is timed apart from its gradients: an inversion builds it once and then
+    runs one gradient per iteration (the second run).  The plain path solves
+    the same models one at a time through the public operator."""
+    import torch
+
+    from tide.batch import TM2DBatch
+    from tide.serve import job_bytes, padded_shape, served_trace
+
+    trace = served_trace(args.batched_models, math.inf, seed=args.batched_seed,
+                         side_range=(128, 1024), nt_range=(500, 4000), shot_range=(1, 16),
+                         gradient_fraction=1.0, n_receivers=64, device=dev)
+    reqs = [kw for _, kw in trace]
+    work = sum(_request_updates(kw) for kw in reqs)
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info(dev)
+    budget = int(0.55 * free)
+
+    def nbytes(kw):
+        p = kw["problem"]
+        ny_p, nx_p = padded_shape(*p["epsilon"].shape, p.get("pml_width", 20))
+        shots, n_src, nt = p["source_amplitude"].shape
+        return job_bytes(ny_p, nx_p, shots, nt, n_src, p["receiver_location"].shape[1], gradient=True)
+
+    groups, cur, used = [], [], 0
+    for kw in reqs:
+        b = nbytes(kw)
+        if cur and used + b > budget:
+            groups.append(cur)
+            cur, used = [], 0
+        cur.append(kw)
+        used += b
+    groups.append(cur)
+    log(f"batched: {len(reqs)} models, {work:.3e} cell-updates, plans of {[len(g) for g in groups]}")
+
+    torch.cuda.synchronize()
+    t_build = t_first = t_again = 0.0
+    for g in groups:
+        t0 = time.perf_counter()
+        plan = TM2DBatch([kw["problem"] for kw in g])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plan.gradient([kw["observed"] for kw in g])
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        plan.gradient([kw["observed"] for kw in g])  # the next iteration of an inversion
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        t_build, t_first, t_again = t_build + t1 - t0, t_first + t2 - t1, t_again + t3 - t2
+        plan.close()
+        del plan
+    t_batched = t_build + t_first
+    torch.cuda.empty_cache()
+    log(f"batched plans: build {t_build:.2f} s, first gradient {t_first:.2f} s, next {t_again:.2f} s")
+
+    t0 = time.perf_counter()
+    for kw in reqs:
+        _plain_solve(kw, dev, budget)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    log(f"plain one at a time: {t_plain:.2f} s")
+    n = len(reqs)
+    return {
+        "workload": f"{n} models (seed {args.batched_seed}): sides log-uniform 128-1024, nt 500-4000, "
+                    "1-16 shots, 64 receivers, fp32, forward + adjoint gradient each",
+        "cell_updates": work,
+        "plans": [len(g) for g in groups],
+        "batched": {"build_s": round(t_build, 3), "first_gradient_s": round(t_first, 3),
+                    "next_gradient_s": round(t_again, 3),
+                    "solves_per_s": round(n / t_again, 3),
+                    "solves_per_s_incl_build": round(n / t_batched, 3),
+                    "gcell_per_s": round(work / t_again / 1e9, 2),
+                    "hbm_frac": round(work / t_again * 32 / 1e9 / HBM_PEAK_GBS, 4)},
+        "plain_one_at_a_time": {"seconds": round(t_plain, 3), "solves_per_s": round(n / t_plain, 3),
+                                "gcell_per_s": round(work / t_plain / 1e9, 2)},
+        "solves_per_s_ratio": round(t_plain / t_again, 3),
+        "solves_per_s_ratio_incl_build": round(t_plain / t_batched, 3),
+    }
+
+
+def config1_bench(dev, reps: int = 20) -> dict:
+    """The reference's CPU-runnable case on the GPU: 96x96 (117x117 padded,
+    CPML 10), eps_r 4, one shot, nt 300, forward only -- launch-latency
+    bound.  Plain operator calls against a one-problem batched plan whose
+    time loop replays a captured HIP graph."""
+    import torch
+
+    import tide
+    from tide.batch import TM2DBatch
+    from tide.maxwell.tm2d import maxwelltm
+
+    n, nt, dt, dx = 96, 300, 4e-11, 0.02
+    kw = dict(epsilon=torch.full((n, n), 4.0, device=dev), sigma=torch.zeros(n, n, device=dev),
+              mu=torch.ones(n, n, device=dev), grid_spacing=dx, dt=dt,
+              source_amplitude=tide.ricker(8e8, nt, dt, dtype=torch.float32, device=dev).reshape(1, 1, -1),
+              source_location=torch.tensor([[[20, 48]]]), receiver_location=torch.tensor([[[20, 60]]]),
+              pml_width=10)
+    updates = 117 * 117 * nt
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    with torch.no_grad():
+        t_plain = timed(lambda: maxwelltm(**kw))
+    plan = TM2DBatch([kw], gradients=False)
+
+    def run_plan():
+        plan.reset_state()
+        plan.forward()
+
+    t_graph = timed(run_plan)
+    plan.close()
+    return {
+        "workload": "96x96 (117x117 padded, CPML 10), eps_r 4, 1 shot, nt 300, forward, fp32",
+        "plain_ms": round(t_plain * 1e3, 3), "plain_us_per_step": round(t_plain / nt * 1e6, 2),
+        "graph_ms": round(t_graph * 1e3, 3), "graph_us_per_step": round(t_graph / nt * 1e6, 2),
+        "graph_gcell_per_s": round(updates / t_graph / 1e9, 3),
     }
 
 
@@ -374,7 +515,13 @@ def main() -> None:
     log("timing the CPU baseline" if not args.no_cpu_baseline else "CPU baseline skipped")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, setup_kwargs)
     del ops, observed, e, s, setup_kwargs
-    serving = serve_bench(args, dev) if args.serve_requests > 0 else None
+    serving = None
+    if args.serve_requests > 0:
+        serving = [serve_bench(args, dev, float(r)) for r in args.serve_rates.split(",")]
+    batched = batched_bench(args, dev) if args.batched_models > 0 else None
+    config1 = None if args.no_config1 else config1_bench(dev)
+    if config1 is not None:
+        log(f"config 1: {config1}")
     out = {
         "metric": METRIC,
         "value": round(gcells, 3),
@@ -407,6 +554,8 @@ def main() -> None:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "serving": serving,
+        "batched": batched,
+        "config1": config1,
     }
     print(json.dumps(out), flush=True)
 

@@ -1,10 +1,11 @@
 """Batched plans (tide.batch.TM2DBatch) against single-problem runs (MI355X).
 
 A plan advances problems of different grid sizes, shot counts and nt with
-one launch per step through the same tile kernels as the single-problem
-path, so receivers are bit-identical to separate runs and FP64 gradients
-agree to rounding (the shot sum order of a workgroup is the same; problems
-with more shots than a workgroup holds add per-group sums atomically).
+one launch per step (or per two steps, with the two-step kernels, problems
+of odd nt taking their last step alone) through the same tile kernels as
+the single-problem path, so receivers are bit-identical to separate runs
+and FP64 gradients agree to rounding (problems with more shots than a
+workgroup holds sum per-group FP64 slices, folded after the loop).
 Graph replays are checked against eager runs on every replay, with eager
 work between the replays.
 """
@@ -63,7 +64,18 @@ def problems():
     return [_problem(**p) for p in PROBLEMS]
 
 
-def test_batch_forward_and_adjoint_match_single_runs(problems):
+@pytest.fixture(params=[1, 2], ids=["one_step", "two_step"])
+def steps(request):
+    """Time steps per launch of the plans built in the test (and of the
+    single-problem runs they are compared with)."""
+    before = tide._native.get_tm2d_time_block()
+    tide._native.set_tm2d_time_block(request.param, request.param)
+    print(f"steps per launch {request.param}", flush=True)
+    yield request.param
+    tide._native.set_tm2d_time_block(*before)
+
+
+def test_batch_forward_and_adjoint_match_single_runs(problems, steps):
     batch = TM2DBatch(problems)
     rs = [r.cpu().numpy() for r in batch.forward()]
     rng = np.random.default_rng(0)
@@ -80,7 +92,7 @@ def test_batch_forward_and_adjoint_match_single_runs(problems):
     batch.close()
 
 
-def test_batch_graph_replays_match_eager(problems):
+def test_batch_graph_replays_match_eager(problems, steps):
     eager = TM2DBatch(problems, use_graph=False)
     graph = TM2DBatch(problems, use_graph=True)
     r_e = [r.clone() for r in eager.forward()]
@@ -118,7 +130,7 @@ def test_batch_update_models_in_place(problems):
     batch.close()
 
 
-def test_batch_gradient_matches_operator(problems):
+def test_batch_gradient_matches_operator(problems, steps):
     batch = TM2DBatch(problems[:3])
     observed = []
     for kw in problems[:3]:

@@ -1,5 +1,8 @@
 // Batched 2-D TM plans: many independent problems (different grid sizes,
-// shot counts and nt) advanced together, one launch per time step.
+// shot counts and nt) advanced together, one launch per time step, or (the
+// default, as in the single-problem path) one launch per two time steps
+// with the two-step kernels of tm2d_fused.h; a problem whose nt is odd takes
+// its last step in a one-step launch over its own tiles.
 //
 // Every problem keeps its own arrays; a plan holds a device table with one
 // descriptor per problem and a tile -> problem map.  Problems are sorted by
@@ -28,23 +31,59 @@
 namespace tide_hip {
 namespace tm2d {
 
+// One-step launches over the flat tiles [first, first + gridDim.x) of the
+// one-step tile map; `par` / `epar` select the state buffers (see
+// item_fwd_args).
 template <typename T, int TY, int FMT>
 __global__ __launch_bounds__(512) void fwd_batch(const BatchItem<T> *__restrict__ items,
-                                                 const int32_t *__restrict__ tile_item, int k) {
-  const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+                                                 const int32_t *__restrict__ tile_item, int k,
+                                                 int first, int par, int epar) {
+  const int64_t flat = first + xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
   const BatchItem<T> &it = items[tile_item[flat]];
   const TileCoord tc = batch_tile((int)(flat - it.tile0), it.pts.n_xt, it.n_yt);
-  fwd_workgroup<T, TY, FMT>(item_fwd_args(it, k), tc);
+  fwd_workgroup<T, TY, FMT>(item_fwd_args(it, k, par, epar), tc);
 }
 
 template <typename T, int TY, int FMT>
 __global__ __launch_bounds__(512) void adj_batch(const BatchItem<T> *__restrict__ items,
-                                                 const int32_t *__restrict__ tile_item, int k) {
+                                                 const int32_t *__restrict__ tile_item, int k,
+                                                 int first, int par) {
+  extern __shared__ double red[];
+  const int64_t flat = first + xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  const BatchItem<T> &it = items[tile_item[flat]];
+  const TileCoord tc = batch_tile((int)(flat - it.tile0), it.pts.n_xt, it.n_yt);
+  adj_workgroup<T, TY, FMT>(item_adj_args(it, k, par), tc, red);
+}
+
+// Two-step launches over the first gridDim.x tiles of the two-step tile map
+// (the problems with at least two steps left at step k).
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(512) void fwd2_batch(const BatchItem<T> *__restrict__ items,
+                                                  const int32_t *__restrict__ tile_item, int k,
+                                                  int par) {
+  const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  const BatchItem<T> &it = items[tile_item[flat]];
+  const TileCoord tc = batch_tile((int)(flat - it.tile0_2), it.pts2.n_xt, it.n_yt2);
+  const Fwd2Args<T> a = item_fwd2_args(it, k, par);
+  const int s = tc.grp * (int)blockDim.y + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
+  if (in_split<TY>(a.split, tc.xt, tc.yt))
+    fwd2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt);
+  else
+    fwd2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt);
+}
+
+template <typename T, int TY, int FMT>
+__global__ __launch_bounds__(512) void adj2_batch(const BatchItem<T> *__restrict__ items,
+                                                  const int32_t *__restrict__ tile_item, int k,
+                                                  int par) {
   extern __shared__ double red[];
   const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
   const BatchItem<T> &it = items[tile_item[flat]];
-  const TileCoord tc = batch_tile((int)(flat - it.tile0), it.pts.n_xt, it.n_yt);
-  adj_workgroup<T, TY, FMT>(item_adj_args(it, k), tc, red);
+  const TileCoord tc = batch_tile((int)(flat - it.tile0_2), it.pts2.n_xt, it.n_yt2);
+  const Adj2Args<T> a = item_adj2_args(it, k, par);
+  const int edge = in_split<TY>(a.split, tc.xt, tc.yt) ? 0 : 3;  // uniform over the workgroup
+  adj2_workgroup<T, TY, FMT, 1 | 8>(a, tc, edge, red);
 }
 
 template <typename T>
@@ -61,14 +100,20 @@ struct Plan {
   int64_t n = 0;
   int fmt_fwd = 0, fmt_adj = 0;  // 0 none, 1 T payload, 2 BF16
   int rows_f = 8, rows_a = 4, group = 8;
+  int steps_f = 1, steps_a = 1;  // time steps per launch of each pass
   int max_nt = 0;
   std::vector<tide_tm2d_problem> probs;  // sorted by nt, descending
   std::vector<void *> allocs;            // plan-owned device memory
   BatchItem<float> *d_fwd = nullptr, *d_adj = nullptr;
   int32_t *d_tiles_fwd = nullptr, *d_tiles_adj = nullptr;
   std::vector<int64_t> active_fwd, active_adj;  // tiles running at step k
+  // two-step launches: tiles with >= 2 steps left at step 2j (active2[j]),
+  // and the one-step tile range [rem_*[j], rem_*[j+1]...) of the problems
+  // taking their single last step at step 2j (first, end per j)
+  int32_t *d_tiles2 = nullptr;
+  std::vector<int64_t> active2, rem_f_first, rem_f_end, rem_a_first, rem_a_end;
   std::vector<BatchItem<float>> h_fwd, h_adj;
-  std::vector<DevicePointTable> dpts;
+  std::vector<DevicePointTable> dpts, dpts2;
   hipStream_t capture = nullptr;
   hipGraphExec_t g_fwd = nullptr, g_adj = nullptr;
   hipGraph_t gr_fwd = nullptr, gr_adj = nullptr;
@@ -115,26 +160,60 @@ void launch_zero(T *p, int64_t n, hipStream_t s) {
 
 template <int FMT>
 void launch_fwd_b(int rows, int64_t grid, int group, hipStream_t s, const BatchItem<float> *d,
-                  const int32_t *tiles, int k) {
+                  const int32_t *tiles, int k, int first = 0, int par = -1, int epar = 0) {
   const dim3 block(64, (unsigned)group);
+  if (par < 0) par = k & 1;
   switch (rows) {
-    case 2: fwd_batch<float, 2, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k); break;
-    case 4: fwd_batch<float, 4, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k); break;
-    case 16: fwd_batch<float, 16, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k); break;
-    default: fwd_batch<float, 8, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k); break;
+    case 2: fwd_batch<float, 2, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k, first, par, epar); break;
+    case 4: fwd_batch<float, 4, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k, first, par, epar); break;
+    case 16: fwd_batch<float, 16, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k, first, par, epar); break;
+    default: fwd_batch<float, 8, FMT><<<(unsigned)grid, block, 0, s>>>(d, tiles, k, first, par, epar); break;
   }
 }
 
 template <int FMT>
 void launch_adj_b(int rows, int64_t grid, int group, hipStream_t s, const BatchItem<float> *d,
-                  const int32_t *tiles, int k) {
+                  const int32_t *tiles, int k, int first = 0, int par = -1) {
   const dim3 block(64, (unsigned)group);
   const size_t lds = 2 * (size_t)group * rows * 64 * sizeof(double);
+  if (par < 0) par = k & 1;
   switch (rows) {
-    case 2: adj_batch<float, 2, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k); break;
-    case 8: adj_batch<float, 8, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k); break;
-    default: adj_batch<float, 4, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k); break;
+    case 2: adj_batch<float, 2, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k, first, par); break;
+    case 8: adj_batch<float, 8, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k, first, par); break;
+    default: adj_batch<float, 4, FMT><<<(unsigned)grid, block, lds, s>>>(d, tiles, k, first, par); break;
   }
+}
+
+constexpr int kBatchRows2 = 4;  // tile rows of the batched two-step kernels
+
+void launch_step_f(const Plan &p, int fmt, int64_t grid, hipStream_t s, int k, int first, int par,
+                   int epar) {
+  if (fmt == 0) launch_fwd_b<0>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k, first, par, epar);
+  else if (fmt == 2) launch_fwd_b<2>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k, first, par, epar);
+  else launch_fwd_b<1>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k, first, par, epar);
+}
+
+void launch_step_a(const Plan &p, int fmt, int64_t grid, hipStream_t s, int k, int first, int par) {
+  if (fmt == 0) launch_adj_b<0>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k, first, par);
+  else if (fmt == 2) launch_adj_b<2>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k, first, par);
+  else launch_adj_b<1>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k, first, par);
+}
+
+void launch_pair_f(const Plan &p, int fmt, int64_t grid, hipStream_t s, int k, int par) {
+  const dim3 block(64, (unsigned)p.group);
+  constexpr int TY = kBatchRows2;
+  if (fmt == 0) fwd2_batch<float, TY, 0><<<(unsigned)grid, block, 0, s>>>(p.d_fwd, p.d_tiles2, k, par);
+  else if (fmt == 2) fwd2_batch<float, TY, 2><<<(unsigned)grid, block, 0, s>>>(p.d_fwd, p.d_tiles2, k, par);
+  else fwd2_batch<float, TY, 1><<<(unsigned)grid, block, 0, s>>>(p.d_fwd, p.d_tiles2, k, par);
+}
+
+void launch_pair_a(const Plan &p, int fmt, int64_t grid, hipStream_t s, int k, int par) {
+  const dim3 block(64, (unsigned)p.group);
+  constexpr int TY = kBatchRows2;
+  const size_t lds = 2 * (size_t)p.group * TY * 64 * sizeof(double);
+  if (fmt == 0) adj2_batch<float, TY, 0><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2, k, par);
+  else if (fmt == 2) adj2_batch<float, TY, 2><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2, k, par);
+  else adj2_batch<float, TY, 1><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2, k, par);
 }
 
 // Enqueues one whole forward run on `s`.
@@ -147,22 +226,42 @@ int enqueue_forward(Plan &p, hipStream_t s) {
     // the second copy of the H-phase memories starts equal to the first
     TIDE_HIP_TRY(hipMemcpyAsync(it.fwd[1][3], it.fwd[0][3], total * fb, hipMemcpyDeviceToDevice, s));
     TIDE_HIP_TRY(hipMemcpyAsync(it.fwd[1][4], it.fwd[0][4], total * fb, hipMemcpyDeviceToDevice, s));
+    if (p.steps_f == 2) {  // and of the E-phase memories, which the two-step kernels ping-pong
+      TIDE_HIP_TRY(hipMemcpyAsync(it.mhzx_b, it.mhzx, total * fb, hipMemcpyDeviceToDevice, s));
+      TIDE_HIP_TRY(hipMemcpyAsync(it.mhxz_b, it.mhxz, total * fb, hipMemcpyDeviceToDevice, s));
+    }
     if (q.n_rec > 0) launch_zero(q.r, q.nt * q.n_shots * q.n_rec, s);
   }
-  for (int k = 0; k < p.max_nt; ++k) {
-    const int64_t grid = p.active_fwd[(size_t)k];
-    if (grid <= 0) break;
-    if (p.fmt_fwd == 0) launch_fwd_b<0>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k);
-    else if (p.fmt_fwd == 2) launch_fwd_b<2>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k);
-    else launch_fwd_b<1>(p.rows_f, grid, p.group, s, p.d_fwd, p.d_tiles_fwd, k);
+  if (p.steps_f == 2) {
+    for (int k = 0; k < p.max_nt; k += 2) {
+      const size_t j = (size_t)k / 2;
+      const int par = (int)(j & 1);
+      const int64_t grid2 = p.active2[j];
+      const int64_t first = p.rem_f_first[j], end = p.rem_f_end[j];
+      if (grid2 > 0) launch_pair_f(p, p.fmt_fwd, grid2, s, k, par);
+      if (end > first) launch_step_f(p, p.fmt_fwd, end - first, s, k, (int)first, par, par);
+    }
+  } else {
+    for (int k = 0; k < p.max_nt; ++k) {
+      const int64_t grid = p.active_fwd[(size_t)k];
+      if (grid <= 0) break;
+      launch_step_f(p, p.fmt_fwd, grid, s, k, 0, k & 1, 0);
+    }
   }
+  // final state back into the caller's arrays
   for (size_t m = 0; m < p.probs.size(); ++m) {
     const tide_tm2d_problem &q = p.probs[m];
-    if ((q.nt & 1) == 0) continue;
     const BatchItem<float> &it = p.h_fwd[m];
     const int64_t total = q.n_shots * q.ny * q.nx;
-    for (int c = 0; c < 5; ++c)
-      TIDE_HIP_TRY(hipMemcpyAsync(it.fwd[0][c], it.fwd[1][c], total * fb, hipMemcpyDeviceToDevice, s));
+    const bool two = p.steps_f == 2;
+    const int fpar = (int)((two ? (q.nt + 1) / 2 : q.nt) & 1), epar = two ? (int)((q.nt / 2) & 1) : 0;
+    if (fpar)
+      for (int c = 0; c < 5; ++c)
+        TIDE_HIP_TRY(hipMemcpyAsync(it.fwd[0][c], it.fwd[1][c], total * fb, hipMemcpyDeviceToDevice, s));
+    if (epar) {
+      TIDE_HIP_TRY(hipMemcpyAsync(it.mhzx, it.mhzx_b, total * fb, hipMemcpyDeviceToDevice, s));
+      TIDE_HIP_TRY(hipMemcpyAsync(it.mhxz, it.mhxz_b, total * fb, hipMemcpyDeviceToDevice, s));
+    }
   }
   TIDE_HIP_TRY(hipGetLastError());
   return kOk;
@@ -185,12 +284,21 @@ int enqueue_backward(Plan &p, hipStream_t s) {
     }
     if (q.grad_f && q.n_src > 0) launch_zero(q.grad_f, q.nt * q.n_shots * q.n_src, s);
   }
-  for (int k = 0; k < p.max_nt; ++k) {
-    const int64_t grid = p.active_adj[(size_t)k];
-    if (grid <= 0) break;
-    if (p.fmt_adj == 0) launch_adj_b<0>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
-    else if (p.fmt_adj == 2) launch_adj_b<2>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
-    else launch_adj_b<1>(p.rows_a, grid, p.group, s, p.d_adj, p.d_tiles_adj, k);
+  if (p.steps_a == 2) {
+    for (int k = 0; k < p.max_nt; k += 2) {
+      const size_t j = (size_t)k / 2;
+      const int par = (int)(j & 1);
+      const int64_t grid2 = p.active2[j];
+      const int64_t first = p.rem_a_first[j], end = p.rem_a_end[j];
+      if (grid2 > 0) launch_pair_a(p, p.fmt_adj, grid2, s, k, par);
+      if (end > first) launch_step_a(p, p.fmt_adj, end - first, s, k, (int)first, par);
+    }
+  } else {
+    for (int k = 0; k < p.max_nt; ++k) {
+      const int64_t grid = p.active_adj[(size_t)k];
+      if (grid <= 0) break;
+      launch_step_a(p, p.fmt_adj, grid, s, k, 0, k & 1);
+    }
   }
   for (size_t m = 0; m < p.probs.size(); ++m) {
     const BatchItem<float> &it = p.h_adj[m];
@@ -203,7 +311,7 @@ int enqueue_backward(Plan &p, hipStream_t s) {
   }
   for (size_t m = 0; m < p.probs.size(); ++m) {
     const tide_tm2d_problem &q = p.probs[m];
-    if ((q.nt & 1) == 0) continue;
+    if ((((p.steps_a == 2) ? (q.nt + 1) / 2 : q.nt) & 1) == 0) continue;
     const BatchItem<float> &it = p.h_adj[m];
     const int64_t total = q.n_shots * q.ny * q.nx;
     for (int c = 0; c < 7; ++c)
@@ -237,6 +345,7 @@ void destroy(Plan *p) {
   if (p->gr_fwd) (void)hipGraphDestroy(p->gr_fwd);
   if (p->gr_adj) (void)hipGraphDestroy(p->gr_adj);
   for (DevicePointTable &d : p->dpts) d.release(nullptr);
+  for (DevicePointTable &d : p->dpts2) d.release(nullptr);
   (void)hipDeviceSynchronize();
   for (void *a : p->allocs) (void)hipFree(a);
   if (p->capture) (void)hipStreamDestroy(p->capture);
@@ -276,6 +385,64 @@ int build_items(Plan &p, int rows, std::vector<BatchItem<float>> &items,
     while (m > 0 && p.probs[m - 1].nt <= k) --m;
     run = prefix[m];
     active[(size_t)k] = run;
+  }
+  return kOk;
+}
+
+// Two-step tiles (kOwn2 columns x kBatchRows2 rows, the same shot groups as
+// the one-step tiles): fills n_yt2 / tile0_2 / split2 of both passes' items
+// and returns the two-step tile -> item map; also the per-pair launch sizes.
+int build_items2(Plan &p, std::vector<int32_t> &tile_item) {
+  constexpr int TY = kBatchRows2;
+  const size_t n = p.probs.size();
+  tile_item.clear();
+  std::vector<int64_t> prefix(n + 1, 0);
+  for (size_t m = 0; m < n; ++m) {
+    BatchItem<float> &f = p.h_fwd[m];
+    const int32_t n_yt2 = (int32_t)((p.probs[m].ny + TY - 1) / TY);
+    const int64_t tiles = (int64_t)f.pts2.n_xt * n_yt2 * f.n_groups;
+    if (prefix[m] + tiles >= ((int64_t)1 << 31)) {
+      set_error("batch: too many tiles for one launch");
+      return kInvalidArgument;
+    }
+    const TileSplit sp = make_split(f.g, f.pts2.n_xt, n_yt2, TY);
+    for (BatchItem<float> *it : {&p.h_fwd[m], &p.h_adj[m]}) {
+      it->n_yt2 = n_yt2;
+      it->tile0_2 = (int32_t)prefix[m];
+      it->split2 = sp;
+    }
+    tile_item.insert(tile_item.end(), (size_t)tiles, (int32_t)m);
+    prefix[m + 1] = prefix[m] + tiles;
+  }
+  // problems are sorted by nt descending: count those with nt >= v
+  auto count_at_least = [&](int64_t v) {
+    size_t c = 0;
+    while (c < n && p.probs[c].nt >= v) ++c;
+    return c;
+  };
+  const size_t pairs = (size_t)(p.max_nt + 1) / 2;
+  p.active2.assign(pairs, 0);
+  p.rem_f_first.assign(pairs, 0);
+  p.rem_f_end.assign(pairs, 0);
+  p.rem_a_first.assign(pairs, 0);
+  p.rem_a_end.assign(pairs, 0);
+  auto tile_start = [&](const std::vector<BatchItem<float>> &items, size_t m, int64_t end) {
+    return m < n ? (int64_t)items[m].tile0 : end;
+  };
+  const int64_t end_f = n ? p.h_fwd[n - 1].tile0 + (int64_t)p.h_fwd[n - 1].pts.n_xt *
+                                                      p.h_fwd[n - 1].n_yt * p.h_fwd[n - 1].n_groups
+                          : 0;
+  const int64_t end_a = n ? p.h_adj[n - 1].tile0 + (int64_t)p.h_adj[n - 1].pts.n_xt *
+                                                      p.h_adj[n - 1].n_yt * p.h_adj[n - 1].n_groups
+                          : 0;
+  for (size_t j = 0; j < pairs; ++j) {
+    const int64_t k = 2 * (int64_t)j;
+    const size_t two = count_at_least(k + 2), one = count_at_least(k + 1);
+    p.active2[j] = prefix[two];
+    p.rem_f_first[j] = tile_start(p.h_fwd, two, end_f);
+    p.rem_f_end[j] = tile_start(p.h_fwd, one, end_f);
+    p.rem_a_first[j] = tile_start(p.h_adj, two, end_a);
+    p.rem_a_end[j] = tile_start(p.h_adj, one, end_a);
   }
   return kOk;
 }
@@ -323,6 +490,9 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
   p->rows_f = rows_for<float>(g_tuning.fwd_rows);
   p->rows_a = rows_for<float>(g_tuning.adj_rows);
   p->group = g_tuning.adj_group;  // one group size for both passes
+  p->steps_f = g_tuning.fwd_steps == 2 ? 2 : 1;
+  p->steps_a = g_tuning.adj_steps == 2 ? 2 : 1;
+  const bool two = p->steps_f == 2 || p->steps_a == 2;
   p->probs.assign(problems, problems + n);
   std::stable_sort(p->probs.begin(), p->probs.end(),
                    [](const tide_tm2d_problem &a, const tide_tm2d_problem &b) { return a.nt > b.nt; });
@@ -331,6 +501,7 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
   auto fail = [](int rc) { return rc; };
   std::vector<BatchItem<float>> proto(p->probs.size());
   p->dpts.reserve(p->probs.size());
+  p->dpts2.reserve(p->probs.size());
   for (size_t m = 0; m < p->probs.size(); ++m) {
     const tide_tm2d_problem &q = p->probs[m];
     if (q.n_shots <= 0 || q.ny < 3 || q.nx < 3 || q.nt < 0 || q.interval <= 0 ||
@@ -388,6 +559,16 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
     DevicePointTable &dpt = p->dpts.back();
     PLAN_CHECK(dpt.upload(table, nullptr));
     it.pts = Points{dpt.row_slot, dpt.seg, dpt.ent, dpt.n_xt};
+    if (two) {  // point table of the two-step tiles (halo copies included)
+      PointTable table2;
+      if (!build_point_table_halo(hs.data(), q.n_src, hr.data(), q.n_rec, q.n_shots, q.ny, q.nx,
+                                  kOwn2, 2, table2))
+        return fail(kInvalidArgument);
+      p->dpts2.emplace_back();
+      DevicePointTable &dpt2 = p->dpts2.back();
+      PLAN_CHECK(dpt2.upload(table2, nullptr));
+      it.pts2 = Points{dpt2.row_slot, dpt2.seg, dpt2.ent, dpt2.n_xt};
+    }
     it.ca = q.ca;
     it.cb = q.cb;
     it.cq = q.cq;
@@ -401,6 +582,10 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
     for (int c = 3; c < 5; ++c) TIDE_HIP_TRY(hipMemset(it.fwd[1][c], 0, (size_t)total * sizeof(float)));
     it.mhzx = q.fields[6];
     it.mhxz = q.fields[5];
+    if (p->steps_f == 2) {
+      PLAN_CHECK(plan_alloc(*p, &it.mhzx_b, (size_t)total));
+      PLAN_CHECK(plan_alloc(*p, &it.mhxz_b, (size_t)total));
+    }
     it.f = q.f;
     it.r = q.r;
     it.ey_store = q.ey_store;
@@ -438,6 +623,8 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
   std::vector<int32_t> tiles_f, tiles_a;
   PLAN_CHECK(build_items(*p, p->rows_f, p->h_fwd, tiles_f, p->active_fwd, proto));
   PLAN_CHECK(build_items(*p, p->rows_a, p->h_adj, tiles_a, p->active_adj, proto));
+  std::vector<int32_t> tiles2;
+  if (two) PLAN_CHECK(build_items2(*p, tiles2));
   // problems with several shot groups accumulate into per-group FP64 slices
   for (size_t m = 0; m < p->probs.size(); ++m) {
     BatchItem<float> &it = p->h_adj[m];
@@ -453,6 +640,11 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
   PLAN_CHECK(plan_alloc(*p, &p->d_adj, p->h_adj.size()));
   PLAN_CHECK(plan_alloc(*p, &p->d_tiles_fwd, tiles_f.size()));
   PLAN_CHECK(plan_alloc(*p, &p->d_tiles_adj, tiles_a.size()));
+  if (two) {
+    PLAN_CHECK(plan_alloc(*p, &p->d_tiles2, tiles2.size()));
+    TIDE_HIP_TRY(hipMemcpy(p->d_tiles2, tiles2.data(), tiles2.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice));
+  }
   TIDE_HIP_TRY(hipMemcpy(p->d_fwd, p->h_fwd.data(), p->h_fwd.size() * sizeof(BatchItem<float>),
                          hipMemcpyHostToDevice));
   TIDE_HIP_TRY(hipMemcpy(p->d_adj, p->h_adj.data(), p->h_adj.size() * sizeof(BatchItem<float>),
