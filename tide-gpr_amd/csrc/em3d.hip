@@ -114,7 +114,10 @@ __device__ __forceinline__ Tile3 decode3(int n_xt, int ny, int n_zb, int n_group
   return c;
 }
 
-// Reads p[z, y, x] of one shot plane; zero outside the grid.
+// Reads p[z, y, x] of one shot volume (p wave-uniform); zero outside the
+// grid.  Branch-free (clamped line and column, then a mask), so a wavefront
+// issues all its loads before waiting on any; the line start is a uniform
+// 64-bit pointer and the column a 32-bit lane offset.
 template <typename T> struct Ld3 {
   int xc, nx, ny, nz;
   bool xv;
@@ -123,9 +126,13 @@ template <typename T> struct Ld3 {
     xc = min(max(x, 0), nx_ - 1);
   }
   __device__ __forceinline__ T operator()(const T *__restrict__ p, int z, int y) const {
-    if (z < 0 || z >= nz || y < 0 || y >= ny) return (T)0;  // uniform
-    const T v = p[(z * ny + y) * nx + xc];
-    return xv ? v : (T)0;
+    const int zc = min(max(z, 0), nz - 1), yc = min(max(y, 0), ny - 1);
+    const T v = *elem32(p + (int64_t)(zc * ny + yc) * nx, (uint32_t)xc);
+    return keep_or_zero(v, xv && z == zc && y == yc);
+  }
+  // Element x of line (z, y), both inside the grid, at the clamped column.
+  __device__ __forceinline__ T at(const T *__restrict__ p, int z, int y) const {
+    return *elem32(p + (int64_t)(z * ny + y) * nx, (uint32_t)xc);
   }
 };
 
@@ -145,7 +152,9 @@ __global__ __launch_bounds__(512) void h3_step(Args3<T> a) {
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
   const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
-  const int s = tc.grp * G + (int)threadIdx.y;
+  // threadIdx.y is one value per wavefront (64-wide rows): made explicit so
+  // the shot's volume pointers stay scalar.
+  const int s = tc.grp * G + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   if (s >= a.n_shots) return;
   const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
   const int lane = threadIdx.x;
@@ -172,10 +181,12 @@ __global__ __launch_bounds__(512) void h3_step(Args3<T> a) {
     if (z >= nz) break;  // uniform
     const T exn = L(ex, z + 1, y), eyn = L(ey, z + 1, y), ezn = L(ez, z + 1, y);
     const T ex_yp = L(ex, z, y + 1), ez_yp = L(ez, z, y + 1);
+    // this cell's coefficient and H, loaded with the neighbours (no branch)
+    const T cqv = L.at(cq, z, y);
+    const T hx0 = L.at(hx, z, y), hy0 = L.at(hy, z, y), hz0 = L.at(hz, z, y);
     const T ez_xp = shfl_dn1(ezc), ey_xp = shfl_dn1(eyc);
     if (own && z >= 1) {
       const int j = (z * ny + y) * nx + x;
-      const T cqv = cq[j];
       const bool has_dz = z < nz - 1;
       const bool pzh = z < g.z0 || z >= g.z1h;
       T dey_dz = (T)0, dez_dy = (T)0, dez_dx = (T)0, dex_dz = (T)0, dex_dy = (T)0, dey_dx = (T)0;
@@ -206,9 +217,10 @@ __global__ __launch_bounds__(512) void h3_step(Args3<T> a) {
           dey_dx = cpml(dey_dx, a.f[M_EY_X] + base, j, ax, bx, kx, true);
         }
       }
-      hx[j] = hx[j] - cqv * (dey_dz - dez_dy);
-      hy[j] = hy[j] - cqv * (dez_dx - dex_dz);
-      hz[j] = hz[j] - cqv * (dex_dy - dey_dx);
+      const int64_t line = (int64_t)(z * ny + y) * nx;
+      *elem32(hx + line, (uint32_t)x) = hx0 - cqv * (dey_dz - dez_dy);
+      *elem32(hy + line, (uint32_t)x) = hy0 - cqv * (dez_dx - dex_dz);
+      *elem32(hz + line, (uint32_t)x) = hz0 - cqv * (dex_dy - dey_dx);
     }
     exc = exn;
     eyc = eyn;
@@ -223,7 +235,7 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
   const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
-  const int s = tc.grp * G + (int)threadIdx.y;
+  const int s = tc.grp * G + __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one per wavefront
   if (s >= a.n_shots) return;
   const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
   const int lane = threadIdx.x;
@@ -250,6 +262,9 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
     if (z >= nz) break;
     const T hxc = L(hx, z, y), hyc = L(hy, z, y), hzc = L(hz, z, y);
     const T hz_ym = L(hz, z, y - 1), hx_ym = L(hx, z, y - 1);
+    // this cell's E and coefficients, loaded with the neighbours (no branch)
+    const T ex0 = L.at(ex, z, y), ey0 = L.at(ey, z, y), ez0 = L.at(ez, z, y);
+    const T cav = L.at(ca, z, y), cbv = L.at(cb, z, y);
     const T hz_xm = shfl_up1(hzc), hy_xm = shfl_up1(hyc);
     if (own && z >= 1) {
       const int j = (z * ny + y) * nx + x;
@@ -278,7 +293,7 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
       const T cx = dhy_dz - dhz_dy;
       const T cy = dhz_dx - dhx_dz;
       const T cz = dhx_dy - dhy_dx;
-      T e0 = ex[j], e1 = ey[j], e2 = ez[j];
+      T e0 = ex0, e1 = ey0, e2 = ez0;
       if constexpr (FMT != 0) {
         const int64_t i = base + j;
         if (a.st[0]) {
@@ -292,7 +307,6 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
           put_store<T, FMT>(a.st[5], i, cz);
         }
       }
-      const T cav = ca[j], cbv = cb[j];
       e0 = cav * e0 + cbv * cx;
       e1 = cav * e1 + cbv * cy;
       e2 = cav * e2 + cbv * cz;
@@ -314,9 +328,10 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
           }
         }
       }
-      ex[j] = e0;
-      ey[j] = e1;
-      ez[j] = e2;
+      const int64_t line = (int64_t)(z * ny + y) * nx;
+      *elem32(ex + line, (uint32_t)x) = e0;
+      *elem32(ey + line, (uint32_t)x) = e1;
+      *elem32(ez + line, (uint32_t)x) = e2;
     }
     hx_zm = hxc;
     hy_zm = hyc;

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace tide_hip {
@@ -128,6 +129,74 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t n) {
   const int64_t per = n >> 3, rem = n & 7;
   const int64_t xcd = b & 7, idx = b >> 3;
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+// -------------------------------------------------- addressing and loads
+// Element i of an array through a base pointer that is the same in every
+// lane of the wavefront (a shot's plane, say).  The byte offset is formed in
+// 32 bits, so the access compiles to the scalar-base plus 32-bit
+// vector-offset address form instead of 64-bit address arithmetic per access.
+// Callers keep i * sizeof(T) below 2^32: a 2-D shot plane holds fewer than
+// 2^29 cells (fp64 included); 3-D code indexes within a row whose start is
+// a wave-uniform 64-bit pointer.
+template <typename T>
+__device__ __forceinline__ T *elem32(T *p, uint32_t i) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T *>(reinterpret_cast<B *>(p) + i * (uint32_t)sizeof(T));
+}
+
+// Shot plane of a snapshot buffer (payload T, or BF16 for FMT 2) starting at
+// element `base`; null stays null.
+template <typename T, int FMT, typename V>
+__device__ __forceinline__ V *snap_plane(V *p, int64_t base) {
+  using B = typename std::conditional<std::is_const<V>::value, const char, char>::type;
+  constexpr int64_t bytes = FMT == 2 ? 2 : (int64_t)sizeof(T);
+  return p == nullptr ? nullptr : reinterpret_cast<V *>(reinterpret_cast<B *>(p) + base * bytes);
+}
+
+// put_store / get_store through a snapshot plane (snap_plane) and a 32-bit
+// in-plane index.
+template <typename T, int FMT>
+__device__ __forceinline__ void put_store32(void *plane, uint32_t j, T v) {
+  if constexpr (FMT == 2) {
+    __builtin_nontemporal_store(f32_to_bf16((float)v), elem32((uint16_t *)plane, j));
+  } else {
+    __builtin_nontemporal_store(v, elem32((T *)plane, j));
+  }
+}
+
+template <typename T, int FMT>
+__device__ __forceinline__ double get_store32(const void *plane, uint32_t j) {
+  if constexpr (FMT == 2) {
+    return (double)bf16_to_f32(__builtin_nontemporal_load(elem32((const uint16_t *)plane, j)));
+  } else {
+    return (double)__builtin_nontemporal_load(elem32((const T *)plane, j));
+  }
+}
+
+// v where `keep`, else +0, without a select: a select with a loaded operand
+// may be turned into a branch around the load, and the wavefront then waits
+// for that load on the spot instead of with the other loads in flight.  The
+// empty asm hides where the mask comes from, so the AND stays an AND.
+template <typename T>
+__device__ __forceinline__ T keep_or_zero(T v, bool keep) {
+  using U = typename std::conditional<sizeof(T) == 8, uint64_t, uint32_t>::type;
+  U m = keep ? ~U(0) : U(0);
+  asm("" : "+v"(m));
+  return __builtin_bit_cast(T, __builtin_bit_cast(U, v) & m);
+}
+
+// Value of v in lane `lane` (wave-uniform index), as a scalar.
+template <typename T>
+__device__ __forceinline__ T lane_read(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+  } else {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(T, ((uint64_t)hi << 32) | lo);
+  }
 }
 
 }  // namespace tide_hip
