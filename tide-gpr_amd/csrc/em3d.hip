@@ -174,16 +174,37 @@ __global__ __launch_bounds__(512) void h3_step(Args3<T> a) {
   const bool pxh = x < g.x0 || x >= g.x1h;
   const int zs = tc.zb * TZ;
 
+  // Inputs of plane z: E at z + 1 (z-difference) and at y + 1, and this
+  // cell's coefficient and H.  Plane z + 1's inputs are loaded before plane
+  // z's stores (software pipelining: the compiler cannot move the loads
+  // above stores that might alias them).
+  struct In {
+    T exn, eyn, ezn, ex_yp, ez_yp, cqv, hx0, hy0, hz0;
+  };
+  auto load_in = [&](int z) {
+    In v;
+    const int zc = min(z, nz - 1);
+    v.exn = L(ex, z + 1, y);
+    v.eyn = L(ey, z + 1, y);
+    v.ezn = L(ez, z + 1, y);
+    v.ex_yp = L(ex, zc, y + 1);
+    v.ez_yp = L(ez, zc, y + 1);
+    v.cqv = L.at(cq, zc, y);
+    v.hx0 = L.at(hx, zc, y);
+    v.hy0 = L.at(hy, zc, y);
+    v.hz0 = L.at(hz, zc, y);
+    return v;
+  };
   T exc = L(ex, zs, y), eyc = L(ey, zs, y), ezc = L(ez, zs, y);
+  In next = load_in(zs);
 #pragma unroll
   for (int r = 0; r < TZ; ++r) {
     const int z = zs + r;
     if (z >= nz) break;  // uniform
-    const T exn = L(ex, z + 1, y), eyn = L(ey, z + 1, y), ezn = L(ez, z + 1, y);
-    const T ex_yp = L(ex, z, y + 1), ez_yp = L(ez, z, y + 1);
-    // this cell's coefficient and H, loaded with the neighbours (no branch)
-    const T cqv = L.at(cq, z, y);
-    const T hx0 = L.at(hx, z, y), hy0 = L.at(hy, z, y), hz0 = L.at(hz, z, y);
+    const In in = next;
+    if (r + 1 < TZ) next = load_in(z + 1);
+    const T exn = in.exn, eyn = in.eyn, ezn = in.ezn, ex_yp = in.ex_yp, ez_yp = in.ez_yp;
+    const T cqv = in.cqv, hx0 = in.hx0, hy0 = in.hy0, hz0 = in.hz0;
     const T ez_xp = shfl_dn1(ezc), ey_xp = shfl_dn1(eyc);
     if (own && z >= 1) {
       const int j = (z * ny + y) * nx + x;
@@ -255,16 +276,36 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
   const bool px = x < g.x0 || x >= g.x1;
   const int zs = tc.zb * TZ;
 
+  // Inputs of plane z: H at z and y - 1, and this cell's E and
+  // coefficients; plane z + 1's are loaded before plane z's stores.
+  struct In {
+    T hxc, hyc, hzc, hz_ym, hx_ym, ex0, ey0, ez0, cav, cbv;
+  };
+  auto load_in = [&](int z) {
+    In v;
+    const int zc = min(z, nz - 1);
+    v.hxc = L(hx, z, y);
+    v.hyc = L(hy, z, y);
+    v.hzc = L(hz, z, y);
+    v.hz_ym = L(hz, z, y - 1);
+    v.hx_ym = L(hx, z, y - 1);
+    v.ex0 = L.at(ex, zc, y);
+    v.ey0 = L.at(ey, zc, y);
+    v.ez0 = L.at(ez, zc, y);
+    v.cav = L.at(ca, zc, y);
+    v.cbv = L.at(cb, zc, y);
+    return v;
+  };
   T hx_zm = L(hx, zs - 1, y), hy_zm = L(hy, zs - 1, y);
+  In next = load_in(zs);
 #pragma unroll
   for (int r = 0; r < TZ; ++r) {
     const int z = zs + r;
     if (z >= nz) break;
-    const T hxc = L(hx, z, y), hyc = L(hy, z, y), hzc = L(hz, z, y);
-    const T hz_ym = L(hz, z, y - 1), hx_ym = L(hx, z, y - 1);
-    // this cell's E and coefficients, loaded with the neighbours (no branch)
-    const T ex0 = L.at(ex, z, y), ey0 = L.at(ey, z, y), ez0 = L.at(ez, z, y);
-    const T cav = L.at(ca, z, y), cbv = L.at(cb, z, y);
+    const In in = next;
+    if (r + 1 < TZ) next = load_in(z + 1);
+    const T hxc = in.hxc, hyc = in.hyc, hzc = in.hzc, hz_ym = in.hz_ym, hx_ym = in.hx_ym;
+    const T ex0 = in.ex0, ey0 = in.ey0, ez0 = in.ez0, cav = in.cav, cbv = in.cbv;
     const T hz_xm = shfl_up1(hzc), hy_xm = shfl_up1(hyc);
     if (own && z >= 1) {
       const int j = (z * ny + y) * nx + x;
@@ -349,14 +390,16 @@ __device__ __forceinline__ T adj_raw(T q, T m, T a, T b, T k, bool strip) {
 }
 
 // Transpose of the E update: lambda_H += D^T (raw E-curl adjoints); the
-// E-phase adjoint memories advance into a.mn.
+// E-phase adjoint memories advance into a.mn.  Every load is issued without
+// a per-lane branch (clamped cell, masked value); the CPML memories are read
+// only behind wave-uniform strip tests, so interior tiles never touch them.
 template <typename T, int TZ>
 __global__ __launch_bounds__(512) void adj_e3(Args3<T> a) {
   const Geom3<T> &g = a.g;
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
   const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
-  const int s = tc.grp * G + (int)threadIdx.y;
+  const int s = tc.grp * G + __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one per wavefront
   if (s >= a.n_shots) return;
   const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
   const int lane = threadIdx.x;
@@ -380,6 +423,8 @@ __global__ __launch_bounds__(512) void adj_e3(Args3<T> a) {
   const bool xdom = x >= 1 && x < nx;
   const bool py = y < g.y0 || y >= g.y1, py1 = y + 1 < g.y0 || y + 1 >= g.y1;
   const bool px = x < g.x0 || x >= g.x1;
+  const int x_first = tc.xt * kOwn - 1;
+  const bool tile_px = x_first < g.x0 || x_first + 63 >= g.x1;  // some lane in an x strip
   const int zs = tc.zb * TZ;
   // y profiles at y and y + 1; x profiles at x (clamped for halo lanes)
   const int xc = min(max(x, 0), nx - 1);
@@ -388,85 +433,138 @@ __global__ __launch_bounds__(512) void adj_e3(Args3<T> a) {
   const T ay1 = g.Y(kA, yp), by1 = g.Y(kB, yp), ky1 = g.Y(kK, yp);
   const T ax = g.X(kA, xc), bx = g.X(kB, xc), kx = g.X(kK, xc);
 
-  // z-derivative raws at plane zz: (dHy/dz from Ex, +) and (dHx/dz from Ey, -)
-  auto z_raws = [&](int zz, T &r_yz, T &r_xz) {
-    r_yz = (T)0;
-    r_xz = (T)0;
-    if (zz < 1 || zz >= nz || y < 1 || !xdom) return;
-    const int j = (zz * ny + y) * nx + x;
-    const T cbv = cb[j];
-    const T q_yz = cbv * lex[j];
-    const T q_xz = -(cbv * ley[j]);
-    const bool pz = zz < g.z0 || zz >= g.z1;
-    const T az = g.Z(kA, zz), bz = g.Z(kB, zz), kz = g.Z(kK, zz);
-    r_yz = adj_raw(q_yz, pz ? m_hy_z[j] : (T)0, az, bz, kz, pz);
-    r_xz = adj_raw(q_xz, pz ? m_hx_z[j] : (T)0, az, bz, kz, pz);
+  // z-derivative raws at plane zz: (dHy/dz from Ex, +) and (dHx/dz from Ey, -);
+  // loads (z_load) kept apart from arithmetic (z_raws) so a later plane's
+  // loads can be issued early.
+  struct ZIn {
+    T cbv, l0, l1, m_yz, m_xz;
+  };
+  auto z_load = [&](int zz) {
+    ZIn v;
+    v.cbv = L(cb, zz, y);
+    v.l0 = L(lex, zz, y);
+    v.l1 = L(ley, zz, y);
+    v.m_yz = v.m_xz = (T)0;
+    if (zz < g.z0 || zz >= g.z1) {  // uniform
+      v.m_yz = L(m_hy_z, zz, y);
+      v.m_xz = L(m_hx_z, zz, y);
+    }
+    return v;
+  };
+  auto z_raws = [&](int zz, const ZIn &v, T &r_yz, T &r_xz) {
+    const bool in = zz >= 1 && zz < nz && y >= 1 && xdom;
+    const T q_yz = v.cbv * v.l0;
+    const T q_xz = -(v.cbv * v.l1);
+    const bool pz = zz < g.z0 || zz >= g.z1;  // uniform
+    const int zzc = min(max(zz, 0), nz - 1);
+    const T az = g.Z(kA, zzc), bz = g.Z(kB, zzc), kz = g.Z(kK, zzc);
+    r_yz = keep_or_zero(adj_raw(q_yz, v.m_yz, az, bz, kz, pz), in);
+    r_xz = keep_or_zero(adj_raw(q_xz, v.m_xz, az, bz, kz, pz), in);
+  };
+  // this cell and its y + 1 neighbour (plane z), loaded before any branch
+  struct PIn {
+    T l2, cbp, l0p, l2p, h0, h1, h2, m_zx, m_yx, m_zy, m_xy, m_zy1, m_xy1;
+  };
+  auto p_load = [&](int z) {
+    PIn v;
+    const int zc = min(z, nz - 1);
+    v.l2 = L(lez, zc, y);
+    v.cbp = L(cb, zc, y + 1);
+    v.l0p = L(lex, zc, y + 1);
+    v.l2p = L(lez, zc, y + 1);
+    v.h0 = L.at(lhx, zc, y);
+    v.h1 = L.at(lhy, zc, y);
+    v.h2 = L.at(lhz, zc, y);
+    v.m_zx = v.m_yx = v.m_zy = v.m_xy = v.m_zy1 = v.m_xy1 = (T)0;
+    if (tile_px) {
+      v.m_zx = keep_or_zero(L(m_hz_x, zc, y), px);
+      v.m_yx = keep_or_zero(L(m_hy_x, zc, y), px);
+    }
+    if (py) {
+      v.m_zy = L(m_hz_y, zc, y);
+      v.m_xy = L(m_hx_y, zc, y);
+    }
+    if (py1) {
+      v.m_zy1 = L(m_hz_y, zc, y + 1);
+      v.m_xy1 = L(m_hx_y, zc, y + 1);
+    }
+    return v;
   };
 
+  // Plane z + 1's loads are issued before plane z's stores (software
+  // pipelining: the compiler cannot move loads above stores that might
+  // alias them).
+  ZIn zc_in = z_load(zs), zn_in = z_load(zs + 1);
+  PIn p_in = p_load(zs);
   T ryz_c, rxz_c;
-  z_raws(zs, ryz_c, rxz_c);
+  z_raws(zs, zc_in, ryz_c, rxz_c);
 #pragma unroll
   for (int r = 0; r < TZ; ++r) {
     const int z = zs + r;
     if (z >= nz) break;
+    const ZIn zcur = zc_in, znext = zn_in;
+    const PIn pin = p_in;
+    if (r + 1 < TZ) {
+      zc_in = zn_in;
+      zn_in = z_load(z + 2);
+      p_in = p_load(z + 1);
+    }
     T ryz_n, rxz_n;
-    z_raws(z + 1, ryz_n, rxz_n);
+    z_raws(z + 1, znext, ryz_n, rxz_n);
     const bool zdom = z >= 1;
     const int j = (z * ny + y) * nx + x;
+    const bool pz = z < g.z0 || z >= g.z1;  // uniform
+    const T cbv = zcur.cbv, l0 = zcur.l0, l1 = zcur.l1, l2 = pin.l2;
+    const T cbp = pin.cbp, l0p = pin.l0p, l2p = pin.l2p;
+    const T h0 = pin.h0, h1 = pin.h1, h2 = pin.h2;
+    const T m_zx = pin.m_zx, m_yx = pin.m_yx, m_zy = pin.m_zy, m_xy = pin.m_xy;
+    const T m_zy1 = pin.m_zy1, m_xy1 = pin.m_xy1, m_yz = zcur.m_yz, m_xz = zcur.m_xz;
     // x-derivative raws at (z, y, x) for every lane (the left lane needs ours)
-    T r_zx = (T)0, r_yx = (T)0;
-    const T cbv = L(cb, z, y);
-    const T l0 = L(lex, z, y), l1 = L(ley, z, y), l2 = L(lez, z, y);
-    if (zdom && y >= 1 && xdom) {
-      const T q_zx = cbv * l1;
-      const T q_yx = -(cbv * l2);
-      r_zx = adj_raw(q_zx, px ? m_hz_x[j] : (T)0, ax, bx, kx, px);
-      r_yx = adj_raw(q_yx, px ? m_hy_x[j] : (T)0, ax, bx, kx, px);
-    }
+    const bool inx = zdom && y >= 1 && xdom;
+    const T r_zx = keep_or_zero(adj_raw(cbv * l1, m_zx, ax, bx, kx, px), inx);
+    const T r_yx = keep_or_zero(adj_raw(-(cbv * l2), m_yx, ax, bx, kx, px), inx);
     const T r_zx_xp = shfl_dn1(r_zx);
     const T r_yx_xp = shfl_dn1(r_yx);
     if (own && zdom) {
       // y-derivative raws at y and y + 1: (dHz/dy from Ex, -) and (dHx/dy from Ez, +)
       const T q_zy = -(cbv * l0);
       const T q_xy = cbv * l2;
-      const T r_zy = adj_raw(q_zy, py ? m_hz_y[j] : (T)0, ay, by, ky, py);
-      const T r_xy = adj_raw(q_xy, py ? m_hx_y[j] : (T)0, ay, by, ky, py);
+      const T r_zy = adj_raw(q_zy, m_zy, ay, by, ky, py);
+      const T r_xy = adj_raw(q_xy, m_xy, ay, by, ky, py);
       T r_zy_yp = (T)0, r_xy_yp = (T)0;
       if (y + 1 < ny) {
-        const int jp = j + nx;
-        const T cbp = cb[jp];
-        const T q_zy1 = -(cbp * lex[jp]);
-        const T q_xy1 = cbp * lez[jp];
-        r_zy_yp = adj_raw(q_zy1, py1 ? m_hz_y[jp] : (T)0, ay1, by1, ky1, py1);
-        r_xy_yp = adj_raw(q_xy1, py1 ? m_hx_y[jp] : (T)0, ay1, by1, ky1, py1);
+        const T q_zy1 = -(cbp * l0p);
+        const T q_xy1 = cbp * l2p;
+        r_zy_yp = adj_raw(q_zy1, m_zy1, ay1, by1, ky1, py1);
+        r_xy_yp = adj_raw(q_xy1, m_xy1, ay1, by1, ky1, py1);
       }
       // next E-phase memories at this cell
-      const bool pz = z < g.z0 || z >= g.z1;
       if (pz) {
         const T bz = g.Z(kB, z);
-        a.mn[M_HY_Z][base + j] = bz * m_hy_z[j] + cbv * l0;
-        a.mn[M_HX_Z][base + j] = bz * m_hx_z[j] + -(cbv * l1);
+        a.mn[M_HY_Z][base + j] = bz * m_yz + cbv * l0;
+        a.mn[M_HX_Z][base + j] = bz * m_xz + -(cbv * l1);
       }
       if (py) {
-        a.mn[M_HZ_Y][base + j] = by * m_hz_y[j] + -(cbv * l0);
-        a.mn[M_HX_Y][base + j] = by * m_hx_y[j] + cbv * l2;
+        a.mn[M_HZ_Y][base + j] = by * m_zy + -(cbv * l0);
+        a.mn[M_HX_Y][base + j] = by * m_xy + cbv * l2;
       }
       if (px) {
-        a.mn[M_HZ_X][base + j] = bx * m_hz_x[j] + cbv * l1;
-        a.mn[M_HY_X][base + j] = bx * m_hy_x[j] + -(cbv * l2);
+        a.mn[M_HZ_X][base + j] = bx * m_zx + cbv * l1;
+        a.mn[M_HY_X][base + j] = bx * m_yx + -(cbv * l2);
       }
-      T vx = lhx[j];
+      T vx = h0;
       vx = vx + (rxz_c - rxz_n) * g.rdz;
       vx = vx + (r_xy - r_xy_yp) * g.rdy;
-      T vy = lhy[j];
+      T vy = h1;
       vy = vy + (ryz_c - ryz_n) * g.rdz;
       vy = vy + (r_yx - r_yx_xp) * g.rdx;
-      T vz = lhz[j];
+      T vz = h2;
       vz = vz + (r_zy - r_zy_yp) * g.rdy;
       vz = vz + (r_zx - r_zx_xp) * g.rdx;
-      lhx[j] = vx;
-      lhy[j] = vy;
-      lhz[j] = vz;
+      const int64_t line = (int64_t)(z * ny + y) * nx;
+      *elem32(lhx + line, (uint32_t)x) = vx;
+      *elem32(lhy + line, (uint32_t)x) = vy;
+      *elem32(lhz + line, (uint32_t)x) = vz;
     }
     ryz_c = ryz_n;
     rxz_c = rxz_n;
@@ -476,13 +574,15 @@ __global__ __launch_bounds__(512) void adj_e3(Args3<T> a) {
 // Transpose of the H update: lambda_E = ca lambda_E + D^T (raw H-term
 // adjoints); H-phase adjoint memories advance into a.mn; then receivers
 // inject grad_r[t], sources read grad_f[t], and the coefficient gradient
-// accumulates against the stored forward state.
+// accumulates against the stored forward state.  Loads are issued as in
+// adj_e3 (no per-lane branch; memories behind wave-uniform strip tests; the
+// snapshots of the cell read before the owner's update).
 template <typename T, int TZ, int FMT>
 __global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
   extern __shared__ double red[];  // [2][G][TZ][64] when reducing
   const Geom3<T> &g = a.g;
   const int G = (int)blockDim.y;
-  const int w = (int)threadIdx.y;
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one per wavefront
   const int n_groups = (a.n_shots + G - 1) / G;
   const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
   const int s = tc.grp * G + w;
@@ -529,46 +629,133 @@ __global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
     const T ay = g.Y(kAh, y), by = g.Y(kBh, y), ky = g.Y(kKh, y);
     const T aym = g.Y(kAh, ymc), bym = g.Y(kBh, ymc), kym = g.Y(kKh, ymc);
     const bool pxh = x < g.x0 || x >= g.x1h;
+    const int x_first = tc.xt * kOwn - 1;
+    const bool tile_pxh = x_first < g.x0 || x_first + 63 >= g.x1h;  // some lane in an x strip
     const T ax = g.X(kAh, xc), bx = g.X(kBh, xc), kx = g.X(kKh, xc);
 
     // z-derivative raws at plane zz (cell (zz, y, x)):
-    //   Hx term dEy/dz (first, -cq lHx) and Hy term dEx/dz (second, +cq lHy)
-    auto z_raws = [&](int zz, T &r_yz, T &r_xz) {
-      r_yz = (T)0;
-      r_xz = (T)0;
-      if (zz < 1 || zz >= nz - 1 || y < 1 || !xdom) return;
-      const int j = (zz * ny + y) * nx + x;
-      const T cqv = cq[j];
-      const T adj_yz = -cqv * lhx[j];
-      const T adj_xz = cqv * lhy[j];
-      const bool pz = zz < g.z0 || zz >= g.z1h;
-      const T az = g.Z(kAh, zz), bz = g.Z(kBh, zz), kz = g.Z(kKh, zz);
-      r_yz = adj_raw(adj_yz, pz ? m_ey_z[j] : (T)0, az, bz, kz, pz);
-      r_xz = adj_raw(adj_xz, pz ? m_ex_z[j] : (T)0, az, bz, kz, pz);
+    //   Hx term dEy/dz (first, -cq lHx) and Hy term dEx/dz (second, +cq lHy);
+    // loads (z_load) kept apart from arithmetic (z_raws).
+    struct ZIn {
+      T cqv, h0, h1, m_yz, m_xz;
+    };
+    auto z_load = [&](int zz) {
+      ZIn v;
+      v.cqv = L(cq, zz, y);
+      v.h0 = L(lhx, zz, y);
+      v.h1 = L(lhy, zz, y);
+      v.m_yz = v.m_xz = (T)0;
+      if (zz < g.z0 || zz >= g.z1h) {  // uniform
+        v.m_yz = L(m_ey_z, zz, y);
+        v.m_xz = L(m_ex_z, zz, y);
+      }
+      return v;
+    };
+    auto z_raws = [&](int zz, const ZIn &v, T &r_yz, T &r_xz) {
+      const bool in = zz >= 1 && zz < nz - 1 && y >= 1 && xdom;
+      const T adj_yz = -v.cqv * v.h0;
+      const T adj_xz = v.cqv * v.h1;
+      const bool pz = zz < g.z0 || zz >= g.z1h;  // uniform
+      const int zzc = min(max(zz, 0), nz - 1);
+      const T az = g.Z(kAh, zzc), bz = g.Z(kBh, zzc), kz = g.Z(kKh, zzc);
+      r_yz = keep_or_zero(adj_raw(adj_yz, v.m_yz, az, bz, kz, pz), in);
+      r_xz = keep_or_zero(adj_raw(adj_xz, v.m_xz, az, bz, kz, pz), in);
+    };
+    // the rest of plane z: this cell, its y - 1 neighbour, memories and
+    // snapshots, loaded before any branch
+    struct PIn {
+      T h2, cqm, h0m, h2m, cav, e0, e1, e2, m_zx, m_yx, m_zy, m_xy, m_zym, m_xym;
+      float sa[3], sb[3];
+    };
+    auto p_load = [&](int z) {
+      PIn v;
+      const int zc = min(z, nz - 1);
+      const int64_t line = (int64_t)(zc * ny + y) * nx;
+      v.h2 = L(lhz, zc, y);
+      v.cqm = L(cq, zc, ym);
+      v.h0m = L(lhx, zc, ym);
+      v.h2m = L(lhz, zc, ym);
+      v.cav = L.at(ca, zc, y);
+      v.e0 = L.at(lex, zc, y);
+      v.e1 = L.at(ley, zc, y);
+      v.e2 = L.at(lez, zc, y);
+      v.m_zx = v.m_yx = v.m_zy = v.m_xy = v.m_zym = v.m_xym = (T)0;
+      if (tile_pxh) {
+        v.m_zx = keep_or_zero(L(m_ez_x, zc, y), pxh);
+        v.m_yx = keep_or_zero(L(m_ey_x, zc, y), pxh);
+      }
+      if (pyh) {
+        v.m_zy = L(m_ez_y, zc, y);
+        v.m_xy = L(m_ex_y, zc, y);
+      }
+      if (pyh_m) {
+        v.m_zym = L(m_ez_y, zc, ym);
+        v.m_xym = L(m_ex_y, zc, ym);
+      }
+      // snapshots of this cell (forward state before step t), when wanted;
+      // exact in float (fp32 or BF16 payloads of fp32 fields)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v.sa[c] = v.sb[c] = 0.0f;
+      if constexpr (FMT != 0 && sizeof(T) == 4) {
+        if (want_grad) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const void *const es = a.st[c], *const cs = a.st[3 + c];
+            if (a.gca && es) v.sa[c] = (float)get_store32<T, FMT>(snap_plane<T, FMT>(es, base + line), (uint32_t)xc);
+            if (a.gcb && cs) v.sb[c] = (float)get_store32<T, FMT>(snap_plane<T, FMT>(cs, base + line), (uint32_t)xc);
+          }
+        }
+      }
+      return v;
     };
 
+    // Plane z + 1's loads are issued before plane z's stores.
+    ZIn z_in = z_load(zs);
+    PIn p_in = p_load(zs);
     T ryz_p, rxz_p;
-    z_raws(zs - 1, ryz_p, rxz_p);
-    float se[3][TZ], sc[3][TZ];
+    z_raws(zs - 1, z_load(zs - 1), ryz_p, rxz_p);
 #pragma unroll
     for (int r = 0; r < TZ; ++r) {
       const int z = zs + r;
       if (z >= nz) break;
+      const ZIn zcur = z_in;
+      const PIn pin = p_in;
+      if (r + 1 < TZ) {
+        z_in = z_load(z + 1);
+        p_in = p_load(z + 1);
+      }
       const bool zdom = z >= 1;
       const int j = (z * ny + y) * nx + x;
+      const int64_t line = (int64_t)(z * ny + y) * nx;
       T ryz_c, rxz_c;
-      z_raws(z, ryz_c, rxz_c);
+      z_raws(z, zcur, ryz_c, rxz_c);
+      const bool pzh = z < g.z0 || z >= g.z1h;  // uniform
+      const T cqv = zcur.cqv, h0 = zcur.h0, h1 = zcur.h1, h2 = pin.h2;
+      const T cqm = pin.cqm, h0m = pin.h0m, h2m = pin.h2m;
+      const T cav = pin.cav, e0 = pin.e0, e1 = pin.e1, e2 = pin.e2;
+      const T m_zx = pin.m_zx, m_yx = pin.m_yx, m_zy = pin.m_zy, m_xy = pin.m_xy;
+      const T m_zym = pin.m_zym, m_xym = pin.m_xym, m_yz = zcur.m_yz, m_xz = zcur.m_xz;
+      double sa[3], sb[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sa[c] = (double)pin.sa[c];
+        sb[c] = (double)pin.sb[c];
+      }
+      if constexpr (sizeof(T) == 8 && FMT != 0) {  // fp64 payloads: read at use
+        if (want_grad) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const void *const es = a.st[c], *const cs = a.st[3 + c];
+            if (a.gca && es) sa[c] = get_store32<T, FMT>(snap_plane<T, FMT>(es, base + line), (uint32_t)xc);
+            if (a.gcb && cs) sb[c] = get_store32<T, FMT>(snap_plane<T, FMT>(cs, base + line), (uint32_t)xc);
+          }
+        }
+      }
       // x-derivative raws at (z, y, x): Hy term dEz/dx (first, -cq lHy) and
       // Hz term dEy/dx (second, +cq lHz); the right lane needs ours
-      T r_zx = (T)0, r_yx = (T)0;
-      const T cqv = L(cq, z, y);
-      const T h0 = L(lhx, z, y), h1 = L(lhy, z, y), h2 = L(lhz, z, y);
-      if (zdom && y >= 1 && xdom && has_dx) {
-        const T adj_zx = -cqv * h1;
-        const T adj_yx = cqv * h2;
-        r_zx = adj_raw(adj_zx, pxh ? m_ez_x[j] : (T)0, ax, bx, kx, pxh);
-        r_yx = adj_raw(adj_yx, pxh ? m_ey_x[j] : (T)0, ax, bx, kx, pxh);
-      }
+      const bool inx = zdom && y >= 1 && xdom && has_dx;
+      const T r_zx = keep_or_zero(adj_raw(-cqv * h1, m_zx, ax, bx, kx, pxh), inx);
+      const T r_yx = keep_or_zero(adj_raw(cqv * h2, m_yx, ax, bx, kx, pxh), inx);
       const T r_zx_xm = shfl_up1(r_zx);
       const T r_yx_xm = shfl_up1(r_yx);
       if (own && zdom) {
@@ -578,48 +765,44 @@ __global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
         if (y < ny - 1) {
           const T adj_zy = cqv * h0;
           const T adj_xy = -cqv * h2;
-          r_zy = adj_raw(adj_zy, pyh ? m_ez_y[j] : (T)0, ay, by, ky, pyh);
-          r_xy = adj_raw(adj_xy, pyh ? m_ex_y[j] : (T)0, ay, by, ky, pyh);
+          r_zy = adj_raw(adj_zy, m_zy, ay, by, ky, pyh);
+          r_xy = adj_raw(adj_xy, m_xy, ay, by, ky, pyh);
         }
         T r_zy_ym = (T)0, r_xy_ym = (T)0;
         if (ym >= 1) {
-          const int jm = j - nx;
-          const T cqm = cq[jm];
-          const T adj_zy1 = cqm * lhx[jm];
-          const T adj_xy1 = -cqm * lhz[jm];
-          r_zy_ym = adj_raw(adj_zy1, pyh_m ? m_ez_y[jm] : (T)0, aym, bym, kym, pyh_m);
-          r_xy_ym = adj_raw(adj_xy1, pyh_m ? m_ex_y[jm] : (T)0, aym, bym, kym, pyh_m);
+          const T adj_zy1 = cqm * h0m;
+          const T adj_xy1 = -cqm * h2m;
+          r_zy_ym = adj_raw(adj_zy1, m_zym, aym, bym, kym, pyh_m);
+          r_xy_ym = adj_raw(adj_xy1, m_xym, aym, bym, kym, pyh_m);
         }
         const T rzx_m = x - 1 >= 1 ? r_zx_xm : (T)0;
         const T ryx_m = x - 1 >= 1 ? r_yx_xm : (T)0;
         // next H-phase memories at this cell
-        const bool pzh = z < g.z0 || z >= g.z1h;
         if (z < nz - 1 && pzh) {
           const T bz = g.Z(kBh, z);
-          a.mn[M_EY_Z][base + j] = bz * m_ey_z[j] + -cqv * h0;
-          a.mn[M_EX_Z][base + j] = bz * m_ex_z[j] + cqv * h1;
+          a.mn[M_EY_Z][base + j] = bz * m_yz + -cqv * h0;
+          a.mn[M_EX_Z][base + j] = bz * m_xz + cqv * h1;
         }
         if (y < ny - 1 && pyh) {
-          a.mn[M_EZ_Y][base + j] = by * m_ez_y[j] + cqv * h0;
-          a.mn[M_EX_Y][base + j] = by * m_ex_y[j] + -cqv * h2;
+          a.mn[M_EZ_Y][base + j] = by * m_zy + cqv * h0;
+          a.mn[M_EX_Y][base + j] = by * m_xy + -cqv * h2;
         }
         if (has_dx && pxh) {
-          a.mn[M_EZ_X][base + j] = bx * m_ez_x[j] + -cqv * h1;
-          a.mn[M_EY_X][base + j] = bx * m_ey_x[j] + cqv * h2;
+          a.mn[M_EZ_X][base + j] = bx * m_zx + -cqv * h1;
+          a.mn[M_EY_X][base + j] = bx * m_yx + cqv * h2;
         }
-        const T cav = ca[j];
         const T tx = (rxz_p - rxz_c) * g.rdz + (r_xy_ym - r_xy) * g.rdy;
         const T ty = (ryz_p - ryz_c) * g.rdz + (ryx_m - r_yx) * g.rdx;
         const T tz = (r_zy_ym - r_zy) * g.rdy + (rzx_m - r_zx) * g.rdx;
-        T v0 = cav * lex[j] + tx;
-        T v1 = cav * ley[j] + ty;
-        T v2 = cav * lez[j] + tz;
+        T v0 = cav * e0 + tx;
+        T v1 = cav * e1 + ty;
+        T v2 = cav * e2 + tz;
         // receivers inject grad_r[t]; sources read grad_f[t]
         const int slot = a.pts.row_slot[z * ny + y];
         if (slot >= 0) {
           const int64_t si = ((int64_t)slot * a.n_shots + s) * a.pts.n_xt + tc.xt;
-          const int e0 = a.pts.seg[si], e1 = a.pts.seg[si + 1];
-          for (int e = e0; e < e1; ++e) {
+          const int p0 = a.pts.seg[si], p1 = a.pts.seg[si + 1];
+          for (int e = p0; e < p1; ++e) {
             const int2 en = a.pts.ent[e];
             if ((en.x >> 8) != 1 || (en.x & 0xff) != lane) continue;
             const T v = a.src_amp[en.y];
@@ -628,27 +811,21 @@ __global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
             else v2 += v;
           }
           if (a.rec_out != nullptr) {
-            for (int e = e0; e < e1; ++e) {
+            for (int e = p0; e < p1; ++e) {
               const int2 en = a.pts.ent[e];
               if ((en.x >> 8) != 0 || (en.x & 0xff) != lane) continue;
               a.rec_out[en.y] = a.src_comp == 0 ? v0 : (a.src_comp == 1 ? v1 : v2);
             }
           }
         }
-        lex[j] = v0;
-        ley[j] = v1;
-        lez[j] = v2;
+        *elem32(lex + line, (uint32_t)x) = v0;
+        *elem32(ley + line, (uint32_t)x) = v1;
+        *elem32(lez + line, (uint32_t)x) = v2;
         if (want_grad) {
           const int64_t i = base + j;
           double pa = 0.0, pb = 0.0;
-          if (a.gca)
-            pa = ((double)v0 * get_store<T, FMT>(a.st[0], i) +
-                  (double)v1 * get_store<T, FMT>(a.st[1], i) +
-                  (double)v2 * get_store<T, FMT>(a.st[2], i)) * a.step_scale;
-          if (a.gcb)
-            pb = ((double)v0 * get_store<T, FMT>(a.st[3], i) +
-                  (double)v1 * get_store<T, FMT>(a.st[4], i) +
-                  (double)v2 * get_store<T, FMT>(a.st[5], i)) * a.step_scale;
+          if (a.gca) pa = ((double)v0 * sa[0] + (double)v1 * sa[1] + (double)v2 * sa[2]) * a.step_scale;
+          if (a.gcb) pb = ((double)v0 * sb[0] + (double)v1 * sb[1] + (double)v2 * sb[2]) * a.step_scale;
           if (reduce) {
             red_ca[r * 64 + lane] = pa;
             red_cb[r * 64 + lane] = pb;
@@ -661,8 +838,6 @@ __global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
       ryz_p = ryz_c;
       rxz_p = rxz_c;
     }
-    (void)se;
-    (void)sc;
   }
   if (!reduce) return;  // uniform over the workgroup
   __syncthreads();
