@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # (scaled by a 1 GiB copy's counters) for the kernels as built, default
 # launch shapes; see profiles/tm2d_config2_hbm_counters.md.  Reported as
 # roofline.traffic.
-MEASURED_TRAFFIC_B_PER_CELL = {"fwd2_step": 31.02, "adj2_step": 32.53}
+MEASURED_TRAFFIC_B_PER_CELL = {"fwd2_step": 32.40, "adj2_step": 36.50}
 
 
 def log(msg: str) -> None:
@@ -72,6 +72,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--batched-seed", type=int, default=3030)
     ap.add_argument("--no-config1", action="store_true",
                     help="skip the small launch-latency-bound forward measurement")
+    ap.add_argument("--em3d-nt", type=int, default=100,
+                    help="time steps of the 3-D measurement (0: skip)")
+    ap.add_argument("--em3d-shots", type=int, default=4)
     return ap.parse_args()
 
 
@@ -363,6 +366,69 @@ def config1_bench(dev, reps: int = 20) -> dict:
     }
 
 
+def em3d_bench(dev, nt: int, shots: int) -> dict:
+    """3-D Maxwell on the 256^3 geometry (281^3 padded, CPML 12, dx 0.02 m,
+    dt at CFL 0.9), reduced nt and shot count so the run stays short: seeded
+    8-layer eps_r 1-25, sigma U[0, 0.01], x-directed dipole sources (300 MHz
+    Ricker) near the top face, 32 x 32 receivers (Ex) on it.  Forward alone,
+    then forward + adjoint gradient (snapshots kept whole; they fit at this
+    nt); useful work = the forward and adjoint sweeps."""
+    import torch
+
+    import tide
+    from tide.maxwell.maxwell3d import maxwell3d
+
+    n, pml, dx = 256, 12, 0.02
+    dt = 0.9 * dx / (3**0.5 * tide.C0)
+    g = torch.Generator().manual_seed(5005)
+    eps = torch.ones(n, n, n)
+    z = 0
+    for _ in range(8):
+        th = int(torch.randint(16, 48, (1,), generator=g))
+        eps[z:z + th] = 1 + 24 * float(torch.rand(1, generator=g))
+        z += th
+    eps = eps.to(dev)
+    sig = (0.01 * torch.rand(n, n, n, generator=g)).to(dev)
+    mu = torch.ones(n, n, n, device=dev)
+    amp = tide.ricker(3e8, nt, dt, dtype=torch.float32, device=dev).reshape(1, 1, -1).repeat(shots, 1, 1)
+    src = torch.stack([torch.tensor([[2, 64 + 32 * (k % 4), 64 + 32 * (k // 4)]]) for k in range(shots)])
+    yy, xx = torch.meshgrid(torch.arange(32) * 8 + 2, torch.arange(32) * 8 + 2, indexing="ij")
+    rec = torch.stack([torch.full_like(yy, 2), yy, xx], -1).reshape(1, -1, 3).repeat(shots, 1, 1)
+    kw = dict(grid_spacing=dx, dt=dt, source_amplitude=amp, source_location=src,
+              receiver_location=rec, pml_width=pml, source_component="ex", receiver_component="ex")
+    updates = float((n + 2 * pml + 1) ** 3 * shots * nt)
+    with torch.no_grad():
+        maxwell3d(eps, sig, mu, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        maxwell3d(eps, sig, mu, **kw)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+    e = eps.clone().requires_grad_(True)
+    s = sig.clone().requires_grad_(True)
+
+    def gradient():
+        e.grad = None
+        s.grad = None
+        r = maxwell3d(e, s, mu, **kw)[-1]
+        (0.5 * (r.double() ** 2).sum()).backward()
+
+    gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gradient()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    return {
+        "workload": f"3D Maxwell 256^3 (281^3 padded, CPML 12), {shots} shots x 1024 receivers, "
+                    f"nt {nt}, fp32, fp32 snapshots",
+        "forward": {"seconds": round(t_fwd, 3), "gcell_per_s": round(updates / t_fwd / 1e9, 2),
+                    "hbm_frac_48B": round(updates / t_fwd * 48 / 1e9 / HBM_PEAK_GBS, 4)},
+        "gradient": {"seconds": round(t_grad, 3), "gcell_per_s": round(2 * updates / t_grad / 1e9, 2),
+                     "hbm_frac_72B": round(2 * updates / t_grad * 72 / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+
+
 def main() -> None:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -522,6 +588,9 @@ def main() -> None:
     config1 = None if args.no_config1 else config1_bench(dev)
     if config1 is not None:
         log(f"config 1: {config1}")
+    em3d = em3d_bench(dev, args.em3d_nt, args.em3d_shots) if args.em3d_nt > 0 else None
+    if em3d is not None:
+        log(f"3-D: {em3d}")
     out = {
         "metric": METRIC,
         "value": round(gcells, 3),
@@ -556,6 +625,7 @@ def main() -> None:
         "serving": serving,
         "batched": batched,
         "config1": config1,
+        "em3d": em3d,
     }
     print(json.dumps(out), flush=True)
 
