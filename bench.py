@@ -465,6 +465,9 @@ def main() -> None:
     sizes = [base + (1 if i < extra else 0) for i in range(n_chunks)]
     bounds = [sum(sizes[:i]) for i in range(n_chunks + 1)]
 
+    from tide.autotune import _install, current
+
+    default_shapes = current()  # the library's defaults, restored for the other workloads
     tuning = None
     if not args.no_autotune:
         from tide.autotune import tune_tm2d
@@ -472,7 +475,6 @@ def main() -> None:
         log(f"autotuned launch shapes forward {tuning['forward']} adjoint {tuning['adjoint']} "
             f"in {tuning.get('seconds', 0)} s")
     else:
-        from tide.autotune import current
         fwd, adj = current()
         tuning = {"forward": fwd, "adjoint": adj, "cached": False}
 
@@ -581,6 +583,9 @@ def main() -> None:
     log("timing the CPU baseline" if not args.no_cpu_baseline else "CPU baseline skipped")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, setup_kwargs)
     del ops, observed, e, s, setup_kwargs
+    # the shapes tuned above fit this workload's grid and shot chunks; the
+    # mixed-size workloads below run with the library's defaults
+    _install(*default_shapes)
     serving = None
     if args.serve_requests > 0:
         serving = [serve_bench(args, dev, float(r)) for r in args.serve_rates.split(",")]

@@ -332,7 +332,7 @@ int tide_tm2d_plan_destroy(void *plan);
  */
 typedef struct tide_tm2d_job {
   tide_tm2d_problem p;       /* fields[0..4] / lambda: parity-0 buffers, zero at start */
-  float *fields_b[5];        /* parity-1 ey hx hz m_ey_x m_ey_z, zero at start */
+  float *fields_b[7];        /* parity-1 copies of fields[7] (same order), zero at start */
   float *lambda_b[7];        /* parity-1 adjoint state, same order as lambda, zero at start */
   double *grad_ca_groups;    /* [ceil(n_shots/group), ny, nx] zeroed FP64, when n_shots > group */
   double *grad_cb_groups;

@@ -7,7 +7,11 @@
 // sorted by the steps they take in this tick (longest first), so the jobs
 // still running at step j of the tick own a prefix of the flat tile range,
 // and a workgroup finds its job by a binary search over the jobs' first
-// tiles.  The job's phase picks the forward step or the adjoint iteration
+// tiles.  With two steps per launch (the default time block), each launch
+// advances the jobs by two steps of their phase with the two-step tiles
+// (tm2d_fused.h); a job whose steps in the tick are odd takes its last one
+// in a one-step launch over its own tiles.  The host tracks which state
+// buffer of each job is current.  The job's phase picks the forward step or the adjoint iteration
 // (the single-problem tile code), so forward-only solves, forward passes of
 // gradient solves and adjoints all share the same launches.
 //
@@ -33,15 +37,21 @@ struct ServeItem {
   int32_t phase;  // 0 forward, 2 adjoint
   int32_t fmt;    // snapshot payload: 0 none, 1 fp32, 2 BF16
   int32_t k0;     // step of the phase at the start of the tick
+  int32_t par0;   // two-step ticks: current state buffer at the start of the tick
+  int32_t epar0;  // two-step ticks: current E-phase memory copy (forward)
   int32_t pad;
 };
 
-// n: jobs active at this step (a prefix of the tick's job list).
+// n: jobs active at this step (a prefix of the tick's job list); the
+// workgroups cover flat tiles [first, first + gridDim.x).  pair_mode: the
+// tick runs two-step launches, so the state parity comes from par0 / epar0
+// and the pairs done before tick step j, else from the step index.
 template <int TYF, int TYA>
 __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ items,
-                                                  const int32_t *__restrict__ start, int n, int j) {
+                                                  const int32_t *__restrict__ start, int n, int j,
+                                                  int first, int pair_mode) {
   extern __shared__ double red[];
-  const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  const int64_t flat = first + xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
   int lo = 0, hi = n - 1;  // largest i with start[i] <= flat
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -53,6 +63,8 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
   const ServeItem &it = items[lo];
   const int k = it.k0 + j;
   const int local = (int)(flat - start[lo]);
+  const int flip = (j >> 1) & 1;
+  const int par = pair_mode ? it.par0 ^ flip : (k & 1), epar = pair_mode ? it.epar0 ^ flip : 0;
   if (it.b.n_shots == 1 && blockDim.y > 1) {
     // A single-shot job gives each wavefront of the workgroup its own tile
     // (no idle wavefronts, no cross-wavefront gradient sum).
@@ -61,7 +73,7 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
     if (it.phase == 0) {
       if (tile >= n_xt * it.b.n_yt) return;  // uniform over the wavefront; no barriers here
       const int yt = tile / n_xt, xt = tile - yt * n_xt;
-      const FwdArgs<float> a = item_fwd_args(it.b, k);
+      const FwdArgs<float> a = item_fwd_args(it.b, k, par, epar);
       const bool inner = interior_tile<float, TYF>(a.g, xt, yt);
       if (it.fmt == 0)
         inner ? fwd_tile<float, TYF, 0, false>(a, 0, xt, yt) : fwd_tile<float, TYF, 0, true>(a, 0, xt, yt);
@@ -72,7 +84,7 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
     } else {
       if (tile >= n_xt * it.b.n_yt) return;
       const int yt = tile / n_xt, xt = tile - yt * n_xt;
-      const AdjArgs<float> a = item_adj_args(it.b, k);
+      const AdjArgs<float> a = item_adj_args(it.b, k, par);
       const bool inner = interior_tile<float, TYA>(a.g, xt, yt);
       if (it.fmt == 2) {
         if (inner)
@@ -90,7 +102,7 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
   }
   const TileCoord tc = batch_tile(local, it.b.pts.n_xt, it.b.n_yt);
   if (it.phase == 0) {
-    const FwdArgs<float> a = item_fwd_args(it.b, k);
+    const FwdArgs<float> a = item_fwd_args(it.b, k, par, epar);
     if (it.fmt == 0)
       fwd_workgroup<float, TYF, 0>(a, tc);
     else if (it.fmt == 2)
@@ -98,11 +110,88 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
     else
       fwd_workgroup<float, TYF, 1>(a, tc);
   } else {
-    const AdjArgs<float> a = item_adj_args(it.b, k);
+    const AdjArgs<float> a = item_adj_args(it.b, k, par);
     if (it.fmt == 2)
       adj_workgroup<float, TYA, 2>(a, tc, red);
     else
       adj_workgroup<float, TYA, 1>(a, tc, red);
+  }
+}
+
+constexpr int kServeRows2 = 4;  // tile rows of the two-step engine launches
+
+// Two-step launch of pair q of the tick: jobs [0, n) advance steps
+// k0 + 2q and k0 + 2q + 1 of their phase.
+template <int TY>
+__global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ items,
+                                                  const int32_t *__restrict__ start, int n, int q) {
+  extern __shared__ double red[];
+  const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  int lo = 0, hi = n - 1;  // largest i with start[i] <= flat
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= flat)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const ServeItem &it = items[lo];
+  const int k = it.k0 + 2 * q;
+  const int par = it.par0 ^ (q & 1);
+  const int local = (int)(flat - start[lo]);
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  if (it.b.n_shots == 1 && blockDim.y > 1) {
+    // single-shot job: each wavefront its own tile; the gradient goes
+    // straight to the job's array (its tiles own disjoint cells)
+    const int n_xt = it.b.pts2.n_xt;
+    const int tile = local * (int)blockDim.y + w;
+    if (tile >= n_xt * it.b.n_yt2) return;  // uniform over the wavefront; no barriers here
+    const int yt = tile / n_xt, xt = tile - yt * n_xt;
+    if (it.phase == 0) {
+      const Fwd2Args<float> a = item_fwd2_args(it.b, k, par);
+      const bool inner = in_split<TY>(a.split, xt, yt);
+      if (it.fmt == 0)
+        inner ? fwd2_tile<float, TY, 0, 0>(a, 0, xt, yt) : fwd2_tile<float, TY, 0, 3>(a, 0, xt, yt);
+      else if (it.fmt == 2)
+        inner ? fwd2_tile<float, TY, 2, 0>(a, 0, xt, yt) : fwd2_tile<float, TY, 2, 3>(a, 0, xt, yt);
+      else
+        inner ? fwd2_tile<float, TY, 1, 0>(a, 0, xt, yt) : fwd2_tile<float, TY, 1, 3>(a, 0, xt, yt);
+    } else {
+      const Adj2Args<float> a = item_adj2_args(it.b, k, par);
+      const bool inner = in_split<TY>(a.split, xt, yt);
+      if (it.fmt == 2) {
+        if (inner)
+          adj2_tile<float, TY, 2, 0>(a, 0, xt, yt, nullptr, nullptr, false);
+        else
+          adj2_tile<float, TY, 2, 3>(a, 0, xt, yt, nullptr, nullptr, false);
+      } else {
+        if (inner)
+          adj2_tile<float, TY, 1, 0>(a, 0, xt, yt, nullptr, nullptr, false);
+        else
+          adj2_tile<float, TY, 1, 3>(a, 0, xt, yt, nullptr, nullptr, false);
+      }
+    }
+    return;
+  }
+  const TileCoord tc = batch_tile(local, it.b.pts2.n_xt, it.b.n_yt2);
+  if (it.phase == 0) {
+    const Fwd2Args<float> a = item_fwd2_args(it.b, k, par);
+    const int s = tc.grp * (int)blockDim.y + w;
+    if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
+    const bool inner = in_split<TY>(a.split, tc.xt, tc.yt);
+    if (it.fmt == 0)
+      inner ? fwd2_tile<float, TY, 0, 0>(a, s, tc.xt, tc.yt) : fwd2_tile<float, TY, 0, 3>(a, s, tc.xt, tc.yt);
+    else if (it.fmt == 2)
+      inner ? fwd2_tile<float, TY, 2, 0>(a, s, tc.xt, tc.yt) : fwd2_tile<float, TY, 2, 3>(a, s, tc.xt, tc.yt);
+    else
+      inner ? fwd2_tile<float, TY, 1, 0>(a, s, tc.xt, tc.yt) : fwd2_tile<float, TY, 1, 3>(a, s, tc.xt, tc.yt);
+  } else {
+    const Adj2Args<float> a = item_adj2_args(it.b, k, par);
+    const int edge = in_split<TY>(a.split, tc.xt, tc.yt) ? 0 : 3;  // uniform over the workgroup
+    if (it.fmt == 2)
+      adj2_workgroup<float, TY, 2, 1 | 8>(a, tc, edge, red);
+    else
+      adj2_workgroup<float, TY, 1, 1 | 8>(a, tc, edge, red);
   }
 }
 
@@ -115,7 +204,8 @@ struct EJob {
   int64_t id = 0;
   tide_tm2d_job spec;
   BatchItem<float> item;
-  PointTable table;
+  PointTable table, table2;  // one-step tiles; two-step tiles (with halo copies)
+  int par = 0, epar = 0;     // current state buffer / E-phase memory copy (two-step engines)
   char *block = nullptr;  // packed profiles + point table (device)
   size_t block_bytes = 0;
   bool uploaded = false;
@@ -138,11 +228,12 @@ struct Engine {
   int device = 0;
   int group = 4;
   int rows_f = 4, rows_a = 2;
+  bool two_step = false;  // launches of two steps (set from the time block at creation)
   int64_t max_jobs = 64;
   std::map<int64_t, EJob> jobs;
   int64_t next_id = 1;
   ServeItem *d_items = nullptr;
-  int32_t *d_start = nullptr;
+  int32_t *d_start = nullptr, *d_start2 = nullptr;
   Slot slots[2];
   int cur = 0;
   std::vector<int64_t> done;  // jobs whose phase ended since the last report
@@ -163,17 +254,24 @@ void pick_rows(int rows_f, int rows_a, int *out_f, int *out_a) {
   }
 }
 
-void launch_step(const Engine &e, int64_t grid, int n, int j, hipStream_t s) {
+void launch_step(const Engine &e, int64_t grid, int n, int j, hipStream_t s, int first = 0,
+                 int pair_mode = 0) {
   const dim3 block(64, (unsigned)e.group);
   const size_t lds = 2 * (size_t)e.group * e.rows_a * 64 * sizeof(double);
   if (e.rows_f == 8 && e.rows_a == 4)
-    serve_step<8, 4><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j);
+    serve_step<8, 4><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j, first, pair_mode);
   else if (e.rows_f == 2 && e.rows_a == 2)
-    serve_step<2, 2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j);
+    serve_step<2, 2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j, first, pair_mode);
   else if (e.rows_f == 4 && e.rows_a == 4)
-    serve_step<4, 4><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j);
+    serve_step<4, 4><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j, first, pair_mode);
   else
-    serve_step<4, 2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j);
+    serve_step<4, 2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j, first, pair_mode);
+}
+
+void launch_pair(const Engine &e, int64_t grid, int n, int q, hipStream_t s) {
+  const dim3 block(64, (unsigned)e.group);
+  const size_t lds = 2 * (size_t)e.group * kServeRows2 * 64 * sizeof(double);
+  serve_pair<kServeRows2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start2, n, q);
 }
 
 int slot_reserve(Slot &sl, size_t bytes) {
@@ -191,12 +289,15 @@ int slot_reserve(Slot &sl, size_t bytes) {
   return kOk;
 }
 
+size_t point_bytes(const PointTable &t) {
+  return align_up(t.row_slot.size() * sizeof(int32_t)) + align_up(t.seg.size() * sizeof(int32_t)) +
+         align_up(std::max<size_t>(1, t.ent.size()) * sizeof(int2));
+}
+
 size_t table_bytes(const EJob &j) {
   const int64_t ny = j.spec.p.ny, nx = j.spec.p.nx;
-  return align_up(6 * (size_t)(ny + nx) * sizeof(float)) +
-         align_up(j.table.row_slot.size() * sizeof(int32_t)) +
-         align_up(j.table.seg.size() * sizeof(int32_t)) +
-         align_up(std::max<size_t>(1, j.table.ent.size()) * sizeof(int2));
+  return align_up(6 * (size_t)(ny + nx) * sizeof(float)) + point_bytes(j.table) +
+         (j.table2.row_slot.empty() ? 0 : point_bytes(j.table2));
 }
 
 // Allocates the job's device block and enqueues its profile and table
@@ -228,19 +329,27 @@ int upload_job(EJob &j, char *stage, size_t *staged, hipStream_t s) {
     off += align_up(bytes);
     return kOk;
   };
-  char *row_slot, *seg, *ent;
-  int rc = put(j.table.row_slot.data(), j.table.row_slot.size() * sizeof(int32_t), &row_slot);
-  if (rc == kOk) rc = put(j.table.seg.data(), j.table.seg.size() * sizeof(int32_t), &seg);
   const int2 zero = make_int2(0, 0);
-  if (rc == kOk)
-    rc = j.table.ent.empty() ? put(&zero, sizeof(int2), &ent)
-                             : put(j.table.ent.data(), j.table.ent.size() * sizeof(int2), &ent);
-  if (rc != kOk) return rc;
+  auto put_table = [&](const PointTable &t, Points *out) -> int {
+    char *row_slot, *seg, *ent;
+    int rc = put(t.row_slot.data(), t.row_slot.size() * sizeof(int32_t), &row_slot);
+    if (rc == kOk) rc = put(t.seg.data(), t.seg.size() * sizeof(int32_t), &seg);
+    if (rc == kOk)
+      rc = t.ent.empty() ? put(&zero, sizeof(int2), &ent) : put(t.ent.data(), t.ent.size() * sizeof(int2), &ent);
+    *out = Points{(const int32_t *)row_slot, (const int32_t *)seg, (const int2 *)ent, t.n_xt};
+    return rc;
+  };
+  if (int rc = put_table(j.table, &j.item.pts); rc != kOk) return rc;
+  if (!j.table2.row_slot.empty()) {
+    if (int rc = put_table(j.table2, &j.item.pts2); rc != kOk) return rc;
+    j.item.n_yt2 = (int32_t)((q.ny + kServeRows2 - 1) / kServeRows2);
+    j.item.split2 = make_split(j.item.g, j.item.pts2.n_xt, j.item.n_yt2, kServeRows2);
+  }
   j.item.g.py = prof;
   j.item.g.px = prof + (size_t)6 * q.ny;
-  j.item.pts = Points{(const int32_t *)row_slot, (const int32_t *)seg, (const int2 *)ent, j.table.n_xt};
   j.uploaded = true;
-  j.table = PointTable();  // host copy no longer needed
+  j.table = PointTable();  // host copies no longer needed
+  j.table2 = PointTable();
   *staged = off;
   return kOk;
 }
@@ -262,8 +371,8 @@ int validate(const tide_tm2d_job &job, int group, EJob &out) {
       return kInvalidArgument;
     }
   for (int c = 0; c < 7; ++c)
-    if (!q.fields[c] || (c < 5 && !job.fields_b[c])) {
-      set_error("engine: forward state needs fields[7] and fields_b[5]");
+    if (!q.fields[c] || !job.fields_b[c]) {
+      set_error("engine: forward state needs fields[7] and fields_b[7]");
       return kInvalidArgument;
     }
   if ((q.n_src > 0 && (!q.f || !q.sources_i)) || (q.n_rec > 0 && (!q.r || !q.receivers_i))) {
@@ -319,6 +428,8 @@ void fill_item(EJob &j) {
   }
   it.mhzx = q.fields[6];
   it.mhxz = q.fields[5];
+  it.mhzx_b = j.spec.fields_b[6];
+  it.mhxz_b = j.spec.fields_b[5];
   it.f = q.f;
   it.r = q.r;
   j.fmt = j.spec.storage_format == 1 ? 2 : 1;
@@ -355,6 +466,10 @@ void fill_item(EJob &j) {
 }
 
 int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running) {
+  // two-step engines advance an even number of steps per tick (one step
+  // when max_steps is 1); jobs with an odd remainder end on a one-step launch
+  const bool pairs = e.two_step && max_steps >= 2;
+  if (pairs) max_steps &= ~(int64_t)1;
   std::vector<std::pair<int64_t, EJob *>> run;  // (steps this tick, job)
   for (auto &kv : e.jobs) {
     EJob &j = kv.second;
@@ -376,7 +491,7 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
                      return a.first > b.first;
                    });
   const int n = (int)run.size();
-  size_t need = align_up((size_t)n * sizeof(ServeItem)) + align_up((size_t)(n + 1) * sizeof(int32_t));
+  size_t need = align_up((size_t)n * sizeof(ServeItem)) + 2 * align_up((size_t)(n + 1) * sizeof(int32_t));
   for (auto &r : run)
     if (!r.second->uploaded) need += table_bytes(*r.second);
   Slot &sl = e.slots[e.cur];
@@ -390,7 +505,8 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
   }
   ServeItem *items = (ServeItem *)(sl.host + off);
   int32_t *start = (int32_t *)(sl.host + off + align_up((size_t)n * sizeof(ServeItem)));
-  int64_t tile = 0;
+  int32_t *start2 = (int32_t *)((char *)start + align_up((size_t)(n + 1) * sizeof(int32_t)));
+  int64_t tile = 0, tile2 = 0;
   for (int i = 0; i < n; ++i) {
     EJob &j = *run[(size_t)i].second;
     ServeItem &si = items[i];
@@ -401,33 +517,63 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
     si.phase = j.phase;
     si.fmt = j.phase == 0 ? (j.grad ? j.fmt : 0) : j.fmt;
     si.k0 = (int32_t)j.k;
+    si.par0 = j.par;
+    si.epar0 = j.epar;
     si.pad = 0;
     start[i] = (int32_t)tile;
+    start2[i] = (int32_t)tile2;
     const int64_t tiles = (int64_t)si.b.pts.n_xt * si.b.n_yt;
     // single-shot jobs pack `group` tiles per workgroup (see serve_step)
     tile += j.spec.p.n_shots == 1 ? (tiles + e.group - 1) / e.group : tiles * j.n_groups;
-    if (tile >= ((int64_t)1 << 31)) {
+    if (pairs) {
+      const int64_t tiles2 = (int64_t)si.b.pts2.n_xt * si.b.n_yt2;
+      tile2 += j.spec.p.n_shots == 1 ? (tiles2 + e.group - 1) / e.group : tiles2 * j.n_groups;
+    }
+    if (tile >= ((int64_t)1 << 31) || tile2 >= ((int64_t)1 << 31)) {
       set_error("engine: too many tiles in one tick");
       return kInvalidArgument;
     }
   }
   start[n] = (int32_t)tile;
+  start2[n] = (int32_t)tile2;
   TIDE_HIP_TRY(hipMemcpyAsync(e.d_items, items, (size_t)n * sizeof(ServeItem), hipMemcpyHostToDevice, s));
   TIDE_HIP_TRY(hipMemcpyAsync(e.d_start, start, (size_t)(n + 1) * sizeof(int32_t),
                               hipMemcpyHostToDevice, s));
+  if (pairs)
+    TIDE_HIP_TRY(hipMemcpyAsync(e.d_start2, start2, (size_t)(n + 1) * sizeof(int32_t),
+                                hipMemcpyHostToDevice, s));
   TIDE_HIP_TRY(hipEventRecord(sl.ev, s));
   sl.pending = true;
   e.cur ^= 1;
   const int64_t steps = run[0].first;
-  int m = n;
-  for (int64_t jstep = 0; jstep < steps; ++jstep) {
-    while (m > 0 && run[(size_t)m - 1].first <= jstep) --m;
-    launch_step(e, start[m], m, (int)jstep, s);
+  if (pairs) {
+    // count of jobs taking at least v steps (the list is sorted, longest first)
+    auto at_least = [&](int64_t v) {
+      int c = 0;
+      while (c < n && run[(size_t)c].first >= v) ++c;
+      return c;
+    };
+    for (int64_t q = 0; 2 * q < steps; ++q) {
+      const int m2 = at_least(2 * q + 2), m1 = at_least(2 * q + 1);
+      if (m2 > 0) launch_pair(e, start2[m2], m2, (int)q, s);
+      if (m1 > m2) launch_step(e, start[m1] - start[m2], m1, (int)(2 * q), s, start[m2], 1);
+    }
+  } else {
+    int m = n;
+    for (int64_t jstep = 0; jstep < steps; ++jstep) {
+      while (m > 0 && run[(size_t)m - 1].first <= jstep) --m;
+      launch_step(e, start[m], m, (int)jstep, s, 0, e.two_step ? 1 : 0);
+    }
   }
   TIDE_HIP_TRY(hipGetLastError());
   int64_t still = 0;
   for (auto &r : run) {
     EJob &j = *r.second;
+    if (e.two_step) {  // state buffers after this tick's launches
+      const int64_t n_pairs = pairs ? r.first / 2 : 0, single = pairs ? (r.first & 1) : r.first;
+      j.par ^= (int)((n_pairs + single) & 1);
+      j.epar ^= (int)(n_pairs & 1);
+    }
     j.k += r.first;
     if (j.k >= j.nt) {
       j.phase += 1;
@@ -449,6 +595,7 @@ void engine_destroy(Engine *e) {
   (void)hipDeviceSynchronize();
   if (e->d_items) (void)hipFree(e->d_items);
   if (e->d_start) (void)hipFree(e->d_start);
+  if (e->d_start2) (void)hipFree(e->d_start2);
   for (Slot &sl : e->slots) {
     if (sl.host) (void)hipHostFree(sl.host);
     if (sl.ev) (void)hipEventDestroy(sl.ev);
@@ -479,9 +626,11 @@ extern "C" int tide_tm2d_engine_create(int64_t device, int64_t group, int64_t ma
   e->device = (int)device;
   e->group = (int)group;
   pick_rows(rows_for<float>(g_tuning.fwd_rows), rows_for<float>(g_tuning.adj_rows), &e->rows_f, &e->rows_a);
+  e->two_step = g_tuning.fwd_steps == 2 && g_tuning.adj_steps == 2;
   e->max_jobs = max_jobs;
   hipError_t err = hipMalloc((void **)&e->d_items, (size_t)max_jobs * sizeof(ServeItem));
   if (err == hipSuccess) err = hipMalloc((void **)&e->d_start, (size_t)(max_jobs + 1) * sizeof(int32_t));
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_start2, (size_t)(max_jobs + 1) * sizeof(int32_t));
   for (Slot &sl : e->slots)
     if (err == hipSuccess) err = hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming);
   if (err != hipSuccess) {
@@ -531,6 +680,10 @@ extern "C" int tide_tm2d_engine_add(void *engine, const tide_tm2d_job *job, int6
   if (!hs.empty()) std::memcpy(hs.data(), q.sources_i, hs.size() * sizeof(int64_t));
   if (!hr.empty()) std::memcpy(hr.data(), q.receivers_i, hr.size() * sizeof(int64_t));
   if (!build_point_table(hs.data(), q.n_src, hr.data(), q.n_rec, q.n_shots, q.ny, q.nx, kOwn, 1, j.table))
+    return kInvalidArgument;
+  if (e->two_step &&
+      !build_point_table_halo(hs.data(), q.n_src, hr.data(), q.n_rec, q.n_shots, q.ny, q.nx, kOwn2, 2,
+                              j.table2))
     return kInvalidArgument;
   fill_item(j);
   j.nt = q.nt;
@@ -596,6 +749,7 @@ extern "C" int tide_tm2d_engine_begin_adjoint(void *engine, int64_t job_id) {
   }
   j.phase = 2;
   j.k = 0;
+  j.par = j.epar = 0;  // the adjoint state starts in the parity-0 buffers
   return kOk;
 }
 

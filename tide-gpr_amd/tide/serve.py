@@ -57,7 +57,7 @@ class _Job(ctypes.Structure):
 
     _fields_ = [
         ("p", _Problem),
-        ("fields_b", _P * 5),
+        ("fields_b", _P * 7),
         ("lambda_b", _P * 7),
         ("grad_ca_groups", _P),
         ("grad_cb_groups", _P),
@@ -103,7 +103,7 @@ def job_bytes(
 ) -> int:
     """Device bytes one engine job holds (fp32 fields, FP64 gradients)."""
     a = ny_p * nx_p
-    b = 3 * a * 4 + 12 * n_shots * a * 4 + nt * n_shots * (n_src + n_rec) * 4
+    b = 3 * a * 4 + 14 * n_shots * a * 4 + nt * n_shots * (n_src + n_rec) * 4
     b += 12 * (ny_p + nx_p) * 4
     if gradient:
         n_store = (nt + interval - 1) // interval if nt > 0 else 0
@@ -147,7 +147,7 @@ def state_spec(
     shape = (n_shots, ny_p, nx_p)
     spec = [
         ("fields", f32, shape, 7, True),
-        ("fields_b", f32, shape, 5, True),
+        ("fields_b", f32, shape, 7, True),
         ("r", f32, (nt, n_shots, max(n_rec, 1)), 1, False),
     ]
     if gradient:
