@@ -191,3 +191,22 @@ def test_many_small_solves_share_launches():
     server.close()
     lat = np.array([t.latency for t in tickets])
     assert np.all(lat > 0)
+
+
+@pytest.mark.parametrize("tick", [1, 3])
+def test_odd_and_single_step_ticks_match_the_operator(problems, tick):
+    """Ticks of one step (one-step launches only) and of an odd count (the
+    two-step engine rounds it down to pairs) give the operator's receivers
+    bitwise and its gradients within 1e-5, for a solve of odd nt."""
+    spec, p = SPECS[4], problems[4]  # nt 31, BF16 snapshots, gradient
+    print(f"tick {tick}: {spec}", flush=True)
+    server = TM2DServer(DEV, hbm_budget=BUDGET, steps_per_tick=tick)
+    t = server.submit(p, **_submit_kwargs(spec, p))
+    server.drain()
+    torch.cuda.synchronize()
+    r, loss, ge, gs = _reference(spec, p)
+    same = torch.equal(t.result.receiver_data, r)
+    assert same, "receivers differ from the operator's"
+    assert relative_l2(t.result.grad_epsilon.cpu().numpy(), ge.cpu().numpy()) < 1e-5
+    assert relative_l2(t.result.grad_sigma.cpu().numpy(), gs.cpu().numpy()) < 1e-5
+    server.close()
