@@ -471,7 +471,7 @@ def main() -> None:
     tuning = None
     if not args.no_autotune:
         from tide.autotune import tune_tm2d
-        tuning = tune_tm2d(ny_p, nx_p, max(sizes), device=dev)
+        tuning = tune_tm2d(ny_p, nx_p, sizes, device=dev)  # scored over this run's shot chunks
         log(f"autotuned launch shapes forward {tuning['forward']} adjoint {tuning['adjoint']} "
             f"in {tuning.get('seconds', 0)} s")
     else:

@@ -33,3 +33,21 @@ def test_autotune_installs_a_candidate_and_caches_it():
         assert np.array_equal(r, O.tm2d_forward(case)[1])
     finally:
         _install(*before)
+
+
+def test_autotune_scores_a_list_of_shot_chunks():
+    from tide.autotune import _install, current
+
+    before = current()
+    try:
+        info = tune_tm2d(150, 170, [3, 2, 2], nt=16)
+        print(info, flush=True)
+        assert not info["cached"] and info["chunks"] == [3, 2, 2]
+        assert info["forward"] in FWD_CANDIDATES and info["adjoint"] in ADJ_CANDIDATES
+        assert current() == (info["forward"], info["adjoint"])
+        again = tune_tm2d(150, 170, [2, 3, 2], nt=16)
+        assert again["cached"] and again["adjoint"] == info["adjoint"]
+        with pytest.raises(ValueError):
+            tune_tm2d(150, 170, [], nt=16)
+    finally:
+        _install(*before)

@@ -290,10 +290,12 @@ def shot_loop(fwd, adj):
 
 
 @pytest.mark.parametrize("shots,loop,steps", [(1, 0, 1), (5, 0, 1), (12, 0, 1), (5, 1, 1), (12, 1, 1),
-                                              (5, 0, 2), (12, 0, 2)])
+                                              (5, 0, 2), (6, 0, 2), (12, 0, 2)])
 def test_shot_groups_match_oracle(shots, loop, steps):
-    """One workgroup holds up to 8 shots; 12 shots exercise two groups (each
-    group sums into its own FP64 slice, folded afterwards).  The shot-looping
+    """One workgroup holds up to 8 shots; 12 shots exercise several groups
+    (each group sums into its own FP64 slice, folded afterwards; groups are
+    balanced, so 5 and 6 shots at the default 4 per group run as 3 + 2 and
+    3 + 3).  The shot-looping
     kernels sum every shot of a cell in registers instead; the two-step
     kernels reduce both steps' terms per launch."""
     print(f"shots {shots} shot loop {loop} steps {steps}", flush=True)

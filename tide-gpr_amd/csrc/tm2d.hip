@@ -382,7 +382,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
   const int rows = rows_for<T>(g_tuning.adj_rows);
   const bool loop = g_tuning.adj_loop != 0 && !fused;  // two-step launches group shots
   const int group = loop ? std::min<int>(g_tuning.adj_group, kMaxLoopTiles)
-                         : std::min<int>(p.n_shots, g_tuning.adj_group);
+                         : balanced_group(p.n_shots, g_tuning.adj_group);
   a.n_yt = (p.g.ny + rows - 1) / rows;
   const int64_t n_tiles = (int64_t)dpt.n_xt * a.n_yt;
   const int64_t n_groups = loop ? 1 : (p.n_shots + group - 1) / group;

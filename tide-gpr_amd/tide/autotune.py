@@ -122,7 +122,7 @@ def _time(fn, reps: int = 2) -> float:
 def tune_tm2d(
     ny: int,
     nx: int,
-    n_shots: int,
+    n_shots: int | Sequence[int],
     *,
     dtype: torch.dtype = torch.float32,
     nt: int = 48,
@@ -134,60 +134,83 @@ def tune_tm2d(
     """Pick and install the fastest (rows, group, shot loop, steps, split,
     border rows) -- keeping the installed shape when it is within
     ``KEEP_MARGIN`` of the fastest -- for the forward-with-snapshots and adjoint kernels on a padded
-    ``ny x nx`` grid with ``n_shots`` shots per call.  Returns the choice
-    (``forward`` and ``adjoint`` candidates) and the timings."""
+    ``ny x nx`` grid with ``n_shots`` shots per call.  ``n_shots`` may
+    also be the list of a caller's shot chunks (e.g. ``[7, 7, 6, 6, 6]``):
+    each candidate is then timed on every distinct chunk size and scored by
+    the total over the list, since the best shots-per-workgroup depends on
+    the chunk.  Returns the choice (``forward`` and ``adjoint`` candidates)
+    and the timings (microseconds per time step, summed over the chunks)."""
     device = torch.device(device)
-    key = _size_class(ny, nx, n_shots, dtype)
+    chunks = [int(n) for n in n_shots] if hasattr(n_shots, "__len__") else [int(n_shots)]
+    if not chunks or min(chunks) < 1:
+        raise ValueError("n_shots must be a positive count or a list of them")
+    sizes = sorted(set(chunks))
+    key = _size_class(ny, nx, max(chunks), dtype)
+    if len(sizes) > 1:
+        key = key + (tuple(sizes),)
     if use_cache and key in _cache:
         fwd, adj = _cache[key]
         _install(fwd, adj)
         return {"forward": fwd, "adjoint": adj, "cached": True}
     t_start = time.perf_counter()
-    st = _problem(ny, nx, n_shots, nt, dtype, device)
     lib = _native.load()
     P = _native.ptr
-    S, pny, pnx = st.n_shots, st.ny, st.nx
-    stores = [torch.empty(nt, S, pny, pnx, device=device, dtype=dtype) for _ in range(2)]
-    r = torch.zeros(nt, S, st.n_rec, device=device, dtype=dtype)
-    gr = torch.ones_like(r)
-    gf = torch.zeros(nt, S, st.n_src, device=device, dtype=dtype)
-    gca = torch.zeros(pny, pnx, device=device, dtype=torch.float64)
-    gcb = torch.zeros_like(gca)
     stream = _native.stream_handle(device)
     fwd = _native.get_function("maxwell_tm", "forward_with_storage", 2, dtype)
     bwd = getattr(lib, f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
     bpe = dtype.itemsize
 
+    def runners(n: int):
+        st = _problem(ny, nx, n, nt, dtype, device)
+        S, pny, pnx = st.n_shots, st.ny, st.nx
+        stores = [torch.empty(nt, S, pny, pnx, device=device, dtype=dtype) for _ in range(2)]
+        r = torch.zeros(nt, S, st.n_rec, device=device, dtype=dtype)
+        gr = torch.ones_like(r)
+        gf = torch.zeros(nt, S, st.n_src, device=device, dtype=dtype)
+        gca = torch.zeros(pny, pnx, device=device, dtype=torch.float64)
+        gcb = torch.zeros_like(gca)
+
+        def run_fwd():
+            fields = [t.clone() for t in st.fields]
+            _native.check(fwd(
+                P(st.ca), P(st.cb), P(st.cq), P(st.f), *[P(t) for t in fields], P(r),
+                P(stores[0]), None, None, P(stores[1]), None, None, *[P(p) for p in st.profiles],
+                P(st.sources_i), P(st.receivers_i), st.rdy, st.rdx, st.dt, nt, S, pny, pnx,
+                st.n_src, st.n_rec, 1, 0, 0, pny * pnx * bpe, True, True, False, False, False, 0,
+                *st.pml, 0, device.index or 0, stream, stream))
+
+        def run_bwd():
+            lam = [torch.zeros(S, pny, pnx, device=device, dtype=dtype) for _ in range(7)]
+            _native.check(bwd(
+                P(st.ca), P(st.cb), P(st.cq), P(gr), *[P(t) for t in lam], P(stores[0]),
+                P(stores[1]), P(gf), P(gca), P(gcb), *[P(p) for p in st.profiles],
+                P(st.sources_i), P(st.receivers_i), st.rdy, st.rdx, nt, S, pny, pnx, st.n_src,
+                st.n_rec, 1, 0, pny * pnx * bpe, False, False, False, nt, *st.pml,
+                device.index or 0, stream))
+
+        return run_fwd, run_bwd
+
+    runs = [(chunks.count(n), *runners(n)) for n in sizes]
+
     def run_fwd():
-        fields = [t.clone() for t in st.fields]
-        _native.check(fwd(
-            P(st.ca), P(st.cb), P(st.cq), P(st.f), *[P(t) for t in fields], P(r),
-            P(stores[0]), None, None, P(stores[1]), None, None, *[P(p) for p in st.profiles],
-            P(st.sources_i), P(st.receivers_i), st.rdy, st.rdx, st.dt, nt, S, pny, pnx,
-            st.n_src, st.n_rec, 1, 0, 0, pny * pnx * bpe, True, True, False, False, False, 0,
-            *st.pml, 0, device.index or 0, stream, stream))
+        return sum(c * _time(f) for c, f, _ in runs)
 
     def run_bwd():
-        lam = [torch.zeros(S, pny, pnx, device=device, dtype=dtype) for _ in range(7)]
-        _native.check(bwd(
-            P(st.ca), P(st.cb), P(st.cq), P(gr), *[P(t) for t in lam], P(stores[0]),
-            P(stores[1]), P(gf), P(gca), P(gcb), *[P(p) for p in st.profiles],
-            P(st.sources_i), P(st.receivers_i), st.rdy, st.rdx, nt, S, pny, pnx, st.n_src,
-            st.n_rec, 1, 0, pny * pnx * bpe, False, False, False, nt, *st.pml,
-            device.index or 0, stream))
+        return sum(c * _time(b) for c, _, b in runs)
 
     fwd_before, adj_before = current()
-    run_fwd()
-    run_bwd()
+    for _, f, b in runs:
+        f()
+        b()
     torch.cuda.synchronize()
     f_times, a_times = {}, {}
     for cand in fwd_candidates:
         _install(tuple(cand), adj_before)
-        f_times[tuple(cand)] = _time(run_fwd)
+        f_times[tuple(cand)] = run_fwd()
     best_f = _pick(f_times, fwd_before)
     for cand in adj_candidates:
         _install(best_f, tuple(cand))
-        a_times[tuple(cand)] = _time(run_bwd)
+        a_times[tuple(cand)] = run_bwd()
     best_a = _pick(a_times, adj_before)
     _install(best_f, best_a)
     _cache[key] = (best_f, best_a)
@@ -200,6 +223,7 @@ def tune_tm2d(
         "forward": best_f,
         "adjoint": best_a,
         "cached": False,
+        "chunks": chunks,
         "forward_us_per_step": {label(k): round(v * 1e3 / nt, 2) for k, v in f_times.items()},
         "adjoint_us_per_step": {label(k): round(v * 1e3 / nt, 2) for k, v in a_times.items()},
         "seconds": round(time.perf_counter() - t_start, 2),
