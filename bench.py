@@ -460,23 +460,46 @@ def main() -> None:
     per_shot = bytes_per_shot(ny_p, nx_p, nt, snap_bytes)
     free, total_mem = torch.cuda.mem_get_info(dev)
     cap = args.chunk or max(1, int(0.90 * free // per_shot))
-    n_chunks = math.ceil(n_shots / cap)
-    base, extra = divmod(n_shots, n_chunks)
-    sizes = [base + (1 if i < extra else 0) for i in range(n_chunks)]
-    bounds = [sum(sizes[:i]) for i in range(n_chunks + 1)]
+
+    def split(c):
+        k = math.ceil(n_shots / c)
+        base, extra = divmod(n_shots, k)
+        return [base + (1 if i < extra else 0) for i in range(k)]
 
     from tide.autotune import _install, current
 
     default_shapes = current()  # the library's defaults, restored for the other workloads
     tuning = None
+    chunk_scores = None
     if not args.no_autotune:
         from tide.autotune import tune_tm2d
-        tuning = tune_tm2d(ny_p, nx_p, sizes, device=dev)  # scored over this run's shot chunks
-        log(f"autotuned launch shapes forward {tuning['forward']} adjoint {tuning['adjoint']} "
-            f"in {tuning.get('seconds', 0)} s")
+        # Shots per chunk is tuned with the launch shapes: the largest chunk
+        # that fits HBM, and the largest multiple of the adjoint's default
+        # shots per workgroup below it (full workgroups).  Each option is
+        # scored by its tuned time per step summed over its chunks.
+        g = default_shapes[1][1]
+        options = [cap] if args.chunk else sorted({cap, max(g, cap // g * g) if cap >= g else cap},
+                                                  reverse=True)
+        best = None
+        chunk_scores = {}
+        for c in options:
+            _install(*default_shapes)  # each option starts from the same installed shape
+            t = tune_tm2d(ny_p, nx_p, split(c), device=dev, max_store_bytes=64 << 30)
+            chunk_scores[c] = t["chosen_us_per_step"]
+            log(f"chunk {c}: forward {t['forward']} adjoint {t['adjoint']} "
+                f"{t['chosen_us_per_step']} us per step over {len(split(c))} chunks "
+                f"(tuned in {t.get('seconds', 0)} s)")
+            if best is None or t["chosen_us_per_step"] < best[0]:
+                best = (t["chosen_us_per_step"], c, t)
+        _, cap, tuning = best
+        _install(tuning["forward"], tuning["adjoint"])
+        log(f"autotuned: chunk {cap}, forward {tuning['forward']} adjoint {tuning['adjoint']}")
     else:
         fwd, adj = current()
         tuning = {"forward": fwd, "adjoint": adj, "cached": False}
+    sizes = split(cap)
+    n_chunks = len(sizes)
+    bounds = [sum(sizes[:i]) for i in range(n_chunks + 1)]
 
     disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
     storage = tide.StorageOptions(compression=args.compression == "bf16")
@@ -614,6 +637,7 @@ def main() -> None:
                         f"{n_shots} shots x 256 receivers, nt {nt}, stencil 2, "
                         f"{'bf16' if snap_bytes == 2 else 'fp32'} snapshots",
             "shot_chunks": sizes,
+            "chunk_scores_us_per_step": chunk_scores,
             "cell_updates_per_step": updates_per_step,
             "hbm_frac_whole_job": round(gcells * 1e9 * (24 + b_snap) / 1e9 / HBM_PEAK_GBS, 4),
             "misfit": float(misfit),

@@ -106,9 +106,13 @@ def _problem(ny: int, nx: int, n_shots: int, nt: int, dtype: torch.dtype, device
     return prepare_tm2d(eps, sig, torch.ones_like(eps), dx, dt, amp, src, rec, pml_width=pml)
 
 
-def _time(fn, reps: int = 2) -> float:
+def _time(fn, reps: int = 2, setup=None) -> float:
+    """Best of `reps` timings of fn() in milliseconds; `setup` (zeroing the
+    state buffers) runs before each, outside the timed region."""
     best = float("inf")
     for _ in range(reps):
+        if setup is not None:
+            setup()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -125,7 +129,8 @@ def tune_tm2d(
     n_shots: int | Sequence[int],
     *,
     dtype: torch.dtype = torch.float32,
-    nt: int = 48,
+    nt: int = 400,
+    max_store_bytes: int = 24 << 30,
     device: torch.device | str = "cuda:0",
     fwd_candidates: Sequence[tuple[int, ...]] = FWD_CANDIDATES,
     adj_candidates: Sequence[tuple[int, ...]] = ADJ_CANDIDATES,
@@ -138,7 +143,9 @@ def tune_tm2d(
     also be the list of a caller's shot chunks (e.g. ``[7, 7, 6, 6, 6]``):
     each candidate is then timed on every distinct chunk size and scored by
     the total over the list, since the best shots-per-workgroup depends on
-    the chunk.  Returns the choice (``forward`` and ``adjoint`` candidates)
+    the chunk.  ``nt`` steps per timed call (long enough that per-call set-up
+    weighs as little as in a real run) are cut so that the snapshot buffers
+    of all chunk sizes stay within ``max_store_bytes``.  Returns the choice (``forward`` and ``adjoint`` candidates)
     and the timings (microseconds per time step, summed over the chunks)."""
     device = torch.device(device)
     chunks = [int(n) for n in n_shots] if hasattr(n_shots, "__len__") else [int(n_shots)]
@@ -153,6 +160,8 @@ def tune_tm2d(
         _install(fwd, adj)
         return {"forward": fwd, "adjoint": adj, "cached": True}
     t_start = time.perf_counter()
+    per_step = 2 * dtype.itemsize * ny * nx * sum(sizes)  # two snapshot planes per shot
+    nt = max(16, min(int(nt), int(max_store_bytes // max(per_step, 1))) // 2 * 2)
     lib = _native.load()
     P = _native.ptr
     stream = _native.stream_handle(device)
@@ -170,8 +179,16 @@ def tune_tm2d(
         gca = torch.zeros(pny, pnx, device=device, dtype=torch.float64)
         gcb = torch.zeros_like(gca)
 
+        fields = [torch.empty_like(t) for t in st.fields]
+        lam = [torch.empty(S, pny, pnx, device=device, dtype=dtype) for _ in range(7)]
+
+        def reset():
+            for t, t0 in zip(fields, st.fields):
+                t.copy_(t0)
+            for t in lam:
+                t.zero_()
+
         def run_fwd():
-            fields = [t.clone() for t in st.fields]
             _native.check(fwd(
                 P(st.ca), P(st.cb), P(st.cq), P(st.f), *[P(t) for t in fields], P(r),
                 P(stores[0]), None, None, P(stores[1]), None, None, *[P(p) for p in st.profiles],
@@ -180,7 +197,6 @@ def tune_tm2d(
                 *st.pml, 0, device.index or 0, stream, stream))
 
         def run_bwd():
-            lam = [torch.zeros(S, pny, pnx, device=device, dtype=dtype) for _ in range(7)]
             _native.check(bwd(
                 P(st.ca), P(st.cb), P(st.cq), P(gr), *[P(t) for t in lam], P(stores[0]),
                 P(stores[1]), P(gf), P(gca), P(gcb), *[P(p) for p in st.profiles],
@@ -188,19 +204,24 @@ def tune_tm2d(
                 st.n_rec, 1, 0, pny * pnx * bpe, False, False, False, nt, *st.pml,
                 device.index or 0, stream))
 
-        return run_fwd, run_bwd
+        return run_fwd, run_bwd, reset
 
     runs = [(chunks.count(n), *runners(n)) for n in sizes]
 
+    # The gradient-slice set-up and fold of a call is timed with it (it
+    # belongs to the shape), but its weight is that of a call of nt steps;
+    # nt should be long enough that the per-step cost dominates.
     def run_fwd():
-        return sum(c * _time(f) for c, f, _ in runs)
+        return sum(c * _time(f, setup=z) for c, f, _, z in runs)
 
     def run_bwd():
-        return sum(c * _time(b) for c, _, b in runs)
+        return sum(c * _time(b, setup=z) for c, _, b, z in runs)
 
     fwd_before, adj_before = current()
-    for _, f, b in runs:
+    for _, f, b, z in runs:
+        z()
         f()
+        z()
         b()
     torch.cuda.synchronize()
     f_times, a_times = {}, {}
@@ -224,6 +245,8 @@ def tune_tm2d(
         "adjoint": best_a,
         "cached": False,
         "chunks": chunks,
+        "nt": nt,
+        "chosen_us_per_step": round((f_times[best_f] + a_times[best_a]) * 1e3 / nt, 2),
         "forward_us_per_step": {label(k): round(v * 1e3 / nt, 2) for k, v in f_times.items()},
         "adjoint_us_per_step": {label(k): round(v * 1e3 / nt, 2) for k, v in a_times.items()},
         "seconds": round(time.perf_counter() - t_start, 2),
