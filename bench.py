@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # (scaled by a 1 GiB copy's counters) for the kernels as built, default
 # launch shapes; see profiles/tm2d_config2_hbm_counters.md.  Reported as
 # roofline.traffic.
-MEASURED_TRAFFIC_B_PER_CELL = {"fwd2_step": 32.40, "adj2_step": 36.50}
+MEASURED_TRAFFIC_B_PER_CELL = {"fwd2_step": 32.52, "adj2_step": 35.30}
 
 
 def log(msg: str) -> None:
