@@ -120,13 +120,18 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
 
 constexpr int kServeRows2 = 4;  // tile rows of the two-step engine launches
 
-// Two-step launch of pair q of the tick: jobs [0, n) advance steps
-// k0 + 2q and k0 + 2q + 1 of their phase.
-template <int TY>
+// Two-step launch of pair q of the tick over the tiles [first, first +
+// gridDim.x) of jobs [0, n): each advances steps k0 + 2q and k0 + 2q + 1.
+// PH 0 runs forward jobs, PH 1 adjoint jobs (the tick orders its jobs by
+// phase, so each launch covers one phase): a kernel holding both phases
+// would run the forward tiles at the adjoint's register count (172 VGPRs,
+// two wavefronts per SIMD).
+template <int TY, int PH>
 __global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ items,
-                                                  const int32_t *__restrict__ start, int n, int q) {
+                                                  const int32_t *__restrict__ start, int n, int q,
+                                                  int first) {
   extern __shared__ double red[];
-  const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
+  const int64_t flat = first + xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
   int lo = 0, hi = n - 1;  // largest i with start[i] <= flat
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ 
     const int tile = local * (int)blockDim.y + w;
     if (tile >= n_xt * it.b.n_yt2) return;  // uniform over the wavefront; no barriers here
     const int yt = tile / n_xt, xt = tile - yt * n_xt;
-    if (it.phase == 0) {
+    if constexpr (PH == 0) {
       const Fwd2Args<float> a = item_fwd2_args(it.b, k, par);
       const bool inner = in_split<TY>(a.split, xt, yt);
       if (it.fmt == 0)
@@ -174,7 +179,7 @@ __global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ 
     return;
   }
   const TileCoord tc = batch_tile(local, it.b.pts2.n_xt, it.b.n_yt2);
-  if (it.phase == 0) {
+  if constexpr (PH == 0) {
     const Fwd2Args<float> a = item_fwd2_args(it.b, k, par);
     const int s = tc.grp * (int)blockDim.y + w;
     if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
@@ -268,10 +273,17 @@ void launch_step(const Engine &e, int64_t grid, int n, int j, hipStream_t s, int
     serve_step<4, 2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start, n, j, first, pair_mode);
 }
 
-void launch_pair(const Engine &e, int64_t grid, int n, int q, hipStream_t s) {
+// Tiles [first, first + grid) of jobs [0, n), all of one phase (0 forward,
+// 1 adjoint).
+void launch_pair(const Engine &e, int phase, int64_t grid, int n, int q, int64_t first,
+                 hipStream_t s) {
   const dim3 block(64, (unsigned)e.group);
-  const size_t lds = 2 * (size_t)e.group * kServeRows2 * 64 * sizeof(double);
-  serve_pair<kServeRows2><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start2, n, q);
+  if (phase == 0) {
+    serve_pair<kServeRows2, 0><<<(unsigned)grid, block, 0, s>>>(e.d_items, e.d_start2, n, q, (int)first);
+  } else {
+    const size_t lds = 2 * (size_t)e.group * kServeRows2 * 64 * sizeof(double);
+    serve_pair<kServeRows2, 1><<<(unsigned)grid, block, lds, s>>>(e.d_items, e.d_start2, n, q, (int)first);
+  }
 }
 
 int slot_reserve(Slot &sl, size_t bytes) {
@@ -486,8 +498,12 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
     *n_running = 0;
     return kOk;
   }
+  // Longest first; with two-step launches, forward jobs before adjoint jobs
+  // (longest first within each), so a pair launch covers one phase.
   std::stable_sort(run.begin(), run.end(),
-                   [](const std::pair<int64_t, EJob *> &a, const std::pair<int64_t, EJob *> &b) {
+                   [pairs](const std::pair<int64_t, EJob *> &a, const std::pair<int64_t, EJob *> &b) {
+                     if (pairs && a.second->phase != b.second->phase)
+                       return a.second->phase < b.second->phase;
                      return a.first > b.first;
                    });
   const int n = (int)run.size();
@@ -547,16 +563,24 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
   e.cur ^= 1;
   const int64_t steps = run[0].first;
   if (pairs) {
-    // count of jobs taking at least v steps (the list is sorted, longest first)
-    auto at_least = [&](int64_t v) {
-      int c = 0;
-      while (c < n && run[(size_t)c].first >= v) ++c;
+    // jobs [0, nf) run forward, [nf, n) adjoint, each segment longest first
+    int nf = 0;
+    while (nf < n && run[(size_t)nf].second->phase == 0) ++nf;
+    // end of the jobs in [b, e) taking at least v steps
+    auto at_least = [&](int b, int e, int64_t v) {
+      int c = b;
+      while (c < e && run[(size_t)c].first >= v) ++c;
       return c;
     };
-    for (int64_t q = 0; 2 * q < steps; ++q) {
-      const int m2 = at_least(2 * q + 2), m1 = at_least(2 * q + 1);
-      if (m2 > 0) launch_pair(e, start2[m2], m2, (int)q, s);
-      if (m1 > m2) launch_step(e, start[m1] - start[m2], m1, (int)(2 * q), s, start[m2], 1);
+    int64_t longest = 0;
+    for (auto &r : run) longest = std::max(longest, r.first);
+    for (int64_t q = 0; 2 * q < longest; ++q) {
+      for (int ph = 0; ph < 2; ++ph) {
+        const int b = ph == 0 ? 0 : nf, en = ph == 0 ? nf : n;
+        const int m2 = at_least(b, en, 2 * q + 2), m1 = at_least(b, en, 2 * q + 1);
+        if (m2 > b) launch_pair(e, ph, start2[m2] - start2[b], m2, (int)q, start2[b], s);
+        if (m1 > m2) launch_step(e, start[m1] - start[m2], m1, (int)(2 * q), s, start[m2], 1);
+      }
     }
   } else {
     int m = n;
