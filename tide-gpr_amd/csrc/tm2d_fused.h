@@ -102,6 +102,65 @@ constexpr int set_edge() {
   return SET == kInnerTiles ? 0 : (SET == kSideTiles ? 1 : 3);
 }
 
+// Tile b of the ordered tile set and the edge handling it needs: border
+// tiles of TYB rows first (edge 3 in the bands above and below the interior
+// rows, 1 beside them), then the interior tiles of TY rows (edge 0); each set
+// repeated per shot group.  With `remap`, b is a launch's workgroup id and the
+// interior tiles are grouped by XCD; without it b is an index that is
+// already XCD-grouped.
+template <int TY, int TYB>
+__device__ __forceinline__ TileCoord ordered_tile(const TileSplit &sp, int n_xt, int ny, int n_groups,
+                                                  int64_t b, bool remap, int *edge) {
+  const int64_t nb = split_tiles<TYB>(sp, n_xt, ny, kBorderTiles);
+  const int64_t ni = split_tiles<TY>(sp, n_xt, ny, kInnerTiles);
+  // indices 0 .. nb*n_groups-1 (dispatched first, spread over the XCDs):
+  // border tiles; the rest: the interior
+  const int64_t nbg = nb * n_groups;
+  TileCoord c;
+  int64_t i;
+  const bool border = b < nbg;
+  if (border) {
+    c.grp = (int)(b / nb);
+    i = b - (int64_t)c.grp * nb;
+  } else {
+    const int64_t f = remap ? xcd_remap(b - nbg, ni * n_groups) : b - nbg;
+    c.grp = (int)(f / ni);
+    i = f - (int64_t)c.grp * ni;
+  }
+  const int iw = sp.xb - sp.xa;
+  if (!border) {
+    c.yt = sp.ya / TY + (int)(i / iw);
+    c.xt = sp.xa + (int)(i % iw);
+    *edge = 0;
+    return c;
+  }
+  const int top = (sp.ya / TYB) * n_xt, w = n_xt - iw, mid = ((sp.yb - sp.ya) / TYB) * w;
+  int k = (int)i;
+  if (k < top) {
+    c.yt = k / n_xt;
+    c.xt = k % n_xt;
+    *edge = 3;
+  } else if (k < top + mid) {
+    k -= top;
+    c.yt = sp.ya / TYB + k / w;
+    const int col = k % w;
+    c.xt = col < sp.xa ? col : sp.xb + (col - sp.xa);
+    *edge = 1;
+  } else {
+    k -= top + mid;
+    c.yt = sp.yb / TYB + k / n_xt;
+    c.xt = k % n_xt;
+    *edge = 3;
+  }
+  return c;
+}
+
+// Number of tiles of the ordered set per shot group.
+template <int TY, int TYB>
+__host__ __device__ __forceinline__ int64_t ordered_tiles(const TileSplit &sp, int n_xt, int ny) {
+  return split_tiles<TYB>(sp, n_xt, ny, kBorderTiles) + split_tiles<TY>(sp, n_xt, ny, kInnerTiles);
+}
+
 // This workgroup's tile for tile set SET; for the one-kernel sets also the
 // edge handling the tile needs.  kOrderedTiles hands out the border tiles
 // (their long per-wavefront latency) before the interior ones, so the
@@ -115,48 +174,7 @@ __device__ __forceinline__ TileCoord pick_tile(const TileSplit &sp, int n_xt, in
     *edge = in_split<TY>(sp, tc.xt, tc.yt) ? 0 : 3;
     return tc;
   } else if constexpr (SET == kOrderedTiles) {
-    const int64_t nb = split_tiles<TYB>(sp, n_xt, ny, kBorderTiles);
-    const int64_t ni = split_tiles<TY>(sp, n_xt, ny, kInnerTiles);
-    // blocks 0 .. nb*n_groups-1 (dispatched first, spread over the XCDs):
-    // border tiles; the rest: the interior, XCD-grouped
-    const int64_t b = blockIdx.x, nbg = nb * n_groups;
-    TileCoord c;
-    int64_t i;
-    const bool border = b < nbg;
-    if (border) {
-      c.grp = (int)(b / nb);
-      i = b - (int64_t)c.grp * nb;
-    } else {
-      const int64_t f = xcd_remap(b - nbg, ni * n_groups);
-      c.grp = (int)(f / ni);
-      i = f - (int64_t)c.grp * ni;
-    }
-    const int iw = sp.xb - sp.xa;
-    if (!border) {
-      c.yt = sp.ya / TY + (int)(i / iw);
-      c.xt = sp.xa + (int)(i % iw);
-      *edge = 0;
-      return c;
-    }
-    const int top = (sp.ya / TYB) * n_xt, w = n_xt - iw, mid = ((sp.yb - sp.ya) / TYB) * w;
-    int k = (int)i;
-    if (k < top) {
-      c.yt = k / n_xt;
-      c.xt = k % n_xt;
-      *edge = 3;
-    } else if (k < top + mid) {
-      k -= top;
-      c.yt = sp.ya / TYB + k / w;
-      const int col = k % w;
-      c.xt = col < sp.xa ? col : sp.xb + (col - sp.xa);
-      *edge = 1;
-    } else {
-      k -= top + mid;
-      c.yt = sp.yb / TYB + k / n_xt;
-      c.xt = k % n_xt;
-      *edge = 3;
-    }
-    return c;
+    return ordered_tile<TY, TYB>(sp, n_xt, ny, n_groups, (int64_t)blockIdx.x, true, edge);
   } else {
     *edge = set_edge<SET>();
     return split_tile<SET, TY>(sp, n_xt, ny, n_groups);

@@ -150,9 +150,9 @@ __global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ 
     // straight to the job's array (its tiles own disjoint cells)
     const int n_xt = it.b.pts2.n_xt;
     const int tile = local * (int)blockDim.y + w;
-    if (tile >= n_xt * it.b.n_yt2) return;  // uniform over the wavefront; no barriers here
-    const int yt = tile / n_xt, xt = tile - yt * n_xt;
     if constexpr (PH == 0) {
+      if (tile >= n_xt * it.b.n_yt2) return;  // uniform over the wavefront; no barriers here
+      const int yt = tile / n_xt, xt = tile - yt * n_xt;
       const Fwd2Args<float> a = item_fwd2_args(it.b, k, par);
       const bool inner = in_split<TY>(a.split, xt, yt);
       if (it.fmt == 0)
@@ -162,24 +162,31 @@ __global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ 
       else
         inner ? fwd2_tile<float, TY, 1, 0>(a, 0, xt, yt) : fwd2_tile<float, TY, 1, 3>(a, 0, xt, yt);
     } else {
+      // ordered tiles: 2-row border tiles, TY-row interior tiles
       const Adj2Args<float> a = item_adj2_args(it.b, k, par);
-      const bool inner = in_split<TY>(a.split, xt, yt);
+      if (tile >= ordered_tiles<TY, 2>(a.split, n_xt, a.g.ny)) return;  // uniform over the wavefront
+      int edge = 3;
+      const TileCoord tc = ordered_tile<TY, 2>(a.split, n_xt, a.g.ny, 1, tile, false, &edge);
       if (it.fmt == 2) {
-        if (inner)
-          adj2_tile<float, TY, 2, 0>(a, 0, xt, yt, nullptr, nullptr, false);
+        if (edge == 0)
+          adj2_tile<float, TY, 2, 0>(a, 0, tc.xt, tc.yt, nullptr, nullptr, false);
+        else if (edge == 1)
+          adj2_tile<float, 2, 2, 1>(a, 0, tc.xt, tc.yt, nullptr, nullptr, false);
         else
-          adj2_tile<float, TY, 2, 3>(a, 0, xt, yt, nullptr, nullptr, false);
+          adj2_tile<float, 2, 2, 3>(a, 0, tc.xt, tc.yt, nullptr, nullptr, false);
       } else {
-        if (inner)
-          adj2_tile<float, TY, 1, 0>(a, 0, xt, yt, nullptr, nullptr, false);
+        if (edge == 0)
+          adj2_tile<float, TY, 1, 0>(a, 0, tc.xt, tc.yt, nullptr, nullptr, false);
+        else if (edge == 1)
+          adj2_tile<float, 2, 1, 1>(a, 0, tc.xt, tc.yt, nullptr, nullptr, false);
         else
-          adj2_tile<float, TY, 1, 3>(a, 0, xt, yt, nullptr, nullptr, false);
+          adj2_tile<float, 2, 1, 3>(a, 0, tc.xt, tc.yt, nullptr, nullptr, false);
       }
     }
     return;
   }
-  const TileCoord tc = batch_tile(local, it.b.pts2.n_xt, it.b.n_yt2);
   if constexpr (PH == 0) {
+    const TileCoord tc = batch_tile(local, it.b.pts2.n_xt, it.b.n_yt2);
     const Fwd2Args<float> a = item_fwd2_args(it.b, k, par);
     const int s = tc.grp * (int)blockDim.y + w;
     if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
@@ -191,12 +198,23 @@ __global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ 
     else
       inner ? fwd2_tile<float, TY, 1, 0>(a, s, tc.xt, tc.yt) : fwd2_tile<float, TY, 1, 3>(a, s, tc.xt, tc.yt);
   } else {
+    // ordered tiles, as the single-problem adjoint: 2-row border tiles keep
+    // the register count of the CPML paths down
     const Adj2Args<float> a = item_adj2_args(it.b, k, par);
-    const int edge = in_split<TY>(a.split, tc.xt, tc.yt) ? 0 : 3;  // uniform over the workgroup
-    if (it.fmt == 2)
-      adj2_workgroup<float, TY, 2, 1 | 8>(a, tc, edge, red);
-    else
-      adj2_workgroup<float, TY, 1, 1 | 8>(a, tc, edge, red);
+    int edge = 3;
+    const TileCoord tc = ordered_tile<TY, 2>(a.split, it.b.pts2.n_xt, a.g.ny, it.b.n_groups, local, false, &edge);
+    // edge is uniform over the workgroup
+    if (it.fmt == 2) {
+      if (edge == 0)
+        adj2_workgroup<float, TY, 2, 1>(a, tc, 0, red);
+      else
+        adj2_workgroup<float, 2, 2, 2 | 8>(a, tc, edge, red);
+    } else {
+      if (edge == 0)
+        adj2_workgroup<float, TY, 1, 1>(a, tc, 0, red);
+      else
+        adj2_workgroup<float, 2, 1, 2 | 8>(a, tc, edge, red);
+    }
   }
 }
 
@@ -542,7 +560,10 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
     // single-shot jobs pack `group` tiles per workgroup (see serve_step)
     tile += j.spec.p.n_shots == 1 ? (tiles + e.group - 1) / e.group : tiles * j.n_groups;
     if (pairs) {
-      const int64_t tiles2 = (int64_t)si.b.pts2.n_xt * si.b.n_yt2;
+      // forward: TY-row tiles; adjoint: the ordered set (2-row border tiles)
+      const int64_t tiles2 = j.phase == 0 ? (int64_t)si.b.pts2.n_xt * si.b.n_yt2
+                                          : ordered_tiles<kServeRows2, 2>(si.b.split2, si.b.pts2.n_xt,
+                                                                           (int)j.spec.p.ny);
       tile2 += j.spec.p.n_shots == 1 ? (tiles2 + e.group - 1) / e.group : tiles2 * j.n_groups;
     }
     if (tile >= ((int64_t)1 << 31) || tile2 >= ((int64_t)1 << 31)) {
