@@ -80,10 +80,17 @@ __global__ __launch_bounds__(512) void adj2_batch(const BatchItem<T> *__restrict
   extern __shared__ double red[];
   const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
   const BatchItem<T> &it = items[tile_item[flat]];
-  const TileCoord tc = batch_tile((int)(flat - it.tile0_2), it.pts2.n_xt, it.n_yt2);
   const Adj2Args<T> a = item_adj2_args(it, k, par);
-  const int edge = in_split<TY>(a.split, tc.xt, tc.yt) ? 0 : 3;  // uniform over the workgroup
-  adj2_workgroup<T, TY, FMT, 1 | 8>(a, tc, edge, red);
+  // ordered tiles, as the single-problem adjoint: 2-row border tiles keep
+  // the register count of the CPML paths down; edge is uniform over the
+  // workgroup
+  int edge = 3;
+  const TileCoord tc = ordered_tile<TY, 2>(a.split, it.pts2.n_xt, a.g.ny, it.n_groups,
+                                           flat - it.tile0_2, false, &edge);
+  if (edge == 0)
+    adj2_workgroup<T, TY, FMT, 1>(a, tc, 0, red);
+  else
+    adj2_workgroup<T, 2, FMT, 2 | 8>(a, tc, edge, red);
 }
 
 template <typename T>
@@ -110,7 +117,9 @@ struct Plan {
   // two-step launches: tiles with >= 2 steps left at step 2j (active2[j]),
   // and the one-step tile range [rem_*[j], rem_*[j+1]...) of the problems
   // taking their single last step at step 2j (first, end per j)
-  int32_t *d_tiles2 = nullptr;
+  int32_t *d_tiles2 = nullptr;   // forward two-step tile -> problem
+  int32_t *d_tiles2a = nullptr;  // adjoint two-step (ordered) tile -> problem
+  std::vector<int64_t> active2a;  // adjoint tiles with >= 2 steps left at step 2j
   std::vector<int64_t> active2, rem_f_first, rem_f_end, rem_a_first, rem_a_end;
   std::vector<BatchItem<float>> h_fwd, h_adj;
   std::vector<DevicePointTable> dpts, dpts2;
@@ -211,9 +220,9 @@ void launch_pair_a(const Plan &p, int fmt, int64_t grid, hipStream_t s, int k, i
   const dim3 block(64, (unsigned)p.group);
   constexpr int TY = kBatchRows2;
   const size_t lds = 2 * (size_t)p.group * TY * 64 * sizeof(double);
-  if (fmt == 0) adj2_batch<float, TY, 0><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2, k, par);
-  else if (fmt == 2) adj2_batch<float, TY, 2><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2, k, par);
-  else adj2_batch<float, TY, 1><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2, k, par);
+  if (fmt == 0) adj2_batch<float, TY, 0><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2a, k, par);
+  else if (fmt == 2) adj2_batch<float, TY, 2><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2a, k, par);
+  else adj2_batch<float, TY, 1><<<(unsigned)grid, block, lds, s>>>(p.d_adj, p.d_tiles2a, k, par);
 }
 
 // Enqueues one whole forward run on `s`.
@@ -288,7 +297,7 @@ int enqueue_backward(Plan &p, hipStream_t s) {
     for (int k = 0; k < p.max_nt; k += 2) {
       const size_t j = (size_t)k / 2;
       const int par = (int)(j & 1);
-      const int64_t grid2 = p.active2[j];
+      const int64_t grid2 = p.active2a[j];
       const int64_t first = p.rem_a_first[j], end = p.rem_a_end[j];
       if (grid2 > 0) launch_pair_a(p, p.fmt_adj, grid2, s, k, par);
       if (end > first) launch_step_a(p, p.fmt_adj, end - first, s, k, (int)first, par);
@@ -391,12 +400,14 @@ int build_items(Plan &p, int rows, std::vector<BatchItem<float>> &items,
 
 // Two-step tiles (kOwn2 columns x kBatchRows2 rows, the same shot groups as
 // the one-step tiles): fills n_yt2 / tile0_2 / split2 of both passes' items
-// and returns the two-step tile -> item map; also the per-pair launch sizes.
-int build_items2(Plan &p, std::vector<int32_t> &tile_item) {
+// and returns the two-step tile -> item maps (forward: TY-row tiles; adjoint:
+// the ordered set with 2-row border tiles); also the per-pair launch sizes.
+int build_items2(Plan &p, std::vector<int32_t> &tile_item, std::vector<int32_t> &tile_item_a) {
   constexpr int TY = kBatchRows2;
   const size_t n = p.probs.size();
   tile_item.clear();
-  std::vector<int64_t> prefix(n + 1, 0);
+  tile_item_a.clear();
+  std::vector<int64_t> prefix(n + 1, 0), prefix_a(n + 1, 0);
   for (size_t m = 0; m < n; ++m) {
     BatchItem<float> &f = p.h_fwd[m];
     const int32_t n_yt2 = (int32_t)((p.probs[m].ny + TY - 1) / TY);
@@ -406,13 +417,21 @@ int build_items2(Plan &p, std::vector<int32_t> &tile_item) {
       return kInvalidArgument;
     }
     const TileSplit sp = make_split(f.g, f.pts2.n_xt, n_yt2, TY);
+    const int64_t tiles_a = ordered_tiles<TY, 2>(sp, f.pts2.n_xt, (int)p.probs[m].ny) * p.h_adj[m].n_groups;
+    if (prefix_a[m] + tiles_a >= ((int64_t)1 << 31)) {
+      set_error("batch: too many tiles for one launch");
+      return kInvalidArgument;
+    }
     for (BatchItem<float> *it : {&p.h_fwd[m], &p.h_adj[m]}) {
       it->n_yt2 = n_yt2;
-      it->tile0_2 = (int32_t)prefix[m];
       it->split2 = sp;
     }
+    p.h_fwd[m].tile0_2 = (int32_t)prefix[m];
+    p.h_adj[m].tile0_2 = (int32_t)prefix_a[m];
     tile_item.insert(tile_item.end(), (size_t)tiles, (int32_t)m);
+    tile_item_a.insert(tile_item_a.end(), (size_t)tiles_a, (int32_t)m);
     prefix[m + 1] = prefix[m] + tiles;
+    prefix_a[m + 1] = prefix_a[m] + tiles_a;
   }
   // problems are sorted by nt descending: count those with nt >= v
   auto count_at_least = [&](int64_t v) {
@@ -422,6 +441,7 @@ int build_items2(Plan &p, std::vector<int32_t> &tile_item) {
   };
   const size_t pairs = (size_t)(p.max_nt + 1) / 2;
   p.active2.assign(pairs, 0);
+  p.active2a.assign(pairs, 0);
   p.rem_f_first.assign(pairs, 0);
   p.rem_f_end.assign(pairs, 0);
   p.rem_a_first.assign(pairs, 0);
@@ -439,6 +459,7 @@ int build_items2(Plan &p, std::vector<int32_t> &tile_item) {
     const int64_t k = 2 * (int64_t)j;
     const size_t two = count_at_least(k + 2), one = count_at_least(k + 1);
     p.active2[j] = prefix[two];
+    p.active2a[j] = prefix_a[two];
     p.rem_f_first[j] = tile_start(p.h_fwd, two, end_f);
     p.rem_f_end[j] = tile_start(p.h_fwd, one, end_f);
     p.rem_a_first[j] = tile_start(p.h_adj, two, end_a);
@@ -623,8 +644,8 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
   std::vector<int32_t> tiles_f, tiles_a;
   PLAN_CHECK(build_items(*p, p->rows_f, p->h_fwd, tiles_f, p->active_fwd, proto));
   PLAN_CHECK(build_items(*p, p->rows_a, p->h_adj, tiles_a, p->active_adj, proto));
-  std::vector<int32_t> tiles2;
-  if (two) PLAN_CHECK(build_items2(*p, tiles2));
+  std::vector<int32_t> tiles2, tiles2a;
+  if (two) PLAN_CHECK(build_items2(*p, tiles2, tiles2a));
   // problems with several shot groups accumulate into per-group FP64 slices
   for (size_t m = 0; m < p->probs.size(); ++m) {
     BatchItem<float> &it = p->h_adj[m];
@@ -643,6 +664,9 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
   if (two) {
     PLAN_CHECK(plan_alloc(*p, &p->d_tiles2, tiles2.size()));
     TIDE_HIP_TRY(hipMemcpy(p->d_tiles2, tiles2.data(), tiles2.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice));
+    PLAN_CHECK(plan_alloc(*p, &p->d_tiles2a, tiles2a.size()));
+    TIDE_HIP_TRY(hipMemcpy(p->d_tiles2a, tiles2a.data(), tiles2a.size() * sizeof(int32_t),
                            hipMemcpyHostToDevice));
   }
   TIDE_HIP_TRY(hipMemcpy(p->d_fwd, p->h_fwd.data(), p->h_fwd.size() * sizeof(BatchItem<float>),
