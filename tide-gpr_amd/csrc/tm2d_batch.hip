@@ -63,12 +63,19 @@ __global__ __launch_bounds__(512) void fwd2_batch(const BatchItem<T> *__restrict
                                                   int par) {
   const int64_t flat = xcd_remap((int64_t)blockIdx.x, (int64_t)gridDim.x);
   const BatchItem<T> &it = items[tile_item[flat]];
-  const TileCoord tc = batch_tile((int)(flat - it.tile0_2), it.pts2.n_xt, it.n_yt2);
   const Fwd2Args<T> a = item_fwd2_args(it, k, par);
+  // ordered tiles with border tiles of the interior height (the same tile
+  // count as the TY-row grid): border tiles first, x-only checks beside the
+  // interior rows, as the single-problem forward
+  int edge = 3;
+  const TileCoord tc = ordered_tile<TY, TY>(a.split, it.pts2.n_xt, a.g.ny, it.n_groups,
+                                            flat - it.tile0_2, false, &edge);
   const int s = tc.grp * (int)blockDim.y + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
-  if (in_split<TY>(a.split, tc.xt, tc.yt))
+  if (edge == 0)
     fwd2_tile<T, TY, FMT, 0>(a, s, tc.xt, tc.yt);
+  else if (edge == 1)
+    fwd2_tile<T, TY, FMT, 1>(a, s, tc.xt, tc.yt);
   else
     fwd2_tile<T, TY, FMT, 3>(a, s, tc.xt, tc.yt);
 }

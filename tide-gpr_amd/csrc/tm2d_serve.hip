@@ -120,6 +120,29 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
 
 constexpr int kServeRows2 = 4;  // tile rows of the two-step engine launches
 
+// One forward tile of a two-step engine launch: edge 0 interior, 1 x checks
+// only, 3 x and y checks; fmt is the snapshot payload (0 none, 1 fp32, 2
+// BF16).
+template <int TY>
+__device__ __forceinline__ void fwd2_serve_tile(const Fwd2Args<float> &a, int s, const TileCoord &tc,
+                                                int edge, int fmt) {
+  auto run = [&](auto f) {
+    constexpr int F = decltype(f)::value;
+    if (edge == 0)
+      fwd2_tile<float, TY, F, 0>(a, s, tc.xt, tc.yt);
+    else if (edge == 1)
+      fwd2_tile<float, TY, F, 1>(a, s, tc.xt, tc.yt);
+    else
+      fwd2_tile<float, TY, F, 3>(a, s, tc.xt, tc.yt);
+  };
+  if (fmt == 0)
+    run(std::integral_constant<int, 0>{});
+  else if (fmt == 2)
+    run(std::integral_constant<int, 2>{});
+  else
+    run(std::integral_constant<int, 1>{});
+}
+
 // Two-step launch of pair q of the tick over the tiles [first, first +
 // gridDim.x) of jobs [0, n): each advances steps k0 + 2q and k0 + 2q + 1.
 // PH 0 runs forward jobs, PH 1 adjoint jobs (the tick orders its jobs by
@@ -152,15 +175,10 @@ __global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ 
     const int tile = local * (int)blockDim.y + w;
     if constexpr (PH == 0) {
       if (tile >= n_xt * it.b.n_yt2) return;  // uniform over the wavefront; no barriers here
-      const int yt = tile / n_xt, xt = tile - yt * n_xt;
       const Fwd2Args<float> a = item_fwd2_args(it.b, k, par);
-      const bool inner = in_split<TY>(a.split, xt, yt);
-      if (it.fmt == 0)
-        inner ? fwd2_tile<float, TY, 0, 0>(a, 0, xt, yt) : fwd2_tile<float, TY, 0, 3>(a, 0, xt, yt);
-      else if (it.fmt == 2)
-        inner ? fwd2_tile<float, TY, 2, 0>(a, 0, xt, yt) : fwd2_tile<float, TY, 2, 3>(a, 0, xt, yt);
-      else
-        inner ? fwd2_tile<float, TY, 1, 0>(a, 0, xt, yt) : fwd2_tile<float, TY, 1, 3>(a, 0, xt, yt);
+      int edge = 3;
+      const TileCoord tc = ordered_tile<TY, TY>(a.split, n_xt, a.g.ny, 1, tile, false, &edge);
+      fwd2_serve_tile<TY>(a, 0, tc, edge, it.fmt);
     } else {
       // ordered tiles: 2-row border tiles, TY-row interior tiles
       const Adj2Args<float> a = item_adj2_args(it.b, k, par);
@@ -186,17 +204,14 @@ __global__ __launch_bounds__(512) void serve_pair(const ServeItem *__restrict__ 
     return;
   }
   if constexpr (PH == 0) {
-    const TileCoord tc = batch_tile(local, it.b.pts2.n_xt, it.b.n_yt2);
+    // ordered tiles with border tiles of the interior height (the same tile
+    // count as the TY-row grid), as the single-problem forward
     const Fwd2Args<float> a = item_fwd2_args(it.b, k, par);
+    int edge = 3;
+    const TileCoord tc = ordered_tile<TY, TY>(a.split, it.b.pts2.n_xt, a.g.ny, it.b.n_groups, local, false, &edge);
     const int s = tc.grp * (int)blockDim.y + w;
     if (s >= a.n_shots) return;  // uniform over the wavefront; no barriers here
-    const bool inner = in_split<TY>(a.split, tc.xt, tc.yt);
-    if (it.fmt == 0)
-      inner ? fwd2_tile<float, TY, 0, 0>(a, s, tc.xt, tc.yt) : fwd2_tile<float, TY, 0, 3>(a, s, tc.xt, tc.yt);
-    else if (it.fmt == 2)
-      inner ? fwd2_tile<float, TY, 2, 0>(a, s, tc.xt, tc.yt) : fwd2_tile<float, TY, 2, 3>(a, s, tc.xt, tc.yt);
-    else
-      inner ? fwd2_tile<float, TY, 1, 0>(a, s, tc.xt, tc.yt) : fwd2_tile<float, TY, 1, 3>(a, s, tc.xt, tc.yt);
+    fwd2_serve_tile<TY>(a, s, tc, edge, it.fmt);
   } else {
     // ordered tiles, as the single-problem adjoint: 2-row border tiles keep
     // the register count of the CPML paths down
