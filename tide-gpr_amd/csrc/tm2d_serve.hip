@@ -515,6 +515,20 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
   // when max_steps is 1); jobs with an odd remainder end on a one-step launch
   const bool pairs = e.two_step && max_steps >= 2;
   if (pairs) max_steps &= ~(int64_t)1;
+  // Steps per job this tick: the largest running job (cells x shots per
+  // step) takes max_steps, a smaller one proportionally more, up to
+  // kTickBoost times: in lock-step launches a small job would otherwise move
+  // at the large one's pace.  Allotments are multiples of max_steps, so
+  // every non-final tick of a job is still an even number of steps.
+  constexpr int64_t kTickBoost = 16;
+  auto cells = [](const EJob &j) {
+    return (double)j.spec.p.n_shots * (double)j.spec.p.ny * (double)j.spec.p.nx;
+  };
+  double biggest = 0.0;
+  for (auto &kv : e.jobs) {
+    const EJob &j = kv.second;
+    if ((j.phase == 0 || j.phase == 2) && j.nt > j.k) biggest = std::max(biggest, cells(j));
+  }
   std::vector<std::pair<int64_t, EJob *>> run;  // (steps this tick, job)
   for (auto &kv : e.jobs) {
     EJob &j = kv.second;
@@ -525,7 +539,9 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
       e.done.push_back(j.id);
       continue;
     }
-    run.emplace_back(std::min<int64_t>(left, max_steps), &j);
+    const double ratio = biggest / std::max(cells(j), 1.0);
+    const int64_t boost = std::max<int64_t>(1, std::min<int64_t>(kTickBoost, (int64_t)ratio));
+    run.emplace_back(std::min<int64_t>(left, max_steps * boost), &j);
   }
   if (run.empty()) {
     *n_running = 0;

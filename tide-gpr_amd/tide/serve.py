@@ -357,8 +357,10 @@ class TM2DServer:
     max_jobs:
         Jobs resident at once.
     steps_per_tick:
-        Time steps each running job advances per engine tick (admission and
-        completion happen at tick boundaries).
+        Time steps the largest running job advances per engine tick
+        (admission and completion happen at tick boundaries); a smaller job
+        advances proportionally more (by the ratio of cells x shots per step,
+        up to 16 times), so it is not held to the large job's pace.
     group:
         Shots per workgroup, the same for every job of a launch (default 2:
         requests with few shots leave fewer idle wavefronts).
