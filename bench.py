@@ -34,7 +34,7 @@ for _p in (ROOT / "tide-gpr_amd", ROOT):
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per cell-update measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by a 1 GiB copy's counters) for the kernels as built, default
-# launch shapes; see profiles/tm2d_config2_hbm_counters.md.  Reported as
+# launch shapes; see profiles/tm2d_1024x1024_64shot_gradient_hbm_counters.md.  Reported as
 # roofline.traffic.
 MEASURED_TRAFFIC_B_PER_CELL = {"fwd2_step": 32.52, "adj2_step": 35.30}
 
