@@ -231,3 +231,17 @@ def test_autotune_size_classes():
             assert not (lp and steps == 2)  # two-step launches group shots per wavefront
             if steps == 2:
                 assert r <= 8
+
+
+def test_autotune_rejects_bad_chunk_lists_before_touching_a_device():
+    """A chunk list must hold positive shot counts; the check runs before any
+    device work (so it holds on a host without a GPU)."""
+    from tide.autotune import _pick, tune_tm2d
+    for bad in ([], [4, 0], [3, -1]):
+        with pytest.raises(ValueError):
+            tune_tm2d(64, 64, bad, device="cpu")
+    # the installed shape is kept within the margin, else the fastest wins
+    times = {(4, 4): 100.0, (4, 8): 98.0, (2, 2): 90.0}
+    assert _pick(times, (4, 4)) == (2, 2)
+    assert _pick({(4, 4): 100.0, (4, 8): 98.0}, (4, 4)) == (4, 4)
+    assert _pick({(4, 4): 100.0, (4, 8): 98.0}, (8, 8)) == (4, 8)
