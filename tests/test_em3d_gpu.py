@@ -29,9 +29,9 @@ DEV = "cuda:0"
 
 @pytest.fixture(autouse=True)
 def _default_tuning():
-    _native.set_em3d_tuning(8, 8, 8)
+    _native.set_em3d_tuning(8, 4, 8)  # the library's defaults
     yield
-    _native.set_em3d_tuning(8, 8, 8)
+    _native.set_em3d_tuning(8, 4, 8)
 
 
 def _case(dtype=torch.float32, **kw):

@@ -872,7 +872,9 @@ __global__ void fold_groups3(double *__restrict__ out, const double *__restrict_
 // ------------------------------------------------------------------ driver
 
 struct Tuning3 {
-  int fwd_z = 8, adj_z = 8, group = 8;
+  // adjoint 4 planes per wavefront: about 5% more gradient throughput than 8
+  // on a 281^3 grid with 4 shots
+  int fwd_z = 8, adj_z = 4, group = 8;
 };
 Tuning3 g_tuning3;
 
