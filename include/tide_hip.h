@@ -444,10 +444,17 @@ int tide_em3d_double_backward_f64acc(
 
 /*
  * Launch shape of the 3-D step kernels: z planes marched per wavefront
- * (forward 4/8/16, adjoint 4/8) and shots per workgroup (1..8).  No
- * reference counterpart.
+ * (forward 4/8/16, adjoint 4/8) and shots per workgroup (1..8, both
+ * passes).  Defaults: 8 / 4 planes, 1 shot per forward workgroup, 8 per
+ * adjoint workgroup.  No reference counterpart.
  */
 int tide_em3d_set_tuning(int fwd_z, int adj_z, int group);
+
+/*
+ * Shots per workgroup of the 3-D forward kernels alone (1..8), after
+ * tide_em3d_set_tuning.  No reference counterpart.
+ */
+int tide_em3d_set_forward_group(int group);
 
 #ifdef __cplusplus
 }

@@ -69,3 +69,14 @@ def test_launch_shape_setters_round_trip_and_reject_bad_values():
         _native.set_tm2d_time_block(*before[0])
         _native.set_tm2d_tile_split(*before[1])
         _native.set_tm2d_border_rows(*before[2])
+
+
+def test_em3d_forward_group_setter_rejects_bad_values():
+    """Host-only: the 3-D forward group takes 1..8 shots per workgroup."""
+    try:
+        for bad in (0, 9, -1):
+            with pytest.raises(_native.TideNativeError, match="forward group"):
+                _native.set_em3d_forward_group(bad)
+        _native.set_em3d_forward_group(2)
+    finally:
+        _native.set_em3d_forward_group(1)

@@ -30,8 +30,10 @@ DEV = "cuda:0"
 @pytest.fixture(autouse=True)
 def _default_tuning():
     _native.set_em3d_tuning(8, 4, 8)  # the library's defaults
+    _native.set_em3d_forward_group(1)
     yield
     _native.set_em3d_tuning(8, 4, 8)
+    _native.set_em3d_forward_group(1)
 
 
 def _case(dtype=torch.float32, **kw):

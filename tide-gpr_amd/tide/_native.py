@@ -223,6 +223,8 @@ def load() -> ctypes.CDLL:
         fn.argtypes = _em3d_backward_types(ctype)
     lib.tide_em3d_set_tuning.restype = ctypes.c_int
     lib.tide_em3d_set_tuning.argtypes = [ctypes.c_int] * 3
+    lib.tide_em3d_set_forward_group.restype = ctypes.c_int
+    lib.tide_em3d_set_forward_group.argtypes = [ctypes.c_int]
     lib.tide_tm2d_set_tuning.restype = ctypes.c_int
     lib.tide_tm2d_set_tuning.argtypes = [ctypes.c_int] * 4
     lib.tide_tm2d_get_tuning.restype = ctypes.c_int
@@ -297,6 +299,11 @@ def set_em3d_tuning(fwd_z: int, adj_z: int, group: int) -> None:
     """z planes marched per wavefront (forward, adjoint) and shots per
     workgroup of the 3-D kernels (see include/tide_hip.h)."""
     check(load().tide_em3d_set_tuning(fwd_z, adj_z, group))
+
+
+def set_em3d_forward_group(group: int) -> None:
+    """Shots per workgroup of the 3-D forward kernels alone (see include/tide_hip.h)."""
+    check(load().tide_em3d_set_forward_group(int(group)))
 
 
 def ptr_array(tensors) -> ctypes.Array:
@@ -391,6 +398,7 @@ __all__ = [
     "int64_array",
     "ptr_array",
     "set_em3d_tuning",
+    "set_em3d_forward_group",
     "set_tm2d_border_rows",
     "set_tm2d_loop_timing",
     "set_tm2d_shot_loop",
