@@ -75,14 +75,9 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--em3d-nt", type=int, default=100,
                     help="time steps of the 3-D measurement (0: skip)")
     ap.add_argument("--em3d-shots", type=int, default=4)
+    ap.add_argument("--accuracy-shots", type=int, default=4,
+                    help="shots of the fp32-vs-fp64 and BF16 error sample (0: skip)")
     return ap.parse_args()
-
-
-def bytes_per_shot(ny: int, nx: int, nt: int, snap_bytes: int) -> int:
-    numel = ny * nx
-    snapshots = 2 * nt * numel * snap_bytes
-    fields = (7 + 5 + 7 + 7 + 4) * numel * 4  # forward state + scratch, adjoint state + scratch
-    return snapshots + fields
 
 
 def cpu_baseline(args, setup_kwargs) -> dict:
@@ -443,6 +438,50 @@ def em3d_bench(dev, nt: int, shots: int) -> dict:
     }
 
 
+def accuracy_bench(dev, eps, eps_obs, sig, mu, dx, dt, pml, amp, src, rec, shots: int) -> dict:
+    """Error of the fp32 gradient step on a bounded sample of the headline
+    workload (the first ``shots`` shots, full grid and nt): receivers and
+    eps / sigma gradients of the same least-squares objective against an
+    FP64 run (the same HIP kernels in double precision, which match the
+    FP64 oracle to 1e-12 in the parity tests), and the extra gradient error
+    of BF16-compressed snapshots."""
+    import torch
+
+    import tide
+
+    disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
+
+    def run(dtype, compression=False):
+        c = lambda t: t.to(dtype)  # noqa: E731
+        exp = tide.Experiment(tide.Acquisition(src[:shots], rec[:shots]), c(amp[:shots]))
+        op = tide.MaxwellTM(disc, exp, storage=tide.StorageOptions(compression=compression))
+        with torch.no_grad():
+            obs = op(tide.EMModel(c(eps_obs), c(sig), c(mu))).receiver_data
+        with op.linearize(tide.EMModel(c(eps), c(sig), c(mu))) as lin:
+            value, g = tide.workflow.ReceiverObjective(obs).value_and_gradient(lin)
+            r = lin.primal.receiver_data
+        out = [x.detach().double().cpu() for x in (r, g.epsilon, g.sigma)]
+        del op, lin, g
+        torch.cuda.empty_cache()
+        return out + [float(value)]
+
+    rel = lambda a, b: float((a - b).norm() / b.norm())  # noqa: E731
+    cos = lambda a, b: float((a * b).sum() / (a.norm() * b.norm()))  # noqa: E731
+    r64, e64, s64, v64 = run(torch.float64)
+    r32, e32, s32, v32 = run(torch.float32)
+    rb, eb, sb, vb = run(torch.float32, "bf16")
+    return {
+        "sample": f"first {shots} shots of the workload, full grid and nt",
+        "fp32_vs_fp64": {"receivers_rel_l2": rel(r32, r64), "misfit_rel": abs(v32 - v64) / abs(v64),
+                         "grad_eps_rel_l2": rel(e32, e64), "grad_sigma_rel_l2": rel(s32, s64)},
+        "bf16_snapshots_vs_fp32": {"grad_eps_rel_l2": rel(eb, e32), "grad_sigma_rel_l2": rel(sb, s32),
+                                   "grad_eps_cosine": cos(eb, e32)},
+        "bf16_snapshots_vs_fp64": {"grad_eps_rel_l2": rel(eb, e64), "grad_sigma_rel_l2": rel(sb, s64)},
+        "fp64_reference": "the same HIP kernels in double precision (FP64 fields, FP64 gradient "
+                          "accumulation); parity-tested against the FP64 oracle to 1e-12",
+    }
+
+
 def main() -> None:
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -471,75 +510,51 @@ def main() -> None:
     ny_p, nx_p = n + 2 * pml + 1, n + 2 * pml + 1
 
     snap_bytes = 2 if args.compression == "bf16" else 4
-    per_shot = bytes_per_shot(ny_p, nx_p, nt, snap_bytes)
-    free, total_mem = torch.cuda.mem_get_info(dev)
-    cap = args.chunk or max(1, int(0.90 * free // per_shot))
-
-    def split(c):
-        k = math.ceil(n_shots / c)
-        base, extra = divmod(n_shots, k)
-        return [base + (1 if i < extra else 0) for i in range(k)]
-
     from tide.autotune import _install, current
+    from tide.maxwell.tm2d import device_snapshot_budget, plan_shot_chunks
 
+    disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
+    per_shot = 2 * nt * ny_p * nx_p * snap_bytes
+    # --chunk: a device byte limit that holds exactly that many shots'
+    # snapshots; by default the operator sizes its chunks from free HBM.
+    limit = args.chunk * per_shot if args.chunk else None
+    storage = tide.StorageOptions(compression=args.compression == "bf16", bytes_limit_device=limit)
+    op = tide.MaxwellTM(disc, tide.Experiment(tide.Acquisition(src, rec), amp), storage=storage)
+    with torch.no_grad():
+        observed = op(tide.EMModel(eps_obs, sig, mu)).receiver_data
+    torch.cuda.synchronize()
+    log(f"observed data ready ({n_shots} shots, one operator call)")
+    # the chunks the operator will run (same rule as inside the call)
+    budget = device_snapshot_budget(dev, limit, 21 * ny_p * nx_p * 4, n_shots)
     default_shapes = current()  # the library's defaults, restored for the other workloads
+    plan = plan_shot_chunks(n_shots, per_shot, budget, group=max(1, default_shapes[1][1]))
+    sizes = plan.sizes
     tuning = None
-    chunk_scores = None
     if not args.no_autotune:
         from tide.autotune import tune_tm2d
-        # Shots per chunk is tuned with the launch shapes: the largest chunk
-        # that fits HBM, and the largest multiple of the adjoint's default
-        # shots per workgroup below it (full workgroups).  Each option is
-        # scored by its tuned time per step summed over its chunks.
-        g = default_shapes[1][1]
-        options = [cap] if args.chunk else sorted({cap, max(g, cap // g * g) if cap >= g else cap},
-                                                  reverse=True)
-        best = None
-        chunk_scores = {}
-        for c in options:
-            _install(*default_shapes)  # each option starts from the same installed shape
-            t = tune_tm2d(ny_p, nx_p, split(c), device=dev, max_store_bytes=64 << 30)
-            chunk_scores[c] = t["chosen_us_per_step"]
-            log(f"chunk {c}: forward {t['forward']} adjoint {t['adjoint']} "
-                f"{t['chosen_us_per_step']} us per step over {len(split(c))} chunks "
-                f"(tuned in {t.get('seconds', 0)} s)")
-            if best is None or t["chosen_us_per_step"] < best[0]:
-                best = (t["chosen_us_per_step"], c, t)
-        _, cap, tuning = best
-        _install(tuning["forward"], tuning["adjoint"])
-        log(f"autotuned: chunk {cap}, forward {tuning['forward']} adjoint {tuning['adjoint']}")
+
+        t = tune_tm2d(ny_p, nx_p, sizes, device=dev, max_store_bytes=64 << 30)
+        log(f"autotuned on chunks {sizes}: forward {t['forward']} adjoint {t['adjoint']} "
+            f"{t['chosen_us_per_step']} us per step (tuned in {t.get('seconds', 0)} s)")
+        tuning = t
+        _install(t["forward"], t["adjoint"])
     else:
         fwd, adj = current()
         tuning = {"forward": fwd, "adjoint": adj, "cached": False}
-    sizes = split(cap)
     n_chunks = len(sizes)
-    bounds = [sum(sizes[:i]) for i in range(n_chunks + 1)]
 
-    disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
-    storage = tide.StorageOptions(compression=args.compression == "bf16")
-
-    def operator(lo, hi):
-        exp = tide.Experiment(tide.Acquisition(src[lo:hi], rec[lo:hi]), amp[lo:hi])
-        return tide.MaxwellTM(disc, exp, storage=storage)
-
-    ops = [operator(bounds[i], bounds[i + 1]) for i in range(n_chunks)]
-    with torch.no_grad():
-        observed = [op(tide.EMModel(eps_obs, sig, mu)).receiver_data for op in ops]
-    torch.cuda.synchronize()
-    log(f"observed data ready ({n_chunks} shot chunks of {sizes})")
-
-    e = eps.clone().requires_grad_(True)
-    s = sig.clone().requires_grad_(True)
+    objective = tide.workflow.ReceiverObjective(observed)
+    model = tide.EMModel(eps, sig, mu)
+    chunks_run = []
 
     def step():
-        e.grad = None
-        s.grad = None
-        misfit = torch.zeros((), device=dev, dtype=torch.float64)
-        for op, d_obs in zip(ops, observed):
-            r = op(tide.EMModel(e, s, mu)).receiver_data
-            loss = 0.5 * ((r.double() - d_obs.double()) ** 2).sum()
-            loss.backward()
-            misfit = misfit + loss.detach()
+        # the public API: J = 0.5 |d(m) - d_obs|^2 and dJ/d(eps, sigma) over all
+        # shots of one operator; the operator runs them in HBM-sized chunks
+        # (forward with snapshots, residual, exact adjoint per chunk), one
+        # FP64 gradient accumulator across chunks
+        with op.linearize(model) as lin:
+            misfit, grad = objective.value_and_gradient(lin)
+            chunks_run[:] = getattr(lin, "last_plan", {}).get("shot_chunks", [])
         return misfit
 
     for k in range(args.warmup):
@@ -619,7 +634,14 @@ def main() -> None:
     )
     log("timing the CPU baseline" if not args.no_cpu_baseline else "CPU baseline skipped")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args, setup_kwargs)
-    del ops, observed, e, s, setup_kwargs
+    del op, observed, model, setup_kwargs
+    torch.cuda.empty_cache()
+    accuracy = None
+    if args.accuracy_shots > 0:
+        log(f"accuracy sample: {args.accuracy_shots} shots in fp64, fp32 and BF16 snapshots")
+        accuracy = accuracy_bench(dev, eps, eps_obs, sig, mu, dx, dt, pml, amp, src, rec,
+                                  args.accuracy_shots)
+        log(f"accuracy: {accuracy}")
     # the shapes tuned above fit this workload's grid and shot chunks; the
     # mixed-size workloads below run with the library's defaults
     _install(*default_shapes)
@@ -650,8 +672,8 @@ def main() -> None:
             "workload": f"2D TM fwd+adj gradient, {n}x{n} model ({ny_p}x{nx_p} padded, CPML {pml}), "
                         f"{n_shots} shots x 256 receivers, nt {nt}, stencil 2, "
                         f"{'bf16' if snap_bytes == 2 else 'fp32'} snapshots",
-            "shot_chunks": sizes,
-            "chunk_scores_us_per_step": chunk_scores,
+            "shot_chunks": chunks_run or sizes,
+            "tuned_us_per_step": tuning.get("chosen_us_per_step"),
             "cell_updates_per_step": updates_per_step,
             "hbm_frac_whole_job": round(gcells * 1e9 * (24 + b_snap) / 1e9 / HBM_PEAK_GBS, 4),
             "misfit": float(misfit),
@@ -669,6 +691,7 @@ def main() -> None:
         "batched": batched,
         "config1": config1,
         "em3d": em3d,
+        "accuracy": accuracy,
     }
     print(json.dumps(out), flush=True)
 

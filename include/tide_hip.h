@@ -443,6 +443,191 @@ int tide_em3d_double_backward_f64acc(
     int64_t receiver_component, int64_t device, void *compute_stream);
 
 /*
+ * The reference's own 3-D entry points, argument for argument
+ * (src/tide/backend_utils.py:446-555: _3D_FORWARD_SPEC,
+ * _3D_FORWARD_WITH_STORAGE_SPEC, _3D_BACKWARD_SPEC; symbol names :771-781;
+ * CUDA implementations src/tide/csrc/em3d/maxwell_3d_cuda_inst.cu:3012-3289
+ * (forward), :3290-3765 (forward_with_storage), :4542-5260 (backward)), so
+ * the reference's maxwell3d_cuda.py binds them unchanged.  They run
+ * tide_em3d_* and return a status.  Snapshots are device resident: step t
+ * (t a multiple of step_ratio) goes to slot t / step_ratio of each *_s1
+ * buffer (shot_bytes_uncomp * n_shots bytes per slot); *_s3 and *_fn
+ * (host/disk staging) must be unused (storage_mode 0 or 3).  The backward
+ * adds dJ/dca, dJ/dcb to grad_ca / grad_cb (accumulated in FP64), writes
+ * grad_eps / grad_sigma (when non-NULL) from the totals with the
+ * reference's closed-form chain rule, and clears grad_f, grad_ca, grad_cb,
+ * grad_eps, grad_sigma first when zero_grad_on_entry is set.
+ * adjoint_memory_scratch, grad_ca_shot, grad_cb_shot, n_threads and
+ * execution_backend are unused; m_lambda_X_Y is the adjoint memory of the
+ * forward memory m_X_Y.  CPML memories, forward and adjoint, are in the
+ * reference's compact layout: [n_shots, ..] with only the strips [0, lo)
+ * and [max(lo, hi - 1), n) of the memory's own axis stored.  Mixed ca_batched / cb_batched with both
+ * gradients requested returns status 1.
+ */
+int maxwell_3d_2_float_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
+    float *m_hz_y, float *m_hy_z, float *m_hx_z, float *m_hz_x, float *m_hy_x,
+    float *m_hx_y, float *m_ey_z, float *m_ez_y, float *m_ez_x, float *m_ex_z,
+    float *m_ex_y, float *m_ey_x, float const *debye_a, float const *debye_b,
+    float const *debye_cp, float *pol_ex, float *pol_ey, float *pol_ez,
+    float *ex_prev, float *ey_prev, float *ez_prev, float *r, int64_t n_poles,
+    float const *az, float const *bz, float const *azh, float const *bzh,
+    float const *ay, float const *by, float const *ayh, float const *byh,
+    float const *ax, float const *bx, float const *axh, float const *bxh,
+    float const *kz, float const *kzh, float const *ky, float const *kyh,
+    float const *kx, float const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, float rdz, float rdy, float rdx, float dt,
+    int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_z0, int64_t pml_y0,
+    int64_t pml_x0, int64_t pml_z1, int64_t pml_y1, int64_t pml_x1,
+    int64_t source_component, int64_t receiver_component, int64_t n_threads,
+    int64_t device, int64_t execution_backend, void *compute_stream);
+
+int maxwell_3d_2_double_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ex, double *ey, double *ez, double *hx, double *hy, double *hz,
+    double *m_hz_y, double *m_hy_z, double *m_hx_z, double *m_hz_x,
+    double *m_hy_x, double *m_hx_y, double *m_ey_z, double *m_ez_y,
+    double *m_ez_x, double *m_ex_z, double *m_ex_y, double *m_ey_x,
+    double const *debye_a, double const *debye_b, double const *debye_cp,
+    double *pol_ex, double *pol_ey, double *pol_ez, double *ex_prev,
+    double *ey_prev, double *ez_prev, double *r, int64_t n_poles,
+    double const *az, double const *bz, double const *azh, double const *bzh,
+    double const *ay, double const *by, double const *ayh, double const *byh,
+    double const *ax, double const *bx, double const *axh, double const *bxh,
+    double const *kz, double const *kzh, double const *ky, double const *kyh,
+    double const *kx, double const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, double rdz, double rdy, double rdx, double dt,
+    int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_z0, int64_t pml_y0,
+    int64_t pml_x0, int64_t pml_z1, int64_t pml_y1, int64_t pml_x1,
+    int64_t source_component, int64_t receiver_component, int64_t n_threads,
+    int64_t device, int64_t execution_backend, void *compute_stream);
+
+int maxwell_3d_2_float_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ex, float *ey, float *ez, float *hx, float *hy, float *hz,
+    float *m_hz_y, float *m_hy_z, float *m_hx_z, float *m_hz_x, float *m_hy_x,
+    float *m_hx_y, float *m_ey_z, float *m_ez_y, float *m_ez_x, float *m_ex_z,
+    float *m_ex_y, float *m_ey_x, float *r, void *ex_s1, void *ex_s3,
+    char const *const *ex_fn, void *ey_s1, void *ey_s3,
+    char const *const *ey_fn, void *ez_s1, void *ez_s3,
+    char const *const *ez_fn, void *cx_s1, void *cx_s3,
+    char const *const *cx_fn, void *cy_s1, void *cy_s3,
+    char const *const *cy_fn, void *cz_s1, void *cz_s3,
+    char const *const *cz_fn, float const *az, float const *bz,
+    float const *azh, float const *bzh, float const *ay, float const *by,
+    float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *kz, float const *kzh,
+    float const *ky, float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdz,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t nz,
+    int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_z0, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_z1, int64_t pml_y1, int64_t pml_x1, int64_t source_component,
+    int64_t receiver_component, int64_t n_threads, int64_t device,
+    int64_t execution_backend, void *compute_stream, void *storage_stream);
+
+int maxwell_3d_2_double_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ex, double *ey, double *ez, double *hx, double *hy, double *hz,
+    double *m_hz_y, double *m_hy_z, double *m_hx_z, double *m_hz_x,
+    double *m_hy_x, double *m_hx_y, double *m_ey_z, double *m_ez_y,
+    double *m_ez_x, double *m_ex_z, double *m_ex_y, double *m_ey_x, double *r,
+    void *ex_s1, void *ex_s3, char const *const *ex_fn, void *ey_s1,
+    void *ey_s3, char const *const *ey_fn, void *ez_s1, void *ez_s3,
+    char const *const *ez_fn, void *cx_s1, void *cx_s3,
+    char const *const *cx_fn, void *cy_s1, void *cy_s3,
+    char const *const *cy_fn, void *cz_s1, void *cz_s3,
+    char const *const *cz_fn, double const *az, double const *bz,
+    double const *azh, double const *bzh, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *kz, double const *kzh,
+    double const *ky, double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdz,
+    double rdy, double rdx, double dt, int64_t nt, int64_t n_shots,
+    int64_t nz, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_z0, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_z1, int64_t pml_y1, int64_t pml_x1, int64_t source_component,
+    int64_t receiver_component, int64_t n_threads, int64_t device,
+    int64_t execution_backend, void *compute_stream, void *storage_stream);
+
+int maxwell_3d_2_float_backward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *grad_r,
+    float *lambda_ex, float *lambda_ey, float *lambda_ez, float *lambda_hx,
+    float *lambda_hy, float *lambda_hz, float *m_lambda_ey_z,
+    float *m_lambda_ez_y, float *m_lambda_ez_x, float *m_lambda_ex_z,
+    float *m_lambda_ex_y, float *m_lambda_ey_x, float *m_lambda_hz_y,
+    float *m_lambda_hy_z, float *m_lambda_hx_z, float *m_lambda_hz_x,
+    float *m_lambda_hy_x, float *m_lambda_hx_y, float *adjoint_memory_scratch,
+    void *ex_s1, void *ex_s3, char const *const *ex_fn, void *ey_s1,
+    void *ey_s3, char const *const *ey_fn, void *ez_s1, void *ez_s3,
+    char const *const *ez_fn, void *cx_s1, void *cx_s3,
+    char const *const *cx_fn, void *cy_s1, void *cy_s3,
+    char const *const *cy_fn, void *cz_s1, void *cz_s3,
+    char const *const *cz_fn, float *grad_f, float *grad_ca, float *grad_cb,
+    float *grad_eps, float *grad_sigma, float *grad_ca_shot,
+    float *grad_cb_shot, bool zero_grad_on_entry, float const *az,
+    float const *bz, float const *azh, float const *bzh, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *kz,
+    float const *kzh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdz, float rdy, float rdx, float dt, int64_t nt, int64_t n_shots,
+    int64_t nz, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_z0, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_z1, int64_t pml_y1, int64_t pml_x1, int64_t source_component,
+    int64_t receiver_component, int64_t n_threads, int64_t device,
+    int64_t execution_backend, void *compute_stream, void *storage_stream);
+
+int maxwell_3d_2_double_backward_cuda(
+    double const *ca, double const *cb, double const *cq,
+    double const *grad_r, double *lambda_ex, double *lambda_ey,
+    double *lambda_ez, double *lambda_hx, double *lambda_hy,
+    double *lambda_hz, double *m_lambda_ey_z, double *m_lambda_ez_y,
+    double *m_lambda_ez_x, double *m_lambda_ex_z, double *m_lambda_ex_y,
+    double *m_lambda_ey_x, double *m_lambda_hz_y, double *m_lambda_hy_z,
+    double *m_lambda_hx_z, double *m_lambda_hz_x, double *m_lambda_hy_x,
+    double *m_lambda_hx_y, double *adjoint_memory_scratch, void *ex_s1,
+    void *ex_s3, char const *const *ex_fn, void *ey_s1, void *ey_s3,
+    char const *const *ey_fn, void *ez_s1, void *ez_s3,
+    char const *const *ez_fn, void *cx_s1, void *cx_s3,
+    char const *const *cx_fn, void *cy_s1, void *cy_s3,
+    char const *const *cy_fn, void *cz_s1, void *cz_s3,
+    char const *const *cz_fn, double *grad_f, double *grad_ca,
+    double *grad_cb, double *grad_eps, double *grad_sigma,
+    double *grad_ca_shot, double *grad_cb_shot, bool zero_grad_on_entry,
+    double const *az, double const *bz, double const *azh, double const *bzh,
+    double const *ay, double const *by, double const *ayh, double const *byh,
+    double const *ax, double const *bx, double const *axh, double const *bxh,
+    double const *kz, double const *kzh, double const *ky, double const *kyh,
+    double const *kx, double const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, double rdz, double rdy, double rdx, double dt,
+    int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_z0, int64_t pml_y0, int64_t pml_x0, int64_t pml_z1,
+    int64_t pml_y1, int64_t pml_x1, int64_t source_component,
+    int64_t receiver_component, int64_t n_threads, int64_t device,
+    int64_t execution_backend, void *compute_stream, void *storage_stream);
+
+/*
  * Launch shape of the 3-D step kernels: z planes marched per wavefront
  * (forward 4/8/16, adjoint 4/8) and shots per workgroup (1..8, both
  * passes).  Defaults: 8 / 4 planes, 1 shot per forward workgroup, 8 per

@@ -109,3 +109,51 @@ def load_golden3d(name: str) -> tuple[O.EMCase, dict]:
     )
     expected = {"r": z["r"], **{k: z["out_" + k] for k in FIELD_NAMES_3D}}
     return case, expected
+
+
+def oracle_solve(problem: dict, observed=None, *, compression=False):
+    """The oracle's receivers and, given observed data, the FP64 misfit
+    ``0.5 |d - d_obs|^2`` and its epsilon / sigma gradients: oracle forward
+    and exact adjoint, then torch's chain rule (padding fold, coefficient
+    and source-term derivatives) on the CPU.  ``problem`` holds
+    ``maxwelltm`` keyword arguments on any device."""
+    cpu = {k: (v.detach().cpu() if isinstance(v, torch.Tensor) else v) for k, v in problem.items()}
+    want = observed is not None
+    eps = cpu["epsilon"].clone().requires_grad_(want)
+    sig = cpu["sigma"].clone().requires_grad_(want)
+    st = prepare_tm2d(eps, sig, cpu["mu"], cpu["grid_spacing"], cpu["dt"],
+                      cpu.get("source_amplitude"), cpu.get("source_location"),
+                      cpu.get("receiver_location"), pml_width=cpu.get("pml_width", 20),
+                      nt=cpu.get("nt"))
+    case = O.case_from_setup(st)
+    store = ("bf16" if compression else "full") if want else None
+    _, r, es, cs = O.tm2d_forward(case, store=store)
+    if not want:
+        return r, None, None, None
+    obs = observed.detach().cpu().double().numpy()
+    resid = r.astype(np.float64) - obs
+    misfit = 0.5 * float((resid**2).sum())
+    gca, gcb, gf, _ = O.tm2d_backward(case, resid.astype(r.dtype), es, cs, store=store)
+    dt_ = st.ca.dtype
+    outs, cots = [st.ca, st.cb], [torch.from_numpy(gca).to(dt_)[None], torch.from_numpy(gcb).to(dt_)[None]]
+    if st.n_src > 0:
+        outs.append(st.f)
+        cots.append(torch.from_numpy(gf))
+    torch.autograd.backward(outs, cots)
+    return r, misfit, eps.grad.numpy(), sig.grad.numpy()
+
+
+def truncate_problem(problem: dict, nt: int) -> dict:
+    """The same problem over its first ``nt`` steps."""
+    out = dict(problem)
+    out["source_amplitude"] = problem["source_amplitude"][..., :nt].contiguous()
+    return out
+
+
+def padded_work(problem: dict) -> float:
+    """Cell-updates of one sweep of ``problem`` (stencil 2, square CPML)."""
+    ny, nx = problem["epsilon"].shape[-2:]
+    w = problem.get("pml_width", 20)
+    w = w if isinstance(w, int) else w[0]
+    shots, _, nt = problem["source_amplitude"].shape
+    return float((ny + 2 * w + 1) * (nx + 2 * w + 1) * shots * nt)

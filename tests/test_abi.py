@@ -4,6 +4,7 @@ declared in include/*.h (no compute calls)."""
 from __future__ import annotations
 
 import ctypes
+import json
 import re
 from pathlib import Path
 
@@ -27,7 +28,86 @@ def test_header_declares_reference_entry_points():
     for d in ("float", "double"):
         for p in ("forward", "forward_with_storage", "backward"):
             assert f"maxwell_tm_2_{d}_{p}_cuda" in names
+            assert f"maxwell_3d_2_{d}_{p}_cuda" in names
     assert "tide_last_error" in names
+
+
+SPEC = json.loads((Path(__file__).resolve().parent / "golden" / "abi_spec.json").read_text())
+REF_ENTRY = [(prop, d, p) for prop in ("tm", "3d") for d in ("float", "double")
+             for p in ("forward", "forward_with_storage", "backward")]
+
+
+def _prototype(name: str) -> list[tuple[str, str]]:
+    """(kind, parameter name) of each parameter of ``name`` in the header:
+    P pointer, F float/double value, I int64_t, B bool."""
+    text = re.sub(r"/\*.*?\*/", "", (INCLUDE / "tide_hip.h").read_text(), flags=re.S)
+    m = re.search(rf"\b{name}\s*\((.*?)\);", text, flags=re.S)
+    assert m, name
+    out = []
+    for param in " ".join(m.group(1).split()).split(","):
+        param = param.strip()
+        pname = re.findall(r"(\w+)$", param)[0]
+        if "*" in param:
+            kind = "P"
+        elif param.startswith(("float", "double")):
+            kind = "F"
+        elif param.startswith("int64_t"):
+            kind = "I"
+        elif param.startswith("bool"):
+            kind = "B"
+        else:
+            raise AssertionError(f"{name}: unexpected parameter {param!r}")
+        out.append((kind, pname))
+    return out
+
+
+def _spec_key(prop: str, p: str) -> str:
+    return f"maxwell_{prop}_{p}"
+
+
+@pytest.mark.parametrize("prop,dtype,pass_name", REF_ENTRY)
+def test_prototypes_follow_the_reference_spec(prop, dtype, pass_name):
+    """Each reference-compatible prototype in include/tide_hip.h has the
+    parameters of the reference's ctypes spec (tests/golden/abi_spec.json,
+    written from src/tide/backend_utils.py): same kinds, same names, same
+    order."""
+    proto = _prototype(f"maxwell_{prop}_2_{dtype}_{pass_name}_cuda")
+    spec = [tuple(e.split(" ", 1)) for e in SPEC[_spec_key(prop, pass_name)]]
+    assert proto == spec
+
+
+@pytest.mark.parametrize("prop,dtype,pass_name", REF_ENTRY)
+def test_ctypes_bindings_follow_the_reference_spec(prop, dtype, pass_name):
+    """The argtypes bound by tide._native match the same spec."""
+    lib = _native.load()
+    fn = getattr(lib, f"maxwell_{prop}_2_{dtype}_{pass_name}_cuda")
+    f = ctypes.c_float if dtype == "float" else ctypes.c_double
+    kinds = {ctypes.c_void_p: "P", ctypes.c_int64: "I", ctypes.c_bool: "B", f: "F"}
+    got = [kinds[t] for t in fn.argtypes]
+    assert got == [e.split(" ", 1)[0] for e in SPEC[_spec_key(prop, pass_name)]]
+    assert fn.restype is ctypes.c_int
+
+
+def _default_args(prop: str, dtype: str, pass_name: str, **named):
+    spec = [tuple(e.split(" ", 1)) for e in SPEC[_spec_key(prop, pass_name)]]
+    default = {"P": None, "F": 0.0, "I": 0, "B": False}
+    args = [named.pop(n, default[k]) for k, n in spec]
+    assert not named, named
+    return args
+
+
+@pytest.mark.parametrize("prop", ["tm", "3d"])
+def test_backward_rejects_mixed_coefficient_batching(prop):
+    """A per-shot ca with a shared cb (both gradients wanted) would index
+    the shared gradient per shot; the entry point refuses it before any
+    device call."""
+    lib = _native.load()
+    fn = getattr(lib, f"maxwell_{prop}_2_float_backward_cuda")
+    args = _default_args(prop, "float", "backward", ca_requires_grad=True,
+                         cb_requires_grad=True, ca_batched=True, cb_batched=False, n_shots=2,
+                         nt=4)
+    assert fn(*args) == 1
+    assert b"same batching" in lib.tide_last_error()
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,109 @@
+"""The batched and served paths on draws of their benchmark distributions,
+against the CPU oracle.
+
+* Batched uneven models: six models of the batched distribution (grid
+  sides log-uniform 128-1024 per axis, 1-16 shots, the seeded layered +
+  anomaly generator), nt cut to 200 so the oracle stays cheap, run as one
+  batched plan; each model's receivers, FP64 misfit and epsilon / sigma
+  gradients against the oracle's forward, exact adjoint and torch chain
+  rule.
+* Served solves: eight requests of the served distribution (sides
+  128-1536, 1-32 shots, forward and gradient requests), nt cut to 120,
+  all queued at once on one server so they share launches; each result
+  against the oracle.
+
+Tolerances: receivers relative L2 1e-6, misfit 1e-6, gradients 1e-4
+(fp32 fields, the operator-level bound).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import oracle_solve, padded_work, relative_l2, truncate_problem
+from tide.batch import TM2DBatch
+from tide.maxwell.tm2d import maxwelltm
+from tide.serve import TM2DServer, served_trace
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _to(problem, dev):
+    return {k: (v.to(dev) if isinstance(v, torch.Tensor) else v) for k, v in problem.items()}
+
+
+def test_batched_distribution_draw_matches_oracle():
+    trace = served_trace(6, math.inf, seed=32, side_range=(128, 1024), nt_range=(500, 4000),
+                         shot_range=(1, 16), gradient_fraction=1.0, n_receivers=64)
+    problems = [truncate_problem(kw["problem"], 200) for _, kw in trace]
+    shapes = [(tuple(p["epsilon"].shape), p["source_amplitude"].shape[0]) for p in problems]
+    print(f"models (side, shots): {shapes}, {sum(map(padded_work, problems)):.2e} "
+          "cell-updates per sweep", flush=True)
+    # observed data: the same models with a perturbed permittivity
+    with torch.no_grad():
+        observed = [maxwelltm(**_to(dict(p, epsilon=p["epsilon"] * 1.02), DEV))[-1].cpu()
+                    for p in problems]
+    plan = TM2DBatch([_to(p, DEV) for p in problems])
+    misfit, ge, gs = plan.gradient([o.to(DEV) for o in observed])
+    plan.reset_state()
+    rs = [r.detach().cpu().numpy() for r in plan.forward()]
+    torch.cuda.synchronize()
+    for i, (p, obs) in enumerate(zip(problems, observed)):
+        t0 = time.perf_counter()
+        r_o, m_o, ge_o, gs_o = oracle_solve(p, obs)
+        e_r = relative_l2(rs[i], r_o)
+        e_m = abs(float(misfit[i]) - m_o) / abs(m_o)
+        e_e = relative_l2(ge[i].cpu().numpy(), ge_o)
+        e_s = relative_l2(gs[i].cpu().numpy(), gs_o)
+        print(f"model {i} {shapes[i]}: receivers {e_r:.2e} misfit {e_m:.2e} grad eps {e_e:.2e} "
+              f"sigma {e_s:.2e} (oracle {time.perf_counter() - t0:.1f} s)", flush=True)
+        assert e_r <= 1e-6 and e_m <= 1e-6 and e_e <= 1e-4 and e_s <= 1e-4
+    plan.close()
+
+
+def test_served_distribution_mix_matches_oracle():
+    trace = served_trace(8, math.inf, seed=47, n_receivers=64)
+    reqs = []
+    for _, kw in trace:
+        p = truncate_problem(kw["problem"], 120)
+        if kw["kind"] == "gradient":
+            n_rec = p["receiver_location"].shape[1]
+            g = torch.Generator().manual_seed(len(reqs))
+            obs = 1e-3 * torch.randn(120, p["source_amplitude"].shape[0], n_rec, generator=g)
+            reqs.append((p, obs))
+        else:
+            reqs.append((p, None))
+    kinds = ["gradient" if o is not None else "forward" for _, o in reqs]
+    print("requests: " + ", ".join(
+        f"{k[0]} {tuple(p['epsilon'].shape)}x{p['source_amplitude'].shape[0]}"
+        for k, (p, _) in zip(kinds, reqs)), flush=True)
+    assert "gradient" in kinds and "forward" in kinds
+    server = TM2DServer(DEV, hbm_budget=64 << 30, steps_per_tick=32)
+    submitted = []
+    for (p, obs), kind in zip(reqs, kinds):
+        kw = {"kind": kind}
+        if obs is not None:
+            kw["observed"] = obs.to(DEV)
+        submitted.append((0.0, dict(problem=_to(p, DEV), **kw)))
+    tickets = server.serve(submitted)
+    torch.cuda.synchronize()
+    for i, ((p, obs), t) in enumerate(zip(reqs, tickets)):
+        t0 = time.perf_counter()
+        r_o, m_o, ge_o, gs_o = oracle_solve(p, obs)
+        e_r = relative_l2(t.result.receiver_data.cpu().numpy(), r_o)
+        msg = f"request {i} {kinds[i]}: receivers {e_r:.2e}"
+        assert e_r <= 1e-6
+        if obs is not None:
+            e_m = abs(float(t.result.misfit) - m_o) / abs(m_o)
+            e_e = relative_l2(t.result.grad_epsilon.cpu().numpy(), ge_o)
+            e_s = relative_l2(t.result.grad_sigma.cpu().numpy(), gs_o)
+            msg += f" misfit {e_m:.2e} grad eps {e_e:.2e} sigma {e_s:.2e}"
+            assert e_m <= 1e-6 and e_e <= 1e-4 and e_s <= 1e-4
+        print(msg + f" (oracle {time.perf_counter() - t0:.1f} s)", flush=True)
+    server.close()

@@ -1096,6 +1096,12 @@ int backward_impl(Problem3<T> p, const T *grad_r, T *const *lam, const void *con
     set_error("coefficient gradients require the stored forward snapshots");
     return kInvalidArgument;
   }
+  // One gradient layout serves both accumulators (per shot when a
+  // coefficient is per shot); mixed batching would overrun the shared one.
+  if (gca && gcb && p.g.ca_b != p.g.cb_b) {
+    set_error("ca and cb gradients need the same batching (ca_batched == cb_batched)");
+    return kInvalidArgument;
+  }
   Scratch scr(stream);
   DevicePointTable dpt;
   TIDE_CHECK(prepare(p, src, rec, scr, dpt, stream));
@@ -1323,6 +1329,337 @@ int em3d_backward(const T *ca, const T *cb, const T *cq, const T *grad_r, T *con
 
 TIDE_EM3D_ENTRY_POINTS(float, float)
 TIDE_EM3D_ENTRY_POINTS(double, double)
+
+// ------------------------------------------------ reference-order 3-D ABI
+//
+// maxwell_3d_2_{float,double}_{forward,forward_with_storage,backward}_cuda
+// take the reference's arguments in the order of its ctypes specs
+// (src/tide/backend_utils.py:446-555: _3D_FORWARD_SPEC,
+// _3D_FORWARD_WITH_STORAGE_SPEC, _3D_BACKWARD_SPEC) and return a status
+// instead of void.  They run the tide_em3d_* drivers above.  Snapshots are
+// device resident (storage_mode 0): the snapshot of step t goes to slot
+// t / step_ratio of each *_s1 buffer, as in the reference's device mode.
+// CPML memories (and their adjoint counterparts) are in the reference's
+// compact layout.  Debye dispersion, host/disk staging and the
+// reference's scratch and per-shot gradient buffers are not used (the
+// adjoint keeps its own ping-pong memories and FP64 accumulators).
+
+namespace {
+
+constexpr int64_t kRefStorageDevice = 0;
+constexpr int64_t kRefStorageNone = 3;
+
+// d(ca, cb)/d(eps_r, sigma) chain rule of one cell (reference
+// maxwell_3d_cuda_inst.cu:2729-2777 and utils.py:185-217):
+//   dca/deps = (1 - ca) cb / dt, dcb/deps = -cb^2 / dt (per unit of
+//   absolute permittivity, times EP0 per unit of eps_r),
+//   dca/dsigma = -(1 + ca) cb / 2, dcb/dsigma = -cb^2 / 2.
+template <typename T>
+__global__ void coeff_grad_to_material3(const T *__restrict__ ca, const T *__restrict__ cb,
+                                        const T *__restrict__ gca, const T *__restrict__ gcb,
+                                        T *__restrict__ geps, T *__restrict__ gsig, T dt,
+                                        int64_t numel, int64_t total, int ca_b, int cb_b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int64_t shot = ca_b ? i / numel : 0;
+  const int64_t j = i - shot * numel;
+  const T a = ca[ca_b ? i : j];
+  const T b = cb[cb_b ? shot * numel + j : j];
+  const T b2 = b * b;
+  const T inv_dt = (T)1 / dt;
+  const T g_a = gca ? gca[i] : (T)0;
+  const T g_b = gcb ? gcb[i] : (T)0;
+  const T da_de = ((T)1 - a) * b * inv_dt;
+  const T db_de = -b2 * inv_dt;
+  const T da_ds = -((T)0.5) * ((T)1 + a) * b;
+  const T db_ds = -((T)0.5) * b2;
+  if (geps) geps[i] = (g_a * da_de + g_b * db_de) * (T)8.8541878128e-12;
+  if (gsig) gsig[i] = g_a * da_ds + g_b * db_ds;
+}
+
+// The reference keeps each 3-D CPML memory compact along the axis of its
+// strips (src/tide/grid_utils.py:96-160, maxwell_3d_cuda_inst.cu:307-363):
+// along that axis only the cells [0, lo) and [hs, n) are stored, hs =
+// max(lo, hi - 1), as [n_shots, .., lo + n - hs, ..].  The drivers here keep
+// full-size memories and touch only the strips, so the reference-order
+// entry points copy the strips between the two layouts.
+__host__ __device__ inline int memory_axis3(int slot) {
+  // slots M_HZ_Y .. M_EY_X: y z z x x y z y x z y x
+  constexpr int axis[12] = {1, 0, 0, 2, 2, 1, 0, 1, 2, 0, 1, 2};
+  return axis[slot - M_HZ_Y];
+}
+
+template <typename T>
+__global__ void strip_copy3(T *__restrict__ full, T *__restrict__ compact, int64_t n_shots,
+                            int nz, int ny, int nx, int axis, int lo, int hs, int to_full) {
+  const int n[3] = {nz, ny, nx};
+  const int ext = lo + n[axis] - hs;
+  int dims[3] = {nz, ny, nx};
+  dims[axis] = ext;
+  const int64_t total = n_shots * (int64_t)dims[0] * dims[1] * dims[2];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  int64_t rest = i;
+  int c[3];
+  c[2] = (int)(rest % dims[2]); rest /= dims[2];
+  c[1] = (int)(rest % dims[1]); rest /= dims[1];
+  c[0] = (int)(rest % dims[0]); rest /= dims[0];
+  const int64_t shot = rest;
+  if (c[axis] >= lo) c[axis] += hs - lo;
+  const int64_t j = ((shot * nz + c[0]) * ny + c[1]) * (int64_t)nx + c[2];
+  if (to_full) full[j] = compact[i];
+  else compact[i] = full[j];
+}
+
+// Full-size working copies of the 12 compact memories of `state`
+// (slots M_HZ_Y..M_EY_X), filled from them; `back` copies them back.
+template <typename T> struct FullMemories {
+  T *full[N_SLOTS] = {};
+  int64_t n_shots = 0;
+  int n[3] = {0, 0, 0}, lo[3] = {0, 0, 0}, hs[3] = {0, 0, 0};
+  int open(T *const *state, int64_t shots, int nz, int ny, int nx, const int64_t *pml,
+           Scratch &scr, hipStream_t stream) {
+    n_shots = shots;
+    n[0] = nz; n[1] = ny; n[2] = nx;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = (int)pml[a];
+      hs[a] = std::max<int>((int)pml[a], (int)pml[3 + a] - 1);
+      if (lo[a] < 0 || hs[a] > n[a]) {
+        set_error("pml bounds outside the padded grid");
+        return kInvalidArgument;
+      }
+    }
+    const int64_t numel = (int64_t)nz * ny * nx;
+    for (int k = M_HZ_Y; k < N_SLOTS; ++k) {
+      TIDE_CHECK(scr.alloc(&full[k], (size_t)(shots * numel)));
+      if (state[k] != nullptr) TIDE_CHECK(copy(k, state[k], 1, stream));
+    }
+    return kOk;
+  }
+  int copy(int k, T *compact, int to_full, hipStream_t stream) {
+    const int a = memory_axis3(k);
+    int dims[3] = {n[0], n[1], n[2]};
+    dims[a] = lo[a] + n[a] - hs[a];
+    const int64_t total = n_shots * (int64_t)dims[0] * dims[1] * dims[2];
+    if (total == 0) return kOk;
+    strip_copy3<T><<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(
+        full[k], compact, n_shots, n[0], n[1], n[2], a, lo[a], hs[a], to_full);
+    TIDE_HIP_TRY(hipGetLastError());
+    return kOk;
+  }
+  int back(T *const *state, hipStream_t stream) {
+    for (int k = M_HZ_Y; k < N_SLOTS; ++k)
+      if (state[k] != nullptr) TIDE_CHECK(copy(k, state[k], 0, stream));
+    return kOk;
+  }
+};
+
+template <typename T>
+int ref3_forward(const T *ca, const T *cb, const T *cq, const T *f, T *const fields[18], T *r,
+                 const T *const prof[18], const int64_t *src, const int64_t *rec, T rdz, T rdy,
+                 T rdx, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,
+                 int64_t n_src, int64_t n_rec, int64_t step_ratio, bool ca_b, bool cb_b,
+                 bool cq_b, int64_t start_t, const int64_t pml[6], int64_t src_comp,
+                 int64_t rec_comp, int64_t device, void *stream, void *const stores[6],
+                 int64_t storage_format, int64_t shot_bytes) {
+  TIDE_CHECK(set_device(device));
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  FullMemories<T> mem;
+  TIDE_CHECK(mem.open(fields, n_shots, (int)nz, (int)ny, (int)nx, pml, scr, s));
+  T *work[N_SLOTS];
+  for (int k = 0; k < N_SLOTS; ++k) work[k] = k < M_HZ_Y ? fields[k] : mem.full[k];
+  TIDE_CHECK(em3d_forward<T>(ca, cb, cq, f, work, r, prof, src, rec, rdz, rdy, rdx, nt, n_shots,
+                             nz, ny, nx, n_src, n_rec, step_ratio, ca_b, cb_b, cq_b, start_t,
+                             pml, src_comp, rec_comp, stores, storage_format,
+                             shot_bytes * n_shots, 0, device, stream));
+  return mem.back(fields, s);
+}
+
+template <typename T>
+int ref3_backward(const T *ca, const T *cb, const T *cq, const T *grad_r, T *const lam[18],
+                  const void *const stores[6], T *grad_f, T *grad_ca, T *grad_cb, T *grad_eps,
+                  T *grad_sigma, bool zero_grad_on_entry, const T *const prof[18],
+                  const int64_t *src, const int64_t *rec, T rdz, T rdy, T rdx, T dt, int64_t nt,
+                  int64_t n_shots, int64_t nz, int64_t ny, int64_t nx, int64_t n_src,
+                  int64_t n_rec, int64_t step_ratio, int64_t storage_mode,
+                  int64_t storage_format, int64_t shot_bytes, bool ca_rg, bool cb_rg,
+                  bool ca_b, bool cb_b, bool cq_b, int64_t start_t, const int64_t pml[6],
+                  int64_t src_comp, int64_t rec_comp, int64_t device, void *stream_handle) {
+  if (ca_rg && cb_rg && ca_b != cb_b) {  // checked before any device call
+    set_error("ca and cb gradients need the same batching (ca_batched == cb_batched)");
+    return kInvalidArgument;
+  }
+  const bool want = ca_rg || cb_rg;
+  if (want && storage_mode != kRefStorageDevice) {
+    set_error(storage_mode == kRefStorageNone
+                  ? "model gradients need stored snapshots (storage_mode 'none')"
+                  : "host- and disk-staged 3-D snapshots are not built; use storage_mode 'device'");
+    return storage_mode == kRefStorageNone ? kInvalidArgument : kUnsupported;
+  }
+  TIDE_CHECK(set_device(device));
+  hipStream_t stream = (hipStream_t)stream_handle;
+  const int64_t numel = nz * ny * nx;
+  const int64_t na = ca_b ? n_shots * numel : numel;
+  const int64_t nb = cb_b ? n_shots * numel : numel;
+  if (zero_grad_on_entry) {
+    if (grad_f && nt > 0 && n_src > 0)
+      TIDE_HIP_TRY(hipMemsetAsync(grad_f, 0, (size_t)(nt * n_shots * n_src) * sizeof(T), stream));
+    if (ca_rg && grad_ca) TIDE_HIP_TRY(hipMemsetAsync(grad_ca, 0, (size_t)na * sizeof(T), stream));
+    if (cb_rg && grad_cb) TIDE_HIP_TRY(hipMemsetAsync(grad_cb, 0, (size_t)nb * sizeof(T), stream));
+    if (want && grad_eps) TIDE_HIP_TRY(hipMemsetAsync(grad_eps, 0, (size_t)na * sizeof(T), stream));
+    if (want && grad_sigma)
+      TIDE_HIP_TRY(hipMemsetAsync(grad_sigma, 0, (size_t)na * sizeof(T), stream));
+  }
+  if (want && ((ca_rg && !grad_ca) || (cb_rg && !grad_cb))) {
+    set_error("grad_ca / grad_cb buffers are required for the requested gradients");
+    return kInvalidArgument;
+  }
+  Scratch scr(stream);
+  double *ga = nullptr, *gb = nullptr;
+  if (ca_rg) {
+    TIDE_CHECK(scr.alloc(&ga, (size_t)na));
+    TIDE_HIP_TRY(hipMemsetAsync(ga, 0, (size_t)na * sizeof(double), stream));
+  }
+  if (cb_rg) {
+    TIDE_CHECK(scr.alloc(&gb, (size_t)nb));
+    TIDE_HIP_TRY(hipMemsetAsync(gb, 0, (size_t)nb * sizeof(double), stream));
+  }
+  FullMemories<T> mem;
+  TIDE_CHECK(mem.open(lam, n_shots, (int)nz, (int)ny, (int)nx, pml, scr, stream));
+  T *work[N_SLOTS];
+  for (int k = 0; k < N_SLOTS; ++k) work[k] = k < M_HZ_Y ? lam[k] : mem.full[k];
+  TIDE_CHECK(em3d_backward<T>(ca, cb, cq, grad_r, work, want ? stores : nullptr, storage_format,
+                              shot_bytes * n_shots, 0, grad_f, ga, gb, prof, src, rec, rdz, rdy,
+                              rdx, nt, n_shots, nz, ny, nx, n_src, n_rec, step_ratio, ca_b,
+                              cb_b, cq_b, start_t, pml, src_comp, rec_comp, device,
+                              stream_handle));
+  TIDE_CHECK(mem.back(lam, stream));
+  const unsigned blocks_a = (unsigned)((na + 255) / 256), blocks_b = (unsigned)((nb + 255) / 256);
+  if (ca_rg) fold_grad3<T><<<blocks_a, 256, 0, stream>>>(grad_ca, ga, na);
+  if (cb_rg) fold_grad3<T><<<blocks_b, 256, 0, stream>>>(grad_cb, gb, nb);
+  if (want && (grad_eps || grad_sigma))
+    coeff_grad_to_material3<T><<<blocks_a, 256, 0, stream>>>(
+        ca, cb, ca_rg ? grad_ca : nullptr, cb_rg ? grad_cb : nullptr, grad_eps, grad_sigma, dt,
+        numel, na, ca_b ? 1 : 0, cb_b ? 1 : 0);
+  TIDE_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace
+
+#define TIDE_REF3_FIELDS(T, P)                                                                  \
+  T *P##ex, T *P##ey, T *P##ez, T *P##hx, T *P##hy, T *P##hz, T *P##m_hz_y, T *P##m_hy_z,   \
+      T *P##m_hx_z, T *P##m_hz_x, T *P##m_hy_x, T *P##m_hx_y, T *P##m_ey_z, T *P##m_ez_y,   \
+      T *P##m_ez_x, T *P##m_ex_z, T *P##m_ex_y, T *P##m_ey_x
+#define TIDE_REF3_PROFILES(T)                                                                   \
+  T const *az, T const *bz, T const *azh, T const *bzh, T const *ay, T const *by,            \
+      T const *ayh, T const *byh, T const *ax, T const *bx, T const *axh, T const *bxh,      \
+      T const *kz, T const *kzh, T const *ky, T const *kyh, T const *kx, T const *kxh,       \
+      int64_t const *sources_i, int64_t const *receivers_i
+#define TIDE_REF3_PROF_ARRAY(T)                                                                 \
+  const T *prof[18] = {az, bz, azh, bzh, ay, by, ayh, byh, ax, bx, axh, bxh,                 \
+                       kz, kzh, ky, kyh, kx, kxh}
+#define TIDE_REF3_TAIL(T)                                                                       \
+  T rdz, T rdy, T rdx, T dt, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx, \
+      int64_t n_src, int64_t n_rec, int64_t step_ratio
+#define TIDE_REF3_FLAGS                                                                      \
+  bool ca_b, bool cb_b, bool cq_b, int64_t start_t, int64_t pml_z0, int64_t pml_y0,         \
+      int64_t pml_x0, int64_t pml_z1, int64_t pml_y1, int64_t pml_x1,                        \
+      int64_t source_component, int64_t receiver_component, int64_t n_threads,               \
+      int64_t device, int64_t execution_backend
+#define TIDE_REF3_STORES(P)                                                                  \
+  void *P##ex_s1, void *P##ex_s3, char const *const *P##ex_fn, void *P##ey_s1,               \
+      void *P##ey_s3, char const *const *P##ey_fn, void *P##ez_s1, void *P##ez_s3,           \
+      char const *const *P##ez_fn, void *P##cx_s1, void *P##cx_s3,                           \
+      char const *const *P##cx_fn, void *P##cy_s1, void *P##cy_s3,                           \
+      char const *const *P##cy_fn, void *P##cz_s1, void *P##cz_s3, char const *const *P##cz_fn
+
+#define TIDE_REF3_ENTRY_POINTS(T, NAME)                                                      \
+  extern "C" int maxwell_3d_2_##NAME##_forward_cuda(                                        \
+      T const *ca, T const *cb, T const *cq, T const *f, TIDE_REF3_FIELDS(T, ),                \
+      T const *debye_a, T const *debye_b, T const *debye_cp, T *pol_ex, T *pol_ey,          \
+      T *pol_ez, T *ex_prev, T *ey_prev, T *ez_prev, T *r, int64_t n_poles,                 \
+      TIDE_REF3_PROFILES(T), TIDE_REF3_TAIL(T), bool has_dispersion, TIDE_REF3_FLAGS,             \
+      void *compute_stream) {                                                               \
+    (void)debye_a; (void)debye_b; (void)debye_cp; (void)pol_ex; (void)pol_ey;             \
+    (void)pol_ez; (void)ex_prev; (void)ey_prev; (void)ez_prev; (void)n_poles; (void)dt;   \
+    (void)n_threads; (void)execution_backend;                                               \
+    if (has_dispersion) {                                                                   \
+      set_error("Debye dispersion is not part of this build");                              \
+      return kUnsupported;                                                                  \
+    }                                                                                       \
+    T *const fields[18] = {ex, ey, ez, hx, hy, hz, m_hz_y, m_hy_z, m_hx_z, m_hz_x,         \
+                           m_hy_x, m_hx_y, m_ey_z, m_ez_y, m_ez_x, m_ex_z, m_ex_y, m_ey_x}; \
+    TIDE_REF3_PROF_ARRAY(T);                                                                   \
+    const int64_t pml[6] = {pml_z0, pml_y0, pml_x0, pml_z1, pml_y1, pml_x1};               \
+    return ref3_forward<T>(ca, cb, cq, f, fields, r, prof, sources_i, receivers_i, rdz,     \
+                           rdy, rdx, nt, n_shots, nz, ny, nx, n_src, n_rec, step_ratio,     \
+                           ca_b, cb_b, cq_b, start_t, pml, source_component,                \
+                           receiver_component, device, compute_stream, nullptr, 0, 0);      \
+  }                                                                                         \
+  extern "C" int maxwell_3d_2_##NAME##_forward_with_storage_cuda(                           \
+      T const *ca, T const *cb, T const *cq, T const *f, TIDE_REF3_FIELDS(T, ), T *r,          \
+      TIDE_REF3_STORES(), TIDE_REF3_PROFILES(T), TIDE_REF3_TAIL(T), int64_t storage_mode,        \
+      int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_rg, bool cb_rg,           \
+      TIDE_REF3_FLAGS, void *compute_stream, void *storage_stream) {                        \
+    (void)ex_s3; (void)ey_s3; (void)ez_s3; (void)cx_s3; (void)cy_s3; (void)cz_s3;          \
+    (void)ex_fn; (void)ey_fn; (void)ez_fn; (void)cx_fn; (void)cy_fn; (void)cz_fn;          \
+    (void)dt; (void)n_threads; (void)execution_backend; (void)storage_stream;              \
+    const bool want = ca_rg || cb_rg;                                                       \
+    if (want && storage_mode != kRefStorageDevice && storage_mode != kRefStorageNone) {    \
+      set_error("host- and disk-staged 3-D snapshots are not built; use storage_mode "      \
+                "'device'");                                                                \
+      return kUnsupported;                                                                  \
+    }                                                                                       \
+    T *const fields[18] = {ex, ey, ez, hx, hy, hz, m_hz_y, m_hy_z, m_hx_z, m_hz_x,         \
+                           m_hy_x, m_hx_y, m_ey_z, m_ez_y, m_ez_x, m_ex_z, m_ex_y, m_ey_x}; \
+    TIDE_REF3_PROF_ARRAY(T);                                                                   \
+    const int64_t pml[6] = {pml_z0, pml_y0, pml_x0, pml_z1, pml_y1, pml_x1};               \
+    const bool store = want && storage_mode == kRefStorageDevice;                          \
+    void *const stores[6] = {store && ca_rg ? ex_s1 : nullptr, store && ca_rg ? ey_s1 : nullptr, \
+                             store && ca_rg ? ez_s1 : nullptr, store && cb_rg ? cx_s1 : nullptr, \
+                             store && cb_rg ? cy_s1 : nullptr, store && cb_rg ? cz_s1 : nullptr}; \
+    return ref3_forward<T>(ca, cb, cq, f, fields, r, prof, sources_i, receivers_i, rdz,     \
+                           rdy, rdx, nt, n_shots, nz, ny, nx, n_src, n_rec, step_ratio,     \
+                           ca_b, cb_b, cq_b, start_t, pml, source_component,                \
+                           receiver_component, device, compute_stream,                      \
+                           store ? stores : nullptr, storage_format, shot_bytes_uncomp);    \
+  }                                                                                         \
+  extern "C" int maxwell_3d_2_##NAME##_backward_cuda(                                       \
+      T const *ca, T const *cb, T const *cq, T const *grad_r, T *lambda_ex, T *lambda_ey,   \
+      T *lambda_ez, T *lambda_hx, T *lambda_hy, T *lambda_hz, T *m_lambda_ey_z,             \
+      T *m_lambda_ez_y, T *m_lambda_ez_x, T *m_lambda_ex_z, T *m_lambda_ex_y,               \
+      T *m_lambda_ey_x, T *m_lambda_hz_y, T *m_lambda_hy_z, T *m_lambda_hx_z,               \
+      T *m_lambda_hz_x, T *m_lambda_hy_x, T *m_lambda_hx_y,                                 \
+      T *adjoint_memory_scratch, TIDE_REF3_STORES(), T *grad_f, T *grad_ca, T *grad_cb,     \
+      T *grad_eps, T *grad_sigma, T *grad_ca_shot, T *grad_cb_shot,                         \
+      bool zero_grad_on_entry, TIDE_REF3_PROFILES(T), TIDE_REF3_TAIL(T), int64_t storage_mode,   \
+      int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_rg, bool cb_rg,           \
+      TIDE_REF3_FLAGS, void *compute_stream, void *storage_stream) {                        \
+    (void)adjoint_memory_scratch; (void)grad_ca_shot; (void)grad_cb_shot;                   \
+    (void)ex_s3; (void)ey_s3; (void)ez_s3; (void)cx_s3; (void)cy_s3; (void)cz_s3;          \
+    (void)ex_fn; (void)ey_fn; (void)ez_fn; (void)cx_fn; (void)cy_fn; (void)cz_fn;          \
+    (void)n_threads; (void)execution_backend; (void)storage_stream;                         \
+    /* slots in the forward state order: each adjoint memory sits in the */              \
+    /* slot of the forward memory it is the transpose of                   */              \
+    T *const lam[18] = {lambda_ex, lambda_ey, lambda_ez, lambda_hx, lambda_hy, lambda_hz,  \
+                        m_lambda_hz_y, m_lambda_hy_z, m_lambda_hx_z, m_lambda_hz_x,         \
+                        m_lambda_hy_x, m_lambda_hx_y, m_lambda_ey_z, m_lambda_ez_y,         \
+                        m_lambda_ez_x, m_lambda_ex_z, m_lambda_ex_y, m_lambda_ey_x};        \
+    TIDE_REF3_PROF_ARRAY(T);                                                                   \
+    const int64_t pml[6] = {pml_z0, pml_y0, pml_x0, pml_z1, pml_y1, pml_x1};               \
+    const void *const stores[6] = {ex_s1, ey_s1, ez_s1, cx_s1, cy_s1, cz_s1};              \
+    return ref3_backward<T>(ca, cb, cq, grad_r, lam, stores, grad_f, grad_ca, grad_cb,      \
+                            grad_eps, grad_sigma, zero_grad_on_entry, prof, sources_i,      \
+                            receivers_i, rdz, rdy, rdx, dt, nt, n_shots, nz, ny, nx, n_src, \
+                            n_rec, step_ratio, storage_mode, storage_format,                \
+                            shot_bytes_uncomp, ca_rg, cb_rg, ca_b, cb_b, cq_b, start_t, pml,\
+                            source_component, receiver_component, device, compute_stream);  \
+  }
+
+TIDE_REF3_ENTRY_POINTS(float, float)
+TIDE_REF3_ENTRY_POINTS(double, double)
 
 extern "C" int tide_em3d_set_tuning(int fwd_z, int adj_z, int group) {
   const bool ok = (fwd_z == 4 || fwd_z == 8 || fwd_z == 16) && (adj_z == 4 || adj_z == 8) &&

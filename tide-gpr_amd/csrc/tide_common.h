@@ -82,13 +82,20 @@ bool build_point_table_halo(const int64_t *src, int64_t n_src, const int64_t *re
                             int64_t n_rec, int64_t n_shots, int64_t ny, int64_t nx,
                             int tile_w, int lane_offset, PointTable &out);
 
-// Device copy of a PointTable; owns three device allocations.
+// Device copy of a PointTable: either three device allocations of its own
+// (upload) or three caller-provided arrays of the sizes below (place).
 struct DevicePointTable {
   int32_t *row_slot = nullptr;
   int32_t *seg = nullptr;
   int2 *ent = nullptr;
   int32_t n_xt = 0;
+  bool owned = true;
   int upload(const PointTable &t, hipStream_t stream);
+  int place(const PointTable &t, int32_t *row_slot_mem, int32_t *seg_mem, int2 *ent_mem,
+            hipStream_t stream);
+  static size_t row_slot_count(const PointTable &t) { return t.row_slot.size(); }
+  static size_t seg_count(const PointTable &t) { return t.seg.size(); }
+  static size_t ent_count(const PointTable &t) { return t.ent.empty() ? 1 : t.ent.size(); }
   void release(hipStream_t stream);
 };
 

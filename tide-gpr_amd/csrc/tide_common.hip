@@ -116,10 +116,31 @@ int DevicePointTable::upload(const PointTable &t, hipStream_t stream) {
   return kOk;
 }
 
+int DevicePointTable::place(const PointTable &t, int32_t *row_slot_mem, int32_t *seg_mem,
+                            int2 *ent_mem, hipStream_t stream) {
+  n_xt = t.n_xt;
+  owned = false;
+  row_slot = row_slot_mem;
+  seg = seg_mem;
+  ent = ent_mem;
+  TIDE_HIP_TRY(hipMemcpyAsync(row_slot, t.row_slot.data(), t.row_slot.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, stream));
+  TIDE_HIP_TRY(hipMemcpyAsync(seg, t.seg.data(), t.seg.size() * sizeof(int32_t),
+                              hipMemcpyHostToDevice, stream));
+  if (!t.ent.empty())
+    TIDE_HIP_TRY(hipMemcpyAsync(ent, t.ent.data(), t.ent.size() * sizeof(int2),
+                                hipMemcpyHostToDevice, stream));
+  // The host vectors are pageable and may be freed when the caller returns.
+  TIDE_HIP_TRY(hipStreamSynchronize(stream));
+  return kOk;
+}
+
 void DevicePointTable::release(hipStream_t stream) {
-  if (row_slot) (void)hipFreeAsync(row_slot, stream);
-  if (seg) (void)hipFreeAsync(seg, stream);
-  if (ent) (void)hipFreeAsync(ent, stream);
+  if (owned) {
+    if (row_slot) (void)hipFreeAsync(row_slot, stream);
+    if (seg) (void)hipFreeAsync(seg, stream);
+    if (ent) (void)hipFreeAsync(ent, stream);
+  }
   row_slot = nullptr;
   seg = nullptr;
   ent = nullptr;

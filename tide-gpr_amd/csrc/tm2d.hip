@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <string>
 
 namespace tide_hip {
 namespace tm2d {
@@ -116,16 +117,34 @@ inline int fetch_indices(const int64_t *p, int64_t n, std::vector<int64_t> &out,
   return kOk;
 }
 
+// Scratch device memory of one call: stream-ordered allocations freed when
+// the call returns, or, with an arena, consecutive 256-byte aligned pieces
+// of caller-provided device memory (nothing allocated or freed).
 struct Scratch {
   std::vector<void *> bufs;
   hipStream_t stream;
-  explicit Scratch(hipStream_t s) : stream(s) {}
+  char *arena = nullptr;
+  size_t cap = 0, used = 0;
+  explicit Scratch(hipStream_t s, char *arena_ = nullptr, size_t cap_ = 0)
+      : stream(s), arena(arena_), cap(cap_) {}
   ~Scratch() {
     for (void *p : bufs) (void)hipFreeAsync(p, stream);
   }
   template <typename U> int alloc(U **out, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(U);
+    if (arena != nullptr) {
+      const size_t at = (used + 255) & ~(size_t)255;
+      if (at + bytes > cap) {
+        set_error("the call needs more scratch memory than the fixed scratch array holds (" +
+                  std::to_string(cap >> 20) + " MiB)");
+        return kInvalidArgument;
+      }
+      *out = (U *)(arena + at);
+      used = at + bytes;
+      return kOk;
+    }
     void *p = nullptr;
-    TIDE_HIP_TRY(hipMallocAsync(&p, std::max<size_t>(count, 1) * sizeof(U), stream));
+    TIDE_HIP_TRY(hipMallocAsync(&p, bytes, stream));
     bufs.push_back(p);
     *out = (U *)p;
     return kOk;
@@ -160,9 +179,22 @@ int pack_profiles(Problem<T> &p, Scratch &scr, hipStream_t stream) {
 
 // halo2: the table of the two-step kernels (60-column tiles, lane offset 2,
 // halo copies) in `dpt2`.
+// With a Scratch arena the tables are placed in it instead of allocated.
+inline int put_table(const PointTable &t, DevicePointTable &d, Scratch *arena_scr,
+                     hipStream_t stream) {
+  if (arena_scr == nullptr || arena_scr->arena == nullptr) return d.upload(t, stream);
+  int32_t *rs, *sg;
+  int2 *en;
+  TIDE_CHECK(arena_scr->alloc(&rs, DevicePointTable::row_slot_count(t)));
+  TIDE_CHECK(arena_scr->alloc(&sg, DevicePointTable::seg_count(t)));
+  TIDE_CHECK(arena_scr->alloc(&en, DevicePointTable::ent_count(t)));
+  return d.place(t, rs, sg, en, stream);
+}
+
 template <typename T>
 int upload_points(const Problem<T> &p, const int64_t *src, const int64_t *rec,
-                  DevicePointTable &dpt, hipStream_t stream, DevicePointTable *dpt2 = nullptr) {
+                  DevicePointTable &dpt, hipStream_t stream, DevicePointTable *dpt2 = nullptr,
+                  Scratch *arena_scr = nullptr) {
   std::vector<int64_t> hsrc, hrec;
   TIDE_CHECK(fetch_indices(src, (int64_t)p.n_shots * p.n_src, hsrc, stream));
   TIDE_CHECK(fetch_indices(rec, (int64_t)p.n_shots * p.n_rec, hrec, stream));
@@ -170,20 +202,21 @@ int upload_points(const Problem<T> &p, const int64_t *src, const int64_t *rec,
   if (!build_point_table(hsrc.data(), p.n_src, hrec.data(), p.n_rec, p.n_shots, p.g.ny,
                          p.g.nx, kOwn, 1, table))
     return kInvalidArgument;
-  TIDE_CHECK(dpt.upload(table, stream));
+  TIDE_CHECK(put_table(table, dpt, arena_scr, stream));
   if (dpt2 == nullptr) return kOk;
   PointTable table2;
   if (!build_point_table_halo(hsrc.data(), p.n_src, hrec.data(), p.n_rec, p.n_shots, p.g.ny,
                               p.g.nx, kOwn2, 2, table2))
     return kInvalidArgument;
-  return dpt2->upload(table2, stream);
+  return put_table(table2, *dpt2, arena_scr, stream);
 }
 
 template <typename T>
 int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_hx_z,
                  T *m_hz_x, const T *f, T *r, const int64_t *src, const int64_t *rec,
                  int64_t nt, int64_t start_t, int64_t step_ratio, void *ey_store,
-                 void *curl_store, int fmt, size_t store_step_bytes, hipStream_t stream) {
+                 void *curl_store, int fmt, size_t store_step_bytes, hipStream_t stream,
+                 char *arena = nullptr, size_t arena_cap = 0) {
   TIDE_CHECK(check_geometry(p.n_shots, p.g.ny, p.g.nx));
   if (nt <= 0) return kOk;
   if (step_ratio <= 0) {
@@ -196,9 +229,9 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
   }
   const int64_t n_shots = p.n_shots, total = n_shots * p.g.numel;
   const bool fused = g_tuning.fwd_steps == 2 && nt >= 2;
+  Scratch scr(stream, arena, arena_cap);
   DevicePointTable dpt, dpt2;
-  TIDE_CHECK(upload_points(p, src, rec, dpt, stream, fused ? &dpt2 : nullptr));
-  Scratch scr(stream);
+  TIDE_CHECK(upload_points(p, src, rec, dpt, stream, fused ? &dpt2 : nullptr, &scr));
   TIDE_CHECK(pack_profiles(p, scr, stream));
   T *ey2, *hx2, *hz2, *mex2, *mez2, *mhzx2 = nullptr, *mhxz2 = nullptr;
   TIDE_CHECK(scr.alloc(&ey2, (size_t)total));
@@ -352,6 +385,13 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
   }
   if ((gca && !ey_store) || (gcb && !curl_store)) {
     set_error("coefficient gradients require the stored forward snapshots");
+    return kInvalidArgument;
+  }
+  // One gradient layout serves both accumulators: per shot when a
+  // coefficient is per shot.  A per-shot ca with a shared cb (or the
+  // reverse) would index the shared [ny,nx] buffer per shot.
+  if (gca && gcb && p.g.ca_b != p.g.cb_b) {
+    set_error("ca and cb gradients need the same batching (ca_batched == cb_batched)");
     return kInvalidArgument;
   }
   const int64_t n_shots = p.n_shots, total = n_shots * p.g.numel;
@@ -598,6 +638,10 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
                    bool ca_rg, bool cb_rg, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,
                    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
                    int64_t device, void *stream_handle) {
+  if (ca_rg && cb_rg && ca_b != cb_b) {  // checked before any device call
+    set_error("ca and cb gradients need the same batching (ca_batched == cb_batched)");
+    return kInvalidArgument;
+  }
   if (hipSetDevice((int)device) != hipSuccess) {
     set_error("hipSetDevice failed");
     return kHipError;

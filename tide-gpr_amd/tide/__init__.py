@@ -3,7 +3,8 @@
 Public surface for the propagation hot path, mirroring the reference
 package: ``MaxwellTM`` / ``Maxwell3D`` operators over ``EMModel`` with
 ``Experiment`` / ``Acquisition`` / ``Discretization`` / ``CPML`` inputs,
-``StorageOptions`` snapshot policy, ``linearize().vjp``; plus ``ricker`` and
+``StorageOptions`` snapshot policy, ``linearize().vjp``, the ``workflow``
+shot-batching and receiver-objective helpers; plus ``ricker`` and
 ``cfl_condition``.  The compute path is the HIP library in ``_lib``; there
 is no CPU fallback.
 """
@@ -30,6 +31,7 @@ from .maxwell import (
 )
 from .utils import C0, EP0, MU0
 from .wavelets import gaussian, gaussian_derivative, ricker
+from . import workflow  # noqa: E402  (imports the operators above)
 
 __all__ = [
     "Acquisition",
@@ -64,4 +66,5 @@ __all__ = [
     "storage",
     "utils",
     "wavelets",
+    "workflow",
 ]

@@ -188,6 +188,32 @@ def _em3d_backward_types(f: Any) -> list[Any]:
     ]
 
 
+def _expand_kinds(spec: str, f: Any) -> list[Any]:
+    """``"P4 F2 I1"`` -> four pointers, two field floats, one int64."""
+    kinds = {"P": _P, "I": _I, "B": _B, "F": f}
+    out: list[Any] = []
+    for tok in spec.split():
+        out += [kinds[tok[0]]] * int(tok[1:])
+    return out
+
+
+# Argument kinds of the reference-order 3-D entry points, in the order of
+# the reference's ctypes specs (src/tide/backend_utils.py:446-555).
+_REF3_KINDS = {
+    # ca cb cq f | 18 state | 9 Debye | r | n_poles | 18 profiles | src rec |
+    # rdz rdy rdx dt | nt n_shots nz ny nx n_src n_rec step_ratio |
+    # has_dispersion | 3 batched | start_t | 6 pml | 2 components |
+    # n_threads device execution_backend | stream
+    "forward": "P4 P18 P9 P1 I1 P18 P2 F4 I8 B1 B3 I1 I6 I2 I1 I1 I1 P1",
+    # ca cb cq f | 18 state | r | 18 storage | 18 profiles | src rec | F4 | I8 |
+    # storage_mode format shot_bytes | ca/cb requires_grad | flags | 2 streams
+    "forward_with_storage": "P4 P18 P1 P18 P18 P2 F4 I8 I3 B2 B3 I1 I6 I2 I1 I1 I1 P2",
+    # ca cb cq grad_r | 18 lambda | scratch | 18 storage | grad_f | grad_ca
+    # grad_cb grad_eps grad_sigma | 2 per-shot | zero_grad_on_entry | ...
+    "backward": "P4 P18 P1 P18 P1 P4 P2 B1 P18 P2 F4 I8 I3 B2 B3 I1 I6 I2 I1 I1 I1 P2",
+}
+
+
 def load() -> ctypes.CDLL:
     """Load the HIP library, raising if it has not been built."""
     global _lib
@@ -221,6 +247,11 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, f"tide_em3d_{name}_backward_f64acc")
         fn.restype = ctypes.c_int
         fn.argtypes = _em3d_backward_types(ctype)
+    for name, ctype in (("float", ctypes.c_float), ("double", ctypes.c_double)):
+        for pass_name, kinds in _REF3_KINDS.items():
+            fn = getattr(lib, f"maxwell_3d_2_{name}_{pass_name}_cuda")
+            fn.restype = ctypes.c_int
+            fn.argtypes = _expand_kinds(kinds, ctype)
     lib.tide_em3d_set_tuning.restype = ctypes.c_int
     lib.tide_em3d_set_tuning.argtypes = [ctypes.c_int] * 3
     lib.tide_em3d_set_forward_group.restype = ctypes.c_int

@@ -15,7 +15,7 @@ import torch
 
 from ..core import StorageOptions
 from .contracts import EMModel
-from .results import EMGradient, ForwardResult
+from .results import EM3DState, EMGradient, ForwardResult, TMState
 
 _TARGETS = ("epsilon", "sigma", "mu")
 
@@ -91,11 +91,67 @@ class _LinearizedMaxwell:
 
 
 class LinearizedMaxwellTM(_LinearizedMaxwell):
-    pass
+    def _separable_gradient(self, observed: torch.Tensor, residual):
+        """Value and gradient of a shot-separable receiver objective, chunk
+        by chunk with one FP64 accumulator (``tm2d.separable_gradient``).
+        Also fills the primal (receivers and final state).  Returns None
+        when the call needs the general path (structured batches, callbacks,
+        host-staged storage, dispersion)."""
+        op, model = self.operator, self.model
+        d, e = op.discretization, op.experiment
+        storage = self.storage
+        if (model.epsilon.ndim != 2 or model.dispersion is not None
+                or storage.mode.value not in ("device", "auto")
+                or (storage.mode.value == "auto" and storage.bytes_limit_device is not None)
+                or e.frequency_taper_fraction or e.time_padding_fraction or e.time_taper):
+            return None
+        from .tm2d import separable_gradient
+
+        value, grads, r, fields, info = separable_gradient(
+            model.epsilon, model.sigma, model.mu, d.spacing, d.dt, e.source_amplitude,
+            e.acquisition.source_location, e.acquisition.receiver_location, observed,
+            residual=residual, stencil=d.stencil, pml_width=d.boundary.width,
+            max_vel=d.max_velocity, nt=e.nt,
+            model_gradient_sampling_interval=op.model_gradient_sampling_interval,
+            storage_compression=storage.compression,
+            storage_bytes_limit_device=storage.bytes_limit_device,
+            targets=tuple(n for n in ("epsilon", "sigma") if n in self.targets),
+        )
+        self.last_plan = info
+        self._primal = ForwardResult(receiver_data=r, final_state=TMState.from_tensors(tuple(fields)))
+        return value, EMGradient(epsilon=grads.get("epsilon"), sigma=grads.get("sigma"), mu=None)
 
 
 class LinearizedMaxwell3D(_LinearizedMaxwell):
-    pass
+    def _separable_gradient(self, observed: torch.Tensor, residual):
+        """3-D counterpart of :meth:`LinearizedMaxwellTM._separable_gradient`
+        (``maxwell3d.separable_gradient3d``)."""
+        op, model = self.operator, self.model
+        d, e = op.discretization, op.experiment
+        storage = self.storage
+        if (model.epsilon.ndim != 3 or model.dispersion is not None
+                or storage.mode.value not in ("device", "auto")
+                or (storage.mode.value == "auto" and storage.bytes_limit_device is not None)
+                or e.frequency_taper_fraction or e.time_padding_fraction or e.time_taper):
+            return None
+        from .maxwell3d import separable_gradient3d
+
+        value, grads, r, fields, info = separable_gradient3d(
+            model.epsilon, model.sigma, model.mu, d.spacing, d.dt, e.source_amplitude,
+            e.acquisition.source_location, e.acquisition.receiver_location, observed,
+            residual=residual, stencil=d.stencil, pml_width=d.boundary.width,
+            max_vel=d.max_velocity, nt=e.nt,
+            model_gradient_sampling_interval=op.model_gradient_sampling_interval,
+            source_component=e.source_component, receiver_component=e.receiver_component,
+            storage_compression=storage.compression,
+            storage_bytes_limit_device=storage.bytes_limit_device,
+            storage_chunk_steps=storage.chunk_steps,
+            targets=tuple(n for n in ("epsilon", "sigma") if n in self.targets),
+        )
+        self.last_plan = info
+        self._primal = ForwardResult(receiver_data=r,
+                                     final_state=EM3DState.from_tensors(tuple(fields)))
+        return value, EMGradient(epsilon=grads.get("epsilon"), sigma=grads.get("sigma"), mu=None)
 
 
 __all__ = ["LinearizedMaxwell3D", "LinearizedMaxwellTM"]

@@ -230,6 +230,7 @@ def prepare_em3d(
         pml=pml, fd_pad=fd_pad, pml_width=widths,
         source_component=_component(source_component),
         receiver_component=_component(receiver_component), batched=batched,
+        extra={"zero_init": all(t0 is None for t0 in initial)},
     )
 
 
@@ -311,14 +312,97 @@ def _unpack_state(setup: EM3DSetup, packed: list, fields: list[torch.Tensor]) ->
         t.narrow(1 + ax, hi, t.shape[1 + ax] - hi).copy_(hi_part)
 
 
+def _chunk_setup3(setup: EM3DSetup, s0: int, s1: int, fields: list[torch.Tensor]) -> EM3DSetup:
+    """The shots ``[s0, s1)`` of ``setup`` as a setup of their own (views)."""
+    if s0 == 0 and s1 == setup.n_shots and fields is setup.fields:
+        return setup
+    sub = EM3DSetup(**{k: getattr(setup, k) for k in setup.__dataclass_fields__})
+    sub.n_shots = s1 - s0
+    sub.sources_i = setup.sources_i[s0:s1].contiguous() if setup.n_src > 0 else setup.sources_i
+    sub.receivers_i = setup.receivers_i[s0:s1].contiguous() if setup.n_rec > 0 else setup.receivers_i
+    sub.fields = fields
+    sub.extra = {}
+    return sub
+
+
+def _shots(t: torch.Tensor | None, s0: int, s1: int, batched: bool):
+    return t if (t is None or not batched) else t[s0:s1]
+
+
+def _forward_chunk(sub: EM3DSetup, ca, cb, cq, f, r, *, policy: str, seg: int, interval: int,
+                   fmt: int, want_ca: bool, want_cb: bool, keep: bool):
+    """One chunk's forward.  ``policy`` "full" stores every sampled step,
+    "checkpoint" keeps the state every ``seg`` steps; with ``keep`` False
+    nothing is kept (the backward re-runs the chunk).  Returns what the
+    backward needs: ``("full", stores)``, ``("checkpoint", checkpoints)`` or
+    None."""
+    fields = sub.fields
+    if keep and policy == "full":
+        n_slots = (sub.nt + interval - 1) // interval
+        stores = _alloc_stores(sub, n_slots, fmt, ca.dtype, ca.device, want_ca, want_cb)
+        _call_forward(sub, ca, cb, cq, f, fields, r, start_t=0, nt=sub.nt, interval=interval,
+                      stores=stores, fmt=fmt)
+        return ("full", stores)
+    if keep and policy == "checkpoint":
+        checkpoints = []
+        for t0 in range(0, sub.nt, seg):
+            checkpoints.append(_pack_state(sub, fields))
+            _call_forward(sub, ca, cb, cq, f, fields, r, start_t=t0, nt=min(seg, sub.nt - t0),
+                          interval=interval)
+        return ("checkpoint", checkpoints)
+    _call_forward(sub, ca, cb, cq, f, fields, r, start_t=0, nt=sub.nt, interval=interval)
+    return None
+
+
+def _backward_chunk(sub: EM3DSetup, ca, cb, cq, f, grad_r, lam, saved, *, seg: int,
+                    interval: int, fmt: int, want_ca: bool, want_cb: bool, grad_f, gca, gcb,
+                    pool: dict) -> None:
+    """The exact adjoint of one chunk from its saved snapshots or
+    checkpoints; ``gca``/``gcb`` are FP64 accumulators (added to)."""
+    kind, data = saved
+    if kind == "full":
+        _call_backward(sub, ca, cb, cq, grad_r, lam, data, fmt=fmt, store_t0=0,
+                       start_t=sub.nt, nt=sub.nt, interval=interval, grad_f=grad_f, gca=gca,
+                       gcb=gcb)
+        return
+    dev, dtype = ca.device, ca.dtype
+    shape = (sub.n_shots, sub.nz, sub.ny, sub.nx)
+    n_slots = (seg + interval - 1) // interval
+    if "work" not in pool:  # sized for the first (largest) chunk that needs it
+        numel = sub.n_shots * sub.nz * sub.ny * sub.nx
+        pool["work"] = [torch.empty(numel, device=dev, dtype=dtype) for _ in range(18)]
+        pool["stores"] = _alloc_stores(sub, n_slots, fmt, dtype, dev, want_ca, want_cb)
+        pool["r"] = torch.zeros(sub.nt * sub.n_shots * max(sub.n_rec, 1), device=dev, dtype=dtype)
+    numel = sub.n_shots * sub.nz * sub.ny * sub.nx
+    work = [w[:numel].view(shape) for w in pool["work"]]
+    store_shape = (n_slots, *shape)
+    stores = [None if t is None else t.reshape(-1)[: n_slots * numel].view(store_shape)
+              for t in pool["stores"]]
+    r_scratch = pool["r"][: sub.nt * sub.n_shots * sub.n_rec].view(sub.nt, sub.n_shots, sub.n_rec)
+    starts = list(range(0, sub.nt, seg))
+    for k in reversed(range(len(starts))):
+        t0 = starts[k]
+        t1 = min(t0 + seg, sub.nt)
+        _unpack_state(sub, data[k], work)
+        data[k] = None
+        _call_forward(sub, ca, cb, cq, f, work, r_scratch, start_t=t0, nt=t1 - t0,
+                      interval=interval, stores=stores, fmt=fmt, store_t0=t0)
+        _call_backward(sub, ca, cb, cq, grad_r, lam, stores, fmt=fmt, store_t0=t0, start_t=t1,
+                       nt=t1 - t0, interval=interval, grad_f=grad_f, gca=gca, gcb=gcb)
+
+
 class EM3DPropagate(torch.autograd.Function):
     """3-D forward kernels + exact adjoint behind torch autograd.
 
-    ``plan`` selects the snapshot policy for a model gradient: ``"full"``
-    stores every sampled step's (E, curl H) in HBM during the forward;
-    ``"checkpoint"`` keeps the state every ``segment`` steps and recomputes
-    one segment's snapshots at a time in the backward (one extra forward
-    sweep).  Cotangents of the final state are ignored, as in the reference.
+    ``opts["policy"]`` selects the snapshot policy for a model gradient:
+    ``"full"`` stores every sampled step's (E, curl H) in HBM during the
+    forward; ``"checkpoint"`` keeps the state every ``segment`` steps and
+    recomputes one segment's snapshots at a time in the backward (one extra
+    forward sweep).  ``opts["plan"]`` splits the shots into chunks: the
+    first ``resident`` chunks keep their snapshots or checkpoints until the
+    backward, the others are re-run there from their initial state.  One
+    FP64 accumulator per coefficient collects every chunk.  Cotangents of
+    the final state are ignored, as in the reference.
     """
 
     @staticmethod
@@ -329,24 +413,26 @@ class EM3DPropagate(torch.autograd.Function):
         f_c = f.detach().contiguous()
         want_ca, want_cb = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         interval, fmt = opts["interval"], opts["format"]
-        plan = opts["plan"] if (want_ca or want_cb) else "none"
-        stores, checkpoints = None, []
-        if plan == "full":
-            n_slots = (setup.nt + interval - 1) // interval
-            stores = _alloc_stores(setup, n_slots, fmt, dtype, dev, want_ca, want_cb)
-            _call_forward(setup, ca, cb, cq, f_c, fields, r, start_t=0, nt=setup.nt,
-                          interval=interval, stores=stores, fmt=fmt)
-        elif plan == "checkpoint":
-            seg = opts["segment"]
-            for t0 in range(0, setup.nt, seg):
-                checkpoints.append(_pack_state(setup, fields))
-                _call_forward(setup, ca, cb, cq, f_c, fields, r, start_t=t0,
-                              nt=min(seg, setup.nt - t0), interval=interval)
-        else:
-            _call_forward(setup, ca, cb, cq, f_c, fields, r, start_t=0, nt=setup.nt,
-                          interval=interval)
-        ctx.setup, ctx.opts, ctx.plan = setup, opts, plan
-        ctx.stores, ctx.checkpoints = stores, checkpoints
+        policy = opts["policy"] if (want_ca or want_cb) else "none"
+        plan = opts.get("plan") or _single(setup.n_shots)
+        chunked = plan.n_chunks > 1
+        saved, initial = [], []
+        for i in range(plan.n_chunks):
+            s0, s1 = plan.bounds[i], plan.bounds[i + 1]
+            sub = _chunk_setup3(setup, s0, s1, [t[s0:s1] for t in fields] if chunked else fields)
+            keep = policy != "none" and i < plan.resident
+            initial.append(None if (policy == "none" or keep or setup.extra.get("zero_init"))
+                           else [t.clone() for t in sub.fields])
+            f_i = f_c[:, s0:s1].contiguous() if (chunked and setup.n_src > 0) else f_c
+            r_i = torch.zeros(setup.nt, s1 - s0, setup.n_rec, device=dev, dtype=dtype) if chunked else r
+            saved.append(_forward_chunk(sub, *(_shots(t, s0, s1, setup.batched) for t in (ca, cb, cq)),
+                                        f_i, r_i, policy=policy, seg=opts["segment"],
+                                        interval=interval, fmt=fmt, want_ca=want_ca,
+                                        want_cb=want_cb, keep=keep))
+            if chunked and setup.n_rec > 0:
+                r[:, s0:s1].copy_(r_i)
+        ctx.setup, ctx.opts, ctx.policy, ctx.plan = setup, opts, policy, plan
+        ctx.saved_chunks, ctx.initial = saved, initial
         ctx.save_for_backward(ca, cb, cq, f_c)
         ctx.mark_non_differentiable(*fields)
         return (*fields, r)
@@ -355,66 +441,127 @@ class EM3DPropagate(torch.autograd.Function):
     def backward(ctx, *grads):
         grad_r = grads[-1]
         setup: EM3DSetup = ctx.setup
+        plan = ctx.plan
         ca, cb, cq, f_c = ctx.saved_tensors
         want_ca, want_cb = ctx.needs_input_grad[0], ctx.needs_input_grad[1]
         want_f = ctx.needs_input_grad[3] and setup.n_src > 0
-        if (want_ca or want_cb) and ctx.plan == "none":
+        if (want_ca or want_cb) and ctx.policy == "none":
             raise RuntimeError("model gradients need stored snapshots; storage_mode='none' was used")
         dev, dtype = ca.device, ca.dtype
-        shape = (setup.n_shots, setup.nz, setup.ny, setup.nx)
         if grad_r is None:
             grad_r = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
-        grad_r = grad_r.to(dtype).contiguous()
-        lam = [torch.zeros(shape, device=dev, dtype=dtype) for _ in range(18)]
+        grad_r = grad_r.to(dtype)
+        chunked = plan.n_chunks > 1
+        max_c = max(plan.sizes)
+        numel = setup.nz * setup.ny * setup.nx
+        lam_all = [torch.empty(max_c * numel, device=dev, dtype=dtype) for _ in range(18)]
         grad_f = (torch.zeros(setup.nt, setup.n_shots, setup.n_src, device=dev, dtype=dtype)
                   if want_f else None)
-        gshape = shape if setup.batched else shape[1:]
+        full_shape = (setup.n_shots, setup.nz, setup.ny, setup.nx)
+        gshape = full_shape if setup.batched else full_shape[1:]
         gca = torch.zeros(gshape, device=dev, dtype=torch.float64) if want_ca else None
         gcb = torch.zeros(gshape, device=dev, dtype=torch.float64) if want_cb else None
-        interval, fmt = ctx.opts["interval"], ctx.opts["format"]
-        if ctx.plan == "checkpoint":
-            seg = ctx.opts["segment"]
-            work = [torch.empty(shape, device=dev, dtype=dtype) for _ in range(18)]
-            r_scratch = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
-            n_slots = (seg + interval - 1) // interval
-            stores = _alloc_stores(setup, n_slots, fmt, dtype, dev, want_ca, want_cb)
-            starts = list(range(0, setup.nt, seg))
-            for k in reversed(range(len(starts))):
-                t0 = starts[k]
-                t1 = min(t0 + seg, setup.nt)
-                _unpack_state(setup, ctx.checkpoints[k], work)
-                ctx.checkpoints[k] = None
-                _call_forward(setup, ca, cb, cq, f_c, work, r_scratch, start_t=t0, nt=t1 - t0,
-                              interval=interval, stores=stores, fmt=fmt, store_t0=t0)
-                _call_backward(setup, ca, cb, cq, grad_r, lam, stores, fmt=fmt, store_t0=t0,
-                               start_t=t1, nt=t1 - t0, interval=interval, grad_f=grad_f,
-                               gca=gca, gcb=gcb)
-        else:
-            _call_backward(setup, ca, cb, cq, grad_r, lam, ctx.stores, fmt=fmt, store_t0=0,
-                           start_t=setup.nt, nt=setup.nt, interval=interval, grad_f=grad_f,
-                           gca=gca, gcb=gcb)
-        ctx.stores = None
+        interval, fmt, seg = ctx.opts["interval"], ctx.opts["format"], ctx.opts["segment"]
+        pool: dict = {}
+        for i in range(plan.n_chunks):  # resident chunks come first
+            s0, s1 = plan.bounds[i], plan.bounds[i + 1]
+            n_c = s1 - s0
+            cs = [_shots(t, s0, s1, setup.batched) for t in (ca, cb, cq)]
+            f_i = f_c[:, s0:s1].contiguous() if (chunked and setup.n_src > 0) else f_c
+            saved = ctx.saved_chunks[i]
+            ctx.saved_chunks[i] = None
+            if saved is None:
+                # re-run this chunk's forward, keeping checkpoints only
+                init = ctx.initial[i]
+                ctx.initial[i] = None
+                state = ([torch.zeros(n_c, setup.nz, setup.ny, setup.nx, device=dev, dtype=dtype)
+                          for _ in range(18)] if init is None else init)
+                sub = _chunk_setup3(setup, s0, s1, state)
+                r_tmp = torch.empty(setup.nt, n_c, setup.n_rec, device=dev, dtype=dtype)
+                saved = _forward_chunk(sub, *cs, f_i, r_tmp, policy="checkpoint",
+                                       seg=seg if seg > 0 else _default_segment(setup.nt, interval),
+                                       interval=interval, fmt=fmt, want_ca=want_ca,
+                                       want_cb=want_cb, keep=True)
+                seg_i = seg if seg > 0 else _default_segment(setup.nt, interval)
+            else:
+                sub = _chunk_setup3(setup, s0, s1, [])
+                seg_i = seg
+            lam = [t[: n_c * numel].view(n_c, setup.nz, setup.ny, setup.nx) for t in lam_all]
+            for t in lam:
+                t.zero_()
+            gr_i = grad_r[:, s0:s1].contiguous() if chunked else grad_r.contiguous()
+            gf_i = (torch.zeros(setup.nt, n_c, setup.n_src, device=dev, dtype=dtype)
+                    if (want_f and chunked) else grad_f)
+            _backward_chunk(sub, *cs, f_i, gr_i, lam, saved, seg=seg_i, interval=interval,
+                            fmt=fmt, want_ca=want_ca, want_cb=want_cb, grad_f=gf_i,
+                            gca=_shots(gca, s0, s1, setup.batched),
+                            gcb=_shots(gcb, s0, s1, setup.batched), pool=pool)
+            if want_f and chunked:
+                grad_f[:, s0:s1].copy_(gf_i)
+            del saved
         out_ca = gca.to(dtype).reshape(ca.shape) if want_ca else None
         out_cb = gcb.to(dtype).reshape(cb.shape) if want_cb else None
         return out_ca, out_cb, None, grad_f, None, None
 
 
-def _plan_storage(setup: EM3DSetup, interval: int, bpe: int, limit_device, chunk_steps: int,
-                  device: torch.device) -> tuple[str, int]:
-    """``("full", 0)`` when every sampled step's six snapshot components fit
-    the device budget, else ``("checkpoint", segment)`` with a segment of
-    about sqrt(nt) steps (a multiple of the sampling interval)."""
+def _default_segment(nt: int, interval: int) -> int:
+    seg = math.ceil(math.sqrt(max(nt, 1)))
+    return max(interval, math.ceil(seg / interval) * interval)
+
+
+def _single(n_shots: int):
+    from .tm2d import ShotPlan
+
+    return ShotPlan([0, n_shots], 1)
+
+
+def _state_bytes_per_shot(setup: EM3DSetup) -> int:
+    """Bytes of one checkpoint of one shot: six fields plus the strips of
+    the twelve CPML memories."""
+    itemsize = setup.ca.element_size()
     numel = setup.nz * setup.ny * setup.nx
+    dims = (setup.nz, setup.ny, setup.nx)
+    strips = 0
+    for name in STATE_NAMES[6:]:
+        ax = _MEMORY_AXIS[name]
+        lo, hi = setup.strips(ax)
+        strips += (lo + dims[ax] - hi) * numel // dims[ax]
+    return (6 * numel + strips) * itemsize
+
+
+def _plan_storage(setup: EM3DSetup, interval: int, bpe: int, limit_device, chunk_steps: int,
+                  device: torch.device):
+    """Snapshot policy and shot chunks for a model gradient.
+
+    ``("full", 0, plan)`` when every sampled step's six snapshot components
+    of every shot fit the device budget; else ``("checkpoint", segment,
+    plan)`` with a segment of about sqrt(nt) steps (a multiple of the
+    sampling interval), the shots split so that one chunk's checkpoints and
+    one segment's snapshots fit, and as many chunks resident as their
+    checkpoints allow beside one segment's snapshots.
+    """
+    from .tm2d import ShotPlan, device_snapshot_budget, plan_shot_chunks
+
+    numel = setup.nz * setup.ny * setup.nx
+    n_shots = setup.n_shots
     n_slots = (setup.nt + interval - 1) // interval
-    full_bytes = 6 * n_slots * setup.n_shots * numel * bpe
-    if limit_device is None:
-        free, _ = torch.cuda.mem_get_info(device)
-        limit_device = int(0.75 * free)
-    if full_bytes <= limit_device and chunk_steps <= 0:
-        return "full", 0
-    seg = chunk_steps if chunk_steps > 0 else math.ceil(math.sqrt(max(setup.nt, 1)))
-    seg = max(interval, math.ceil(seg / interval) * interval)
-    return "checkpoint", seg
+    full_ps = 6 * n_slots * numel * bpe
+    working = 60 * numel * setup.ca.element_size()
+    budget = device_snapshot_budget(device, limit_device, working, n_shots)
+    if chunk_steps <= 0 and full_ps * n_shots <= budget:
+        return "full", 0, ShotPlan([0, n_shots], 1)
+    seg = _default_segment(setup.nt, interval) if chunk_steps <= 0 else \
+        max(interval, math.ceil(chunk_steps / interval) * interval)
+    ck_ps = math.ceil(setup.nt / seg) * _state_bytes_per_shot(setup)
+    seg_ps = math.ceil(seg / interval) * 6 * numel * bpe
+    plan = plan_shot_chunks(n_shots, ck_ps + seg_ps, budget, group=1)
+    if plan.n_chunks > 1:
+        c = max(plan.sizes)
+        resident = 0
+        while resident < plan.n_chunks and (resident + 1) * c * ck_ps + c * seg_ps <= budget:
+            resident += 1
+        plan = ShotPlan(plan.bounds, max(1, resident))
+    return "checkpoint", seg, plan
 
 
 def _reject_unsupported(python_backend, stencil, forward_callback, backward_callback,
@@ -537,11 +684,12 @@ def maxwell3d(
         raise NotImplementedError(
             f"storage_mode='{mode}' (host-staged snapshots) is not part of this build; "
             "device mode checkpoints and recomputes instead")
-    plan, segment = "none", 0
+    policy, segment, plan = "none", 0, _single(setup.n_shots)
     if mode == "device" and do_save and model_rg:
-        plan, segment = _plan_storage(setup, interval, bpe, storage_bytes_limit_device,
-                                      int(storage_chunk_steps), epsilon.device)
-    opts = {"plan": plan, "segment": segment, "interval": interval, "format": fmt, "kind": kind}
+        policy, segment, plan = _plan_storage(setup, interval, bpe, storage_bytes_limit_device,
+                                              int(storage_chunk_steps), epsilon.device)
+    opts = {"policy": policy, "segment": segment, "interval": interval, "format": fmt,
+            "kind": kind, "plan": plan}
     f = setup.f if setup.f.numel() > 0 else torch.empty(0, device=epsilon.device, dtype=epsilon.dtype)
     out = EM3DPropagate.apply(setup.ca, setup.cb, setup.cq, f, setup, opts)
     fields = [setup.unpad(t) for t in out[:18]]
@@ -551,4 +699,110 @@ def maxwell3d(
     return (*fields, r)
 
 
-__all__ = ["EM3DPropagate", "EM3DSetup", "STATE_NAMES", "maxwell3d", "prepare_em3d"]
+def separable_gradient3d(
+    epsilon: torch.Tensor,
+    sigma: torch.Tensor,
+    mu: torch.Tensor,
+    grid_spacing: float | Sequence[float],
+    dt: float,
+    source_amplitude: torch.Tensor | None,
+    source_location: torch.Tensor | None,
+    receiver_location: torch.Tensor | None,
+    observed: torch.Tensor,
+    *,
+    residual: Any,
+    stencil: int = 2,
+    pml_width: int | Sequence[int] = 20,
+    max_vel: float | None = None,
+    nt: int | None = None,
+    model_gradient_sampling_interval: int = 1,
+    source_component: str = "ey",
+    receiver_component: str = "ey",
+    storage_compression: bool | str = False,
+    storage_bytes_limit_device: int | None = None,
+    storage_chunk_steps: int = 0,
+    targets: Sequence[str] = ("epsilon", "sigma"),
+):
+    """Gradient of a receiver objective that is a sum over shots (3-D).
+
+    Each shot chunk runs its forward (snapshots, or checkpoints every
+    ~sqrt(nt) steps when the snapshots do not fit), its receiver residual
+    and its exact adjoint before the next chunk, so no chunk is re-run
+    beyond the segment recompute; gradients of all chunks sum into one FP64
+    accumulator and the chain rule is applied once by torch autograd.
+    Returns ``(value, {"epsilon": g, "sigma": g}, receivers, final fields)``
+    plus a dict with the policy (``"full"``/``"checkpoint"``), the segment
+    and the chunk sizes.
+    """
+    eps = epsilon.detach().requires_grad_("epsilon" in targets)
+    sig = sigma.detach().requires_grad_("sigma" in targets)
+    spacing = _normalize_grid_spacing_3d(grid_spacing)
+    if epsilon.ndim != 3:
+        raise NotImplementedError("separable_gradient3d takes one model shared by all shots")
+    if max_vel is None:
+        max_vel = float((1.0 / torch.sqrt(epsilon.detach() * mu.detach())).max().item()) * C0
+    if cfl_condition(spacing, dt, max_vel)[1] > 1:
+        raise NotImplementedError("dt exceeds the CFL limit; pass a stable dt")
+    setup = prepare_em3d(eps, sig, mu.detach(), spacing, dt, source_amplitude, source_location,
+                         receiver_location, stencil=stencil, pml_width=pml_width, nt=nt,
+                         source_component=source_component,
+                         receiver_component=receiver_component)
+    interval = max(1, min(int(model_gradient_sampling_interval), max(setup.nt, 1)))
+    kind, _, bpe, fmt = _resolve_storage_compression(
+        storage_compression, epsilon.dtype, epsilon.device, context="storage_compression")
+    policy, seg, plan = _plan_storage(setup, interval, bpe, storage_bytes_limit_device,
+                                      int(storage_chunk_steps), epsilon.device)
+    # one chunk at a time: the plan's chunks need not be resident together
+    if tuple(observed.shape) != (setup.nt, setup.n_shots, setup.n_rec):
+        raise ValueError("observed data must be [nt, shots, receivers]")
+    dev, dtype = setup.ca.device, setup.ca.dtype
+    ca, cb, cq = (t.detach() for t in (setup.ca, setup.cb, setup.cq))
+    f = setup.f.detach().contiguous() if setup.f.numel() > 0 else torch.empty(0, device=dev, dtype=dtype)
+    fields = [t.contiguous() for t in setup.fields]
+    r = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
+    numel = setup.nz * setup.ny * setup.nx
+    max_c = max(plan.sizes)
+    lam_all = [torch.empty(max_c * numel, device=dev, dtype=dtype) for _ in range(18)]
+    gca = torch.zeros(setup.nz, setup.ny, setup.nx, device=dev, dtype=torch.float64)
+    gcb = torch.zeros_like(gca)
+    grad_f = torch.zeros(setup.nt, setup.n_shots, setup.n_src, device=dev, dtype=dtype)
+    value = torch.zeros((), device=dev, dtype=torch.float64)
+    chunked = plan.n_chunks > 1
+    pool: dict = {}
+    for i in range(plan.n_chunks):
+        s0, s1 = plan.bounds[i], plan.bounds[i + 1]
+        n_c = s1 - s0
+        sub = _chunk_setup3(setup, s0, s1, [t[s0:s1] for t in fields] if chunked else fields)
+        f_i = f[:, s0:s1].contiguous() if (chunked and setup.n_src > 0) else f
+        r_i = torch.zeros(setup.nt, n_c, setup.n_rec, device=dev, dtype=dtype) if chunked else r
+        saved = _forward_chunk(sub, ca, cb, cq, f_i, r_i, policy=policy, seg=seg,
+                               interval=interval, fmt=fmt, want_ca=True, want_cb=True, keep=True)
+        if chunked:
+            r[:, s0:s1].copy_(r_i)
+        v_i, g_i = residual(r_i, observed[:, s0:s1])
+        value = value + v_i.to(torch.float64)
+        lam = [t[: n_c * numel].view(n_c, setup.nz, setup.ny, setup.nx) for t in lam_all]
+        for t in lam:
+            t.zero_()
+        gf_i = grad_f[:, s0:s1].contiguous() if chunked else grad_f
+        _backward_chunk(sub, ca, cb, cq, f_i, g_i.to(dtype).contiguous(), lam, saved, seg=seg,
+                        interval=interval, fmt=fmt, want_ca=True, want_cb=True,
+                        grad_f=gf_i if setup.n_src > 0 else None, gca=gca, gcb=gcb, pool=pool)
+        del saved
+        if chunked and setup.n_src > 0:
+            grad_f[:, s0:s1].copy_(gf_i)
+    params = [p for p, n in ((eps, "epsilon"), (sig, "sigma")) if n in targets]
+    outs = [setup.ca, setup.cb]
+    grads = [gca.to(dtype)[None], gcb.to(dtype)[None]]
+    if setup.n_src > 0 and setup.f.requires_grad:
+        outs.append(setup.f)
+        grads.append(grad_f)
+    pg = torch.autograd.grad(outs, params, grads, allow_unused=True)
+    names = [n for n in ("epsilon", "sigma") if n in targets]
+    unpadded = [setup.unpad(t) for t in fields]
+    info = {"policy": policy, "segment": seg, "shot_chunks": plan.sizes}
+    return value, dict(zip(names, pg)), r, unpadded, info
+
+
+__all__ = ["EM3DPropagate", "EM3DSetup", "STATE_NAMES", "maxwell3d", "prepare_em3d",
+           "separable_gradient3d"]

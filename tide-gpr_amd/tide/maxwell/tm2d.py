@@ -245,11 +245,157 @@ def prepare_tm2d(
         rdy=1.0 / dy, rdx=1.0 / dx, dt=float(dt), nt=nt, n_shots=n_shots,
         ny=ny, nx=nx, n_src=n_src, n_rec=n_rec, pml=pml, fd_pad=fd_pad,
         pml_width=widths, batched=batched,
+        extra={"zero_init": all(t0 is None for t0 in initial)},
     )
 
 
 def _profile_ptrs(profiles: list[torch.Tensor]) -> list[int | None]:
     return [_native.ptr(p) for p in profiles]
+
+
+def _chunk_setup(setup: TM2DSetup, s0: int, s1: int, fields: list[torch.Tensor]) -> TM2DSetup:
+    """The shots ``[s0, s1)`` of ``setup`` as a setup of their own (views)."""
+    if s0 == 0 and s1 == setup.n_shots and fields is setup.fields:
+        return setup
+    sub = TM2DSetup(**{k: getattr(setup, k) for k in setup.__dataclass_fields__})
+    sub.n_shots = s1 - s0
+    sub.sources_i = setup.sources_i[s0:s1].contiguous() if setup.n_src > 0 else setup.sources_i
+    sub.receivers_i = setup.receivers_i[s0:s1].contiguous() if setup.n_rec > 0 else setup.receivers_i
+    sub.fields = fields
+    sub.extra = {}
+    return sub
+
+
+def _shots(t: torch.Tensor, s0: int, s1: int, batched: bool) -> torch.Tensor:
+    return t[s0:s1] if batched else t
+
+
+def _run_forward(setup: TM2DSetup, ca, cb, cq, f, fields, r, *, interval: int, fmt: int,
+                 ey_store=None, curl_store=None) -> None:
+    """One native forward over ``setup.nt`` steps on the caller's stream;
+    stores snapshots when a store buffer is given."""
+    dev, dtype = ca.device, ca.dtype
+    stream = _native.stream_handle(dev)
+    dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    tail = (
+        _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
+        setup.rdy, setup.rdx, setup.dt,
+        setup.nt, setup.n_shots, setup.ny, setup.nx, setup.n_src, setup.n_rec,
+    )
+    if ey_store is not None or curl_store is not None:
+        fn = _native.get_function("maxwell_tm", "forward_with_storage", 2, dtype)
+        shot_bytes = setup.ny * setup.nx * (2 if fmt == STORAGE_FORMAT_BF16 else dtype.itemsize)
+        status = fn(
+            _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
+            *[_native.ptr(t) for t in fields], _native.ptr(r),
+            _native.ptr(ey_store), None, None, _native.ptr(curl_store), None, None,
+            *_profile_ptrs(setup.profiles), *tail, interval,
+            STORAGE_DEVICE, fmt, shot_bytes, ey_store is not None, curl_store is not None,
+            setup.batched, setup.batched, setup.batched, 0,
+            *setup.pml, 0, dev_idx, stream, stream,
+        )
+    else:
+        fn = _native.get_function("maxwell_tm", "forward", 2, dtype)
+        status = fn(
+            _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
+            *[_native.ptr(t) for t in fields],
+            None, None, None, None, None, _native.ptr(r), 0,
+            *_profile_ptrs(setup.profiles), *tail, interval, False,
+            setup.batched, setup.batched, setup.batched, 0,
+            *setup.pml, 0, dev_idx, stream,
+        )
+    _native.check(status)
+
+
+def _run_backward(setup: TM2DSetup, ca, cb, cq, grad_r, lam, ey_store, curl_store, grad_f,
+                  gca, gcb, *, interval: int, fmt: int) -> None:
+    """One native exact-adjoint sweep; ``gca``/``gcb`` are FP64 accumulators
+    (added to)."""
+    dev, dtype = ca.device, ca.dtype
+    shot_bytes = setup.ny * setup.nx * (2 if fmt == STORAGE_FORMAT_BF16 else dtype.itemsize)
+    fn = getattr(_native.load(), f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
+    dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    status = fn(
+        _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(grad_r),
+        *[_native.ptr(t) for t in lam],
+        _native.ptr(ey_store) if gca is not None else None,
+        _native.ptr(curl_store) if gcb is not None else None,
+        _native.ptr(grad_f), _native.ptr(gca), _native.ptr(gcb),
+        *_profile_ptrs(setup.profiles),
+        _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
+        setup.rdy, setup.rdx, setup.nt, setup.n_shots, setup.ny, setup.nx,
+        setup.n_src, setup.n_rec, interval, fmt, shot_bytes,
+        setup.batched, setup.batched, setup.batched, setup.nt, *setup.pml,
+        dev_idx, _native.stream_handle(dev),
+    )
+    _native.check(status)
+
+
+@dataclass
+class ShotPlan:
+    """Shot chunks of one propagation: chunk ``i`` holds shots
+    ``bounds[i]:bounds[i+1]``; the first ``resident`` chunks keep their
+    snapshots in device memory between the forward and the backward, the
+    others are re-run (forward with storage) inside the backward."""
+
+    bounds: list[int]
+    resident: int
+
+    @property
+    def n_chunks(self) -> int:
+        return len(self.bounds) - 1
+
+    @property
+    def sizes(self) -> list[int]:
+        return [b - a for a, b in zip(self.bounds[:-1], self.bounds[1:])]
+
+
+def plan_shot_chunks(n_shots: int, per_shot_bytes: int, budget_bytes: float,
+                     group: int = 4) -> ShotPlan:
+    """Split ``n_shots`` so that one chunk's snapshots fit ``budget_bytes``.
+
+    Chunks are balanced; a chunk of at least ``group`` shots is cut to a
+    multiple of ``group`` (the adjoint's shots per workgroup) so its
+    workgroups are full.  As many chunks as the budget holds together stay
+    resident; the rest are recomputed in the backward.  Raises when not even
+    one shot's snapshots fit.
+    """
+    if n_shots <= 0:
+        return ShotPlan([0, max(n_shots, 0)], 1)
+    if per_shot_bytes <= 0 or per_shot_bytes * n_shots <= budget_bytes:
+        return ShotPlan([0, n_shots], 1)
+    cap = int(budget_bytes // per_shot_bytes)
+    if cap < 1:
+        raise RuntimeError(
+            f"the snapshots of one shot ({per_shot_bytes / 1e9:.2f} GB) exceed the device "
+            f"budget ({budget_bytes / 1e9:.2f} GB); use BF16 snapshot compression, a larger "
+            "model_gradient_sampling_interval or storage_mode='cpu'/'disk'"
+        )
+    if cap >= group:
+        cap = cap // group * group
+    k = -(-n_shots // cap)
+    base, extra = divmod(n_shots, k)
+    sizes = [base + (1 if i < extra else 0) for i in range(k)]
+    bounds = [0]
+    for c in sizes:
+        bounds.append(bounds[-1] + c)
+    resident, used = 0, 0
+    for c in sizes:
+        if used + c * per_shot_bytes > budget_bytes:
+            break
+        used += c * per_shot_bytes
+        resident += 1
+    return ShotPlan(bounds, resident)
+
+
+def device_snapshot_budget(device: torch.device, limit: int | None, working_per_shot: int,
+                           n_shots: int) -> float:
+    """Bytes of snapshots one call may hold on ``device``: the caller's
+    limit, else 90% of the free memory less the working state."""
+    if limit is not None:
+        return float(limit)
+    free, _ = torch.cuda.mem_get_info(device)
+    return max(0.0, 0.9 * free - 2.0 * working_per_shot * n_shots - (256 << 20))
 
 
 class TM2DPropagate(torch.autograd.Function):
@@ -261,6 +407,12 @@ class TM2DPropagate(torch.autograd.Function):
     Gradients flow to ``ca``, ``cb`` and ``f`` from the receiver
     cotangent; cotangents of the final state are ignored, as in the
     reference (``tm2d_autograd.py:519-547``).
+
+    Shots run in the chunks of ``storage["plan"]``: chunks whose snapshots
+    fit the device budget keep them until the backward, the others are
+    re-run from their initial state in the backward.  One FP64 accumulator
+    per coefficient collects every chunk's gradient, cast to the field
+    dtype once at the end.
     """
 
     @staticmethod
@@ -270,56 +422,52 @@ class TM2DPropagate(torch.autograd.Function):
         fields = [t.contiguous() for t in setup.fields]
         r = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
         f_c = f.detach().contiguous()
-        stream = _native.stream_handle(dev)
-        dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
         interval = storage["interval"]
+        fmt = storage["format"]
         want_ca = ctx.needs_input_grad[0]
         want_cb = ctx.needs_input_grad[1]
         store = storage["mode"] == "device" and (want_ca or want_cb)
-        common_tail = (
-            _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
-            setup.rdy, setup.rdx, setup.dt,
-            setup.nt, setup.n_shots, setup.ny, setup.nx, setup.n_src, setup.n_rec,
-        )
-        ey_store = curl_store = None
+        plan: ShotPlan = storage.get("plan") or ShotPlan([0, setup.n_shots], 1)
+        n_store = (setup.nt + interval - 1) // interval
+        sdtype = torch.int16 if fmt == STORAGE_FORMAT_BF16 else dtype
+        chunked = plan.n_chunks > 1
+        stores: list[tuple[torch.Tensor | None, torch.Tensor | None] | None] = []
+        initial: list[list[torch.Tensor] | None] = []
         t0 = _timer.start() if _timer is not None else None
-        if store:
-            n_store = (setup.nt + interval - 1) // interval
-            sdtype = torch.int16 if storage["format"] == STORAGE_FORMAT_BF16 else dtype
-            shape = (n_store, setup.n_shots, setup.ny, setup.nx)
-            ey_store = torch.empty(shape, device=dev, dtype=sdtype) if want_ca else None
-            curl_store = torch.empty(shape, device=dev, dtype=sdtype) if want_cb else None
-            fn = _native.get_function("maxwell_tm", "forward_with_storage", 2, dtype)
-            shot_bytes = setup.ny * setup.nx * (2 if storage["format"] == STORAGE_FORMAT_BF16 else dtype.itemsize)
-            status = fn(
-                _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f_c),
-                *[_native.ptr(t) for t in fields], _native.ptr(r),
-                _native.ptr(ey_store), None, None, _native.ptr(curl_store), None, None,
-                *_profile_ptrs(setup.profiles),
-                *common_tail[:5], *common_tail[5:], interval,
-                STORAGE_DEVICE, storage["format"], shot_bytes, want_ca, want_cb,
-                setup.batched, setup.batched, setup.batched, 0,
-                *setup.pml, 0, dev_idx, stream, stream,
-            )
-        else:
-            fn = _native.get_function("maxwell_tm", "forward", 2, dtype)
-            status = fn(
-                _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f_c),
-                *[_native.ptr(t) for t in fields],
-                None, None, None, None, None, _native.ptr(r), 0,
-                *_profile_ptrs(setup.profiles),
-                *common_tail, interval, False,
-                setup.batched, setup.batched, setup.batched, 0,
-                *setup.pml, 0, dev_idx, stream,
-            )
-        _native.check(status)
+        for i in range(plan.n_chunks):
+            s0, s1 = plan.bounds[i], plan.bounds[i + 1]
+            sub = _chunk_setup(setup, s0, s1, [t[s0:s1] for t in fields] if chunked else fields)
+            keep = store and i < plan.resident
+            if store and not keep:
+                # re-run in the backward from this initial state
+                initial.append(None if setup.extra.get("zero_init") else
+                               [t.clone() for t in sub.fields])
+            else:
+                initial.append(None)
+            ey_s = curl_s = None
+            if keep:
+                shape = (n_store, s1 - s0, setup.ny, setup.nx)
+                ey_s = torch.empty(shape, device=dev, dtype=sdtype) if want_ca else None
+                curl_s = torch.empty(shape, device=dev, dtype=sdtype) if want_cb else None
+            stores.append((ey_s, curl_s) if keep else None)
+            f_i = f_c[:, s0:s1].contiguous() if (chunked and setup.n_src > 0) else f_c
+            r_i = torch.zeros(setup.nt, s1 - s0, setup.n_rec, device=dev, dtype=dtype) if chunked else r
+            _run_forward(sub, _shots(ca, s0, s1, setup.batched), _shots(cb, s0, s1, setup.batched),
+                         _shots(cq, s0, s1, setup.batched), f_i, sub.fields, r_i,
+                         interval=interval, fmt=fmt, ey_store=ey_s, curl_store=curl_s)
+            if chunked and setup.n_rec > 0:
+                r[:, s0:s1].copy_(r_i)
         if t0 is not None:
             _timer.stop("forward_store" if store else "forward", setup.nt,
                         setup.n_shots * setup.ny * setup.nx, t0, _native.get_tm2d_time_block()[0])
         ctx.setup = setup
         ctx.storage = storage
         ctx.stored = store
-        ctx.save_for_backward(ca, cb, cq, ey_store, curl_store)
+        ctx.plan = plan
+        ctx.stores = stores
+        ctx.initial = initial
+        ctx.f_data = f_c if plan.resident < plan.n_chunks else None
+        ctx.save_for_backward(ca, cb, cq)
         ctx.mark_non_differentiable(*fields)
         return (*fields, r)
 
@@ -327,7 +475,8 @@ class TM2DPropagate(torch.autograd.Function):
     def backward(ctx, *grads):
         grad_r = grads[-1]
         setup: TM2DSetup = ctx.setup
-        ca, cb, cq, ey_store, curl_store = ctx.saved_tensors
+        plan: ShotPlan = ctx.plan
+        ca, cb, cq = ctx.saved_tensors
         want_ca = ctx.needs_input_grad[0]
         want_cb = ctx.needs_input_grad[1]
         want_f = ctx.needs_input_grad[3] and setup.n_src > 0
@@ -336,38 +485,68 @@ class TM2DPropagate(torch.autograd.Function):
                 "model gradients need stored snapshots; storage_mode='none' was used"
             )
         dev, dtype = ca.device, ca.dtype
-        shape = (setup.n_shots, setup.ny, setup.nx)
+        interval = ctx.storage["interval"]
+        fmt = ctx.storage["format"]
         if grad_r is None:
             grad_r = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
-        grad_r = grad_r.to(dtype).contiguous()
-        lam = [torch.zeros(shape, device=dev, dtype=dtype) for _ in range(7)]
+        grad_r = grad_r.to(dtype)
+        chunked = plan.n_chunks > 1
+        max_c = max(plan.sizes)
+        lam_all = [torch.empty(max_c, setup.ny, setup.nx, device=dev, dtype=dtype) for _ in range(7)]
         grad_f = (
             torch.zeros(setup.nt, setup.n_shots, setup.n_src, device=dev, dtype=dtype)
             if want_f else None
         )
-        gshape = shape if setup.batched else (setup.ny, setup.nx)
+        gshape = (setup.n_shots, setup.ny, setup.nx) if setup.batched else (setup.ny, setup.nx)
         gca = torch.zeros(gshape, device=dev, dtype=torch.float64) if want_ca else None
         gcb = torch.zeros(gshape, device=dev, dtype=torch.float64) if want_cb else None
-        fmt = ctx.storage["format"]
-        shot_bytes = setup.ny * setup.nx * (2 if fmt == STORAGE_FORMAT_BF16 else dtype.itemsize)
-        fn = getattr(_native.load(), f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
-        stream = _native.stream_handle(dev)
-        dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        n_store = (setup.nt + interval - 1) // interval
+        sdtype = torch.int16 if fmt == STORAGE_FORMAT_BF16 else dtype
         t0 = _timer.start() if _timer is not None else None
-        status = fn(
-            _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(grad_r),
-            *[_native.ptr(t) for t in lam],
-            _native.ptr(ey_store) if want_ca else None,
-            _native.ptr(curl_store) if want_cb else None,
-            _native.ptr(grad_f), _native.ptr(gca), _native.ptr(gcb),
-            *_profile_ptrs(setup.profiles),
-            _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
-            setup.rdy, setup.rdx, setup.nt, setup.n_shots, setup.ny, setup.nx,
-            setup.n_src, setup.n_rec, ctx.storage["interval"], fmt, shot_bytes,
-            setup.batched, setup.batched, setup.batched, setup.nt, *setup.pml,
-            dev_idx, stream,
-        )
-        _native.check(status)
+        rerun_ey = rerun_curl = None
+        for i in range(plan.n_chunks):  # resident chunks come first
+            s0, s1 = plan.bounds[i], plan.bounds[i + 1]
+            n_c = s1 - s0
+            ca_i, cb_i, cq_i = (_shots(t, s0, s1, setup.batched) for t in (ca, cb, cq))
+            held = ctx.stores[i]
+            ctx.stores[i] = None
+            if held is not None or not (want_ca or want_cb):
+                ey_s, curl_s = held if held is not None else (None, None)
+                sub = _chunk_setup(setup, s0, s1, [])
+            else:
+                # re-run this chunk's forward with storage, into one buffer
+                # per coefficient sized for the largest chunk
+                numel = n_store * max_c * setup.ny * setup.nx
+                if rerun_ey is None and rerun_curl is None:
+                    rerun_ey = torch.empty(numel, device=dev, dtype=sdtype) if want_ca else None
+                    rerun_curl = torch.empty(numel, device=dev, dtype=sdtype) if want_cb else None
+                init = ctx.initial[i]
+                state = ([torch.zeros(n_c, setup.ny, setup.nx, device=dev, dtype=dtype)
+                          for _ in range(7)] if init is None else init)
+                sub = _chunk_setup(setup, s0, s1, state)
+                f_d = ctx.f_data
+                f_i = f_d[:, s0:s1].contiguous() if setup.n_src > 0 else f_d
+                r_i = torch.empty(setup.nt, n_c, setup.n_rec, device=dev, dtype=dtype)
+                view = (n_store, n_c, setup.ny, setup.nx)
+                used = n_store * n_c * setup.ny * setup.nx
+                ey_s = rerun_ey[:used].view(view) if rerun_ey is not None else None
+                curl_s = rerun_curl[:used].view(view) if rerun_curl is not None else None
+                _run_forward(sub, ca_i, cb_i, cq_i, f_i, sub.fields, r_i,
+                             interval=interval, fmt=fmt, ey_store=ey_s, curl_store=curl_s)
+                ctx.initial[i] = None
+            lam = [t[:n_c] for t in lam_all]
+            for t in lam:
+                t.zero_()
+            gr_i = grad_r[:, s0:s1].contiguous() if chunked else grad_r.contiguous()
+            gf_i = (torch.zeros(setup.nt, n_c, setup.n_src, device=dev, dtype=dtype)
+                    if (want_f and chunked) else grad_f)
+            _run_backward(sub, ca_i, cb_i, cq_i, gr_i, lam, ey_s, curl_s, gf_i,
+                          _shots(gca, s0, s1, setup.batched) if gca is not None else None,
+                          _shots(gcb, s0, s1, setup.batched) if gcb is not None else None,
+                          interval=interval, fmt=fmt)
+            if want_f and chunked:
+                grad_f[:, s0:s1].copy_(gf_i)
+            del ey_s, curl_s, held
         if t0 is not None:
             _timer.stop("adjoint", setup.nt, setup.n_shots * setup.ny * setup.nx, t0,
                         _native.get_tm2d_time_block()[1])
@@ -473,8 +652,52 @@ def maxwelltm(
     m_Hz_x, receiver_amplitudes)``; fields unpadded ``[shots, ny, nx]``,
     receivers ``[nt, shots, n_rec]`` (``[nt, B, S, n_rec]`` for a batch of
     several models)."""
-    del freq_taper_frac, time_pad_frac, time_taper, storage_path, storage_chunk_steps
-    del n_threads, fallback, callback_frequency
+    call = _prepare_call(
+        epsilon, sigma, mu, grid_spacing, dt, source_amplitude, source_location,
+        receiver_location, stencil=stencil, pml_width=pml_width, max_vel=max_vel,
+        initial=(Ey_0, Hx_0, Hz_0, m_Ey_x, m_Ey_z, m_Hx_z, m_Hz_x), nt=nt,
+        model_gradient_sampling_interval=model_gradient_sampling_interval,
+        save_snapshots=save_snapshots, forward_callback=forward_callback,
+        backward_callback=backward_callback, python_backend=python_backend,
+        storage_mode=storage_mode, storage_compression=storage_compression,
+        storage_bytes_limit_device=storage_bytes_limit_device,
+        storage_bytes_limit_host=storage_bytes_limit_host, dispersion=dispersion,
+    )
+    setup, storage = call.setup, call.storage
+    f = setup.f if setup.f.numel() > 0 else torch.empty(0, device=setup.ca.device, dtype=setup.ca.dtype)
+    out = TM2DPropagate.apply(setup.ca, setup.cb, setup.cq, f, setup, storage)
+    return call.finish(list(out[:7]), out[7])
+
+
+@dataclass
+class _Call:
+    """A validated ``maxwelltm`` call: padded setup, storage policy, and
+    the structured-batch shape to restore on the outputs."""
+
+    setup: TM2DSetup
+    storage: dict[str, Any]
+    bs: tuple[int, int] | None
+
+    def finish(self, padded_fields: list[torch.Tensor], r: torch.Tensor):
+        setup = self.setup
+        fields = [setup.unpad(t) for t in padded_fields]
+        if setup.n_rec == 0:
+            r = torch.empty(0, device=setup.ca.device, dtype=setup.ca.dtype)
+        if self.bs is not None and self.bs[0] > 1:
+            B, S = self.bs
+            fields = [t.reshape(B, S, *t.shape[1:]) for t in fields]
+            if r.numel() > 0:
+                r = r.reshape(r.shape[0], B, S, *r.shape[2:])
+        return (*fields, r)
+
+
+def _prepare_call(
+    epsilon, sigma, mu, grid_spacing, dt, source_amplitude, source_location, receiver_location,
+    *, stencil, pml_width, max_vel, initial, nt, model_gradient_sampling_interval,
+    save_snapshots, forward_callback, backward_callback, python_backend, storage_mode,
+    storage_compression, storage_bytes_limit_device, storage_bytes_limit_host, dispersion,
+    force_storage: bool = False,
+) -> _Call:
     _reject_unsupported(
         python_backend, stencil, forward_callback, backward_callback, dispersion, epsilon.device
     )
@@ -509,19 +732,19 @@ def maxwelltm(
 
     setup = prepare_tm2d(
         epsilon, sigma, mu, spacing, dt, source_amplitude, source_location, receiver_location,
-        stencil=stencil, pml_width=pml_width, nt=nt,
-        initial=(Ey_0, Hx_0, Hz_0, m_Ey_x, m_Ey_z, m_Hx_z, m_Hz_x),
+        stencil=stencil, pml_width=pml_width, nt=nt, initial=initial,
     )
     if setup.nt > 0:
         interval = min(interval, setup.nt)
 
-    model_rg = epsilon_in.requires_grad or sigma_in.requires_grad
+    model_rg = force_storage or epsilon_in.requires_grad or sigma_in.requires_grad
     do_save = model_rg if save_snapshots is None else bool(save_snapshots)
     kind, _, bpe, fmt = _resolve_storage_compression(
         storage_compression, epsilon.dtype, epsilon.device, context="storage_compression"
     )
     n_store = (setup.nt + interval - 1) // interval
-    total_bytes = n_store * setup.n_shots * setup.ny * setup.nx * bpe * 2
+    per_shot = n_store * setup.ny * setup.nx * bpe * 2
+    total_bytes = per_shot * setup.n_shots
     mode = resolve_storage_mode(
         storage_mode,
         needs_storage=do_save and model_rg,
@@ -534,19 +757,131 @@ def maxwelltm(
             f"storage_mode='{mode}' (host-staged snapshots) is not part of this build; "
             "use 'device' and split the shots into chunks that fit in HBM"
         )
-    storage = {"mode": mode, "format": fmt, "interval": interval, "kind": kind}
-    f = setup.f if setup.f.numel() > 0 else torch.empty(0, device=epsilon.device, dtype=epsilon.dtype)
-    out = TM2DPropagate.apply(setup.ca, setup.cb, setup.cq, f, setup, storage)
-    fields = [setup.unpad(t) for t in out[:7]]
-    r = out[7]
-    if setup.n_rec == 0:
-        r = torch.empty(0, device=epsilon.device, dtype=epsilon.dtype)
-    if bs is not None and bs[0] > 1:
-        B, S = bs
-        fields = [t.reshape(B, S, *t.shape[1:]) for t in fields]
-        if r.numel() > 0:
-            r = r.reshape(r.shape[0], B, S, *r.shape[2:])
-    return (*fields, r)
+    plan = ShotPlan([0, setup.n_shots], 1)
+    if mode == "device" and setup.nt > 0:
+        working = 21 * setup.ny * setup.nx * epsilon.dtype.itemsize
+        budget = device_snapshot_budget(epsilon.device, storage_bytes_limit_device, working,
+                                        setup.n_shots)
+        plan = plan_shot_chunks(setup.n_shots, per_shot, budget,
+                                group=max(1, _native.get_tm2d_tuning()[3]))
+    storage = {"mode": mode, "format": fmt, "interval": interval, "kind": kind, "plan": plan}
+    return _Call(setup, storage, bs)
 
 
-__all__ = ["KernelTimer", "TM2DPropagate", "TM2DSetup", "maxwelltm", "prepare_tm2d", "set_kernel_timer"]
+def separable_gradient(
+    epsilon: torch.Tensor,
+    sigma: torch.Tensor,
+    mu: torch.Tensor,
+    grid_spacing: float | Sequence[float],
+    dt: float,
+    source_amplitude: torch.Tensor | None,
+    source_location: torch.Tensor | None,
+    receiver_location: torch.Tensor | None,
+    observed: torch.Tensor,
+    *,
+    residual: Any,
+    stencil: int = 2,
+    pml_width: int | Sequence[int] = 20,
+    max_vel: float | None = None,
+    nt: int | None = None,
+    model_gradient_sampling_interval: int = 1,
+    storage_compression: bool | str = False,
+    storage_bytes_limit_device: int | None = None,
+    targets: Sequence[str] = ("epsilon", "sigma"),
+):
+    """Gradient of a receiver objective that is a sum over shots.
+
+    ``residual(pred_chunk, observed_chunk) -> (value, d value / d pred)``
+    for one chunk of shots.  Each chunk of ``storage`` runs its forward
+    with device snapshots, its receiver residual and its exact adjoint
+    before the next chunk starts, so no chunk is ever re-run; the
+    coefficient gradients of all chunks sum into one FP64 accumulator, the
+    objective value is summed in FP64, and the chain rule to epsilon/sigma
+    (padding fold and source term included) is applied once at the end by
+    torch autograd, as in the operator path.
+
+    Returns ``(value, {"epsilon": g, "sigma": g}, receivers, final fields,
+    {"shot_chunks": sizes})`` with the receivers and final fields of every
+    shot.
+    """
+    eps = epsilon.detach().requires_grad_("epsilon" in targets)
+    sig = sigma.detach().requires_grad_("sigma" in targets)
+    call = _prepare_call(
+        eps, sig, mu.detach(), grid_spacing, dt, source_amplitude, source_location,
+        receiver_location, stencil=stencil, pml_width=pml_width, max_vel=max_vel,
+        initial=(None,) * 7, nt=nt,
+        model_gradient_sampling_interval=model_gradient_sampling_interval,
+        save_snapshots=True, forward_callback=None, backward_callback=None,
+        python_backend=False, storage_mode="device", storage_compression=storage_compression,
+        storage_bytes_limit_device=storage_bytes_limit_device, storage_bytes_limit_host=None,
+        dispersion=None, force_storage=True,
+    )
+    if call.bs is not None and call.bs[0] > 1:
+        raise NotImplementedError("separable_gradient takes one model shared by all shots")
+    setup, storage = call.setup, call.storage
+    plan: ShotPlan = storage["plan"]
+    dev, dtype = setup.ca.device, setup.ca.dtype
+    interval, fmt = storage["interval"], storage["format"]
+    if tuple(observed.shape) != (setup.nt, setup.n_shots, setup.n_rec):
+        raise ValueError("observed data must be [nt, shots, receivers]")
+    ca, cb, cq = (t.detach() for t in (setup.ca, setup.cb, setup.cq))
+    f = setup.f.detach().contiguous() if setup.f.numel() > 0 else torch.empty(0, device=dev, dtype=dtype)
+    fields = [t.contiguous() for t in setup.fields]
+    r = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
+    n_store = (setup.nt + interval - 1) // interval
+    sdtype = torch.int16 if fmt == STORAGE_FORMAT_BF16 else dtype
+    max_c = max(plan.sizes)
+    numel = n_store * max_c * setup.ny * setup.nx
+    ey_buf = torch.empty(numel, device=dev, dtype=sdtype)
+    curl_buf = torch.empty(numel, device=dev, dtype=sdtype)
+    lam_all = [torch.empty(max_c, setup.ny, setup.nx, device=dev, dtype=dtype) for _ in range(7)]
+    gca = torch.zeros(setup.ny, setup.nx, device=dev, dtype=torch.float64)
+    gcb = torch.zeros(setup.ny, setup.nx, device=dev, dtype=torch.float64)
+    grad_f = torch.zeros(setup.nt, setup.n_shots, setup.n_src, device=dev, dtype=dtype)
+    value = torch.zeros((), device=dev, dtype=torch.float64)
+    chunked = plan.n_chunks > 1
+    blocks = _native.get_tm2d_time_block()
+    for i in range(plan.n_chunks):
+        s0, s1 = plan.bounds[i], plan.bounds[i + 1]
+        n_c = s1 - s0
+        sub = _chunk_setup(setup, s0, s1, [t[s0:s1] for t in fields] if chunked else fields)
+        view = (n_store, n_c, setup.ny, setup.nx)
+        used = n_store * n_c * setup.ny * setup.nx
+        ey_s, curl_s = ey_buf[:used].view(view), curl_buf[:used].view(view)
+        f_i = f[:, s0:s1].contiguous() if (chunked and setup.n_src > 0) else f
+        r_i = torch.zeros(setup.nt, n_c, setup.n_rec, device=dev, dtype=dtype) if chunked else r
+        tf = _timer.start() if _timer is not None else None
+        _run_forward(sub, ca, cb, cq, f_i, sub.fields, r_i, interval=interval, fmt=fmt,
+                     ey_store=ey_s, curl_store=curl_s)
+        if tf is not None:
+            _timer.stop("forward_store", setup.nt, n_c * setup.ny * setup.nx, tf, blocks[0])
+        if chunked:
+            r[:, s0:s1].copy_(r_i)
+        v_i, g_i = residual(r_i, observed[:, s0:s1])
+        value = value + v_i.to(torch.float64)
+        lam = [t[:n_c] for t in lam_all]
+        for t in lam:
+            t.zero_()
+        gf_i = grad_f[:, s0:s1].contiguous() if chunked else grad_f
+        ta = _timer.start() if _timer is not None else None
+        _run_backward(sub, ca, cb, cq, g_i.to(dtype).contiguous(), lam, ey_s, curl_s,
+                      gf_i if setup.n_src > 0 else None, gca, gcb, interval=interval, fmt=fmt)
+        if ta is not None:
+            _timer.stop("adjoint", setup.nt, n_c * setup.ny * setup.nx, ta, blocks[1])
+        if chunked and setup.n_src > 0:
+            grad_f[:, s0:s1].copy_(gf_i)
+    params = [p for p, n in ((eps, "epsilon"), (sig, "sigma")) if n in targets]
+    outs = [setup.ca, setup.cb]
+    grads = [gca.to(dtype)[None], gcb.to(dtype)[None]]
+    if setup.n_src > 0 and setup.f.requires_grad:
+        outs.append(setup.f)
+        grads.append(grad_f)
+    pg = torch.autograd.grad(outs, params, grads, allow_unused=True)
+    names = [n for n in ("epsilon", "sigma") if n in targets]
+    out = call.finish(fields, r)
+    info = {"shot_chunks": plan.sizes}
+    return value, dict(zip(names, pg)), out[7], list(out[:7]), info
+
+
+__all__ = ["KernelTimer", "ShotPlan", "TM2DPropagate", "TM2DSetup", "maxwelltm",
+           "plan_shot_chunks", "prepare_tm2d", "separable_gradient", "set_kernel_timer"]
