@@ -80,49 +80,105 @@ def parse() -> argparse.Namespace:
     return ap.parse_args()
 
 
+def _has_avx2() -> bool:
+    try:
+        return " avx2" in Path("/proc/cpuinfo").read_text()
+    except OSError:
+        return False
+
+
 def cpu_baseline(args, setup_kwargs) -> dict:
-    """The reference's own CPU forward_with_storage + backward (compiled from
-    its sources by oracle/Makefile) on a bounded sample, all host cores."""
+    """CPU baselines on the host cores, on bounded samples of the headline
+    workload (the first shots, full grid, shortened nt):
+
+    * value: the reference's own CPU forward_with_storage + backward built
+      with its Release flags (-Ofast -mavx2, src/tide/csrc/CMakeLists.txt)
+      from its sources by oracle/Makefile, all host cores (OpenMP);
+    * the same at -O2; the reference CPU forward on the 96x96 x 300-step
+      case; and this project's CPU restatement with the exact adjoint (the
+      reference's CPU backward re-runs the forward kernels on the adjoint
+      fields, which is not the exact transpose with CPML).
+    """
     import numpy as np
-    import torch
 
     from oracle import oracle as O
     from tide.maxwell.tm2d import prepare_tm2d
 
     cores = len(os.sched_getaffinity(0))
+
+    def sample(n_shots, nt):
+        kw = dict(setup_kwargs)
+        kw["source_amplitude"] = kw["source_amplitude"][:n_shots, :, :nt].cpu()
+        kw["source_location"] = kw["source_location"][:n_shots].cpu()
+        kw["receiver_location"] = kw["receiver_location"][:n_shots].cpu()
+        for k in ("epsilon", "sigma", "mu"):
+            kw[k] = kw[k].detach().cpu()
+        case = O.case_from_setup(prepare_tm2d(**kw, nt=nt))
+        grad_r = np.zeros((nt, n_shots, case.rec.shape[1]), np.float32)
+        grad_r[nt // 2 :] = 1.0
+        return case, grad_r
+
+    def rate(updates, secs):
+        return round(updates / secs / 1e9, 4)
+
+    out: dict = {"unit": "Gcell-updates/s", "cores": cores}
     n_shots, nt = 2, args.cpu_sample_nt
-    kw = dict(setup_kwargs)
-    kw["source_amplitude"] = kw["source_amplitude"][:n_shots, :, :nt].cpu()
-    kw["source_location"] = kw["source_location"][:n_shots].cpu()
-    kw["receiver_location"] = kw["receiver_location"][:n_shots].cpu()
-    for k in ("epsilon", "sigma", "mu"):
-        kw[k] = kw[k].detach().cpu()
-    st = prepare_tm2d(**kw, nt=nt)
-    case = O.case_from_setup(st)
-    grad_r = np.zeros((nt, n_shots, case.rec.shape[1]), np.float32)
-    grad_r[nt // 2 :] = 1.0
+    case, grad_r = sample(n_shots, nt)
+    updates = 2.0 * n_shots * case.ny * case.nx * nt
+    desc = f"{n_shots} shots x {case.ny}x{case.nx} x nt {nt} ({updates:.3e} cell-updates)"
+    if O.ref_available(fast=True) and _has_avx2():
+        t0 = time.perf_counter()
+        O.ref_tm2d_forward_with_storage_and_backward(case, grad_r, n_threads=cores, fast=True)
+        secs = time.perf_counter() - t0
+        out.update(value=rate(updates, secs), kind="reference",
+                   sample=f"reference CPU forward_with_storage + backward, Release flags "
+                          f"(-Ofast -mavx2, OpenMP): {desc}, {secs:.1f} s")
     if O.ref_available():
-        kind = "reference"
         t0 = time.perf_counter()
         O.ref_tm2d_forward_with_storage_and_backward(case, grad_r, n_threads=cores)
         secs = time.perf_counter() - t0
-        what = "reference CPU forward_with_storage + backward (OpenMP, -O2)"
-    else:
-        kind = "port"
-        t0 = time.perf_counter()
-        _, _, es, cs = O.tm2d_forward(case, store="full")
-        O.tm2d_backward(case, grad_r, es, cs)
-        secs = time.perf_counter() - t0
-        what = "CPU oracle restatement, forward with storage + exact adjoint"
-    updates = 2.0 * n_shots * case.ny * case.nx * nt
-    return {
-        "value": round(updates / secs / 1e9, 4),
-        "unit": "Gcell-updates/s",
-        "cores": cores,
-        "kind": kind,
-        "sample": f"{what}: {n_shots} shots x {case.ny}x{case.nx} x nt {nt} "
-                  f"({updates:.3e} cell-updates, {secs:.1f} s)",
-    }
+        out["reference_O2"] = {"value": rate(updates, secs), "seconds": round(secs, 2),
+                               "sample": f"same, -O2: {desc}"}
+        if "value" not in out:
+            out.update(value=out["reference_O2"]["value"], kind="reference",
+                       sample="reference CPU forward_with_storage + backward (-O2): " + desc)
+        # the 96x96 (117x117 padded) x 300-step one-shot forward
+        import torch
+
+        import tide
+
+        n1, nt1, dt1 = 96, 300, 4e-11
+        eps1 = torch.full((n1, n1), 4.0)
+        st1 = prepare_tm2d(eps1, torch.zeros_like(eps1), torch.ones_like(eps1), 0.02, dt1,
+                           tide.ricker(8e8, nt1, dt1, dtype=torch.float32).reshape(1, 1, -1),
+                           torch.tensor([[[20, 48]]]), torch.tensor([[[20, 60]]]), pml_width=10)
+        case1 = O.case_from_setup(st1)
+        upd1 = float(case1.ny * case1.nx * nt1)
+        for fast in ((True, False) if _has_avx2() and O.ref_available(fast=True) else (False,)):
+            O.ref_tm2d_forward(case1, n_threads=cores, fast=fast)
+            reps = 20
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                O.ref_tm2d_forward(case1, n_threads=cores, fast=fast)
+            secs = (time.perf_counter() - t0) / reps
+            out["config1_forward_" + ("release" if fast else "O2")] = {
+                "value": rate(upd1, secs), "ms": round(secs * 1e3, 3),
+                "sample": "reference CPU forward, 96x96 (117x117 padded) x 300 steps, one shot"}
+    # this project's CPU restatement: forward with storage + exact adjoint
+    r_shots, r_nt = 1, 300
+    case_r, grad_r_r = sample(r_shots, r_nt)
+    t0 = time.perf_counter()
+    _, _, es, cs = O.tm2d_forward(case_r, store="full")
+    O.tm2d_backward(case_r, grad_r_r, es, cs)
+    secs = time.perf_counter() - t0
+    upd_r = 2.0 * r_shots * case_r.ny * case_r.nx * r_nt
+    port = {"value": rate(upd_r, secs), "seconds": round(secs, 2),
+            "sample": f"CPU restatement (oracle), forward with storage + exact adjoint, -O2 "
+                      f"OpenMP: {r_shots} shot x {case_r.ny}x{case_r.nx} x nt {r_nt}"}
+    out["port_exact_adjoint"] = port
+    if "value" not in out:
+        out.update(value=port["value"], kind="port", sample=port["sample"])
+    return out
 
 
 def _request_updates(kw) -> float:

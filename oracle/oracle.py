@@ -9,6 +9,9 @@ import this module, and only to check (or time beside) the HIP product path.
 * ``_ref/libtide_ref.so``: the reference's own CPU propagators compiled from
   the reference sources in place (``make -C oracle ref``), called through
   their own C ABI (reference ``backend_utils.py:256-308``).
+* ``_ref/libtide_ref_fast.so``: the same sources with the reference's
+  Release flags (-Ofast -mavx2), for timing the CPU baseline only (fast
+  math: never a parity reference).
 
 Arrays are numpy, C-contiguous; fields are ``[shots, ny, nx]``.
 """
@@ -25,9 +28,10 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 ORACLE_LIB = HERE / "_build" / "liboracle.so"
 REF_LIB = HERE / "_ref" / "libtide_ref.so"
+REF_FAST_LIB = HERE / "_ref" / "libtide_ref_fast.so"
 
 _oracle: ctypes.CDLL | None = None
-_ref: ctypes.CDLL | None = None
+_ref: dict[bool, ctypes.CDLL] = {}
 
 
 def build(ref_root: str | None = None) -> None:
@@ -35,7 +39,8 @@ def build(ref_root: str | None = None) -> None:
     subprocess.run(["make", "-s", "-C", str(HERE), "oracle"], check=True)
     root = Path(ref_root or "/root/reference")
     if (root / "src" / "tide" / "csrc").exists():
-        subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REF_ROOT={root}"], check=True)
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref", "ref_fast", f"REF_ROOT={root}"],
+                       check=True)
 
 
 def oracle_lib() -> ctypes.CDLL:
@@ -47,15 +52,14 @@ def oracle_lib() -> ctypes.CDLL:
     return _oracle
 
 
-def ref_available() -> bool:
-    return REF_LIB.exists()
+def ref_available(fast: bool = False) -> bool:
+    return (REF_FAST_LIB if fast else REF_LIB).exists()
 
 
-def ref_lib() -> ctypes.CDLL:
-    global _ref
-    if _ref is None:
-        _ref = ctypes.CDLL(str(REF_LIB))
-    return _ref
+def ref_lib(fast: bool = False) -> ctypes.CDLL:
+    if fast not in _ref:
+        _ref[fast] = ctypes.CDLL(str(REF_FAST_LIB if fast else REF_LIB))
+    return _ref[fast]
 
 
 def _p(a: np.ndarray | None):
@@ -250,11 +254,13 @@ def bf16_decode(h: np.ndarray) -> np.ndarray:
 
 
 # ------------------------------------------------------------ reference lib
-def ref_tm2d_forward(case: TMCase, *, n_threads: int = 0, nt: int | None = None):
+def ref_tm2d_forward(case: TMCase, *, n_threads: int = 0, nt: int | None = None,
+                     fast: bool = False):
     """The reference's own CPU forward (``maxwell_tm_<s>_<dtype>_forward_cpu``).
 
-    Returns ``(fields, r)``; ``n_threads`` 0 lets OpenMP decide."""
-    lib = ref_lib()
+    Returns ``(fields, r)``; ``n_threads`` 0 lets OpenMP decide; ``fast``
+    runs the Release-flag build."""
+    lib = ref_lib(fast)
     dt_ = case.dtype
     name = "double" if dt_ == np.float64 else "float"
     fn = getattr(lib, f"maxwell_tm_{case.stencil}_{name}_forward_cpu")
@@ -275,12 +281,14 @@ def ref_tm2d_forward(case: TMCase, *, n_threads: int = 0, nt: int | None = None)
     return fields, r
 
 
-def ref_tm2d_forward_with_storage_and_backward(case: TMCase, grad_r: np.ndarray, *, n_threads: int = 0):
+def ref_tm2d_forward_with_storage_and_backward(case: TMCase, grad_r: np.ndarray, *,
+                                                n_threads: int = 0, fast: bool = False):
     """The reference CPU forward_with_storage followed by its CPU backward.
 
     Used only to time the reference CPU path (its CPU backward re-runs the
-    forward kernels on the adjoint fields and is not the exact adjoint)."""
-    lib = ref_lib()
+    forward kernels on the adjoint fields and is not the exact adjoint);
+    ``fast`` runs the Release-flag build."""
+    lib = ref_lib(fast)
     dt_ = case.dtype
     name = "double" if dt_ == np.float64 else "float"
     ct = ctypes.c_double if dt_ == np.float64 else ctypes.c_float

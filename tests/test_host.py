@@ -60,6 +60,93 @@ def test_pml_profile_ranges_and_interior():
     assert torch.all(by[y0 + 1 : y1 - 1] == 0) and torch.all(ky[y0 + 1 : y1 - 1] == 1)
 
 
+def _cpml_closed_form(n, widths, pads, h, dt, half):
+    """The CPML grading of Roden and Gedney with the reference's constants
+    (N = 4, sigma0 = (N+1)/(150 pi h), kappa_max = 5, alpha = alpha_max (1 -
+    d/L) + 0.1 alpha_max, alpha_max = 0.008), evaluated in FP64 at every
+    integer (or half-integer) point of an axis whose strips end at
+    pad_lo + w_lo and n - 1 - pad_hi - w_hi."""
+    import numpy as np
+
+    coord = np.arange(n, dtype=np.float64) + (0.5 if half else 0.0)
+    a = np.zeros(n)
+    b = np.zeros(n)
+    k = np.ones(n)
+    edges = (pads[0] + widths[0], n - 1 - pads[1] - widths[1])
+    sigma0 = 5.0 / (150.0 * math.pi * h)
+    for side in (0, 1):
+        w = widths[side]
+        if w <= 0:
+            continue
+        depth = edges[0] - coord if side == 0 else coord - edges[1]
+        inside = depth >= 0
+        frac = np.clip(depth / w, 0.0, 1.0)
+        sig = sigma0 * frac**4
+        kap = 1.0 + 4.0 * frac**4
+        alpha = 0.008 * (1.0 - frac) + 0.1 * 0.008
+        bs = np.exp(-(sig / kap + alpha) * dt / utils.EP0)
+        as_ = np.where(sig > 1e-6, sig * (bs - 1.0) / (kap * (sig + kap * alpha) + 1e-9), 0.0)
+        k = np.where(inside, kap, k)
+        b = np.where(inside, bs, b)
+        a = np.where(inside, as_, a)
+    return a, b, k
+
+
+@pytest.mark.parametrize("dtype,tol", [(torch.float64, 1e-13), (torch.float32, 2e-6)])
+@pytest.mark.parametrize("widths,size", [([8, 6, 5, 7], 48), ([20, 20, 20, 20], 1065), ([3, 0, 0, 9], 40)])
+def test_pml_profiles_match_the_closed_form_everywhere(dtype, tol, widths, size):
+    """Every profile value (interior of the strips included), integer and
+    half points, against the FP64 closed form of the grading."""
+    import numpy as np
+
+    h = (0.02, 0.025)
+    dt = 0.9 * min(h) / (2**0.5 * utils.C0)
+    prof = staggered.set_pml_profiles(
+        pml_width=widths, accuracy=2, fd_pad=[1, 0, 1, 0], dt=dt, grid_spacing=list(h),
+        max_vel=3e8, dtype=dtype, device=torch.device("cpu"), pml_freq=0.5 / dt,
+        ny=size, nx=size + 3)
+    ay, ayh, ax, axh, by, byh, bx, bxh, ky, kyh, kx, kxh = [p.reshape(-1).double().numpy() for p in prof]
+    for (a, b, k), n, w, hh, half in (((ay, by, ky), size, widths[:2], h[0], False),
+                                      ((ayh, byh, kyh), size, widths[:2], h[0], True),
+                                      ((ax, bx, kx), size + 3, widths[2:], h[1], False),
+                                      ((axh, bxh, kxh), size + 3, widths[2:], h[1], True)):
+        a0, b0, k0 = _cpml_closed_form(n, w, (1, 0), hh, dt, half)
+        scale = lambda r: max(np.abs(r).max(), 1e-30)  # noqa: E731
+        assert np.abs(k - k0).max() <= tol * scale(k0)
+        assert np.abs(b - b0).max() <= tol * scale(b0)
+        assert np.abs(a - a0).max() <= 4 * tol * scale(a0)
+
+
+def test_pml_grading_is_quartic_and_a_b_consistent():
+    """Invariants of the grading, independent of its constants: inside a
+    strip kappa - 1 grows as the fourth power of depth (ratios of depths
+    fix the exponent and the half-point offset), and a, b, kappa satisfy
+    a * kappa * (sigma + kappa alpha) = sigma (b - 1) with sigma and alpha
+    read back from b = exp(-(sigma/kappa + alpha) dt / eps0)."""
+    import numpy as np
+
+    h, w, n = 0.02, 10, 60
+    dt = 2e-11
+    for half in (False, True):
+        setup = utils.setup_pml_half if half else utils.setup_pml
+        a, b, k = (t.numpy() for t in setup([w, w], [float(w), float(n - 1 - w)], w * h, dt, n,
+                                            3e8, torch.float64, torch.device("cpu"), 1.0 / dt,
+                                            grid_spacing=h))
+        coord = np.arange(n) + (0.5 if half else 0.0)
+        depth = w - coord
+        inside = (depth > 0) & (depth <= w)
+        ratio = (k[inside] - 1.0) / (k[inside][0] - 1.0)
+        np.testing.assert_allclose(ratio, (depth[inside] / depth[inside][0]) ** 4, rtol=1e-10, atol=1e-15)
+        # alpha is linear in depth; sigma = sigma0 (d/L)^4
+        frac = depth[inside] / w
+        alpha = 0.008 * (1 - frac) + 0.0008
+        sig_over_k = -np.log(b[inside]) * utils.EP0 / dt - alpha
+        sig = sig_over_k * k[inside]
+        np.testing.assert_allclose(sig, 5.0 / (150 * math.pi * h) * frac**4, rtol=1e-9)
+        lhs = a[inside] * (k[inside] * (sig + k[inside] * alpha) + 1e-9)
+        np.testing.assert_allclose(lhs, sig * (b[inside] - 1.0), rtol=1e-9, atol=1e-15)
+
+
 def test_pml_profile_closed_form_at_outer_edge():
     """At the outermost cell (depth = width) the grading is 1: sigma =
     sigma0 = 5/(150 pi h), kappa = 5, alpha = 0.1 alpha_max,
