@@ -218,21 +218,27 @@ def tune_tm2d(
         return sum(c * _time(b, setup=z) for c, _, b, z in runs)
 
     fwd_before, adj_before = current()
-    for _, f, b, z in runs:
-        z()
-        f()
-        z()
-        b()
-    torch.cuda.synchronize()
     f_times, a_times = {}, {}
-    for cand in fwd_candidates:
-        _install(tuple(cand), adj_before)
-        f_times[tuple(cand)] = run_fwd()
-    best_f = _pick(f_times, fwd_before)
-    for cand in adj_candidates:
-        _install(best_f, tuple(cand))
-        a_times[tuple(cand)] = run_bwd()
-    best_a = _pick(a_times, adj_before)
+    try:
+        for _, f, b, z in runs:
+            z()
+            f()
+            z()
+            b()
+        torch.cuda.synchronize()
+        for cand in fwd_candidates:
+            _install(tuple(cand), adj_before)
+            f_times[tuple(cand)] = run_fwd()
+        best_f = _pick(f_times, fwd_before)
+        for cand in adj_candidates:
+            _install(best_f, tuple(cand))
+            a_times[tuple(cand)] = run_bwd()
+        best_a = _pick(a_times, adj_before)
+    except BaseException:
+        # a candidate that fails (unsupported shape, out of memory) must not
+        # leave the process-wide launch shape at that candidate
+        _install(fwd_before, adj_before)
+        raise
     _install(best_f, best_a)
     _cache[key] = (best_f, best_a)
 

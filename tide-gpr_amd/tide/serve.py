@@ -256,7 +256,16 @@ def model_gradient(
         w = (gf.double() * amp).sum(0) * (-1.0 / (dx * dy))  # [shots, n_src]
         idx = (st.sources_i if source_index is None else source_index).to(gf.device).reshape(-1)
         keep = idx >= 0  # skipped sources add zero at index 0 (no data-dependent shapes)
-        g_cb.view(-1).index_add_(0, torch.where(keep, idx, 0), torch.where(keep, w.reshape(-1), 0.0))
+        idx = torch.where(keep, idx, 0)
+        w = torch.where(keep, w.reshape(-1), 0.0)
+        # Sources that share a cell: every entry of a cell gets the cell's
+        # total (a fixed-order product with the same-cell matrix) and writes
+        # the same new value, so the scatter needs no atomics and gives the
+        # same bits on every run.
+        same = (idx[:, None] == idx[None, :]).double()
+        total = same @ w
+        flat = g_cb.view(-1)
+        flat.index_put_((idx,), flat[idx] + total, accumulate=False)
     dca_de, dcb_de, dca_ds, dcb_ds = coefficient_partials(ca, cb, st.dt)
     g_eps_p = g_ca * dca_de + g_cb * dcb_de
     g_sig_p = g_ca * dca_ds + g_cb * dcb_ds
