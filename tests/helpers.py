@@ -61,7 +61,12 @@ def hetero_problem(*, ny=40, nx=52, nt=160, shots=3, seed=1234, dtype=torch.floa
     dt = 0.9 * dx / (2**0.5 * tide.C0)
     amp = tide.ricker(6e8, nt, dt, dtype=dtype, device=device).reshape(1, 1, -1).repeat(shots, 2, 1)
     amp[:, 1] *= 0.5
-    src = torch.stack([torch.tensor([[3, 5 + 7 * s], [ny - 4, 9 + 5 * s]]) for s in range(shots)])
+    def col(x0, step, s, margin):  # wraps only positions that would leave the grid
+        x = x0 + step * s
+        return x if x < nx - 2 else x0 + (step * s) % (nx - margin)
+
+    src = torch.stack([torch.tensor([[3, col(5, 7, s, 8)], [ny - 4, col(9, 5, s, 12)]])
+                       for s in range(shots)])
     rec = torch.stack([torch.tensor([[2, x] for x in range(1, nx - 1, 5)]) for _ in range(shots)])
     return dict(epsilon=eps, sigma=sig, mu=torch.ones_like(eps), grid_spacing=dx, dt=dt,
                 source_amplitude=amp, source_location=src, receiver_location=rec,
