@@ -13,9 +13,12 @@
  *  - the return value is 0 on success or a nonzero status; the message of
  *    the last failure on the calling thread is returned by tide_last_error().
  *    (The reference functions return void and exit the process on error.)
- *  - storage_mode: 0 device, 3 none (1 cpu and 2 disk return status 2);
+ *  - storage_mode: 0 device (the whole history in *_store_1), 1 cpu (a
+ *    three-step device ring in *_store_1 staged through the pinned host
+ *    history *_store_3), 2 disk (the ring staged through a three-step
+ *    pinned host ring *_store_3 to the file filenames[0]), 3 none;
  *    storage_format: 0 full precision, 1 BF16 (round-to-nearest-even).
- * Only stencil order 2 is built; {dtype} is float or double.
+ *  - stencil orders 2, 4, 6 and 8 are built; {dtype} is float or double.
  */
 #ifndef TIDE_HIP_H
 #define TIDE_HIP_H
@@ -76,8 +79,8 @@ int maxwell_tm_2_double_forward_cuda(
  * (t / step_ratio) * shot_bytes_uncomp * n_shots.
  * Replaces maxwell_tm_2_{dtype}_forward_with_storage_cuda: reference ABI
  * src/tide/backend_utils.py:271-287, CUDA implementation
- * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1371-1609.  ey_store_3,
- * curl_store_3 and the filename arrays (host staging) are unused.
+ * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1371-1609 (host staging
+ * :1407-1571; storage helpers src/tide/csrc/storage/storage_utils.cu).
  */
 int maxwell_tm_2_float_forward_with_storage_cuda(
     float const *ca, float const *cb, float const *cq, float const *f,
@@ -121,7 +124,8 @@ int maxwell_tm_2_double_forward_with_storage_cuda(
  * Exact adjoint over nt steps ending at start_t: injects grad_r, writes
  * grad_f, and adds dJ/dca, dJ/dcb (accumulated in FP64, summed over shots
  * unless batched) into grad_ca / grad_cb.  grad_*_shot and
- * adjoint_memory_scratch are not needed (pass NULL).
+ * adjoint_memory_scratch are not needed (pass NULL).  Host-staged modes
+ * read the snapshots back through the same buffers the forward used.
  * Replaces maxwell_tm_2_{dtype}_backward_cuda: reference ABI
  * src/tide/backend_utils.py:289-308, CUDA implementation
  * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:2405-2783.
@@ -169,46 +173,534 @@ int maxwell_tm_2_double_backward_cuda(
     void *storage_stream);
 
 /*
+ * Stencil orders 4, 6 and 8: the same arguments and meaning as the order-2
+ * entry points above (fd_pad = [S/2, S/2-1, S/2, S/2-1] and the profiles of
+ * that order).  Replace maxwell_tm_{4,6,8}_{dtype}_*_cuda: reference
+ * instantiations src/tide/csrc/tm2d/maxwell_tm_cuda_instantiations.inc and
+ * names src/tide/backend_utils.py:771-781; staggered differences
+ * src/tide/csrc/tm2d/maxwell_tm_core.cuh:61-365.
+ */
+int maxwell_tm_4_float_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float const *debye_a, float const *debye_b,
+    float const *debye_cp, float *polarization, float *ey_prev, float *r,
+    int64_t n_poles, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+int maxwell_tm_4_float_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float *r, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_4_float_backward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *grad_r,
+    float *lambda_ey, float *lambda_hx, float *lambda_hz, float *m_lambda_ey_x,
+    float *m_lambda_ey_z, float *m_lambda_hx_z, float *m_lambda_hz_x,
+    float *adjoint_memory_scratch, void *ey_store_1, void *ey_store_3,
+    char const *const *ey_filenames, void *curl_store_1, void *curl_store_3,
+    char const *const *curl_filenames, float *grad_f, float *grad_ca,
+    float *grad_cb, float *grad_ca_shot, float *grad_cb_shot, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_4_double_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double const *debye_a,
+    double const *debye_b, double const *debye_cp, double *polarization,
+    double *ey_prev, double *r, int64_t n_poles, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax,
+    double const *bx, double const *axh, double const *bxh, double const *ky,
+    double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+int maxwell_tm_4_double_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double *r, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax,
+    double const *bx, double const *axh, double const *bxh, double const *ky,
+    double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_4_double_backward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *grad_r,
+    double *lambda_ey, double *lambda_hx, double *lambda_hz,
+    double *m_lambda_ey_x, double *m_lambda_ey_z, double *m_lambda_hx_z,
+    double *m_lambda_hz_x, double *adjoint_memory_scratch, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double *grad_f,
+    double *grad_ca, double *grad_cb, double *grad_ca_shot,
+    double *grad_cb_shot, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh,
+    double const *kx, double const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, double rdy, double rdx, double dt, int64_t nt,
+    int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_6_float_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float const *debye_a, float const *debye_b,
+    float const *debye_cp, float *polarization, float *ey_prev, float *r,
+    int64_t n_poles, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+int maxwell_tm_6_float_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float *r, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_6_float_backward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *grad_r,
+    float *lambda_ey, float *lambda_hx, float *lambda_hz, float *m_lambda_ey_x,
+    float *m_lambda_ey_z, float *m_lambda_hx_z, float *m_lambda_hz_x,
+    float *adjoint_memory_scratch, void *ey_store_1, void *ey_store_3,
+    char const *const *ey_filenames, void *curl_store_1, void *curl_store_3,
+    char const *const *curl_filenames, float *grad_f, float *grad_ca,
+    float *grad_cb, float *grad_ca_shot, float *grad_cb_shot, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_6_double_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double const *debye_a,
+    double const *debye_b, double const *debye_cp, double *polarization,
+    double *ey_prev, double *r, int64_t n_poles, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax,
+    double const *bx, double const *axh, double const *bxh, double const *ky,
+    double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+int maxwell_tm_6_double_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double *r, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax,
+    double const *bx, double const *axh, double const *bxh, double const *ky,
+    double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_6_double_backward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *grad_r,
+    double *lambda_ey, double *lambda_hx, double *lambda_hz,
+    double *m_lambda_ey_x, double *m_lambda_ey_z, double *m_lambda_hx_z,
+    double *m_lambda_hz_x, double *adjoint_memory_scratch, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double *grad_f,
+    double *grad_ca, double *grad_cb, double *grad_ca_shot,
+    double *grad_cb_shot, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh,
+    double const *kx, double const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, double rdy, double rdx, double dt, int64_t nt,
+    int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_8_float_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float const *debye_a, float const *debye_b,
+    float const *debye_cp, float *polarization, float *ey_prev, float *r,
+    int64_t n_poles, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+int maxwell_tm_8_float_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float *r, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_8_float_backward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *grad_r,
+    float *lambda_ey, float *lambda_hx, float *lambda_hz, float *m_lambda_ey_x,
+    float *m_lambda_ey_z, float *m_lambda_hx_z, float *m_lambda_hz_x,
+    float *adjoint_memory_scratch, void *ey_store_1, void *ey_store_3,
+    char const *const *ey_filenames, void *curl_store_1, void *curl_store_3,
+    char const *const *curl_filenames, float *grad_f, float *grad_ca,
+    float *grad_cb, float *grad_ca_shot, float *grad_cb_shot, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_8_double_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double const *debye_a,
+    double const *debye_b, double const *debye_cp, double *polarization,
+    double *ey_prev, double *r, int64_t n_poles, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax,
+    double const *bx, double const *axh, double const *bxh, double const *ky,
+    double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+int maxwell_tm_8_double_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *f,
+    double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double *r, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax,
+    double const *bx, double const *axh, double const *bxh, double const *ky,
+    double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
+    int64_t shot_bytes_uncomp, bool ca_requires_grad, bool cb_requires_grad,
+    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_8_double_backward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *grad_r,
+    double *lambda_ey, double *lambda_hx, double *lambda_hz,
+    double *m_lambda_ey_x, double *m_lambda_ey_z, double *m_lambda_hx_z,
+    double *m_lambda_hz_x, double *adjoint_memory_scratch, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double *grad_f,
+    double *grad_ca, double *grad_cb, double *grad_ca_shot,
+    double *grad_cb_shot, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh,
+    double const *kx, double const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, double rdy, double rdx, double dt, int64_t nt,
+    int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+/*
+ * Born (tangent-linear) forward: the background fields (ey .. m_hz_x) and
+ * the scattered fields (dey .. dm_hz_x) advance together over nt steps from
+ * start_t; the scattered E update adds dca * Ey^n + dcb * curl H^n of the
+ * background, df is injected into the scattered field and f0 into the
+ * background.  r receives the scattered receivers, background_r the
+ * background's.  dca / dcb NULL: no coefficient perturbation; step_ratio
+ * must be 1.  Replaces maxwell_tm_{2,4,6,8}_{dtype}_born_forward_cuda:
+ * reference ABI src/tide/backend_utils.py:336-349 (_TM_BORN_FORWARD_SPEC),
+ * CUDA implementation src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1611-1800.
+ */
+int maxwell_tm_2_float_born_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float *ey, float *hx, float *hz, float *m_ey_x,
+    float *m_ey_z, float *m_hx_z, float *m_hz_x, float *dey, float *dhx, float *dhz,
+    float *dm_ey_x, float *dm_ey_z, float *dm_hx_z, float *dm_hz_x, float *r,
+    float *background_r, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
+
+int maxwell_tm_2_double_born_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double *ey, double *hx, double *hz, double *m_ey_x,
+    double *m_ey_z, double *m_hx_z, double *m_hz_x, double *dey, double *dhx, double *dhz,
+    double *dm_ey_x, double *dm_ey_z, double *dm_hx_z, double *dm_hz_x, double *r,
+    double *background_r, double const *ay, double const *by, double const *ayh,
+    double const *byh, double const *ax, double const *bx, double const *axh,
+    double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    double rdy, double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
+
+int maxwell_tm_4_float_born_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float *ey, float *hx, float *hz, float *m_ey_x,
+    float *m_ey_z, float *m_hx_z, float *m_hz_x, float *dey, float *dhx, float *dhz,
+    float *dm_ey_x, float *dm_ey_z, float *dm_hx_z, float *dm_hz_x, float *r,
+    float *background_r, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
+
+int maxwell_tm_4_double_born_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double *ey, double *hx, double *hz, double *m_ey_x,
+    double *m_ey_z, double *m_hx_z, double *m_hz_x, double *dey, double *dhx, double *dhz,
+    double *dm_ey_x, double *dm_ey_z, double *dm_hx_z, double *dm_hz_x, double *r,
+    double *background_r, double const *ay, double const *by, double const *ayh,
+    double const *byh, double const *ax, double const *bx, double const *axh,
+    double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    double rdy, double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
+
+int maxwell_tm_6_float_born_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float *ey, float *hx, float *hz, float *m_ey_x,
+    float *m_ey_z, float *m_hx_z, float *m_hz_x, float *dey, float *dhx, float *dhz,
+    float *dm_ey_x, float *dm_ey_z, float *dm_hx_z, float *dm_hz_x, float *r,
+    float *background_r, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
+
+int maxwell_tm_6_double_born_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double *ey, double *hx, double *hz, double *m_ey_x,
+    double *m_ey_z, double *m_hx_z, double *m_hz_x, double *dey, double *dhx, double *dhz,
+    double *dm_ey_x, double *dm_ey_z, double *dm_hx_z, double *dm_hz_x, double *r,
+    double *background_r, double const *ay, double const *by, double const *ayh,
+    double const *byh, double const *ax, double const *bx, double const *axh,
+    double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    double rdy, double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
+
+int maxwell_tm_8_float_born_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float *ey, float *hx, float *hz, float *m_ey_x,
+    float *m_ey_z, float *m_hx_z, float *m_hz_x, float *dey, float *dhx, float *dhz,
+    float *dm_ey_x, float *dm_ey_z, float *dm_hx_z, float *dm_hz_x, float *r,
+    float *background_r, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
+
+int maxwell_tm_8_double_born_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double *ey, double *hx, double *hz, double *m_ey_x,
+    double *m_ey_z, double *m_hx_z, double *m_hz_x, double *dey, double *dhx, double *dhz,
+    double *dm_ey_x, double *dm_ey_z, double *dm_hx_z, double *dm_hz_x, double *r,
+    double *background_r, double const *ay, double const *by, double const *ayh,
+    double const *byh, double const *ax, double const *bx, double const *axh,
+    double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    double rdy, double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
+
+/*
  * Same adjoint as maxwell_tm_2_{dtype}_backward_cuda, with grad_ca / grad_cb
  * as FP64 accumulators that are added to (so a caller running shot chunks
- * keeps one FP64 sum across them).  Storage arguments are the device
- * snapshot buffers written by forward_with_storage (NULL when that
- * coefficient's gradient is not wanted).  No reference counterpart; the
- * reference keeps fp32 per-shot accumulators
+ * keeps one FP64 sum across them).  The six storage arguments and
+ * storage_mode have the reference backward's meaning: mode 0 reads the
+ * device history in *_store_1; modes 1 (cpu) and 2 (disk) read a
+ * three-step device ring in *_store_1, filled from the pinned host history
+ * (cpu) or from the file filenames[0] through a three-step pinned host ring
+ * in *_store_3 (disk), on storage_stream.  stencil: 2, 4, 6 or 8.  NULL
+ * grad_ca / grad_cb: that
+ * coefficient's gradient is not wanted (its storage is not read).  No
+ * reference counterpart; the reference keeps fp32 per-shot accumulators
  * (src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:834-871).
  */
 int tide_tm2d_float_backward_f64acc(
     float const *ca, float const *cb, float const *cq, float const *grad_r,
     float *lambda_ey, float *lambda_hx, float *lambda_hz, float *m_lambda_ey_x,
     float *m_lambda_ey_z, float *m_lambda_hx_z, float *m_lambda_hz_x,
-    void const *ey_store, void const *curl_store, float *grad_f,
-    double *grad_ca, double *grad_cb, float const *ay, float const *by,
-    float const *ayh, float const *byh, float const *ax, float const *bx,
-    float const *axh, float const *bxh, float const *ky, float const *kyh,
-    float const *kx, float const *kxh, int64_t const *sources_i,
-    int64_t const *receivers_i, float rdy, float rdx, int64_t nt,
-    int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
-    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_format,
+    void *ey_store_1, void *ey_store_3, char const *const *ey_filenames,
+    void *curl_store_1, void *curl_store_3, char const *const *curl_filenames,
+    float *grad_f, double *grad_ca, double *grad_cb, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax,
+    float const *bx, float const *axh, float const *bxh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy, float rdx,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, int64_t storage_mode, int64_t storage_format,
     int64_t shot_bytes_uncomp, bool ca_batched, bool cb_batched,
     bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
-    int64_t pml_y1, int64_t pml_x1, int64_t device, void *compute_stream);
+    int64_t pml_y1, int64_t pml_x1, int64_t stencil, int64_t device,
+    void *compute_stream, void *storage_stream);
 
 int tide_tm2d_double_backward_f64acc(
     double const *ca, double const *cb, double const *cq,
     double const *grad_r, double *lambda_ey, double *lambda_hx,
     double *lambda_hz, double *m_lambda_ey_x, double *m_lambda_ey_z,
-    double *m_lambda_hx_z, double *m_lambda_hz_x, void const *ey_store,
-    void const *curl_store, double *grad_f, double *grad_ca, double *grad_cb,
-    double const *ay, double const *by, double const *ayh, double const *byh,
-    double const *ax, double const *bx, double const *axh, double const *bxh,
-    double const *ky, double const *kyh, double const *kx, double const *kxh,
-    int64_t const *sources_i, int64_t const *receivers_i, double rdy,
-    double rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
-    int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
-    int64_t step_ratio, int64_t storage_format, int64_t shot_bytes_uncomp,
-    bool ca_batched, bool cb_batched, bool cq_batched, int64_t start_t,
-    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
-    int64_t device, void *compute_stream);
+    double *m_lambda_hx_z, double *m_lambda_hz_x, void *ey_store_1,
+    void *ey_store_3, char const *const *ey_filenames, void *curl_store_1,
+    void *curl_store_3, char const *const *curl_filenames, double *grad_f,
+    double *grad_ca, double *grad_cb, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh,
+    double const *kx, double const *kxh, int64_t const *sources_i,
+    int64_t const *receivers_i, double rdy, double rdx, int64_t nt,
+    int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_batched,
+    bool cb_batched, bool cq_batched, int64_t start_t, int64_t pml_y0,
+    int64_t pml_x0, int64_t pml_y1, int64_t pml_x1, int64_t stencil,
+    int64_t device, void *compute_stream, void *storage_stream);
 
 /*
  * Launch shape of the TM2D step kernels: rows marched per wavefront
@@ -235,11 +727,12 @@ int tide_tm2d_set_shot_loop(int fwd_loop, int adj_loop);
 int tide_tm2d_get_shot_loop(int *out2);
 
 /*
- * Time steps per launch of the single-problem TM2D path, per pass (1 or
- * 2).  With 2, pairs of steps run in one launch of the two-step kernels
- * (60 owned columns per wavefront, a recomputed halo of two), which read
- * and write the wavefield state once per two steps; results are bitwise
- * those of single steps.  An odd remainder runs as a single step.  No
+ * Time steps per launch of the single-problem TM2D path, per pass (1 to
+ * 4).  With K >= 2, K steps run in one launch of the K-step kernels
+ * (64 - 2K owned columns per wavefront, a recomputed halo of K), which read
+ * and write the wavefield state once per K steps; results are bitwise
+ * those of single steps.  A remainder runs as two-step and single-step
+ * launches, and host-staged storage (modes 1, 2) caps K at 2.  No
  * reference counterpart (one launch per kernel and step,
  * src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1371-1609).
  */

@@ -130,7 +130,7 @@ def case_from_setup(setup) -> TMCase:
         src=n(setup.sources_i).reshape(setup.n_shots, setup.n_src).astype(np.int64),
         rec=n(setup.receivers_i).reshape(setup.n_shots, setup.n_rec).astype(np.int64),
         rdy=setup.rdy, rdx=setup.rdx, dt=setup.dt, nt=setup.nt, pml=tuple(setup.pml),
-        fields=[n(t) for t in setup.fields],
+        fields=[n(t) for t in setup.fields], stencil=int(getattr(setup, "stencil", 2)),
     )
 
 

@@ -45,7 +45,7 @@ def gpu_forward(case: TMCase, *, store: str | None = None, interval: int = 1,
     P = _native.ptr
     es = cs = None
     if store is None:
-        fn = (_native.get_function("maxwell_tm", "forward", 2, dtype) if entry is None
+        fn = (_native.get_function("maxwell_tm", "forward", case.stencil, dtype) if entry is None
               else getattr(_native.load(), entry))
         st = fn(P(ca), P(cb), P(cq), P(f), *[P(x) for x in flds],
                 None, None, None, None, None, P(r), 0, *[P(p) for p in prof],
@@ -59,7 +59,7 @@ def gpu_forward(case: TMCase, *, store: str | None = None, interval: int = 1,
         es = torch.zeros(n_store, N, ny, nx, device=dev, dtype=sdt)
         cs = torch.zeros_like(es)
         shot_bytes = ny * nx * (2 if fmt == 1 else case.dtype.itemsize)
-        fn = _native.get_function("maxwell_tm", "forward_with_storage", 2, dtype)
+        fn = _native.get_function("maxwell_tm", "forward_with_storage", case.stencil, dtype)
         st = fn(P(ca), P(cb), P(cq), P(f), *[P(x) for x in flds], P(r),
                 P(es), None, None, P(cs), None, None, *[P(p) for p in prof],
                 P(src), P(rec), case.rdy, case.rdx, case.dt, nt, N, ny, nx, n_src, n_rec,
@@ -106,7 +106,7 @@ def gpu_backward(case: TMCase, grad_r: np.ndarray, ey_store: np.ndarray | None,
     if reference_abi:
         gca = torch.zeros(gshape, device=dev, dtype=dtype)
         gcb = torch.zeros(gshape, device=dev, dtype=dtype)
-        fn = _native.get_function("maxwell_tm", "backward", 2, dtype)
+        fn = _native.get_function("maxwell_tm", "backward", case.stencil, dtype)
         st = fn(P(ca), P(cb), P(cq), P(gr), *[P(x) for x in lam], None,
                 P(es), None, None, P(cs), None, None, P(gf), P(gca), P(gcb), None, None,
                 *[P(p) for p in prof], P(src), P(rec), case.rdy, case.rdx, case.dt,
@@ -117,11 +117,12 @@ def gpu_backward(case: TMCase, grad_r: np.ndarray, ey_store: np.ndarray | None,
         gca = torch.zeros(gshape, device=dev, dtype=torch.float64)
         gcb = torch.zeros(gshape, device=dev, dtype=torch.float64)
         fn = getattr(_native.load(), f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
-        st = fn(P(ca), P(cb), P(cq), P(gr), *[P(x) for x in lam], P(es), P(cs), P(gf),
+        st = fn(P(ca), P(cb), P(cq), P(gr), *[P(x) for x in lam], P(es), None, None,
+                P(cs), None, None, P(gf),
                 P(gca) if es is not None else None, P(gcb) if cs is not None else None,
                 *[P(p) for p in prof], P(src), P(rec), case.rdy, case.rdx, nt, N, ny, nx,
-                n_src, n_rec, interval, fmt, shot_bytes, batched, batched, batched, nt,
-                *case.pml, dev.index or 0, stream)
+                n_src, n_rec, interval, STORAGE_DEVICE, fmt, shot_bytes, batched, batched,
+                batched, nt, *case.pml, case.stencil, dev.index or 0, stream, stream)
     _native.check(st)
     torch.cuda.synchronize(dev)
     return gca.cpu().numpy(), gcb.cpu().numpy(), gf.cpu().numpy(), [x.cpu().numpy() for x in lam]

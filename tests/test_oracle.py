@@ -155,3 +155,63 @@ def test_bf16_snapshot_gradient_close_to_fp32():
     g16 = O.tm2d_backward(case, w, eb, cb, store="bf16")
     assert relative_l2(g16[0], g32[0]) < 1e-2
     assert relative_l2(g16[1], g32[1]) < 1e-2
+
+
+# ------------------------------------------------------- stencils 4, 6, 8
+def _stencil_case(stencil, dtype, **kw):
+    p = hetero_problem(seed=40 + stencil, dtype=dtype, **kw)
+    return O.case_from_setup(prepare_tm2d(**p, stencil=stencil))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference CPU library not built here")
+@pytest.mark.parametrize("stencil", [4, 6, 8])
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_oracle_high_order_forward_matches_reference_library(stencil, dtype):
+    """The reference's own compiled CPU forward for each order
+    (maxwell_tm_{4,6,8}_*_forward_cpu) on a heterogeneous case with CPML on
+    every side: fields and receivers bit-identical."""
+    case = _stencil_case(stencil, dtype, ny=34, nx=41, nt=70)
+    assert case.stencil == stencil
+    f_o, r_o, _, _ = O.tm2d_forward(case)
+    f_r, r_r = O.ref_tm2d_forward(case)
+    assert np.any(r_o)
+    assert np.array_equal(r_o, r_r)
+    for a, b in zip(f_o, f_r):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("stencil", [4, 6, 8])
+def test_high_order_adjoint_dot_product_identity(stencil):
+    """<J dm, w> = <dm, J^T w> per term (ca, cb, source) in FP64."""
+    case = _stencil_case(stencil, torch.float64, nt=90)
+    rng = np.random.default_rng(stencil)
+    dca = rng.standard_normal((case.ny, case.nx)) * 1e-3
+    dcb = rng.standard_normal((case.ny, case.nx)) * 1e-3 * float(np.abs(case.cb).mean())
+    df = rng.standard_normal(case.f.shape) * float(np.abs(case.f).max()) * 1e-2
+    _, _, es, cs = O.tm2d_forward(case, store="full")
+    w = rng.standard_normal((case.nt, case.n_shots, case.rec.shape[1]))
+    gca, gcb, gf, _ = O.tm2d_backward(case, w, es, cs)
+    zero_a, zero_b, zero_f = np.zeros_like(dca), np.zeros_like(dcb), np.zeros_like(df)
+    for args, rhs in (
+        ((dca, zero_b, zero_f), (gca * dca).sum()),
+        ((zero_a, dcb, zero_f), (gcb * dcb).sum()),
+        ((zero_a, zero_b, df), (gf * df).sum()),
+    ):
+        _, dr = O.tm2d_tangent(case, *args)
+        lhs = (dr * w).sum()
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("stencil", [4, 8])
+def test_high_order_tangent_matches_central_differences(stencil):
+    case = _stencil_case(stencil, torch.float64, nt=70)
+    rng = np.random.default_rng(10 + stencil)
+    dcb = rng.standard_normal((case.ny, case.nx)) * 1e-3 * float(np.abs(case.cb).mean())
+    zero = np.zeros_like(dcb)
+    _, dr = O.tm2d_tangent(case, zero, dcb, np.zeros_like(case.f))
+    h = 1e-4
+    plus = O.TMCase(**{**case.__dict__, "cb": case.cb + h * dcb[None]})
+    minus = O.TMCase(**{**case.__dict__, "cb": case.cb - h * dcb[None]})
+    _, rp, _, _ = O.tm2d_forward(plus)
+    _, rm, _, _ = O.tm2d_forward(minus)
+    assert relative_l2((rp - rm) / (2 * h), dr) < 1e-6

@@ -20,8 +20,9 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 OUT = HERE / "tide" / "_lib" / "libtide_hip.so"
-SOURCES = ["tide_common.hip", "tm2d.hip", "tm2d_launch_adj_f32.hip", "tm2d_launch_adj_f64.hip",
-           "tm2d_launch_fwd_f32.hip", "tm2d_launch_fwd_f64.hip"]
+SOURCES = ["tide_common.hip", "tide_storage.hip", "tm2d.hip", "tm2d_highorder.hip", "tm2d_launch_adj_f32.hip", "tm2d_launch_adj_f64.hip",
+           "tm2d_launch_fwd_f32.hip", "tm2d_launch_fwd_f64.hip", "tm2d_launch_k_fwd_f32.hip",
+           "tm2d_launch_k_fwd_f64.hip", "tm2d_launch_k_adj_f32.hip", "tm2d_launch_k_adj_f64.hip"]
 OPTIONAL_SOURCES = ["em3d.hip", "tm2d_batch.hip", "tm2d_serve.hip"]
 ARCH = os.environ.get("TIDE_HIP_ARCH", "gfx950")
 

@@ -19,6 +19,7 @@ enum Status : int {
   kUnsupported = 2,
   kHipError = 3,
   kOutOfMemory = 4,
+  kIoError = 5,
 };
 
 void set_error(const std::string &msg);

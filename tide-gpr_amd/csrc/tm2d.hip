@@ -29,11 +29,15 @@
 #include "tm2d_device.h"
 #include "tm2d_fused.h"
 #include "tm2d_launch.h"
+#include "tm2d_launch_k.h"
 #include "tide_hip.h"
+#include "tide_storage.h"
+#include "tm2d_highorder.h"
 
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 namespace tide_hip {
 namespace tm2d {
@@ -191,6 +195,21 @@ inline int put_table(const PointTable &t, DevicePointTable &d, Scratch *arena_sc
   return d.place(t, rs, sg, en, stream);
 }
 
+// The table of the K-step kernels (64-2K column tiles, lane offset K, halo
+// copies) in `dptk`.
+template <typename T>
+int upload_points_k(const Problem<T> &p, const int64_t *src, const int64_t *rec, int k,
+                    DevicePointTable &dptk, hipStream_t stream, Scratch *arena_scr = nullptr) {
+  std::vector<int64_t> hsrc, hrec;
+  TIDE_CHECK(fetch_indices(src, (int64_t)p.n_shots * p.n_src, hsrc, stream));
+  TIDE_CHECK(fetch_indices(rec, (int64_t)p.n_shots * p.n_rec, hrec, stream));
+  PointTable table;
+  if (!build_point_table_halo(hsrc.data(), p.n_src, hrec.data(), p.n_rec, p.n_shots, p.g.ny,
+                              p.g.nx, 64 - 2 * k, k, table))
+    return kInvalidArgument;
+  return put_table(table, dptk, arena_scr, stream);
+}
+
 template <typename T>
 int upload_points(const Problem<T> &p, const int64_t *src, const int64_t *rec,
                   DevicePointTable &dpt, hipStream_t stream, DevicePointTable *dpt2 = nullptr,
@@ -216,7 +235,8 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
                  T *m_hz_x, const T *f, T *r, const int64_t *src, const int64_t *rec,
                  int64_t nt, int64_t start_t, int64_t step_ratio, void *ey_store,
                  void *curl_store, int fmt, size_t store_step_bytes, hipStream_t stream,
-                 char *arena = nullptr, size_t arena_cap = 0) {
+                 char *arena = nullptr, size_t arena_cap = 0,
+                 SnapshotStage *ey_stage = nullptr, SnapshotStage *curl_stage = nullptr) {
   TIDE_CHECK(check_geometry(p.n_shots, p.g.ny, p.g.nx));
   if (nt <= 0) return kOk;
   if (step_ratio <= 0) {
@@ -228,10 +248,16 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     return kUnsupported;
   }
   const int64_t n_shots = p.n_shots, total = n_shots * p.g.numel;
-  const bool fused = g_tuning.fwd_steps == 2 && nt >= 2;
+  const bool staged = (ey_stage && ey_stage->active()) || (curl_stage && curl_stage->active());
+  // K-step launches (K = 3, 4) need K ring slots at once, more than a
+  // host-staged ring holds: staged calls use two-step launches.
+  const int kblk = (g_tuning.fwd_steps >= 3 && nt >= g_tuning.fwd_steps && !staged)
+                       ? g_tuning.fwd_steps : 0;
+  const bool fused = g_tuning.fwd_steps >= 2 && nt >= 2;
   Scratch scr(stream, arena, arena_cap);
-  DevicePointTable dpt, dpt2;
+  DevicePointTable dpt, dpt2, dptk;
   TIDE_CHECK(upload_points(p, src, rec, dpt, stream, fused ? &dpt2 : nullptr, &scr));
+  if (kblk) TIDE_CHECK(upload_points_k(p, src, rec, kblk, dptk, stream, &scr));
   TIDE_CHECK(pack_profiles(p, scr, stream));
   T *ey2, *hx2, *hz2, *mex2, *mez2, *mhzx2 = nullptr, *mhxz2 = nullptr;
   TIDE_CHECK(scr.alloc(&ey2, (size_t)total));
@@ -275,9 +301,28 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
   T *cur_e[2] = {m_hz_x, m_hx_z};  // E-phase memories (swapped by two-step launches)
   T *nxt_e[2] = {mhzx2, mhxz2};
   const bool storing = ey_store != nullptr || curl_store != nullptr;
-  auto store_at = [&](int64_t t, void *buf) -> void * {
+  // Device address of the snapshot of step t: the history buffer, or the
+  // ring slot of a host-staged stream.
+  auto store_at = [&](int64_t t, void *buf, SnapshotStage *stage = nullptr) -> void * {
     if (buf == nullptr || !storing || (t % step_ratio) != 0) return nullptr;
+    if (stage != nullptr && stage->active()) return stage->slot(t / step_ratio);
     return (char *)buf + (size_t)(t / step_ratio) * store_step_bytes;
+  };
+  // Host-staged streams: a ring slot is written only after its previous
+  // content has been copied out, and copied out after the launch.
+  SnapshotStage *stages[2] = {ey_stage, curl_stage};
+  void *bufs[2] = {ey_store, curl_store};
+  auto stage_steps = [&](int64_t t0, int n, bool after) -> int {
+    for (int k = 0; k < 2; ++k) {
+      if (stages[k] == nullptr || !stages[k]->active() || bufs[k] == nullptr) continue;
+      for (int64_t tt = t0; tt < t0 + n; ++tt) {
+        if ((tt % step_ratio) != 0) continue;
+        const int64_t idx = tt / step_ratio;
+        const int rc = after ? stages[k]->after_write(idx) : stages[k]->before_write(idx);
+        if (rc != kOk) return rc;
+      }
+    }
+    return kOk;
   };
   Fwd2Args<T> a2;
   const bool split2 = g_tuning.fwd_split != 0;
@@ -299,6 +344,45 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
   int64_t t = start_t;
   LoopTimer timer(0, stream);
   int64_t launches = 0;
+  if (kblk) {
+    FwdKArgs<T> ak;
+    std::memset(&ak, 0, sizeof(ak));
+    ak.g = p.g;
+    ak.pts = Points{dptk.row_slot, dptk.seg, dptk.ent, dptk.n_xt};
+    ak.ca = p.ca;
+    ak.cb = p.cb;
+    ak.cq = p.cq;
+    ak.n_shots = p.n_shots;
+    const int rowsk = rows2 >= 8 ? 8 : (rows2 >= 4 ? 4 : 2);
+    ak.n_yt = (p.g.ny + rowsk - 1) / rowsk;
+    ak.split = make_split_k(p.g, dptk.n_xt, ak.n_yt, rowsk, kblk);
+    for (; t + kblk - 1 < start_t + nt; t += kblk, ++launches) {
+      ak.ey_i = cur[0]; ak.hx_i = cur[1]; ak.hz_i = cur[2];
+      ak.mex_i = cur[3]; ak.mez_i = cur[4];
+      ak.mhzx_i = cur_e[0]; ak.mhxz_i = cur_e[1];
+      ak.ey_o = nxt[0]; ak.hx_o = nxt[1]; ak.hz_o = nxt[2];
+      ak.mex_o = nxt[3]; ak.mez_o = nxt[4];
+      ak.mhzx_o = nxt_e[0]; ak.mhxz_o = nxt_e[1];
+      bool st = false;
+      for (int j = 0; j < kblk; ++j) {
+        ak.f[j] = p.n_src > 0 ? f + (t + j) * n_shots * p.n_src : nullptr;
+        ak.r[j] = p.n_rec > 0 ? r + (t + j) * n_shots * p.n_rec : nullptr;
+        ak.ey_store[j] = store_at(t + j, ey_store);
+        ak.curl_store[j] = store_at(t + j, curl_store);
+        st = st || ak.ey_store[j] || ak.curl_store[j];
+      }
+      if (!st) {
+        launch_fwdk<T, 0>(kblk, rowsk, g_tuning.fwd_border_rows, groups2, block2, stream, ak);
+      } else if (fmt == 2) {
+        if constexpr (sizeof(T) == 4)
+          launch_fwdk<T, 2>(kblk, rowsk, g_tuning.fwd_border_rows, groups2, block2, stream, ak);
+      } else {
+        launch_fwdk<T, 1>(kblk, rowsk, g_tuning.fwd_border_rows, groups2, block2, stream, ak);
+      }
+      for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
+      for (int k = 0; k < 2; ++k) std::swap(cur_e[k], nxt_e[k]);
+    }
+  }
   for (; fused && t + 1 < start_t + nt; t += 2, ++launches) {
     a2.ey_i = cur[0]; a2.hx_i = cur[1]; a2.hz_i = cur[2];
     a2.mex_i = cur[3]; a2.mez_i = cur[4];
@@ -310,11 +394,12 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     a2.f1 = p.n_src > 0 ? f + (t + 1) * n_shots * p.n_src : nullptr;
     a2.r0 = p.n_rec > 0 ? r + t * n_shots * p.n_rec : nullptr;
     a2.r1 = p.n_rec > 0 ? r + (t + 1) * n_shots * p.n_rec : nullptr;
-    a2.ey_store0 = store_at(t, ey_store);
-    a2.curl_store0 = store_at(t, curl_store);
-    a2.ey_store1 = store_at(t + 1, ey_store);
-    a2.curl_store1 = store_at(t + 1, curl_store);
+    a2.ey_store0 = store_at(t, ey_store, ey_stage);
+    a2.curl_store0 = store_at(t, curl_store, curl_stage);
+    a2.ey_store1 = store_at(t + 1, ey_store, ey_stage);
+    a2.curl_store1 = store_at(t + 1, curl_store, curl_stage);
     const bool st = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
+    if (st) TIDE_CHECK(stage_steps(t, 2, false));
     if (!st) {
       launch_fwd2<T, 0>(rows2, g_tuning.fwd_border_rows, groups2, block2, stream, split2, a2);
     } else if (fmt == 2) {
@@ -322,6 +407,7 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     } else {
       launch_fwd2<T, 1>(rows2, g_tuning.fwd_border_rows, groups2, block2, stream, split2, a2);
     }
+    if (st) TIDE_CHECK(stage_steps(t, 2, true));
     for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
     for (int k = 0; k < 2; ++k) std::swap(cur_e[k], nxt_e[k]);
   }
@@ -335,9 +421,9 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     a.f_t = p.n_src > 0 ? f + t * n_shots * p.n_src : nullptr;
     a.r_t = p.n_rec > 0 ? r + t * n_shots * p.n_rec : nullptr;
     const bool st = storing && (t % step_ratio) == 0;
-    const size_t off = (size_t)(t / step_ratio) * store_step_bytes;
-    a.ey_store = st && ey_store ? (char *)ey_store + off : nullptr;
-    a.curl_store = st && curl_store ? (char *)curl_store + off : nullptr;
+    a.ey_store = st ? store_at(t, ey_store, ey_stage) : nullptr;
+    a.curl_store = st ? store_at(t, curl_store, curl_stage) : nullptr;
+    if (st) TIDE_CHECK(stage_steps(t, 1, false));
     if (!st) {
       launch_fwd<T, 0>(rows, loop, grid, block, stream, a);
     } else if (fmt == 2) {
@@ -345,6 +431,7 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
     } else {
       launch_fwd<T, 1>(rows, loop, grid, block, stream, a);
     }
+    if (st) TIDE_CHECK(stage_steps(t, 1, true));
     for (int k = 0; k < 5; ++k) std::swap(cur[k], nxt[k]);
   }
   timer.finish(launches);
@@ -362,6 +449,7 @@ int forward_impl(Problem<T> p, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_h
   }
   dpt.release(stream);
   if (fused) dpt2.release(stream);
+  if (kblk) dptk.release(stream);
   return kOk;
 }
 
@@ -372,7 +460,8 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
                   T *ml_ey_z, T *ml_hx_z, T *ml_hz_x, const void *ey_store,
                   const void *curl_store, int fmt, size_t store_step_bytes, T *grad_f,
                   double *gca, double *gcb, const int64_t *src, const int64_t *rec,
-                  int64_t nt, int64_t start_t, int64_t step_ratio, hipStream_t stream) {
+                  int64_t nt, int64_t start_t, int64_t step_ratio, hipStream_t stream,
+                  SnapshotStage *ey_stage = nullptr, SnapshotStage *curl_stage = nullptr) {
   TIDE_CHECK(check_geometry(p.n_shots, p.g.ny, p.g.nx));
   if (nt <= 0) return kOk;
   if (step_ratio <= 0) {
@@ -395,9 +484,14 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     return kInvalidArgument;
   }
   const int64_t n_shots = p.n_shots, total = n_shots * p.g.numel;
-  const bool fused = g_tuning.adj_steps == 2 && nt >= 2;
-  DevicePointTable dpt, dpt2;
+  const bool staged = (ey_stage && ey_stage->active()) || (curl_stage && curl_stage->active());
+  // K-step launches need K ring slots at once (see forward_impl)
+  const int kblk = (g_tuning.adj_steps >= 3 && nt >= g_tuning.adj_steps && !staged)
+                       ? g_tuning.adj_steps : 0;
+  const bool fused = g_tuning.adj_steps >= 2 && nt >= 2;
+  DevicePointTable dpt, dpt2, dptk;
   TIDE_CHECK(upload_points(p, src, rec, dpt, stream, fused ? &dpt2 : nullptr));
+  if (kblk) TIDE_CHECK(upload_points_k(p, src, rec, kblk, dptk, stream));
   Scratch scr(stream);
   TIDE_CHECK(pack_profiles(p, scr, stream));
   T *ley2, *lhx2, *lhz2, *m2[4];
@@ -450,13 +544,74 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
   // order: ley, lhx, lhz, mx (hz_x), my (hx_z), nx (ey_x), nz (ey_z)
   T *cur[7] = {l_ey, l_hx, l_hz, ml_hz_x, ml_hx_z, ml_ey_x, ml_ey_z};
   T *nxt[7] = {ley2, lhx2, lhz2, m2[0], m2[1], m2[2], m2[3]};
-  auto snap = [&](int64_t t, const void *buf, const double *acc) -> const void * {
+  auto snap = [&](int64_t t, const void *buf, const double *acc,
+                  SnapshotStage *stage) -> const void * {
     if (acc == nullptr || (t % step_ratio) != 0) return nullptr;
+    if (stage != nullptr && stage->active()) return stage->slot(t / step_ratio);
     return (const char *)buf + (size_t)(t / step_ratio) * store_step_bytes;
+  };
+  // Host-staged streams: the snapshots of steps t0-n+1 .. t0 are copied into
+  // their ring slots before the launch that reads them; after it, the slots
+  // are marked used and the next launch's snapshots are prefetched.
+  SnapshotStage *stages[2] = {gca ? ey_stage : nullptr, gcb ? curl_stage : nullptr};
+  auto stage_steps = [&](int64_t t0, int n, int phase) -> int {
+    for (int k = 0; k < 2; ++k) {
+      if (stages[k] == nullptr || !stages[k]->active()) continue;
+      for (int64_t tt = t0; tt > t0 - n; --tt) {
+        if (tt < start_t - nt || (tt % step_ratio) != 0) continue;
+        const int64_t idx = tt / step_ratio;
+        const int rc = phase == 0 ? stages[k]->before_read(idx)
+                       : phase == 1 ? stages[k]->after_read(idx)
+                                    : stages[k]->prefetch(idx);
+        if (rc != kOk) return rc;
+      }
+    }
+    return kOk;
   };
   int64_t t = start_t - 1;
   LoopTimer timer(1, stream);
   int64_t launches = 0;
+  if (kblk) {
+    const int rowsk = std::min<int>(rows_for<T>(g_tuning.adj_rows), 4) >= 4 ? 4 : 2;
+    AdjKArgs<T> ak;
+    std::memset(&ak, 0, sizeof(ak));
+    ak.g = p.g;
+    ak.pts = Points{dptk.row_slot, dptk.seg, dptk.ent, dptk.n_xt};
+    ak.ca = p.ca;
+    ak.cb = p.cb;
+    ak.cq = p.cq;
+    ak.per_shot_grad = a.per_shot_grad;
+    ak.step_scale = a.step_scale;
+    ak.n_shots = p.n_shots;
+    ak.n_yt = (p.g.ny + rowsk - 1) / rowsk;
+    ak.group_stride = a.group_stride;
+    ak.gca = ga_k;
+    ak.gcb = gb_k;
+    ak.split = make_split_k(p.g, dptk.n_xt, ak.n_yt, rowsk, kblk);
+    for (; t - (kblk - 1) >= start_t - nt; t -= kblk, ++launches) {
+      ak.ley_i = cur[0]; ak.lhx_i = cur[1]; ak.lhz_i = cur[2];
+      ak.mx_i = cur[3]; ak.my_i = cur[4]; ak.nx_i = cur[5]; ak.nz_i = cur[6];
+      ak.ley_o = nxt[0]; ak.lhx_o = nxt[1]; ak.lhz_o = nxt[2];
+      ak.mx_o = nxt[3]; ak.my_o = nxt[4]; ak.nx_o = nxt[5]; ak.nz_o = nxt[6];
+      bool g_any = false;
+      for (int j = 0; j < kblk; ++j) {
+        ak.grad_r[j] = p.n_rec > 0 ? grad_r + (t - j) * n_shots * p.n_rec : nullptr;
+        ak.grad_f[j] = (p.n_src > 0 && grad_f) ? grad_f + (t - j) * n_shots * p.n_src : nullptr;
+        ak.ey_store[j] = snap(t - j, ey_store, gca, ey_stage);
+        ak.curl_store[j] = snap(t - j, curl_store, gcb, curl_stage);
+        g_any = g_any || ak.ey_store[j] || ak.curl_store[j];
+      }
+      if (!g_any) {
+        launch_adjk<T, 0>(kblk, rowsk, g_tuning.adj_border_rows, n_groups, block, stream, ak);
+      } else if (fmt == 2) {
+        if constexpr (sizeof(T) == 4)
+          launch_adjk<T, 2>(kblk, rowsk, g_tuning.adj_border_rows, n_groups, block, stream, ak);
+      } else {
+        launch_adjk<T, 1>(kblk, rowsk, g_tuning.adj_border_rows, n_groups, block, stream, ak);
+      }
+      for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
+    }
+  }
   if (fused) {
     const bool split2 = g_tuning.adj_split != 0;
     const int rows2 = std::min<int>(rows_for<T>(g_tuning.adj_rows), 4);
@@ -485,11 +640,12 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
       a2.grad_r1 = p.n_rec > 0 ? grad_r + (t - 1) * n_shots * p.n_rec : nullptr;
       a2.grad_f0 = (p.n_src > 0 && grad_f) ? grad_f + t * n_shots * p.n_src : nullptr;
       a2.grad_f1 = (p.n_src > 0 && grad_f) ? grad_f + (t - 1) * n_shots * p.n_src : nullptr;
-      a2.ey_store0 = snap(t, ey_store, gca);
-      a2.curl_store0 = snap(t, curl_store, gcb);
-      a2.ey_store1 = snap(t - 1, ey_store, gca);
-      a2.curl_store1 = snap(t - 1, curl_store, gcb);
+      a2.ey_store0 = snap(t, ey_store, gca, ey_stage);
+      a2.curl_store0 = snap(t, curl_store, gcb, curl_stage);
+      a2.ey_store1 = snap(t - 1, ey_store, gca, ey_stage);
+      a2.curl_store1 = snap(t - 1, curl_store, gcb, curl_stage);
       const bool g_any = a2.ey_store0 || a2.curl_store0 || a2.ey_store1 || a2.curl_store1;
+      TIDE_CHECK(stage_steps(t, 2, 0));
       if (!g_any) {
         launch_adj2<T, 0>(rows2, g_tuning.adj_border_rows, n_groups, block2, stream, split2, a2);
       } else if (fmt == 2) {
@@ -497,6 +653,8 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
       } else {
         launch_adj2<T, 1>(rows2, g_tuning.adj_border_rows, n_groups, block2, stream, split2, a2);
       }
+      TIDE_CHECK(stage_steps(t, 2, 1));
+      TIDE_CHECK(stage_steps(t - 2, t - 3 >= start_t - nt ? 2 : 1, 2));
       for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
     }
   }
@@ -508,17 +666,21 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
     a.grad_r_t = p.n_rec > 0 ? grad_r + t * n_shots * p.n_rec : nullptr;
     a.grad_f_t = (p.n_src > 0 && grad_f) ? grad_f + t * n_shots * p.n_src : nullptr;
     const bool g_step = (gca || gcb) && (t % step_ratio) == 0;
-    const size_t off = (size_t)(t / step_ratio) * store_step_bytes;
-    a.ey_store = g_step && gca ? (const char *)ey_store + off : nullptr;
-    a.curl_store = g_step && gcb ? (const char *)curl_store + off : nullptr;
+    a.ey_store = g_step ? snap(t, ey_store, gca, ey_stage) : nullptr;
+    a.curl_store = g_step ? snap(t, curl_store, gcb, curl_stage) : nullptr;
     a.gca = g_step ? ga_k : nullptr;
     a.gcb = g_step ? gb_k : nullptr;
+    if (g_step) TIDE_CHECK(stage_steps(t, 1, 0));
     if (!g_step) {
       launch_adj<T, 0>(rows, loop, grid, block, lds, stream, a);
     } else if (fmt == 2) {
       if constexpr (sizeof(T) == 4) launch_adj<T, 2>(rows, loop, grid, block, lds, stream, a);
     } else {
       launch_adj<T, 1>(rows, loop, grid, block, lds, stream, a);
+    }
+    if (g_step) {
+      TIDE_CHECK(stage_steps(t, 1, 1));
+      TIDE_CHECK(stage_steps(t - 1, 1, 2));
     }
     for (int k = 0; k < 7; ++k) std::swap(cur[k], nxt[k]);
   }
@@ -534,6 +696,7 @@ int backward_impl(Problem<T> p, const T *grad_r, T *l_ey, T *l_hx, T *l_hz, T *m
       TIDE_HIP_TRY(hipMemcpyAsync(orig[k], cur[k], bytes, hipMemcpyDeviceToDevice, stream));
   dpt.release(stream);
   if (fused) dpt2.release(stream);
+  if (kblk) dptk.release(stream);
   return kOk;
 }
 
@@ -597,6 +760,67 @@ Problem<T> make_problem(const T *ca, const T *cb, const T *cq, const T *const pr
   return p;
 }
 
+// Host-staged snapshot arguments of a call: the reference's store_3 buffers,
+// file-name arrays and storage stream (mode 0: device history or none).
+struct StageArgs {
+  int64_t mode = 0;
+  void *ey_host = nullptr, *curl_host = nullptr;
+  char const *const *ey_files = nullptr;
+  char const *const *curl_files = nullptr;
+  void *storage_stream = nullptr;
+};
+
+// Opens the stages of the snapshot streams in use (store_1 non-null).
+inline int open_stages(const StageArgs &sa, bool saving, size_t step_bytes, void *ey_ring,
+                       void *curl_ring, int64_t first_store, hipStream_t stream,
+                       SnapshotStage &ey_st, SnapshotStage &curl_st) {
+  if (sa.mode != kStageCpu && sa.mode != kStageDisk) return kOk;
+  const hipStream_t storage = (hipStream_t)sa.storage_stream;
+  if (ey_ring != nullptr)
+    TIDE_CHECK(ey_st.open(sa.mode, saving, step_bytes, ey_ring, sa.ey_host,
+                          sa.ey_files ? sa.ey_files[0] : nullptr, first_store, stream, storage));
+  if (curl_ring != nullptr)
+    TIDE_CHECK(curl_st.open(sa.mode, saving, step_bytes, curl_ring, sa.curl_host,
+                            sa.curl_files ? sa.curl_files[0] : nullptr, first_store, stream,
+                            storage));
+  return kOk;
+}
+
+// Closes both stages; the first failure (of the call or a stage) wins.
+inline int close_stages(int rc, SnapshotStage &ey_st, SnapshotStage &curl_st) {
+  const int r1 = ey_st.finish();
+  const int r2 = curl_st.finish();
+  if (rc != kOk) return rc;
+  return r1 != kOk ? r1 : r2;
+}
+
+template <typename T>
+tm2d_ho::Problem<T> ho_problem(const T *ca, const T *cb, const T *cq, const T *const prof[12],
+                               T rdy, T rdx, int64_t n_shots, int64_t ny, int64_t nx,
+                               int64_t n_src, int64_t n_rec, bool ca_b, bool cb_b, bool cq_b,
+                               int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1) {
+  tm2d_ho::Problem<T> hp;
+  hp.ca = ca;
+  hp.cb = cb;
+  hp.cq = cq;
+  for (int k = 0; k < 12; ++k) hp.prof[k] = prof[k];
+  hp.rdy = rdy;
+  hp.rdx = rdx;
+  hp.n_shots = n_shots;
+  hp.ny = ny;
+  hp.nx = nx;
+  hp.n_src = n_src;
+  hp.n_rec = n_rec;
+  hp.pml_y0 = pml_y0;
+  hp.pml_x0 = pml_x0;
+  hp.pml_y1 = pml_y1;
+  hp.pml_x1 = pml_x1;
+  hp.ca_b = ca_b;
+  hp.cb_b = cb_b;
+  hp.cq_b = cq_b;
+  return hp;
+}
+
 template <typename T>
 int forward_entry(T const *ca, T const *cb, T const *cq, T const *f, T *ey, T *hx, T *hz,
                   T *m_ey_x, T *m_ey_z, T *m_hx_z, T *m_hz_x, T *r, T const *ay,
@@ -607,17 +831,41 @@ int forward_entry(T const *ca, T const *cb, T const *cq, T const *f, T *ey, T *h
                   int64_t n_rec, int64_t step_ratio, bool ca_b, bool cb_b, bool cq_b,
                   int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
                   int64_t pml_x1, int64_t device, void *stream_handle, void *ey_store,
-                  void *curl_store, int fmt, size_t store_step_bytes) {
+                  void *curl_store, int fmt, size_t store_step_bytes,
+                  char *(*arena_for)(int, size_t *) = nullptr,
+                  const StageArgs &sa = StageArgs(), int stencil = 2) {
   if (hipSetDevice((int)device) != hipSuccess) {
     set_error("hipSetDevice failed");
     return kHipError;
   }
+  char *arena = nullptr;
+  size_t arena_cap = 0;
+  if (arena_for != nullptr && (arena = arena_for((int)device, &arena_cap)) == nullptr)
+    return kHipError;  // the message is set by arena_for
   const T *prof[12] = {ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh};
   Problem<T> p = make_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots, ny, nx, n_src, n_rec,
                                  ca_b, cb_b, cq_b, pml_y0, pml_x0, pml_y1, pml_x1);
-  return forward_impl<T>(p, ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x, f, r, sources_i,
+  const hipStream_t stream = (hipStream_t)stream_handle;
+  SnapshotStage ey_st, curl_st;
+  if (step_ratio > 0)
+    TIDE_CHECK(open_stages(sa, true, store_step_bytes, ey_store, curl_store,
+                           (start_t + step_ratio - 1) / step_ratio, stream, ey_st, curl_st));
+  int rc;
+  if (stencil == 2) {
+    rc = forward_impl<T>(p, ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x, f, r, sources_i,
                          receivers_i, nt, start_t, step_ratio, ey_store, curl_store, fmt,
-                         store_step_bytes, (hipStream_t)stream_handle);
+                         store_step_bytes, stream, arena, arena_cap, &ey_st, &curl_st);
+  } else {
+    const tm2d_ho::Problem<T> hp = ho_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots, ny, nx,
+                                                 n_src, n_rec, ca_b, cb_b, cq_b, pml_y0, pml_x0,
+                                                 pml_y1, pml_x1);
+    T *const fld[7] = {ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x};
+    rc = step_ratio <= 0 ? (set_error("step_ratio must be positive"), (int)kInvalidArgument)
+                         : tm2d_ho::forward<T>(stencil, hp, fld, f, r, sources_i, receivers_i, nt,
+                                               start_t, step_ratio, ey_store, curl_store, fmt,
+                                               store_step_bytes, stream, &ey_st, &curl_st);
+  }
+  return close_stages(rc, ey_st, curl_st);
 }
 
 // Shared backward body.  When gca64/gcb64 are given (FP64 accumulators, +=)
@@ -637,7 +885,8 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
                    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
                    bool ca_rg, bool cb_rg, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,
                    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
-                   int64_t device, void *stream_handle) {
+                   int64_t device, void *stream_handle, const StageArgs &sa = StageArgs(),
+                   int stencil = 2) {
   if (ca_rg && cb_rg && ca_b != cb_b) {  // checked before any device call
     set_error("ca and cb gradients need the same batching (ca_batched == cb_batched)");
     return kInvalidArgument;
@@ -648,12 +897,14 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
   }
   hipStream_t stream = (hipStream_t)stream_handle;
   const bool want = ca_rg || cb_rg;
-  if (want && storage_mode != kStorageDevice) {
-    set_error(storage_mode == kStorageNone
-                  ? "model gradients need stored snapshots (storage_mode 'none')"
-                  : "host- and disk-staged snapshot storage is not built; use "
-                    "storage_mode 'device'");
-    return storage_mode == kStorageNone ? kInvalidArgument : kUnsupported;
+  if (want && storage_mode == kStorageNone) {
+    set_error("model gradients need stored snapshots (storage_mode 'none')");
+    return kInvalidArgument;
+  }
+  if (storage_mode != kStorageDevice && storage_mode != kStorageNone &&
+      storage_mode != kStageCpu && storage_mode != kStageDisk) {
+    set_error("storage_mode must be 0 (device), 1 (cpu), 2 (disk) or 3 (none)");
+    return kInvalidArgument;
   }
   const T *prof[12] = {ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh};
   Problem<T> p = make_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots, ny, nx, n_src, n_rec,
@@ -675,11 +926,35 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
     }
   }
   const int fmt = storage_format == kFormatBf16 ? 2 : 1;
-  TIDE_CHECK(backward_impl<T>(p, grad_r, lambda_ey, lambda_hx, lambda_hz, m_lambda_ey_x,
-                              m_lambda_ey_z, m_lambda_hx_z, m_lambda_hz_x, ey_store_1,
-                              curl_store_1, fmt, (size_t)shot_bytes_uncomp * (size_t)n_shots,
+  const size_t step_bytes = (size_t)shot_bytes_uncomp * (size_t)n_shots;
+  SnapshotStage ey_st, curl_st;
+  if (want && step_ratio > 0) {
+    StageArgs s2 = sa;
+    s2.mode = storage_mode;
+    TIDE_CHECK(open_stages(s2, false, step_bytes, ca_rg ? ey_store_1 : nullptr,
+                           cb_rg ? curl_store_1 : nullptr, 0, stream, ey_st, curl_st));
+  }
+  int rc;
+  if (stencil == 2) {
+    rc = backward_impl<T>(p, grad_r, lambda_ey, lambda_hx, lambda_hz, m_lambda_ey_x,
+                          m_lambda_ey_z, m_lambda_hx_z, m_lambda_hz_x, ey_store_1, curl_store_1,
+                          fmt, step_bytes, grad_f, ca_rg ? ga : nullptr, cb_rg ? gb : nullptr,
+                          sources_i, receivers_i, nt, start_t, step_ratio, stream, &ey_st,
+                          &curl_st);
+  } else if (step_ratio <= 0) {
+    set_error("step_ratio must be positive");
+    rc = kInvalidArgument;
+  } else {
+    const tm2d_ho::Problem<T> hp = ho_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots, ny, nx,
+                                                 n_src, n_rec, ca_b, cb_b, cq_b, pml_y0, pml_x0,
+                                                 pml_y1, pml_x1);
+    T *const lam[7] = {lambda_ey, lambda_hx, lambda_hz, m_lambda_ey_x, m_lambda_ey_z,
+                       m_lambda_hx_z, m_lambda_hz_x};
+    rc = tm2d_ho::backward<T>(stencil, hp, lam, grad_r, ey_store_1, curl_store_1, fmt, step_bytes,
                               grad_f, ca_rg ? ga : nullptr, cb_rg ? gb : nullptr, sources_i,
-                              receivers_i, nt, start_t, step_ratio, stream));
+                              receivers_i, nt, start_t, step_ratio, stream, &ey_st, &curl_st);
+  }
+  TIDE_CHECK(close_stages(rc, ey_st, curl_st));
   if (own) {
     if (ca_rg) TIDE_CHECK(fold_into<T>(grad_ca, ga, na, stream));
     if (cb_rg) TIDE_CHECK(fold_into<T>(grad_cb, gb, nb, stream));
@@ -689,8 +964,8 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
 
 }  // namespace
 
-#define TIDE_TM2D_ENTRY_POINTS(T, NAME)                                                    \
-  extern "C" int maxwell_tm_2_##NAME##_forward_cuda(                                      \
+#define TIDE_TM2D_ENTRY_POINTS(S, T, NAME)                                                    \
+  extern "C" int maxwell_tm_##S##_##NAME##_forward_cuda(                                      \
       T const *ca, T const *cb, T const *cq, T const *f, T *ey, T *hx, T *hz, T *m_ey_x,  \
       T *m_ey_z, T *m_hx_z, T *m_hz_x, T const *debye_a, T const *debye_b,                \
       T const *debye_cp, T *polarization, T *ey_prev, T *r, int64_t n_poles, T const *ay, \
@@ -712,10 +987,10 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
                             sources_i, receivers_i, rdy, rdx, nt, n_shots, ny, nx, n_src, \
                             n_rec, step_ratio, ca_b, cb_b, cq_b, start_t, pml_y0, pml_x0, \
                             pml_y1, pml_x1, device, compute_stream, nullptr, nullptr, 0,  \
-                            0);                                                           \
+                            0, nullptr, StageArgs(), S);                                  \
   }                                                                                       \
                                                                                           \
-  extern "C" int maxwell_tm_2_##NAME##_forward_with_storage_cuda(                         \
+  extern "C" int maxwell_tm_##S##_##NAME##_forward_with_storage_cuda(                         \
       T const *ca, T const *cb, T const *cq, T const *f, T *ey, T *hx, T *hz, T *m_ey_x,  \
       T *m_ey_z, T *m_hx_z, T *m_hz_x, T *r, void *ey_store_1, void *ey_store_3,          \
       char const *const *ey_filenames, void *curl_store_1, void *curl_store_3,            \
@@ -728,15 +1003,22 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
       bool ca_rg, bool cb_rg, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,           \
       int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1, int64_t n_threads,  \
       int64_t device, void *compute_stream, void *storage_stream) {                       \
-    (void)ey_store_3; (void)ey_filenames; (void)curl_store_3;                             \
-    (void)curl_filenames; (void)dt; (void)n_threads; (void)storage_stream;                \
-    if (storage_mode != kStorageDevice && storage_mode != kStorageNone) {                 \
-      set_error("host- and disk-staged snapshot storage is not built; use "               \
-                "storage_mode 'device' or 'none'");                                       \
-      return kUnsupported;                                                                \
+    (void)dt; (void)n_threads;                                                            \
+    if (storage_mode < kStorageDevice || storage_mode > kStorageNone) {                   \
+      set_error("storage_mode must be 0 (device), 1 (cpu), 2 (disk) or 3 (none)");       \
+      return kInvalidArgument;                                                            \
     }                                                                                     \
-    const bool st = storage_mode == kStorageDevice;                                       \
+    const bool st = storage_mode != kStorageNone;                                         \
     const int fmt = storage_format == kFormatBf16 ? 2 : 1;                                \
+    StageArgs sa;                                                                         \
+    if (storage_mode == kStageCpu || storage_mode == kStageDisk) {                        \
+      sa.mode = storage_mode;                                                             \
+      sa.ey_host = ey_store_3;                                                            \
+      sa.curl_host = curl_store_3;                                                        \
+      sa.ey_files = ey_filenames;                                                         \
+      sa.curl_files = curl_filenames;                                                     \
+      sa.storage_stream = storage_stream;                                                 \
+    }                                                                                     \
     return forward_entry<T>(ca, cb, cq, f, ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x, r, \
                             ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh,         \
                             sources_i, receivers_i, rdy, rdx, nt, n_shots, ny, nx, n_src, \
@@ -744,10 +1026,10 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
                             pml_y1, pml_x1, device, compute_stream,                       \
                             st && ca_rg ? ey_store_1 : nullptr,                           \
                             st && cb_rg ? curl_store_1 : nullptr, fmt,                    \
-                            (size_t)shot_bytes_uncomp * (size_t)n_shots);                 \
+                            (size_t)shot_bytes_uncomp * (size_t)n_shots, nullptr, sa, S); \
   }                                                                                       \
                                                                                           \
-  extern "C" int maxwell_tm_2_##NAME##_backward_cuda(                                     \
+  extern "C" int maxwell_tm_##S##_##NAME##_backward_cuda(                                     \
       T const *ca, T const *cb, T const *cq, T const *grad_r, T *lambda_ey, T *lambda_hx, \
       T *lambda_hz, T *m_lambda_ey_x, T *m_lambda_ey_z, T *m_lambda_hx_z,                 \
       T *m_lambda_hz_x, T *adjoint_memory_scratch, void *ey_store_1, void *ey_store_3,    \
@@ -762,9 +1044,14 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
       bool ca_rg, bool cb_rg, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,           \
       int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1, int64_t n_threads,  \
       int64_t device, void *compute_stream, void *storage_stream) {                       \
-    (void)adjoint_memory_scratch; (void)ey_store_3; (void)ey_filenames;                   \
-    (void)curl_store_3; (void)curl_filenames; (void)grad_ca_shot;                         \
-    (void)grad_cb_shot; (void)dt; (void)n_threads; (void)storage_stream;                  \
+    (void)adjoint_memory_scratch; (void)grad_ca_shot;                                     \
+    (void)grad_cb_shot; (void)dt; (void)n_threads;                                        \
+    StageArgs sa;                                                                         \
+    sa.ey_host = ey_store_3;                                                              \
+    sa.curl_host = curl_store_3;                                                          \
+    sa.ey_files = ey_filenames;                                                           \
+    sa.curl_files = curl_filenames;                                                       \
+    sa.storage_stream = storage_stream;                                                   \
     return backward_entry<T>(ca, cb, cq, grad_r, lambda_ey, lambda_hx, lambda_hz,         \
                              m_lambda_ey_x, m_lambda_ey_z, m_lambda_hx_z, m_lambda_hz_x,  \
                              ey_store_1, curl_store_1, grad_f, grad_ca, grad_cb, nullptr, \
@@ -772,11 +1059,62 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
                              kxh, sources_i, receivers_i, rdy, rdx, nt, n_shots, ny, nx,  \
                              n_src, n_rec, step_ratio, storage_mode, storage_format,      \
                              shot_bytes_uncomp, ca_rg, cb_rg, ca_b, cb_b, cq_b, start_t,  \
-                             pml_y0, pml_x0, pml_y1, pml_x1, device, compute_stream);     \
+                             pml_y0, pml_x0, pml_y1, pml_x1, device, compute_stream, sa,  \
+                             S);                                                          \
   }
 
-TIDE_TM2D_ENTRY_POINTS(float, float)
-TIDE_TM2D_ENTRY_POINTS(double, double)
+TIDE_TM2D_ENTRY_POINTS(2, float, float)
+TIDE_TM2D_ENTRY_POINTS(2, double, double)
+TIDE_TM2D_ENTRY_POINTS(4, float, float)
+TIDE_TM2D_ENTRY_POINTS(4, double, double)
+TIDE_TM2D_ENTRY_POINTS(6, float, float)
+TIDE_TM2D_ENTRY_POINTS(6, double, double)
+TIDE_TM2D_ENTRY_POINTS(8, float, float)
+TIDE_TM2D_ENTRY_POINTS(8, double, double)
+
+// Born forward entry points: reference ABI src/tide/backend_utils.py:336-349
+// (_TM_BORN_FORWARD_SPEC); r receives the scattered receivers, background_r
+// the background's.  dca / dcb null: no coefficient perturbation.
+#define TIDE_TM2D_BORN_ENTRY(S, T, NAME)                                                   \
+  extern "C" int maxwell_tm_##S##_##NAME##_born_forward_cuda(                             \
+      T const *ca, T const *cb, T const *cq, T const *dca, T const *dcb, T const *f0,     \
+      T const *df, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_hx_z, T *m_hz_x,       \
+      T *dey, T *dhx, T *dhz, T *dm_ey_x, T *dm_ey_z, T *dm_hx_z, T *dm_hz_x, T *r,       \
+      T *background_r, T const *ay, T const *by, T const *ayh, T const *byh, T const *ax, \
+      T const *bx, T const *axh, T const *bxh, T const *ky, T const *kyh, T const *kx,    \
+      T const *kxh, int64_t const *sources_i, int64_t const *receivers_i, T rdy, T rdx,   \
+      T dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,           \
+      int64_t n_rec, int64_t step_ratio, bool ca_b, bool cb_b, bool cq_b,                 \
+      int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,    \
+      int64_t n_threads, int64_t device, void *compute_stream) {                          \
+    (void)dt; (void)n_threads;                                                            \
+    if (step_ratio != 1) {                                                                \
+      set_error("born_forward: step_ratio must be 1 (resample the sources beforehand)");  \
+      return kInvalidArgument;                                                            \
+    }                                                                                     \
+    if (hipSetDevice((int)device) != hipSuccess) {                                        \
+      set_error("hipSetDevice failed");                                                   \
+      return kHipError;                                                                   \
+    }                                                                                     \
+    const T *prof[12] = {ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh};           \
+    const tm2d_ho::Problem<T> hp = ho_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots, ny, \
+                                                 nx, n_src, n_rec, ca_b, cb_b, cq_b,      \
+                                                 pml_y0, pml_x0, pml_y1, pml_x1);         \
+    T *const bg[7] = {ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x};                        \
+    T *const sc[7] = {dey, dhx, dhz, dm_ey_x, dm_ey_z, dm_hx_z, dm_hz_x};                 \
+    return tm2d_ho::born_forward<T>(S, hp, bg, sc, dca, dcb, f0, df, r, background_r,     \
+                                    sources_i, receivers_i, nt, start_t,                  \
+                                    (hipStream_t)compute_stream);                         \
+  }
+
+TIDE_TM2D_BORN_ENTRY(2, float, float)
+TIDE_TM2D_BORN_ENTRY(2, double, double)
+TIDE_TM2D_BORN_ENTRY(4, float, float)
+TIDE_TM2D_BORN_ENTRY(4, double, double)
+TIDE_TM2D_BORN_ENTRY(6, float, float)
+TIDE_TM2D_BORN_ENTRY(6, double, double)
+TIDE_TM2D_BORN_ENTRY(8, float, float)
+TIDE_TM2D_BORN_ENTRY(8, double, double)
 
 // FP64-accumulator backward used by the Python operator (and any caller that
 // runs shot chunks and wants one FP64 sum across them); see tide_hip.h.
@@ -784,24 +1122,36 @@ TIDE_TM2D_ENTRY_POINTS(double, double)
   extern "C" int tide_tm2d_##NAME##_backward_f64acc(                                      \
       T const *ca, T const *cb, T const *cq, T const *grad_r, T *lambda_ey, T *lambda_hx, \
       T *lambda_hz, T *m_lambda_ey_x, T *m_lambda_ey_z, T *m_lambda_hx_z,                 \
-      T *m_lambda_hz_x, void const *ey_store, void const *curl_store, T *grad_f,          \
-      double *grad_ca, double *grad_cb, T const *ay, T const *by, T const *ayh,           \
-      T const *byh, T const *ax, T const *bx, T const *axh, T const *bxh, T const *ky,    \
-      T const *kyh, T const *kx, T const *kxh, int64_t const *sources_i,                  \
-      int64_t const *receivers_i, T rdy, T rdx, int64_t nt, int64_t n_shots, int64_t ny,  \
-      int64_t nx, int64_t n_src, int64_t n_rec, int64_t step_ratio,                       \
-      int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_b, bool cb_b, bool cq_b, \
-      int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,    \
-      int64_t device, void *compute_stream) {                                             \
+      T *m_lambda_hz_x, void *ey_store_1, void *ey_store_3,                               \
+      char const *const *ey_filenames, void *curl_store_1, void *curl_store_3,            \
+      char const *const *curl_filenames, T *grad_f, double *grad_ca, double *grad_cb,     \
+      T const *ay, T const *by, T const *ayh, T const *byh, T const *ax, T const *bx,     \
+      T const *axh, T const *bxh, T const *ky, T const *kyh, T const *kx, T const *kxh,   \
+      int64_t const *sources_i, int64_t const *receivers_i, T rdy, T rdx, int64_t nt,     \
+      int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src, int64_t n_rec,              \
+      int64_t step_ratio, int64_t storage_mode, int64_t storage_format,                   \
+      int64_t shot_bytes_uncomp, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,        \
+      int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1, int64_t stencil,    \
+      int64_t device, void *compute_stream, void *storage_stream) {                       \
+    if (stencil != 2 && stencil != 4 && stencil != 6 && stencil != 8) {                   \
+      set_error("stencil must be 2, 4, 6 or 8");                                          \
+      return kInvalidArgument;                                                            \
+    }                                                                                     \
     const bool want = grad_ca != nullptr || grad_cb != nullptr;                           \
+    StageArgs sa;                                                                         \
+    sa.ey_host = ey_store_3;                                                              \
+    sa.curl_host = curl_store_3;                                                          \
+    sa.ey_files = ey_filenames;                                                           \
+    sa.curl_files = curl_filenames;                                                       \
+    sa.storage_stream = storage_stream;                                                   \
     return backward_entry<T>(                                                             \
         ca, cb, cq, grad_r, lambda_ey, lambda_hx, lambda_hz, m_lambda_ey_x,               \
-        m_lambda_ey_z, m_lambda_hx_z, m_lambda_hz_x, (void *)ey_store, (void *)curl_store, \
-        grad_f, nullptr, nullptr, grad_ca, grad_cb, ay, by, ayh, byh, ax, bx, axh, bxh,   \
-        ky, kyh, kx, kxh, sources_i, receivers_i, rdy, rdx, nt, n_shots, ny, nx, n_src,   \
-        n_rec, step_ratio, want ? kStorageDevice : kStorageNone, storage_format,          \
-        shot_bytes_uncomp, grad_ca != nullptr, grad_cb != nullptr, ca_b, cb_b, cq_b,      \
-        start_t, pml_y0, pml_x0, pml_y1, pml_x1, device, compute_stream);                 \
+        m_lambda_ey_z, m_lambda_hx_z, m_lambda_hz_x, ey_store_1, curl_store_1, grad_f,    \
+        nullptr, nullptr, grad_ca, grad_cb, ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh,  \
+        kx, kxh, sources_i, receivers_i, rdy, rdx, nt, n_shots, ny, nx, n_src, n_rec,     \
+        step_ratio, want ? storage_mode : kStorageNone, storage_format, shot_bytes_uncomp, \
+        grad_ca != nullptr, grad_cb != nullptr, ca_b, cb_b, cq_b, start_t, pml_y0,        \
+        pml_x0, pml_y1, pml_x1, device, compute_stream, sa, (int)stencil);                \
   }
 
 TIDE_TM2D_BACKWARD_F64(float, float)
@@ -845,8 +1195,9 @@ extern "C" int tide_tm2d_get_shot_loop(int *out2) {
 }
 
 extern "C" int tide_tm2d_set_time_block(int fwd_steps, int adj_steps) {
-  if (fwd_steps < 1 || fwd_steps > 2 || adj_steps < 1 || adj_steps > 2) {
-    set_error("time steps per launch must be 1 or 2");
+  if (fwd_steps < 1 || fwd_steps > kMaxBlockSteps || adj_steps < 1 ||
+      adj_steps > kMaxBlockSteps) {
+    set_error("time steps per launch must be 1, 2, 3 or 4");
     return kInvalidArgument;
   }
   g_tuning.fwd_steps = fwd_steps;

@@ -1,0 +1,13 @@
+// Launchers of the K-step fwd kernels for float fields.
+#define TIDE_TM2D_LAUNCH_K_DEFINITIONS
+#include "tm2d_launch_k.h"
+
+namespace tide_hip {
+namespace tm2d {
+
+template void launch_fwdk<float, 0>(int, int, int, int64_t, dim3, hipStream_t, const FwdKArgs<float> &);
+template void launch_fwdk<float, 1>(int, int, int, int64_t, dim3, hipStream_t, const FwdKArgs<float> &);
+template void launch_fwdk<float, 2>(int, int, int, int64_t, dim3, hipStream_t, const FwdKArgs<float> &);
+
+}  // namespace tm2d
+}  // namespace tide_hip

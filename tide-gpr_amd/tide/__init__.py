@@ -3,13 +3,15 @@
 Public surface for the propagation hot path, mirroring the reference
 package: ``MaxwellTM`` / ``Maxwell3D`` operators over ``EMModel`` with
 ``Experiment`` / ``Acquisition`` / ``Discretization`` / ``CPML`` inputs,
-``StorageOptions`` snapshot policy, ``linearize().vjp``, the ``workflow``
+``StorageOptions`` snapshot policy (device, host-staged or disk),
+wavefield callbacks, ``linearize().vjp``, the ``workflow``
 shot-batching and receiver-objective helpers; plus ``ricker`` and
 ``cfl_condition``.  The compute path is the HIP library in ``_lib``; there
 is no CPU fallback.
 """
 
-from . import cfl, padding, staggered, storage, utils, wavelets
+from . import callbacks, cfl, padding, staggered, storage, utils, wavelets
+from .callbacks import Callback, CallbackState
 from .cfl import cfl_condition
 from .core import BackendPreference, FallbackPolicy, StorageMode, StorageOptions
 from .maxwell import (
@@ -27,6 +29,7 @@ from .maxwell import (
     MaxwellTM,
     Observers,
     SourceConvention,
+    TangentResult,
     TMState,
 )
 from .utils import C0, EP0, MU0
@@ -38,6 +41,8 @@ __all__ = [
     "BackendPreference",
     "C0",
     "CPML",
+    "Callback",
+    "CallbackState",
     "Discretization",
     "EM3DState",
     "EMDirection",
@@ -56,6 +61,8 @@ __all__ = [
     "StorageMode",
     "StorageOptions",
     "TMState",
+    "TangentResult",
+    "callbacks",
     "cfl",
     "cfl_condition",
     "gaussian",

@@ -127,8 +127,7 @@ def _tm_backward_f64acc_types(f: Any) -> list[Any]:
         _P,  # grad_r
         *([_P] * 3),  # lambda fields
         *([_P] * 4),  # lambda memories
-        _P,
-        _P,  # ey_store, curl_store
+        *([_P] * 6),  # ey_store_1, ey_store_3, ey_filenames, curl_store_1/_3, curl_filenames
         _P,  # grad_f
         _P,
         _P,  # grad_ca (fp64), grad_cb (fp64)
@@ -138,13 +137,16 @@ def _tm_backward_f64acc_types(f: Any) -> list[Any]:
         f,
         f,  # rdy, rdx
         *([_I] * 7),  # nt, n_shots, ny, nx, n_src, n_rec, step_ratio
+        _I,  # storage_mode
         _I,
         _I,  # storage_format, shot_bytes_uncomp
         *([_B] * 3),
         _I,  # start_t
         *([_I] * 4),
+        _I,  # stencil
         _I,  # device
         _P,  # compute_stream
+        _P,  # storage_stream
     ]
 
 
@@ -234,12 +236,22 @@ def load() -> ctypes.CDLL:
             ("forward_with_storage", _tm_forward_storage_types(ctype)),
             ("backward", _tm_backward_types(ctype)),
         ):
-            fn = getattr(lib, f"maxwell_tm_2_{name}_{pass_name}_cuda")
-            fn.restype = ctypes.c_int
-            fn.argtypes = types
+            for stencil in (2, 4, 6, 8):
+                fn = getattr(lib, f"maxwell_tm_{stencil}_{name}_{pass_name}_cuda")
+                fn.restype = ctypes.c_int
+                fn.argtypes = types
         fn = getattr(lib, f"tide_tm2d_{name}_backward_f64acc")
         fn.restype = ctypes.c_int
         fn.argtypes = _tm_backward_f64acc_types(ctype)
+    # Born forward, reference _TM_BORN_FORWARD_SPEC (backend_utils.py:336-349):
+    # 23 field/coefficient pointers, 12 profiles, 2 point arrays | rdy rdx dt
+    # | nt n_shots ny nx n_src n_rec step_ratio | 3 batched | start_t |
+    # 4 pml | n_threads | device | stream
+    for name, ctype in (("float", ctypes.c_float), ("double", ctypes.c_double)):
+        for stencil in (2, 4, 6, 8):
+            fn = getattr(lib, f"maxwell_tm_{stencil}_{name}_born_forward_cuda")
+            fn.restype = ctypes.c_int
+            fn.argtypes = _expand_kinds("P37 F3 I7 B3 I1 I4 I1 I1 P1", ctype)
     for name, ctype in (("float", ctypes.c_float), ("double", ctypes.c_double)):
         fn = getattr(lib, f"tide_em3d_{name}_forward")
         fn.restype = ctypes.c_int
@@ -364,7 +376,7 @@ def get_tm2d_shot_loop() -> tuple[bool, bool]:
 
 
 def set_tm2d_time_block(fwd_steps: int, adj_steps: int) -> None:
-    """Time steps per launch (1 or 2) of the TM2D kernels per pass (see include/tide_hip.h)."""
+    """Time steps per launch (1 to 4) of the TM2D kernels per pass (see include/tide_hip.h)."""
     check(load().tide_tm2d_set_time_block(int(fwd_steps), int(adj_steps)))
 
 

@@ -199,10 +199,10 @@ def tune_tm2d(
         def run_bwd():
             _native.check(bwd(
                 P(st.ca), P(st.cb), P(st.cq), P(gr), *[P(t) for t in lam], P(stores[0]),
-                P(stores[1]), P(gf), P(gca), P(gcb), *[P(p) for p in st.profiles],
-                P(st.sources_i), P(st.receivers_i), st.rdy, st.rdx, nt, S, pny, pnx, st.n_src,
-                st.n_rec, 1, 0, pny * pnx * bpe, False, False, False, nt, *st.pml,
-                device.index or 0, stream))
+                None, None, P(stores[1]), None, None, P(gf), P(gca), P(gcb),
+                *[P(p) for p in st.profiles], P(st.sources_i), P(st.receivers_i), st.rdy,
+                st.rdx, nt, S, pny, pnx, st.n_src, st.n_rec, 1, 0, 0, pny * pnx * bpe, False,
+                False, False, nt, *st.pml, 2, device.index or 0, stream, stream))
 
         return run_fwd, run_bwd, reset
 

@@ -10,7 +10,8 @@ from .contracts import (
     SourceConvention,
 )
 from .operators import Maxwell3D, MaxwellTM
-from .results import EM3DState, EMGradient, ForwardResult, TMState
+from .results import EM3DState, EMGradient, ForwardResult, TangentResult, TMState
+from .tm2d_born import borntm
 
 __all__ = [
     "Acquisition",
@@ -28,4 +29,6 @@ __all__ = [
     "Observers",
     "SourceConvention",
     "TMState",
+    "TangentResult",
+    "borntm",
 ]

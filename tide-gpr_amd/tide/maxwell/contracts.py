@@ -139,6 +139,14 @@ class EMDirection:
     sigma: torch.Tensor | None = None
     mu: torch.Tensor | None = None
 
+    def validate_for(self, model: "EMModel") -> None:
+        if self.epsilon is None and self.sigma is None and self.mu is None:
+            raise ValueError("At least one model direction must be provided.")
+        for name in ("epsilon", "sigma", "mu"):
+            d, m = getattr(self, name), getattr(model, name)
+            if d is not None and (d.shape != m.shape or d.device != m.device or d.dtype != m.dtype):
+                raise ValueError(f"The {name} direction must match the corresponding model tensor.")
+
 
 @dataclass(frozen=True)
 class Observers:

@@ -1,10 +1,11 @@
-"""Adjoint Jacobian of the Maxwell operators (reference
-``src/tide/maxwell/linearization.py:36-118``).
+"""Jacobian products of the Maxwell operators (reference
+``src/tide/maxwell/linearization.py:36-229``).
 
 ``op.linearize(model)`` runs the primal once with a differentiable model and
 ``.vjp(cotangent)`` applies ``J(model)^T`` to a receiver cotangent through
-the hand-written adjoint.  The tangent (``jvp``) and second-order products
-belong to the Born family, which is not part of this build.
+the hand-written adjoint.  For the 2-D operator ``.jvp(direction)`` applies
+``J(model)`` by Born (tangent-linear) propagation (``tm2d_born.borntm``).
+Second-order products (``second_vjp``) and the 3-D tangent are not built.
 """
 
 from __future__ import annotations
@@ -14,8 +15,8 @@ from collections.abc import Iterable
 import torch
 
 from ..core import StorageOptions
-from .contracts import EMModel
-from .results import EM3DState, EMGradient, ForwardResult, TMState
+from .contracts import EMDirection, EMModel, SourceConvention
+from .results import EM3DState, EMGradient, ForwardResult, TangentResult, TMState
 
 _TARGETS = ("epsilon", "sigma", "mu")
 
@@ -84,13 +85,40 @@ class _LinearizedMaxwell:
         return EMGradient(**values)
 
     def jvp(self, direction):
-        raise NotImplementedError("Born tangent products are not part of this build")
+        raise NotImplementedError("tangent products of this operator are not part of this build")
 
     def second_vjp(self, *args, **kwargs):
         raise NotImplementedError("second-order products are not part of this build")
 
 
 class LinearizedMaxwellTM(_LinearizedMaxwell):
+    def jvp(self, direction: EMDirection) -> TangentResult:
+        """``J(model) direction``: receivers and final state of the Born
+        scattered field (reference ``linearization.py:178-229``).  The
+        background computed alongside is not kept as the primal: the primal
+        stays the differentiable operator run that ``vjp`` needs."""
+        if self._closed:
+            raise RuntimeError("The linearized Maxwell operator is closed.")
+        direction.validate_for(self.model)
+        if direction.mu is not None:
+            raise NotImplementedError("TM2D tangent propagation for mu is not available.")
+        from .tm2d_born import borntm
+
+        op, model = self.operator, self.model
+        d, e = op.discretization, op.experiment
+        out = borntm(
+            model.epsilon, model.sigma, model.mu, grid_spacing=d.spacing, dt=d.dt,
+            source_amplitude=e.source_amplitude,
+            source_location=e.acquisition.source_location,
+            receiver_location=e.acquisition.receiver_location,
+            depsilon=direction.epsilon, dsigma=direction.sigma, stencil=d.stencil,
+            pml_width=d.boundary.width, max_vel=d.max_velocity, nt=e.nt,
+            linearize_source=e.source_convention is SourceConvention.PHYSICAL_CURRENT,
+            freq_taper_frac=e.frequency_taper_fraction, time_pad_frac=e.time_padding_fraction,
+            time_taper=e.time_taper,
+        )
+        return TangentResult(out[-1], TMState.from_tensors(tuple(out[7:14])))
+
     def _separable_gradient(self, observed: torch.Tensor, residual):
         """Value and gradient of a shot-separable receiver objective, chunk
         by chunk with one FP64 accumulator (``tm2d.separable_gradient``).

@@ -63,10 +63,18 @@ class ForwardResult(Generic[StateT]):
 
 
 @dataclass(frozen=True)
+class TangentResult(Generic[StateT]):
+    """Receiver samples and final state of one tangent (Born) propagation."""
+
+    receiver_data: torch.Tensor
+    final_state: StateT
+
+
+@dataclass(frozen=True)
 class EMGradient:
     epsilon: torch.Tensor | None
     sigma: torch.Tensor | None
     mu: torch.Tensor | None = None
 
 
-__all__ = ["EM3DState", "EMGradient", "ForwardResult", "TMState"]
+__all__ = ["EM3DState", "EMGradient", "ForwardResult", "TMState", "TangentResult"]

@@ -16,26 +16,38 @@
   source-scaling chain rules exactly as in the reference;
 * outputs are the unpadded final state and receivers ``[nt, shots, n_rec]``.
 
+Snapshots stay in HBM (``storage_mode="device"``, shot chunks when they do
+not fit) or are staged through pinned host memory (``"cpu"``) or a file
+(``"disk"``) by the native library; ``forward_callback`` /
+``backward_callback`` run the time loop in chunks of ``callback_frequency``
+steps (reference ``tm2d_autograd.py:165-261, 677-802``).
+
 Only the HIP device path exists.  CPU tensors, the Python reference
-backend, callbacks, dispersion, stencils other than 2, host-staged storage
-and internal resampling (step_ratio > 1) raise ``NotImplementedError``.
+backend, dispersion and internal resampling
+(step_ratio > 1) raise ``NotImplementedError``.
 """
 
 from __future__ import annotations
 
+import ctypes
 import math
+import tempfile
 from collections.abc import Sequence
 from dataclasses import dataclass, field
+from pathlib import Path
 from typing import Any
 
 import torch
 
 from .. import _native, staggered
+from ..callbacks import CallbackState
 from ..cfl import cfl_condition
 from ..grid_utils import _normalize_grid_spacing_2d, _normalize_pml_width_2d
 from ..padding import create_or_pad, zero_interior
 from ..storage import (
+    STORAGE_CPU,
     STORAGE_DEVICE,
+    STORAGE_DISK,
     STORAGE_FORMAT_BF16,
     STORAGE_NONE,
     _resolve_storage_compression,
@@ -116,6 +128,7 @@ class TM2DSetup:
     fd_pad: list[int]
     pml_width: list[int]
     batched: bool
+    stencil: int = 2
     extra: dict[str, Any] = field(default_factory=dict)
 
     def unpad(self, t: torch.Tensor) -> torch.Tensor:
@@ -147,6 +160,7 @@ def prepare_tm2d(
     initial: Sequence[torch.Tensor | None] = (None,) * 7,
     validate_material_inputs: bool = True,
     allocate_fields: bool = True,
+    max_vel: float | None = None,
 ) -> TM2DSetup:
     """Pad the model, compile coefficients and profiles, map the points.
 
@@ -222,7 +236,8 @@ def prepare_tm2d(
     for wf, dim in zip(fields[3:], (1, 0, 0, 1)):
         zero_interior(wf, fd_pad, widths, dim)
 
-    max_vel = float((1.0 / torch.sqrt(epsilon.detach() * mu.detach())).max().item()) * C0
+    if max_vel is None:  # a caller's max_vel also sets the profiles (reference tm2d_cuda.py:124-130)
+        max_vel = float((1.0 / torch.sqrt(epsilon.detach() * mu.detach())).max().item()) * C0
     prof = staggered.set_pml_profiles(
         pml_width=widths,
         accuracy=stencil,
@@ -244,8 +259,11 @@ def prepare_tm2d(
         sources_i=sources_i, receivers_i=receivers_i, fields=fields,
         rdy=1.0 / dy, rdx=1.0 / dx, dt=float(dt), nt=nt, n_shots=n_shots,
         ny=ny, nx=nx, n_src=n_src, n_rec=n_rec, pml=pml, fd_pad=fd_pad,
-        pml_width=widths, batched=batched,
-        extra={"zero_init": all(t0 is None for t0 in initial)},
+        pml_width=widths, batched=batched, stencil=int(stencil),
+        extra={"zero_init": all(t0 is None for t0 in initial),
+               # padded models, for wavefield callbacks
+               "models": {"epsilon": eps_p.detach(), "sigma": sig_p.detach(),
+                          "mu": mu_p.detach()}},
     )
 
 
@@ -270,65 +288,189 @@ def _shots(t: torch.Tensor, s0: int, s1: int, batched: bool) -> torch.Tensor:
     return t[s0:s1] if batched else t
 
 
+class _HostStaged:
+    """Snapshots of one propagation staged outside HBM (storage_mode
+    ``"cpu"`` or ``"disk"``): per snapshot stream (Ey, curl H) a device ring
+    of three stored steps that the kernels read and write, pinned host
+    memory behind it (the whole history for ``cpu``; a three-step ring in
+    front of a file for ``disk``) and the stream the native library copies
+    on (reference ``storage.py:SnapshotAllocator``,
+    ``tm2d_helpers.py:_make_tm_storage_streams``)."""
+
+    RING = 3
+
+    def __init__(self, mode: str, want: tuple[bool, bool], n_store: int, shot_shape, sdtype,
+                 device: torch.device, path: str) -> None:
+        self.mode = STORAGE_CPU if mode == "cpu" else STORAGE_DISK
+        self.rings = [torch.empty(self.RING, *shot_shape, device=device, dtype=sdtype) if w else None
+                      for w in want]
+        rows = n_store if self.mode == STORAGE_CPU else self.RING
+        self.hosts = [torch.empty(rows, *shot_shape, dtype=sdtype, pin_memory=True) if w else None
+                      for w in want]
+        self.tmp = None
+        self.names: list[Any] = [None, None]
+        self._keep: list[Any] = []
+        if self.mode == STORAGE_DISK:
+            Path(path).mkdir(parents=True, exist_ok=True)
+            self.tmp = tempfile.TemporaryDirectory(prefix="tide_tmp_", dir=path)
+            for k, (w, stem) in enumerate(zip(want, ("ey", "curl"))):
+                if not w:
+                    continue
+                buf = ctypes.create_string_buffer(str(Path(self.tmp.name) / f"{stem}.bin").encode())
+                arr = (ctypes.c_char_p * 1)(ctypes.cast(buf, ctypes.c_char_p))
+                self._keep += [buf, arr]
+                self.names[k] = ctypes.cast(arr, ctypes.c_void_p).value
+        self.stream = torch.cuda.Stream(device)  # default priority
+
+    def args(self, k: int) -> tuple:
+        """(store_1, store_3, filenames) of stream k (0 Ey, 1 curl H)."""
+        return (_native.ptr(self.rings[k]), _native.ptr(self.hosts[k]), self.names[k])
+
+    def stream_handle(self) -> int:
+        return self.stream.cuda_stream
+
+    def close(self) -> None:
+        """Waits for the copies still in flight, then drops the buffers."""
+        self.stream.synchronize()
+        self.rings = [None, None]
+        self.hosts = [None, None]
+        if self.tmp is not None:
+            self.tmp.cleanup()
+            self.tmp = None
+
+
 def _run_forward(setup: TM2DSetup, ca, cb, cq, f, fields, r, *, interval: int, fmt: int,
-                 ey_store=None, curl_store=None) -> None:
-    """One native forward over ``setup.nt`` steps on the caller's stream;
-    stores snapshots when a store buffer is given."""
+                 ey_store=None, curl_store=None, start_t: int = 0, nt: int | None = None,
+                 staged: _HostStaged | None = None, want: tuple[bool, bool] = (False, False)) -> None:
+    """One native forward over steps ``start_t .. start_t+nt-1`` (default:
+    the whole run) on the caller's stream; stores snapshots when a store
+    buffer is given, or stages them through ``staged`` (``want``: which
+    snapshot streams)."""
     dev, dtype = ca.device, ca.dtype
+    nt = setup.nt if nt is None else nt
     stream = _native.stream_handle(dev)
     dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
     tail = (
         _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
         setup.rdy, setup.rdx, setup.dt,
-        setup.nt, setup.n_shots, setup.ny, setup.nx, setup.n_src, setup.n_rec,
+        nt, setup.n_shots, setup.ny, setup.nx, setup.n_src, setup.n_rec,
     )
-    if ey_store is not None or curl_store is not None:
-        fn = _native.get_function("maxwell_tm", "forward_with_storage", 2, dtype)
+    storing = staged is not None or ey_store is not None or curl_store is not None
+    if storing:
+        fn = _native.get_function("maxwell_tm", "forward_with_storage", setup.stencil, dtype)
         shot_bytes = setup.ny * setup.nx * (2 if fmt == STORAGE_FORMAT_BF16 else dtype.itemsize)
+        if staged is not None:
+            mode, (e1, e3, en), (c1, c3, cn) = staged.mode, staged.args(0), staged.args(1)
+            w_ca, w_cb, s_stream = want[0], want[1], staged.stream_handle()
+        else:
+            mode, e1, e3, en = STORAGE_DEVICE, _native.ptr(ey_store), None, None
+            c1, c3, cn = _native.ptr(curl_store), None, None
+            w_ca, w_cb, s_stream = ey_store is not None, curl_store is not None, stream
         status = fn(
             _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
             *[_native.ptr(t) for t in fields], _native.ptr(r),
-            _native.ptr(ey_store), None, None, _native.ptr(curl_store), None, None,
+            e1, e3, en, c1, c3, cn,
             *_profile_ptrs(setup.profiles), *tail, interval,
-            STORAGE_DEVICE, fmt, shot_bytes, ey_store is not None, curl_store is not None,
-            setup.batched, setup.batched, setup.batched, 0,
-            *setup.pml, 0, dev_idx, stream, stream,
+            mode, fmt, shot_bytes, w_ca, w_cb,
+            setup.batched, setup.batched, setup.batched, start_t,
+            *setup.pml, 0, dev_idx, stream, s_stream,
         )
     else:
-        fn = _native.get_function("maxwell_tm", "forward", 2, dtype)
+        fn = _native.get_function("maxwell_tm", "forward", setup.stencil, dtype)
         status = fn(
             _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
             *[_native.ptr(t) for t in fields],
             None, None, None, None, None, _native.ptr(r), 0,
             *_profile_ptrs(setup.profiles), *tail, interval, False,
-            setup.batched, setup.batched, setup.batched, 0,
+            setup.batched, setup.batched, setup.batched, start_t,
             *setup.pml, 0, dev_idx, stream,
         )
     _native.check(status)
 
 
 def _run_backward(setup: TM2DSetup, ca, cb, cq, grad_r, lam, ey_store, curl_store, grad_f,
-                  gca, gcb, *, interval: int, fmt: int) -> None:
-    """One native exact-adjoint sweep; ``gca``/``gcb`` are FP64 accumulators
-    (added to)."""
+                  gca, gcb, *, interval: int, fmt: int, start_t: int | None = None,
+                  nt: int | None = None, staged: _HostStaged | None = None) -> None:
+    """One native exact-adjoint sweep over steps ``start_t-1`` down to
+    ``start_t-nt`` (default: the whole run); ``gca``/``gcb`` are FP64
+    accumulators (added to)."""
     dev, dtype = ca.device, ca.dtype
+    nt = setup.nt if nt is None else nt
+    start_t = setup.nt if start_t is None else start_t
     shot_bytes = setup.ny * setup.nx * (2 if fmt == STORAGE_FORMAT_BF16 else dtype.itemsize)
     fn = getattr(_native.load(), f"tide_tm2d_{_native.dtype_name(dtype)}_backward_f64acc")
     dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    stream = _native.stream_handle(dev)
+    if staged is not None:
+        mode, (e1, e3, en), (c1, c3, cn) = staged.mode, staged.args(0), staged.args(1)
+        s_stream = staged.stream_handle()
+    else:
+        mode, e1, e3, en = STORAGE_DEVICE, _native.ptr(ey_store), None, None
+        c1, c3, cn = _native.ptr(curl_store), None, None
+        s_stream = stream
     status = fn(
         _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(grad_r),
         *[_native.ptr(t) for t in lam],
-        _native.ptr(ey_store) if gca is not None else None,
-        _native.ptr(curl_store) if gcb is not None else None,
+        e1 if gca is not None else None, e3, en,
+        c1 if gcb is not None else None, c3, cn,
         _native.ptr(grad_f), _native.ptr(gca), _native.ptr(gcb),
         *_profile_ptrs(setup.profiles),
         _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
-        setup.rdy, setup.rdx, setup.nt, setup.n_shots, setup.ny, setup.nx,
-        setup.n_src, setup.n_rec, interval, fmt, shot_bytes,
-        setup.batched, setup.batched, setup.batched, setup.nt, *setup.pml,
-        dev_idx, _native.stream_handle(dev),
+        setup.rdy, setup.rdx, nt, setup.n_shots, setup.ny, setup.nx,
+        setup.n_src, setup.n_rec, interval, mode, fmt, shot_bytes,
+        setup.batched, setup.batched, setup.batched, start_t, *setup.pml,
+        setup.stencil, dev_idx, stream, s_stream,
     )
     _native.check(status)
+
+
+def _time_chunks(nt: int, every: int | None, reverse: bool = False) -> list[tuple[int, int]]:
+    """``(start, steps)`` of the time chunks of a sweep: one chunk, or
+    chunks of ``every`` steps for callbacks (reverse: from the end, as the
+    adjoint runs, with ``start`` the chunk's upper end)."""
+    if not every or every >= nt:
+        return [(nt if reverse else 0, nt)]
+    if not reverse:
+        return [(s, min(every, nt - s)) for s in range(0, nt, every)]
+    return [(s, min(s, every)) for s in range(nt, 0, -every)]
+
+
+def _callback_state(setup: TM2DSetup, step: int, wavefields, gradients, backward: bool):
+    models = dict(setup.extra.get("models", {}))
+    models.update(ca=setup.ca, cb=setup.cb, cq=setup.cq)
+    return CallbackState(dt=setup.dt, step=step, nt=setup.nt, wavefields=wavefields,
+                         models=models, gradients=gradients, fd_pad=list(setup.fd_pad),
+                         pml_width=list(setup.pml_width), is_backward=backward)
+
+
+def _model_gradients(setup: TM2DSetup, gca, gcb) -> dict[str, torch.Tensor]:
+    """Gradients a backward callback sees: dJ/dca, dJ/dcb so far and their
+    epsilon / sigma images through the coefficient formulas (padded grid;
+    the source term and the pad fold are applied after the sweep)."""
+    out: dict[str, torch.Tensor] = {}
+    dtype = setup.ca.dtype
+    if gca is not None:
+        out["ca"] = gca.to(dtype)
+    if gcb is not None:
+        out["cb"] = gcb.to(dtype)
+    models = setup.extra.get("models")
+    if not out or not models:
+        return out
+    with torch.enable_grad():
+        eps = models["epsilon"].detach().requires_grad_(True)
+        sig = models["sigma"].detach().requires_grad_(True)
+        coef = compile_material_coefficients(eps, sig, models["mu"], setup.dt, validate_inputs=False)
+        outs, grads = [], []
+        if gca is not None:
+            outs.append(coef["ca"])
+            grads.append(out["ca"].reshape(coef["ca"].shape))
+        if gcb is not None:
+            outs.append(coef["cb"])
+            grads.append(out["cb"].reshape(coef["cb"].shape))
+        ge, gs = torch.autograd.grad(outs, (eps, sig), grads, allow_unused=True)
+    out["epsilon"] = ge if ge is not None else torch.zeros_like(eps)
+    out["sigma"] = gs if gs is not None else torch.zeros_like(sig)
+    return out
 
 
 @dataclass
@@ -391,10 +533,13 @@ def plan_shot_chunks(n_shots: int, per_shot_bytes: int, budget_bytes: float,
 def device_snapshot_budget(device: torch.device, limit: int | None, working_per_shot: int,
                            n_shots: int) -> float:
     """Bytes of snapshots one call may hold on ``device``: the caller's
-    limit, else 90% of the free memory less the working state."""
+    limit, else 90% of the free memory (device free memory plus what torch's
+    caching allocator holds unused) less the working state."""
     if limit is not None:
         return float(limit)
     free, _ = torch.cuda.mem_get_info(device)
+    # memory torch's caching allocator holds but does not use is free to it
+    free += torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
     return max(0.0, 0.9 * free - 2.0 * working_per_shot * n_shots - (256 << 20))
 
 
@@ -408,11 +553,14 @@ class TM2DPropagate(torch.autograd.Function):
     cotangent; cotangents of the final state are ignored, as in the
     reference (``tm2d_autograd.py:519-547``).
 
-    Shots run in the chunks of ``storage["plan"]``: chunks whose snapshots
-    fit the device budget keep them until the backward, the others are
-    re-run from their initial state in the backward.  One FP64 accumulator
-    per coefficient collects every chunk's gradient, cast to the field
-    dtype once at the end.
+    With device storage, shots run in the chunks of ``storage["plan"]``:
+    chunks whose snapshots fit the device budget keep them until the
+    backward, the others are re-run from their initial state in the
+    backward.  With ``cpu`` / ``disk`` storage all shots run together and
+    the native library stages their snapshots through host memory.  One
+    FP64 accumulator per coefficient collects every chunk's gradient, cast
+    to the field dtype once at the end.  Callbacks split each sweep into
+    time chunks of ``storage["callback_frequency"]`` steps.
     """
 
     @staticmethod
@@ -426,12 +574,16 @@ class TM2DPropagate(torch.autograd.Function):
         fmt = storage["format"]
         want_ca = ctx.needs_input_grad[0]
         want_cb = ctx.needs_input_grad[1]
-        store = storage["mode"] == "device" and (want_ca or want_cb)
+        mode = storage["mode"]
+        store = mode in ("device", "cpu", "disk") and (want_ca or want_cb)
+        staged_mode = store and mode in ("cpu", "disk")
         plan: ShotPlan = storage.get("plan") or ShotPlan([0, setup.n_shots], 1)
         n_store = (setup.nt + interval - 1) // interval
         sdtype = torch.int16 if fmt == STORAGE_FORMAT_BF16 else dtype
         chunked = plan.n_chunks > 1
-        stores: list[tuple[torch.Tensor | None, torch.Tensor | None] | None] = []
+        fwd_cb = storage.get("forward_callback")
+        every = storage.get("callback_frequency") if fwd_cb is not None else None
+        stores: list[Any] = []
         initial: list[list[torch.Tensor] | None] = []
         t0 = _timer.start() if _timer is not None else None
         for i in range(plan.n_chunks):
@@ -444,17 +596,28 @@ class TM2DPropagate(torch.autograd.Function):
                                [t.clone() for t in sub.fields])
             else:
                 initial.append(None)
-            ey_s = curl_s = None
-            if keep:
+            ey_s = curl_s = staged = None
+            if keep and staged_mode:
+                staged = _HostStaged(mode, (want_ca, want_cb), n_store,
+                                     (s1 - s0, setup.ny, setup.nx), sdtype, dev,
+                                     storage.get("path", "."))
+            elif keep:
                 shape = (n_store, s1 - s0, setup.ny, setup.nx)
                 ey_s = torch.empty(shape, device=dev, dtype=sdtype) if want_ca else None
                 curl_s = torch.empty(shape, device=dev, dtype=sdtype) if want_cb else None
-            stores.append((ey_s, curl_s) if keep else None)
+            stores.append(staged if staged is not None else ((ey_s, curl_s) if keep else None))
             f_i = f_c[:, s0:s1].contiguous() if (chunked and setup.n_src > 0) else f_c
             r_i = torch.zeros(setup.nt, s1 - s0, setup.n_rec, device=dev, dtype=dtype) if chunked else r
-            _run_forward(sub, _shots(ca, s0, s1, setup.batched), _shots(cb, s0, s1, setup.batched),
-                         _shots(cq, s0, s1, setup.batched), f_i, sub.fields, r_i,
-                         interval=interval, fmt=fmt, ey_store=ey_s, curl_store=curl_s)
+            for start, steps in _time_chunks(setup.nt, every):
+                _run_forward(sub, _shots(ca, s0, s1, setup.batched), _shots(cb, s0, s1, setup.batched),
+                             _shots(cq, s0, s1, setup.batched), f_i, sub.fields, r_i,
+                             interval=interval, fmt=fmt, ey_store=ey_s, curl_store=curl_s,
+                             start_t=start, nt=steps, staged=staged, want=(want_ca, want_cb))
+                if fwd_cb is not None:
+                    fwd_cb(_callback_state(setup, start + steps, dict(zip(_FIELD_NAMES, sub.fields)),
+                                           {}, False))
+            if staged is not None:
+                staged.stream.synchronize()  # the host history is complete
             if chunked and setup.n_rec > 0:
                 r[:, s0:s1].copy_(r_i)
         if t0 is not None:
@@ -487,6 +650,8 @@ class TM2DPropagate(torch.autograd.Function):
         dev, dtype = ca.device, ca.dtype
         interval = ctx.storage["interval"]
         fmt = ctx.storage["format"]
+        bwd_cb = ctx.storage.get("backward_callback")
+        every = ctx.storage.get("callback_frequency") if bwd_cb is not None else None
         if grad_r is None:
             grad_r = torch.zeros(setup.nt, setup.n_shots, setup.n_rec, device=dev, dtype=dtype)
         grad_r = grad_r.to(dtype)
@@ -510,8 +675,11 @@ class TM2DPropagate(torch.autograd.Function):
             ca_i, cb_i, cq_i = (_shots(t, s0, s1, setup.batched) for t in (ca, cb, cq))
             held = ctx.stores[i]
             ctx.stores[i] = None
-            if held is not None or not (want_ca or want_cb):
-                ey_s, curl_s = held if held is not None else (None, None)
+            staged = held if isinstance(held, _HostStaged) else None
+            ey_s = curl_s = None
+            if staged is not None or held is not None or not (want_ca or want_cb):
+                if staged is None and held is not None:
+                    ey_s, curl_s = held
                 sub = _chunk_setup(setup, s0, s1, [])
             else:
                 # re-run this chunk's forward with storage, into one buffer
@@ -540,13 +708,21 @@ class TM2DPropagate(torch.autograd.Function):
             gr_i = grad_r[:, s0:s1].contiguous() if chunked else grad_r.contiguous()
             gf_i = (torch.zeros(setup.nt, n_c, setup.n_src, device=dev, dtype=dtype)
                     if (want_f and chunked) else grad_f)
-            _run_backward(sub, ca_i, cb_i, cq_i, gr_i, lam, ey_s, curl_s, gf_i,
-                          _shots(gca, s0, s1, setup.batched) if gca is not None else None,
-                          _shots(gcb, s0, s1, setup.batched) if gcb is not None else None,
-                          interval=interval, fmt=fmt)
+            ga_i = _shots(gca, s0, s1, setup.batched) if gca is not None else None
+            gb_i = _shots(gcb, s0, s1, setup.batched) if gcb is not None else None
+            for start, steps in _time_chunks(setup.nt, every, reverse=True):
+                _run_backward(sub, ca_i, cb_i, cq_i, gr_i, lam, ey_s, curl_s, gf_i, ga_i, gb_i,
+                              interval=interval, fmt=fmt, start_t=start, nt=steps, staged=staged)
+                if bwd_cb is not None:
+                    names = ("lambda_Ey", "lambda_Hx", "lambda_Hz", "m_lambda_Ey_x",
+                             "m_lambda_Ey_z", "m_lambda_Hx_z", "m_lambda_Hz_x")
+                    bwd_cb(_callback_state(setup, start - 1, dict(zip(names, lam)),
+                                           _model_gradients(setup, gca, gcb), True))
+            if staged is not None:
+                staged.close()
             if want_f and chunked:
                 grad_f[:, s0:s1].copy_(gf_i)
-            del ey_s, curl_s, held
+            del ey_s, curl_s, held, staged
         if t0 is not None:
             _timer.stop("adjoint", setup.nt, setup.n_shots * setup.ny * setup.nx, t0,
                         _native.get_tm2d_time_block()[1])
@@ -562,10 +738,8 @@ def _reject_unsupported(
         raise NotImplementedError(
             "this build has one backend (HIP on MI355X); python_backend is not available"
         )
-    if stencil != 2:
-        raise NotImplementedError(f"stencil {stencil} is not part of this build (stencil 2 is)")
-    if forward_callback is not None or backward_callback is not None:
-        raise NotImplementedError("wavefield callbacks are not part of this build")
+    if stencil not in (2, 4, 6, 8):
+        raise ValueError(f"stencil must be 2, 4, 6 or 8, got {stencil}")
     if dispersion is not None:
         raise NotImplementedError("Debye dispersion is not part of this build")
     if device.type != "cuda":
@@ -662,6 +836,7 @@ def maxwelltm(
         storage_mode=storage_mode, storage_compression=storage_compression,
         storage_bytes_limit_device=storage_bytes_limit_device,
         storage_bytes_limit_host=storage_bytes_limit_host, dispersion=dispersion,
+        storage_path=storage_path, callback_frequency=callback_frequency,
     )
     setup, storage = call.setup, call.storage
     f = setup.f if setup.f.numel() > 0 else torch.empty(0, device=setup.ca.device, dtype=setup.ca.dtype)
@@ -696,7 +871,7 @@ def _prepare_call(
     *, stencil, pml_width, max_vel, initial, nt, model_gradient_sampling_interval,
     save_snapshots, forward_callback, backward_callback, python_backend, storage_mode,
     storage_compression, storage_bytes_limit_device, storage_bytes_limit_host, dispersion,
-    force_storage: bool = False,
+    force_storage: bool = False, storage_path: str = ".", callback_frequency: int = 1,
 ) -> _Call:
     _reject_unsupported(
         python_backend, stencil, forward_callback, backward_callback, dispersion, epsilon.device
@@ -732,7 +907,7 @@ def _prepare_call(
 
     setup = prepare_tm2d(
         epsilon, sigma, mu, spacing, dt, source_amplitude, source_location, receiver_location,
-        stencil=stencil, pml_width=pml_width, nt=nt, initial=initial,
+        stencil=stencil, pml_width=pml_width, nt=nt, initial=initial, max_vel=max_vel,
     )
     if setup.nt > 0:
         interval = min(interval, setup.nt)
@@ -752,19 +927,29 @@ def _prepare_call(
         bytes_limit_device=storage_bytes_limit_device,
         bytes_limit_host=storage_bytes_limit_host,
     )
-    if mode in ("cpu", "disk"):
-        raise NotImplementedError(
-            f"storage_mode='{mode}' (host-staged snapshots) is not part of this build; "
-            "use 'device' and split the shots into chunks that fit in HBM"
-        )
     plan = ShotPlan([0, setup.n_shots], 1)
-    if mode == "device" and setup.nt > 0:
+    storing = do_save and model_rg
+    # Device snapshots are split into shot chunks that fit the device budget;
+    # host-staged modes (cpu, disk) keep one device ring per stream and run
+    # all shots together.
+    if storing and mode == "device" and setup.nt > 0:
         working = 21 * setup.ny * setup.nx * epsilon.dtype.itemsize
         budget = device_snapshot_budget(epsilon.device, storage_bytes_limit_device, working,
                                         setup.n_shots)
         plan = plan_shot_chunks(setup.n_shots, per_shot, budget,
                                 group=max(1, _native.get_tm2d_tuning()[3]))
-    storage = {"mode": mode, "format": fmt, "interval": interval, "kind": kind, "plan": plan}
+    if (forward_callback is not None or backward_callback is not None) and plan.n_chunks > 1:
+        raise RuntimeError(
+            f"callbacks see every shot at once, but the snapshots of {setup.n_shots} shots do "
+            "not fit the device budget together; use storage_mode='cpu' or 'disk', BF16 "
+            "compression or a larger storage_bytes_limit_device"
+        )
+    freq = int(callback_frequency)
+    if (forward_callback is not None or backward_callback is not None) and freq < 1:
+        raise ValueError("callback_frequency must be a positive number of steps")
+    storage = {"mode": mode, "format": fmt, "interval": interval, "kind": kind, "plan": plan,
+               "path": storage_path, "forward_callback": forward_callback,
+               "backward_callback": backward_callback, "callback_frequency": freq}
     return _Call(setup, storage, bs)
 
 

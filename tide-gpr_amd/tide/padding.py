@@ -1,14 +1,54 @@
 """Model and wavefield padding (reference ``src/tide/padding.py``).
 
 Models are replicate-padded into the CPML and finite-difference halo, so the
-gradient of a padded cell folds back onto the nearest edge cell through
-``F.pad``'s backward; wavefields are zero-padded.  Pads are given per side as
-``[y0, y1, x0, x1]`` (2-D) or ``[z0, z1, y0, y1, x0, x1]`` (3-D).
+gradient of a padded cell folds back onto the nearest edge cell; wavefields
+are zero-padded.  Pads are given per side as ``[y0, y1, x0, x1]`` (2-D) or
+``[z0, z1, y0, y1, x0, x1]`` (3-D).
+
+The fold is done here with slices and reductions rather than by
+``F.pad``'s backward, which adds with atomics on the GPU and so is not the
+same from run to run; with this fold the operators' gradients are bitwise
+repeatable.
 """
 
 from __future__ import annotations
 
 import torch
+
+
+def fold_replicate(g_p: torch.Tensor, pad: list[int], spatial: int) -> torch.Tensor:
+    """Adjoint of replicate padding of the last ``spatial`` dimensions by
+    ``pad`` (``[lo, hi]`` per dimension, first dimension first): each padded
+    cell's gradient is added to the edge cell it copies, the slab of each
+    side summed by one reduction (fixed order, no atomics)."""
+    out = g_p
+    for d in range(spatial):
+        lo, hi = int(pad[2 * d]), int(pad[2 * d + 1])
+        if lo == 0 and hi == 0:
+            continue
+        axis = out.ndim - spatial + d
+        n = out.shape[axis] - lo - hi
+        center = out.narrow(axis, lo, n).clone()
+        if lo:
+            center.narrow(axis, 0, 1).add_(out.narrow(axis, 0, lo).sum(axis, keepdim=True))
+        if hi:
+            center.narrow(axis, n - 1, 1).add_(out.narrow(axis, lo + n, hi).sum(axis, keepdim=True))
+        out = center
+    return out
+
+
+class _ReplicatePad(torch.autograd.Function):
+    """Replicate padding with the deterministic fold as its backward."""
+
+    @staticmethod
+    def forward(ctx, tensor, pad_rev, pad, spatial):
+        ctx.pad = pad
+        ctx.spatial = spatial
+        return torch.nn.functional.pad(tensor, pad_rev, mode="replicate")
+
+    @staticmethod
+    def backward(ctx, grad):
+        return fold_replicate(grad, ctx.pad, ctx.spatial), None, None, None
 
 
 def reverse_pad(pad: list[int]) -> list[int]:
@@ -37,7 +77,10 @@ def create_or_pad(
     # Non-constant modes need a leading batch/channel dimension.
     squeeze = mode != "constant" and tensor.ndim in (spatial, spatial + 1)
     src = tensor.unsqueeze(0) if squeeze else tensor
-    out = torch.nn.functional.pad(src, reverse_pad(pad), mode=mode)
+    if mode == "replicate":
+        out = _ReplicatePad.apply(src, reverse_pad(pad), list(pad), spatial)
+    else:
+        out = torch.nn.functional.pad(src, reverse_pad(pad), mode=mode)
     return out.squeeze(0) if squeeze else out
 
 
@@ -66,4 +109,4 @@ def zero_interior(
     return tensor
 
 
-__all__ = ["create_or_pad", "reverse_pad", "zero_interior"]
+__all__ = ["create_or_pad", "fold_replicate", "reverse_pad", "zero_interior"]

@@ -45,6 +45,7 @@ from . import _native
 from .batch import _Problem
 from .cfl import cfl_condition
 from .grid_utils import _normalize_grid_spacing_2d, _normalize_pml_width_2d
+from .padding import fold_replicate
 from .maxwell.tm2d import prepare_tm2d
 from .utils import C0, coefficient_partials
 
@@ -275,22 +276,10 @@ def model_gradient(
 
 
 def _fold_replicate(g_p: torch.Tensor, pad: Sequence[int], shape: tuple[int, int]) -> torch.Tensor:
-    """Adjoint of replicate padding by ``pad = [top, bottom, left, right]``:
-    each padded cell's gradient goes to the edge cell it copies.  Built from
-    slices and reductions (no atomics), so the result is the same on every
-    run, unlike torch's replicate-pad backward."""
-    top, bottom, left, right = (int(v) for v in pad)
-    ny, nx = shape
-    cols = g_p[:, left : left + nx].clone()
-    if left:
-        cols[:, 0] += g_p[:, :left].sum(1)
-    if right:
-        cols[:, nx - 1] += g_p[:, left + nx :].sum(1)
-    out = cols[top : top + ny].clone()
-    if top:
-        out[0] += cols[:top].sum(0)
-    if bottom:
-        out[ny - 1] += cols[top + ny :].sum(0)
+    """Adjoint of replicate padding by ``pad = [top, bottom, left, right]``
+    (deterministic, see ``tide.padding.fold_replicate``)."""
+    out = fold_replicate(g_p, [int(v) for v in pad], 2)
+    assert tuple(out.shape[-2:]) == tuple(shape)
     return out
 
 
