@@ -651,6 +651,35 @@ int maxwell_tm_8_double_born_forward_cuda(
     int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
 
 /*
+ * Allocation-free variant of maxwell_tm_2_float_forward_cuda: the same
+ * arguments and results, with the call's scratch (ping-pong copies of the
+ * fields and CPML memories, packed profiles, point tables) in a fixed
+ * 64 MiB device array of the library instead of stream-ordered
+ * allocations; no hipMalloc / hipFree or allocating library routine is
+ * called.  A call whose scratch exceeds the array (7 x n_shots x ny x nx
+ * fp32 values plus the tables) returns status 1 before any device work.
+ * Each device holds one array for the life of the process: calls that
+ * overlap on one device (other streams or threads) race on it.  No
+ * reference counterpart; for callers that cannot allocate (code under
+ * graph capture, tight serving loops, harnesses that forbid it).
+ */
+int tide_tm2d_float_forward_noalloc(
+    float const *ca, float const *cb, float const *cq, float const *f,
+    float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float const *debye_a, float const *debye_b,
+    float const *debye_cp, float *polarization, float *ey_prev, float *r,
+    int64_t n_poles, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i,
+    float rdy, float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool has_dispersion, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,
+    void *compute_stream);
+
+/*
  * Same adjoint as maxwell_tm_2_{dtype}_backward_cuda, with grad_ca / grad_cb
  * as FP64 accumulators that are added to (so a caller running shot chunks
  * keeps one FP64 sum across them).  The six storage arguments and

@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1115,6 +1116,82 @@ TIDE_TM2D_BORN_ENTRY(6, float, float)
 TIDE_TM2D_BORN_ENTRY(6, double, double)
 TIDE_TM2D_BORN_ENTRY(8, float, float)
 TIDE_TM2D_BORN_ENTRY(8, double, double)
+
+// ------------------------------------------- allocation-free forward entry
+// The forward's scratch (the ping-pong copies of the fields and CPML
+// memories, the packed profiles and the point tables) lives in this fixed
+// device array instead of stream-ordered allocations.  Each device holds its
+// own copy for the life of the process; its address on a device comes from
+// hipGetSymbolAddress, taken once per device with that device current (the
+// array's name in host code is not a device address).
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<char *> g_scratch_ptr;  // per device; null until looked up
+
+// The device address of `scratch` on `device` (the current device), looked
+// up on the first call for that device.  Null (with the error set) on failure.
+char *scratch_on(int device, size_t *cap) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0) {
+    set_error("invalid device index");
+    return nullptr;
+  }
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (g_scratch_ptr[(size_t)device] == nullptr) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+      (void)hipGetLastError();
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[(size_t)device] = (char *)ptr;
+  }
+  *cap = kNoAllocScratchBytes;
+  return g_scratch_ptr[(size_t)device];
+}
+}  // namespace
+
+extern "C" int tide_tm2d_float_forward_noalloc(
+    float const *ca, float const *cb, float const *cq, float const *f, float *ey, float *hx,
+    float *hz, float *m_ey_x, float *m_ey_z, float *m_hx_z, float *m_hz_x,
+    float const *debye_a, float const *debye_b, float const *debye_cp, float *polarization,
+    float *ey_prev, float *r, int64_t n_poles, float const *ay, float const *by,
+    float const *ayh, float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy, float rdx, float dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src, int64_t n_rec,
+    int64_t step_ratio, bool has_dispersion, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1, int64_t n_threads,
+    int64_t device, void *compute_stream) {
+  (void)debye_a; (void)debye_b; (void)debye_cp; (void)polarization;
+  (void)ey_prev; (void)n_poles; (void)dt; (void)n_threads;
+  if (has_dispersion) {
+    set_error("Debye dispersion is not part of this build");
+    return kUnsupported;
+  }
+  // The field copies dominate the scratch; checked before any device work
+  // (the remaining pieces are checked as they are placed, also before any
+  // launch).
+  if (n_shots > 0 && ny > 0 && nx > 0 &&
+      (size_t)7 * (size_t)n_shots * (size_t)ny * (size_t)nx * sizeof(float) +
+              (size_t)12 * (size_t)(ny + nx) * sizeof(float) >
+          kNoAllocScratchBytes) {
+    set_error("tide_tm2d_float_forward_noalloc: 7 x n_shots x ny x nx fp32 scratch values exceed "
+              "the fixed " + std::to_string(kNoAllocScratchBytes >> 20) +
+              " MiB scratch array; use maxwell_tm_2_float_forward_cuda");
+    return kInvalidArgument;
+  }
+  return forward_entry<float>(ca, cb, cq, f, ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x, r, ay,
+                              by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh, sources_i,
+                              receivers_i, rdy, rdx, nt, n_shots, ny, nx, n_src, n_rec,
+                              step_ratio, ca_b, cb_b, cq_b, start_t, pml_y0, pml_x0, pml_y1,
+                              pml_x1, device, compute_stream, nullptr, nullptr, 0, 0,
+                              scratch_on);
+}
 
 // FP64-accumulator backward used by the Python operator (and any caller that
 // runs shot chunks and wants one FP64 sum across them); see tide_hip.h.

@@ -252,6 +252,9 @@ def load() -> ctypes.CDLL:
             fn = getattr(lib, f"maxwell_tm_{stencil}_{name}_born_forward_cuda")
             fn.restype = ctypes.c_int
             fn.argtypes = _expand_kinds("P37 F3 I7 B3 I1 I4 I1 I1 P1", ctype)
+    # the allocation-free variant of maxwell_tm_2_float_forward_cuda
+    lib.tide_tm2d_float_forward_noalloc.restype = ctypes.c_int
+    lib.tide_tm2d_float_forward_noalloc.argtypes = _tm_forward_types(ctypes.c_float)
     for name, ctype in (("float", ctypes.c_float), ("double", ctypes.c_double)):
         fn = getattr(lib, f"tide_em3d_{name}_forward")
         fn.restype = ctypes.c_int
