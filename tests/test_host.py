@@ -313,11 +313,16 @@ def test_autotune_size_classes():
     for cands, rows in ((FWD_CANDIDATES, (2, 4, 8, 16)), (ADJ_CANDIDATES, (2, 4, 8))):
         for r, g, lp, steps, split, border in cands:
             assert r in rows and 1 <= g <= (4 if lp else 8) and lp in (0, 1)
-            assert steps in (1, 2) and split in (0, 1) and border in (0, 1, 2)
-            assert border == 0 or steps == 2  # only two-step launches have border tiles
-            assert not (lp and steps == 2)  # two-step launches group shots per wavefront
-            if steps == 2:
+            assert steps in (1, 2, 3, 4) and split in (0, 1) and border in (0, 1, 2)
+            assert border == 0 or steps >= 2  # only blocked launches have border tiles
+            assert not (lp and steps >= 2)  # blocked launches group shots per wavefront
+            if steps >= 2:
                 assert r <= 8
+            if steps >= 3:
+                # K-step launches: ordered tiles only, border tiles of 2 rows or
+                # the interior height, 8-row tiles only in the fp32 forward
+                assert split == 0 and border in (0, 2) and r in ((2, 4, 8) if cands is
+                                                                 FWD_CANDIDATES else (2, 4))
 
 
 def test_autotune_rejects_bad_chunk_lists_before_touching_a_device():

@@ -290,7 +290,8 @@ def shot_loop(fwd, adj):
 
 
 @pytest.mark.parametrize("shots,loop,steps", [(1, 0, 1), (5, 0, 1), (12, 0, 1), (5, 1, 1), (12, 1, 1),
-                                              (5, 0, 2), (6, 0, 2), (12, 0, 2)])
+                                              (5, 0, 2), (6, 0, 2), (12, 0, 2), (5, 0, 3),
+                                              (12, 0, 3), (6, 0, 4)])
 def test_shot_groups_match_oracle(shots, loop, steps):
     """One workgroup holds up to 8 shots; 12 shots exercise several groups
     (each group sums into its own FP64 slice, folded afterwards; groups are
@@ -360,6 +361,11 @@ LAUNCH_SHAPES = [
     ((4, 4, 2, 1), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
     ((4, 4, 4, 4), (0, 0), (2, 2), (0, 1), "tm2d_hetero_f64"),
     ((2, 2, 2, 4), (0, 0), (2, 2), (0, 1), "tm2d_config1_f32"),
+    # three and four steps per launch (ordered tiles)
+    ((4, 1, 4, 4), (0, 0), (2, 3), (0, 0), "tm2d_hetero_f32"),
+    ((8, 2, 2, 4), (0, 0), (3, 4), (0, 0), "tm2d_hetero_f32"),
+    ((4, 4, 4, 2), (0, 0), (4, 3), (0, 0), "tm2d_config1_f32"),
+    ((4, 2, 4, 4), (0, 0), (3, 3), (0, 0), "tm2d_hetero_f64"),
     # interior and border tiles as one kernel or three (library default
     # border tile heights; the test below sweeps them)
     ((4, 2, 4, 4), (0, 0), (2, 2), (1, 0), "tm2d_hetero_f32"),

@@ -518,8 +518,9 @@ int build_plan(Plan *p, const tide_tm2d_problem *problems, int64_t n, int64_t st
   p->rows_f = rows_for<float>(g_tuning.fwd_rows);
   p->rows_a = rows_for<float>(g_tuning.adj_rows);
   p->group = g_tuning.adj_group;  // one group size for both passes
-  p->steps_f = g_tuning.fwd_steps == 2 ? 2 : 1;
-  p->steps_a = g_tuning.adj_steps == 2 ? 2 : 1;
+  // plans run one- or two-step launches (a K-step setting runs at 2)
+  p->steps_f = g_tuning.fwd_steps >= 2 ? 2 : 1;
+  p->steps_a = g_tuning.adj_steps >= 2 ? 2 : 1;
   const bool two = p->steps_f == 2 || p->steps_a == 2;
   p->probs.assign(problems, problems + n);
   std::stable_sort(p->probs.begin(), p->probs.end(),

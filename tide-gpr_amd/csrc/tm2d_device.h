@@ -685,13 +685,17 @@ constexpr int kMaxGroup = 8;  // shots (wavefronts) per workgroup
 // (twice the registers per value).
 // fwd_loop / adj_loop select the shot-looping kernels; their group is then
 // the number of tiles (wavefronts) per workgroup.
-// fwd_steps / adj_steps (1 or 2) are the time steps per launch: 2 selects
-// the two-step kernels (tm2d_fused.h) for pairs of steps; fwd_split /
-// adj_split run their interior and border tiles as two kernels.
+// fwd_steps / adj_steps (1 to 4) are the time steps per launch: 2 selects
+// the two-step kernels (tm2d_fused.h) for pairs of steps, 3 and 4 the K-step
+// kernels (tm2d_blockk.h); fwd_split / adj_split run the two-step kernels'
+// interior and border tiles as two kernels.  On a 1065^2 grid with 4 or 7
+// shots per call the forward is fastest at 2 steps per launch (20.8 us per
+// step against 22+ at 3 or 4) and the adjoint at 3 (32.2 against 33.6 at
+// 2 and 40.5 at 4).
 struct Tuning {
   int fwd_rows = 4, fwd_group = 1, adj_rows = 4, adj_group = 4;
   int fwd_loop = 0, adj_loop = 0;
-  int fwd_steps = 2, adj_steps = 2;
+  int fwd_steps = 2, adj_steps = 3;
   int fwd_split = 0, adj_split = 0;
   // Border tile height of the two-step kernels: 1, 2 or 0 (the interior
   // height).  The forward is fastest with interior-height border tiles, the

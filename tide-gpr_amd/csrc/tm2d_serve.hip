@@ -702,7 +702,7 @@ extern "C" int tide_tm2d_engine_create(int64_t device, int64_t group, int64_t ma
   e->device = (int)device;
   e->group = (int)group;
   pick_rows(rows_for<float>(g_tuning.fwd_rows), rows_for<float>(g_tuning.adj_rows), &e->rows_f, &e->rows_a);
-  e->two_step = g_tuning.fwd_steps == 2 && g_tuning.adj_steps == 2;
+  e->two_step = g_tuning.fwd_steps >= 2 && g_tuning.adj_steps >= 2;  // K-step settings run at 2
   e->max_jobs = max_jobs;
   hipError_t err = hipMalloc((void **)&e->d_items, (size_t)max_jobs * sizeof(ServeItem));
   if (err == hipSuccess) err = hipMalloc((void **)&e->d_start, (size_t)(max_jobs + 1) * sizeof(int32_t));

@@ -271,6 +271,10 @@ def load() -> ctypes.CDLL:
     lib.tide_em3d_set_tuning.argtypes = [ctypes.c_int] * 3
     lib.tide_em3d_set_forward_group.restype = ctypes.c_int
     lib.tide_em3d_set_forward_group.argtypes = [ctypes.c_int]
+    lib.tide_em3d_set_fused.restype = ctypes.c_int
+    lib.tide_em3d_set_fused.argtypes = [ctypes.c_int] * 2
+    lib.tide_em3d_get_fused.restype = ctypes.c_int
+    lib.tide_em3d_get_fused.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.tide_tm2d_set_tuning.restype = ctypes.c_int
     lib.tide_tm2d_set_tuning.argtypes = [ctypes.c_int] * 4
     lib.tide_tm2d_get_tuning.restype = ctypes.c_int
@@ -305,6 +309,26 @@ def check(status: int) -> None:
     if status != 0:
         msg = load().tide_last_error().decode(errors="replace")
         raise TideNativeError(f"HIP propagator failed (status {status}): {msg}")
+
+
+OUT_OF_MEMORY = 4  # kOutOfMemory of the C ABI
+
+
+def call(fn, *args) -> None:
+    """Runs a propagator entry point and raises on a failed status.
+
+    The library allocates its scratch from HIP's stream-ordered pool, which
+    cannot use blocks that torch's caching allocator holds unused: on an
+    out-of-memory status the call runs once more after those blocks are
+    released.  The entry points allocate all scratch before they write any
+    caller array, so the retried call starts from the same state."""
+    status = fn(*args)
+    if (status == OUT_OF_MEMORY and torch.cuda.is_initialized()
+            and torch.cuda.memory_reserved() > torch.cuda.memory_allocated()):
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        status = fn(*args)
+    check(status)
 
 
 def dtype_name(dtype: torch.dtype) -> str:
@@ -350,6 +374,19 @@ def set_em3d_tuning(fwd_z: int, adj_z: int, group: int) -> None:
 def set_em3d_forward_group(group: int) -> None:
     """Shots per workgroup of the 3-D forward kernels alone (see include/tide_hip.h)."""
     check(load().tide_em3d_set_forward_group(int(group)))
+
+
+def set_em3d_fused(rows: int, planes: int = 8) -> None:
+    """One-launch 3-D forward step: rows per wavefront (2 or 4; 0 runs the
+    two-kernel step) and z planes per march (8 or 16)."""
+    check(load().tide_em3d_set_fused(int(rows), int(planes)))
+
+
+def get_em3d_fused() -> tuple[int, int]:
+    """(rows, planes) of the one-launch 3-D forward step (rows 0: off)."""
+    out = (ctypes.c_int * 2)()
+    check(load().tide_em3d_get_fused(out))
+    return int(out[0]), int(out[1])
 
 
 def ptr_array(tensors) -> ctypes.Array:
@@ -433,9 +470,12 @@ def stream_handle(device: torch.device) -> int | None:
 
 
 __all__ = [
+    "OUT_OF_MEMORY",
     "TideNativeError",
+    "call",
     "check",
     "dtype_name",
+    "get_em3d_fused",
     "get_tm2d_border_rows",
     "get_tm2d_shot_loop",
     "get_tm2d_tile_split",
@@ -449,6 +489,7 @@ __all__ = [
     "set_tm2d_loop_timing",
     "set_tm2d_shot_loop",
     "set_tm2d_tile_split",
+    "set_em3d_fused",
     "set_tm2d_time_block",
     "set_tm2d_tuning",
     "get_function",

@@ -8,9 +8,10 @@ launch parameters:
   shots and summing the FP64 gradient in LDS; the shot-loop family gives
   each wavefront one tile and walks every shot of the call (G is then tiles
   per workgroup);
-* time steps per launch: 1, or 2 (two-step kernels that recompute a halo of
-  two and touch the wavefield state once per two steps), the latter with
-  the interior and border tiles as one kernel or two;
+* time steps per launch: 1, 2 (two-step kernels that recompute a halo of
+  two and touch the wavefield state once per two steps, the interior and
+  border tiles as one kernel or two) or K = 3, 4 (a halo of K, border
+  tiles first in one kernel; host-staged snapshots run at 2);
 * rows marched per wavefront (the tile height), and for two-step launches
   the height of the border tiles (0: the interior height, or 1 or 2 rows).
 
@@ -39,13 +40,13 @@ FWD_CANDIDATES: tuple[tuple[int, int, int, int, int, int], ...] = (
     (16, 4, 0, 1, 0, 0), (2, 4, 1, 1, 0, 0), (4, 2, 1, 1, 0, 0),
     (2, 2, 0, 2, 0, 0), (4, 1, 0, 2, 0, 0), (4, 2, 0, 2, 0, 0), (4, 4, 0, 2, 0, 0),
     (8, 2, 0, 2, 0, 0), (4, 1, 0, 2, 0, 2), (4, 2, 0, 2, 0, 2),
-    (4, 1, 0, 2, 1, 0), (4, 2, 0, 2, 1, 0))
+    (4, 1, 0, 2, 1, 0), (4, 2, 0, 2, 1, 0), (4, 2, 0, 3, 0, 0), (8, 2, 0, 3, 0, 0))
 ADJ_CANDIDATES: tuple[tuple[int, int, int, int, int, int], ...] = (
     (2, 2, 0, 1, 0, 0), (2, 4, 0, 1, 0, 0), (2, 8, 0, 1, 0, 0), (4, 4, 0, 1, 0, 0),
     (8, 2, 0, 1, 0, 0), (2, 4, 1, 1, 0, 0), (4, 2, 1, 1, 0, 0),
     (4, 4, 0, 2, 1, 0), (4, 8, 0, 2, 1, 0), (4, 4, 0, 2, 1, 2), (4, 8, 0, 2, 1, 2),
     (2, 4, 0, 2, 0, 0), (4, 2, 0, 2, 0, 0), (4, 4, 0, 2, 0, 0), (4, 8, 0, 2, 0, 0),
-    (4, 4, 0, 2, 0, 2), (4, 8, 0, 2, 0, 2))
+    (4, 4, 0, 2, 0, 2), (4, 8, 0, 2, 0, 2), (4, 4, 0, 3, 0, 2), (4, 2, 0, 3, 0, 2))
 
 _cache: dict[tuple, tuple[tuple[int, ...], tuple[int, ...]]] = {}
 
@@ -243,7 +244,7 @@ def tune_tm2d(
     _cache[key] = (best_f, best_a)
 
     def label(k):
-        return (f"{k[0]}x{k[1]}{'L' if k[2] else ''}{'/2' if k[3] == 2 else ''}"
+        return (f"{k[0]}x{k[1]}{'L' if k[2] else ''}{f'/{k[3]}' if k[3] > 1 else ''}"
                 f"{'s' if k[4] else ''}{f'b{k[5]}' if k[5] else ''}")
 
     return {

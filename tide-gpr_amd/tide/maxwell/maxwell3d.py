@@ -246,7 +246,8 @@ def _call_forward(setup: EM3DSetup, ca, cb, cq, f, fields, r, *, start_t: int, n
     bpe = 2 if fmt == STORAGE_FORMAT_BF16 else dtype.itemsize
     step_bytes = setup.n_shots * numel * bpe
     dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
-    status = fn(
+    _native.call(
+        fn,
         _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
         _native.ptr_array(fields), _native.ptr(r), _native.ptr_array(setup.profiles),
         _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
@@ -257,7 +258,6 @@ def _call_forward(setup: EM3DSetup, ca, cb, cq, f, fields, r, *, start_t: int, n
         _native.ptr_array(stores if stores is not None else [None] * 6), fmt, step_bytes,
         store_t0, dev_idx, _native.stream_handle(dev),
     )
-    _native.check(status)
 
 
 def _call_backward(setup: EM3DSetup, ca, cb, cq, grad_r, lam, stores, *, fmt: int,
@@ -267,7 +267,8 @@ def _call_backward(setup: EM3DSetup, ca, cb, cq, grad_r, lam, stores, *, fmt: in
     numel = setup.nz * setup.ny * setup.nx
     bpe = 2 if fmt == STORAGE_FORMAT_BF16 else dtype.itemsize
     dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
-    status = fn(
+    _native.call(
+        fn,
         _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(grad_r),
         _native.ptr_array(lam), _native.ptr_array(stores if stores is not None else [None] * 6),
         fmt, setup.n_shots * numel * bpe, store_t0, _native.ptr(grad_f), _native.ptr(gca),
@@ -277,7 +278,6 @@ def _call_backward(setup: EM3DSetup, ca, cb, cq, grad_r, lam, stores, *, fmt: in
         setup.batched, setup.batched, start_t, _native.int64_array(setup.pml),
         setup.source_component, setup.receiver_component, dev_idx, _native.stream_handle(dev),
     )
-    _native.check(status)
 
 
 def _alloc_stores(setup: EM3DSetup, n_slots: int, fmt: int, dtype, device, want_ca, want_cb):

@@ -147,6 +147,7 @@ class Maxwell3D:
             storage_compression=self.storage.compression,
             storage_bytes_limit_device=self.storage.bytes_limit_device,
             storage_bytes_limit_host=self.storage.bytes_limit_host,
+            storage_chunk_steps=self.storage.chunk_steps,
             dispersion=model.dispersion,
         )
         return ForwardResult(receiver_data=out[-1], final_state=EM3DState.from_tensors(tuple(out[:-1])))

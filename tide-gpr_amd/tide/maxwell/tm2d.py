@@ -366,7 +366,8 @@ def _run_forward(setup: TM2DSetup, ca, cb, cq, f, fields, r, *, interval: int, f
             mode, e1, e3, en = STORAGE_DEVICE, _native.ptr(ey_store), None, None
             c1, c3, cn = _native.ptr(curl_store), None, None
             w_ca, w_cb, s_stream = ey_store is not None, curl_store is not None, stream
-        status = fn(
+        _native.call(
+            fn,
             _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
             *[_native.ptr(t) for t in fields], _native.ptr(r),
             e1, e3, en, c1, c3, cn,
@@ -377,7 +378,8 @@ def _run_forward(setup: TM2DSetup, ca, cb, cq, f, fields, r, *, interval: int, f
         )
     else:
         fn = _native.get_function("maxwell_tm", "forward", setup.stencil, dtype)
-        status = fn(
+        _native.call(
+            fn,
             _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
             *[_native.ptr(t) for t in fields],
             None, None, None, None, None, _native.ptr(r), 0,
@@ -385,7 +387,6 @@ def _run_forward(setup: TM2DSetup, ca, cb, cq, f, fields, r, *, interval: int, f
             setup.batched, setup.batched, setup.batched, start_t,
             *setup.pml, 0, dev_idx, stream,
         )
-    _native.check(status)
 
 
 def _run_backward(setup: TM2DSetup, ca, cb, cq, grad_r, lam, ey_store, curl_store, grad_f,
@@ -408,7 +409,8 @@ def _run_backward(setup: TM2DSetup, ca, cb, cq, grad_r, lam, ey_store, curl_stor
         mode, e1, e3, en = STORAGE_DEVICE, _native.ptr(ey_store), None, None
         c1, c3, cn = _native.ptr(curl_store), None, None
         s_stream = stream
-    status = fn(
+    _native.call(
+        fn,
         _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(grad_r),
         *[_native.ptr(t) for t in lam],
         e1 if gca is not None else None, e3, en,
@@ -421,7 +423,6 @@ def _run_backward(setup: TM2DSetup, ca, cb, cq, grad_r, lam, ey_store, curl_stor
         setup.batched, setup.batched, setup.batched, start_t, *setup.pml,
         setup.stencil, dev_idx, stream, s_stream,
     )
-    _native.check(status)
 
 
 def _time_chunks(nt: int, every: int | None, reverse: bool = False) -> list[tuple[int, int]]:
