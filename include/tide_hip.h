@@ -1163,6 +1163,17 @@ int tide_em3d_set_tuning(int fwd_z, int adj_z, int group);
  */
 int tide_em3d_set_forward_group(int group);
 
+/*
+ * One-launch 3-D forward step (the H and E updates of a time step in one
+ * kernel, fields and H-phase CPML memories ping-ponged through a second
+ * copy): rows per wavefront 2 or 4 (0: the two-kernel step) and z planes
+ * marched per wavefront 8 or 16.  Results are bit-identical to the
+ * two-kernel step.  tide_em3d_get_fused writes {rows, planes}.  No
+ * reference counterpart.
+ */
+int tide_em3d_set_fused(int rows, int planes);
+int tide_em3d_get_fused(int *out2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -869,6 +869,14 @@ __global__ void fold_groups3(double *__restrict__ out, const double *__restrict_
   out[i] += sum;
 }
 
+}  // namespace em3d
+}  // namespace tide_hip
+
+#include "em3d_fused.h"
+
+namespace tide_hip {
+namespace em3d {
+
 // ------------------------------------------------------------------ driver
 
 struct Tuning3 {
@@ -877,6 +885,9 @@ struct Tuning3 {
   // (6% faster there than 8), the adjoint sums 8 shots' gradient terms per
   // workgroup
   int fwd_z = 8, adj_z = 4, group = 8, fwd_group = 1;
+  // one-launch forward step (em3d_fused.h): rows per wavefront (2 or 4; 0
+  // runs the two-kernel step) and planes per march (8 or 16)
+  int fused_rows = 0, fused_z = 8;
 };
 Tuning3 g_tuning3;
 
@@ -1023,6 +1034,95 @@ void launch_adj_h3(int tz, dim3 grid, dim3 block, size_t lds, hipStream_t st,
   }
 }
 
+template <typename T, int FMT>
+void launch_fused(int ty, int tz, dim3 grid, dim3 block, hipStream_t st, const Fused3<T> &a) {
+  if (ty >= 4) {
+    if (tz >= 16) em3_fused<T, 4, 16, FMT><<<grid, block, 0, st>>>(a);
+    else em3_fused<T, 4, 8, FMT><<<grid, block, 0, st>>>(a);
+  } else {
+    if (tz >= 16) em3_fused<T, 2, 16, FMT><<<grid, block, 0, st>>>(a);
+    else em3_fused<T, 2, 8, FMT><<<grid, block, 0, st>>>(a);
+  }
+}
+
+// The forward time loop on the one-launch step: the fields and the H-phase
+// memories ping-pong between the caller's arrays and a second copy (both
+// start equal, so cells a step never writes agree in both); the final state
+// ends in the caller's arrays.
+template <typename T>
+int forward_fused(const Problem3<T> &p, T *const *fields, const T *f, T *r, int64_t nt,
+                  int64_t start_t, int64_t step_ratio, void *const *stores, int fmt,
+                  size_t store_step_bytes, int64_t store_t0, hipStream_t stream, Scratch &scr,
+                  const DevicePointTable &dpt) {
+  static constexpr int kHMem[6] = {M_EY_Z, M_EX_Z, M_EZ_Y, M_EX_Y, M_EZ_X, M_EY_X};
+  static constexpr int kEMem[6] = {M_HY_Z, M_HX_Z, M_HZ_Y, M_HX_Y, M_HZ_X, M_HY_X};
+  const int64_t total = (int64_t)p.n_shots * p.g.numel;
+  const size_t bytes = (size_t)total * sizeof(T);
+  T *fa[6], *fb[6], *ma[6], *mb[6];
+  for (int k = 0; k < 6; ++k) {
+    fa[k] = fields[k];
+    ma[k] = fields[kHMem[k]];
+    TIDE_CHECK(scr.alloc(&fb[k], (size_t)total));
+    TIDE_CHECK(scr.alloc(&mb[k], (size_t)total));
+    TIDE_HIP_TRY(hipMemcpyAsync(fb[k], fa[k], bytes, hipMemcpyDeviceToDevice, stream));
+    TIDE_HIP_TRY(hipMemcpyAsync(mb[k], ma[k], bytes, hipMemcpyDeviceToDevice, stream));
+  }
+  const int ty = g_tuning3.fused_rows >= 4 ? 4 : 2;
+  const int tz = g_tuning3.fused_z >= 16 ? 16 : 8;
+  Fused3<T> a;
+  a.g = p.g;
+  a.pts = Points{dpt.row_slot, dpt.seg, dpt.ent, dpt.n_xt};
+  a.ca = p.ca;
+  a.cb = p.cb;
+  a.cq = p.cq;
+  a.n_shots = p.n_shots;
+  a.n_yt = (p.g.ny + ty - 1) / ty;
+  a.n_zb = (p.g.nz + tz - 1) / tz;
+  a.src_comp = p.src_comp;
+  a.rec_comp = p.rec_comp;
+  for (int k = 0; k < 6; ++k) a.me[k] = fields[kEMem[k]];
+  const int group = std::min<int>(p.n_shots, g_tuning3.fwd_group);
+  const int64_t n_groups = (p.n_shots + group - 1) / group;
+  const dim3 block(64, (unsigned)group);
+  const dim3 grid((unsigned)((int64_t)dpt.n_xt * a.n_yt * a.n_zb * n_groups));
+  bool storing = false;
+  for (int k = 0; k < 6; ++k) storing = storing || (stores && stores[k]);
+  for (int64_t t = start_t; t < start_t + nt; ++t) {
+    for (int k = 0; k < 6; ++k) {
+      a.fi[k] = fa[k];
+      a.fo[k] = fb[k];
+      a.mhi[k] = ma[k];
+      a.mho[k] = mb[k];
+    }
+    a.src_amp = p.n_src > 0 ? f + t * p.n_shots * p.n_src : nullptr;
+    a.rec_out = p.n_rec > 0 ? r + t * p.n_shots * p.n_rec : nullptr;
+    const bool st = storing && (t % step_ratio) == 0;
+    for (int k = 0; k < 6; ++k)
+      a.st[k] = st && stores[k]
+                    ? (char *)stores[k] + (size_t)((t - store_t0) / step_ratio) * store_step_bytes
+                    : nullptr;
+    if (!st) {
+      launch_fused<T, 0>(ty, tz, grid, block, stream, a);
+    } else if (fmt == 2) {
+      if constexpr (sizeof(T) == 4) launch_fused<T, 2>(ty, tz, grid, block, stream, a);
+    } else {
+      launch_fused<T, 1>(ty, tz, grid, block, stream, a);
+    }
+    for (int k = 0; k < 6; ++k) {
+      std::swap(fa[k], fb[k]);
+      std::swap(ma[k], mb[k]);
+    }
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  for (int k = 0; k < 6; ++k) {
+    if (fa[k] != fields[k])
+      TIDE_HIP_TRY(hipMemcpyAsync(fields[k], fa[k], bytes, hipMemcpyDeviceToDevice, stream));
+    if (ma[k] != fields[kHMem[k]])
+      TIDE_HIP_TRY(hipMemcpyAsync(fields[kHMem[k]], ma[k], bytes, hipMemcpyDeviceToDevice, stream));
+  }
+  return kOk;
+}
+
 // Forward over [start_t, start_t + nt).  Snapshot slot of step t is
 // (t - store_t0) / step_ratio (t a multiple of step_ratio).
 template <typename T>
@@ -1042,6 +1142,12 @@ int forward_impl(Problem3<T> p, T *const *fields, const T *f, T *r, const int64_
   Scratch scr(stream);
   DevicePointTable dpt;
   TIDE_CHECK(prepare(p, src, rec, scr, dpt, stream));
+  if (g_tuning3.fused_rows > 0) {
+    TIDE_CHECK(forward_fused(p, fields, f, r, nt, start_t, step_ratio, stores, fmt,
+                             store_step_bytes, store_t0, stream, scr, dpt));
+    dpt.release(stream);
+    return kOk;
+  }
   const int tz = g_tuning3.fwd_z;
   const int group = std::min<int>(p.n_shots, g_tuning3.fwd_group);
   Args3<T> a;
@@ -1668,7 +1774,30 @@ extern "C" int tide_em3d_set_tuning(int fwd_z, int adj_z, int group) {
     set_error("tuning: fwd_z in {4,8,16}, adj_z in {4,8}, group 1..8");
     return kInvalidArgument;
   }
-  g_tuning3 = Tuning3{fwd_z, adj_z, group, group};
+  g_tuning3.fwd_z = fwd_z;
+  g_tuning3.adj_z = adj_z;
+  g_tuning3.group = group;
+  g_tuning3.fwd_group = group;
+  return kOk;
+}
+
+extern "C" int tide_em3d_set_fused(int rows, int planes) {
+  if (!(rows == 0 || rows == 2 || rows == 4) || !(planes == 8 || planes == 16)) {
+    set_error("fused step: rows in {0 (off), 2, 4}, planes in {8, 16}");
+    return kInvalidArgument;
+  }
+  g_tuning3.fused_rows = rows;
+  g_tuning3.fused_z = planes;
+  return kOk;
+}
+
+extern "C" int tide_em3d_get_fused(int *out2) {
+  if (out2 == nullptr) {
+    set_error("get_fused: null output");
+    return kInvalidArgument;
+  }
+  out2[0] = g_tuning3.fused_rows;
+  out2[1] = g_tuning3.fused_z;
   return kOk;
 }
 
