@@ -32,11 +32,25 @@ for _p in (ROOT / "tide-gpr_amd", ROOT):
         sys.path.insert(0, str(_p))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per cell-update measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
-# (scaled by a 1 GiB copy's counters) for the kernels as built, default
-# launch shapes; see profiles/tm2d_1024x1024_64shot_gradient_hbm_counters.md.  Reported as
-# roofline.traffic.
-MEASURED_TRAFFIC_B_PER_CELL = {"fwd2_step": 32.52, "adj2_step": 35.30}
+# Counted bytes per cell-update of the 2-D kernels come from the committed
+# rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated); its fenced
+# ``json traffic`` block maps "kernel:steps per launch:snapshot format" to
+# bytes per cell-update.  Reported as roofline.traffic when the bench's
+# dominant kernel has an entry.
+TRAFFIC_SUMMARY = ROOT / "profiles" / "tm2d_1024x1024_64shot_gradient_hbm_counters.md"
+
+
+def counted_traffic() -> dict:
+    """{"kernel:K:fmt": bytes per cell-update} from the counter summary."""
+    import json
+    import re
+
+    try:
+        text = TRAFFIC_SUMMARY.read_text()
+    except OSError:
+        return {}
+    m = re.search(r"```json traffic\n(.*?)```", text, re.S)
+    return json.loads(m.group(1)) if m else {}
 
 
 def log(msg: str) -> None:
@@ -72,9 +86,11 @@ def parse() -> argparse.Namespace:
     ap.add_argument("--batched-seed", type=int, default=3030)
     ap.add_argument("--no-config1", action="store_true",
                     help="skip the small launch-latency-bound forward measurement")
-    ap.add_argument("--em3d-nt", type=int, default=100,
+    ap.add_argument("--em3d-nt", type=int, default=2000,
                     help="time steps of the 3-D measurement (0: skip)")
-    ap.add_argument("--em3d-shots", type=int, default=4)
+    ap.add_argument("--em3d-shots", type=int, default=8)
+    ap.add_argument("--em3d-only", action="store_true",
+                    help="run only the 3-D measurement (for profiling it) and print its JSON")
     ap.add_argument("--accuracy-shots", type=int, default=4,
                     help="shots of the fp32-vs-fp64 and BF16 error sample (0: skip)")
     return ap.parse_args()
@@ -432,65 +448,90 @@ def config1_bench(dev, reps: int = 20) -> dict:
 
 
 def em3d_bench(dev, nt: int, shots: int) -> dict:
-    """3-D Maxwell on the 256^3 geometry (281^3 padded, CPML 12, dx 0.02 m,
-    dt at CFL 0.9), reduced nt and shot count so the run stays short: seeded
-    8-layer eps_r 1-25, sigma U[0, 0.01], x-directed dipole sources (300 MHz
-    Ricker) near the top face, 32 x 32 receivers (Ex) on it.  Forward alone,
-    then forward + adjoint gradient (snapshots kept whole; they fit at this
-    nt); useful work = the forward and adjoint sweeps."""
+    """3-D Maxwell forward + adjoint gradient on the 256^3 workload (281^3
+    padded, CPML 12, dx 0.02 m, dt at CFL 0.9): seeded 8-layer eps_r 1-25
+    with 20 ellipsoidal anomalies, sigma U[0, 0.01], mu_r 1, x-directed
+    dipole sources (300 MHz Ricker) near the top face, 32 x 32 receivers (Ex)
+    on it; fp32 fields and snapshots.  Through the public operator: the
+    observed data (a model with one more anomaly) is one timed forward; the
+    gradient of 0.5 |d - d_obs|^2 is one timed ``op.linearize`` call, which
+    checkpoints the state every ~sqrt(nt) steps when the snapshots do not
+    fit HBM and recomputes each segment's snapshots in the backward.  Useful
+    work = the forward and adjoint sweeps; the recompute factor is the
+    forward sweeps run per useful forward sweep."""
     import torch
 
     import tide
-    from tide.maxwell.maxwell3d import maxwell3d
+    from tide.synthetic import layered_model_3d
 
     n, pml, dx = 256, 12, 0.02
     dt = 0.9 * dx / (3**0.5 * tide.C0)
-    g = torch.Generator().manual_seed(5005)
-    eps = torch.ones(n, n, n)
-    z = 0
-    for _ in range(8):
-        th = int(torch.randint(16, 48, (1,), generator=g))
-        eps[z:z + th] = 1 + 24 * float(torch.rand(1, generator=g))
-        z += th
-    eps = eps.to(dev)
-    sig = (0.01 * torch.rand(n, n, n, generator=g)).to(dev)
-    mu = torch.ones(n, n, n, device=dev)
+    eps, sig, mu = layered_model_3d(n, n, n, seed=5005, device=dev)
+    zz = torch.arange(n, dtype=torch.float64, device=dev)
+    bump = 3.0 * torch.exp(-0.5 * (((zz[:, None, None] - 60) / 20) ** 2
+                                   + ((zz[None, :, None] - 128) / 30) ** 2
+                                   + ((zz[None, None, :] - 128) / 30) ** 2))
+    eps_obs = (eps.double() + bump).to(eps.dtype)
+    del bump
     amp = tide.ricker(3e8, nt, dt, dtype=torch.float32, device=dev).reshape(1, 1, -1).repeat(shots, 1, 1)
     src = torch.stack([torch.tensor([[2, 64 + 32 * (k % 4), 64 + 32 * (k // 4)]]) for k in range(shots)])
     yy, xx = torch.meshgrid(torch.arange(32) * 8 + 2, torch.arange(32) * 8 + 2, indexing="ij")
     rec = torch.stack([torch.full_like(yy, 2), yy, xx], -1).reshape(1, -1, 3).repeat(shots, 1, 1)
-    kw = dict(grid_spacing=dx, dt=dt, source_amplitude=amp, source_location=src,
-              receiver_location=rec, pml_width=pml, source_component="ex", receiver_component="ex")
+    disc = tide.Discretization(spacing=dx, dt=dt, boundary=tide.CPML(pml))
+
+    def operator(n_steps):
+        exp = tide.Experiment(tide.Acquisition(src.to(dev), rec.to(dev)), amp[..., :n_steps],
+                              source_component="ex", receiver_component="ex")
+        return tide.Maxwell3D(disc, exp)
+
     updates = float((n + 2 * pml + 1) ** 3 * shots * nt)
+    # warm-up on a few steps (kernel loading, allocator pools)
+    small = operator(8)
     with torch.no_grad():
-        maxwell3d(eps, sig, mu, **kw)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        maxwell3d(eps, sig, mu, **kw)
-        torch.cuda.synchronize()
-        t_fwd = time.perf_counter() - t0
-    e = eps.clone().requires_grad_(True)
-    s = sig.clone().requires_grad_(True)
-
-    def gradient():
-        e.grad = None
-        s.grad = None
-        r = maxwell3d(e, s, mu, **kw)[-1]
-        (0.5 * (r.double() ** 2).sum()).backward()
-
-    gradient()
+        obs_small = small(tide.EMModel(eps_obs, sig, mu)).receiver_data
+    with small.linearize(tide.EMModel(eps, sig, mu)) as lin:
+        tide.workflow.ReceiverObjective(obs_small).value_and_gradient(lin)
+    del small, obs_small
+    op = operator(nt)
     torch.cuda.synchronize()
+    log(f"3-D: forward of {shots} shots x {nt} steps")
     t0 = time.perf_counter()
-    gradient()
+    with torch.no_grad():
+        observed = op(tide.EMModel(eps_obs, sig, mu)).receiver_data
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    del eps_obs
+    log(f"3-D: forward {t_fwd:.2f} s; gradient")
+    objective = tide.workflow.ReceiverObjective(observed)
+    t0 = time.perf_counter()
+    with op.linearize(tide.EMModel(eps, sig, mu)) as lin:
+        misfit, grad = objective.value_and_gradient(lin)
+        plan = dict(getattr(lin, "last_plan", {}))
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    ok = bool(torch.isfinite(grad.epsilon).all() and torch.isfinite(grad.sigma).all())
+    log(f"3-D: gradient {t_grad:.2f} s, plan {plan}")
+    seg = int(plan.get("segment") or 0)
+    policy = plan.get("policy", "?")
+    # checkpointing runs the forward twice (checkpoints, then each segment's
+    # snapshots); full storage once
+    recompute = 2.0 if policy == "checkpoint" else 1.0
+    gbytes = 48 + 24
     return {
-        "workload": f"3D Maxwell 256^3 (281^3 padded, CPML 12), {shots} shots x 1024 receivers, "
-                    f"nt {nt}, fp32, fp32 snapshots",
+        "workload": f"3D Maxwell 256^3 (281^3 padded, CPML 12), 8-layer eps_r 1-25 + 20 ellipsoidal "
+                    f"anomalies, {shots} shots x 1024 receivers, nt {nt}, fp32, fp32 snapshots",
         "forward": {"seconds": round(t_fwd, 3), "gcell_per_s": round(updates / t_fwd / 1e9, 2),
                     "hbm_frac_48B": round(updates / t_fwd * 48 / 1e9 / HBM_PEAK_GBS, 4)},
-        "gradient": {"seconds": round(t_grad, 3), "gcell_per_s": round(2 * updates / t_grad / 1e9, 2),
-                     "hbm_frac_72B": round(2 * updates / t_grad * 72 / 1e9 / HBM_PEAK_GBS, 4)},
+        "gradient": {"seconds": round(t_grad, 3),
+                     "gcell_per_s": round(2 * updates / t_grad / 1e9, 2),
+                     "policy": policy, "segment_steps": seg,
+                     "segments": (nt + seg - 1) // seg if seg else 0,
+                     "shot_chunks": plan.get("shot_chunks"),
+                     "recompute_factor": recompute,
+                     "finite": ok, "misfit": float(misfit),
+                     "note": f"useful work = forward + adjoint sweeps ({gbytes} algorithmic B per "
+                             "forward+adjoint cell-update pair at one sweep each); the recompute "
+                             "sweep is overhead"},
     }
 
 
@@ -554,6 +595,9 @@ def main() -> None:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.em3d_only:
+        print(json.dumps({"em3d": em3d_bench(dev, args.em3d_nt, args.em3d_shots)}), flush=True)
+        return
     n = args.size
     pml, dx = 20, 0.01
     dt = cfl_dt(dx, 2)
@@ -654,10 +698,10 @@ def main() -> None:
         spl = int(d["steps_per_launch"])
         k_ms, k_launches, _ = loop_ms.get(name, (d["ms"], d["launches"], 0))
         achieved = b_alg[name] * d["cell_updates"] / (k_ms * 1e-3) / 1e9
-        two = spl == 2
+        stem = "adj" if name == "adjoint" else "fwd"
         kernels[name] = {
-            "kernel": ("adj2_step" if two else "adj_step") if name == "adjoint"
-                      else ("fwd2_step" if two else "fwd_step"),
+            "kernel": f"{stem}_step" if spl == 1 else (f"{stem}2_step" if spl == 2
+                                                      else f"{stem}k_step"),
             "steps_per_launch": spl,
             "launches": k_launches,
             "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
@@ -669,7 +713,8 @@ def main() -> None:
     dom = max(summ, key=lambda k: summ[k]["ms"])
     dom_k = kernels[dom]
     shots_per_launch = n_shots / n_chunks
-    measured = MEASURED_TRAFFIC_B_PER_CELL.get(dom_k["kernel"]) if snap_bytes == 4 else None
+    fmt = "bf16" if snap_bytes == 2 else "fp32"
+    measured = counted_traffic().get(f"{dom_k['kernel']}:{dom_k['steps_per_launch']}:{fmt}")
     cells_per_launch = ny_p * nx_p * shots_per_launch * dom_k["steps_per_launch"]
     traffic = None if measured is None else round(measured * cells_per_launch)
     roofline = {
@@ -679,7 +724,8 @@ def main() -> None:
         "unit": "GB/s",
         "frac": round(dom_k["achieved_GBs"] / HBM_PEAK_GBS, 4),
         "traffic": traffic,
-        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated)",
+        "traffic_unit": "bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE (L2-to-fabric, "
+                        "MALL hits included), copy-calibrated, from " + TRAFFIC_SUMMARY.name,
         "algorithmic_bytes_per_launch": round(b_alg[dom] * cells_per_launch),
         "kernel": dom_k["kernel"],
         "kernels": kernels,

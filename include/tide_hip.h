@@ -651,6 +651,484 @@ int maxwell_tm_8_double_born_forward_cuda(
     int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream);
 
 /*
+ * Born forward that keeps the Born adjoint's snapshots: ey_store_1 /
+ * curl_store_1 receive the background's pre-update Ey^n and curl H^n,
+ * dey_store / dcurl_store the scattered field's, one [n_shots, ny, nx] slot
+ * per step (slot t for step t), on the device in full precision
+ * (storage_mode 0, storage_format 0, shot_bytes_uncomp = ny * nx * sizeof
+ * of the field type); all four stores or none.  Otherwise as
+ * maxwell_tm_*_born_forward_cuda.  Replaces
+ * maxwell_tm_{2,4,6,8}_{dtype}_born_forward_with_storage_cuda: reference ABI
+ * src/tide/backend_utils.py:351-370 (_TM_BORN_FORWARD_WITH_STORAGE_SPEC).
+ */
+int maxwell_tm_2_float_born_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float *dey, float *dhx, float *dhz, float *dm_ey_x, float *dm_ey_z,
+    float *dm_hx_z, float *dm_hz_x, float *r, float *background_r, void *ey_store_1,
+    void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,
+    void *curl_filenames, void *dey_store, void *dcurl_store, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot, int64_t step_ratio,
+    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_requires_grad, bool cb_requires_grad, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_2_double_born_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double *dey, double *dhx, double *dhz, double *dm_ey_x, double *dm_ey_z,
+    double *dm_hx_z, double *dm_hz_x, double *r, double *background_r, void *ey_store_1,
+    void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,
+    void *curl_filenames, void *dey_store, void *dcurl_store, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot, int64_t step_ratio,
+    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_requires_grad, bool cb_requires_grad, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_4_float_born_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float *dey, float *dhx, float *dhz, float *dm_ey_x, float *dm_ey_z,
+    float *dm_hx_z, float *dm_hz_x, float *r, float *background_r, void *ey_store_1,
+    void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,
+    void *curl_filenames, void *dey_store, void *dcurl_store, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot, int64_t step_ratio,
+    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_requires_grad, bool cb_requires_grad, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_4_double_born_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double *dey, double *dhx, double *dhz, double *dm_ey_x, double *dm_ey_z,
+    double *dm_hx_z, double *dm_hz_x, double *r, double *background_r, void *ey_store_1,
+    void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,
+    void *curl_filenames, void *dey_store, void *dcurl_store, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot, int64_t step_ratio,
+    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_requires_grad, bool cb_requires_grad, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_6_float_born_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float *dey, float *dhx, float *dhz, float *dm_ey_x, float *dm_ey_z,
+    float *dm_hx_z, float *dm_hz_x, float *r, float *background_r, void *ey_store_1,
+    void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,
+    void *curl_filenames, void *dey_store, void *dcurl_store, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot, int64_t step_ratio,
+    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_requires_grad, bool cb_requires_grad, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_6_double_born_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double *dey, double *dhx, double *dhz, double *dm_ey_x, double *dm_ey_z,
+    double *dm_hx_z, double *dm_hz_x, double *r, double *background_r, void *ey_store_1,
+    void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,
+    void *curl_filenames, void *dey_store, void *dcurl_store, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot, int64_t step_ratio,
+    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_requires_grad, bool cb_requires_grad, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_8_float_born_forward_with_storage_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float *ey, float *hx, float *hz, float *m_ey_x, float *m_ey_z,
+    float *m_hx_z, float *m_hz_x, float *dey, float *dhx, float *dhz, float *dm_ey_x, float *dm_ey_z,
+    float *dm_hx_z, float *dm_hz_x, float *r, float *background_r, void *ey_store_1,
+    void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,
+    void *curl_filenames, void *dey_store, void *dcurl_store, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot, int64_t step_ratio,
+    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_requires_grad, bool cb_requires_grad, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+int maxwell_tm_8_double_born_forward_with_storage_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double *ey, double *hx, double *hz, double *m_ey_x, double *m_ey_z,
+    double *m_hx_z, double *m_hz_x, double *dey, double *dhx, double *dhz, double *dm_ey_x, double *dm_ey_z,
+    double *dm_hx_z, double *dm_hz_x, double *r, double *background_r, void *ey_store_1,
+    void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,
+    void *curl_filenames, void *dey_store, void *dcurl_store, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, double dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+    int64_t n_sources_per_shot, int64_t n_receivers_per_shot, int64_t step_ratio,
+    int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,
+    bool ca_requires_grad, bool cb_requires_grad, bool ca_batched, bool cb_batched,
+    bool cq_batched, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t n_threads, int64_t device, void *compute_stream,
+    void *storage_stream);
+
+/*
+ * Adjoint of the Born forward (second-order products): gradients of
+ * <r, grad_r> + <background_r, grad_background_r> with respect to ca, cb
+ * (through the background and the scattered recurrences), dca, dcb (FP64
+ * accumulation, added to the outputs; grad_ca / grad_cb only when
+ * ca/cb_requires_grad), f0 and df (grad_f0 / grad_df written per step).
+ * ey, hx, hz + m_lambda_* hold the scattered adjoint state and dey, dhx, dhz
+ * + m_eta_* the background's (in/out, the caller zeroes them for a fresh
+ * run); adjoint_memory_scratch (8 [n_shots, ny, nx] arrays) or NULL.  The
+ * stores are those of born_forward_with_storage; unbatched coefficients,
+ * device storage, step_ratio 1, no background-adjoint capture or reuse;
+ * lambda_store, eta_source_old, work_eta_*, grad_*_shot are not read.
+ * Replaces maxwell_tm_{2,4,6,8}_{dtype}_born_backward_bggrad_cuda: reference
+ * ABI src/tide/backend_utils.py:407-444 (_TM_BORN_BACKWARD_BGGRAD_SPEC),
+ * CUDA implementation src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1611-2404.
+ */
+int maxwell_tm_2_float_born_backward_bggrad_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float const *grad_r, float const *grad_background_r,
+    void *ey_store_1, void *ey_store_3, void *ey_filenames, void *curl_store_1,
+    void *curl_store_3, void *curl_filenames, void *dey_store, void *dcurl_store,
+    void *lambda_store, int64_t background_n_shots, float *ey, float *hx, float *hz, float *dey,
+    float *dhx, float *dhz, float *grad_f0, float *grad_df, float *grad_ca, float *grad_cb,
+    float *grad_dca, float *grad_dcb, float *m_lambda_ey_x, float *m_lambda_ey_z,
+    float *m_lambda_hx_z, float *m_lambda_hz_x, float *m_eta_ey_x, float *m_eta_ey_z,
+    float *m_eta_hx_z, float *m_eta_hz_x, float *eta_source_old, float *work_eta_x,
+    float *work_eta_z, float *grad_ca_shot, float *grad_cb_shot, float *grad_dca_shot,
+    float *grad_dcb_shot, float *adjoint_memory_scratch, float const *ay, float const *by,
+    float const *ayh, float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy, float rdx, float dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool capture_background_adjoint,
+    bool reuse_background_adjoint, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream, void *storage_stream);
+
+int maxwell_tm_2_double_born_backward_bggrad_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double const *grad_r, double const *grad_background_r,
+    void *ey_store_1, void *ey_store_3, void *ey_filenames, void *curl_store_1,
+    void *curl_store_3, void *curl_filenames, void *dey_store, void *dcurl_store,
+    void *lambda_store, int64_t background_n_shots, double *ey, double *hx, double *hz, double *dey,
+    double *dhx, double *dhz, double *grad_f0, double *grad_df, double *grad_ca, double *grad_cb,
+    double *grad_dca, double *grad_dcb, double *m_lambda_ey_x, double *m_lambda_ey_z,
+    double *m_lambda_hx_z, double *m_lambda_hz_x, double *m_eta_ey_x, double *m_eta_ey_z,
+    double *m_eta_hx_z, double *m_eta_hz_x, double *eta_source_old, double *work_eta_x,
+    double *work_eta_z, double *grad_ca_shot, double *grad_cb_shot, double *grad_dca_shot,
+    double *grad_dcb_shot, double *adjoint_memory_scratch, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx, double const *axh,
+    double const *bxh, double const *ky, double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy, double rdx, double dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool capture_background_adjoint,
+    bool reuse_background_adjoint, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream, void *storage_stream);
+
+int maxwell_tm_4_float_born_backward_bggrad_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float const *grad_r, float const *grad_background_r,
+    void *ey_store_1, void *ey_store_3, void *ey_filenames, void *curl_store_1,
+    void *curl_store_3, void *curl_filenames, void *dey_store, void *dcurl_store,
+    void *lambda_store, int64_t background_n_shots, float *ey, float *hx, float *hz, float *dey,
+    float *dhx, float *dhz, float *grad_f0, float *grad_df, float *grad_ca, float *grad_cb,
+    float *grad_dca, float *grad_dcb, float *m_lambda_ey_x, float *m_lambda_ey_z,
+    float *m_lambda_hx_z, float *m_lambda_hz_x, float *m_eta_ey_x, float *m_eta_ey_z,
+    float *m_eta_hx_z, float *m_eta_hz_x, float *eta_source_old, float *work_eta_x,
+    float *work_eta_z, float *grad_ca_shot, float *grad_cb_shot, float *grad_dca_shot,
+    float *grad_dcb_shot, float *adjoint_memory_scratch, float const *ay, float const *by,
+    float const *ayh, float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy, float rdx, float dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool capture_background_adjoint,
+    bool reuse_background_adjoint, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream, void *storage_stream);
+
+int maxwell_tm_4_double_born_backward_bggrad_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double const *grad_r, double const *grad_background_r,
+    void *ey_store_1, void *ey_store_3, void *ey_filenames, void *curl_store_1,
+    void *curl_store_3, void *curl_filenames, void *dey_store, void *dcurl_store,
+    void *lambda_store, int64_t background_n_shots, double *ey, double *hx, double *hz, double *dey,
+    double *dhx, double *dhz, double *grad_f0, double *grad_df, double *grad_ca, double *grad_cb,
+    double *grad_dca, double *grad_dcb, double *m_lambda_ey_x, double *m_lambda_ey_z,
+    double *m_lambda_hx_z, double *m_lambda_hz_x, double *m_eta_ey_x, double *m_eta_ey_z,
+    double *m_eta_hx_z, double *m_eta_hz_x, double *eta_source_old, double *work_eta_x,
+    double *work_eta_z, double *grad_ca_shot, double *grad_cb_shot, double *grad_dca_shot,
+    double *grad_dcb_shot, double *adjoint_memory_scratch, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx, double const *axh,
+    double const *bxh, double const *ky, double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy, double rdx, double dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool capture_background_adjoint,
+    bool reuse_background_adjoint, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream, void *storage_stream);
+
+int maxwell_tm_6_float_born_backward_bggrad_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float const *grad_r, float const *grad_background_r,
+    void *ey_store_1, void *ey_store_3, void *ey_filenames, void *curl_store_1,
+    void *curl_store_3, void *curl_filenames, void *dey_store, void *dcurl_store,
+    void *lambda_store, int64_t background_n_shots, float *ey, float *hx, float *hz, float *dey,
+    float *dhx, float *dhz, float *grad_f0, float *grad_df, float *grad_ca, float *grad_cb,
+    float *grad_dca, float *grad_dcb, float *m_lambda_ey_x, float *m_lambda_ey_z,
+    float *m_lambda_hx_z, float *m_lambda_hz_x, float *m_eta_ey_x, float *m_eta_ey_z,
+    float *m_eta_hx_z, float *m_eta_hz_x, float *eta_source_old, float *work_eta_x,
+    float *work_eta_z, float *grad_ca_shot, float *grad_cb_shot, float *grad_dca_shot,
+    float *grad_dcb_shot, float *adjoint_memory_scratch, float const *ay, float const *by,
+    float const *ayh, float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy, float rdx, float dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool capture_background_adjoint,
+    bool reuse_background_adjoint, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream, void *storage_stream);
+
+int maxwell_tm_6_double_born_backward_bggrad_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double const *grad_r, double const *grad_background_r,
+    void *ey_store_1, void *ey_store_3, void *ey_filenames, void *curl_store_1,
+    void *curl_store_3, void *curl_filenames, void *dey_store, void *dcurl_store,
+    void *lambda_store, int64_t background_n_shots, double *ey, double *hx, double *hz, double *dey,
+    double *dhx, double *dhz, double *grad_f0, double *grad_df, double *grad_ca, double *grad_cb,
+    double *grad_dca, double *grad_dcb, double *m_lambda_ey_x, double *m_lambda_ey_z,
+    double *m_lambda_hx_z, double *m_lambda_hz_x, double *m_eta_ey_x, double *m_eta_ey_z,
+    double *m_eta_hx_z, double *m_eta_hz_x, double *eta_source_old, double *work_eta_x,
+    double *work_eta_z, double *grad_ca_shot, double *grad_cb_shot, double *grad_dca_shot,
+    double *grad_dcb_shot, double *adjoint_memory_scratch, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx, double const *axh,
+    double const *bxh, double const *ky, double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy, double rdx, double dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool capture_background_adjoint,
+    bool reuse_background_adjoint, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream, void *storage_stream);
+
+int maxwell_tm_8_float_born_backward_bggrad_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *f0, float const *df, float const *grad_r, float const *grad_background_r,
+    void *ey_store_1, void *ey_store_3, void *ey_filenames, void *curl_store_1,
+    void *curl_store_3, void *curl_filenames, void *dey_store, void *dcurl_store,
+    void *lambda_store, int64_t background_n_shots, float *ey, float *hx, float *hz, float *dey,
+    float *dhx, float *dhz, float *grad_f0, float *grad_df, float *grad_ca, float *grad_cb,
+    float *grad_dca, float *grad_dcb, float *m_lambda_ey_x, float *m_lambda_ey_z,
+    float *m_lambda_hx_z, float *m_lambda_hz_x, float *m_eta_ey_x, float *m_eta_ey_z,
+    float *m_eta_hx_z, float *m_eta_hz_x, float *eta_source_old, float *work_eta_x,
+    float *work_eta_z, float *grad_ca_shot, float *grad_cb_shot, float *grad_dca_shot,
+    float *grad_dcb_shot, float *adjoint_memory_scratch, float const *ay, float const *by,
+    float const *ayh, float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *ky, float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdy, float rdx, float dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool capture_background_adjoint,
+    bool reuse_background_adjoint, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream, void *storage_stream);
+
+int maxwell_tm_8_double_born_backward_bggrad_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *f0, double const *df, double const *grad_r, double const *grad_background_r,
+    void *ey_store_1, void *ey_store_3, void *ey_filenames, void *curl_store_1,
+    void *curl_store_3, void *curl_filenames, void *dey_store, void *dcurl_store,
+    void *lambda_store, int64_t background_n_shots, double *ey, double *hx, double *hz, double *dey,
+    double *dhx, double *dhz, double *grad_f0, double *grad_df, double *grad_ca, double *grad_cb,
+    double *grad_dca, double *grad_dcb, double *m_lambda_ey_x, double *m_lambda_ey_z,
+    double *m_lambda_hx_z, double *m_lambda_hz_x, double *m_eta_ey_x, double *m_eta_ey_z,
+    double *m_eta_hx_z, double *m_eta_hz_x, double *eta_source_old, double *work_eta_x,
+    double *work_eta_z, double *grad_ca_shot, double *grad_cb_shot, double *grad_dca_shot,
+    double *grad_dcb_shot, double *adjoint_memory_scratch, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx, double const *axh,
+    double const *bxh, double const *ky, double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdy, double rdx, double dt,
+    int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, int64_t storage_mode,
+    int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_requires_grad,
+    bool cb_requires_grad, bool capture_background_adjoint,
+    bool reuse_background_adjoint, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,
+    int64_t n_threads, int64_t device, void *compute_stream, void *storage_stream);
+
+/*
+ * This library's form of the Born adjoint (used by its own autograd
+ * function): the four snapshot histories as written by
+ * born_forward_with_storage, the scattered (l_*) and background (e_*)
+ * adjoint states, cotangents w (scattered receivers) and w_bg (background
+ * receivers; either may be NULL), per-step source gradients grad_df /
+ * grad_f0 (NULL: not wanted), FP64 accumulators gca, gcb, gdca, gdcb (added
+ * to; NULL: not wanted) over steps start_t-1 .. start_t-nt.  No reference
+ * counterpart (the reference folds into field-typed outputs).
+ */
+int tide_tm2d_2_float_born_backward_f64acc(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *w, float const *w_bg, float const *bg_ey_hist, float const *bg_curl_hist,
+    float const *sc_ey_hist, float const *sc_curl_hist, float *l_ey, float *l_hx, float *l_hz,
+    float *ml_ey_x, float *ml_ey_z, float *ml_hx_z, float *ml_hz_x, float *e_ey, float *e_hx,
+    float *e_hz, float *me_ey_x, float *me_ey_z, float *me_hx_z, float *me_hz_x, float *grad_df,
+    float *grad_f0, double *gca, double *gcb, double *gdca, double *gdcb, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t device, void *compute_stream);
+
+int tide_tm2d_2_double_born_backward_f64acc(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *w, double const *w_bg, double const *bg_ey_hist, double const *bg_curl_hist,
+    double const *sc_ey_hist, double const *sc_curl_hist, double *l_ey, double *l_hx, double *l_hz,
+    double *ml_ey_x, double *ml_ey_z, double *ml_hx_z, double *ml_hz_x, double *e_ey, double *e_hx,
+    double *e_hz, double *me_ey_x, double *me_ey_z, double *me_hx_z, double *me_hz_x, double *grad_df,
+    double *grad_f0, double *gca, double *gcb, double *gdca, double *gdcb, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t device, void *compute_stream);
+
+int tide_tm2d_4_float_born_backward_f64acc(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *w, float const *w_bg, float const *bg_ey_hist, float const *bg_curl_hist,
+    float const *sc_ey_hist, float const *sc_curl_hist, float *l_ey, float *l_hx, float *l_hz,
+    float *ml_ey_x, float *ml_ey_z, float *ml_hx_z, float *ml_hz_x, float *e_ey, float *e_hx,
+    float *e_hz, float *me_ey_x, float *me_ey_z, float *me_hx_z, float *me_hz_x, float *grad_df,
+    float *grad_f0, double *gca, double *gcb, double *gdca, double *gdcb, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t device, void *compute_stream);
+
+int tide_tm2d_4_double_born_backward_f64acc(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *w, double const *w_bg, double const *bg_ey_hist, double const *bg_curl_hist,
+    double const *sc_ey_hist, double const *sc_curl_hist, double *l_ey, double *l_hx, double *l_hz,
+    double *ml_ey_x, double *ml_ey_z, double *ml_hx_z, double *ml_hz_x, double *e_ey, double *e_hx,
+    double *e_hz, double *me_ey_x, double *me_ey_z, double *me_hx_z, double *me_hz_x, double *grad_df,
+    double *grad_f0, double *gca, double *gcb, double *gdca, double *gdcb, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t device, void *compute_stream);
+
+int tide_tm2d_6_float_born_backward_f64acc(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *w, float const *w_bg, float const *bg_ey_hist, float const *bg_curl_hist,
+    float const *sc_ey_hist, float const *sc_curl_hist, float *l_ey, float *l_hx, float *l_hz,
+    float *ml_ey_x, float *ml_ey_z, float *ml_hx_z, float *ml_hz_x, float *e_ey, float *e_hx,
+    float *e_hz, float *me_ey_x, float *me_ey_z, float *me_hx_z, float *me_hz_x, float *grad_df,
+    float *grad_f0, double *gca, double *gcb, double *gdca, double *gdcb, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t device, void *compute_stream);
+
+int tide_tm2d_6_double_born_backward_f64acc(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *w, double const *w_bg, double const *bg_ey_hist, double const *bg_curl_hist,
+    double const *sc_ey_hist, double const *sc_curl_hist, double *l_ey, double *l_hx, double *l_hz,
+    double *ml_ey_x, double *ml_ey_z, double *ml_hx_z, double *ml_hz_x, double *e_ey, double *e_hx,
+    double *e_hz, double *me_ey_x, double *me_ey_z, double *me_hx_z, double *me_hz_x, double *grad_df,
+    double *grad_f0, double *gca, double *gcb, double *gdca, double *gdcb, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t device, void *compute_stream);
+
+int tide_tm2d_8_float_born_backward_f64acc(
+    float const *ca, float const *cb, float const *cq, float const *dca, float const *dcb,
+    float const *w, float const *w_bg, float const *bg_ey_hist, float const *bg_curl_hist,
+    float const *sc_ey_hist, float const *sc_curl_hist, float *l_ey, float *l_hx, float *l_hz,
+    float *ml_ey_x, float *ml_ey_z, float *ml_hx_z, float *ml_hz_x, float *e_ey, float *e_hx,
+    float *e_hz, float *me_ey_x, float *me_ey_z, float *me_hx_z, float *me_hz_x, float *grad_df,
+    float *grad_f0, double *gca, double *gcb, double *gdca, double *gdcb, float const *ay,
+    float const *by, float const *ayh, float const *byh, float const *ax, float const *bx,
+    float const *axh, float const *bxh, float const *ky, float const *kyh, float const *kx,
+    float const *kxh, int64_t const *sources_i, int64_t const *receivers_i, float rdy,
+    float rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t device, void *compute_stream);
+
+int tide_tm2d_8_double_born_backward_f64acc(
+    double const *ca, double const *cb, double const *cq, double const *dca, double const *dcb,
+    double const *w, double const *w_bg, double const *bg_ey_hist, double const *bg_curl_hist,
+    double const *sc_ey_hist, double const *sc_curl_hist, double *l_ey, double *l_hx, double *l_hz,
+    double *ml_ey_x, double *ml_ey_z, double *ml_hx_z, double *ml_hz_x, double *e_ey, double *e_hx,
+    double *e_hz, double *me_ey_x, double *me_ey_z, double *me_hx_z, double *me_hz_x, double *grad_df,
+    double *grad_f0, double *gca, double *gcb, double *gdca, double *gdcb, double const *ay,
+    double const *by, double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *ky, double const *kyh, double const *kx,
+    double const *kxh, int64_t const *sources_i, int64_t const *receivers_i, double rdy,
+    double rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,
+    int64_t pml_x1, int64_t device, void *compute_stream);
+
+
+/*
  * Allocation-free variant of maxwell_tm_2_float_forward_cuda: the same
  * arguments and results, with the call's scratch (ping-pong copies of the
  * fields and CPML memories, packed profiles, point tables) in a fixed
@@ -1173,6 +1651,14 @@ int tide_em3d_set_forward_group(int group);
  */
 int tide_em3d_set_fused(int rows, int planes);
 int tide_em3d_get_fused(int *out2);
+
+/*
+ * Workgroup order of the 3-D kernels: 0 runs all tiles of one shot group
+ * before the next group, 1 runs the shot groups of one tile next to each
+ * other (same XCD, shared L2 for the coefficient lines).  Results do not
+ * depend on it.  No reference counterpart.
+ */
+int tide_em3d_set_tile_order(int shot_fast);
 
 #ifdef __cplusplus
 }

@@ -242,6 +242,95 @@ def tm2d_tangent(case: TMCase, dca: np.ndarray, dcb: np.ndarray, df: np.ndarray)
     return r, dr
 
 
+def tm2d_forward_debye(case: TMCase, debye_a: np.ndarray, debye_b: np.ndarray,
+                       debye_cp: np.ndarray, *, ey_prev: np.ndarray | None = None):
+    """Debye-dispersive forward (``tm2d_oracle.cpp:forward_debye``, reference
+    ``csrc/tm2d/maxwell_tm_inst.cpp:214-300, 780-805``): ``case.ca``/``cb``
+    are the Debye coefficients, ``debye_*`` ``[n_poles, ny, nx]``.  Returns
+    ``(fields, r, polarization, ey_prev)``."""
+    lib = oracle_lib()
+    dt_ = case.dtype
+    suffix = "f64" if dt_ == np.float64 else "f32"
+    fields = [np.ascontiguousarray(x.copy()) for x in case.fields]
+    n_poles = int(debye_a.shape[0])
+    pol = np.zeros((case.n_shots, n_poles, case.ny, case.nx), dt_)
+    prev = (np.zeros((case.n_shots, case.ny, case.nx), dt_) if ey_prev is None
+            else np.ascontiguousarray(ey_prev, dt_).copy())
+    n_rec = case.rec.shape[1]
+    r = np.zeros((case.nt, case.n_shots, n_rec), dt_)
+    ct = ctypes.c_double if dt_ == np.float64 else ctypes.c_float
+    deb = [np.ascontiguousarray(x, dt_) for x in (debye_a, debye_b, debye_cp)]
+    fn = getattr(lib, f"oracle_tm2d_forward_debye_{suffix}")
+    fn.restype = ctypes.c_int
+    rc = fn(ctypes.c_int(case.stencil), _p(case.ca), _p(case.cb), _p(case.cq), _p(case.f),
+            _parr(fields), _parr(deb), _p(pol), _p(prev), ctypes.c_int64(n_poles), _p(r),
+            _parr(case.profiles), _p(case.src), _p(case.rec), ct(case.rdy), ct(case.rdx),
+            ctypes.c_int64(case.nt), ctypes.c_int64(case.n_shots), ctypes.c_int64(case.ny),
+            ctypes.c_int64(case.nx), ctypes.c_int64(case.src.shape[1]), ctypes.c_int64(n_rec),
+            ctypes.c_int64(0), (ctypes.c_int64 * 4)(*case.pml))
+    if rc != 0:
+        raise RuntimeError(f"oracle Debye forward failed ({rc})")
+    return fields, r, pol, prev
+
+
+def tm2d_tangent_store(case: TMCase, dca: np.ndarray, dcb: np.ndarray, df: np.ndarray):
+    """FP64 tangent-linear forward keeping the Born adjoint's snapshots:
+    returns ``(r, dr, hist)`` with ``hist`` = background Ey^n, curl^n and
+    tangent dEy^n, dcurl^n, each ``[nt, shots, ny, nx]``
+    (``tm2d_oracle.cpp:tangent``)."""
+    lib = oracle_lib()
+    c = case.astype(np.float64)
+    fields = [x.copy() for x in c.fields]
+    dfields = [np.zeros_like(x) for x in c.fields]
+    n_rec = c.rec.shape[1]
+    r = np.zeros((c.nt, c.n_shots, n_rec))
+    dr = np.zeros_like(r)
+    hist = [np.zeros((c.nt, c.n_shots, c.ny, c.nx)) for _ in range(4)]
+    fn = lib.oracle_tm2d_tangent_store_f64
+    fn.restype = ctypes.c_int
+    rc = fn(ctypes.c_int(c.stencil), _p(c.ca), _p(c.cb), _p(c.cq), _p(c.f),
+            _p(np.ascontiguousarray(dca, np.float64)), _p(np.ascontiguousarray(dcb, np.float64)),
+            _p(np.ascontiguousarray(df, np.float64)), _parr(fields), _parr(dfields), _p(r), _p(dr),
+            _parr(c.profiles), _p(c.src), _p(c.rec), ctypes.c_double(c.rdy), ctypes.c_double(c.rdx),
+            ctypes.c_int64(c.nt), ctypes.c_int64(c.n_shots), ctypes.c_int64(c.ny),
+            ctypes.c_int64(c.nx), ctypes.c_int64(c.src.shape[1]), ctypes.c_int64(n_rec),
+            (ctypes.c_int64 * 4)(*c.pml), _parr(hist))
+    if rc != 0:
+        raise RuntimeError(f"oracle tangent failed ({rc})")
+    return r, dr, hist
+
+
+def tm2d_tangent_adjoint(case: TMCase, dca: np.ndarray, dcb: np.ndarray, w: np.ndarray | None,
+                         w_bg: np.ndarray | None, hist) -> dict:
+    """FP64 adjoint of the tangent-linear forward (``tm2d_oracle.cpp:
+    tangent_adjoint``): gradients of ``<dr, w> + <r, w_bg>`` with respect to
+    ``ca, cb, dca, dcb`` ([ny, nx]), ``f`` and ``df`` ([nt, shots, n_src]),
+    plus the final adjoint states ``lam`` (tangent) and ``eta`` (background)."""
+    lib = oracle_lib()
+    c = case.astype(np.float64)
+    n_src, n_rec = c.src.shape[1], c.rec.shape[1]
+    lam = [np.zeros_like(x) for x in c.fields]
+    eta = [np.zeros_like(x) for x in c.fields]
+    out = {k: np.zeros((c.ny, c.nx)) for k in ("ca", "cb", "dca", "dcb")}
+    out["df"] = np.zeros((c.nt, c.n_shots, n_src))
+    out["f"] = np.zeros_like(out["df"])
+    fn = lib.oracle_tm2d_tangent_adjoint_f64
+    fn.restype = ctypes.c_int
+    opt = lambda a: None if a is None else _p(np.ascontiguousarray(a, np.float64))  # noqa: E731
+    rc = fn(ctypes.c_int(c.stencil), _p(c.ca), _p(c.cb), _p(c.cq),
+            _p(np.ascontiguousarray(dca, np.float64)), _p(np.ascontiguousarray(dcb, np.float64)),
+            opt(w), opt(w_bg), _parr(hist), _parr(lam), _parr(eta), _p(out["df"]), _p(out["f"]),
+            _p(out["ca"]), _p(out["cb"]), _p(out["dca"]), _p(out["dcb"]), _parr(c.profiles),
+            _p(c.src), _p(c.rec), ctypes.c_double(c.rdy), ctypes.c_double(c.rdx),
+            ctypes.c_int64(c.nt), ctypes.c_int64(c.n_shots), ctypes.c_int64(c.ny),
+            ctypes.c_int64(c.nx), ctypes.c_int64(n_src), ctypes.c_int64(n_rec),
+            (ctypes.c_int64 * 4)(*c.pml))
+    if rc != 0:
+        raise RuntimeError(f"oracle tangent adjoint failed ({rc})")
+    out["lam"], out["eta"] = lam, eta
+    return out
+
+
 def bf16_encode(x: np.ndarray) -> np.ndarray:
     """Round-to-nearest-even fp32 -> bf16 bits (same rule as the kernels)."""
     u = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
@@ -254,6 +343,33 @@ def bf16_decode(h: np.ndarray) -> np.ndarray:
 
 
 # ------------------------------------------------------------ reference lib
+def ref_tm2d_forward_debye(case: TMCase, debye_a, debye_b, debye_cp):
+    """The reference's own CPU forward with Debye dispersion
+    (``has_dispersion``): returns ``(fields, r, polarization)``.  Its
+    ``ey_prev`` buffer starts at zero, as this module's."""
+    lib = ref_lib(False)
+    dt_ = case.dtype
+    name = "double" if dt_ == np.float64 else "float"
+    fn = getattr(lib, f"maxwell_tm_{case.stencil}_{name}_forward_cpu")
+    ct = ctypes.c_double if dt_ == np.float64 else ctypes.c_float
+    I, P, B = ctypes.c_int64, ctypes.c_void_p, ctypes.c_bool
+    fn.restype = None
+    fn.argtypes = ([P] * 17 + [I] + [P] * 14 + [ct] * 3 + [I] * 7 + [B] * 4 + [I] * 7 + [P])
+    fields = [np.ascontiguousarray(x.copy()) for x in case.fields]
+    n_rec = case.rec.shape[1]
+    n_poles = int(debye_a.shape[0])
+    r = np.zeros((case.nt, case.n_shots, n_rec), dt_)
+    pol = np.zeros((case.n_shots, n_poles, case.ny, case.nx), dt_)
+    prev = np.zeros((case.n_shots, case.ny, case.nx), dt_)
+    deb = [np.ascontiguousarray(x, dt_) for x in (debye_a, debye_b, debye_cp)]
+    fn(_p(case.ca), _p(case.cb), _p(case.cq), _p(case.f), *[_p(x) for x in fields],
+       *[_p(x) for x in deb], _p(pol), _p(prev), _p(r), n_poles, *[_p(p) for p in case.profiles],
+       _p(case.src), _p(case.rec), case.rdy, case.rdx, case.dt, case.nt, case.n_shots,
+       case.ny, case.nx, case.src.shape[1], n_rec, 1, True, False, False, False, 0,
+       *case.pml, 1, 0, None)
+    return fields, r, pol
+
+
 def ref_tm2d_forward(case: TMCase, *, n_threads: int = 0, nt: int | None = None,
                      fast: bool = False):
     """The reference's own CPU forward (``maxwell_tm_<s>_<dtype>_forward_cpu``).

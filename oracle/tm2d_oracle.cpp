@@ -10,8 +10,10 @@
 //   * the exact discrete adjoint of that step (gather form, ping-pong CPML
 //     adjoint memories, the index-shifted reverse loop) accumulating
 //     dJ/dca and dJ/dcb in FP64;
-//   * the tangent-linear (Born) forward in the coefficients, used only to pin
-//     the adjoint with the dot-product identity <J dm, w> = <dm, J^T w>.
+//   * the tangent-linear (Born) forward in the coefficients, used to pin the
+//     adjoint with the dot-product identity <J dm, w> = <dm, J^T w>, and its
+//     own adjoint (second-order products);
+//   * the Debye-dispersive forward (forward inference).
 //
 // Followed reference code (paths relative to the reference's src/tide/):
 //   stencils        csrc/tm2d/maxwell_tm_core.cuh:61-365 (DiffForward/Adjoint)
@@ -20,6 +22,7 @@
 //   BF16 codec      csrc/tm2d/maxwell.cpp:63-83
 //   forward loop    csrc/tm2d/maxwell_tm_inst.cpp:32-131, 727-820
 //   point ops       csrc/tm2d/maxwell.cpp:85-160
+//   Debye forward   csrc/tm2d/maxwell_tm_inst.cpp:214-300, 780-805
 //   exact adjoint   csrc/tm2d/maxwell_tm_cuda_inst.cu:922-1254
 //   reverse loop    csrc/tm2d/maxwell_tm_cuda_inst.cu:2540-2731
 //
@@ -279,6 +282,75 @@ void forward(const T *ca, const T *cb, const T *cq, const T *f, T *ey, T *hx,
   }
 }
 
+// ----------------------------------------------------------- Debye forward
+// Debye-dispersive forward (forward inference; reference
+// csrc/tm2d/maxwell_tm_inst.cpp:214-300 and the time loop :780-805).  The E
+// step keeps Ey^n in ey_prev, applies the plain update with the Debye
+// coefficients, then adds sum_p cp_p P_p (poles in order, from zero); after
+// the sources every cell's polarization advances
+//   P_p = a_p P_p + b_p (Ey^{n+1} + Ey^n).
+// cp, a, b: [n_poles, ny, nx]; pol: [shots, n_poles, ny, nx].
+template <typename T, int S>
+void e_step_debye(const Grid<T> &g, const T *ca, const T *cb, const T *hx, const T *hz, T *ey,
+                  T *m_hx_z, T *m_hz_x, T *ey_prev, const T *cp, const T *pol,
+                  int64_t n_poles, int64_t shot) {
+  constexpr int P = Coef<S>::P;
+  const int64_t ny = g.ny, nx = g.nx, numel = g.numel(), base = shot * numel;
+  for (int64_t y = P; y < ny - P + 1; ++y) {
+    for (int64_t x = P; x < nx - P + 1; ++x) {
+      const int64_t j = y * nx + x, i = base + j;
+      ey_prev[i] = ey[i];
+      const T cav = g.ca_b ? ca[i] : ca[j];
+      const T cbv = g.cb_b ? cb[i] : cb[j];
+      T dhz = d_int<T, S>([&](int o) { return hz[i + o]; }, g.rdx);
+      T dhx = d_int<T, S>([&](int o) { return hx[i + o * nx]; }, g.rdy);
+      if (x < g.pml_x0 || x >= g.pml_x1) {
+        m_hz_x[i] = g.bx[x] * m_hz_x[i] + g.ax[x] * dhz;
+        dhz = dhz / g.kx[x] + m_hz_x[i];
+      }
+      if (y < g.pml_y0 || y >= g.pml_y1) {
+        m_hx_z[i] = g.by[y] * m_hx_z[i] + g.ay[y] * dhx;
+        dhx = dhx / g.ky[y] + m_hx_z[i];
+      }
+      const T curl = dhz - dhx;
+      ey[i] = cav * ey[i] + cbv * curl;
+      T pol_term = (T)0;
+      for (int64_t p = 0; p < n_poles; ++p)
+        pol_term += cp[p * numel + j] * pol[(shot * n_poles + p) * numel + j];
+      ey[i] += pol_term;
+    }
+  }
+}
+
+template <typename T, int S>
+void forward_debye(const T *ca, const T *cb, const T *cq, const T *f, T *ey, T *hx, T *hz,
+                   T *m_ey_x, T *m_ey_z, T *m_hx_z, T *m_hz_x, const T *da, const T *db,
+                   const T *cp, T *pol, T *ey_prev, int64_t n_poles, T *r, const Grid<T> &g,
+                   const int64_t *src, const int64_t *rec, int64_t n_src, int64_t n_rec,
+                   int64_t nt, int64_t start_t) {
+  const int64_t numel = g.numel();
+  for (int64_t t = start_t; t < start_t + nt; ++t) {
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < g.n_shots; ++s) {
+      h_step<T, S>(g, cq, ey, hx, hz, m_ey_x, m_ey_z, s);
+      e_step_debye<T, S>(g, ca, cb, hx, hz, ey, m_hx_z, m_hz_x, ey_prev, cp, pol, n_poles, s);
+      for (int64_t k = 0; k < n_src; ++k) {
+        const int64_t q = s * n_src + k;
+        if (src[q] >= 0) ey[s * numel + src[q]] += f[t * g.n_shots * n_src + q];
+      }
+      for (int64_t p = 0; p < n_poles; ++p)
+        for (int64_t j = 0; j < numel; ++j) {
+          const int64_t i = s * numel + j, pi = (s * n_poles + p) * numel + j;
+          pol[pi] = da[p * numel + j] * pol[pi] + db[p * numel + j] * (ey[i] + ey_prev[i]);
+        }
+      for (int64_t k = 0; k < n_rec; ++k) {
+        const int64_t q = s * n_rec + k;
+        if (rec[q] >= 0) r[t * g.n_shots * n_rec + q] = ey[s * numel + rec[q]];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------ exact adjoint
 //
 // Forward step n maps (Ey, Hx, Hz, psi) -> (Ey', Hx', Hz', psi').  Its
@@ -288,10 +360,13 @@ void forward(const T *ca, const T *cb, const T *cq, const T *f, T *ey, T *hx,
 // a M_n + q_n / k (strip) or q_n (interior).  M_{n+1} is read by the
 // neighbours while M_n is written, hence the separate `next` arrays.
 
+// With `dcb` and `l2` (the Born adjoint's background pass) the adjoint of
+// curl H is cb lambda + dcb lambda2: the scattered E update adds dcb curl H of
+// the background, so the scattered adjoint lambda2 reaches it too.
 template <typename T, int S>
 void e_adjoint(const Grid<T> &g, const T *cb, const T *l_ey, T *l_hx, T *l_hz,
                const T *m_hx_z, const T *m_hz_x, T *m_hx_z_n, T *m_hz_x_n,
-               int64_t shot) {
+               int64_t shot, const T *dcb = nullptr, const T *l2 = nullptr) {
   constexpr int P = Coef<S>::P;
   const int64_t ny = g.ny, nx = g.nx, base = shot * g.numel();
   auto in_e = [&](int64_t y, int64_t x) {
@@ -299,7 +374,8 @@ void e_adjoint(const Grid<T> &g, const T *cb, const T *l_ey, T *l_hx, T *l_hz,
   };
   auto q_at = [&](int64_t y, int64_t x) {
     const int64_t j = y * nx + x;
-    return (g.cb_b ? cb[base + j] : cb[j]) * l_ey[base + j];
+    const T q = (g.cb_b ? cb[base + j] : cb[j]) * l_ey[base + j];
+    return l2 ? q + dcb[j] * l2[base + j] : q;
   };
   // raw x-derivative adjoint at (y, x) (FusedRawXAcc).
   auto raw_x = [&](int64_t y, int64_t x) -> T {
@@ -330,10 +406,13 @@ void e_adjoint(const Grid<T> &g, const T *cb, const T *l_ey, T *l_hx, T *l_hz,
   }
 }
 
+// With `dca` and `l2`: lambda_Ey = ca lambda_Ey + dca lambda2 (the scattered
+// E update adds dca Ey^n of the background) before the derivative terms.
 template <typename T, int S>
 void h_adjoint(const Grid<T> &g, const T *ca, const T *cq, const T *l_hx,
                const T *l_hz, T *l_ey, const T *m_ey_x, const T *m_ey_z,
-               T *m_ey_x_n, T *m_ey_z_n, int64_t shot) {
+               T *m_ey_x_n, T *m_ey_z_n, int64_t shot, const T *dca = nullptr,
+               const T *l2 = nullptr) {
   constexpr int P = Coef<S>::P;
   const int64_t ny = g.ny, nx = g.nx, base = shot * g.numel();
   const int64_t y1h = g.pml_y1 > g.pml_y0 ? g.pml_y1 - 1 : g.pml_y0;
@@ -368,7 +447,7 @@ void h_adjoint(const Grid<T> &g, const T *ca, const T *cq, const T *l_hx,
       m_ey_z_n[i] = (y < ny - P && py) ? g.byh[y] * m_ey_z[i] + adj_y : (T)0;
       m_ey_x_n[i] = (x < nx - P && px) ? g.bxh[x] * m_ey_x[i] + adj_x : (T)0;
       const T cav = g.ca_b ? ca[i] : ca[j];
-      l_ey[i] = cav * l_ey[i];
+      l_ey[i] = l2 ? cav * l_ey[i] + dca[j] * l2[i] : cav * l_ey[i];
       l_ey[i] += d_half_adj<T, S>([&](int o) { return raw_z(y + o, x); }, g.rdy) +
                  d_half_adj<T, S>([&](int o) { return raw_xh(y, x + o); }, g.rdx);
     }
@@ -456,33 +535,39 @@ void backward(const T *ca, const T *cb, const T *cq, const T *grad_r, T *l_ey,
 // --------------------------------------------------- tangent-linear forward
 // Runs the background and the coefficient/source tangent together:
 //   dEy' = ca dEy + cb dcurl + dca Ey^n + dcb curl^n;  dEy[src] += df.
-// dca/dcb are [ny, nx] (unbatched) perturbations.
+// dca/dcb are [ny, nx] (unbatched) perturbations.  With the four history
+// arrays ([nt, shots, ny, nx] each, null for none) it keeps, per step, the
+// background's Ey^n and curl^n and the tangent's dEy^n and dcurl^n (the
+// Born adjoint's snapshots).
 template <typename T, int S>
 void tangent(const T *ca, const T *cb, const T *cq, const T *f, const T *dca,
              const T *dcb, const T *df, T *ey, T *hx, T *hz, T *m_ey_x,
              T *m_ey_z, T *m_hx_z, T *m_hz_x, T *dey, T *dhx, T *dhz,
              T *dm_ey_x, T *dm_ey_z, T *dm_hx_z, T *dm_hz_x, T *r, T *dr,
              const Grid<T> &g, const int64_t *src, const int64_t *rec,
-             int64_t n_src, int64_t n_rec, int64_t nt) {
+             int64_t n_src, int64_t n_rec, int64_t nt, T *bg_e_hist = nullptr,
+             T *bg_c_hist = nullptr, T *sc_e_hist = nullptr, T *sc_c_hist = nullptr) {
   constexpr int P = Coef<S>::P;
-  const int64_t numel = g.numel(), ny = g.ny, nx = g.nx;
-  std::vector<T> ey_n(numel), curl_n(numel);
+  const int64_t numel = g.numel(), ny = g.ny, nx = g.nx, total = g.n_shots * numel;
+  std::vector<T> ey_n(total), curl_n(total);
   for (int64_t t = 0; t < nt; ++t) {
+    T *be = bg_e_hist ? bg_e_hist + t * total : ey_n.data();
+    T *bc = bg_c_hist ? bg_c_hist + t * total : curl_n.data();
+    T *se = sc_e_hist ? sc_e_hist + t * total : nullptr;
+    T *scc = sc_c_hist ? sc_c_hist + t * total : nullptr;
     for (int64_t s = 0; s < g.n_shots; ++s) {
       const int64_t base = s * numel;
       // Background step, capturing Ey^n and curl^n.
       h_step<T, S>(g, cq, ey, hx, hz, m_ey_x, m_ey_z, s);
-      e_step<T, S>(g, ca, cb, hx, hz, ey, m_hx_z, m_hz_x,
-                   ey_n.data() - base, curl_n.data() - base, 0, s);
+      e_step<T, S>(g, ca, cb, hx, hz, ey, m_hx_z, m_hz_x, be, bc, 0, s);
       // Tangent: the same linear step on the d-fields...
       h_step<T, S>(g, cq, dey, dhx, dhz, dm_ey_x, dm_ey_z, s);
-      e_step<T, S>(g, ca, cb, dhx, dhz, dey, dm_hx_z, dm_hz_x, nullptr,
-                   nullptr, 0, s);
+      e_step<T, S>(g, ca, cb, dhx, dhz, dey, dm_hx_z, dm_hz_x, se, scc, 0, s);
       // ...plus the coefficient terms on the E-update domain.
       for (int64_t y = P; y < ny - P + 1; ++y)
         for (int64_t x = P; x < nx - P + 1; ++x) {
           const int64_t j = y * nx + x;
-          dey[base + j] += dca[j] * ey_n[j] + dcb[j] * curl_n[j];
+          dey[base + j] += dca[j] * be[base + j] + dcb[j] * bc[base + j];
         }
       for (int64_t k = 0; k < n_src; ++k) {
         const int64_t q = s * n_src + k;
@@ -500,6 +585,98 @@ void tangent(const T *ca, const T *cb, const T *cq, const T *f, const T *dca,
       }
     }
   }
+}
+
+// ------------------------------------------------- adjoint of the tangent
+// Transpose of `tangent` with respect to every input it is linear or
+// differentiable in: given cotangents w (on dr) and w_bg (on r), returns
+// d<dr, w> + <r, w_bg> with respect to ca, cb (FP64, through both the
+// background and the tangent recurrences), dca, dcb (FP64), f and df (per
+// step).  Per reverse step, the background adjoint eta runs first, coupled to
+// the tangent adjoint lambda of the same step (the tangent E update reads the
+// background's Ey^n and curl^n), then lambda runs the plain adjoint step;
+// receivers, sources and the FP64 terms follow as in `backward`:
+//   dJ/dca  += eta Ey^n + lambda dEy^n     dJ/dcb  += eta curl^n + lambda dcurl^n
+//   dJ/ddca += lambda Ey^n                 dJ/ddcb += lambda curl^n
+// lam / eta: 7 arrays each (Ey, Hx, Hz, m_Ey_x, m_Ey_z, m_Hx_z, m_Hz_x), in/out.
+template <typename T, int S>
+void tangent_adjoint(const T *ca, const T *cb, const T *cq, const T *dca, const T *dcb,
+                     const T *w, const T *w_bg, const T *bg_e, const T *bg_c, const T *sc_e,
+                     const T *sc_c, T *const *lam, T *const *eta, T *g_df, T *g_f0,
+                     double *g_ca, double *g_cb, double *g_dca, double *g_dcb,
+                     const Grid<T> &g, const int64_t *src, const int64_t *rec,
+                     int64_t n_src, int64_t n_rec, int64_t nt) {
+  constexpr int P = Coef<S>::P;
+  const int64_t numel = g.numel(), total = g.n_shots * numel;
+  std::vector<T> nxt(8 * total, (T)0);
+  // memories: [system][ey_x, ey_z, hx_z, hz_x]
+  T *cur[2][4], *nx_[2][4];
+  for (int k = 0; k < 4; ++k) {
+    cur[0][k] = eta[3 + k];
+    cur[1][k] = lam[3 + k];
+    nx_[0][k] = nxt.data() + (size_t)k * total;
+    nx_[1][k] = nxt.data() + (size_t)(4 + k) * total;
+  }
+  std::vector<double> a_ca(total, 0.0), a_cb(total, 0.0), a_dca(total, 0.0), a_dcb(total, 0.0);
+  T *const sys_f[2][3] = {{eta[0], eta[1], eta[2]}, {lam[0], lam[1], lam[2]}};
+  for (int64_t t = nt - 1; t >= 0; --t) {
+    for (int k = 0; k < 2; ++k) {  // 0: background (coupled), 1: tangent
+      const T *l2 = k == 0 ? lam[0] : nullptr;
+#pragma omp parallel for schedule(static)
+      for (int64_t s = 0; s < g.n_shots; ++s)
+        e_adjoint<T, S>(g, cb, sys_f[k][0], sys_f[k][1], sys_f[k][2], cur[k][2], cur[k][3],
+                        nx_[k][2], nx_[k][3], s, dcb, l2);
+      std::swap(cur[k][2], nx_[k][2]);
+      std::swap(cur[k][3], nx_[k][3]);
+#pragma omp parallel for schedule(static)
+      for (int64_t s = 0; s < g.n_shots; ++s)
+        h_adjoint<T, S>(g, ca, cq, sys_f[k][1], sys_f[k][2], sys_f[k][0], cur[k][0],
+                        cur[k][1], nx_[k][0], nx_[k][1], s, dca, l2);
+      std::swap(cur[k][0], nx_[k][0]);
+      std::swap(cur[k][1], nx_[k][1]);
+    }
+    T *l_ey = lam[0], *e_ey = eta[0];
+    for (int64_t s = 0; s < g.n_shots; ++s) {
+      for (int64_t k = 0; k < n_rec; ++k) {
+        const int64_t q = s * n_rec + k;
+        if (rec[q] < 0) continue;
+        if (w) l_ey[s * numel + rec[q]] += w[t * g.n_shots * n_rec + q];
+        if (w_bg) e_ey[s * numel + rec[q]] += w_bg[t * g.n_shots * n_rec + q];
+      }
+      for (int64_t k = 0; k < n_src; ++k) {
+        const int64_t q = s * n_src + k;
+        if (src[q] < 0) continue;
+        if (g_df) g_df[t * g.n_shots * n_src + q] = l_ey[s * numel + src[q]];
+        if (g_f0) g_f0[t * g.n_shots * n_src + q] = e_ey[s * numel + src[q]];
+      }
+    }
+    const int64_t o = t * total;
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < g.n_shots; ++s)
+      for (int64_t y = P; y < g.ny - P + 1; ++y)
+        for (int64_t x = P; x < g.nx - P + 1; ++x) {
+          const int64_t i = s * numel + y * g.nx + x;
+          const double le = (double)l_ey[i], ee = (double)e_ey[i];
+          a_ca[i] += ee * (double)bg_e[o + i];
+          a_ca[i] += le * (double)sc_e[o + i];
+          a_cb[i] += ee * (double)bg_c[o + i];
+          a_cb[i] += le * (double)sc_c[o + i];
+          a_dca[i] += le * (double)bg_e[o + i];
+          a_dcb[i] += le * (double)bg_c[o + i];
+        }
+  }
+  for (int k = 0; k < 4; ++k) {
+    if (cur[0][k] != eta[3 + k]) std::memcpy(eta[3 + k], cur[0][k], total * sizeof(T));
+    if (cur[1][k] != lam[3 + k]) std::memcpy(lam[3 + k], cur[1][k], total * sizeof(T));
+  }
+  for (int64_t s = 0; s < g.n_shots; ++s)
+    for (int64_t j = 0; j < numel; ++j) {
+      const int64_t i = s * numel + j;
+      if (g_ca) g_ca[j] += a_ca[i];
+      if (g_cb) g_cb[j] += a_cb[i];
+      if (g_dca) g_dca[j] += a_dca[i];
+      if (g_dcb) g_dcb[j] += a_dcb[i];
+    }
 }
 
 template <typename T>
@@ -565,6 +742,31 @@ int dispatch_backward(int stencil, const T *ca, const T *cb, const T *cq,
     case 8: TIDE_ORACLE_BWD(8); return 0;
   }
 #undef TIDE_ORACLE_BWD
+  return 1;
+}
+
+// Debye-dispersive forward (unbatched coefficients): debye = {a, b, cp}
+// ([n_poles, ny, nx] each), pol [shots, n_poles, ny, nx] and ey_prev
+// [shots, ny, nx] in/out.
+template <typename T>
+int dispatch_debye(int stencil, const T *ca, const T *cb, const T *cq, const T *f,
+                   T *const *fields, const T *const *debye, T *pol, T *ey_prev, int64_t n_poles,
+                   T *r, const T *const *prof, const int64_t *src, const int64_t *rec, T rdy,
+                   T rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+                   int64_t n_rec, int64_t start_t, const int64_t *pml) {
+  const bool batched[3] = {false, false, false};
+  Grid<T> g = make_grid<T>(prof, rdy, rdx, n_shots, ny, nx, pml, batched);
+#define TIDE_ORACLE_DEB(S)                                                                \
+  forward_debye<T, S>(ca, cb, cq, f, fields[0], fields[1], fields[2], fields[3], fields[4], \
+                      fields[5], fields[6], debye[0], debye[1], debye[2], pol, ey_prev,    \
+                      n_poles, r, g, src, rec, n_src, n_rec, nt, start_t)
+  switch (stencil) {
+    case 2: TIDE_ORACLE_DEB(2); return 0;
+    case 4: TIDE_ORACLE_DEB(4); return 0;
+    case 6: TIDE_ORACLE_DEB(6); return 0;
+    case 8: TIDE_ORACLE_DEB(8); return 0;
+  }
+#undef TIDE_ORACLE_DEB
   return 1;
 }
 
@@ -659,6 +861,32 @@ int oracle_tm2d_backward_f64(int stencil, const double *ca, const double *cb,
                                    cb_rg);
 }
 
+int oracle_tm2d_forward_debye_f32(int stencil, const float *ca, const float *cb,
+                                  const float *cq, const float *f, float *const *fields,
+                                  const float *const *debye, float *pol, float *ey_prev,
+                                  int64_t n_poles, float *r, const float *const *prof,
+                                  const int64_t *src, const int64_t *rec, float rdy, float rdx,
+                                  int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,
+                                  int64_t n_src, int64_t n_rec, int64_t start_t,
+                                  const int64_t *pml) {
+  return dispatch_debye<float>(stencil, ca, cb, cq, f, fields, debye, pol, ey_prev, n_poles, r,
+                               prof, src, rec, rdy, rdx, nt, n_shots, ny, nx, n_src, n_rec,
+                               start_t, pml);
+}
+
+int oracle_tm2d_forward_debye_f64(int stencil, const double *ca, const double *cb,
+                                  const double *cq, const double *f, double *const *fields,
+                                  const double *const *debye, double *pol, double *ey_prev,
+                                  int64_t n_poles, double *r, const double *const *prof,
+                                  const int64_t *src, const int64_t *rec, double rdy,
+                                  double rdx, int64_t nt, int64_t n_shots, int64_t ny,
+                                  int64_t nx, int64_t n_src, int64_t n_rec, int64_t start_t,
+                                  const int64_t *pml) {
+  return dispatch_debye<double>(stencil, ca, cb, cq, f, fields, debye, pol, ey_prev, n_poles,
+                                r, prof, src, rec, rdy, rdx, nt, n_shots, ny, nx, n_src, n_rec,
+                                start_t, pml);
+}
+
 // Tangent-linear forward in FP64 (unbatched coefficients).  `fields` and
 // `dfields` are the 7 background and 7 tangent state arrays (in/out).
 int oracle_tm2d_tangent_f64(int stencil, const double *ca, const double *cb,
@@ -686,6 +914,64 @@ int oracle_tm2d_tangent_f64(int stencil, const double *ca, const double *cb,
     case 8: TIDE_ORACLE_TAN(8); return 0;
   }
 #undef TIDE_ORACLE_TAN
+  return 1;
+}
+
+// Tangent-linear forward that also keeps the four snapshot histories of the
+// tangent adjoint (bg_e, bg_c, sc_e, sc_c: [nt, shots, ny, nx] each).
+int oracle_tm2d_tangent_store_f64(int stencil, const double *ca, const double *cb,
+                                  const double *cq, const double *f, const double *dca,
+                                  const double *dcb, const double *df,
+                                  double *const *fields, double *const *dfields, double *r,
+                                  double *dr, const double *const *prof, const int64_t *src,
+                                  const int64_t *rec, double rdy, double rdx, int64_t nt,
+                                  int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+                                  int64_t n_rec, const int64_t *pml, double *const *hist) {
+  const bool batched[3] = {false, false, false};
+  Grid<double> g = make_grid<double>(prof, rdy, rdx, n_shots, ny, nx, pml, batched);
+#define TIDE_ORACLE_TANS(S)                                                     \
+  tangent<double, S>(ca, cb, cq, f, dca, dcb, df, fields[0], fields[1],         \
+                     fields[2], fields[3], fields[4], fields[5], fields[6],     \
+                     dfields[0], dfields[1], dfields[2], dfields[3],            \
+                     dfields[4], dfields[5], dfields[6], r, dr, g, src, rec,    \
+                     n_src, n_rec, nt, hist[0], hist[1], hist[2], hist[3])
+  switch (stencil) {
+    case 2: TIDE_ORACLE_TANS(2); return 0;
+    case 4: TIDE_ORACLE_TANS(4); return 0;
+    case 6: TIDE_ORACLE_TANS(6); return 0;
+    case 8: TIDE_ORACLE_TANS(8); return 0;
+  }
+#undef TIDE_ORACLE_TANS
+  return 1;
+}
+
+// Adjoint of the tangent-linear forward (FP64, unbatched); see
+// tangent_adjoint.  hist as written by oracle_tm2d_tangent_store_f64; grads:
+// g_df, g_f0 ([nt, shots, n_src]), g_ca, g_cb, g_dca, g_dcb ([ny, nx] FP64,
+// added to); any may be null.
+int oracle_tm2d_tangent_adjoint_f64(int stencil, const double *ca, const double *cb,
+                                    const double *cq, const double *dca, const double *dcb,
+                                    const double *w, const double *w_bg,
+                                    const double *const *hist, double *const *lam,
+                                    double *const *eta, double *g_df, double *g_f0,
+                                    double *g_ca, double *g_cb, double *g_dca, double *g_dcb,
+                                    const double *const *prof, const int64_t *src,
+                                    const int64_t *rec, double rdy, double rdx, int64_t nt,
+                                    int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,
+                                    int64_t n_rec, const int64_t *pml) {
+  const bool batched[3] = {false, false, false};
+  Grid<double> g = make_grid<double>(prof, rdy, rdx, n_shots, ny, nx, pml, batched);
+#define TIDE_ORACLE_TADJ(S)                                                           \
+  tangent_adjoint<double, S>(ca, cb, cq, dca, dcb, w, w_bg, hist[0], hist[1], hist[2], \
+                             hist[3], lam, eta, g_df, g_f0, g_ca, g_cb, g_dca, g_dcb, g, \
+                             src, rec, n_src, n_rec, nt)
+  switch (stencil) {
+    case 2: TIDE_ORACLE_TADJ(2); return 0;
+    case 4: TIDE_ORACLE_TADJ(4); return 0;
+    case 6: TIDE_ORACLE_TADJ(6); return 0;
+    case 8: TIDE_ORACLE_TADJ(8); return 0;
+  }
+#undef TIDE_ORACLE_TADJ
   return 1;
 }
 

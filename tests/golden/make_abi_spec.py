@@ -26,6 +26,8 @@ TEMPLATES = {
     "maxwell_tm_forward_with_storage": "_TM_FORWARD_WITH_STORAGE_SPEC",
     "maxwell_tm_backward": "_TM_BACKWARD_SPEC",
     "maxwell_tm_born_forward": "_TM_BORN_FORWARD_SPEC",
+    "maxwell_tm_born_forward_with_storage": "_TM_BORN_FORWARD_WITH_STORAGE_SPEC",
+    "maxwell_tm_born_backward_bggrad": "_TM_BORN_BACKWARD_BGGRAD_SPEC",
     "maxwell_3d_forward": "_3D_FORWARD_SPEC",
     "maxwell_3d_forward_with_storage": "_3D_FORWARD_WITH_STORAGE_SPEC",
     "maxwell_3d_backward": "_3D_BACKWARD_SPEC",

@@ -35,7 +35,8 @@ def test_header_declares_reference_entry_points():
 SPEC = json.loads((Path(__file__).resolve().parent / "golden" / "abi_spec.json").read_text())
 REF_ENTRY = [(prop, d, p) for prop in ("tm", "3d") for d in ("float", "double")
              for p in ("forward", "forward_with_storage", "backward")]
-REF_ENTRY += [("tm", d, "born_forward") for d in ("float", "double")]
+REF_ENTRY += [("tm", d, p) for d in ("float", "double")
+              for p in ("born_forward", "born_forward_with_storage", "born_backward_bggrad")]
 
 
 def _prototype(name: str) -> list[tuple[str, str]]:

@@ -31,9 +31,11 @@ DEV = "cuda:0"
 def _default_tuning():
     _native.set_em3d_tuning(8, 4, 8)  # the library's defaults
     _native.set_em3d_forward_group(1)
+    _native.set_em3d_tile_order(0)
     yield
     _native.set_em3d_tuning(8, 4, 8)
     _native.set_em3d_forward_group(1)
+    _native.set_em3d_tile_order(0)
 
 
 def _case(dtype=torch.float32, **kw):
@@ -92,9 +94,11 @@ def test_em3d_adjoint_many_shots_and_tunings():
     rng = np.random.default_rng(5)
     w = rng.standard_normal(r_o.shape)
     go = O.em3d_backward(case, w, s_o)
-    for fz, az, grp in ((8, 8, 8), (4, 4, 3), (16, 8, 1)):
-        print(f"case tuning={fz},{az},{grp}", flush=True)
+    for fz, az, grp, order in ((8, 8, 8, 0), (4, 4, 3, 0), (16, 8, 1, 0), (8, 4, 3, 1),
+                               (4, 8, 2, 1)):
+        print(f"case tuning={fz},{az},{grp} tile order {order}", flush=True)
         _native.set_em3d_tuning(fz, az, grp)
+        _native.set_em3d_tile_order(order)
         f_g, r_g, _ = gpu_em3d_forward(case)
         assert np.array_equal(r_g, r_o)
         gg = gpu_em3d_backward(case, w, s_o)

@@ -215,3 +215,147 @@ def test_high_order_tangent_matches_central_differences(stencil):
     _, rp, _, _ = O.tm2d_forward(plus)
     _, rm, _, _ = O.tm2d_forward(minus)
     assert relative_l2((rp - rm) / (2 * h), dr) < 1e-6
+
+
+# ------------------------------------------------------ adjoint of the tangent
+def _tangent_inputs(case, seed):
+    rng = np.random.default_rng(seed)
+    dca = rng.standard_normal((case.ny, case.nx)) * 1e-3
+    dcb = rng.standard_normal((case.ny, case.nx)) * 1e-3 * float(np.abs(case.cb).mean())
+    df = rng.standard_normal(case.f.shape) * float(np.abs(case.f).max()) * 1e-2
+    return dca, dcb, df
+
+
+@pytest.mark.parametrize("stencil", [2, 4])
+def test_tangent_adjoint_dot_product_in_the_direction(stencil):
+    """<dr(dca, dcb, df), w> = <g_dca, dca> + <g_dcb, dcb> + <g_df, df>: the
+    tangent is linear in the direction, so its adjoint satisfies this
+    identity to FP64 rounding for any w."""
+    st = prepare_tm2d(**hetero_problem(seed=70 + stencil, dtype=torch.float64, ny=30, nx=36,
+                                       nt=70, shots=2), stencil=stencil)
+    case = O.case_from_setup(st)
+    dca, dcb, df = _tangent_inputs(case, stencil)
+    _, dr, hist = O.tm2d_tangent_store(case, dca, dcb, df)
+    w = np.random.default_rng(9).standard_normal(dr.shape)
+    g = O.tm2d_tangent_adjoint(case, dca, dcb, w, None, hist)
+    lhs = float((dr * w).sum())
+    rhs = float((g["dca"] * dca).sum() + (g["dcb"] * dcb).sum() + (g["df"] * df).sum())
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+def test_tangent_adjoint_background_part_is_the_plain_adjoint():
+    """With a cotangent on the background receivers only, the tangent
+    adjoint's ca/cb gradients and source gradient are the plain adjoint's
+    (`backward`), and <r, w_bg> = <g_f, f> (r is linear in f)."""
+    case = _f64_case(nt=80)
+    zero = np.zeros((case.ny, case.nx))
+    r, _, hist = O.tm2d_tangent_store(case, zero, zero, np.zeros_like(case.f))
+    w_bg = np.random.default_rng(3).standard_normal(r.shape)
+    g = O.tm2d_tangent_adjoint(case, zero, zero, None, w_bg, hist)
+    _, _, es, cs = O.tm2d_forward(case, store="full")
+    gca, gcb, gf, _ = O.tm2d_backward(case, w_bg, es, cs)
+    assert relative_l2(g["ca"], gca.reshape(g["ca"].shape)) < 1e-12
+    assert relative_l2(g["cb"], gcb.reshape(g["cb"].shape)) < 1e-12
+    assert relative_l2(g["f"], gf) < 1e-12
+    assert abs(float((r * w_bg).sum()) - float((g["f"] * case.f).sum())) <= 1e-11 * abs(float((r * w_bg).sum()))
+
+
+@pytest.mark.parametrize("stencil", [2, 4])
+def test_tangent_adjoint_model_gradient_matches_central_differences(stencil):
+    """The second-order part: d<dr, w>/dca and d<dr, w>/dcb (through the
+    background and the tangent recurrences) against central differences of
+    <dr, w> along random ca and cb perturbations, and <dr, w> = <g_f, f>
+    with df = 0 (dr is linear in the background source)."""
+    st = prepare_tm2d(**hetero_problem(seed=80 + stencil, dtype=torch.float64, ny=30, nx=34,
+                                       nt=70, shots=2), stencil=stencil)
+    case = O.case_from_setup(st)
+    dca, dcb, _ = _tangent_inputs(case, 10 + stencil)
+    df0 = np.zeros_like(case.f)
+    _, dr, hist = O.tm2d_tangent_store(case, dca, dcb, df0)
+    w = np.random.default_rng(5).standard_normal(dr.shape)
+    g = O.tm2d_tangent_adjoint(case, dca, dcb, w, None, hist)
+    rng = np.random.default_rng(20 + stencil)
+    for key, scale in (("ca", 1e-3), ("cb", 1e-3 * float(np.abs(case.cb).mean()))):
+        u = rng.standard_normal((case.ny, case.nx)) * scale
+        h = 1e-4
+
+        def objective(sign):
+            moved = O.TMCase(**{**case.__dict__, key: getattr(case, key) + sign * h * u[None]})
+            return float((O.tm2d_tangent(moved, dca, dcb, df0)[1] * w).sum())
+
+        fd = (objective(1) - objective(-1)) / (2 * h)
+        an = float((g[key] * u).sum())
+        assert abs(an - fd) <= 1e-7 * abs(fd), (key, an, fd)
+    assert abs(float((dr * w).sum()) - float((g["f"] * case.f).sum())) <= 1e-11 * abs(float((dr * w).sum()))
+
+
+# ------------------------------------------------------------------ Debye
+def _debye_case(dtype=torch.float64, stencil=2, poles=2, seed=3, **kw):
+    p = hetero_problem(seed=seed, dtype=dtype, **kw)
+    g = torch.Generator().manual_seed(seed + 100)
+    shape = p["epsilon"].shape
+    d_eps = (2.0 + 3.0 * torch.rand((poles, *shape), generator=g, dtype=torch.float64)).to(dtype)
+    tau = (torch.tensor([1e-9, 3e-10][:poles], dtype=torch.float64)
+           .reshape(poles, 1, 1).expand(poles, *shape)).to(dtype)
+    st = prepare_tm2d(**p, stencil=stencil, dispersion=tide.DebyeDispersion(d_eps, tau))
+    deb = st.extra["debye"]
+    return O.case_from_setup(st), [deb[k].numpy() for k in ("a", "b", "cp")]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference CPU library not built here")
+@pytest.mark.parametrize("dtype,stencil", [(torch.float64, 2), (torch.float32, 2), (torch.float64, 4)])
+def test_debye_forward_bit_identical_to_reference_cpu(dtype, stencil):
+    """The oracle's Debye forward against the reference's own CPU Debye
+    forward (maxwell_tm_<s>_<dtype>_forward_cpu with has_dispersion),
+    compiled from its sources: receivers, fields and polarization bitwise."""
+    case, deb = _debye_case(dtype, stencil, nt=80, shots=2)
+    f_o, r_o, pol_o, _ = O.tm2d_forward_debye(case, *deb)
+    f_r, r_r, pol_r = O.ref_tm2d_forward_debye(case, *deb)
+    assert np.abs(r_o).max() > 0
+    assert np.array_equal(r_o, r_r)
+    for a, b in zip(f_o, f_r):
+        assert np.array_equal(a, b)
+    # polarization in the E-update domain (the reference's ey_prev outside
+    # it is an uninitialised buffer in its own operator; both start at zero here)
+    assert np.array_equal(pol_o, pol_r)
+
+
+def test_debye_coefficients_closed_forms():
+    """delta_epsilon = 0 gives the plain coefficients and a zero
+    polarization update; the trapezoidal pole recurrence has the static
+    susceptibility b * 2 / (1 - a) = EP0 * delta_epsilon; cp = (1 - a) / den."""
+    from tide.utils import compile_material_coefficients, prepare_parameters
+
+    eps = torch.full((3, 4), 4.0, dtype=torch.float64)
+    sig = torch.full_like(eps, 0.01)
+    mu = torch.ones_like(eps)
+    dt = 1e-11
+    plain = prepare_parameters(eps, sig, mu, dt)
+    zero = compile_material_coefficients(eps, sig, mu, dt, dispersion=tide.DebyeDispersion(0.0, 1e-9))
+    assert torch.allclose(zero["ca"], plain[0], rtol=0, atol=1e-15)
+    assert torch.allclose(zero["cb"], plain[1], rtol=1e-15, atol=0)
+    assert float(zero["debye"]["b"].abs().max()) == 0.0
+    c = compile_material_coefficients(eps, sig, mu, dt,
+                                      dispersion=tide.DebyeDispersion(torch.tensor([2.0, 5.0]).reshape(2, 1, 1).double(),
+                                                                      torch.tensor([1e-9, 2e-10]).reshape(2, 1, 1).double()))
+    a, b = c["debye"]["a"], c["debye"]["b"]
+    static = 2.0 * b / (1.0 - a)
+    expect = tide.EP0 * torch.tensor([2.0, 5.0], dtype=torch.float64).reshape(2, 1, 1)
+    assert torch.allclose(static, expect.expand_as(static), rtol=1e-12)
+    den = tide.EP0 * eps + sig * dt / 2 + b.sum(0)
+    assert torch.allclose(c["debye"]["cp"], (1 - a) / den, rtol=1e-14)
+    with pytest.raises(ValueError, match="dt < min"):
+        compile_material_coefficients(eps, sig, mu, 1e-9, dispersion=tide.DebyeDispersion(1.0, 1e-9))
+
+
+def test_debye_with_zero_poles_is_the_plain_forward():
+    """delta_epsilon = 0: the Debye forward is bit-identical to the plain one
+    (the polarization stays zero and adds exactly nothing)."""
+    p = hetero_problem(seed=4, dtype=torch.float64, nt=60)
+    st = prepare_tm2d(**p, dispersion=tide.DebyeDispersion(0.0, 1e-9))
+    case = O.case_from_setup(st)
+    deb = [st.extra["debye"][k].numpy() for k in ("a", "b", "cp")]
+    _, r_d, pol, _ = O.tm2d_forward_debye(case, *deb)
+    _, r_p, _, _ = O.tm2d_forward(case)
+    assert not np.any(pol)
+    assert np.array_equal(r_d, r_p)

@@ -74,6 +74,7 @@ template <typename T> struct Args3 {
   int32_t per_shot_grad;
   int32_t n_shots, n_zb, src_comp, rec_comp;
   int64_t group_stride;  // shot group g adds into gca/gcb + g*group_stride
+  int32_t shot_fast;     // tile order (decode3)
 };
 
 // Snapshots are written once and read once, much later: non-temporal stores
@@ -101,12 +102,24 @@ struct Tile3 {
   int xt, y, zb, grp;
 };
 
-__device__ __forceinline__ Tile3 decode3(int n_xt, int ny, int n_zb, int n_groups) {
+// Tile and shot group of this workgroup.  Shot-major order (shot_fast 0)
+// puts all tiles of one shot group before the next group; shot-minor order
+// (shot_fast 1) puts the shot groups of one tile next to each other, so they
+// run on the same XCD at about the same time and share its L2 for the
+// coefficient lines and the neighbouring rows.
+__device__ __forceinline__ Tile3 decode3(int n_xt, int ny, int n_zb, int n_groups,
+                                         int shot_fast) {
   const int64_t n_tiles = (int64_t)n_xt * ny * n_zb;
   const int64_t flat = xcd_remap((int64_t)blockIdx.x, n_tiles * n_groups);
   Tile3 c;
-  c.grp = (int)(flat / n_tiles);
-  int64_t t = flat - (int64_t)c.grp * n_tiles;
+  int64_t t;
+  if (shot_fast) {
+    t = flat / n_groups;
+    c.grp = (int)(flat - t * n_groups);
+  } else {
+    c.grp = (int)(flat / n_tiles);
+    t = flat - (int64_t)c.grp * n_tiles;
+  }
   c.zb = (int)(t / ((int64_t)n_xt * ny));
   t -= (int64_t)c.zb * n_xt * ny;
   c.y = (int)(t / n_xt);
@@ -151,7 +164,7 @@ __global__ __launch_bounds__(512) void h3_step(Args3<T> a) {
   const Geom3<T> &g = a.g;
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
-  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
+  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups, a.shot_fast);
   // threadIdx.y is one value per wavefront (64-wide rows): made explicit so
   // the shot's volume pointers stay scalar.
   const int s = tc.grp * G + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
@@ -255,7 +268,7 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
   const Geom3<T> &g = a.g;
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
-  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
+  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups, a.shot_fast);
   const int s = tc.grp * G + __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one per wavefront
   if (s >= a.n_shots) return;
   const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
@@ -398,7 +411,7 @@ __global__ __launch_bounds__(512) void adj_e3(Args3<T> a) {
   const Geom3<T> &g = a.g;
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
-  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
+  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups, a.shot_fast);
   const int s = tc.grp * G + __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one per wavefront
   if (s >= a.n_shots) return;
   const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
@@ -584,7 +597,7 @@ __global__ __launch_bounds__(512) void adj_h3(Args3<T> a) {
   const int G = (int)blockDim.y;
   const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one per wavefront
   const int n_groups = (a.n_shots + G - 1) / G;
-  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups);
+  const Tile3 tc = decode3(a.pts.n_xt, g.ny, a.n_zb, n_groups, a.shot_fast);
   const int s = tc.grp * G + w;
   const int nz = g.nz, ny = g.ny, nx = g.nx, y = tc.y;
   const int lane = threadIdx.x;
@@ -888,6 +901,9 @@ struct Tuning3 {
   // one-launch forward step (em3d_fused.h): rows per wavefront (2 or 4; 0
   // runs the two-kernel step) and planes per march (8 or 16)
   int fused_rows = 0, fused_z = 8;
+  // workgroup order: shot groups of one tile adjacent (1) or all tiles of
+  // one shot group first (0)
+  int shot_fast = 0;
 };
 Tuning3 g_tuning3;
 
@@ -994,6 +1010,7 @@ void fill_common(Args3<T> &a, const Problem3<T> &p, const DevicePointTable &dpt,
   a.gca = a.gcb = nullptr;
   a.per_shot_grad = 0;
   a.group_stride = 0;
+  a.shot_fast = g_tuning3.shot_fast;
   a.step_scale = 1.0;
   for (int k = 0; k < 6; ++k) a.st[k] = nullptr;
   for (int k = 0; k < N_SLOTS; ++k) a.mn[k] = nullptr;
@@ -1080,6 +1097,7 @@ int forward_fused(const Problem3<T> &p, T *const *fields, const T *f, T *r, int6
   a.n_zb = (p.g.nz + tz - 1) / tz;
   a.src_comp = p.src_comp;
   a.rec_comp = p.rec_comp;
+  a.shot_fast = g_tuning3.shot_fast;
   for (int k = 0; k < 6; ++k) a.me[k] = fields[kEMem[k]];
   const int group = std::min<int>(p.n_shots, g_tuning3.fwd_group);
   const int64_t n_groups = (p.n_shots + group - 1) / group;
@@ -1788,6 +1806,15 @@ extern "C" int tide_em3d_set_fused(int rows, int planes) {
   }
   g_tuning3.fused_rows = rows;
   g_tuning3.fused_z = planes;
+  return kOk;
+}
+
+extern "C" int tide_em3d_set_tile_order(int shot_fast) {
+  if (shot_fast != 0 && shot_fast != 1) {
+    set_error("tile order: 0 (shot groups outermost) or 1 (shot groups of a tile adjacent)");
+    return kInvalidArgument;
+  }
+  g_tuning3.shot_fast = shot_fast;
   return kOk;
 }
 

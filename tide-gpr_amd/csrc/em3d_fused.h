@@ -38,6 +38,7 @@ template <typename T> struct Fused3 {
   const T *src_amp;
   T *rec_out;
   int32_t n_shots, n_yt, n_zb, src_comp, rec_comp;
+  int32_t shot_fast;  // workgroup order, as decode3
 };
 
 // CPML correction reading the memory from `mi` and writing the new one to
@@ -58,8 +59,15 @@ __global__ __launch_bounds__(256) void em3_fused(Fused3<T> a) {
   const int n_xt = a.pts.n_xt;
   const int64_t n_tiles = (int64_t)n_xt * a.n_yt * a.n_zb;
   const int64_t flat = xcd_remap((int64_t)blockIdx.x, n_tiles * n_groups);
-  const int grp = (int)(flat / n_tiles);
-  int64_t rem = flat - (int64_t)grp * n_tiles;
+  int grp;
+  int64_t rem;
+  if (a.shot_fast) {
+    rem = flat / n_groups;
+    grp = (int)(flat - rem * n_groups);
+  } else {
+    grp = (int)(flat / n_tiles);
+    rem = flat - (int64_t)grp * n_tiles;
+  }
   const int zb = (int)(rem / ((int64_t)n_xt * a.n_yt));
   rem -= (int64_t)zb * n_xt * a.n_yt;
   const int yt = (int)(rem / n_xt);

@@ -977,11 +977,24 @@ int backward_entry(T const *ca, T const *cb, T const *cq, T const *grad_r, T *la
       int64_t step_ratio, bool has_dispersion, bool ca_b, bool cb_b, bool cq_b,           \
       int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1,    \
       int64_t n_threads, int64_t device, void *compute_stream) {                          \
-    (void)debye_a; (void)debye_b; (void)debye_cp; (void)polarization;                     \
-    (void)ey_prev; (void)n_poles; (void)dt; (void)n_threads;                              \
+    (void)dt; (void)n_threads;                                                            \
     if (has_dispersion) {                                                                 \
-      set_error("Debye dispersion is not part of this build");                            \
-      return kUnsupported;                                                                \
+      if (step_ratio != 1) {                                                              \
+        set_error("Debye dispersion: step_ratio must be 1");                              \
+        return kInvalidArgument;                                                          \
+      }                                                                                   \
+      if (hipSetDevice((int)device) != hipSuccess) {                                      \
+        set_error("hipSetDevice failed");                                                 \
+        return kHipError;                                                                 \
+      }                                                                                   \
+      const T *prof[12] = {ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh};         \
+      const tm2d_ho::Problem<T> hp = ho_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots,   \
+                                                   ny, nx, n_src, n_rec, ca_b, cb_b,      \
+                                                   cq_b, pml_y0, pml_x0, pml_y1, pml_x1); \
+      T *const fld[7] = {ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x};                     \
+      return tm2d_ho::forward_debye<T>(S, hp, fld, f, r, sources_i, receivers_i, nt,      \
+                                       start_t, debye_a, debye_b, debye_cp, polarization, \
+                                       ey_prev, n_poles, (hipStream_t)compute_stream);    \
     }                                                                                     \
     return forward_entry<T>(ca, cb, cq, f, ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x, r, \
                             ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh,         \
@@ -1105,7 +1118,161 @@ TIDE_TM2D_ENTRY_POINTS(8, double, double)
     T *const sc[7] = {dey, dhx, dhz, dm_ey_x, dm_ey_z, dm_hx_z, dm_hz_x};                 \
     return tm2d_ho::born_forward<T>(S, hp, bg, sc, dca, dcb, f0, df, r, background_r,     \
                                     sources_i, receivers_i, nt, start_t,                  \
-                                    (hipStream_t)compute_stream);                         \
+                                    (hipStream_t)compute_stream, nullptr);                \
+  }                                                                                       \
+                                                                                          \
+  /* reference ABI src/tide/backend_utils.py:351-370 (_TM_BORN_FORWARD_WITH_STORAGE_SPEC): \
+     device storage of full-precision snapshots; ey_store_1 / curl_store_1 keep the    \
+     background's Ey^n / curl^n and dey_store / dcurl_store the scattered field's, one  \
+     [n_shots, ny, nx] slot per step (slot t for step t) */                             \
+  extern "C" int maxwell_tm_##S##_##NAME##_born_forward_with_storage_cuda(                \
+      T const *ca, T const *cb, T const *cq, T const *dca, T const *dcb, T const *f0,     \
+      T const *df, T *ey, T *hx, T *hz, T *m_ey_x, T *m_ey_z, T *m_hx_z, T *m_hz_x,       \
+      T *dey, T *dhx, T *dhz, T *dm_ey_x, T *dm_ey_z, T *dm_hx_z, T *dm_hz_x, T *r,       \
+      T *background_r, void *ey_store_1, void *ey_store_3, void *ey_filenames,            \
+      void *curl_store_1, void *curl_store_3, void *curl_filenames, void *dey_store,      \
+      void *dcurl_store, T const *ay, T const *by, T const *ayh, T const *byh,            \
+      T const *ax, T const *bx, T const *axh, T const *bxh, T const *ky, T const *kyh,    \
+      T const *kx, T const *kxh, int64_t const *sources_i, int64_t const *receivers_i,    \
+      T rdy, T rdx, T dt, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx,            \
+      int64_t n_src, int64_t n_rec, int64_t step_ratio, int64_t storage_mode,             \
+      int64_t storage_format, int64_t shot_bytes_uncomp, bool ca_rg, bool cb_rg,          \
+      bool ca_b, bool cb_b, bool cq_b, int64_t start_t, int64_t pml_y0, int64_t pml_x0,   \
+      int64_t pml_y1, int64_t pml_x1, int64_t n_threads, int64_t device,                  \
+      void *compute_stream, void *storage_stream) {                                       \
+    (void)dt; (void)n_threads; (void)ca_rg; (void)cb_rg; (void)storage_stream;            \
+    if (step_ratio != 1) {                                                                \
+      set_error("born_forward: step_ratio must be 1 (resample the sources beforehand)");  \
+      return kInvalidArgument;                                                            \
+    }                                                                                     \
+    const bool any = ey_store_1 || curl_store_1 || dey_store || dcurl_store;              \
+    const bool all = ey_store_1 && curl_store_1 && dey_store && dcurl_store;              \
+    if (any && (!all || storage_mode != kStorageDevice || storage_format == kFormatBf16 ||\
+                ey_store_3 || curl_store_3 || ey_filenames || curl_filenames ||           \
+                shot_bytes_uncomp != ny * nx * (int64_t)sizeof(T))) {                     \
+      set_error("born_forward_with_storage: the four snapshot stores are kept on the "    \
+                "device in full precision, all four or none, shot_bytes_uncomp = "        \
+                "ny * nx * sizeof(field)");                                               \
+      return kUnsupported;                                                                \
+    }                                                                                     \
+    if (hipSetDevice((int)device) != hipSuccess) {                                        \
+      set_error("hipSetDevice failed");                                                   \
+      return kHipError;                                                                   \
+    }                                                                                     \
+    const T *prof[12] = {ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh};           \
+    const tm2d_ho::Problem<T> hp = ho_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots, ny, \
+                                                 nx, n_src, n_rec, ca_b, cb_b, cq_b,      \
+                                                 pml_y0, pml_x0, pml_y1, pml_x1);         \
+    T *const bg[7] = {ey, hx, hz, m_ey_x, m_ey_z, m_hx_z, m_hz_x};                        \
+    T *const sc[7] = {dey, dhx, dhz, dm_ey_x, dm_ey_z, dm_hx_z, dm_hz_x};                 \
+    T *const hist[4] = {(T *)ey_store_1, (T *)curl_store_1, (T *)dey_store,               \
+                        (T *)dcurl_store};                                                \
+    return tm2d_ho::born_forward<T>(S, hp, bg, sc, dca, dcb, f0, df, r, background_r,     \
+                                    sources_i, receivers_i, nt, start_t,                  \
+                                    (hipStream_t)compute_stream, any ? hist : nullptr);   \
+  }                                                                                       \
+                                                                                          \
+  /* reference ABI src/tide/backend_utils.py:407-444 (_TM_BORN_BACKWARD_BGGRAD_SPEC):   \
+     ey/hx/hz + m_lambda_* the scattered adjoint (cotangent grad_r), dey/dhx/dhz +      \
+     m_eta_* the background adjoint (cotangent grad_background_r); gradients are       \
+     accumulated in FP64 and added to grad_ca, grad_cb, grad_dca, grad_dcb; grad_f0 /   \
+     grad_df are written per step.  Device storage, unbatched coefficients;          \
+     lambda_store, eta_source_old, work_eta_*, grad_*_shot are not used. */            \
+  extern "C" int maxwell_tm_##S##_##NAME##_born_backward_bggrad_cuda(                     \
+      T const *ca, T const *cb, T const *cq, T const *dca, T const *dcb, T const *f0,     \
+      T const *df, T const *grad_r, T const *grad_background_r, void *ey_store_1,         \
+      void *ey_store_3, void *ey_filenames, void *curl_store_1, void *curl_store_3,       \
+      void *curl_filenames, void *dey_store, void *dcurl_store, void *lambda_store,       \
+      int64_t background_n_shots, T *ey, T *hx, T *hz, T *dey, T *dhx, T *dhz,            \
+      T *grad_f0, T *grad_df, T *grad_ca, T *grad_cb, T *grad_dca, T *grad_dcb,           \
+      T *m_lambda_ey_x, T *m_lambda_ey_z, T *m_lambda_hx_z, T *m_lambda_hz_x,             \
+      T *m_eta_ey_x, T *m_eta_ey_z, T *m_eta_hx_z, T *m_eta_hz_x, T *eta_source_old,      \
+      T *work_eta_x, T *work_eta_z, T *grad_ca_shot, T *grad_cb_shot, T *grad_dca_shot,   \
+      T *grad_dcb_shot, T *adjoint_memory_scratch, T const *ay, T const *by,              \
+      T const *ayh, T const *byh, T const *ax, T const *bx, T const *axh, T const *bxh,   \
+      T const *ky, T const *kyh, T const *kx, T const *kxh, int64_t const *sources_i,     \
+      int64_t const *receivers_i, T rdy, T rdx, T dt, int64_t nt, int64_t n_shots,        \
+      int64_t ny, int64_t nx, int64_t n_src, int64_t n_rec, int64_t step_ratio,           \
+      int64_t storage_mode, int64_t storage_format, int64_t shot_bytes_uncomp,            \
+      bool ca_rg, bool cb_rg, bool capture_background_adjoint,                            \
+      bool reuse_background_adjoint, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,    \
+      int64_t pml_y0, int64_t pml_x0, int64_t pml_y1, int64_t pml_x1, int64_t n_threads,  \
+      int64_t device, void *compute_stream, void *storage_stream) {                       \
+    (void)f0; (void)df; (void)lambda_store; (void)eta_source_old; (void)work_eta_x;       \
+    (void)work_eta_z; (void)grad_ca_shot; (void)grad_cb_shot; (void)grad_dca_shot;        \
+    (void)grad_dcb_shot; (void)dt; (void)n_threads; (void)storage_stream;                 \
+    if (step_ratio != 1 || storage_mode != kStorageDevice ||                              \
+        storage_format == kFormatBf16 || capture_background_adjoint ||                    \
+        reuse_background_adjoint || ey_store_3 || curl_store_3 || ey_filenames ||         \
+        curl_filenames || (background_n_shots != 0 && background_n_shots != n_shots) ||   \
+        shot_bytes_uncomp != ny * nx * (int64_t)sizeof(T)) {                              \
+      set_error("born_backward_bggrad: device storage of full-precision snapshots, "      \
+                "step_ratio 1, no background-adjoint capture or reuse");                  \
+      return kUnsupported;                                                                \
+    }                                                                                     \
+    if (hipSetDevice((int)device) != hipSuccess) {                                        \
+      set_error("hipSetDevice failed");                                                   \
+      return kHipError;                                                                   \
+    }                                                                                     \
+    const hipStream_t stream = (hipStream_t)compute_stream;                               \
+    const T *prof[12] = {ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh};           \
+    const tm2d_ho::Problem<T> hp = ho_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots, ny, \
+                                                 nx, n_src, n_rec, ca_b, cb_b, cq_b,      \
+                                                 pml_y0, pml_x0, pml_y1, pml_x1);         \
+    T *const lam[7] = {ey, hx, hz, m_lambda_ey_x, m_lambda_ey_z, m_lambda_hx_z,           \
+                       m_lambda_hz_x};                                                    \
+    T *const eta[7] = {dey, dhx, dhz, m_eta_ey_x, m_eta_ey_z, m_eta_hx_z, m_eta_hz_x};    \
+    const T *const hist[4] = {(const T *)ey_store_1, (const T *)curl_store_1,             \
+                              (const T *)dey_store, (const T *)dcurl_store};              \
+    const int64_t numel = ny * nx;                                                        \
+    T *const outs[4] = {ca_rg ? grad_ca : nullptr, cb_rg ? grad_cb : nullptr, grad_dca,   \
+                        grad_dcb};                                                        \
+    Scratch scr(stream);                                                                  \
+    double *acc[4] = {nullptr, nullptr, nullptr, nullptr};                                \
+    for (int k = 0; k < 4; ++k)                                                           \
+      if (outs[k] != nullptr) {                                                           \
+        TIDE_CHECK(scr.alloc(&acc[k], (size_t)numel));                                    \
+        TIDE_HIP_TRY(hipMemsetAsync(acc[k], 0, (size_t)numel * sizeof(double), stream));  \
+      }                                                                                   \
+    TIDE_CHECK(tm2d_ho::born_backward<T>(S, hp, lam, eta, dca, dcb, grad_r,               \
+                                         grad_background_r, hist, grad_df, grad_f0,       \
+                                         acc[0], acc[1], acc[2], acc[3], sources_i,       \
+                                         receivers_i, nt, start_t, stream,                \
+                                         adjoint_memory_scratch));                        \
+    for (int k = 0; k < 4; ++k)                                                           \
+      if (outs[k] != nullptr) TIDE_CHECK(fold_into<T>(outs[k], acc[k], numel, stream));   \
+    return kOk;                                                                           \
+  }                                                                                       \
+                                                                                          \
+  /* this library's form of the Born adjoint: FP64 accumulators (added to), the four   \
+     snapshot histories and both adjoint states as plain arrays */                      \
+  extern "C" int tide_tm2d_##S##_##NAME##_born_backward_f64acc(                           \
+      T const *ca, T const *cb, T const *cq, T const *dca, T const *dcb, T const *w,      \
+      T const *w_bg, T const *bg_ey_hist, T const *bg_curl_hist, T const *sc_ey_hist,     \
+      T const *sc_curl_hist, T *l_ey, T *l_hx, T *l_hz, T *ml_ey_x, T *ml_ey_z,           \
+      T *ml_hx_z, T *ml_hz_x, T *e_ey, T *e_hx, T *e_hz, T *me_ey_x, T *me_ey_z,          \
+      T *me_hx_z, T *me_hz_x, T *grad_df, T *grad_f0, double *gca, double *gcb,           \
+      double *gdca, double *gdcb, T const *ay, T const *by, T const *ayh, T const *byh,   \
+      T const *ax, T const *bx, T const *axh, T const *bxh, T const *ky, T const *kyh,    \
+      T const *kx, T const *kxh, int64_t const *sources_i, int64_t const *receivers_i,    \
+      T rdy, T rdx, int64_t nt, int64_t n_shots, int64_t ny, int64_t nx, int64_t n_src,   \
+      int64_t n_rec, int64_t start_t, int64_t pml_y0, int64_t pml_x0, int64_t pml_y1,     \
+      int64_t pml_x1, int64_t device, void *compute_stream) {                             \
+    if (hipSetDevice((int)device) != hipSuccess) {                                        \
+      set_error("hipSetDevice failed");                                                   \
+      return kHipError;                                                                   \
+    }                                                                                     \
+    const T *prof[12] = {ay, by, ayh, byh, ax, bx, axh, bxh, ky, kyh, kx, kxh};           \
+    const tm2d_ho::Problem<T> hp = ho_problem<T>(ca, cb, cq, prof, rdy, rdx, n_shots, ny, \
+                                                 nx, n_src, n_rec, false, false, false,   \
+                                                 pml_y0, pml_x0, pml_y1, pml_x1);         \
+    T *const lam[7] = {l_ey, l_hx, l_hz, ml_ey_x, ml_ey_z, ml_hx_z, ml_hz_x};             \
+    T *const eta[7] = {e_ey, e_hx, e_hz, me_ey_x, me_ey_z, me_hx_z, me_hz_x};             \
+    const T *const hist[4] = {bg_ey_hist, bg_curl_hist, sc_ey_hist, sc_curl_hist};        \
+    return tm2d_ho::born_backward<T>(S, hp, lam, eta, dca, dcb, w, w_bg, hist, grad_df,   \
+                                     grad_f0, gca, gcb, gdca, gdcb, sources_i,            \
+                                     receivers_i, nt, start_t, (hipStream_t)compute_stream,\
+                                     nullptr);                                            \
   }
 
 TIDE_TM2D_BORN_ENTRY(2, float, float)

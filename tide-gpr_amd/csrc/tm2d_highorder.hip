@@ -200,6 +200,63 @@ __global__ __launch_bounds__(256) void e_step(Grid<T> g, T *__restrict__ ey,
   ey[i] = e1;
 }
 
+// Debye E step: Ey^n into ey_prev, the plain update with the Debye
+// coefficients, then + sum_p cp_p P_p (poles in order from zero); cp
+// [n_poles, ny, nx], pol [n_shots, n_poles, ny, nx] (reference
+// src/tide/csrc/tm2d/maxwell_tm_inst.cpp:214-272).
+template <typename T, int P>
+__global__ __launch_bounds__(256) void e_step_debye(Grid<T> g, T *__restrict__ ey,
+                                                    const T *__restrict__ hx,
+                                                    const T *__restrict__ hz,
+                                                    T *__restrict__ m_hx_z, T *__restrict__ m_hz_x,
+                                                    T *__restrict__ ey_prev,
+                                                    const T *__restrict__ cp,
+                                                    const T *__restrict__ pol, int n_poles) {
+  const Cell c = cell_of(g);
+  const int ny = g.ny, nx = g.nx;
+  if (c.y < P || c.y >= ny - P + 1 || c.x < P || c.x >= nx - P + 1) return;
+  const size_t base = (size_t)c.s * g.numel, i = base + c.j;
+  const T ca = g.ca[c.j];
+  const T cb = g.cb[c.j];
+  T dhz = d_int<T, P>([&](int o) { return hz[i + o]; }, g.rdx);
+  T dhx = d_int<T, P>([&](int o) { return hx[i + (ptrdiff_t)o * nx]; }, g.rdy);
+  if (c.x < g.pml_x0 || c.x >= g.pml_x1) {
+    const T m = g.bx[c.x] * m_hz_x[i] + g.ax[c.x] * dhz;
+    m_hz_x[i] = m;
+    dhz = dhz / g.kx[c.x] + m;
+  }
+  if (c.y < g.pml_y0 || c.y >= g.pml_y1) {
+    const T m = g.by[c.y] * m_hx_z[i] + g.ay[c.y] * dhx;
+    m_hx_z[i] = m;
+    dhx = dhx / g.ky[c.y] + m;
+  }
+  const T curl = dhz - dhx;
+  const T e0 = ey[i];
+  ey_prev[i] = e0;
+  T e1 = ca * e0 + cb * curl;
+  T term = (T)0;
+  for (int p = 0; p < n_poles; ++p)
+    term = term + cp[(size_t)p * g.numel + c.j] * pol[((size_t)c.s * n_poles + p) * g.numel + c.j];
+  ey[i] = e1 + term;
+}
+
+// Debye polarization of every cell after the sources:
+// P_p = a_p P_p + b_p (Ey^{n+1} + Ey^n) (maxwell_tm_inst.cpp:275-300).
+template <typename T>
+__global__ void pol_update(const T *__restrict__ ey, const T *__restrict__ ey_prev,
+                           const T *__restrict__ da, const T *__restrict__ db,
+                           T *__restrict__ pol, int64_t numel, int n_shots, int n_poles) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = (int)blockIdx.y;
+  if (j >= numel || s >= n_shots) return;
+  const int64_t i = (int64_t)s * numel + j;
+  const T sum = ey[i] + ey_prev[i];
+  for (int p = 0; p < n_poles; ++p) {
+    const int64_t q = ((int64_t)s * n_poles + p) * numel + j;
+    pol[q] = da[(int64_t)p * numel + j] * pol[q] + db[(int64_t)p * numel + j] * sum;
+  }
+}
+
 // One thread per shot, points in order (duplicates add in a fixed order).
 template <typename T>
 __global__ void add_points(T *field, int numel, int n_shots, const int64_t *idx, int n,
@@ -224,12 +281,17 @@ __global__ void read_points(const T *field, int numel, int n_shots, const int64_
 // Transpose of the E step: lambda_Hz, lambda_Hx gather the x / y
 // derivative adjoints of the neighbours' cb * lambda_Ey (through their CPML
 // memories), and the E-phase adjoint memories advance into the `_n` copies.
+// With `dcb` and `l2` (the Born adjoint's background pass) the adjoint of
+// curl H is cb lambda_Ey + dcb lambda2: the scattered E update adds dcb curl H
+// of the background, so the scattered adjoint lambda2 reaches it too.
 template <typename T, int P>
 __global__ __launch_bounds__(256) void e_adjoint(Grid<T> g, const T *__restrict__ l_ey,
                                                  T *__restrict__ l_hx, T *__restrict__ l_hz,
                                                  const T *__restrict__ m_hx_z,
                                                  const T *__restrict__ m_hz_x,
-                                                 T *__restrict__ m_hx_z_n, T *__restrict__ m_hz_x_n) {
+                                                 T *__restrict__ m_hx_z_n, T *__restrict__ m_hz_x_n,
+                                                 const T *__restrict__ dcb,
+                                                 const T *__restrict__ l2) {
   const Cell c = cell_of(g);
   const int ny = g.ny, nx = g.nx;
   if (c.y < P || c.y >= ny - P + 1 || c.x < P || c.x >= nx - P + 1) return;
@@ -238,7 +300,8 @@ __global__ __launch_bounds__(256) void e_adjoint(Grid<T> g, const T *__restrict_
   auto in_e = [&](int y, int x) { return y >= P && y < ny - P + 1 && x >= P && x < nx - P + 1; };
   auto q_at = [&](int y, int x) {
     const size_t j = (size_t)y * nx + x;
-    return cbp[j] * l_ey[base + j];
+    const T q = cbp[j] * l_ey[base + j];
+    return l2 != nullptr ? q + dcb[j] * l2[base + j] : q;
   };
   auto raw_x = [&](int y, int x) -> T {
     if (!in_e(y, x)) return (T)0;
@@ -267,13 +330,17 @@ __global__ __launch_bounds__(256) void e_adjoint(Grid<T> g, const T *__restrict_
 // Transpose of the H half-step: lambda_Ey = ca lambda_Ey + the half-point
 // derivative adjoints of the neighbours' cq lambda_H (through their CPML
 // memories); the H-phase adjoint memories advance into the `_n` copies.
+// With `dca` and `l2`: ca lambda_Ey + dca lambda2 (the scattered E update
+// adds dca Ey^n of the background) before the derivative terms.
 template <typename T, int P>
 __global__ __launch_bounds__(256) void h_adjoint(Grid<T> g, T *__restrict__ l_ey,
                                                  const T *__restrict__ l_hx,
                                                  const T *__restrict__ l_hz,
                                                  const T *__restrict__ m_ey_x,
                                                  const T *__restrict__ m_ey_z,
-                                                 T *__restrict__ m_ey_x_n, T *__restrict__ m_ey_z_n) {
+                                                 T *__restrict__ m_ey_x_n, T *__restrict__ m_ey_z_n,
+                                                 const T *__restrict__ dca,
+                                                 const T *__restrict__ l2) {
   const Cell c = cell_of(g);
   const int ny = g.ny, nx = g.nx;
   if (c.y < P || c.y >= ny - P + 1 || c.x < P || c.x >= nx - P + 1) return;
@@ -304,7 +371,7 @@ __global__ __launch_bounds__(256) void h_adjoint(Grid<T> g, T *__restrict__ l_ey
   m_ey_z_n[i] = (c.y < ny - P && py) ? g.byh[c.y] * m_ey_z[i] + adj_y : (T)0;
   m_ey_x_n[i] = (c.x < nx - P && px) ? g.bxh[c.x] * m_ey_x[i] + adj_x : (T)0;
   const T ca = g.ca[(g.ca_b ? base : 0) + c.j];
-  const T l = ca * l_ey[i];
+  const T l = l2 != nullptr ? ca * l_ey[i] + dca[c.j] * l2[i] : ca * l_ey[i];
   l_ey[i] = l + (d_half_adj<T, P>([&](int o) { return raw_z(c.y + o, c.x); }, g.rdy) +
                  d_half_adj<T, P>([&](int o) { return raw_xh(c.y, c.x + o); }, g.rdx));
 }
@@ -468,6 +535,53 @@ int forward_p(const Problem<T> &p, T *const fld[7], const T *f, T *r, const int6
   return kOk;
 }
 
+// Debye-dispersive forward (forward inference, unbatched coefficients; the
+// reference's native Debye path: src/tide/csrc/tm2d/maxwell_tm_inst.cpp:
+// 780-805, CUDA src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:382-457).  Per
+// step: H half-step, Debye E step, sources, polarization, receivers.
+template <typename T, int P>
+int forward_debye_p(const Problem<T> &p, T *const fld[7], const T *f, T *r, const int64_t *src,
+                    const int64_t *rec, int64_t nt, int64_t start_t, const T *da, const T *db,
+                    const T *cp, T *pol, T *ey_prev, int64_t n_poles, hipStream_t stream) {
+  TIDE_CHECK_HO(check(p.n_shots, p.ny, p.nx, P));
+  if (p.ca_b || p.cb_b || p.cq_b) {
+    set_error("Debye dispersion: per-shot coefficients are not supported");
+    return kUnsupported;
+  }
+  if (n_poles < 1 || n_poles > 64 || !da || !db || !cp || !pol || !ey_prev) {
+    set_error("Debye dispersion needs 1..64 poles and the a, b, cp, polarization and ey_prev "
+              "arrays");
+    return kInvalidArgument;
+  }
+  if (nt <= 0) return kOk;
+  const Grid<T> g = make_grid(p);
+  DevIdx ds, dr;
+  TIDE_CHECK_HO(ds.take(src, p.n_shots * p.n_src, p.ny * p.nx, stream));
+  TIDE_CHECK_HO(dr.take(rec, p.n_shots * p.n_rec, p.ny * p.nx, stream));
+  const dim3 block(64, 4);
+  const dim3 grid((unsigned)((p.nx + 63) / 64), (unsigned)((p.ny + 3) / 4), (unsigned)p.n_shots);
+  const dim3 pgrid((unsigned)((p.ny * p.nx + 255) / 256), (unsigned)p.n_shots);
+  const unsigned pts_blocks_s = (unsigned)((p.n_shots + 63) / 64);
+  const unsigned pts_blocks_r = (unsigned)((p.n_shots * p.n_rec + 255) / 256);
+  T *ey = fld[0], *hx = fld[1], *hz = fld[2], *mex = fld[3], *mez = fld[4], *mhxz = fld[5],
+    *mhzx = fld[6];
+  for (int64_t t = start_t; t < start_t + nt; ++t) {
+    h_step<T, P><<<grid, block, 0, stream>>>(g, ey, hx, hz, mex, mez);
+    e_step_debye<T, P><<<grid, block, 0, stream>>>(g, ey, hx, hz, mhxz, mhzx, ey_prev, cp, pol,
+                                                   (int)n_poles);
+    if (p.n_src > 0 && f != nullptr)
+      add_points<T><<<pts_blocks_s, 64, 0, stream>>>(ey, g.numel, g.n_shots, ds.p, (int)p.n_src,
+                                                     f + t * p.n_shots * p.n_src);
+    pol_update<T><<<pgrid, 256, 0, stream>>>(ey, ey_prev, da, db, pol, (int64_t)g.numel,
+                                             g.n_shots, (int)n_poles);
+    if (p.n_rec > 0 && r != nullptr)
+      read_points<T><<<pts_blocks_r, 256, 0, stream>>>(ey, g.numel, g.n_shots, dr.p, (int)p.n_rec,
+                                                       r + t * p.n_shots * p.n_rec);
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  return kOk;
+}
+
 template <typename T, int P>
 int backward_p(const Problem<T> &p, T *const lam[7], const T *grad_r, const void *ey_store,
                const void *curl_store, int fmt, size_t step_bytes, T *grad_f, double *gca,
@@ -510,11 +624,11 @@ int backward_p(const Problem<T> &p, T *const lam[7], const T *grad_r, const void
   SnapshotStage *stages[2] = {gca ? ey_stage : nullptr, gcb ? curl_stage : nullptr};
   for (int64_t t = start_t - 1; t >= start_t - nt; --t) {
     e_adjoint<T, P><<<grid, block, 0, stream>>>(g, l_ey, l_hx, l_hz, cur[2], cur[3], nxt[2],
-                                                nxt[3]);
+                                                nxt[3], nullptr, nullptr);
     std::swap(cur[2], nxt[2]);
     std::swap(cur[3], nxt[3]);
     h_adjoint<T, P><<<grid, block, 0, stream>>>(g, l_ey, l_hx, l_hz, cur[0], cur[1], nxt[0],
-                                                nxt[1]);
+                                                nxt[1], nullptr, nullptr);
     std::swap(cur[0], nxt[0]);
     std::swap(cur[1], nxt[1]);
     if (p.n_rec > 0 && grad_r != nullptr)
@@ -557,16 +671,18 @@ int backward_p(const Problem<T> &p, T *const lam[7], const T *grad_r, const void
 
 // Born (tangent-linear) forward: the background and the scattered field
 // advance together, one step at a time.  Per step: background H and E
-// (its pre-update Ey and curl H kept in two scratch planes), scattered H and
-// E with the scattering source dca Ey^n + dcb curl H^n, then the sources
-// (f0 into the background, df into the scattered field) and both receiver
-// sets.  Reference: the tangent recurrence of
+// (its pre-update Ey and curl H kept in two planes), scattered H and E with
+// the scattering source dca Ey^n + dcb curl H^n, then the sources (f0 into
+// the background, df into the scattered field) and both receiver sets.
+// With `hist` (four [nt, shots, ny, nx] arrays: background Ey^n and
+// curl^n, scattered Ey^n and curl^n; slot t for step t) the planes of every
+// step are kept for the Born adjoint.  Reference: the tangent recurrence of
 // src/tide/maxwell/tm2d_born_python.py and the native born_forward
 // (src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:533-742, 1611-1800).
 template <typename T, int P>
 int born_p(const Problem<T> &p, T *const bg[7], T *const sc[7], const T *dca, const T *dcb,
            const T *f0, const T *df, T *r, T *bg_r, const int64_t *src, const int64_t *rec,
-           int64_t nt, int64_t start_t, hipStream_t stream) {
+           int64_t nt, int64_t start_t, hipStream_t stream, T *const *hist) {
   TIDE_CHECK_HO(check(p.n_shots, p.ny, p.nx, P));
   if (nt <= 0) return kOk;
   if ((dca == nullptr) != (dcb == nullptr)) {
@@ -586,18 +702,25 @@ int born_p(const Problem<T> &p, T *const bg[7], T *const sc[7], const T *dca, co
     ~Free() { (void)hipFreeAsync(q, s); }
   } free_planes{planes, stream};
   const BornSrc<T> none{nullptr, nullptr, nullptr, nullptr};
-  const BornSrc<T> born{dca, dcb, planes, planes + total};
   const dim3 block(64, 4);
   const dim3 grid((unsigned)((p.nx + 63) / 64), (unsigned)((p.ny + 3) / 4), (unsigned)p.n_shots);
   const unsigned blocks_s = (unsigned)((p.n_shots + 63) / 64);
   const unsigned blocks_r = (unsigned)((p.n_shots * p.n_rec + 255) / 256);
   for (int64_t t = start_t; t < start_t + nt; ++t) {
+    T *be = planes, *bc = planes + total, *se = nullptr, *scu = nullptr;
+    if (hist != nullptr) {
+      be = hist[0] + (size_t)t * total;
+      bc = hist[1] + (size_t)t * total;
+      se = hist[2] + (size_t)t * total;
+      scu = hist[3] + (size_t)t * total;
+    }
+    const BornSrc<T> born{dca, dcb, be, bc};
     h_step<T, P><<<grid, block, 0, stream>>>(g, bg[0], bg[1], bg[2], bg[3], bg[4]);
-    e_step<T, P><<<grid, block, 0, stream>>>(g, bg[0], bg[1], bg[2], bg[5], bg[6], planes,
-                                             planes + total, 1, none);
+    e_step<T, P><<<grid, block, 0, stream>>>(g, bg[0], bg[1], bg[2], bg[5], bg[6], be, bc, 1,
+                                             none);
     h_step<T, P><<<grid, block, 0, stream>>>(g, sc[0], sc[1], sc[2], sc[3], sc[4]);
-    e_step<T, P><<<grid, block, 0, stream>>>(g, sc[0], sc[1], sc[2], sc[5], sc[6], nullptr,
-                                             nullptr, 1, dca != nullptr ? born : none);
+    e_step<T, P><<<grid, block, 0, stream>>>(g, sc[0], sc[1], sc[2], sc[5], sc[6], se, scu, 1,
+                                             dca != nullptr ? born : none);
     if (p.n_src > 0) {
       if (f0 != nullptr)
         add_points<T><<<blocks_s, 64, 0, stream>>>(bg[0], g.numel, g.n_shots, ds.p, (int)p.n_src,
@@ -619,30 +742,208 @@ int born_p(const Problem<T> &p, T *const bg[7], T *const sc[7], const T *dca, co
   return kOk;
 }
 
+// Adjoint of born_p (unbatched coefficients): gradients of <r, w> +
+// <bg_r, w_bg> with respect to ca, cb, dca, dcb (FP64, added to; summed over
+// shots), f0 and df (per step, written).  lam (scattered) and eta
+// (background) adjoint states: Ey, Hx, Hz, m_Ey_x, m_Ey_z, m_Hx_z, m_Hz_x,
+// advanced in place; `mem` is 8 [shots, ny, nx] arrays of scratch for the
+// adjoint memories' second copies (null: allocated here).  Reverse step t
+// transposes step t + 1 -- the background first, coupled to the scattered
+// adjoint of the same step (the scattered E update reads the background's
+// Ey^n and curl^n), then the scattered field's plain adjoint step -- then
+// takes step t's receivers, source gradients and FP64 terms:
+//   ca  += eta Ey^n + lam dEy^n     cb  += eta curl^n + lam dcurl^n
+//   dca += lam Ey^n                 dcb += lam curl^n
+// Reference counterpart: maxwell_tm_{S}_{T}_born_backward_bggrad_cuda
+// (src/tide/csrc/tm2d/maxwell_tm_cuda_inst.cu:1611-2404).
+template <typename T, int P>
+int born_backward_p(const Problem<T> &p, T *const lam[7], T *const eta[7], const T *dca,
+                    const T *dcb, const T *w, const T *w_bg, const T *const hist[4], T *g_df,
+                    T *g_f0, double *g_ca, double *g_cb, double *g_dca, double *g_dcb,
+                    const int64_t *src, const int64_t *rec, int64_t nt, int64_t start_t,
+                    hipStream_t stream, T *mem) {
+  TIDE_CHECK_HO(check(p.n_shots, p.ny, p.nx, P));
+  if (p.ca_b || p.cb_b || p.cq_b) {
+    set_error("born_backward: per-shot coefficients are not supported");
+    return kUnsupported;
+  }
+  if (dca == nullptr || dcb == nullptr || hist == nullptr || hist[0] == nullptr ||
+      hist[1] == nullptr || hist[2] == nullptr || hist[3] == nullptr) {
+    set_error("born_backward: dca, dcb and the four snapshot histories are required");
+    return kInvalidArgument;
+  }
+  if (nt <= 0) return kOk;
+  if (start_t - nt < 0) {
+    set_error("born_backward: the reverse loop would run past step 0");
+    return kInvalidArgument;
+  }
+  const Grid<T> g = make_grid(p);
+  DevIdx ds, dr;
+  TIDE_CHECK_HO(ds.take(src, p.n_shots * p.n_src, p.ny * p.nx, stream));
+  TIDE_CHECK_HO(dr.take(rec, p.n_shots * p.n_rec, p.ny * p.nx, stream));
+  const size_t total = (size_t)p.n_shots * p.ny * p.nx;
+  T *own = nullptr;
+  if (mem == nullptr) {
+    TIDE_HIP_TRY(hipMallocAsync((void **)&own, 8 * total * sizeof(T), stream));
+    mem = own;
+  }
+  struct Free {
+    T *q;
+    hipStream_t s;
+    ~Free() {
+      if (q) (void)hipFreeAsync(q, s);
+    }
+  } free_mem{own, stream};
+  T *const *sys[2] = {eta, lam};  // 0: background (coupled), 1: scattered
+  T *cur[2][4], *nxt[2][4];       // m_ey_x m_ey_z m_hx_z m_hz_x
+  for (int k = 0; k < 2; ++k)
+    for (int m = 0; m < 4; ++m) {
+      cur[k][m] = sys[k][3 + m];
+      nxt[k][m] = mem + (size_t)(4 * k + m) * total;
+    }
+  const dim3 block(64, 4);
+  const dim3 grid((unsigned)((p.nx + 63) / 64), (unsigned)((p.ny + 3) / 4), (unsigned)p.n_shots);
+  const dim3 grid2((unsigned)((p.nx + 63) / 64), (unsigned)((p.ny + 3) / 4), 1);
+  const unsigned blocks_s = (unsigned)((p.n_shots * p.n_src + 255) / 256);
+  const unsigned blocks_r = (unsigned)((p.n_shots + 63) / 64);
+  for (int64_t t = start_t - 1; t >= start_t - nt; --t) {
+    for (int k = 0; k < 2; ++k) {
+      const T *l2 = k == 0 ? lam[0] : nullptr;
+      e_adjoint<T, P><<<grid, block, 0, stream>>>(g, sys[k][0], sys[k][1], sys[k][2], cur[k][2],
+                                                  cur[k][3], nxt[k][2], nxt[k][3], dcb, l2);
+      std::swap(cur[k][2], nxt[k][2]);
+      std::swap(cur[k][3], nxt[k][3]);
+      h_adjoint<T, P><<<grid, block, 0, stream>>>(g, sys[k][0], sys[k][1], sys[k][2], cur[k][0],
+                                                  cur[k][1], nxt[k][0], nxt[k][1], dca, l2);
+      std::swap(cur[k][0], nxt[k][0]);
+      std::swap(cur[k][1], nxt[k][1]);
+    }
+    if (p.n_rec > 0) {
+      if (w != nullptr)
+        add_points<T><<<blocks_r, 64, 0, stream>>>(lam[0], g.numel, g.n_shots, dr.p,
+                                                   (int)p.n_rec, w + t * p.n_shots * p.n_rec);
+      if (w_bg != nullptr)
+        add_points<T><<<blocks_r, 64, 0, stream>>>(eta[0], g.numel, g.n_shots, dr.p,
+                                                   (int)p.n_rec, w_bg + t * p.n_shots * p.n_rec);
+    }
+    if (p.n_src > 0) {
+      if (g_df != nullptr)
+        read_points<T><<<blocks_s, 256, 0, stream>>>(lam[0], g.numel, g.n_shots, ds.p,
+                                                     (int)p.n_src, g_df + t * p.n_shots * p.n_src);
+      if (g_f0 != nullptr)
+        read_points<T><<<blocks_s, 256, 0, stream>>>(eta[0], g.numel, g.n_shots, ds.p,
+                                                     (int)p.n_src, g_f0 + t * p.n_shots * p.n_src);
+    }
+    const T *be = hist[0] + (size_t)t * total, *bc = hist[1] + (size_t)t * total;
+    const T *se = hist[2] + (size_t)t * total, *scu = hist[3] + (size_t)t * total;
+    if (g_ca != nullptr || g_cb != nullptr) {
+      coeff_grad<T><<<grid2, block, 0, stream>>>(g, P, eta[0], g_ca ? be : nullptr,
+                                                 g_cb ? bc : nullptr, 1, g_ca, g_cb, 0, 1.0);
+      coeff_grad<T><<<grid2, block, 0, stream>>>(g, P, lam[0], g_ca ? se : nullptr,
+                                                 g_cb ? scu : nullptr, 1, g_ca, g_cb, 0, 1.0);
+    }
+    if (g_dca != nullptr || g_dcb != nullptr)
+      coeff_grad<T><<<grid2, block, 0, stream>>>(g, P, lam[0], g_dca ? be : nullptr,
+                                                 g_dcb ? bc : nullptr, 1, g_dca, g_dcb, 0, 1.0);
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  const size_t bytes = total * sizeof(T);
+  for (int k = 0; k < 2; ++k)
+    for (int m = 0; m < 4; ++m)
+      if (cur[k][m] != sys[k][3 + m])
+        TIDE_HIP_TRY(hipMemcpyAsync(sys[k][3 + m], cur[k][m], bytes, hipMemcpyDeviceToDevice,
+                                    stream));
+  return kOk;
+}
+
 template <typename T>
-int born_forward(int stencil, const Problem<T> &p, T *const bg[7], T *const sc[7],
-                 const T *dca, const T *dcb, const T *f0, const T *df, T *r, T *bg_r,
-                 const int64_t *src, const int64_t *rec, int64_t nt, int64_t start_t,
-                 hipStream_t stream) {
+int forward_debye(int stencil, const Problem<T> &p, T *const fld[7], const T *f, T *r,
+                  const int64_t *src, const int64_t *rec, int64_t nt, int64_t start_t,
+                  const T *da, const T *db, const T *cp, T *pol, T *ey_prev, int64_t n_poles,
+                  hipStream_t stream) {
+#define TIDE_DEBYE_P(K) \
+  forward_debye_p<T, K>(p, fld, f, r, src, rec, nt, start_t, da, db, cp, pol, ey_prev, n_poles, stream)
   switch (stencil) {
-    case 2: return born_p<T, 1>(p, bg, sc, dca, dcb, f0, df, r, bg_r, src, rec, nt, start_t, stream);
-    case 4: return born_p<T, 2>(p, bg, sc, dca, dcb, f0, df, r, bg_r, src, rec, nt, start_t, stream);
-    case 6: return born_p<T, 3>(p, bg, sc, dca, dcb, f0, df, r, bg_r, src, rec, nt, start_t, stream);
-    case 8: return born_p<T, 4>(p, bg, sc, dca, dcb, f0, df, r, bg_r, src, rec, nt, start_t, stream);
+    case 2: return TIDE_DEBYE_P(1);
+    case 4: return TIDE_DEBYE_P(2);
+    case 6: return TIDE_DEBYE_P(3);
+    case 8: return TIDE_DEBYE_P(4);
     default:
       set_error("stencil must be 2, 4, 6 or 8");
       return kInvalidArgument;
   }
+#undef TIDE_DEBYE_P
+}
+
+template int forward_debye<float>(int, const Problem<float> &, float *const[7], const float *,
+                                  float *, const int64_t *, const int64_t *, int64_t, int64_t,
+                                  const float *, const float *, const float *, float *, float *,
+                                  int64_t, hipStream_t);
+template int forward_debye<double>(int, const Problem<double> &, double *const[7],
+                                   const double *, double *, const int64_t *, const int64_t *,
+                                   int64_t, int64_t, const double *, const double *,
+                                   const double *, double *, double *, int64_t, hipStream_t);
+
+template <typename T>
+int born_forward(int stencil, const Problem<T> &p, T *const bg[7], T *const sc[7],
+                 const T *dca, const T *dcb, const T *f0, const T *df, T *r, T *bg_r,
+                 const int64_t *src, const int64_t *rec, int64_t nt, int64_t start_t,
+                 hipStream_t stream, T *const *hist) {
+#define TIDE_BORN_P(K) \
+  born_p<T, K>(p, bg, sc, dca, dcb, f0, df, r, bg_r, src, rec, nt, start_t, stream, hist)
+  switch (stencil) {
+    case 2: return TIDE_BORN_P(1);
+    case 4: return TIDE_BORN_P(2);
+    case 6: return TIDE_BORN_P(3);
+    case 8: return TIDE_BORN_P(4);
+    default:
+      set_error("stencil must be 2, 4, 6 or 8");
+      return kInvalidArgument;
+  }
+#undef TIDE_BORN_P
+}
+
+template <typename T>
+int born_backward(int stencil, const Problem<T> &p, T *const lam[7], T *const eta[7],
+                  const T *dca, const T *dcb, const T *w, const T *w_bg, const T *const hist[4],
+                  T *g_df, T *g_f0, double *g_ca, double *g_cb, double *g_dca, double *g_dcb,
+                  const int64_t *src, const int64_t *rec, int64_t nt, int64_t start_t,
+                  hipStream_t stream, T *mem) {
+#define TIDE_BORN_B(K)                                                                     \
+  born_backward_p<T, K>(p, lam, eta, dca, dcb, w, w_bg, hist, g_df, g_f0, g_ca, g_cb, g_dca, \
+                        g_dcb, src, rec, nt, start_t, stream, mem)
+  switch (stencil) {
+    case 2: return TIDE_BORN_B(1);
+    case 4: return TIDE_BORN_B(2);
+    case 6: return TIDE_BORN_B(3);
+    case 8: return TIDE_BORN_B(4);
+    default:
+      set_error("stencil must be 2, 4, 6 or 8");
+      return kInvalidArgument;
+  }
+#undef TIDE_BORN_B
 }
 
 template int born_forward<float>(int, const Problem<float> &, float *const[7], float *const[7],
                                  const float *, const float *, const float *, const float *,
                                  float *, float *, const int64_t *, const int64_t *, int64_t,
-                                 int64_t, hipStream_t);
+                                 int64_t, hipStream_t, float *const *);
 template int born_forward<double>(int, const Problem<double> &, double *const[7],
                                   double *const[7], const double *, const double *,
                                   const double *, const double *, double *, double *,
-                                  const int64_t *, const int64_t *, int64_t, int64_t, hipStream_t);
+                                  const int64_t *, const int64_t *, int64_t, int64_t, hipStream_t,
+                                  double *const *);
+template int born_backward<float>(int, const Problem<float> &, float *const[7], float *const[7],
+                                  const float *, const float *, const float *, const float *,
+                                  const float *const[4], float *, float *, double *, double *,
+                                  double *, double *, const int64_t *, const int64_t *, int64_t,
+                                  int64_t, hipStream_t, float *);
+template int born_backward<double>(int, const Problem<double> &, double *const[7],
+                                   double *const[7], const double *, const double *,
+                                   const double *, const double *, const double *const[4],
+                                   double *, double *, double *, double *, double *, double *,
+                                   const int64_t *, const int64_t *, int64_t, int64_t,
+                                   hipStream_t, double *);
 
 template <typename T>
 int forward(int stencil, const Problem<T> &p, T *const fld[7], const T *f, T *r,

@@ -14,6 +14,7 @@ from . import callbacks, cfl, padding, staggered, storage, utils, wavelets
 from .callbacks import Callback, CallbackState
 from .cfl import cfl_condition
 from .core import BackendPreference, FallbackPolicy, StorageMode, StorageOptions
+from .dispersion import DebyeDispersion
 from .maxwell import (
     CPML,
     Acquisition,
@@ -37,6 +38,7 @@ from .wavelets import gaussian, gaussian_derivative, ricker
 from . import workflow  # noqa: E402  (imports the operators above)
 
 __all__ = [
+    "DebyeDispersion",
     "Acquisition",
     "BackendPreference",
     "C0",

@@ -252,6 +252,22 @@ def load() -> ctypes.CDLL:
             fn = getattr(lib, f"maxwell_tm_{stencil}_{name}_born_forward_cuda")
             fn.restype = ctypes.c_int
             fn.argtypes = _expand_kinds("P37 F3 I7 B3 I1 I4 I1 I1 P1", ctype)
+            # _TM_BORN_FORWARD_WITH_STORAGE_SPEC (backend_utils.py:351-370): the
+            # same plus 8 store pointers | storage mode, format, shot bytes |
+            # ca/cb requires_grad | storage stream
+            fn = getattr(lib, f"maxwell_tm_{stencil}_{name}_born_forward_with_storage_cuda")
+            fn.restype = ctypes.c_int
+            fn.argtypes = _expand_kinds("P45 F3 I10 B5 I7 P2", ctype)
+            # _TM_BORN_BACKWARD_BGGRAD_SPEC (backend_utils.py:407-444)
+            fn = getattr(lib, f"maxwell_tm_{stencil}_{name}_born_backward_bggrad_cuda")
+            fn.restype = ctypes.c_int
+            fn.argtypes = _expand_kinds("P18 I1 P42 F3 I10 B7 I7 P2", ctype)
+            # this library's Born adjoint: 27 field pointers, 4 FP64
+            # accumulators, 12 profiles, 2 point arrays | rdy rdx | nt n_shots ny
+            # nx n_src n_rec start_t | 4 pml | device | stream
+            fn = getattr(lib, f"tide_tm2d_{stencil}_{name}_born_backward_f64acc")
+            fn.restype = ctypes.c_int
+            fn.argtypes = _expand_kinds("P45 F2 I12 P1", ctype)
     # the allocation-free variant of maxwell_tm_2_float_forward_cuda
     lib.tide_tm2d_float_forward_noalloc.restype = ctypes.c_int
     lib.tide_tm2d_float_forward_noalloc.argtypes = _tm_forward_types(ctypes.c_float)
@@ -274,6 +290,8 @@ def load() -> ctypes.CDLL:
     lib.tide_em3d_set_fused.restype = ctypes.c_int
     lib.tide_em3d_set_fused.argtypes = [ctypes.c_int] * 2
     lib.tide_em3d_get_fused.restype = ctypes.c_int
+    lib.tide_em3d_set_tile_order.restype = ctypes.c_int
+    lib.tide_em3d_set_tile_order.argtypes = [ctypes.c_int]
     lib.tide_em3d_get_fused.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.tide_tm2d_set_tuning.restype = ctypes.c_int
     lib.tide_tm2d_set_tuning.argtypes = [ctypes.c_int] * 4
@@ -380,6 +398,12 @@ def set_em3d_fused(rows: int, planes: int = 8) -> None:
     """One-launch 3-D forward step: rows per wavefront (2 or 4; 0 runs the
     two-kernel step) and z planes per march (8 or 16)."""
     check(load().tide_em3d_set_fused(int(rows), int(planes)))
+
+
+def set_em3d_tile_order(shot_fast: int) -> None:
+    """Workgroup order of the 3-D kernels: 0 shot groups outermost, 1 the
+    shot groups of one tile adjacent."""
+    check(load().tide_em3d_set_tile_order(int(shot_fast)))
 
 
 def get_em3d_fused() -> tuple[int, int]:
@@ -490,6 +514,7 @@ __all__ = [
     "set_tm2d_shot_loop",
     "set_tm2d_tile_split",
     "set_em3d_fused",
+    "set_em3d_tile_order",
     "set_tm2d_time_block",
     "set_tm2d_tuning",
     "get_function",

@@ -4,8 +4,10 @@
 ``op.linearize(model)`` runs the primal once with a differentiable model and
 ``.vjp(cotangent)`` applies ``J(model)^T`` to a receiver cotangent through
 the hand-written adjoint.  For the 2-D operator ``.jvp(direction)`` applies
-``J(model)`` by Born (tangent-linear) propagation (``tm2d_born.borntm``).
-Second-order products (``second_vjp``) and the 3-D tangent are not built.
+``J(model)`` by Born (tangent-linear) propagation (``tm2d_born.borntm``) and
+``.second_vjp(direction, cotangent)`` the model gradient of
+``<J(model) direction, cotangent>`` through the native adjoint of the Born
+recurrence.  The 3-D tangent is not built.
 """
 
 from __future__ import annotations
@@ -97,6 +99,27 @@ class LinearizedMaxwellTM(_LinearizedMaxwell):
         scattered field (reference ``linearization.py:178-229``).  The
         background computed alongside is not kept as the primal: the primal
         stays the differentiable operator run that ``vjp`` needs."""
+        with torch.no_grad():
+            out = self._born(direction)
+        return TangentResult(out[-1], TMState.from_tensors(tuple(out[7:14])))
+
+    def second_vjp(self, direction: EMDirection, cotangent: torch.Tensor) -> EMGradient:
+        """Gradient of ``<J(model) direction, cotangent>`` with respect to the
+        linearization targets (reference ``linearization.py:129-146``): the
+        Born receivers differentiated through the native adjoint of the Born
+        recurrence (background and scattered adjoints coupled), the
+        coefficient partials and the source terms by torch's chain rule."""
+        out = self._born(direction)
+        tangent = out[-1]
+        if cotangent.shape != tangent.shape:
+            raise ValueError("Receiver cotangent must match tangent receiver data shape.")
+        grads = torch.autograd.grad(tangent, self._parameters(), cotangent, allow_unused=True)
+        values = {n: None for n in _TARGETS}
+        for n, g in zip((n for n in _TARGETS if n in self.targets), grads):
+            values[n] = g
+        return EMGradient(**values)
+
+    def _born(self, direction: EMDirection):
         if self._closed:
             raise RuntimeError("The linearized Maxwell operator is closed.")
         direction.validate_for(self.model)
@@ -106,7 +129,7 @@ class LinearizedMaxwellTM(_LinearizedMaxwell):
 
         op, model = self.operator, self.model
         d, e = op.discretization, op.experiment
-        out = borntm(
+        return borntm(
             model.epsilon, model.sigma, model.mu, grid_spacing=d.spacing, dt=d.dt,
             source_amplitude=e.source_amplitude,
             source_location=e.acquisition.source_location,
@@ -117,7 +140,6 @@ class LinearizedMaxwellTM(_LinearizedMaxwell):
             freq_taper_frac=e.frequency_taper_fraction, time_pad_frac=e.time_padding_fraction,
             time_taper=e.time_taper,
         )
-        return TangentResult(out[-1], TMState.from_tensors(tuple(out[7:14])))
 
     def _separable_gradient(self, observed: torch.Tensor, residual):
         """Value and gradient of a shot-separable receiver objective, chunk

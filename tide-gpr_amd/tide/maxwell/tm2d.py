@@ -54,6 +54,7 @@ from ..storage import (
     resolve_storage_mode,
 )
 from ..utils import C0, compile_material_coefficients
+from ..utils import validate_material_inputs as validate_debye
 
 _FIELD_NAMES = ("Ey", "Hx", "Hz", "m_Ey_x", "m_Ey_z", "m_Hx_z", "m_Hz_x")
 
@@ -161,6 +162,7 @@ def prepare_tm2d(
     validate_material_inputs: bool = True,
     allocate_fields: bool = True,
     max_vel: float | None = None,
+    dispersion: Any = None,
 ) -> TM2DSetup:
     """Pad the model, compile coefficients and profiles, map the points.
 
@@ -168,6 +170,9 @@ def prepare_tm2d(
     model per shot).  Device-agnostic: runs on CPU tensors too, so the host
     numerics can be tested without a GPU.  ``allocate_fields=False`` leaves
     ``fields`` empty for callers that bring their own (zeroed) state.
+    ``dispersion`` (Debye poles, one model): the Debye coefficients replace
+    ``ca, cb`` and ``extra["debye"]`` holds the padded pole coefficients
+    ``a, b, cp`` ``[n_poles, ny, nx]`` (reference ``tm2d_cuda.py:155-187``).
     """
     if epsilon.ndim not in (2, 3):
         raise RuntimeError("epsilon must be 2D or batched 3D")
@@ -202,7 +207,15 @@ def prepare_tm2d(
     eps_p = create_or_pad(epsilon, total, device, dtype, size, mode="replicate")
     sig_p = create_or_pad(sigma, total, device, dtype, size, mode="replicate")
     mu_p = create_or_pad(mu, total, device, dtype, size, mode="replicate")
-    coef = compile_material_coefficients(eps_p, sig_p, mu_p, dt, validate_inputs=validate_material_inputs)
+    if dispersion is not None:
+        if batched:
+            raise NotImplementedError("Debye dispersion with one model per shot is not supported")
+        if validate_material_inputs:
+            validate_debye(epsilon, dispersion=dispersion, dt=dt)
+        dispersion = _pad_debye(dispersion, tuple(epsilon.shape[-2:]), total, (ny, nx),
+                                device, dtype)
+    coef = compile_material_coefficients(eps_p, sig_p, mu_p, dt, dispersion=dispersion,
+                                         validate_inputs=validate_material_inputs)
     shape3 = (lambda t: t.contiguous()) if batched else (lambda t: t[None].contiguous())
     ca, cb, cq = shape3(coef["ca"]), shape3(coef["cb"]), shape3(coef["cq"])
 
@@ -261,10 +274,35 @@ def prepare_tm2d(
         ny=ny, nx=nx, n_src=n_src, n_rec=n_rec, pml=pml, fd_pad=fd_pad,
         pml_width=widths, batched=batched, stencil=int(stencil),
         extra={"zero_init": all(t0 is None for t0 in initial),
+               "debye": ({k: coef["debye"][k].contiguous() for k in ("a", "b", "cp")}
+                         if coef["has_dispersion"] else None),
                # padded models, for wavefield callbacks
                "models": {"epsilon": eps_p.detach(), "sigma": sig_p.detach(),
                           "mu": mu_p.detach()}},
     )
+
+
+def _pad_debye(dispersion, model_shape, total, padded, device, dtype):
+    """Debye parameters on the padded grid: model-shaped ones (per pole)
+    replicate-padded like the model, scalars kept (reference
+    ``maxwell/common.py:_pad_dispersion_for_model``)."""
+    from ..dispersion import DebyeDispersion
+
+    def pad(value):
+        t = torch.as_tensor(value, device=device, dtype=dtype)
+        if t.ndim == 0:
+            return t
+        if t.ndim <= len(model_shape):
+            return create_or_pad(torch.broadcast_to(t, model_shape), total, device, dtype, padded,
+                                 mode="replicate")
+        if t.ndim == len(model_shape) + 1:
+            return torch.stack([create_or_pad(torch.broadcast_to(t[p], model_shape), total,
+                                              device, dtype, padded, mode="replicate")
+                                for p in range(t.shape[0])])
+        raise ValueError("Debye dispersion parameters must be scalar, model-shaped, or "
+                         f"[n_poles, *model_shape], but got shape {tuple(t.shape)}.")
+
+    return DebyeDispersion(delta_epsilon=pad(dispersion.delta_epsilon), tau=pad(dispersion.tau))
 
 
 def _profile_ptrs(profiles: list[torch.Tensor]) -> list[int | None]:
@@ -378,12 +416,26 @@ def _run_forward(setup: TM2DSetup, ca, cb, cq, f, fields, r, *, interval: int, f
         )
     else:
         fn = _native.get_function("maxwell_tm", "forward", setup.stencil, dtype)
+        debye = setup.extra.get("debye")
+        if debye is not None:
+            # polarization and Ey^n buffers of the Debye recurrence, kept on
+            # the setup so time chunks of one run continue it
+            state = setup.extra.setdefault("debye_state", {
+                "pol": torch.zeros(setup.n_shots, debye["a"].shape[0], setup.ny, setup.nx,
+                                   device=dev, dtype=dtype),
+                "ey_prev": torch.zeros(setup.n_shots, setup.ny, setup.nx, device=dev, dtype=dtype),
+            })
+            deb = (_native.ptr(debye["a"]), _native.ptr(debye["b"]), _native.ptr(debye["cp"]),
+                   _native.ptr(state["pol"]), _native.ptr(state["ey_prev"]))
+            n_poles = int(debye["a"].shape[0])
+        else:
+            deb, n_poles = (None,) * 5, 0
         _native.call(
             fn,
             _native.ptr(ca), _native.ptr(cb), _native.ptr(cq), _native.ptr(f),
             *[_native.ptr(t) for t in fields],
-            None, None, None, None, None, _native.ptr(r), 0,
-            *_profile_ptrs(setup.profiles), *tail, interval, False,
+            *deb, _native.ptr(r), n_poles,
+            *_profile_ptrs(setup.profiles), *tail, interval, debye is not None,
             setup.batched, setup.batched, setup.batched, start_t,
             *setup.pml, 0, dev_idx, stream,
         )
@@ -741,8 +793,6 @@ def _reject_unsupported(
         )
     if stencil not in (2, 4, 6, 8):
         raise ValueError(f"stencil must be 2, 4, 6 or 8, got {stencil}")
-    if dispersion is not None:
-        raise NotImplementedError("Debye dispersion is not part of this build")
     if device.type != "cuda":
         raise NotImplementedError(
             f"the MI355X build runs on HIP devices only; got tensors on {device}"
@@ -906,14 +956,26 @@ def _prepare_call(
             if bool((t >= hi).any()):
                 raise RuntimeError(f"{name} location out of bounds for model shape {tuple(epsilon.shape[-2:])}")
 
+    model_rg = force_storage or epsilon_in.requires_grad or sigma_in.requires_grad
+    if dispersion is not None:
+        # the reference's native Debye path is forward inference only
+        # (tm2d_cuda.py:360-370); its gradients run on its Python backend
+        src_rg = source_amplitude is not None and source_amplitude.requires_grad
+        if model_rg or src_rg or save_snapshots or forward_callback or backward_callback:
+            raise NotImplementedError(
+                "Debye dispersion runs forward inference only (as the reference's native "
+                "backend); gradients, snapshots and callbacks with dispersion are not part of "
+                "this build")
+        if bs is not None and bs[0] > 1:
+            raise NotImplementedError("Debye dispersion with a batch of models is not supported")
     setup = prepare_tm2d(
         epsilon, sigma, mu, spacing, dt, source_amplitude, source_location, receiver_location,
         stencil=stencil, pml_width=pml_width, nt=nt, initial=initial, max_vel=max_vel,
+        dispersion=dispersion,
     )
     if setup.nt > 0:
         interval = min(interval, setup.nt)
 
-    model_rg = force_storage or epsilon_in.requires_grad or sigma_in.requires_grad
     do_save = model_rg if save_snapshots is None else bool(save_snapshots)
     kind, _, bpe, fmt = _resolve_storage_compression(
         storage_compression, epsilon.dtype, epsilon.device, context="storage_compression"

@@ -42,9 +42,51 @@ def prepare_parameters(
     return ca, cb, cq
 
 
-def validate_material_inputs(epsilon_r: torch.Tensor) -> None:
+def debye_parameter(value, *, reference: torch.Tensor, name: str) -> torch.Tensor:
+    """A Debye parameter as ``[n_poles, *model_shape]``: scalars and
+    model-shaped (broadcastable) tensors are one pole, a leading extra axis
+    counts the poles (reference ``utils.py:44-76``)."""
+    t = torch.as_tensor(value, device=reference.device, dtype=reference.dtype)
+    shape = tuple(reference.shape)
+    if t.ndim <= len(shape):
+        try:
+            torch.broadcast_shapes(tuple(t.shape), shape)
+        except RuntimeError as exc:
+            raise ValueError(f"{name} with shape {tuple(t.shape)} is not broadcastable "
+                             f"to model shape {shape}.") from exc
+        t = t.reshape((1,) + tuple(t.shape))
+    elif t.ndim == len(shape) + 1:
+        try:
+            torch.broadcast_shapes(tuple(t.shape[1:]), shape)
+        except RuntimeError as exc:
+            raise ValueError(f"{name} with shape {tuple(t.shape)} is not broadcastable "
+                             f"to model shape {shape} on trailing dimensions.") from exc
+    else:
+        raise ValueError(f"{name} must be a scalar, model-shaped tensor, or "
+                         f"[n_poles, *model_shape], but got shape {tuple(t.shape)}.")
+    return torch.broadcast_to(t, (t.shape[0], *shape))
+
+
+def validate_material_inputs(epsilon_r: torch.Tensor, *, dispersion=None,
+                             dt: float | None = None) -> None:
+    """Positive permittivity; for Debye poles delta_epsilon >= 0, tau > 0 and
+    dt < min(tau) (reference ``utils.py:79-116``)."""
     if torch.any(epsilon_r <= 0):
         raise ValueError("epsilon must be strictly positive.")
+    if dispersion is None:
+        return
+    if dt is None:
+        raise ValueError("dt must be provided when validating dispersion inputs.")
+    d_eps = debye_parameter(dispersion.delta_epsilon, reference=epsilon_r, name="delta_epsilon")
+    tau = debye_parameter(dispersion.tau, reference=epsilon_r, name="tau")
+    if torch.any(d_eps < 0):
+        raise ValueError("delta_epsilon must be non-negative.")
+    if torch.any(tau <= 0):
+        raise ValueError("tau must be strictly positive.")
+    min_tau = float(tau.detach().amin().item())
+    if dt >= min_tau:
+        raise ValueError(
+            f"Debye dispersion requires dt < min(tau), but got dt={dt} and min(tau)={min_tau}.")
 
 
 def compile_material_coefficients(
@@ -53,13 +95,43 @@ def compile_material_coefficients(
     mu_r: torch.Tensor,
     dt: float,
     *,
+    dispersion=None,
     validate_inputs: bool = True,
-) -> dict[str, torch.Tensor]:
-    """Coefficients for the non-dispersive update."""
+) -> dict:
+    """Solver coefficients (reference ``utils.py:119-186``).
+
+    Without dispersion: ``ca, cb, cq`` of the plain update.  With Debye
+    poles (``dispersion.delta_epsilon``, ``dispersion.tau``) the
+    polarization of pole p obeys the trapezoidal recurrence
+    ``P_p = a_p P_p + b_p (E^{n+1} + E^n)``, ``a_p = (2 tau_p - dt) /
+    (2 tau_p + dt)``, ``b_p = EP0 delta_eps_p dt / (2 tau_p + dt)``, and the
+    E update becomes ``E^{n+1} = ca E^n + cb curl H + sum_p cp_p P_p`` with
+    ``den = EP0 eps_inf + sigma dt / 2 + sum_p b_p``, ``ca = (EP0 eps_inf -
+    sigma dt / 2 - sum_p b_p) / den``, ``cb = dt / den``, ``cp_p = (1 - a_p) /
+    den``; the result also holds ``has_dispersion`` and ``debye`` (``a``,
+    ``b``, ``cp`` as ``[n_poles, *model_shape]``, ``n_poles``)."""
     if validate_inputs:
-        validate_material_inputs(epsilon_r)
+        validate_material_inputs(epsilon_r, dispersion=dispersion, dt=dt)
     ca, cb, cq = prepare_parameters(epsilon_r, sigma, mu_r, dt)
-    return {"ca": ca, "cb": cb, "cq": cq}
+    out = {"ca": ca, "cb": cb, "cq": cq, "dispersion": dispersion, "has_dispersion": False}
+    if dispersion is None:
+        return out
+    d_eps = debye_parameter(dispersion.delta_epsilon, reference=epsilon_r, name="delta_epsilon")
+    tau = debye_parameter(dispersion.tau, reference=epsilon_r, name="tau")
+    eps_inf = epsilon_r * EP0
+    tau_term = 2.0 * tau + dt
+    a = (2.0 * tau - dt) / tau_term
+    b = (EP0 * d_eps * dt) / tau_term
+    b_sum = b.sum(dim=0)
+    den = eps_inf + sigma * dt / 2.0 + b_sum
+    out.update({
+        "ca": (eps_inf - sigma * dt / 2.0 - b_sum) / den,
+        "cb": dt / den,
+        "cq": dt / (mu_r * MU0),
+        "has_dispersion": True,
+        "debye": {"a": a, "b": b, "cp": (1.0 - a) / den.unsqueeze(0), "n_poles": int(a.shape[0])},
+    })
+    return out
 
 
 def coefficient_partials(
