@@ -77,6 +77,10 @@ def parse() -> argparse.Namespace:
                     help="also run one 1 GiB device copy (profiler byte-counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="served-solve trace length (0: skip the serving measurement)")
+    ap.add_argument("--serve-skip-plain", action="store_true",
+                    help="serving: time only the served path (policy sweeps)")
+    ap.add_argument("--serve-boost", type=int, default=16,
+                    help="serving engine: most steps of a small job per tick (x largest)")
     ap.add_argument("--serve-rates", default="2,inf",
                     help="Poisson arrival rates (per second) of the serving sweep; inf = all at once")
     ap.add_argument("--serve-seed", type=int, default=2024)
@@ -258,7 +262,7 @@ def serve_bench(args, dev, rate: float) -> dict:
     work = sum(_request_updates(kw) for _, kw in trace)
     log(f"serving trace: {len(trace)} requests, {work:.3e} cell-updates")
     torch.cuda.empty_cache()
-    server = TM2DServer(dev, steps_per_tick=args.serve_tick)
+    server = TM2DServer(dev, steps_per_tick=args.serve_tick, tick_boost=args.serve_boost)
     budget = server.budget
     # first contact with the kernels and allocator outside the measurement
     server.serve([(0.0, dict(trace[0][1]))])
@@ -270,6 +274,10 @@ def serve_bench(args, dev, rate: float) -> dict:
     server.close()
     del server
     torch.cuda.empty_cache()
+    r3 = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
+    if args.serve_skip_plain:
+        return {"rate_per_s": rate if math.isfinite(rate) else "inf", "boost": args.serve_boost,
+                "requests": len(trace), "served": r3(served)}
     # The server allocated its pool before the trace; the plain path gets the
     # same: one untimed pass over the requests leaves its buffers in torch's
     # caching allocator, so its timed solves do not pay first-time device
@@ -293,7 +301,6 @@ def serve_bench(args, dev, rate: float) -> dict:
              "p99_s": pct(0.99), "mean_s": sum(lat) / len(lat), "span_s": t_free - first,
              "gcell_per_s": work / (t_free - first) / 1e9}
     log(f"plain: {plain}")
-    r3 = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
     return {
         "rate_per_s": rate if math.isfinite(rate) else "inf",
         "workload": f"{len(trace)} Poisson arrivals at {rate}/s (seed {args.serve_seed}), "
@@ -439,9 +446,23 @@ def config1_bench(dev, reps: int = 20) -> dict:
 
     t_graph = timed(run_plan)
     plan.close()
+    # where the plain call's time goes: the host set-up (padding,
+    # coefficients, profiles, point indices) and the native time loop alone
+    # (150 two-step launches on a prepared setup)
+    from tide.maxwell.tm2d import _run_forward, prepare_tm2d
+
+    args = (kw["epsilon"], kw["sigma"], kw["mu"], dx, dt, kw["source_amplitude"],
+            kw["source_location"], kw["receiver_location"])
+    t_setup = timed(lambda: prepare_tm2d(*args, pml_width=10))
+    st = prepare_tm2d(*args, pml_width=10)
+    r = torch.zeros(nt, 1, 1, device=dev)
+    t_loop = timed(lambda: _run_forward(st, st.ca, st.cb, st.cq, st.f, st.fields, r, interval=1,
+                                        fmt=1))
     return {
         "workload": "96x96 (117x117 padded, CPML 10), eps_r 4, 1 shot, nt 300, forward, fp32",
         "plain_ms": round(t_plain * 1e3, 3), "plain_us_per_step": round(t_plain / nt * 1e6, 2),
+        "plain_setup_ms": round(t_setup * 1e3, 3),
+        "plain_native_loop_ms": round(t_loop * 1e3, 3),
         "graph_ms": round(t_graph * 1e3, 3), "graph_us_per_step": round(t_graph / nt * 1e6, 2),
         "graph_gcell_per_s": round(updates / t_graph / 1e9, 3),
     }

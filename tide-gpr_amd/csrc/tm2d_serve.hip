@@ -268,6 +268,9 @@ struct Engine {
   int rows_f = 4, rows_a = 2;
   bool two_step = false;  // launches of two steps (set from the time block at creation)
   int64_t max_jobs = 64;
+  // most steps a small job takes per tick, as a multiple of the largest
+  // job's (tide_tm2d_engine_set_boost)
+  int64_t boost = 16;
   std::map<int64_t, EJob> jobs;
   int64_t next_id = 1;
   ServeItem *d_items = nullptr;
@@ -517,10 +520,9 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
   if (pairs) max_steps &= ~(int64_t)1;
   // Steps per job this tick: the largest running job (cells x shots per
   // step) takes max_steps, a smaller one proportionally more, up to
-  // kTickBoost times: in lock-step launches a small job would otherwise move
+  // e.boost times: in lock-step launches a small job would otherwise move
   // at the large one's pace.  Allotments are multiples of max_steps, so
   // every non-final tick of a job is still an even number of steps.
-  constexpr int64_t kTickBoost = 16;
   auto cells = [](const EJob &j) {
     return (double)j.spec.p.n_shots * (double)j.spec.p.ny * (double)j.spec.p.nx;
   };
@@ -540,7 +542,7 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
       continue;
     }
     const double ratio = biggest / std::max(cells(j), 1.0);
-    const int64_t boost = std::max<int64_t>(1, std::min<int64_t>(kTickBoost, (int64_t)ratio));
+    const int64_t boost = std::max<int64_t>(1, std::min<int64_t>(e.boost, (int64_t)ratio));
     run.emplace_back(std::min<int64_t>(left, max_steps * boost), &j);
   }
   if (run.empty()) {
@@ -715,6 +717,16 @@ extern "C" int tide_tm2d_engine_create(int64_t device, int64_t group, int64_t ma
     return kHipError;
   }
   *engine = e;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_engine_set_boost(void *engine, int64_t boost) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr || boost < 1 || boost > 64) {
+    set_error("engine boost: 1..64 (most steps of a small job per tick, in largest-job steps)");
+    return kInvalidArgument;
+  }
+  e->boost = boost;
   return kOk;
 }
 

@@ -318,7 +318,9 @@ def _chunk_setup(setup: TM2DSetup, s0: int, s1: int, fields: list[torch.Tensor])
     sub.sources_i = setup.sources_i[s0:s1].contiguous() if setup.n_src > 0 else setup.sources_i
     sub.receivers_i = setup.receivers_i[s0:s1].contiguous() if setup.n_rec > 0 else setup.receivers_i
     sub.fields = fields
-    sub.extra = {}
+    # a chunk keeps the Debye pole coefficients (shared by all shots); its
+    # polarization state is its own (shots are independent)
+    sub.extra = {"debye": setup.extra.get("debye")}
     return sub
 
 

@@ -71,6 +71,8 @@ def _bind(lib: ctypes.CDLL) -> None:
         return
     lib.tide_tm2d_engine_create.restype = ctypes.c_int
     lib.tide_tm2d_engine_create.argtypes = [_I64, _I64, _I64, ctypes.POINTER(_P)]
+    lib.tide_tm2d_engine_set_boost.restype = ctypes.c_int
+    lib.tide_tm2d_engine_set_boost.argtypes = [_P, _I64]
     lib.tide_tm2d_engine_group.restype = ctypes.c_int
     lib.tide_tm2d_engine_group.argtypes = [_P, ctypes.POINTER(_I64)]
     lib.tide_tm2d_engine_add.restype = ctypes.c_int
@@ -358,7 +360,9 @@ class TM2DServer:
         Time steps the largest running job advances per engine tick
         (admission and completion happen at tick boundaries); a smaller job
         advances proportionally more (by the ratio of cells x shots per step,
-        up to 16 times), so it is not held to the large job's pace.
+        up to ``tick_boost`` times), so it is not held to the large job's pace.
+    tick_boost:
+        That cap (default 16; 1 advances every job by the same steps).
     group:
         Shots per workgroup, the same for every job of a launch (default 2:
         requests with few shots leave fewer idle wavefronts).
@@ -376,6 +380,7 @@ class TM2DServer:
         steps_per_tick: int = 64,
         group: int | None = None,
         max_bypass: int = 8,
+        tick_boost: int = 16,
     ) -> None:
         dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         if dev.type != "cuda":
@@ -392,6 +397,7 @@ class TM2DServer:
         handle = _P()
         _native.check(self._lib.tide_tm2d_engine_create(dev.index, self.group, self.max_jobs, ctypes.byref(handle)))
         self._engine = handle
+        _native.check(self._lib.tide_tm2d_engine_set_boost(self._engine, int(tick_boost)))
         self.stream = torch.cuda.Stream(dev)
         self._prep = torch.cuda.Stream(dev)
         if hbm_budget is None:
