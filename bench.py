@@ -79,8 +79,12 @@ def parse() -> argparse.Namespace:
                     help="served-solve trace length (0: skip the serving measurement)")
     ap.add_argument("--serve-skip-plain", action="store_true",
                     help="serving: time only the served path (policy sweeps)")
-    ap.add_argument("--serve-boost", type=int, default=16,
+    ap.add_argument("--serve-boost", type=int, default=4,
                     help="serving engine: most steps of a small job per tick (x largest)")
+    ap.add_argument("--serve-head-weight", type=int, default=1,
+                    help="serving engine: step multiplier of the oldest request's jobs")
+    ap.add_argument("--serve-max-bypass", type=int, default=8,
+                    help="serving: admissions that may overtake a waiting chunk")
     ap.add_argument("--serve-rates", default="2,inf",
                     help="Poisson arrival rates (per second) of the serving sweep; inf = all at once")
     ap.add_argument("--serve-seed", type=int, default=2024)
@@ -262,7 +266,8 @@ def serve_bench(args, dev, rate: float) -> dict:
     work = sum(_request_updates(kw) for _, kw in trace)
     log(f"serving trace: {len(trace)} requests, {work:.3e} cell-updates")
     torch.cuda.empty_cache()
-    server = TM2DServer(dev, steps_per_tick=args.serve_tick, tick_boost=args.serve_boost)
+    server = TM2DServer(dev, steps_per_tick=args.serve_tick, tick_boost=args.serve_boost,
+                        head_weight=args.serve_head_weight, max_bypass=args.serve_max_bypass)
     budget = server.budget
     # first contact with the kernels and allocator outside the measurement
     server.serve([(0.0, dict(trace[0][1]))])
@@ -277,6 +282,7 @@ def serve_bench(args, dev, rate: float) -> dict:
     r3 = lambda d: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in d.items()}  # noqa: E731
     if args.serve_skip_plain:
         return {"rate_per_s": rate if math.isfinite(rate) else "inf", "boost": args.serve_boost,
+                "head_weight": args.serve_head_weight, "max_bypass": args.serve_max_bypass,
                 "requests": len(trace), "served": r3(served)}
     # The server allocated its pool before the trace; the plain path gets the
     # same: one untimed pass over the requests leaves its buffers in torch's
@@ -296,7 +302,13 @@ def serve_bench(args, dev, rate: float) -> dict:
         t_free = start + svc
         lat.append(t_free - arrival)
     lat.sort()
-    pct = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]  # noqa: E731
+
+    def pct(q: float) -> float:  # the interpolation of tide.serve.latency_stats
+        pos = q * (len(lat) - 1)
+        lo = int(math.floor(pos))
+        hi = min(lo + 1, len(lat) - 1)
+        return lat[lo] + (lat[hi] - lat[lo]) * (pos - lo)
+
     plain = {"solves": len(lat), "solves_per_s": len(lat) / (t_free - first), "p50_s": pct(0.5),
              "p99_s": pct(0.99), "mean_s": sum(lat) / len(lat), "span_s": t_free - first,
              "gcell_per_s": work / (t_free - first) / 1e9}
@@ -308,6 +320,8 @@ def serve_bench(args, dev, rate: float) -> dict:
                     "1-32 shots, 64 receivers, fp32 snapshots",
         "cell_updates": work,
         "hbm_budget_bytes": budget,
+        "engine": {"tick_steps": args.serve_tick, "boost": args.serve_boost,
+                   "head_weight": args.serve_head_weight, "max_bypass": args.serve_max_bypass},
         "served": r3(served),
         "plain_one_at_a_time": r3(plain),
         "solves_per_s_ratio": round(served["solves_per_s"] / plain["solves_per_s"], 3),

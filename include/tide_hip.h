@@ -1347,6 +1347,10 @@ int tide_tm2d_engine_group(void *engine, int64_t *group);
  * proportion to their cells x shots per step. */
 int tide_tm2d_engine_set_boost(void *engine, int64_t boost);
 int tide_tm2d_engine_add(void *engine, const tide_tm2d_job *job, int64_t *job_id);
+/* Step multiplier of one running job (1..64, default 1): its allotment per
+ * tick is multiplied by it, a priority (the server gives its oldest
+ * request's jobs more steps per tick than newer ones). */
+int tide_tm2d_engine_set_weight(void *engine, int64_t job, int64_t weight);
 /* Enqueues up to max_steps steps of every running job on `stream`;
  * *n_running = jobs still running after it. */
 int tide_tm2d_engine_tick(void *engine, int64_t max_steps, void *stream, int64_t *n_running);
@@ -1644,6 +1648,16 @@ int tide_em3d_set_tuning(int fwd_z, int adj_z, int group);
  * tide_em3d_set_tuning.  No reference counterpart.
  */
 int tide_em3d_set_forward_group(int group);
+
+/*
+ * Register budget of the fp32 K-step (3- or 4-step) 2-D adjoint: 4 (the
+ * default) holds the kernel to four waves per SIMD (128 VGPRs, spilling the
+ * rest; workgroups of up to 256 threads), 0 leaves it to the compiler.
+ * fp64 kernels always use the compiler's count.  Results do not depend on
+ * it.  No reference counterpart.
+ */
+int tide_tm2d_set_adjoint_waves(int waves);
+int tide_tm2d_get_adjoint_waves(int *out);
 
 /*
  * One-launch 3-D forward step (the H and E updates of a time step in one

@@ -106,6 +106,21 @@ def test_k_step_fp64_medium_case_matches_oracle(k):
         _check(case)
 
 
+@pytest.mark.parametrize("k", [3, 4])
+@pytest.mark.parametrize("waves", [0, 4])
+def test_k_step_adjoint_register_budgets_match_oracle(k, waves):
+    """The fp32 K-step adjoint at the compiler's register count (0) and held
+    to 128 VGPRs (4, spilling) gives the oracle's results."""
+    case = _medium_case(nt=70, shots=4)
+    before = tide._native.get_tm2d_adjoint_waves()
+    tide._native.set_tm2d_adjoint_waves(waves)
+    try:
+        with shape(k, 4, 4, 2):
+            _check(case)
+    finally:
+        tide._native.set_tm2d_adjoint_waves(before)
+
+
 def test_k_step_bf16_snapshots_match_oracle():
     case = _medium_case(nt=66, shots=2)
     with shape(4, 4, 2, 0):

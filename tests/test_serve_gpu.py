@@ -105,6 +105,26 @@ def test_served_solves_match_the_operator(problems, mixed):
             assert e_eps < 1e-5 and e_sig < 1e-5
 
 
+def test_one_step_ticks_of_a_mix_match_the_operator(problems):
+    """One-step ticks (no two-step launches): smaller jobs and the oldest
+    request take several consecutive single steps per tick (size boost and
+    head weight), each flipping the job's state buffer; receivers equal the
+    operator's bitwise and gradients agree within 1e-5."""
+    server = TM2DServer(DEV, hbm_budget=BUDGET, steps_per_tick=1, tick_boost=4, head_weight=4)
+    tickets = [server.submit(p, **_submit_kwargs(s, p)) for s, p in zip(SPECS, problems)]
+    server.drain()
+    torch.cuda.synchronize()
+    for i, (spec, p, t) in enumerate(zip(SPECS, problems, tickets)):
+        print(f"request {i} (one-step ticks): {spec}", flush=True)
+        r, loss, ge, gs = _reference(spec, p)
+        same = torch.equal(t.result.receiver_data, r)
+        assert same, "receivers differ from the operator's"
+        if spec["kind"] == "gradient":
+            assert relative_l2(t.result.grad_epsilon.cpu().numpy(), ge.cpu().numpy()) < 1e-5
+            assert relative_l2(t.result.grad_sigma.cpu().numpy(), gs.cpu().numpy()) < 1e-5
+    server.close()
+
+
 def test_each_solve_is_the_same_alone_and_in_the_mix(problems, mixed):
     for i, (spec, p, t_mix) in enumerate(zip(SPECS, problems, mixed)):
         print(f"request {i} alone", flush=True)

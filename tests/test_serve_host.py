@@ -129,3 +129,44 @@ def test_latency_stats():
     assert s["solves"] == 101
     assert math.isclose(s["p50_s"], 1.5) and math.isclose(s["p99_s"], 1.99)
     assert math.isclose(s["span_s"], 102.0) and math.isclose(s["solves_per_s"], 101 / 102.0)
+
+
+def test_head_weight_goes_to_the_oldest_request_with_running_jobs():
+    """The oldest request with running jobs has all of its jobs weighted by
+    head_weight and every other job by 1; the weights follow the head as it
+    finishes, and weight 1 makes no engine calls."""
+    from types import SimpleNamespace
+
+    from tide.serve import TM2DServer
+
+    calls = []
+
+    class FakeLib:
+        def tide_tm2d_engine_set_weight(self, engine, job, weight):
+            calls.append((job, weight))
+            return 0
+
+    def chunk(arrival, tid):
+        return SimpleNamespace(ticket=SimpleNamespace(arrival=arrival, id=tid))
+
+    srv = object.__new__(TM2DServer)
+    srv._lib, srv._engine, srv.head_weight, srv._weighted = FakeLib(), None, 4, set()
+    old, new = chunk(1.0, 0), chunk(2.0, 1)
+    old2 = SimpleNamespace(ticket=old.ticket)  # a second chunk of the old request
+    srv._jobs = {10: new, 11: old, 12: old2}
+    srv._prioritize()
+    assert sorted(calls) == [(11, 4), (12, 4)]
+    calls.clear()
+    srv._prioritize()  # nothing changed: no calls
+    assert calls == []
+    del srv._jobs[11], srv._jobs[12]  # the old request finished
+    srv._prioritize()
+    assert calls == [(10, 4)]
+    calls.clear()
+    srv._jobs[13] = chunk(0.5, 2)  # an older request's chunk is admitted late
+    srv._prioritize()
+    assert sorted(calls) == [(10, 1), (13, 4)]
+    calls.clear()
+    srv.head_weight = 1
+    srv._prioritize()
+    assert calls == []

@@ -1438,6 +1438,24 @@ extern "C" int tide_tm2d_get_shot_loop(int *out2) {
   return kOk;
 }
 
+extern "C" int tide_tm2d_set_adjoint_waves(int waves) {
+  if (waves != 0 && waves != 4) {
+    set_error("adjoint waves: 0 (compiler's register count) or 4 (128 VGPRs)");
+    return kInvalidArgument;
+  }
+  g_tuning.adj_waves = waves;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_get_adjoint_waves(int *out) {
+  if (out == nullptr) {
+    set_error("get_adjoint_waves: null output");
+    return kInvalidArgument;
+  }
+  *out = g_tuning.adj_waves;
+  return kOk;
+}
+
 extern "C" int tide_tm2d_set_time_block(int fwd_steps, int adj_steps) {
   if (fwd_steps < 1 || fwd_steps > kMaxBlockSteps || adj_steps < 1 ||
       adj_steps > kMaxBlockSteps) {

@@ -153,12 +153,15 @@ __device__ __forceinline__ void fwdk_tile(const FwdKArgs<T> &a, int s, int xt, i
 #pragma unroll
     for (int k = j; k < NX - j; ++k) {
       const int y = yb + k;
+      const bool pyh = GY && (y < g.pml_y0 || y >= g.pml_y1h);
+      T pa, pb, pk;  // read with every lane active
+      yp.row3(kAh, kBh, kKh, y, pyh, pa, pb, pk);
       if (e_col && (!GY || (y >= 1 && y < ny - 1))) {
         T d = (ey[k + 1] - ey[k]) * g.rdy;
-        if (GY && (y < g.pml_y0 || y >= g.pml_y1h)) {
-          const T m = yp(kBh, y) * mez[k] + yp(kAh, y) * d;
+        if (pyh) {
+          const T m = pb * mez[k] + pa * d;
           mez[k] = m;
-          d = d / yp(kKh, y) + m;
+          d = d / pk + m;
         }
         hx[k] = hx[k] - cqv[k] * d;
       }
@@ -183,6 +186,9 @@ __device__ __forceinline__ void fwdk_tile(const FwdKArgs<T> &a, int s, int xt, i
       }
       const T hz_l = shfl_up1(hz[k]);  // unused at lane 0
       T ey_n = ey_c;
+      const bool pye = GY && (y < g.pml_y0 || y >= g.pml_y1);
+      T pa, pb, pk;  // read with every lane active
+      yp.row3(kA, kB, kK, y, pye, pa, pb, pk);
       if (e_col && row) {
         T dhz = (hz[k] - hz_l) * g.rdx;
         T dhx = (hx[k + 1] - hx[k]) * g.rdy;
@@ -191,10 +197,10 @@ __device__ __forceinline__ void fwdk_tile(const FwdKArgs<T> &a, int s, int xt, i
           mhzx[k] = m;
           dhz = dhz / xk + m;
         }
-        if (GY && (y < g.pml_y0 || y >= g.pml_y1)) {
-          const T m = yp(kB, y) * mhxz[k] + yp(kA, y) * dhx;
+        if (pye) {
+          const T m = pb * mhxz[k] + pa * dhx;
           mhxz[k] = m;
-          dhx = dhx / yp(kK, y) + m;
+          dhx = dhx / pk + m;
         }
         const T curl = dhz - dhx;
         if constexpr (FMT != 0) {
@@ -405,21 +411,29 @@ __device__ __forceinline__ void adjk_tile(const AdjKArgs<T> &a, int s, int xt, i
     m = mn;
     return xa * mn + q / xk;
   };
+  // nraw_y and raw_z read the row's y profile before any lane leaves (the
+  // cross-lane reads need every lane active); call them with all lanes active.
   auto nraw_y = [&](int y, T lm, T cbx, T &m) -> T {  // negated adjoint of dHx/dz
+    const bool pye = GY && (y < g.pml_y0 || y >= g.pml_y1);
+    T pa, pb, pk;
+    yp.row3(kA, kB, kK, y, pye, pa, pb, pk);
     if ((GY && (y < 1 || y >= ny)) || !e_col) return (T)0;
     const T q = cbx * lm;
-    if (!(GY && (y < g.pml_y0 || y >= g.pml_y1))) return q;
-    const T mn = yp(kB, y) * m - q;
+    if (!pye) return q;
+    const T mn = pb * m - q;
     m = mn;
-    return q / yp(kK, y) - yp(kA, y) * mn;
+    return q / pk - pa * mn;
   };
   auto raw_z = [&](int y, T lhxv, T cqx, T &m) -> T {  // adjoint of dEy/dz (Hx update)
+    const bool pyh = GY && (y < g.pml_y0 || y >= g.pml_y1h);
+    T pa, pb, pk;
+    yp.row3(kAh, kBh, kKh, y, pyh, pa, pb, pk);
     if ((GY && (y < 1 || y >= ny - 1)) || !e_col) return (T)0;
     const T adj = -cqx * lhxv;
-    if (!(GY && (y < g.pml_y0 || y >= g.pml_y1h))) return adj;
-    const T mn = yp(kBh, y) * m + adj;
+    if (!pyh) return adj;
+    const T mn = pb * m + adj;
     m = mn;
-    return yp(kAh, y) * mn + adj / yp(kKh, y);
+    return pa * mn + adj / pk;
   };
   auto raw_xh = [&](int y, T lhzv, T cqx, T &m) -> T {  // adjoint of dEy/dx (Hz update)
     if ((GY && (y < 1 || y >= ny)) || (GX && (x < 1 || x >= nx - 1))) return (T)0;
@@ -590,8 +604,8 @@ __device__ __forceinline__ void adjk_workgroup(const AdjKArgs<T> &a, const TileC
   }
 }
 
-template <typename T, int TY, int K, int FMT, int SET, int TYB = TY>
-__global__ __launch_bounds__(512) void adjk_step(AdjKArgs<T> a) {
+template <typename T, int TY, int K, int FMT, int SET, int TYB>
+__device__ __forceinline__ void adjk_body(const AdjKArgs<T> &a) {
   extern __shared__ double red[];
   const int G = (int)blockDim.y;
   const int n_groups = (a.n_shots + G - 1) / G;
@@ -607,6 +621,18 @@ __global__ __launch_bounds__(512) void adjk_step(AdjKArgs<T> a) {
   } else {
     adjk_workgroup<T, TY, K, FMT, 1 | 8>(a, tc, edge, red);
   }
+}
+
+template <typename T, int TY, int K, int FMT, int SET, int TYB = TY>
+__global__ __launch_bounds__(512) void adjk_step(AdjKArgs<T> a) {
+  adjk_body<T, TY, K, FMT, SET, TYB>(a);
+}
+
+// The same kernel held to W waves per SIMD (512 / W VGPRs; workgroups of at
+// most 256 threads), spilling what does not fit.
+template <typename T, int TY, int K, int FMT, int SET, int TYB, int W>
+__global__ __launch_bounds__(256, W) void adjk_step_w(AdjKArgs<T> a) {
+  adjk_body<T, TY, K, FMT, SET, TYB>(a);
 }
 
 }  // namespace tm2d

@@ -124,6 +124,22 @@ template <typename T> struct LaneRows {
     for (int k = 0; k < 6; ++k) v[k] = g.Y(k, yc);
   }
   __device__ __forceinline__ T operator()(int k, int y) const { return lane_read(v[k], y - y_first); }
+  // Entries (pa, pb, pk) = (ka, kb, kk) of row y, or (0, 0, 1) when `on`
+  // (wave-uniform) is false.  These are cross-lane reads: call this where
+  // every lane of the wavefront is active.  In a lane-divergent branch the
+  // lane read may be inactive, and a register that the compiler spills and
+  // reloads there keeps its active lanes only.
+  __device__ __forceinline__ void row3(int ka, int kb, int kk, int y, bool on, T &pa, T &pb,
+                                       T &pk) const {
+    pa = (T)0;
+    pb = (T)0;
+    pk = (T)1;
+    if (on) {
+      pa = (*this)(ka, y);
+      pb = (*this)(kb, y);
+      pk = (*this)(kk, y);
+    }
+  }
 };
 
 // Row loader for one wavefront.  Interior tiles (EDGE false) need no bounds
@@ -691,7 +707,7 @@ constexpr int kMaxGroup = 8;  // shots (wavefronts) per workgroup
 // interior and border tiles as two kernels.  On a 1065^2 grid with 4 or 7
 // shots per call the forward is fastest at 2 steps per launch (20.8 us per
 // step against 22+ at 3 or 4) and the adjoint at 3 (32.2 against 33.6 at
-// 2 and 40.5 at 4).
+// 2 and 40.5 at 4, at the compiler's register count).
 struct Tuning {
   int fwd_rows = 4, fwd_group = 1, adj_rows = 4, adj_group = 4;
   int fwd_loop = 0, adj_loop = 0;
@@ -701,6 +717,13 @@ struct Tuning {
   // height).  The forward is fastest with interior-height border tiles, the
   // adjoint (more live state per border tile) with 2-row ones.
   int fwd_border_rows = 0, adj_border_rows = 2;
+  // fp32 K-step adjoint held to four waves per SIMD (128 VGPRs, about 19
+  // spilled dwords per lane at K = 3) when 4, the compiler's own register
+  // count (148 VGPRs, three waves) when 0.  Four waves hide more latency:
+  // 29.3 against 31.7 us per step on a 1065^2 grid with 4 shots, 54.7
+  // against 58.5 with 7; five waves (96 VGPRs) spill too much: 44.5.  fp64
+  // kernels (229+ VGPRs) always use the compiler's count.
+  int adj_waves = 4;
 };
 extern Tuning g_tuning;
 

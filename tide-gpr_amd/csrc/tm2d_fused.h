@@ -330,12 +330,15 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   for (int k = 0; k < NX; ++k) {
     const int y = y0 - 2 + k;
     hx1[k] = hx0[k];
+    const bool pyh = GY && (y < g.pml_y0 || y >= g.pml_y1h);
+    T pa, pb, pk;  // read with every lane active
+    yp.row3(kAh, kBh, kKh, y, pyh, pa, pb, pk);
     if (e_col && (!GY || (y >= 1 && y < ny - 1))) {
       T d = (ey0[k + 1] - ey0[k]) * g.rdy;
-      if (GY && (y < g.pml_y0 || y >= g.pml_y1h)) {
-        const T m = yp(kBh, y) * mez[k] + yp(kAh, y) * d;
+      if (pyh) {
+        const T m = pb * mez[k] + pa * d;
         mez[k] = m;
-        d = d / yp(kKh, y) + m;
+        d = d / pk + m;
       }
       hx1[k] = hx0[k] - cqv[k] * d;
     }
@@ -360,6 +363,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
     }
     const T hz_l = shfl_up1(hz1[k]);  // unused at lane 0
     T ey_n = ey_c;
+    const bool pye = GY && (y < g.pml_y0 || y >= g.pml_y1);
+    T pa, pb, pk;  // read with every lane active
+    yp.row3(kA, kB, kK, y, pye, pa, pb, pk);
     if (e_col && row) {
       T dhz = (hz1[k] - hz_l) * g.rdx;
       T dhx = (hx1[k + 1] - hx1[k]) * g.rdy;
@@ -368,10 +374,10 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
         mhzx[k] = m;
         dhz = dhz / xk + m;
       }
-      if (GY && (y < g.pml_y0 || y >= g.pml_y1)) {
-        const T m = yp(kB, y) * mhxz[k] + yp(kA, y) * dhx;
+      if (pye) {
+        const T m = pb * mhxz[k] + pa * dhx;
         mhxz[k] = m;
-        dhx = dhx / yp(kK, y) + m;
+        dhx = dhx / pk + m;
       }
       const T curl = dhz - dhx;
       if constexpr (FMT != 0) {
@@ -395,12 +401,15 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
   for (int k = 0; k < TY + 1; ++k) {
     const int y = y0 - 1 + k;
     hx2[k] = hx1[k + 1];
+    const bool pyh = GY && (y < g.pml_y0 || y >= g.pml_y1h);
+    T pa, pb, pk;  // read with every lane active
+    yp.row3(kAh, kBh, kKh, y, pyh, pa, pb, pk);
     if (e_col && (!GY || (y >= 1 && y < ny - 1))) {
       T d = (ey1[k + 1] - ey1[k]) * g.rdy;
-      if (GY && (y < g.pml_y0 || y >= g.pml_y1h)) {
-        const T m = yp(kBh, y) * mez[k + 1] + yp(kAh, y) * d;
+      if (pyh) {
+        const T m = pb * mez[k + 1] + pa * d;
         mez[k + 1] = m;
-        d = d / yp(kKh, y) + m;
+        d = d / pk + m;
       }
       hx2[k] = hx1[k + 1] - cqv[k + 1] * d;
     }
@@ -428,6 +437,9 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
     }
     const T hz_l = shfl_up1(hz_n);
     T ey_n = ey_c;
+    const bool pye = GY && (y < g.pml_y0 || y >= g.pml_y1);
+    T pa, pb, pk;  // read with every lane active
+    yp.row3(kA, kB, kK, y, pye, pa, pb, pk);
     if (e_col && row) {
       T dhz = (hz_n - hz_l) * g.rdx;
       T dhx = (hx2[k + 1] - hx2[k]) * g.rdy;
@@ -436,10 +448,10 @@ __device__ __forceinline__ void fwd2_tile(const Fwd2Args<T> &a, int s, int xt, i
         mhzx[k + 1] = m;
         dhz = dhz / xk + m;
       }
-      if (GY && (y < g.pml_y0 || y >= g.pml_y1)) {
-        const T m = yp(kB, y) * mhxz[k + 1] + yp(kA, y) * dhx;
+      if (pye) {
+        const T m = pb * mhxz[k + 1] + pa * dhx;
         mhxz[k + 1] = m;
-        dhx = dhx / yp(kK, y) + m;
+        dhx = dhx / pk + m;
       }
       const T curl = dhz - dhx;
       if constexpr (FMT != 0) {
@@ -675,21 +687,29 @@ __device__ __forceinline__ void adj2_tile(const Adj2Args<T> &a, int s, int xt, i
     m = mn;
     return xa * mn + q / xk;
   };
+  // nraw_y and raw_z read the row's y profile before any lane leaves (the
+  // cross-lane reads need every lane active); call them with all lanes active.
   auto nraw_y = [&](int y, T lm, T cbx, T &m) -> T {  // negated adjoint of dHx/dz
+    const bool pye = GY && (y < g.pml_y0 || y >= g.pml_y1);
+    T pa, pb, pk;
+    yp.row3(kA, kB, kK, y, pye, pa, pb, pk);
     if ((GY && (y < 1 || y >= ny)) || !e_col) return (T)0;
     const T q = cbx * lm;
-    if (!(GY && (y < g.pml_y0 || y >= g.pml_y1))) return q;
-    const T mn = yp(kB, y) * m - q;
+    if (!pye) return q;
+    const T mn = pb * m - q;
     m = mn;
-    return q / yp(kK, y) - yp(kA, y) * mn;
+    return q / pk - pa * mn;
   };
   auto raw_z = [&](int y, T lhxv, T cqx, T &m) -> T {  // adjoint of dEy/dz (Hx update)
+    const bool pyh = GY && (y < g.pml_y0 || y >= g.pml_y1h);
+    T pa, pb, pk;
+    yp.row3(kAh, kBh, kKh, y, pyh, pa, pb, pk);
     if ((GY && (y < 1 || y >= ny - 1)) || !e_col) return (T)0;
     const T adj = -cqx * lhxv;
-    if (!(GY && (y < g.pml_y0 || y >= g.pml_y1h))) return adj;
-    const T mn = yp(kBh, y) * m + adj;
+    if (!pyh) return adj;
+    const T mn = pb * m + adj;
     m = mn;
-    return yp(kAh, y) * mn + adj / yp(kKh, y);
+    return pa * mn + adj / pk;
   };
   auto raw_xh = [&](int y, T lhzv, T cqx, T &m) -> T {  // adjoint of dEy/dx (Hz update)
     if ((GY && (y < 1 || y >= ny)) || (GX && (x < 1 || x >= nx - 1))) return (T)0;

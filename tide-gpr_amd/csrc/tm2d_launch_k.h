@@ -53,7 +53,13 @@ void adjk_parts(int64_t groups, dim3 block, hipStream_t stream, const AdjKArgs<T
   const size_t lds = 2 * (size_t)block.y * TY * 64 * sizeof(double);
   const int64_t tiles = split_tiles<TY>(a.split, a.pts.n_xt, a.g.ny, kInnerTiles) +
                         split_tiles<TYB>(a.split, a.pts.n_xt, a.g.ny, kBorderTiles);
-  adjk_step<T, TY, K, FMT, kOrderedTiles, TYB><<<(unsigned)(tiles * groups), block, lds, stream>>>(a);
+  const unsigned grid = (unsigned)(tiles * groups);
+  if constexpr (sizeof(T) == 4) {
+    const int waves = block.x * block.y <= 256 ? g_tuning.adj_waves : 0;
+    if (waves == 4)
+      return adjk_step_w<T, TY, K, FMT, kOrderedTiles, TYB, 4><<<grid, block, lds, stream>>>(a);
+  }
+  adjk_step<T, TY, K, FMT, kOrderedTiles, TYB><<<grid, block, lds, stream>>>(a);
 }
 
 template <typename T, int FMT>

@@ -43,9 +43,12 @@ struct ServeItem {
 };
 
 // n: jobs active at this step (a prefix of the tick's job list); the
-// workgroups cover flat tiles [first, first + gridDim.x).  pair_mode: the
-// tick runs two-step launches, so the state parity comes from par0 / epar0
-// and the pairs done before tick step j, else from the step index.
+// workgroups cover flat tiles [first, first + gridDim.x).  State parity:
+// pair_mode 1, the single step after j / 2 two-step launches of this tick
+// (each flips the state and the E-phase memory copy once); pair_mode 2, the
+// j-th of consecutive single steps in a two-step engine (each flips the
+// state, none the memory copy); pair_mode 0, a one-step engine, where it
+// follows the step index.
 template <int TYF, int TYA>
 __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ items,
                                                   const int32_t *__restrict__ start, int n, int j,
@@ -64,7 +67,8 @@ __global__ __launch_bounds__(512) void serve_step(const ServeItem *__restrict__ 
   const int k = it.k0 + j;
   const int local = (int)(flat - start[lo]);
   const int flip = (j >> 1) & 1;
-  const int par = pair_mode ? it.par0 ^ flip : (k & 1), epar = pair_mode ? it.epar0 ^ flip : 0;
+  const int par = pair_mode == 1 ? it.par0 ^ flip : pair_mode == 2 ? it.par0 ^ (j & 1) : (k & 1);
+  const int epar = pair_mode == 1 ? it.epar0 ^ flip : pair_mode == 2 ? it.epar0 : 0;
   if (it.b.n_shots == 1 && blockDim.y > 1) {
     // A single-shot job gives each wavefront of the workgroup its own tile
     // (no idle wavefronts, no cross-wavefront gradient sum).
@@ -253,6 +257,7 @@ struct EJob {
   int64_t k = 0;
   int64_t nt = 0;
   int n_groups = 1;
+  int64_t weight = 1;  // step multiplier (tide_tm2d_engine_set_weight)
 };
 
 struct Slot {
@@ -521,8 +526,10 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
   // Steps per job this tick: the largest running job (cells x shots per
   // step) takes max_steps, a smaller one proportionally more, up to
   // e.boost times: in lock-step launches a small job would otherwise move
-  // at the large one's pace.  Allotments are multiples of max_steps, so
-  // every non-final tick of a job is still an even number of steps.
+  // at the large one's pace.  The host's per-job weight multiplies the
+  // result (a priority: the server gives its oldest request the most).
+  // Allotments are multiples of max_steps, so every non-final tick of a job
+  // is still an even number of steps.
   auto cells = [](const EJob &j) {
     return (double)j.spec.p.n_shots * (double)j.spec.p.ny * (double)j.spec.p.nx;
   };
@@ -543,7 +550,7 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
     }
     const double ratio = biggest / std::max(cells(j), 1.0);
     const int64_t boost = std::max<int64_t>(1, std::min<int64_t>(e.boost, (int64_t)ratio));
-    run.emplace_back(std::min<int64_t>(left, max_steps * boost), &j);
+    run.emplace_back(std::min<int64_t>(left, max_steps * boost * j.weight), &j);
   }
   if (run.empty()) {
     *n_running = 0;
@@ -640,7 +647,7 @@ int engine_tick(Engine &e, int64_t max_steps, hipStream_t s, int64_t *n_running)
     int m = n;
     for (int64_t jstep = 0; jstep < steps; ++jstep) {
       while (m > 0 && run[(size_t)m - 1].first <= jstep) --m;
-      launch_step(e, start[m], m, (int)jstep, s, 0, e.two_step ? 1 : 0);
+      launch_step(e, start[m], m, (int)jstep, s, 0, e.two_step ? 2 : 0);
     }
   }
   TIDE_HIP_TRY(hipGetLastError());
@@ -727,6 +734,21 @@ extern "C" int tide_tm2d_engine_set_boost(void *engine, int64_t boost) {
     return kInvalidArgument;
   }
   e->boost = boost;
+  return kOk;
+}
+
+extern "C" int tide_tm2d_engine_set_weight(void *engine, int64_t job, int64_t weight) {
+  Engine *e = as_engine(engine);
+  if (e == nullptr || weight < 1 || weight > 64) {
+    set_error("engine weight: 1..64 (step multiplier of one job)");
+    return kInvalidArgument;
+  }
+  auto it = e->jobs.find(job);
+  if (it == e->jobs.end()) {
+    set_error("engine weight: unknown job");
+    return kInvalidArgument;
+  }
+  it->second.weight = weight;
   return kOk;
 }
 

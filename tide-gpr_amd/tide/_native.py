@@ -315,6 +315,10 @@ def load() -> ctypes.CDLL:
     lib.tide_tm2d_take_loop_timing.argtypes = [
         ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
         ctypes.POINTER(ctypes.c_int64)]
+    lib.tide_tm2d_set_adjoint_waves.restype = ctypes.c_int
+    lib.tide_tm2d_set_adjoint_waves.argtypes = [ctypes.c_int]
+    lib.tide_tm2d_get_adjoint_waves.restype = ctypes.c_int
+    lib.tide_tm2d_get_adjoint_waves.argtypes = [ctypes.POINTER(ctypes.c_int)]
     lib.tide_tm2d_set_border_rows.restype = ctypes.c_int
     lib.tide_tm2d_set_border_rows.argtypes = [ctypes.c_int, ctypes.c_int]
     lib.tide_tm2d_get_border_rows.restype = ctypes.c_int
@@ -398,6 +402,18 @@ def set_em3d_fused(rows: int, planes: int = 8) -> None:
     """One-launch 3-D forward step: rows per wavefront (2 or 4; 0 runs the
     two-kernel step) and z planes per march (8 or 16)."""
     check(load().tide_em3d_set_fused(int(rows), int(planes)))
+
+
+def set_tm2d_adjoint_waves(waves: int) -> None:
+    """Register budget of the fp32 K-step adjoint in waves per SIMD: 4 (128
+    VGPRs, the default) or 0 (the compiler's count)."""
+    check(load().tide_tm2d_set_adjoint_waves(int(waves)))
+
+
+def get_tm2d_adjoint_waves() -> int:
+    out = ctypes.c_int()
+    check(load().tide_tm2d_get_adjoint_waves(ctypes.byref(out)))
+    return int(out.value)
 
 
 def set_em3d_tile_order(shot_fast: int) -> None:
@@ -500,6 +516,7 @@ __all__ = [
     "check",
     "dtype_name",
     "get_em3d_fused",
+    "get_tm2d_adjoint_waves",
     "get_tm2d_border_rows",
     "get_tm2d_shot_loop",
     "get_tm2d_tile_split",
@@ -514,6 +531,7 @@ __all__ = [
     "set_tm2d_shot_loop",
     "set_tm2d_tile_split",
     "set_em3d_fused",
+    "set_tm2d_adjoint_waves",
     "set_em3d_tile_order",
     "set_tm2d_time_block",
     "set_tm2d_tuning",

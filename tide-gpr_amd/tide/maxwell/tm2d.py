@@ -251,20 +251,9 @@ def prepare_tm2d(
 
     if max_vel is None:  # a caller's max_vel also sets the profiles (reference tm2d_cuda.py:124-130)
         max_vel = float((1.0 / torch.sqrt(epsilon.detach() * mu.detach())).max().item()) * C0
-    prof = staggered.set_pml_profiles(
-        pml_width=widths,
-        accuracy=stencil,
-        fd_pad=fd_pad,
-        dt=dt,
-        grid_spacing=[dy, dx],
-        max_vel=max_vel,
-        dtype=dtype,
-        device=device,
-        pml_freq=0.5 / dt,
-        ny=ny,
-        nx=nx,
-    )
-    ay, ayh, ax, axh, by, byh, bx, bxh, ky, kyh, kx, kxh = [p.reshape(-1).contiguous() for p in prof]
+    ay, ayh, ax, axh, by, byh, bx, bxh, ky, kyh, kx, kxh = _profiles(
+        tuple(widths), int(stencil), tuple(fd_pad), float(dt), (float(dy), float(dx)),
+        float(max_vel), dtype, device, ny, nx)
     pml = (total[0], total[2], ny - fd_pad[1] - widths[1], nx - fd_pad[3] - widths[3])
     return TM2DSetup(
         ca=ca, cb=cb, cq=cq, f=f,
@@ -280,6 +269,32 @@ def prepare_tm2d(
                "models": {"epsilon": eps_p.detach(), "sigma": sig_p.detach(),
                           "mu": mu_p.detach()}},
     )
+
+
+_PROFILE_CACHE: dict[tuple, list[torch.Tensor]] = {}
+_PROFILE_CACHE_SIZE = 32
+
+
+def _profiles(widths, stencil, fd_pad, dt, spacing, max_vel, dtype, device, ny, nx):
+    """The 12 CPML profiles (flat, contiguous) of one geometry.  They are a
+    pure function of these arguments and never written, so repeated calls
+    on the same geometry share one set (about 50 small device operations
+    fewer per call; a launch-bound small problem spends most of its host
+    set-up here).  At most 32 geometries are kept, least recently used
+    first out."""
+    key = (widths, stencil, fd_pad, dt, spacing, max_vel, dtype, str(device), ny, nx)
+    hit = _PROFILE_CACHE.pop(key, None)
+    if hit is None:
+        prof = staggered.set_pml_profiles(
+            pml_width=list(widths), accuracy=stencil, fd_pad=list(fd_pad), dt=dt,
+            grid_spacing=list(spacing), max_vel=max_vel, dtype=dtype, device=device,
+            pml_freq=0.5 / dt, ny=ny, nx=nx,
+        )
+        hit = [p.reshape(-1).contiguous() for p in prof]
+        while len(_PROFILE_CACHE) >= _PROFILE_CACHE_SIZE:
+            _PROFILE_CACHE.pop(next(iter(_PROFILE_CACHE)))
+    _PROFILE_CACHE[key] = hit
+    return hit
 
 
 def _pad_debye(dispersion, model_shape, total, padded, device, dtype):
