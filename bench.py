@@ -729,14 +729,19 @@ def main() -> None:
     b_snap = 2 * snap_bytes
     b_alg = {"forward_store": 24 + b_snap, "adjoint": 24 + b_snap, "forward": 24}
     kernels = {}
+    # the fp32 K-step adjoint held to four waves per SIMD (this workload is
+    # fp32) is its own kernel
+    adj_w4 = _native.get_tm2d_adjoint_waves() == 4 and 64 * _native.get_tm2d_tuning()[3] <= 256
     for name, d in summ.items():
         spl = int(d["steps_per_launch"])
         k_ms, k_launches, _ = loop_ms.get(name, (d["ms"], d["launches"], 0))
         achieved = b_alg[name] * d["cell_updates"] / (k_ms * 1e-3) / 1e9
         stem = "adj" if name == "adjoint" else "fwd"
+        kname = f"{stem}_step" if spl == 1 else (f"{stem}2_step" if spl == 2 else f"{stem}k_step")
+        if stem == "adj" and spl >= 3 and adj_w4:
+            kname = "adjk_step_w4"
         kernels[name] = {
-            "kernel": f"{stem}_step" if spl == 1 else (f"{stem}2_step" if spl == 2
-                                                      else f"{stem}k_step"),
+            "kernel": kname,
             "steps_per_launch": spl,
             "launches": k_launches,
             "avg_launch_us": round(k_ms / max(k_launches, 1) * 1e3, 2),
