@@ -1384,6 +1384,35 @@ int tide_tm2d_engine_destroy(void *engine);
  */
 
 /*
+ * 3-D Born forward with full-size CPML memories and pointer arrays (the
+ * layout of tide_em3d_*_forward): fields / dfields are the 18 background /
+ * scattered arrays, advanced in place; otherwise as
+ * maxwell_3d_2_*_born_forward_cuda.  No reference counterpart (the host
+ * side of Maxwell3D.linearize().jvp calls it).
+ */
+int tide_em3d_float_born_forward(
+    float const *ca, float const *cb, float const *cq, float const *dca,
+    float const *dcb, float const *f0, float const *df, float *const *fields,
+    float *const *dfields, float *r, float *background_r,
+    float const *const *prof, int64_t const *sources_i,
+    int64_t const *receivers_i, float rdz, float rdy, float rdx, int64_t nt,
+    int64_t n_shots, int64_t nz, int64_t ny, int64_t nx, int64_t n_src,
+    int64_t n_rec, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,
+    int64_t const *pml, int64_t source_component, int64_t receiver_component,
+    int64_t device, void *compute_stream);
+int tide_em3d_double_born_forward(
+    double const *ca, double const *cb, double const *cq, double const *dca,
+    double const *dcb, double const *f0, double const *df,
+    double *const *fields, double *const *dfields, double *r,
+    double *background_r, double const *const *prof,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdz,
+    double rdy, double rdx, int64_t nt, int64_t n_shots, int64_t nz,
+    int64_t ny, int64_t nx, int64_t n_src, int64_t n_rec, bool ca_b,
+    bool cb_b, bool cq_b, int64_t start_t, int64_t const *pml,
+    int64_t source_component, int64_t receiver_component, int64_t device,
+    void *compute_stream);
+
+/*
  * Forward propagation over [start_t, start_t + nt), in place.
  * Replaces maxwell_3d_2_{dtype}_forward_cuda and
  * maxwell_3d_2_{dtype}_forward_with_storage_cuda: reference ABI
@@ -1516,6 +1545,68 @@ int maxwell_3d_2_double_forward_cuda(
     int64_t pml_x0, int64_t pml_z1, int64_t pml_y1, int64_t pml_x1,
     int64_t source_component, int64_t receiver_component, int64_t n_threads,
     int64_t device, int64_t execution_backend, void *compute_stream);
+
+/*
+ * Born forward, 3-D (reference src/tide/backend_utils.py:556-577,
+ * _3D_BORN_FORWARD_SPEC; CPU semantics src/tide/csrc/em3d/
+ * maxwell_3d_inst.cpp:1100-1527): the background (ex .. m_ey_x) and the
+ * Born-scattered field (dex .. dm_ey_x) advance together, the scattered E
+ * update adding dca E^n + dcb curl H^n of the background (dca, dcb
+ * [nz, ny, nx]); f0 is injected into the background and df into the
+ * scattered field, background_r records the background and r the scattered
+ * field.  CPML memories in the reference's compact layout, as above; dt,
+ * step_ratio, n_threads and execution_backend are unused (the reference's
+ * CPU version ignores step_ratio too).  Returns a status.
+ */
+int maxwell_3d_2_float_born_forward_cuda(
+    float const *ca, float const *cb, float const *cq, float const *dca,
+    float const *dcb, float const *f0, float const *df, float *ex, float *ey,
+    float *ez, float *hx, float *hy, float *hz, float *m_hz_y, float *m_hy_z,
+    float *m_hx_z, float *m_hz_x, float *m_hy_x, float *m_hx_y, float *m_ey_z,
+    float *m_ez_y, float *m_ez_x, float *m_ex_z, float *m_ex_y, float *m_ey_x,
+    float *dex, float *dey, float *dez, float *dhx, float *dhy, float *dhz,
+    float *dm_hz_y, float *dm_hy_z, float *dm_hx_z, float *dm_hz_x,
+    float *dm_hy_x, float *dm_hx_y, float *dm_ey_z, float *dm_ez_y,
+    float *dm_ez_x, float *dm_ex_z, float *dm_ex_y, float *dm_ey_x, float *r,
+    float *background_r, float const *az, float const *bz, float const *azh,
+    float const *bzh, float const *ay, float const *by, float const *ayh,
+    float const *byh, float const *ax, float const *bx, float const *axh,
+    float const *bxh, float const *kz, float const *kzh, float const *ky,
+    float const *kyh, float const *kx, float const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, float rdz, float rdy,
+    float rdx, float dt, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny,
+    int64_t nx, int64_t n_sources_per_shot, int64_t n_receivers_per_shot,
+    int64_t step_ratio, bool ca_batched, bool cb_batched, bool cq_batched,
+    int64_t start_t, int64_t pml_z0, int64_t pml_y0, int64_t pml_x0,
+    int64_t pml_z1, int64_t pml_y1, int64_t pml_x1, int64_t source_component,
+    int64_t receiver_component, int64_t n_threads, int64_t device,
+    int64_t execution_backend, void *compute_stream);
+
+int maxwell_3d_2_double_born_forward_cuda(
+    double const *ca, double const *cb, double const *cq, double const *dca,
+    double const *dcb, double const *f0, double const *df, double *ex,
+    double *ey, double *ez, double *hx, double *hy, double *hz, double *m_hz_y,
+    double *m_hy_z, double *m_hx_z, double *m_hz_x, double *m_hy_x,
+    double *m_hx_y, double *m_ey_z, double *m_ez_y, double *m_ez_x,
+    double *m_ex_z, double *m_ex_y, double *m_ey_x, double *dex, double *dey,
+    double *dez, double *dhx, double *dhy, double *dhz, double *dm_hz_y,
+    double *dm_hy_z, double *dm_hx_z, double *dm_hz_x, double *dm_hy_x,
+    double *dm_hx_y, double *dm_ey_z, double *dm_ez_y, double *dm_ez_x,
+    double *dm_ex_z, double *dm_ex_y, double *dm_ey_x, double *r,
+    double *background_r, double const *az, double const *bz,
+    double const *azh, double const *bzh, double const *ay, double const *by,
+    double const *ayh, double const *byh, double const *ax, double const *bx,
+    double const *axh, double const *bxh, double const *kz, double const *kzh,
+    double const *ky, double const *kyh, double const *kx, double const *kxh,
+    int64_t const *sources_i, int64_t const *receivers_i, double rdz,
+    double rdy, double rdx, double dt, int64_t nt, int64_t n_shots, int64_t nz,
+    int64_t ny, int64_t nx, int64_t n_sources_per_shot,
+    int64_t n_receivers_per_shot, int64_t step_ratio, bool ca_batched,
+    bool cb_batched, bool cq_batched, int64_t start_t, int64_t pml_z0,
+    int64_t pml_y0, int64_t pml_x0, int64_t pml_z1, int64_t pml_y1,
+    int64_t pml_x1, int64_t source_component, int64_t receiver_component,
+    int64_t n_threads, int64_t device, int64_t execution_backend,
+    void *compute_stream);
 
 int maxwell_3d_2_float_forward_with_storage_cuda(
     float const *ca, float const *cb, float const *cq, float const *f,

@@ -14,7 +14,11 @@
 //     H phase, ping-pong CPML adjoint memories, index-shifted reverse loop),
 //     accumulating dJ/dca and dJ/dcb in FP64;
 //   * the tangent-linear forward in (ca, cb, f), used only to pin the
-//     adjoint with the dot-product identity <J dm, w> = <dm, J^T w>.
+//     adjoint with the dot-product identity <J dm, w> = <dm, J^T w>;
+//   * the Born forward (background and scattered field in one time loop),
+//     in the reference's operation order: csrc/em3d/maxwell_3d_inst.cpp:
+//     1100-1527 (born_forward), pinned bit for bit to the reference's own
+//     CPU build.
 //
 // Followed reference code (paths relative to the reference's src/tide/):
 //   forward loops   csrc/em3d/maxwell_3d_inst.cpp:446-761 (forward),
@@ -133,9 +137,14 @@ void h_step(const Grid3<T> &g, const T *cq, T *const *fd, int64_t s) {
 }
 
 // Stores (when non-null): st[0..2] pre-update Ex/Ey/Ez, st[3..5] curl x/y/z.
+// With `bg` (the Born scattered field): bg[0..2] the background's
+// pre-update E, bg[3..5] its curl H of this step, indexed like the fields,
+// and the update E = ca E + cb curl + dca E_bg + dcb curl_bg in that order
+// (reference maxwell_3d_inst.cpp:1444-1456, dca/dcb [nz, ny, nx]).
 template <typename T>
 void e_step(const Grid3<T> &g, const T *ca, const T *cb, T *const *fd,
-            void *const *st, int fmt, int64_t s) {
+            void *const *st, int fmt, int64_t s, const T *dca = nullptr,
+            const T *dcb = nullptr, const T *const *bg = nullptr) {
   const int64_t base = s * g.numel();
 #pragma omp parallel for schedule(static)
   for (int64_t z = 1; z < g.n[0]; ++z)
@@ -172,7 +181,13 @@ void e_step(const Grid3<T> &g, const T *ca, const T *cb, T *const *fd,
             else ((T *)st[3 + e])[i] = curl[e];
           }
         }
-        for (int e = 0; e < 3; ++e) fd[EX + e][i] = cav * fd[EX + e][i] + cbv * curl[e];
+        if (bg) {
+          for (int e = 0; e < 3; ++e)
+            fd[EX + e][i] = cav * fd[EX + e][i] + cbv * curl[e] + dca[j] * bg[e][i] +
+                            dcb[j] * bg[3 + e][i];
+        } else {
+          for (int e = 0; e < 3; ++e) fd[EX + e][i] = cav * fd[EX + e][i] + cbv * curl[e];
+        }
       }
 }
 
@@ -430,6 +445,49 @@ void tangent(const Grid3<T> &g, const T *ca, const T *cb, const T *cq, const T *
   }
 }
 
+// ---------------------------------------------------- Born forward (3-D)
+// Reference maxwell_3d_inst.cpp:1100-1527 (born_forward): per step the
+// background and scattered H updates, then per cell the scattered E update
+// ca dE + cb dcurl + dca E^n + dcb curl^n (E^n, curl^n the background's
+// pre-update E and curl H of this step) and the background E update; then
+// f0 into the background and df into the scattered field at the sources;
+// r records the scattered field, background_r the background.
+template <typename T>
+void born_forward(const Grid3<T> &g, const T *ca, const T *cb, const T *cq, const T *dca,
+                  const T *dcb, const T *f0, const T *df, T *const *fd, T *const *dfd, T *r,
+                  T *bg_r, const int64_t *src, const int64_t *rec, int64_t n_src, int64_t n_rec,
+                  int src_comp, int rec_comp, int64_t nt, int64_t start_t) {
+  const int64_t numel = g.numel(), total = g.n_shots * numel;
+  std::vector<T> snap((size_t)6 * total);
+  void *st[6];
+  const T *bg[6];
+  for (int k = 0; k < 6; ++k) {
+    st[k] = snap.data() + (size_t)k * total;
+    bg[k] = snap.data() + (size_t)k * total;
+  }
+  for (int64_t t = start_t; t < start_t + nt; ++t) {
+    for (int64_t s = 0; s < g.n_shots; ++s) {
+      h_step<T>(g, cq, fd, s);
+      h_step<T>(g, cq, dfd, s);
+      e_step<T>(g, ca, cb, fd, st, 0, s);
+      e_step<T>(g, ca, cb, dfd, nullptr, 0, s, dca, dcb, bg);
+    }
+    for (int64_t s = 0; s < g.n_shots; ++s) {
+      const int64_t base = s * numel;
+      for (int64_t q = s * n_src; q < (s + 1) * n_src; ++q)
+        if (src[q] >= 0) {
+          if (f0) fd[EX + src_comp][base + src[q]] += f0[t * g.n_shots * n_src + q];
+          if (df) dfd[EX + src_comp][base + src[q]] += df[t * g.n_shots * n_src + q];
+        }
+      for (int64_t q = s * n_rec; q < (s + 1) * n_rec; ++q) {
+        if (r) r[t * g.n_shots * n_rec + q] = rec[q] >= 0 ? dfd[EX + rec_comp][base + rec[q]] : (T)0;
+        if (bg_r)
+          bg_r[t * g.n_shots * n_rec + q] = rec[q] >= 0 ? fd[EX + rec_comp][base + rec[q]] : (T)0;
+      }
+    }
+  }
+}
+
 // prof: az bz azh bzh ay by ayh byh ax bx axh bxh kz kzh ky kyh kx kxh
 // (the reference's argument order); pml: z0 y0 x0 z1 y1 x1.
 template <typename T>
@@ -498,6 +556,22 @@ EM3D_FORWARD(oracle_em3d_forward_f64, double)
   }
 EM3D_BACKWARD(oracle_em3d_backward_f32, float)
 EM3D_BACKWARD(oracle_em3d_backward_f64, double)
+
+#define EM3D_BORN(NAME, T)                                                                  \
+  int NAME(const T *ca, const T *cb, const T *cq, const T *dca, const T *dcb, const T *f0,  \
+           const T *df, T *const *fields, T *const *dfields, T *r, T *bg_r,                 \
+           const T *const *prof, const int64_t *src, const int64_t *rec, T rdz, T rdy,      \
+           T rdx, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,          \
+           int64_t n_src, int64_t n_rec, int64_t start_t, const int64_t *pml,               \
+           const bool *batched, int src_comp, int rec_comp) {                               \
+    if (src_comp < 0 || src_comp > 2 || rec_comp < 0 || rec_comp > 2) return 1;            \
+    const Grid3<T> g = make_grid<T>(prof, rdz, rdy, rdx, n_shots, nz, ny, nx, pml, batched); \
+    born_forward<T>(g, ca, cb, cq, dca, dcb, f0, df, fields, dfields, r, bg_r, src, rec,    \
+                    n_src, n_rec, src_comp, rec_comp, nt, start_t);                         \
+    return 0;                                                                               \
+  }
+EM3D_BORN(oracle_em3d_born_forward_f32, float)
+EM3D_BORN(oracle_em3d_born_forward_f64, double)
 
 int oracle_em3d_tangent_f64(const double *ca, const double *cb, const double *cq,
                             const double *f, const double *dca, const double *dcb,

@@ -634,3 +634,68 @@ def ref_em3d_forward(case: EMCase, *, n_threads: int = 0, nt: int | None = None)
        case.src.shape[1], n_rec, 1, False, b, b, b, 0, *case.pml, case.src_comp,
        case.rec_comp, n_threads, 0, 0, None)
     return fields, r
+
+
+def em3d_born_forward(case: EMCase, dca: np.ndarray, dcb: np.ndarray, df: np.ndarray | None = None,
+                      dfields: list | None = None):
+    """Oracle 3-D Born forward (reference ``maxwell_3d_inst.cpp:1100-1527``):
+    background and scattered field in one time loop.  ``dca`` / ``dcb``
+    ``[nz, ny, nx]``; ``df`` the scattered source ``[nt, N, n_src]`` (None:
+    zero); ``dfields`` the 18 initial scattered arrays (None: zero).
+    Returns ``(fields, dfields, r, background_r)`` with ``r`` the scattered
+    receivers."""
+    lib = oracle_lib()
+    f64, ct, batched = _em_common(case)
+    c = lambda a: np.ascontiguousarray(a, dtype=case.dtype)  # noqa: E731
+    fields = [c(x).copy() for x in case.fields]
+    dfd = [np.zeros_like(x) for x in fields] if dfields is None else [c(x).copy() for x in dfields]
+    n_src, n_rec = case.src.shape[1], case.rec.shape[1]
+    df = np.zeros((case.nt, case.n_shots, n_src), case.dtype) if df is None else c(df)
+    r = np.zeros((case.nt, case.n_shots, n_rec), case.dtype)
+    bg_r = np.zeros_like(r)
+    nz, ny, nx = case.shape
+    fn = lib.oracle_em3d_born_forward_f64 if f64 else lib.oracle_em3d_born_forward_f32
+    fn.restype = ctypes.c_int
+    rc = fn(_p(case.ca), _p(case.cb), _p(case.cq), _p(c(dca)), _p(c(dcb)), _p(case.f), _p(df),
+            _parr(fields), _parr(dfd), _p(r), _p(bg_r), _parr(case.profiles), _p(case.src),
+            _p(case.rec), ct(case.rdz), ct(case.rdy), ct(case.rdx), ctypes.c_int64(case.nt),
+            ctypes.c_int64(case.n_shots), ctypes.c_int64(nz), ctypes.c_int64(ny),
+            ctypes.c_int64(nx), ctypes.c_int64(n_src), ctypes.c_int64(n_rec), ctypes.c_int64(0),
+            (ctypes.c_int64 * 6)(*case.pml), batched, ctypes.c_int(case.src_comp),
+            ctypes.c_int(case.rec_comp))
+    if rc != 0:
+        raise RuntimeError(f"oracle 3-D Born forward failed ({rc})")
+    return fields, dfd, r, bg_r
+
+
+def ref_em3d_born_forward(case: EMCase, dca: np.ndarray, dcb: np.ndarray,
+                          df: np.ndarray | None = None, *, n_threads: int = 0):
+    """The reference's own CPU 3-D Born forward
+    (``maxwell_3d_2_<dtype>_born_forward_cpu``, argument order of the
+    reference ``backend_utils.py:556-577``).  Returns ``(fields, dfields, r,
+    background_r)``."""
+    lib = ref_lib()
+    dt_ = case.dtype
+    name = "double" if dt_ == np.float64 else "float"
+    fn = getattr(lib, f"maxwell_3d_2_{name}_born_forward_cpu")
+    ct = ctypes.c_double if dt_ == np.float64 else ctypes.c_float
+    I, P, B = ctypes.c_int64, ctypes.c_void_p, ctypes.c_bool
+    fn.restype = None
+    fn.argtypes = ([P] * 7 + [P] * 18 + [P] * 18 + [P] * 2 + [P] * 18 + [P] * 2 + [ct] * 4
+                   + [I] * 8 + [B] * 3 + [I] + [I] * 6 + [I] * 2 + [I] * 3 + [P])
+    c = lambda a: np.ascontiguousarray(a, dtype=dt_)  # noqa: E731
+    fields = [c(x).copy() for x in case.fields]
+    dfd = [np.zeros_like(x) for x in fields]
+    n_src, n_rec = case.src.shape[1], case.rec.shape[1]
+    df = np.zeros((case.nt, case.n_shots, n_src), dt_) if df is None else c(df)
+    r = np.zeros((case.nt, case.n_shots, n_rec), dt_)
+    bg_r = np.zeros_like(r)
+    b = case.ca.shape[0] != 1
+    nz, ny, nx = case.shape
+    fn(_p(case.ca), _p(case.cb), _p(case.cq), _p(c(dca)), _p(c(dcb)), _p(case.f), _p(df),
+       *[_p(x) for x in fields], *[_p(x) for x in dfd], _p(r), _p(bg_r),
+       *[_p(p) for p in case.profiles], _p(case.src), _p(case.rec),
+       case.rdz, case.rdy, case.rdx, case.dt, case.nt, case.n_shots, nz, ny, nx,
+       n_src, n_rec, 1, b, b, b, 0, *case.pml, case.src_comp, case.rec_comp, n_threads, 0, 0,
+       None)
+    return fields, dfd, r, bg_r

@@ -31,6 +31,7 @@ TEMPLATES = {
     "maxwell_3d_forward": "_3D_FORWARD_SPEC",
     "maxwell_3d_forward_with_storage": "_3D_FORWARD_WITH_STORAGE_SPEC",
     "maxwell_3d_backward": "_3D_BACKWARD_SPEC",
+    "maxwell_3d_born_forward": "_3D_BORN_FORWARD_SPEC",
 }
 
 

@@ -347,3 +347,47 @@ def gpu_em3d_ref_backward(case, grad_r, stores, *, store: str = "full", device="
     _native.check(st)
     torch.cuda.synchronize(dev)
     return tuple(t.cpu().numpy() for t in (gca, gcb, geps, gsig, gf))
+
+
+def gpu_em3d_ref_born_forward(case, dca, dcb, df, *, device="cuda:0"):
+    """``maxwell_3d_2_<dtype>_born_forward_cuda`` with the reference's
+    arguments (compact CPML memories for both fields).  Returns numpy
+    ``(fields, dfields (memories expanded to full size), r, background_r)``."""
+    dev = torch.device(device)
+    dtype = torch.float64 if case.dtype == np.float64 else torch.float32
+    ca, cb, cq, f = (_dev(a, dev) for a in (case.ca, case.cb, case.cq, case.f))
+    d_ca, d_cb, d_f = (_dev(np.ascontiguousarray(a, case.dtype), dev) for a in (dca, dcb, df))
+    prof = [_dev(p, dev) for p in case.profiles]
+    N = case.n_shots
+    nz, ny, nx = case.shape
+    n_src, n_rec = case.src.shape[1], case.rec.shape[1]
+
+    def state(arrays):
+        out = [_dev(x, dev) for x in arrays[:6]]
+        out += [_dev(compact_memory(x, a, case.pml), dev) for x, a in zip(arrays[6:], _MEM_AXIS_3D)]
+        return out
+
+    flds = state(case.fields)
+    dflds = state([np.zeros_like(x) for x in case.fields])
+    r = torch.zeros(case.nt, N, n_rec, device=dev, dtype=dtype)
+    bg_r = torch.zeros_like(r)
+    src = torch.from_numpy(np.ascontiguousarray(case.src, np.int64)).to(dev)
+    rec = torch.from_numpy(np.ascontiguousarray(case.rec, np.int64)).to(dev)
+    b = case.ca.shape[0] != 1
+    P = _native.ptr
+    fn = getattr(_native.load(), f"maxwell_3d_2_{_native.dtype_name(dtype)}_born_forward_cuda")
+    st = fn(P(ca), P(cb), P(cq), P(d_ca), P(d_cb), P(f), P(d_f), *[P(x) for x in flds],
+            *[P(x) for x in dflds], P(r), P(bg_r), *[P(p) for p in prof], P(src), P(rec),
+            case.rdz, case.rdy, case.rdx, case.dt, case.nt, N, nz, ny, nx, n_src, n_rec, 1,
+            b, b, b, 0, *case.pml, case.src_comp, case.rec_comp, 0, dev.index or 0, 0,
+            _native.stream_handle(dev))
+    _native.check(st)
+    torch.cuda.synchronize(dev)
+    shape = (N, nz, ny, nx)
+
+    def back(arrays):
+        out = [x.cpu().numpy() for x in arrays[:6]]
+        return out + [full_memory(x.cpu().numpy(), a, case.pml, shape)
+                      for x, a in zip(arrays[6:], _MEM_AXIS_3D)]
+
+    return back(flds), back(dflds), r.cpu().numpy(), bg_r.cpu().numpy()

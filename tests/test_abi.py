@@ -37,6 +37,7 @@ REF_ENTRY = [(prop, d, p) for prop in ("tm", "3d") for d in ("float", "double")
              for p in ("forward", "forward_with_storage", "backward")]
 REF_ENTRY += [("tm", d, p) for d in ("float", "double")
               for p in ("born_forward", "born_forward_with_storage", "born_backward_bggrad")]
+REF_ENTRY += [("3d", d, "born_forward") for d in ("float", "double")]
 
 
 def _prototype(name: str) -> list[tuple[str, str]]:

@@ -75,6 +75,11 @@ template <typename T> struct Args3 {
   int32_t n_shots, n_zb, src_comp, rec_comp;
   int64_t group_stride;  // shot group g adds into gca/gcb + g*group_stride
   int32_t shot_fast;     // tile order (decode3)
+  // Born scattered E update (e3_step<.., BORN>): dca, dcb [nz, ny, nx] and
+  // the background's pre-update E x/y/z and curl H x/y/z of this step,
+  // indexed like the fields
+  const T *dca, *dcb;
+  const T *bg[6];
 };
 
 // Snapshots are written once and read once, much later: non-temporal stores
@@ -263,7 +268,10 @@ __global__ __launch_bounds__(512) void h3_step(Args3<T> a) {
 }
 
 // ----------------------------------------------------------- forward: E
-template <typename T, int TZ, int FMT>
+// BORN: the scattered field of the Born forward, updated as
+// ca E + cb curl + dca E_bg + dcb curl_bg in that order (reference
+// maxwell_3d_inst.cpp:1444-1456).
+template <typename T, int TZ, int FMT, bool BORN = false>
 __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
   const Geom3<T> &g = a.g;
   const int G = (int)blockDim.y;
@@ -361,9 +369,17 @@ __global__ __launch_bounds__(512) void e3_step(Args3<T> a) {
           put_store<T, FMT>(a.st[5], i, cz);
         }
       }
-      e0 = cav * e0 + cbv * cx;
-      e1 = cav * e1 + cbv * cy;
-      e2 = cav * e2 + cbv * cz;
+      if constexpr (BORN) {
+        const int64_t i = base + j;
+        const T dcav = a.dca[j], dcbv = a.dcb[j];
+        e0 = cav * e0 + cbv * cx + dcav * a.bg[0][i] + dcbv * a.bg[3][i];
+        e1 = cav * e1 + cbv * cy + dcav * a.bg[1][i] + dcbv * a.bg[4][i];
+        e2 = cav * e2 + cbv * cz + dcav * a.bg[2][i] + dcbv * a.bg[5][i];
+      } else {
+        e0 = cav * e0 + cbv * cx;
+        e1 = cav * e1 + cbv * cy;
+        e2 = cav * e2 + cbv * cz;
+      }
       // Sources of this shot at this cell, then receivers.
       const int slot = a.pts.row_slot[z * ny + y];
       if (slot >= 0) {
@@ -1014,6 +1030,8 @@ void fill_common(Args3<T> &a, const Problem3<T> &p, const DevicePointTable &dpt,
   a.step_scale = 1.0;
   for (int k = 0; k < 6; ++k) a.st[k] = nullptr;
   for (int k = 0; k < N_SLOTS; ++k) a.mn[k] = nullptr;
+  a.dca = a.dcb = nullptr;
+  for (int k = 0; k < 6; ++k) a.bg[k] = nullptr;
 }
 
 template <typename T>
@@ -1025,12 +1043,12 @@ void launch_h3(int tz, dim3 grid, dim3 block, hipStream_t st, const Args3<T> &a)
   }
 }
 
-template <typename T, int FMT>
+template <typename T, int FMT, bool BORN = false>
 void launch_e3(int tz, dim3 grid, dim3 block, hipStream_t st, const Args3<T> &a) {
   switch (tz) {
-    case 4: e3_step<T, 4, FMT><<<grid, block, 0, st>>>(a); break;
-    case 16: e3_step<T, 16, FMT><<<grid, block, 0, st>>>(a); break;
-    default: e3_step<T, 8, FMT><<<grid, block, 0, st>>>(a); break;
+    case 4: e3_step<T, 4, FMT, BORN><<<grid, block, 0, st>>>(a); break;
+    case 16: e3_step<T, 16, FMT, BORN><<<grid, block, 0, st>>>(a); break;
+    default: e3_step<T, 8, FMT, BORN><<<grid, block, 0, st>>>(a); break;
   }
 }
 
@@ -1192,6 +1210,70 @@ int forward_impl(Problem3<T> p, T *const *fields, const T *f, T *r, const int64_
     } else {
       launch_e3<T, 1>(tz, grid, block, stream, a);
     }
+  }
+  TIDE_HIP_TRY(hipGetLastError());
+  dpt.release(stream);
+  return kOk;
+}
+
+// Born forward over [start_t, start_t + nt) (reference
+// maxwell_3d_inst.cpp:1100-1527): each step the background H and E kernels
+// (the E kernel keeping the background's pre-update E and curl H in a
+// one-step buffer), then the scattered H kernel and the scattered E kernel
+// with the Born terms; f0 / df enter the background / scattered field, and
+// background_r / r record them.  The scattered field starts from `dfields`.
+template <typename T>
+int born_forward_impl(Problem3<T> p, const T *dca, const T *dcb, T *const *fields,
+                      T *const *dfields, const T *f0, const T *df, T *r, T *bg_r,
+                      const int64_t *src, const int64_t *rec, int64_t nt, int64_t start_t,
+                      hipStream_t stream) {
+  if (nt <= 0) return kOk;
+  if (dca == nullptr || dcb == nullptr) {
+    set_error("Born forward: dca and dcb are required");
+    return kInvalidArgument;
+  }
+  if (p.n_src > 0 && (f0 == nullptr || df == nullptr)) {
+    set_error("Born forward: f0 and df are required when there are sources");
+    return kInvalidArgument;
+  }
+  Scratch scr(stream);
+  DevicePointTable dpt;
+  TIDE_CHECK(prepare(p, src, rec, scr, dpt, stream));
+  const int64_t total = (int64_t)p.n_shots * p.g.numel;
+  const int64_t rec_total = (start_t + nt) * p.n_shots * p.n_rec;
+  T *snap = nullptr;
+  TIDE_CHECK(scr.alloc(&snap, (size_t)(6 * total)));
+  // receivers nobody asked for still have somewhere to go
+  if (p.n_rec > 0 && r == nullptr) TIDE_CHECK(scr.alloc(&r, (size_t)rec_total));
+  if (p.n_rec > 0 && bg_r == nullptr) TIDE_CHECK(scr.alloc(&bg_r, (size_t)rec_total));
+  const int tz = g_tuning3.fwd_z;
+  const int group = std::min<int>(p.n_shots, g_tuning3.fwd_group);
+  Args3<T> a, b;
+  fill_common(a, p, dpt, tz);
+  fill_common(b, p, dpt, tz);
+  for (int k = 0; k < N_SLOTS; ++k) {
+    a.f[k] = fields[k];
+    b.f[k] = dfields[k];
+  }
+  for (int k = 0; k < 6; ++k) {
+    a.st[k] = snap + k * total;
+    b.bg[k] = snap + k * total;
+  }
+  b.dca = dca;
+  b.dcb = dcb;
+  const int64_t n_groups = (p.n_shots + group - 1) / group;
+  const dim3 block(64, (unsigned)group);
+  const dim3 grid((unsigned)((int64_t)dpt.n_xt * p.g.ny * a.n_zb * n_groups));
+  for (int64_t t = start_t; t < start_t + nt; ++t) {
+    const int64_t so = t * p.n_shots * p.n_src, ro = t * p.n_shots * p.n_rec;
+    a.src_amp = p.n_src > 0 ? f0 + so : nullptr;
+    b.src_amp = p.n_src > 0 ? df + so : nullptr;
+    a.rec_out = p.n_rec > 0 ? bg_r + ro : nullptr;
+    b.rec_out = p.n_rec > 0 ? r + ro : nullptr;
+    launch_h3<T>(tz, grid, block, stream, a);
+    launch_e3<T, 1>(tz, grid, block, stream, a);
+    launch_h3<T>(tz, grid, block, stream, b);
+    launch_e3<T, 0, true>(tz, grid, block, stream, b);
   }
   TIDE_HIP_TRY(hipGetLastError());
   dpt.release(stream);
@@ -1420,6 +1502,26 @@ int em3d_backward(const T *ca, const T *cb, const T *cq, const T *grad_r, T *con
 }  // namespace
 
 #define TIDE_EM3D_ENTRY_POINTS(T, NAME)                                                     \
+  extern "C" int tide_em3d_##NAME##_born_forward(                                          \
+      T const *ca, T const *cb, T const *cq, T const *dca, T const *dcb, T const *f0,      \
+      T const *df, T *const *fields, T *const *dfields, T *r, T *background_r,             \
+      T const *const *prof, int64_t const *sources_i, int64_t const *receivers_i, T rdz,   \
+      T rdy, T rdx, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny, int64_t nx,       \
+      int64_t n_src, int64_t n_rec, bool ca_b, bool cb_b, bool cq_b, int64_t start_t,      \
+      int64_t const *pml, int64_t source_component, int64_t receiver_component,            \
+      int64_t device, void *compute_stream) {                                              \
+    TIDE_CHECK(set_device(device));                                                        \
+    if (fields == nullptr || dfields == nullptr || prof == nullptr || pml == nullptr) {    \
+      set_error("fields, scattered fields, profiles and pml bounds are required");         \
+      return kInvalidArgument;                                                             \
+    }                                                                                      \
+    Problem3<T> p = make_problem<T>(ca, cb, cq, prof, rdz, rdy, rdx, n_shots, nz, ny, nx,  \
+                                    n_src, n_rec, ca_b, cb_b, cq_b, pml, source_component, \
+                                    receiver_component);                                   \
+    return born_forward_impl<T>(p, dca, dcb, fields, dfields, f0, df, r, background_r,     \
+                                sources_i, receivers_i, nt, start_t,                       \
+                                (hipStream_t)compute_stream);                              \
+  }                                                                                        \
   extern "C" int tide_em3d_##NAME##_forward(                                               \
       T const *ca, T const *cb, T const *cq, T const *f, T *const *fields, T *r,           \
       T const *const *prof, int64_t const *sources_i, int64_t const *receivers_i, T rdz,   \
@@ -1601,6 +1703,33 @@ int ref3_forward(const T *ca, const T *cb, const T *cq, const T *f, T *const fie
 }
 
 template <typename T>
+int ref3_born_forward(const T *ca, const T *cb, const T *cq, const T *dca, const T *dcb,
+                      const T *f0, const T *df, T *const fields[18], T *const dfields[18], T *r,
+                      T *bg_r, const T *const prof[18], const int64_t *src, const int64_t *rec,
+                      T rdz, T rdy, T rdx, int64_t nt, int64_t n_shots, int64_t nz, int64_t ny,
+                      int64_t nx, int64_t n_src, int64_t n_rec, bool ca_b, bool cb_b, bool cq_b,
+                      int64_t start_t, const int64_t pml[6], int64_t src_comp, int64_t rec_comp,
+                      int64_t device, void *stream) {
+  TIDE_CHECK(set_device(device));
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  FullMemories<T> mem, dmem;
+  TIDE_CHECK(mem.open(fields, n_shots, (int)nz, (int)ny, (int)nx, pml, scr, s));
+  TIDE_CHECK(dmem.open(dfields, n_shots, (int)nz, (int)ny, (int)nx, pml, scr, s));
+  T *work[N_SLOTS], *dwork[N_SLOTS];
+  for (int k = 0; k < N_SLOTS; ++k) {
+    work[k] = k < M_HZ_Y ? fields[k] : mem.full[k];
+    dwork[k] = k < M_HZ_Y ? dfields[k] : dmem.full[k];
+  }
+  Problem3<T> p = make_problem<T>(ca, cb, cq, prof, rdz, rdy, rdx, n_shots, nz, ny, nx, n_src,
+                                  n_rec, ca_b, cb_b, cq_b, pml, src_comp, rec_comp);
+  TIDE_CHECK(born_forward_impl<T>(p, dca, dcb, work, dwork, f0, df, r, bg_r, src, rec, nt,
+                                  start_t, s));
+  TIDE_CHECK(mem.back(fields, s));
+  return dmem.back(dfields, s);
+}
+
+template <typename T>
 int ref3_backward(const T *ca, const T *cb, const T *cq, const T *grad_r, T *const lam[18],
                   const void *const stores[6], T *grad_f, T *grad_ca, T *grad_cb, T *grad_eps,
                   T *grad_sigma, bool zero_grad_on_entry, const T *const prof[18],
@@ -1721,6 +1850,24 @@ int ref3_backward(const T *ca, const T *cb, const T *cq, const T *grad_r, T *con
                            rdy, rdx, nt, n_shots, nz, ny, nx, n_src, n_rec, step_ratio,     \
                            ca_b, cb_b, cq_b, start_t, pml, source_component,                \
                            receiver_component, device, compute_stream, nullptr, 0, 0);      \
+  }                                                                                         \
+  extern "C" int maxwell_3d_2_##NAME##_born_forward_cuda(                                   \
+      T const *ca, T const *cb, T const *cq, T const *dca, T const *dcb, T const *f0,       \
+      T const *df, TIDE_REF3_FIELDS(T, ), TIDE_REF3_FIELDS(T, d), T *r, T *background_r,     \
+      TIDE_REF3_PROFILES(T), TIDE_REF3_TAIL(T), TIDE_REF3_FLAGS, void *compute_stream) {     \
+    (void)dt; (void)step_ratio; (void)n_threads; (void)execution_backend;                   \
+    T *const fields[18] = {ex, ey, ez, hx, hy, hz, m_hz_y, m_hy_z, m_hx_z, m_hz_x,         \
+                           m_hy_x, m_hx_y, m_ey_z, m_ez_y, m_ez_x, m_ex_z, m_ex_y, m_ey_x}; \
+    T *const dfields[18] = {dex, dey, dez, dhx, dhy, dhz, dm_hz_y, dm_hy_z, dm_hx_z,        \
+                            dm_hz_x, dm_hy_x, dm_hx_y, dm_ey_z, dm_ez_y, dm_ez_x, dm_ex_z,  \
+                            dm_ex_y, dm_ey_x};                                              \
+    TIDE_REF3_PROF_ARRAY(T);                                                                   \
+    const int64_t pml[6] = {pml_z0, pml_y0, pml_x0, pml_z1, pml_y1, pml_x1};               \
+    return ref3_born_forward<T>(ca, cb, cq, dca, dcb, f0, df, fields, dfields, r,           \
+                                background_r, prof, sources_i, receivers_i, rdz, rdy, rdx,  \
+                                nt, n_shots, nz, ny, nx, n_src, n_rec, ca_b, cb_b, cq_b,    \
+                                start_t, pml, source_component, receiver_component, device, \
+                                compute_stream);                                            \
   }                                                                                         \
   extern "C" int maxwell_3d_2_##NAME##_forward_with_storage_cuda(                           \
       T const *ca, T const *cb, T const *cq, T const *f, TIDE_REF3_FIELDS(T, ), T *r,          \

@@ -213,6 +213,11 @@ _REF3_KINDS = {
     # ca cb cq grad_r | 18 lambda | scratch | 18 storage | grad_f | grad_ca
     # grad_cb grad_eps grad_sigma | 2 per-shot | zero_grad_on_entry | ...
     "backward": "P4 P18 P1 P18 P1 P4 P2 B1 P18 P2 F4 I8 I3 B2 B3 I1 I6 I2 I1 I1 I1 P2",
+    # ca cb cq dca dcb f0 df | 18 state | 18 scattered state | r background_r |
+    # 18 profiles | src rec | F4 | I8 | 3 batched | start_t | 6 pml |
+    # 2 components | n_threads device execution_backend | stream
+    # (src/tide/backend_utils.py:556-577)
+    "born_forward": "P7 P18 P18 P2 P18 P2 F4 I8 B3 I1 I6 I2 I1 I1 I1 P1",
 }
 
 
@@ -278,6 +283,13 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, f"tide_em3d_{name}_backward_f64acc")
         fn.restype = ctypes.c_int
         fn.argtypes = _em3d_backward_types(ctype)
+        # Born forward: 7 coefficient / source pointers, fields, dfields,
+        # r, background_r, profiles, 2 point arrays | rdz rdy rdx | nt
+        # n_shots nz ny nx n_src n_rec | 3 batched | start_t pml | 2
+        # components device | stream
+        fn = getattr(lib, f"tide_em3d_{name}_born_forward")
+        fn.restype = ctypes.c_int
+        fn.argtypes = _expand_kinds("P14 F3 I7 B3 I1 P1 I3 P1", ctype)
     for name, ctype in (("float", ctypes.c_float), ("double", ctypes.c_double)):
         for pass_name, kinds in _REF3_KINDS.items():
             fn = getattr(lib, f"maxwell_3d_2_{name}_{pass_name}_cuda")

@@ -173,6 +173,30 @@ class LinearizedMaxwellTM(_LinearizedMaxwell):
 
 
 class LinearizedMaxwell3D(_LinearizedMaxwell):
+    def jvp(self, direction: EMDirection) -> TangentResult:
+        """``J(model) direction``: receivers and final state of the 3-D Born
+        scattered field (reference ``linearization.py:178-229`` over
+        ``maxwell3d_born.py``).  ``second_vjp`` is not part of this build
+        for 3-D."""
+        if self._closed:
+            raise RuntimeError("The linearized Maxwell operator is closed.")
+        direction.validate_for(self.model)
+        if direction.mu is not None:
+            raise NotImplementedError("3-D tangent propagation for mu is not available.")
+        from .maxwell3d import born3d
+
+        op, model = self.operator, self.model
+        d, e = op.discretization, op.experiment
+        out = born3d(
+            model.epsilon, model.sigma, model.mu, d.spacing, d.dt, e.source_amplitude,
+            e.acquisition.source_location, e.acquisition.receiver_location,
+            depsilon=direction.epsilon, dsigma=direction.sigma, stencil=d.stencil,
+            pml_width=d.boundary.width, max_vel=d.max_velocity, nt=e.nt,
+            source_component=e.source_component, receiver_component=e.receiver_component,
+            linearize_source=e.source_convention is SourceConvention.PHYSICAL_CURRENT,
+        )
+        return TangentResult(out[-1], EM3DState.from_tensors(tuple(out[:18])))
+
     def _separable_gradient(self, observed: torch.Tensor, residual):
         """3-D counterpart of :meth:`LinearizedMaxwellTM._separable_gradient`
         (``maxwell3d.separable_gradient3d``)."""

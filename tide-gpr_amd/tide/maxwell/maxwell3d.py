@@ -45,7 +45,7 @@ from ..storage import (
     _resolve_storage_compression,
     resolve_storage_mode,
 )
-from ..utils import C0, compile_material_coefficients
+from ..utils import C0, coefficient_partials, compile_material_coefficients
 
 STATE_NAMES = (
     "Ex", "Ey", "Ez", "Hx", "Hy", "Hz",
@@ -804,5 +804,108 @@ def separable_gradient3d(
     return value, dict(zip(names, pg)), r, unpadded, info
 
 
-__all__ = ["EM3DPropagate", "EM3DSetup", "STATE_NAMES", "maxwell3d", "prepare_em3d",
+def born3d(
+    epsilon: torch.Tensor,
+    sigma: torch.Tensor,
+    mu: torch.Tensor,
+    grid_spacing: float | Sequence[float],
+    dt: float,
+    source_amplitude: torch.Tensor | None,
+    source_location: torch.Tensor | None,
+    receiver_location: torch.Tensor | None,
+    *,
+    depsilon: torch.Tensor | None = None,
+    dsigma: torch.Tensor | None = None,
+    stencil: int = 2,
+    pml_width: int | Sequence[int] = 20,
+    max_vel: float | None = None,
+    nt: int | None = None,
+    source_component: str = "ey",
+    receiver_component: str = "ey",
+    linearize_source: bool = True,
+) -> tuple[torch.Tensor, ...]:
+    """Born (first-order scattered) 3-D propagation for a model direction
+    (``depsilon``, ``dsigma``), the tangent of :func:`maxwell3d` (reference
+    ``maxwell/maxwell3d_born.py:27-430``; native semantics
+    ``csrc/em3d/maxwell_3d_inst.cpp:1100-1527``).  The background and the
+    scattered field advance together on the device; the scattered E update
+    adds ``dca E + dcb curl H`` of the background, with ``(dca, dcb)`` the
+    closed-form coefficient partials times the padded direction, and with
+    ``linearize_source`` the scattered field also takes the source term's
+    tangent (the source scaling holds ``cb``).  Returns the 18 scattered
+    final arrays (unpadded), the background receivers and the scattered
+    receivers ``[nt, N, n_rec]``.  Not differentiable."""
+    if epsilon.ndim != 3:
+        raise NotImplementedError("born3d supports a single 3-D model")
+    if stencil != 2:
+        raise NotImplementedError("3-D Born propagation supports stencil 2")
+    if epsilon.device.type != "cuda":
+        raise NotImplementedError("3-D Born propagation runs on HIP devices only")
+    spacing = _normalize_grid_spacing_3d(grid_spacing)
+    with torch.no_grad():
+        eps, sig, mu_ = epsilon.detach(), sigma.detach(), mu.detach()
+        if max_vel is None:
+            max_vel = float((1.0 / torch.sqrt(eps * mu_)).max().item()) * C0
+        _, step_ratio = cfl_condition(spacing, dt, max_vel)
+        if step_ratio > 1:
+            raise NotImplementedError(
+                f"dt={dt} exceeds the CFL limit (needs {step_ratio} internal steps); "
+                "internal resampling is not part of this build, pass a stable dt")
+        amp = source_amplitude.detach() if source_amplitude is not None else None
+        setup = prepare_em3d(eps, sig, mu_, spacing, dt, amp, source_location,
+                             receiver_location, stencil=stencil, pml_width=pml_width, nt=nt,
+                             source_component=source_component,
+                             receiver_component=receiver_component)
+        dev, dtype = setup.ca.device, setup.ca.dtype
+        total = [a + b for a, b in zip(setup.fd_pad, setup.pml_width)]
+        shape = (setup.nz, setup.ny, setup.nx)
+
+        def padded(t):
+            return create_or_pad(t.detach().to(dtype), total, dev, dtype, shape, mode="replicate")
+
+        dca_de, dcb_de, dca_ds, dcb_ds = coefficient_partials(setup.ca[0], setup.cb[0], setup.dt)
+        d_ca = d_cb = None
+        if depsilon is not None:
+            de = padded(depsilon)
+            d_ca, d_cb = dca_de * de, dcb_de * de
+        if dsigma is not None:
+            ds = padded(dsigma)
+            d_ca = dca_ds * ds if d_ca is None else d_ca + dca_ds * ds
+            d_cb = dcb_ds * ds if d_cb is None else d_cb + dcb_ds * ds
+        d_ca = torch.zeros(shape, device=dev, dtype=dtype) if d_ca is None else d_ca.contiguous()
+        d_cb = torch.zeros(shape, device=dev, dtype=dtype) if d_cb is None else d_cb.contiguous()
+        numel = setup.nz * setup.ny * setup.nx
+        f0 = setup.f.contiguous() if setup.f.numel() > 0 else None
+        df = None
+        if setup.n_src > 0 and f0 is not None:
+            if linearize_source:
+                # tangent of f = amplitude * cb[src] * (-1/(dz dy dx)) along dcb
+                dz, dy, dx = spacing
+                dcb_src = d_cb.reshape(numel)[setup.sources_i.to(dev)]  # [N, n_src]
+                a = amp.permute(2, 0, 1).contiguous().to(dtype)
+                df = (a * dcb_src.unsqueeze(0) * (-1.0 / (dx * dy * dz))).contiguous()
+            else:
+                df = torch.zeros_like(f0)
+        fields = [t.contiguous().clone() for t in setup.fields]
+        dfields = [torch.zeros_like(t) for t in fields]
+        r = torch.zeros(setup.nt, setup.n_shots, max(setup.n_rec, 1), device=dev, dtype=dtype)
+        bg_r = torch.zeros_like(r)
+        fn = getattr(_native.load(), f"tide_em3d_{_native.dtype_name(dtype)}_born_forward")
+        dev_idx = dev.index if dev.index is not None else torch.cuda.current_device()
+        _native.call(
+            fn, _native.ptr(setup.ca), _native.ptr(setup.cb), _native.ptr(setup.cq),
+            _native.ptr(d_ca), _native.ptr(d_cb), _native.ptr(f0), _native.ptr(df),
+            _native.ptr_array(fields), _native.ptr_array(dfields), _native.ptr(r),
+            _native.ptr(bg_r), _native.ptr_array(setup.profiles),
+            _native.ptr(setup.sources_i), _native.ptr(setup.receivers_i),
+            setup.rdz, setup.rdy, setup.rdx, setup.nt, setup.n_shots, setup.nz, setup.ny,
+            setup.nx, setup.n_src, setup.n_rec, setup.batched, setup.batched, setup.batched, 0,
+            _native.int64_array(setup.pml), setup.source_component, setup.receiver_component,
+            dev_idx, _native.stream_handle(dev),
+        )
+        r, bg_r = r[:, :, : setup.n_rec], bg_r[:, :, : setup.n_rec]
+        return (*[setup.unpad(t) for t in dfields], bg_r, r)
+
+
+__all__ = ["EM3DPropagate", "EM3DSetup", "STATE_NAMES", "born3d", "maxwell3d", "prepare_em3d",
            "separable_gradient3d"]
