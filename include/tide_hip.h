@@ -1343,7 +1343,7 @@ int tide_tm2d_engine_create(int64_t device, int64_t group, int64_t max_jobs, voi
 /* shots per workgroup (sizes grad_*_groups) */
 int tide_tm2d_engine_group(void *engine, int64_t *group);
 /* Most steps a small job advances per tick, as a multiple of the largest
- * running job's steps (1..64, default 16); jobs are boosted in inverse
+ * running job's steps (1..64, default 4); jobs are boosted in inverse
  * proportion to their cells x shots per step. */
 int tide_tm2d_engine_set_boost(void *engine, int64_t boost);
 int tide_tm2d_engine_add(void *engine, const tide_tm2d_job *job, int64_t *job_id);

@@ -275,7 +275,7 @@ struct Engine {
   int64_t max_jobs = 64;
   // most steps a small job takes per tick, as a multiple of the largest
   // job's (tide_tm2d_engine_set_boost)
-  int64_t boost = 16;
+  int64_t boost = 4;
   std::map<int64_t, EJob> jobs;
   int64_t next_id = 1;
   ServeItem *d_items = nullptr;
