@@ -840,6 +840,14 @@ def main() -> None:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        # the metric's second part: served-solve latency at the first finite
+        # arrival rate of the sweep (details in "serving")
+        "served_latency": next(({"rate_per_s": s["rate_per_s"], "requests": s["served"]["solves"],
+                                 "p50_s": s["served"]["p50_s"], "p99_s": s["served"]["p99_s"],
+                                 "plain_p50_s": s["plain_one_at_a_time"]["p50_s"],
+                                 "plain_p99_s": s["plain_one_at_a_time"]["p99_s"]}
+                                for s in (serving or [])
+                                if s.get("rate_per_s") != "inf" and "plain_one_at_a_time" in s), None),
         "serving": serving,
         "batched": batched,
         "config1": config1,
