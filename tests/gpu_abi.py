@@ -349,10 +349,13 @@ def gpu_em3d_ref_backward(case, grad_r, stores, *, store: str = "full", device="
     return tuple(t.cpu().numpy() for t in (gca, gcb, geps, gsig, gf))
 
 
-def gpu_em3d_ref_born_forward(case, dca, dcb, df, *, device="cuda:0"):
+def gpu_em3d_ref_born_forward(case, dca, dcb, df, *, device="cuda:0", segments=(None,)):
     """``maxwell_3d_2_<dtype>_born_forward_cuda`` with the reference's
-    arguments (compact CPML memories for both fields).  Returns numpy
-    ``(fields, dfields (memories expanded to full size), r, background_r)``."""
+    arguments (compact CPML memories for both fields), over the whole time
+    axis or, with ``segments`` a list of step counts, as consecutive calls
+    with ``start_t`` / ``nt`` (the reference's time-chunked calls).  Returns
+    numpy ``(fields, dfields (memories expanded to full size), r,
+    background_r)``."""
     dev = torch.device(device)
     dtype = torch.float64 if case.dtype == np.float64 else torch.float32
     ca, cb, cq, f = (_dev(a, dev) for a in (case.ca, case.cb, case.cq, case.f))
@@ -376,12 +379,16 @@ def gpu_em3d_ref_born_forward(case, dca, dcb, df, *, device="cuda:0"):
     b = case.ca.shape[0] != 1
     P = _native.ptr
     fn = getattr(_native.load(), f"maxwell_3d_2_{_native.dtype_name(dtype)}_born_forward_cuda")
-    st = fn(P(ca), P(cb), P(cq), P(d_ca), P(d_cb), P(f), P(d_f), *[P(x) for x in flds],
-            *[P(x) for x in dflds], P(r), P(bg_r), *[P(p) for p in prof], P(src), P(rec),
-            case.rdz, case.rdy, case.rdx, case.dt, case.nt, N, nz, ny, nx, n_src, n_rec, 1,
-            b, b, b, 0, *case.pml, case.src_comp, case.rec_comp, 0, dev.index or 0, 0,
-            _native.stream_handle(dev))
-    _native.check(st)
+    t0 = 0
+    for seg in segments:
+        nt = case.nt - t0 if seg is None else seg
+        st = fn(P(ca), P(cb), P(cq), P(d_ca), P(d_cb), P(f), P(d_f), *[P(x) for x in flds],
+                *[P(x) for x in dflds], P(r), P(bg_r), *[P(p) for p in prof], P(src), P(rec),
+                case.rdz, case.rdy, case.rdx, case.dt, nt, N, nz, ny, nx, n_src, n_rec, 1,
+                b, b, b, t0, *case.pml, case.src_comp, case.rec_comp, 0, dev.index or 0, 0,
+                _native.stream_handle(dev))
+        _native.check(st)
+        t0 += nt
     torch.cuda.synchronize(dev)
     shape = (N, nz, ny, nx)
 
